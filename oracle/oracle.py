@@ -1,0 +1,286 @@
+"""ctypes wrapper over the CPU oracle (oracle/libvxoracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke check and by the
+CPU-baseline leg of bench.py. The product path never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libvxoracle.so")
+
+F32 = 0
+F64 = 1
+IC0 = 0
+BJ = 1
+EMPTY = 0xFFFFFFFF
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+
+
+class _Scene(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_uint32),
+        ("positions", _dp), ("rotations", _dp), ("velocities", _dp),
+        ("mass", _dp), ("kinematic", _u8p),
+        ("young", _dp), ("young_perp", _dp), ("shear", _dp), ("solid", _dp),
+        ("fibers", _dp), ("anisotropy", _dp),
+        ("delta_temp", _dp), ("alpha_par", _dp), ("alpha_perp", _dp),
+        ("neighbor", _u32p), ("damage", _dp), ("kscale", _dp),
+        ("formation_offset", _dp), ("plastic_strain", _dp),
+    ]
+
+
+class _StepConfig(ctypes.Structure):
+    _fields_ = [
+        ("fixed_dt", ctypes.c_double), ("substeps", ctypes.c_uint32), ("iterations", ctypes.c_uint32),
+        ("corotation_iterations", ctypes.c_uint32), ("solver_relative_tolerance", ctypes.c_double),
+        ("corotation_tolerance", ctypes.c_double), ("gravity", ctypes.c_double * 3),
+        ("precond", ctypes.c_int), ("exact", ctypes.c_int), ("exact_tolerance", ctypes.c_double),
+    ]
+
+
+class _StepReport(ctypes.Structure):
+    _fields_ = [("pcg_iterations", ctypes.c_uint32), ("passes", ctypes.c_uint32),
+                ("solver_residual", ctypes.c_double), ("last_angle", ctypes.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise RuntimeError("oracle library missing: build it with `make -C oracle`")
+        L = ctypes.CDLL(_LIB_PATH)
+        L.vxo_build_topology.restype = ctypes.c_uint32
+        L.vxo_build_topology.argtypes = [ctypes.c_uint32, _i32p, _u8p, _u32p]
+        L.vxo_bond_frame.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_uint32, _dp]
+        L.vxo_stiffness_matrix.argtypes = [ctypes.c_int, ctypes.POINTER(_Scene), ctypes.c_uint32, ctypes.c_uint32, _dp, _dp]
+        L.vxo_bond_thermal_rest.argtypes = [ctypes.c_int, ctypes.POINTER(_Scene), ctypes.c_uint32, ctypes.c_uint32, _dp, _dp]
+        L.vxo_fit_rotations.restype = ctypes.c_double
+        L.vxo_fit_rotations.argtypes = [ctypes.c_int, ctypes.POINTER(_Scene)]
+        L.vxo_assemble.restype = ctypes.c_void_p
+        L.vxo_assemble.argtypes = [ctypes.c_int, ctypes.POINTER(_Scene), ctypes.c_double, _dp, _u8p, _dp]
+        L.vxo_system_free.argtypes = [ctypes.c_void_p]
+        L.vxo_system_active_count.restype = ctypes.c_uint32
+        L.vxo_system_active_count.argtypes = [ctypes.c_void_p]
+        L.vxo_system_constraint_count.restype = ctypes.c_uint32
+        L.vxo_system_constraint_count.argtypes = [ctypes.c_void_p]
+        L.vxo_system_export_rows.argtypes = [ctypes.c_void_p, _u32p, _u32p, _dp, _dp, _dp, _dp]
+        L.vxo_system_export_full.argtypes = [ctypes.c_void_p, _dp, _dp, _u32p, _u8p]
+        L.vxo_system_export_constraints.argtypes = [ctypes.c_void_p, _u32p, _dp, _dp, _dp, _dp]
+        L.vxo_solve_cpu.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32, ctypes.c_double, ctypes.c_int,
+                                    ctypes.c_uint32, _dp, _u32p, _dp, _dp]
+        L.vxo_solve_device_semantics.restype = ctypes.c_int
+        L.vxo_solve_device_semantics.argtypes = [ctypes.c_int, ctypes.c_uint32, _dp, _u32p, _dp, _dp, _dp,
+                                                 ctypes.c_uint32, ctypes.c_double, ctypes.c_uint32, _dp, _u32p, _dp]
+        L.vxo_solve_exact.restype = ctypes.c_double
+        L.vxo_solve_exact.argtypes = [ctypes.c_uint32, _dp, _u32p, _dp, _dp, ctypes.c_double, ctypes.c_uint32, _dp, _u32p]
+        L.vxo_matvec.argtypes = [ctypes.c_int, ctypes.c_uint32, _u32p, _dp, _dp, _dp, _dp]
+        L.vxo_step.argtypes = [ctypes.c_int, ctypes.POINTER(_Scene), ctypes.POINTER(_StepConfig), _dp, _dp,
+                               ctypes.POINTER(_StepReport), _dp]
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class OracleScene:
+    """float64 copies of a vxsim.scenes.Scene, kept alive with the struct."""
+
+    def __init__(self, scene):
+        self.src = scene
+        self.positions = _f64(scene.positions).copy()
+        self.rotations = _f64(scene.rotations).copy()
+        self.velocities = _f64(scene.velocities).copy()
+        self.mass = _f64(scene.mass)
+        self.kinematic = np.ascontiguousarray(scene.kinematic, dtype=np.uint8)
+        self.young = _f64(scene.young)
+        self.young_perp = _f64(scene.young_perp)
+        self.shear = _f64(scene.shear)
+        self.solid = _f64(scene.solid)
+        self.fibers = _f64(scene.fibers)
+        self.anisotropy = _f64(scene.anisotropy)
+        self.delta_temp = _f64(scene.delta_temp)
+        self.alpha_par = _f64(scene.alpha_par)
+        self.alpha_perp = _f64(scene.alpha_perp)
+        self.neighbor = np.ascontiguousarray(scene.neighbor, dtype=np.uint32)
+        self.damage = _f64(scene.damage)
+        self.kscale = _f64(scene.kscale)
+        self.formation_offset = _f64(scene.formation_offset)
+        self.plastic_strain = _f64(scene.plastic_strain)
+        self.struct = _Scene(
+            scene.n,
+            _p(self.positions, _dp), _p(self.rotations, _dp), _p(self.velocities, _dp),
+            _p(self.mass, _dp), _p(self.kinematic, _u8p),
+            _p(self.young, _dp), _p(self.young_perp, _dp), _p(self.shear, _dp), _p(self.solid, _dp),
+            _p(self.fibers, _dp), _p(self.anisotropy, _dp),
+            _p(self.delta_temp, _dp), _p(self.alpha_par, _dp), _p(self.alpha_perp, _dp),
+            _p(self.neighbor, _u32p), _p(self.damage, _dp), _p(self.kscale, _dp),
+            _p(self.formation_offset, _dp), _p(self.plastic_strain, _dp),
+        )
+
+    @property
+    def n(self):
+        return self.src.n
+
+
+def build_topology(keys, kinematic):
+    keys = np.ascontiguousarray(keys, dtype=np.int32)
+    kin = np.ascontiguousarray(kinematic, dtype=np.uint8)
+    n = keys.shape[0]
+    nb = np.empty((3, n), np.uint32)
+    count = lib().vxo_build_topology(n, _p(keys, _i32p), _p(kin, _u8p), _p(nb, _u32p))
+    return nb, int(count)
+
+
+def bond_frame(qi, qj, axis, mode=F64):
+    out = np.empty(9)
+    lib().vxo_bond_frame(mode, _p(_f64(qi), _dp), _p(_f64(qj), _dp), axis, _p(out, _dp))
+    return out.reshape(3, 3)
+
+
+def stiffness_matrix(oscene: OracleScene, i, j, frame, mode=F64):
+    out = np.empty(9)
+    lib().vxo_stiffness_matrix(mode, ctypes.byref(oscene.struct), i, j, _p(_f64(frame).ravel(), _dp), _p(out, _dp))
+    return out.reshape(3, 3)
+
+
+def bond_thermal_rest(oscene: OracleScene, i, j, frame, mode=F64):
+    out = np.empty(3)
+    lib().vxo_bond_thermal_rest(mode, ctypes.byref(oscene.struct), i, j, _p(_f64(frame).ravel(), _dp), _p(out, _dp))
+    return out
+
+
+def fit_rotations(oscene: OracleScene, mode=F64) -> float:
+    """Updates oscene.rotations in place; returns the largest angle change."""
+    return float(lib().vxo_fit_rotations(mode, ctypes.byref(oscene.struct)))
+
+
+class System:
+    def __init__(self, oscene: OracleScene, mode, h, predicted, free_mask, loads):
+        self.oscene = oscene
+        self.mode = mode
+        self._pred = _f64(predicted).ravel()
+        self._free = np.ascontiguousarray(free_mask, dtype=np.uint8)
+        self._loads = None if loads is None else _f64(loads).ravel()
+        self.h = lib().vxo_assemble(mode, ctypes.byref(oscene.struct), h, _p(self._pred, _dp),
+                                    _p(self._free, _u8p), _p(self._loads, _dp))
+        self.na = int(lib().vxo_system_active_count(self.h))
+        self.nc = int(lib().vxo_system_constraint_count(self.h))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().vxo_system_free(self.h)
+            self.h = None
+
+    def rows(self):
+        na = self.na
+        active = np.empty(na, np.uint32)
+        nbr = np.empty((na, 6), np.uint32)
+        blocks = np.empty((na, 54))
+        diag = np.empty((na, 9))
+        inv = np.empty((na, 9))
+        rhs = np.empty((na, 3))
+        lib().vxo_system_export_rows(self.h, _p(active, _u32p), _p(nbr, _u32p), _p(blocks, _dp), _p(diag, _dp),
+                                     _p(inv, _dp), _p(rhs, _dp))
+        return dict(active=active, neighbors=nbr, blocks=blocks, diagonal=diag, inverse=inv, rhs=rhs)
+
+    def full(self):
+        n = self.oscene.n
+        diag = np.empty((n, 9))
+        rhs = np.empty((n, 3))
+        inc = np.empty((n, 6), np.uint32)
+        inc_len = np.empty(n, np.uint8)
+        lib().vxo_system_export_full(self.h, _p(diag, _dp), _p(rhs, _dp), _p(inc, _u32p), _p(inc_len, _u8p))
+        return dict(diagonal=diag, rhs=rhs, incidence=inc, incidence_len=inc_len)
+
+    def constraints(self):
+        nc = self.nc
+        ija = np.empty((nc, 3), np.uint32)
+        frame = np.empty((nc, 9))
+        mk = np.empty((nc, 9))
+        k = np.empty((nc, 9))
+        rest = np.empty((nc, 3))
+        lib().vxo_system_export_constraints(self.h, _p(ija, _u32p), _p(frame, _dp), _p(mk, _dp), _p(k, _dp), _p(rest, _dp))
+        return dict(ija=ija, frame=frame, material_k=mk, k=k, rest=rest)
+
+    def solve_cpu(self, precond=BJ, max_iterations=80, tolerance=1e-5, threads=1, parallel_cutoff=32768):
+        dx = np.empty((self.na, 3))
+        it = ctypes.c_uint32()
+        res = ctypes.c_double()
+        sec = ctypes.c_double()
+        lib().vxo_solve_cpu(self.h, precond, max_iterations, tolerance, threads, parallel_cutoff, _p(dx, _dp),
+                            ctypes.byref(it), ctypes.byref(res), ctypes.byref(sec))
+        return dx, int(it.value), float(res.value), float(sec.value)
+
+
+def assemble(oscene: OracleScene, mode, h, predicted, free_mask, loads=None) -> System:
+    return System(oscene, mode, h, predicted, free_mask, loads)
+
+
+def solve_device_semantics(rows, max_iterations, tolerance, mode=F32, check_every=16):
+    n = rows["rhs"].shape[0]
+    dx = np.zeros((n, 3))
+    it = ctypes.c_uint32()
+    rel = ctypes.c_double()
+    nbr = np.ascontiguousarray(rows["neighbors"], np.uint32)
+    status = lib().vxo_solve_device_semantics(
+        mode, n, _p(_f64(rows["rhs"]), _dp), _p(nbr, _u32p), _p(_f64(rows["diagonal"]), _dp),
+        _p(_f64(rows["blocks"]), _dp), _p(_f64(rows["inverse"]), _dp), max_iterations, tolerance, check_every,
+        _p(dx, _dp), ctypes.byref(it), ctypes.byref(rel))
+    return int(status), dx, int(it.value), float(rel.value)
+
+
+def solve_exact(rows, tolerance=1e-13, max_iterations=200000):
+    n = rows["rhs"].shape[0]
+    x = np.zeros((n, 3))
+    it = ctypes.c_uint32()
+    nbr = np.ascontiguousarray(rows["neighbors"], np.uint32)
+    rel = lib().vxo_solve_exact(n, _p(_f64(rows["rhs"]), _dp), _p(nbr, _u32p), _p(_f64(rows["diagonal"]), _dp),
+                                _p(_f64(rows["blocks"]), _dp), tolerance, max_iterations, _p(x, _dp),
+                                ctypes.byref(it))
+    return x, int(it.value), float(rel)
+
+
+def matvec(rows, x, mode=F64):
+    n = rows["diagonal"].shape[0]
+    y = np.empty((n, 3))
+    nbr = np.ascontiguousarray(rows["neighbors"], np.uint32)
+    lib().vxo_matvec(mode, n, _p(nbr, _u32p), _p(_f64(rows["diagonal"]), _dp), _p(_f64(rows["blocks"]), _dp),
+                     _p(_f64(x), _dp), _p(y, _dp))
+    return y
+
+
+def step(oscene: OracleScene, cfg, mode=F64, precond=BJ, exact=False, exact_tolerance=1e-12,
+         targets=None, loads=None, want_forces=False):
+    """One fixed step (solve part). cfg: vxsim.scenes.StepConfig."""
+    c = _StepConfig(cfg.fixed_dt, cfg.substeps, cfg.iterations, cfg.corotation_iterations,
+                    cfg.solver_relative_tolerance, cfg.corotation_tolerance,
+                    (ctypes.c_double * 3)(*cfg.gravity), precond, 1 if exact else 0, exact_tolerance)
+    rep = _StepReport()
+    tg = None if targets is None else _f64(targets).ravel()
+    ld = None if loads is None else _f64(loads).ravel()
+    forces = np.zeros((3, oscene.n, 3)) if want_forces else None
+    lib().vxo_step(mode, ctypes.byref(oscene.struct), ctypes.byref(c), _p(tg, _dp), _p(ld, _dp),
+                   ctypes.byref(rep), _p(forces, _dp))
+    out = dict(pcg_iterations=rep.pcg_iterations, passes=rep.passes, solver_residual=rep.solver_residual,
+               last_angle=rep.last_angle)
+    if want_forces:
+        out["last_force"] = forces
+    return out

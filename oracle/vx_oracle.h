@@ -1,0 +1,189 @@
+/*
+ * CPU oracle for the corotational elastic PCG solve of Voxel-sim.
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the checker the HIP path is
+ * compared against (and the timed CPU baseline). It is never linked into,
+ * loaded by, or called from the product library, and the product path never
+ * falls back to it.
+ *
+ * It restates, in C++ templated on the scalar type, the reference algorithm
+ * (paths relative to the reference repository):
+ *   src/solver.rs:27-73    basis / tensor_compliance / stiffness_matrix
+ *   src/solver.rs:78-95    bond_thermal_rest
+ *   src/solver.rs:327-417  fit_rotations
+ *   src/solver.rs:1325-1345 make_constraint
+ *   src/solver.rs:1347-1537 Cholesky helpers, block IC(0), block-Jacobi
+ *   src/solver.rs:1542-1929 solve_coupled: assembly, projection, row packing,
+ *                           CPU PCG loop
+ *   src/solver.rs:475-570, 668-683 the substep / corotation driver (solve part)
+ *   src/cuda_solver.rs:301-501 + src/cuda/pcg.ptx  device PCG semantics
+ *
+ * Mode VXO_F32 follows the reference's f32 arithmetic order (glam f32 values,
+ * f64 dot accumulation, f32 alpha/beta). Mode VXO_F64 is the FP64 oracle:
+ * the same formulas evaluated in double.
+ *
+ * Every array argument is double (or an integer type) regardless of mode; in
+ * VXO_F32 mode inputs are rounded to float on entry and results are exact
+ * float values widened to double.
+ *
+ * Matrices are exported row-major: m[r*3 + c].
+ */
+#ifndef VX_ORACLE_H
+#define VX_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VXO_F32 0
+#define VXO_F64 1
+
+#define VXO_PRECOND_IC0 0
+#define VXO_PRECOND_BJ 1
+
+#define VXO_EMPTY 0xFFFFFFFFu
+
+/* Status codes mirror include/vx_solver.h. */
+#define VXO_OK 0
+#define VXO_ERR_NONFINITE_RESIDUAL 3
+#define VXO_ERR_NOT_CONVERGED 4
+#define VXO_ERR_NONFINITE_CORRECTION 5
+
+/* Scene state consumed by the solve (reference Scene, src/scene.rs:309-351). */
+typedef struct vxo_scene {
+    uint32_t n;
+    double *positions;            /* n*3 */
+    double *rotations;            /* n*4 quaternion (x, y, z, w) */
+    double *velocities;           /* n*3, used by vxo_step only */
+    const double *mass;           /* n */
+    const uint8_t *kinematic;     /* n */
+    const double *young;          /* n, modulus along the fibre */
+    const double *young_perp;     /* n */
+    const double *shear;          /* n */
+    const double *solid;          /* n, phase solid fraction */
+    const double *fibers;         /* n*3, Microstructure::fibers() (body frame) */
+    const double *anisotropy;     /* n */
+    const double *delta_temp;     /* n, temperature minus stress-free temperature */
+    const double *alpha_par;      /* n, thermal expansion along the fibre */
+    const double *alpha_perp;     /* n */
+    const uint32_t *neighbor;     /* 3*n, [axis][voxel], VXO_EMPTY if no bond */
+    const double *damage;         /* 3*n */
+    const double *kscale;         /* 3*n, per-bond stiffness scale (1 = none) */
+    const double *formation_offset; /* 3*n*3 */
+    const double *plastic_strain;   /* 3*n*3 */
+} vxo_scene;
+
+typedef struct vxo_system vxo_system;
+
+/* ---- topology ------------------------------------------------------------ */
+/* Lattice bond rule (src/scene.rs:558-570, 1668-1681): for each voxel i and
+ * axis a, bond i -> j iff the cell key+e_a is occupied by j and both have the
+ * same kinematic flag. keys: n*3 int32 lattice coordinates. Writes
+ * neighbor[3*n]. Returns the number of bonds. */
+uint32_t vxo_build_topology(uint32_t n, const int32_t *keys, const uint8_t *kinematic,
+                            uint32_t *neighbor);
+
+/* ---- constitutive pieces (exposed for known-answer tests) ---------------- */
+/* frame = basis(axis, slerp(q_i, q_j, 0.5)); outputs row-major 3x3. */
+void vxo_bond_frame(int mode, const double *qi, const double *qj, uint32_t axis, double *frame);
+/* stiffness_matrix(scene, i, j, frame) (src/solver.rs:65-73). */
+void vxo_stiffness_matrix(int mode, const vxo_scene *s, uint32_t i, uint32_t j,
+                          const double *frame, double *k);
+/* bond_thermal_rest (src/solver.rs:78-95). */
+void vxo_bond_thermal_rest(int mode, const vxo_scene *s, uint32_t i, uint32_t j,
+                           const double *frame, double *rest);
+
+/* ---- rotation fit -------------------------------------------------------- */
+/* fit_rotations (src/solver.rs:327-417); updates s->rotations and returns the
+ * largest quaternion angle change. */
+double vxo_fit_rotations(int mode, vxo_scene *s);
+
+/* ---- assembly ------------------------------------------------------------ */
+/* solve_coupled assembly (src/solver.rs:1552-1672).
+ * predicted: n*3; free_mask: n bytes, bit a set when axis a is free;
+ * loads: n*3 applied loads (zero for kinematic voxels). */
+vxo_system *vxo_assemble(int mode, const vxo_scene *s, double h, const double *predicted,
+                         const uint8_t *free_mask, const double *loads);
+void vxo_system_free(vxo_system *sys);
+uint32_t vxo_system_active_count(const vxo_system *sys);
+uint32_t vxo_system_constraint_count(const vxo_system *sys);
+/* Reference device packing (src/solver.rs:1731-1777): per active row the
+ * neighbours [u32;6] in incidence order, 6 row-major off-diagonal blocks
+ * (54 values, zero-filled), the row-major diagonal and its inverse, and the
+ * right-hand side r. active: voxel index of each active row. */
+void vxo_system_export_rows(const vxo_system *sys, uint32_t *active, uint32_t *neighbors,
+                            double *blocks, double *diagonal, double *inverse, double *rhs);
+/* Per voxel: projected diagonal (n*9), masked rhs (n*3), incidence (n*6 edge
+ * ids, VXO_EMPTY padded) and incidence lengths (n). */
+void vxo_system_export_full(const vxo_system *sys, double *diagonal, double *rhs,
+                            uint32_t *incidence, uint8_t *incidence_len);
+/* Per constraint: (i, j, axis), frame, material_k, k (row-major), rest. */
+void vxo_system_export_constraints(const vxo_system *sys, uint32_t *ija, double *frame,
+                                   double *material_k, double *k, double *rest);
+
+/* ---- solves -------------------------------------------------------------- */
+/* Reference CPU PCG (src/solver.rs:1833-1911) in the system's mode.
+ * threads: OpenMP threads (0 = library default). parallel_cutoff: rows below
+ * which loops stay serial (the reference uses 32,768; 0 = always parallel).
+ * dx: active*3. seconds: wall time of the PCG loop alone. */
+void vxo_solve_cpu(vxo_system *sys, int precond, uint32_t max_iterations, double tolerance,
+                   int threads, uint32_t parallel_cutoff, double *dx, uint32_t *iterations,
+                   double *residual, double *seconds);
+
+/* Device PCG semantics (src/cuda_solver.rs:301-501, src/cuda/pcg.ptx) over the
+ * reference packing. check_every = 16 reproduces the reference. Returns a
+ * VXO_* status; dx/iterations/relative are written on VXO_OK. */
+int vxo_solve_device_semantics(int mode, uint32_t n, const double *rhs,
+                               const uint32_t *neighbors, const double *diagonal,
+                               const double *blocks, const double *inverse,
+                               uint32_t max_iterations, double tolerance, uint32_t check_every,
+                               double *dx, uint32_t *iterations, double *relative);
+
+/* FP64 reference solution: block-Jacobi PCG in double over the reference
+ * packing, run until the recomputed relative residual ||b - A x|| / ||b|| is
+ * <= tolerance (or max_iterations). Returns the final true relative residual. */
+double vxo_solve_exact(uint32_t n, const double *rhs, const uint32_t *neighbors,
+                       const double *diagonal, const double *blocks, double tolerance,
+                       uint32_t max_iterations, double *x, uint32_t *iterations);
+
+/* y = A x over the reference packing, in the given mode (PTX fma order in
+ * VXO_F32). */
+void vxo_matvec(int mode, uint32_t n, const uint32_t *neighbors, const double *diagonal,
+                const double *blocks, const double *x, double *y);
+
+/* ---- step driver --------------------------------------------------------- */
+typedef struct vxo_step_config {
+    double fixed_dt;
+    uint32_t substeps;
+    uint32_t iterations;            /* CPU PCG cap */
+    uint32_t corotation_iterations;
+    double solver_relative_tolerance;
+    double corotation_tolerance;
+    double gravity[3];
+    int precond;                    /* VXO_PRECOND_* */
+    int exact;                      /* 1: solve each pass with vxo_solve_exact at exact_tolerance */
+    double exact_tolerance;
+} vxo_step_config;
+
+typedef struct vxo_step_report {
+    uint32_t pcg_iterations;        /* summed over passes and substeps */
+    uint32_t passes;                /* solves performed */
+    double solver_residual;         /* last solve */
+    double last_angle;              /* last corotation fit angle */
+} vxo_step_report;
+
+/* One fixed step without contact, tool, fracture or heat: the solve part of
+ * step_with_axis_targets_and_loads (src/solver.rs:446-570, 668-683).
+ * targets: n*3 world coordinates, NaN for a free axis (may be NULL).
+ * loads: n*3 applied loads (forces + accel*mass) (may be NULL).
+ * last_force: optional 3*n*3 output, per +axis bond, of the bond force of the
+ * last assembly evaluated at the final positions (src/solver.rs:627-640). */
+void vxo_step(int mode, vxo_scene *s, const vxo_step_config *cfg, const double *targets,
+              const double *loads, vxo_step_report *report, double *last_force);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

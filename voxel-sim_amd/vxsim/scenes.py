@@ -1,0 +1,333 @@
+"""Synthetic REV-lattice scenes for the corotational PCG solve.
+
+The state follows the reference's scene data model (src/scene.rs:309-351,
+src/voxel.rs:46-60, src/bond.rs:4-16) flattened into structure-of-arrays
+planes of the reference's f32 values:
+
+* voxel index v = x + nx*(y + ny*z) (the z, y, x loop order of Scene::new,
+  src/scene.rs:1607-1619), centre = key*CELL + CELL/2 with key =
+  (x - nx//2, y - ny//2, z) (src/scene.rs:1589-1611);
+* bonds are stored per +axis (`neighbor[axis][v]`, EMPTY = 0xFFFFFFFF) and
+  follow the lattice rule "both cells occupied and equal kinematic flag"
+  (src/scene.rs:563, 1674). Voxels fixed after construction keep their bonds
+  (Scene::fix_voxels, src/scene.rs:901-930); voxels spawned fixed get none.
+
+The reference has no scene generator; the draws below (SplitMix64 streams)
+define the synthetic workloads.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+CELL = np.float32(0.01)  # src/scene.rs:10
+EMPTY = np.uint32(0xFFFFFFFF)  # src/bond.rs:2
+
+_GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+
+
+class SplitMix64:
+    """Counter form of SplitMix64 (Steele, Lea, Flood 2014): the k-th output is
+    mix(seed + (k + 1) * golden), so blocks of draws vectorise."""
+
+    def __init__(self, seed: int):
+        self.state = np.uint64(seed)
+        self.count = 0
+
+    def next_u64(self, size: int) -> np.ndarray:
+        with np.errstate(over="ignore"):
+            k = np.arange(self.count + 1, self.count + 1 + size, dtype=np.uint64)
+            z = self.state + k * _GOLDEN
+            z = (z ^ (z >> np.uint64(30))) * _M1
+            z = (z ^ (z >> np.uint64(27))) * _M2
+            z = z ^ (z >> np.uint64(31))
+        self.count += size
+        return z
+
+    def uniform(self, size: int) -> np.ndarray:
+        """Doubles in [0, 1) from the top 53 bits."""
+        return (self.next_u64(size) >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+    def normal(self, size: int) -> np.ndarray:
+        """Box-Muller pairs from two uniform streams."""
+        m = (size + 1) // 2
+        u1 = 1.0 - self.uniform(m)
+        u2 = self.uniform(m)
+        r = np.sqrt(-2.0 * np.log(u1))
+        out = np.concatenate([r * np.cos(2 * np.pi * u2), r * np.sin(2 * np.pi * u2)])
+        return out[:size]
+
+
+@dataclasses.dataclass(frozen=True)
+class Material:
+    """Elastic and thermal constants used by the solve (src/material.rs:20-63)."""
+
+    name: str
+    density: float
+    young: float
+    young_perp: float
+    shear: float
+    alpha_par: float = 0.0
+    alpha_perp: float = 0.0
+
+
+# Built-in values of src/material.rs:105-150 for iron and stone. Wood uses the
+# values of the synthetic workloads (600 kg/m^3, 12 / 0.8 GPa), with the
+# built-in shear modulus and expansion (src/material.rs:96-101, 258-262).
+MATERIALS = {
+    "wood": Material("wood", 600.0, 12e9, 0.8e9, 0.65e9, 3.0e-6, 30.0e-6),
+    "wood_builtin": Material("wood_builtin", 650.0, 10e9, 0.6e9, 0.65e9, 3.0e-6, 30.0e-6),
+    "iron": Material("iron", 7870.0, 200e9, 200e9, 77e9, 12.0e-6, 12.0e-6),
+    "stone": Material("stone", 2700.0, 50e9, 50e9, 20e9, 8.0e-6, 8.0e-6),
+}
+
+AXES = np.eye(3, dtype=np.float32)
+
+
+@dataclasses.dataclass
+class StepConfig:
+    """Solver fields of SceneConfig (src/scene.rs:215-266)."""
+
+    fixed_dt: float = 0.001
+    substeps: int = 1
+    iterations: int = 80
+    cuda_iterations: int = 160
+    corotation_iterations: int = 2
+    solver_relative_tolerance: float = 1e-5
+    corotation_tolerance: float = 1e-5
+    gravity: tuple = (0.0, 0.0, -9.81)
+
+
+@dataclasses.dataclass
+class Scene:
+    """SoA scene state; every float plane is float32 like the reference."""
+
+    dims: tuple
+    positions: np.ndarray      # (n, 3)
+    rotations: np.ndarray      # (n, 4) quaternion x, y, z, w
+    velocities: np.ndarray     # (n, 3)
+    mass: np.ndarray           # (n,)
+    kinematic: np.ndarray      # (n,) uint8
+    young: np.ndarray
+    young_perp: np.ndarray
+    shear: np.ndarray
+    solid: np.ndarray
+    fibers: np.ndarray         # (n, 3) body-frame fibre direction
+    anisotropy: np.ndarray
+    delta_temp: np.ndarray
+    alpha_par: np.ndarray
+    alpha_perp: np.ndarray
+    neighbor: np.ndarray       # (3, n) uint32
+    damage: np.ndarray         # (3, n)
+    kscale: np.ndarray         # (3, n)
+    formation_offset: np.ndarray  # (3, n, 3)
+    plastic_strain: np.ndarray    # (3, n, 3)
+    targets: np.ndarray        # (n, 3), NaN where the axis is free
+    loads: np.ndarray          # (n, 3) applied loads (zero on kinematic voxels)
+    keys: np.ndarray           # (n, 3) int32 lattice keys
+    config: StepConfig = dataclasses.field(default_factory=StepConfig)
+    name: str = ""
+
+    @property
+    def n(self) -> int:
+        return int(self.positions.shape[0])
+
+    def free_mask(self) -> np.ndarray:
+        """Bit a set when axis a is solved for (src/solver.rs:1554-1565)."""
+        free = np.isnan(self.targets) & (self.kinematic == 0)[:, None]
+        return (free[:, 0] * 1 + free[:, 1] * 2 + free[:, 2] * 4).astype(np.uint8)
+
+    def copy(self) -> "Scene":
+        kw = {}
+        for f in dataclasses.fields(self):
+            v = getattr(self, f.name)
+            kw[f.name] = v.copy() if isinstance(v, np.ndarray) else (dataclasses.replace(v) if isinstance(v, StepConfig) else v)
+        return Scene(**kw)
+
+    def bond_count(self) -> int:
+        return int(np.count_nonzero(self.neighbor != EMPTY))
+
+    def active_count(self) -> int:
+        return int(np.count_nonzero(self.free_mask()))
+
+
+def lattice_topology(dims, kinematic: np.ndarray) -> np.ndarray:
+    """+axis bonds of a full nx*ny*nz box: i -> i + stride_a when the
+    neighbour cell exists and both kinematic flags are equal
+    (src/scene.rs:558-570)."""
+    nx, ny, nz = dims
+    n = nx * ny * nz
+    idx = np.arange(n, dtype=np.int64)
+    x = idx % nx
+    y = (idx // nx) % ny
+    z = idx // (nx * ny)
+    nb = np.full((3, n), EMPTY, dtype=np.uint32)
+    for axis, (coord, extent, stride) in enumerate(((x, nx, 1), (y, ny, nx), (z, nz, nx * ny))):
+        has = coord + 1 < extent
+        j = idx + stride
+        ok = has.copy()
+        ok[has] = kinematic[idx[has]] == kinematic[j[has]]
+        nb[axis, ok] = j[ok].astype(np.uint32)
+    return nb
+
+
+def _box(dims, material_ids, grain_axis, anisotropy, kinematic, materials) -> Scene:
+    nx, ny, nz = dims
+    n = nx * ny * nz
+    idx = np.arange(n)
+    x = idx % nx
+    y = (idx // nx) % ny
+    z = idx // (nx * ny)
+    keys = np.stack([x - nx // 2, y - ny // 2, z], axis=1).astype(np.int32)
+    positions = keys.astype(np.float32) * CELL + np.float32(CELL * np.float32(0.5))
+    table = [MATERIALS[m] for m in materials]
+    mid = np.asarray(material_ids)
+    f32 = lambda field: np.array([getattr(m, field) for m in table], dtype=np.float32)[mid]  # noqa: E731
+    mass = (f32("density") * np.float32(CELL) ** 3).astype(np.float32)
+    kin = np.asarray(kinematic, dtype=np.uint8)
+    return Scene(
+        dims=(nx, ny, nz),
+        positions=positions.astype(np.float32),
+        rotations=np.tile(np.array([0, 0, 0, 1], np.float32), (n, 1)),
+        velocities=np.zeros((n, 3), np.float32),
+        mass=mass,
+        kinematic=kin,
+        young=f32("young"),
+        young_perp=f32("young_perp"),
+        shear=f32("shear"),
+        solid=np.ones(n, np.float32),
+        fibers=AXES[np.asarray(grain_axis)].astype(np.float32),
+        anisotropy=np.asarray(anisotropy, np.float32) * np.ones(n, np.float32),
+        delta_temp=np.zeros(n, np.float32),
+        alpha_par=f32("alpha_par"),
+        alpha_perp=f32("alpha_perp"),
+        neighbor=lattice_topology(dims, np.zeros(n, np.uint8)),
+        damage=np.zeros((3, n), np.float32),
+        kscale=np.ones((3, n), np.float32),
+        formation_offset=np.zeros((3, n, 3), np.float32),
+        plastic_strain=np.zeros((3, n, 3), np.float32),
+        targets=np.full((n, 3), np.nan, np.float32),
+        loads=np.zeros((n, 3), np.float32),
+        keys=keys,
+    )
+
+
+def uniform_box(dims, material="wood", grain=2, anisotropy=None) -> Scene:
+    """Homogeneous box: one material, one grain axis, no loads or supports."""
+    n = dims[0] * dims[1] * dims[2]
+    if anisotropy is None:
+        anisotropy = 1.0 if material.startswith("wood") else 0.0
+    return _box(dims, np.zeros(n, np.int64), np.full(n, grain), anisotropy, np.zeros(n, np.uint8), [material])
+
+
+def tension_10cube() -> Scene:
+    """10x10x10 wood REVs, grain Z, anisotropy 1 (scenes/wood_chop.ron:9).
+    Bottom and top faces are axis-targeted on all axes (the fixture pattern of
+    src/experiments.rs:73-87); the top target is rest + 0.1 mm in Z. One 1 ms
+    step with one substep, no gravity, at most 4 corotation passes."""
+    s = uniform_box((10, 10, 10), "wood", grain=2)
+    z = s.keys[:, 2]
+    bottom = z == z.min()
+    top = z == z.max()
+    s.targets[bottom] = s.positions[bottom]
+    s.targets[top] = s.positions[top] + np.array([0, 0, 1e-4], np.float32)
+    s.config = StepConfig(fixed_dt=0.001, substeps=1, iterations=1000, corotation_iterations=4,
+                          solver_relative_tolerance=1e-10, gravity=(0.0, 0.0, 0.0))
+    s.name = "tension_10x10x10"
+    return s
+
+
+def _regional_materials(dims, region, rng: SplitMix64):
+    """Per region of region^3 cells: material wood / iron / stone with
+    probabilities 0.6 / 0.2 / 0.2, then the wood grain axis uniform over
+    X, Y, Z. Regions are drawn in z, y, x order."""
+    nx, ny, nz = dims
+    rx, ry, rz = (math.ceil(nx / region), math.ceil(ny / region), math.ceil(nz / region))
+    nreg = rx * ry * rz
+    u = rng.uniform(nreg)
+    mat = np.where(u < 0.6, 0, np.where(u < 0.8, 1, 2))
+    grain = np.minimum((rng.uniform(nreg) * 3).astype(np.int64), 2)
+    idx = np.arange(nx * ny * nz)
+    x = idx % nx
+    y = (idx // nx) % ny
+    z = idx // (nx * ny)
+    reg = (z // region) * (ry * rx) + (y // region) * rx + (x // region)
+    return mat[reg], grain[reg]
+
+
+def _finish_mixed(s: Scene, mat, rng: SplitMix64):
+    s.anisotropy = np.where(mat == 0, 1.0, 0.0).astype(np.float32)
+    s.kscale = (0.95 + 0.1 * rng.uniform(3 * s.n)).astype(np.float32).reshape(3, s.n)
+
+
+def mixed_box(dims, seed: int, fixed_face: str = "z-") -> Scene:
+    """A box with regional wood/iron/stone materials, per-bond stiffness jitter
+    U[0.95, 1.05] and one face fixed with Scene::fix_voxels semantics (the
+    voxels become kinematic but keep their bonds)."""
+    rng = SplitMix64(seed)
+    mat, grain = _regional_materials(dims, 8, rng)
+    n = dims[0] * dims[1] * dims[2]
+    s = _box(dims, mat, grain, 0.0, np.zeros(n, np.uint8), ["wood", "iron", "stone"])
+    _finish_mixed(s, mat, rng)
+    axis = "xyz".index(fixed_face[0])
+    coord = s.keys[:, axis]
+    face = coord == (coord.min() if fixed_face[1] == "-" else coord.max())
+    s.kinematic = face.astype(np.uint8)
+    return s
+
+
+def large_64cube(seed: int = 2, dims=(64, 64, 64)) -> Scene:
+    """Single large mixed scene: 64^3 REVs, materials per 8^3 region, bottom
+    layer fixed, gravity -9.81 Z, persistent +0.2 N Z force on a random 5% of
+    the top layer (drawn without replacement), 1 ms steps of one substep."""
+    s = mixed_box(dims, seed, "z-")
+    rng = SplitMix64(seed ^ 0x5EED)
+    z = s.keys[:, 2]
+    top = np.nonzero(z == z.max())[0]
+    count = int(round(0.05 * top.size))
+    order = np.argsort(rng.uniform(top.size), kind="stable")
+    chosen = top[order[:count]]
+    s.loads[chosen, 2] = np.float32(0.2)
+    s.config = StepConfig(fixed_dt=0.001, substeps=1, iterations=1000, corotation_iterations=2,
+                          solver_relative_tolerance=1e-6, gravity=(0.0, 0.0, -9.81))
+    s.name = "large_%dx%dx%d_mixed" % tuple(dims)
+    return s
+
+
+def batch_boxes(count: int = 256, seed: int = 3, lo: int = 8, hi: int = 48):
+    """Independent boxes with integer edges U[lo, hi] per axis, regional
+    materials, one random face fixed and initial velocities N(0, 0.05 m/s)
+    per component on the free voxels."""
+    rng = SplitMix64(seed)
+    edges = lo + np.minimum((rng.uniform(3 * count) * (hi - lo + 1)).astype(np.int64), hi - lo)
+    faces = np.minimum((rng.uniform(count) * 6).astype(np.int64), 5)
+    scenes = []
+    for b in range(count):
+        dims = tuple(int(e) for e in edges[3 * b:3 * b + 3])
+        face = "xyz"[faces[b] // 2] + "-+"[faces[b] % 2]
+        s = mixed_box(dims, seed * 1_000_003 + b, face)
+        vrng = SplitMix64(seed * 7_000_001 + b)
+        v = (0.05 * vrng.normal(3 * s.n)).astype(np.float32).reshape(s.n, 3)
+        v[s.kinematic != 0] = 0
+        s.velocities = v
+        s.config = StepConfig(fixed_dt=0.001, substeps=1, iterations=1000, corotation_iterations=2,
+                              solver_relative_tolerance=1e-6, gravity=(0.0, 0.0, -9.81))
+        s.name = "batch%03d_%dx%dx%d" % ((b,) + dims)
+        scenes.append(s)
+    return scenes
+
+
+def rotate_scene(s: Scene, quat_xyzw) -> Scene:
+    """Rigidly rotate all positions about the origin (as the reference's
+    rest_and_rigid_rotation_have_no_strain test does, src/solver.rs:1008-1021)."""
+    x, y, z, w = [float(c) for c in quat_xyzw]
+    r = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                  [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                  [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+    out = s.copy()
+    out.positions = (s.positions.astype(np.float64) @ r.T).astype(np.float32)
+    return out
