@@ -1,0 +1,212 @@
+/*
+ * vx_solver.h — C ABI of the MI355X (gfx950) corotational elastic PCG solver.
+ *
+ * Drop-in boundary: vx_solve_pcg replaces the reference's device solve
+ *   pub(crate) fn solve_pcg(rhs: &[Vec3], neighbors: &[[u32; 6]],
+ *       diagonal: &[[f32; 9]], blocks: &[[f32; 54]],
+ *       inverse_diagonal: &[[f32; 9]], max_iterations: usize, tolerance: f32)
+ *       -> Result<(Vec<Vec3>, usize, f32), String>
+ * (src/cuda_solver.rs:301-309, called from src/solver.rs:1778-1786).
+ * Plain pointers and sizes only; the caller owns every buffer.
+ *
+ * Every function returns VX_OK (0) or a nonzero status; vx_last_error()
+ * returns the calling thread's message for the last nonzero status. The
+ * nonzero statuses correspond to the reference's Err branches, so a caller
+ * keeps its CPU fallback (src/solver.rs:1792-1808).
+ *
+ * All entry points serialise on one process-wide device context (device 0,
+ * one HIP stream), as the reference serialises on its CUDA context mutex
+ * (src/cuda_solver.rs:66, 326-336).
+ */
+#ifndef VX_SOLVER_H
+#define VX_SOLVER_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define VX_OK 0
+/* Device/runtime initialisation failed; cached for the process lifetime like
+ * the reference's OnceLock (src/cuda_solver.rs:326-332). */
+#define VX_ERR_INIT 1
+/* Problem does not fit: the reference's "at most u32::MAX active REV"
+ * (src/cuda_solver.rs:314-315), or device memory exhausted. */
+#define VX_ERR_TOO_LARGE 2
+/* "produced a non-finite residual norm" (src/cuda_solver.rs:470-472, 484-486). */
+#define VX_ERR_NONFINITE_RESIDUAL 3
+/* "did not converge" (src/cuda_solver.rs:487-492). */
+#define VX_ERR_NOT_CONVERGED 4
+/* "produced a non-finite correction" (src/cuda_solver.rs:493-495). */
+#define VX_ERR_NONFINITE_CORRECTION 5
+/* Malformed arguments (null pointer with n > 0, a neighbour index that is
+ * neither EMPTY nor < n, unknown handle). No reference counterpart: the
+ * reference would read out of bounds on such input. */
+#define VX_ERR_INVALID_ARGUMENT 6
+/* A HIP runtime call failed after initialisation. */
+#define VX_ERR_DEVICE 7
+
+#define VX_EMPTY 0xFFFFFFFFu
+
+/* Thread-local message of the last nonzero status returned on this thread. */
+const char *vx_last_error(void);
+
+/* Initialise the device context (optional; every entry point does it). */
+int32_t vx_init(void);
+
+/* ---- drop-in solve ----------------------------------------------------- */
+/*
+ * vx_solve_pcg — replaces cuda_solver::solve_pcg (src/cuda_solver.rs:301-501).
+ *   rhs               n*3   f32, right-hand side (reference Vec3 layout)
+ *   neighbors         n*6   u32, per-row neighbour slots, VX_EMPTY if absent
+ *   diagonal          n*9   f32, row-major 3x3 diagonal blocks
+ *   blocks            n*54  f32, 6 row-major 3x3 off-diagonal blocks per row
+ *   inverse_diagonal  n*9   f32, row-major inverse of each diagonal block
+ *   n                 active rows
+ *   max_iterations    cap (the reference passes config.cuda_iterations)
+ *   tolerance         relative residual ||r|| / ||rhs||
+ *   out_correction    n*3   f32, caller-owned
+ *   out_iterations, out_relative_residual
+ * Semantics follow the reference device solve: block-Jacobi PCG with f32
+ * vectors, f64 dot products and f32 alpha/beta, the residual checked every
+ * 16 iterations and at the cap with `<=`. n == 0 and ||rhs|| <= 1e-12 return
+ * VX_OK with a zero correction without touching the device.
+ */
+int32_t vx_solve_pcg(const float *rhs, const uint32_t *neighbors, const float *diagonal,
+                     const float *blocks, const float *inverse_diagonal, uint32_t n,
+                     uint32_t max_iterations, float tolerance, float *out_correction,
+                     uint32_t *out_iterations, float *out_relative_residual);
+
+/*
+ * vx_solve_pcg_f64 — the plain FP64 GPU solve over the same reference
+ * layout: inputs widened to f64, block inverses recomputed in f64 from the
+ * diagonal, FP64 vectors, scalars and reductions, same convergence checks and
+ * statuses as vx_solve_pcg. No reference counterpart (the reference's device
+ * solve is f32); it is the drop-in comparison baseline.
+ */
+int32_t vx_solve_pcg_f64(const float *rhs, const uint32_t *neighbors, const float *diagonal,
+                         const float *blocks, const float *inverse_diagonal, uint32_t n,
+                         uint32_t max_iterations, float tolerance, float *out_correction,
+                         uint32_t *out_iterations, float *out_relative_residual);
+
+/* ---- device-resident structured scenes (build extensions) --------------- */
+/*
+ * A box scene of nx*ny*nz REVs kept resident in HBM: SoA state, on-device
+ * rotation fit (src/solver.rs:327-417), bond assembly
+ * (src/solver.rs:1542-1636), and a mixed-precision PCG (FP32 inner
+ * block-Jacobi PCG with f64 reductions, FP64 true-residual refinement). The
+ * reference has no counterpart beyond solve_pcg; these entry points carry the
+ * same status/error convention.
+ *
+ * Voxel index v = x + nx*(y + ny*z). Bonds are stored per +axis:
+ * neighbor[a*n + v] is v + stride_a or VX_EMPTY.
+ */
+typedef struct vx_scene_desc {
+    uint32_t nx, ny, nz;
+    const float *positions;        /* n*3 */
+    const float *rotations;        /* n*4 quaternion (x, y, z, w) */
+    const float *velocities;       /* n*3 */
+    const float *mass;             /* n */
+    const uint8_t *kinematic;      /* n */
+    const float *young;            /* n, modulus along the fibre */
+    const float *young_perp;       /* n */
+    const float *shear;            /* n */
+    const float *solid;            /* n, solid fraction */
+    const float *fibers;           /* n*3, body-frame fibre direction */
+    const float *anisotropy;       /* n */
+    const float *delta_temp;       /* n, temperature minus stress-free temperature */
+    const float *alpha_par;        /* n */
+    const float *alpha_perp;       /* n */
+    const uint32_t *neighbor;      /* 3*n */
+    const float *damage;           /* 3*n, may be NULL (zero) */
+    const float *kscale;           /* 3*n, may be NULL (one) */
+    const float *formation_offset; /* 3*n*3, may be NULL (zero) */
+    const float *plastic_strain;   /* 3*n*3, may be NULL (zero) */
+    const float *targets;          /* n*3 world coordinates, NaN = free axis; may be NULL */
+    const float *loads;            /* n*3 persistent applied loads; may be NULL */
+} vx_scene_desc;
+
+typedef struct vx_scene vx_scene;
+
+typedef struct vx_solve_params {
+    double tolerance;          /* outer FP64 relative true residual ||b - A x|| / ||b|| */
+    double inner_tolerance;    /* FP32 inner PCG relative residual per refinement sweep */
+    uint32_t max_inner;        /* inner iteration cap per sweep */
+    uint32_t max_outer;        /* refinement sweeps cap */
+    uint32_t warm_start;       /* 1: start from the scene's last correction */
+    uint32_t mode;             /* VX_SOLVE_MIXED or VX_SOLVE_F64 */
+} vx_solve_params;
+
+#define VX_SOLVE_MIXED 0 /* FP32 inner PCG + FP64 refinement */
+#define VX_SOLVE_F64 1   /* plain FP64 PCG on the structured layout */
+
+typedef struct vx_solve_report {
+    uint32_t inner_iterations; /* summed over sweeps */
+    uint32_t outer_sweeps;     /* FP64 true-residual evaluations */
+    uint32_t active_rows;
+    uint32_t status;
+    double relative_residual;  /* final FP64 true relative residual */
+    double rhs_norm;
+} vx_solve_report;
+
+int32_t vx_scene_create(const vx_scene_desc *desc, vx_scene **out);
+int32_t vx_scene_destroy(vx_scene *scene);
+/* Predict (gravity, targets) for one substep of length h; stores the
+ * predicted positions used by assembly. */
+int32_t vx_scene_predict(vx_scene *scene, double h, const double *gravity3);
+/* Rotation fit; writes the largest angle change. */
+int32_t vx_scene_fit_rotations(vx_scene *scene, double *out_max_angle);
+/* Assemble (M/h^2 + J^T K J) dx = -J^T K C for the current state. */
+int32_t vx_scene_assemble(vx_scene *scene, double h);
+/* Solve the assembled system; the correction stays on the device. */
+int32_t vx_scene_solve(vx_scene *scene, const vx_solve_params *params, vx_solve_report *report);
+/* positions += correction on the active rows. */
+int32_t vx_scene_apply(vx_scene *scene);
+/* Velocities from the solved positions after a substep of length h. */
+int32_t vx_scene_finish_substep(vx_scene *scene, double h);
+/* Copy state out (any pointer may be NULL). positions/velocities n*3,
+ * rotations n*4, correction n*3 (zero on inactive rows). */
+int32_t vx_scene_download(vx_scene *scene, double *positions, double *rotations, double *velocities,
+                          double *correction);
+/* Export the assembled system in the reference packing (src/solver.rs:
+ * 1652-1672, 1731-1777) for the active rows: neighbours (active*6), blocks
+ * (active*54 row-major), diagonal (active*9), rhs (active*3) in f64, and the
+ * voxel index of each active row. Any pointer may be NULL. */
+int32_t vx_scene_export_rows(vx_scene *scene, uint32_t *active, uint32_t *neighbors, double *blocks,
+                             double *diagonal, double *rhs, uint32_t *out_active_count);
+/* Mean duration (microseconds) of the FP32 structured bond SpMV kernel over
+ * `reps` launches on the library stream, each bracketed by HIP events. */
+int32_t vx_scene_time_spmv(vx_scene *scene, uint32_t reps, double *out_us);
+
+/* One fixed step of the solve part of step_with_axis_targets_and_loads
+ * (src/solver.rs:475-570, 668-683) on the device: per substep predict, fit
+ * rotations, up to corotation_iterations passes of assemble + solve + apply
+ * with the reference's early exit, then velocities. Contact, tool, fracture
+ * and heat are not part of it. */
+typedef struct vx_step_params {
+    double fixed_dt;
+    double gravity[3];
+    uint32_t substeps;
+    uint32_t corotation_iterations;
+    double corotation_tolerance;
+    vx_solve_params solve;
+} vx_step_params;
+
+typedef struct vx_step_report {
+    uint32_t solves;
+    uint32_t inner_iterations;   /* summed over solves */
+    uint32_t outer_sweeps;       /* summed over solves */
+    uint32_t active_rows;
+    double last_relative_residual;
+    double last_angle;
+    double seconds;              /* device wall time of the step */
+} vx_step_report;
+
+int32_t vx_scene_step(vx_scene *scene, const vx_step_params *params, vx_step_report *report);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,142 @@
+// Shared runtime pieces of the gfx950 PCG library: the process-wide device
+// context, thread-local error messages, grow-only device buffers and the
+// wave64 / workgroup reduction helpers used by every kernel.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <mutex>
+#include <string>
+
+#include "../../include/vx_solver.h"
+
+namespace vx {
+
+constexpr int kBlock = 256;     // 4 wave64s per workgroup
+constexpr int kWaves = kBlock / 64;
+constexpr int kXcds = 8;        // MI355X: 8 XCDs, workgroups dispatched round-robin
+
+// ---- errors -----------------------------------------------------------------
+int32_t fail(int32_t code, const char *fmt, ...);
+void clear_error();
+
+#define VX_HIP(call)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess) {                                                             \
+            return ::vx::fail(e_ == hipErrorOutOfMemory ? VX_ERR_TOO_LARGE : VX_ERR_DEVICE, \
+                              "%s failed: %s", #call, hipGetErrorString(e_));               \
+        }                                                                                   \
+    } while (0)
+
+// ---- device context ----------------------------------------------------------
+// One context per process on device 0, serialised by a mutex, as the
+// reference's OnceLock<Mutex<CudaSolver>> (src/cuda_solver.rs:66, 326-336).
+struct DeviceBuffer {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+};
+
+struct Context {
+    bool initialised = false;
+    int32_t init_status = VX_OK;
+    std::string init_message;
+    hipStream_t stream = nullptr;
+    std::recursive_mutex mutex;
+    std::map<std::string, DeviceBuffer> buffers;  // named, grow-only (src/cuda_solver.rs:162-193)
+    void *pinned = nullptr;                       // small pinned host area for status words
+    size_t pinned_bytes = 0;
+};
+
+Context &context();
+// Initialise once; a failure is cached for the process lifetime.
+int32_t ensure_init();
+// Named grow-only device buffer of at least `bytes`.
+int32_t buffer(const char *name, size_t bytes, void **out);
+
+// XCD-aware tile index: consecutive logical tiles land on the same XCD, so
+// the z-neighbour rows a tile reads were fetched into that XCD's L2 by the
+// tiles beside it. Hardware assigns workgroup b to XCD b % 8.
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t per_xcd) {
+    return (b % kXcds) * per_xcd + b / kXcds;
+}
+inline uint32_t xcd_grid(uint32_t tiles, uint32_t *per_xcd) {
+    *per_xcd = (tiles + kXcds - 1) / kXcds;
+    return *per_xcd * kXcds;
+}
+
+// ---- reductions -------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// Workgroup sum of N values (fixed order, deterministic). Result valid in
+// thread 0.
+template <int N>
+__device__ __forceinline__ void block_sum(double (&v)[N], double (*lds)[kWaves]) {
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    __syncthreads();  // a previous block_sum's reader is done with lds
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        double s = wave_sum(v[k]);
+        if (lane == 0) lds[k][wave] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            double s = lds[k][0];
+#pragma unroll
+            for (int w = 1; w < kWaves; ++w) s += lds[k][w];
+            v[k] = s;
+        }
+    }
+}
+
+// Publishes this workgroup's N partial sums and returns true in every thread
+// of the workgroup that arrives last; that workgroup then sees all partials.
+// Partials are laid out partials[k * stride + block].
+template <int N>
+__device__ __forceinline__ bool publish_partials(const double (&v)[N], double *partials, uint32_t stride,
+                                                 uint32_t *ticket, uint32_t nblocks, bool *lds_last) {
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) partials[size_t(k) * stride + blockIdx.x] = v[k];
+        uint32_t prev = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        *lds_last = (prev == nblocks - 1);
+    }
+    __syncthreads();
+    bool last = *lds_last;
+    if (last) __threadfence();
+    return last;
+}
+
+// In the last workgroup: deterministic sum of N partial arrays.
+template <int N>
+__device__ __forceinline__ void reduce_partials(double (&out)[N], const double *partials, uint32_t stride,
+                                                uint32_t nblocks, double (*lds)[kWaves]) {
+    double v[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        double s = 0.0;
+        for (uint32_t b = threadIdx.x; b < nblocks; b += kBlock) s += partials[size_t(k) * stride + b];
+        v[k] = s;
+    }
+    block_sum<N>(v, lds);
+#pragma unroll
+    for (int k = 0; k < N; ++k) out[k] = v[k];
+}
+
+}  // namespace vx

@@ -1,0 +1,492 @@
+// Drop-in device solve over the reference row packing: vx_solve_pcg (f32
+// vectors, f64 reductions, f32 alpha/beta, residual check every 16
+// iterations — the semantics of src/cuda_solver.rs:301-501 and the kernels of
+// src/cuda/pcg.ptx) and vx_solve_pcg_f64 (the same algorithm in FP64).
+//
+// MI355X design: the caller's array-of-structs rows are uploaded once per
+// solve (the boundary hands over host slices) and re-laid out on the device
+// into structure-of-arrays planes, so every PCG kernel streams coalesced
+// 256-byte wave loads. One iteration is two kernels:
+//   rows_spmv   : p = z + beta*p_old formed on load (own row and neighbour
+//                 rows), Ap = A p, per-workgroup f64 partials of p.Ap; the last
+//                 workgroup computes alpha on the device;
+//   rows_update : dx += alpha p, r -= alpha Ap, z = Dinv r, partials of r.z
+//                 and r.r; the last workgroup computes beta and checks
+//                 convergence on the device.
+// Sixteen iterations are captured once into a hipGraph and replayed; the
+// host reads a pinned status word one graph behind, so the device never
+// waits for the host. Scalars never leave the device.
+#include "vx_common.hpp"
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace vx {
+namespace {
+
+constexpr uint32_t kChunk = 16;  // iterations per graph replay = reference check interval
+
+// Device-resident PCG scalars.
+struct RowsState {
+    double rho;         // r.z of the current iterate
+    double rr_checked;  // r.r at the last convergence check
+    double initial;     // ||rhs||
+    double alpha, beta; // exact values of the vector type
+    double tol;         // tolerance in the vector type
+    double rel;         // relative residual at the last check
+    uint32_t iters, done, max_iter, check_every;
+    uint32_t invalid;
+    uint32_t ticket_a, ticket_b, pad;
+};
+enum : uint32_t { kRunning = 0, kConverged = 1, kNonFinite = 2, kCapReached = 3 };
+
+template <class S> __device__ __forceinline__ S fma_s(S a, S b, S c);
+template <> __device__ __forceinline__ float fma_s<float>(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+template <> __device__ __forceinline__ double fma_s<double>(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// Guarded quotient of scalar_ratio / dot_reduce_ratio_single: num/den cast to
+// the vector type when den > 1e-30, else 0.
+template <class S> __device__ __forceinline__ double guarded_ratio(double num, double den) {
+    return den > 1e-30 ? double(S(num / den)) : 0.0;
+}
+
+// Per-row dot product with f64 products of the widened elements.
+template <class S> __device__ __forceinline__ double row_dot(const S (&a)[3], const S (&b)[3]) {
+    double v = double(a[0]) * double(b[0]);
+    v = __builtin_fma(double(a[1]), double(b[1]), v);
+    v = __builtin_fma(double(a[2]), double(b[2]), v);
+    return v;
+}
+
+// Inverse of a row-major 3x3 by cofactors, in double.
+__device__ __forceinline__ void inverse3(const double (&a)[9], double (&o)[9]) {
+    double c00 = a[4] * a[8] - a[5] * a[7];
+    double c01 = a[5] * a[6] - a[3] * a[8];
+    double c02 = a[3] * a[7] - a[4] * a[6];
+    double inv = 1.0 / (a[0] * c00 + a[1] * c01 + a[2] * c02);
+    o[0] = c00 * inv;
+    o[1] = (a[2] * a[7] - a[1] * a[8]) * inv;
+    o[2] = (a[1] * a[5] - a[2] * a[4]) * inv;
+    o[3] = c01 * inv;
+    o[4] = (a[0] * a[8] - a[2] * a[6]) * inv;
+    o[5] = (a[2] * a[3] - a[0] * a[5]) * inv;
+    o[6] = c02 * inv;
+    o[7] = (a[1] * a[6] - a[0] * a[7]) * inv;
+    o[8] = (a[0] * a[4] - a[1] * a[3]) * inv;
+}
+
+// AoS rows -> SoA planes. Plane q of an array with Q values per row lives at
+// base + q*n. Neighbour indices that are neither EMPTY nor < n are flagged
+// and treated as absent.
+template <class S>
+__global__ __launch_bounds__(kBlock) void rows_relayout(uint32_t n, const uint32_t *__restrict__ nbr_in,
+                                                        const float *__restrict__ blk_in,
+                                                        const float *__restrict__ diag_in,
+                                                        const float *__restrict__ inv_in,
+                                                        const float *__restrict__ rhs_in, uint32_t *__restrict__ nbr,
+                                                        S *__restrict__ blk, S *__restrict__ diag, S *__restrict__ inv,
+                                                        S *__restrict__ r, S *__restrict__ dx, S *__restrict__ p0,
+                                                        RowsState *st, int widen_inverse) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const size_t N = n;
+    for (int s = 0; s < 6; ++s) {
+        uint32_t j = nbr_in[6 * size_t(i) + s];
+        if (j != VX_EMPTY && j >= n) {
+            st->invalid = 1;
+            j = VX_EMPTY;
+        }
+        nbr[s * N + i] = j;
+    }
+    for (int q = 0; q < 54; ++q) blk[q * N + i] = S(blk_in[54 * size_t(i) + q]);
+    double d[9];
+    for (int q = 0; q < 9; ++q) {
+        float v = diag_in[9 * size_t(i) + q];
+        diag[q * N + i] = S(v);
+        d[q] = double(S(v));
+    }
+    if (widen_inverse) {
+        for (int q = 0; q < 9; ++q) inv[q * N + i] = S(inv_in[9 * size_t(i) + q]);
+    } else {
+        double o[9];
+        inverse3(d, o);
+        for (int q = 0; q < 9; ++q) inv[q * N + i] = S(o[q]);
+    }
+    for (int c = 0; c < 3; ++c) {
+        r[c * N + i] = S(rhs_in[3 * size_t(i) + c]);
+        dx[c * N + i] = S(0);
+        p0[c * N + i] = S(0);
+    }
+}
+
+template <class S>
+__device__ __forceinline__ void apply_inv(const S *__restrict__ inv, size_t N, uint32_t i, const S (&r)[3], S (&z)[3]) {
+#pragma unroll
+    for (int row = 0; row < 3; ++row) {
+        S v = inv[(3 * row) * N + i] * r[0];
+        v = fma_s<S>(inv[(3 * row + 1) * N + i], r[1], v);
+        v = fma_s<S>(inv[(3 * row + 2) * N + i], r[2], v);
+        z[row] = v;
+    }
+}
+
+// z = Dinv r and rho = r.z; beta = 0 for the first direction.
+template <class S>
+__global__ __launch_bounds__(kBlock) void rows_init(uint32_t n, uint32_t per_xcd, uint32_t ntiles,
+                                                    const S *__restrict__ inv, const S *__restrict__ r,
+                                                    S *__restrict__ z, RowsState *st, double *partials) {
+    __shared__ double lds[1][kWaves];
+    __shared__ bool last;
+    const size_t N = n;
+    const uint32_t tile = xcd_tile(blockIdx.x, per_xcd);
+    const uint32_t i = tile * kBlock + threadIdx.x;
+    double v[1] = {0.0};
+    if (tile < ntiles && i < n) {
+        S ri[3] = {r[i], r[N + i], r[2 * N + i]};
+        S zi[3];
+        apply_inv<S>(inv, N, i, ri, zi);
+        z[i] = zi[0];
+        z[N + i] = zi[1];
+        z[2 * N + i] = zi[2];
+        v[0] = row_dot<S>(ri, zi);
+    }
+    block_sum<1>(v, lds);
+    if (publish_partials<1>(v, partials, gridDim.x, &st->ticket_a, gridDim.x, &last)) {
+        double tot[1];
+        reduce_partials<1>(tot, partials, gridDim.x, gridDim.x, lds);
+        if (threadIdx.x == 0) {
+            st->rho = tot[0];
+            st->beta = 0.0;
+            st->ticket_a = 0;
+        }
+    }
+}
+
+// Ap = A p with p = z + beta*p_old formed on load; alpha = rho / p.Ap.
+template <class S>
+__global__ __launch_bounds__(kBlock) void rows_spmv(uint32_t n, uint32_t per_xcd, uint32_t ntiles,
+                                                    const uint32_t *__restrict__ nbr, const S *__restrict__ blk,
+                                                    const S *__restrict__ diag, const S *__restrict__ z,
+                                                    const S *__restrict__ p_old, S *__restrict__ p_new,
+                                                    S *__restrict__ ap, RowsState *st, double *partials) {
+    if (st->done) return;
+    __shared__ double lds[1][kWaves];
+    __shared__ bool last;
+    const size_t N = n;
+    const uint32_t tile = xcd_tile(blockIdx.x, per_xcd);
+    const uint32_t i = tile * kBlock + threadIdx.x;
+    double v[1] = {0.0};
+    if (tile < ntiles && i < n) {
+        const S beta = S(st->beta);
+        S p[3], y[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) p[c] = fma_s<S>(beta, p_old[c * N + i], z[c * N + i]);
+#pragma unroll
+        for (int row = 0; row < 3; ++row) {
+            S a = diag[(3 * row) * N + i] * p[0];
+            a = fma_s<S>(diag[(3 * row + 1) * N + i], p[1], a);
+            a = fma_s<S>(diag[(3 * row + 2) * N + i], p[2], a);
+            y[row] = a;
+        }
+#pragma unroll
+        for (int s = 0; s < 6; ++s) {
+            const uint32_t j = nbr[s * N + i];
+            if (j == VX_EMPTY) continue;
+            S pj[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) pj[c] = fma_s<S>(beta, p_old[c * N + j], z[c * N + j]);
+#pragma unroll
+            for (int row = 0; row < 3; ++row) {
+                const S *b = blk + (9 * s + 3 * row) * N + i;
+                y[row] = fma_s<S>(b[0], pj[0], y[row]);
+                y[row] = fma_s<S>(b[N], pj[1], y[row]);
+                y[row] = fma_s<S>(b[2 * N], pj[2], y[row]);
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            p_new[c * N + i] = p[c];
+            ap[c * N + i] = y[c];
+        }
+        v[0] = row_dot<S>(p, y);
+    }
+    block_sum<1>(v, lds);
+    if (publish_partials<1>(v, partials, gridDim.x, &st->ticket_a, gridDim.x, &last)) {
+        double tot[1];
+        reduce_partials<1>(tot, partials, gridDim.x, gridDim.x, lds);
+        if (threadIdx.x == 0) {
+            st->alpha = guarded_ratio<S>(st->rho, tot[0]);
+            st->ticket_a = 0;
+        }
+    }
+}
+
+// dx += alpha p; r -= alpha Ap; z = Dinv r; beta = r.z / rho; the residual
+// check every check_every iterations and at the cap.
+template <class S>
+__global__ __launch_bounds__(kBlock) void rows_update(uint32_t n, uint32_t per_xcd, uint32_t ntiles,
+                                                      const S *__restrict__ inv, S *__restrict__ dx,
+                                                      S *__restrict__ r, S *__restrict__ z,
+                                                      const S *__restrict__ p, const S *__restrict__ ap,
+                                                      RowsState *st, double *partials) {
+    if (st->done) return;
+    __shared__ double lds[2][kWaves];
+    __shared__ bool last;
+    const size_t N = n;
+    const uint32_t tile = xcd_tile(blockIdx.x, per_xcd);
+    const uint32_t i = tile * kBlock + threadIdx.x;
+    double v[2] = {0.0, 0.0};
+    if (tile < ntiles && i < n) {
+        const S alpha = S(st->alpha);
+        S ri[3], zi[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            dx[c * N + i] = fma_s<S>(alpha, p[c * N + i], dx[c * N + i]);
+            ri[c] = fma_s<S>(-alpha, ap[c * N + i], r[c * N + i]);
+            r[c * N + i] = ri[c];
+        }
+        apply_inv<S>(inv, N, i, ri, zi);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) z[c * N + i] = zi[c];
+        v[0] = row_dot<S>(ri, zi);
+        v[1] = row_dot<S>(ri, ri);
+    }
+    block_sum<2>(v, lds);
+    if (publish_partials<2>(v, partials, gridDim.x, &st->ticket_b, gridDim.x, &last)) {
+        double tot[2];
+        reduce_partials<2>(tot, partials, gridDim.x, gridDim.x, lds);
+        if (threadIdx.x == 0) {
+            const double rho_next = tot[0];
+            st->beta = guarded_ratio<S>(rho_next, st->rho);
+            st->rho = rho_next;
+            const uint32_t it = st->iters + 1;
+            st->iters = it;
+            if ((st->check_every && it % st->check_every == 0) || it == st->max_iter) {
+                const double rr = tot[1];
+                st->rr_checked = rr;
+                if (!isfinite(rr) || rr < 0.0) {
+                    st->done = kNonFinite;
+                } else {
+                    const S rel = S(sqrt(rr) / st->initial);
+                    st->rel = double(rel);
+                    if (double(rel) <= st->tol) st->done = kConverged;
+                }
+            }
+            if (!st->done && it >= st->max_iter) st->done = kCapReached;
+            st->ticket_b = 0;
+        }
+    }
+}
+
+// SoA correction -> caller's AoS f32 layout.
+template <class S>
+__global__ __launch_bounds__(kBlock) void rows_pack_out(uint32_t n, const S *__restrict__ dx, float *__restrict__ out) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const size_t N = n;
+    for (int c = 0; c < 3; ++c) out[3 * size_t(i) + c] = float(dx[c * N + i]);
+}
+
+// One captured 16-iteration graph per (precision, size, buffers) key.
+struct RowsGraph {
+    hipGraphExec_t exec = nullptr;
+    const void *key[4] = {nullptr, nullptr, nullptr, nullptr};
+    uint32_t n = 0;
+};
+RowsGraph g_graph[2];
+
+template <class S>
+int32_t solve_rows(const float *rhs, const uint32_t *neighbors, const float *diagonal, const float *blocks,
+                   const float *inverse_diagonal, uint32_t n, uint32_t max_iterations, float tolerance,
+                   float *out_correction, uint32_t *out_iterations, float *out_relative_residual) {
+    constexpr bool kF64 = sizeof(S) == 8;
+    clear_error();
+    if (n > 0 && (!rhs || !neighbors || !diagonal || !blocks || !inverse_diagonal || !out_correction))
+        return fail(VX_ERR_INVALID_ARGUMENT, "null array with n=%u", n);
+    if (n == 0) {
+        if (out_iterations) *out_iterations = 0;
+        if (out_relative_residual) *out_relative_residual = 0.0f;
+        return VX_OK;
+    }
+    // ||rhs|| in f64 on the host (src/cuda_solver.rs:316-324).
+    double initial_sq = 0.0;
+    for (size_t k = 0; k < 3 * size_t(n); ++k) initial_sq += double(rhs[k]) * double(rhs[k]);
+    const double initial = std::sqrt(initial_sq);
+    if (initial <= 1e-12) {
+        std::memset(out_correction, 0, sizeof(float) * 3 * size_t(n));
+        if (out_iterations) *out_iterations = 0;
+        if (out_relative_residual) *out_relative_residual = 0.0f;
+        return VX_OK;
+    }
+    int32_t st_code = ensure_init();
+    if (st_code != VX_OK) return st_code;
+    Context &ctx = context();
+    std::lock_guard<std::recursive_mutex> lock(ctx.mutex);
+    hipStream_t stream = ctx.stream;
+    VX_HIP(hipSetDevice(0));
+    const size_t N = n;
+    uint32_t per_xcd = 0;
+    const uint32_t ntiles = (n + kBlock - 1) / kBlock;
+    const uint32_t grid = xcd_grid(ntiles, &per_xcd);
+
+    void *s_nbr, *s_blk, *s_diag, *s_inv, *s_rhs;
+    void *d_nbr, *d_blk, *d_diag, *d_inv, *d_dx, *d_r, *d_z, *d_p0, *d_p1, *d_ap, *d_part, *d_state, *d_out;
+    const char *pre = kF64 ? "rows64_" : "rows32_";
+    auto B = [&](const char *name, size_t bytes, void **out) {
+        std::string key = std::string(pre) + name;
+        return buffer(key.c_str(), bytes, out);
+    };
+#define VX_BUF(name, bytes, ptr)                      \
+    do {                                              \
+        int32_t c_ = B(name, bytes, ptr);             \
+        if (c_ != VX_OK) return c_;                   \
+    } while (0)
+    VX_BUF("stage_nbr", 6 * N * 4, &s_nbr);
+    VX_BUF("stage_blk", 54 * N * 4, &s_blk);
+    VX_BUF("stage_diag", 9 * N * 4, &s_diag);
+    VX_BUF("stage_inv", 9 * N * 4, &s_inv);
+    VX_BUF("stage_rhs", 3 * N * 4, &s_rhs);
+    VX_BUF("nbr", 6 * N * 4, &d_nbr);
+    VX_BUF("blk", 54 * N * sizeof(S), &d_blk);
+    VX_BUF("diag", 9 * N * sizeof(S), &d_diag);
+    VX_BUF("inv", 9 * N * sizeof(S), &d_inv);
+    VX_BUF("dx", 3 * N * sizeof(S), &d_dx);
+    VX_BUF("r", 3 * N * sizeof(S), &d_r);
+    VX_BUF("z", 3 * N * sizeof(S), &d_z);
+    VX_BUF("p0", 3 * N * sizeof(S), &d_p0);
+    VX_BUF("p1", 3 * N * sizeof(S), &d_p1);
+    VX_BUF("ap", 3 * N * sizeof(S), &d_ap);
+    VX_BUF("partials", 2 * size_t(grid) * 8, &d_part);
+    VX_BUF("state", sizeof(RowsState), &d_state);
+    VX_BUF("out", 3 * N * 4, &d_out);
+#undef VX_BUF
+
+    VX_HIP(hipMemcpyAsync(s_nbr, neighbors, 6 * N * 4, hipMemcpyHostToDevice, stream));
+    VX_HIP(hipMemcpyAsync(s_blk, blocks, 54 * N * 4, hipMemcpyHostToDevice, stream));
+    VX_HIP(hipMemcpyAsync(s_diag, diagonal, 9 * N * 4, hipMemcpyHostToDevice, stream));
+    VX_HIP(hipMemcpyAsync(s_inv, inverse_diagonal, 9 * N * 4, hipMemcpyHostToDevice, stream));
+    VX_HIP(hipMemcpyAsync(s_rhs, rhs, 3 * N * 4, hipMemcpyHostToDevice, stream));
+
+    RowsState hs;
+    std::memset(&hs, 0, sizeof hs);
+    hs.initial = initial;
+    hs.tol = kF64 ? double(tolerance) : double(float(tolerance));
+    hs.rel = INFINITY;
+    hs.rr_checked = INFINITY;
+    hs.max_iter = max_iterations;
+    hs.check_every = kChunk;
+    hs.done = max_iterations == 0 ? kCapReached : kRunning;
+    RowsState *pinned = static_cast<RowsState *>(ctx.pinned);
+    std::memcpy(&pinned[2], &hs, sizeof hs);
+    VX_HIP(hipMemcpyAsync(d_state, &pinned[2], sizeof hs, hipMemcpyHostToDevice, stream));
+
+    RowsState *st = static_cast<RowsState *>(d_state);
+    S *dx = static_cast<S *>(d_dx), *r = static_cast<S *>(d_r), *z = static_cast<S *>(d_z);
+    S *p0 = static_cast<S *>(d_p0), *p1 = static_cast<S *>(d_p1), *ap = static_cast<S *>(d_ap);
+    S *blk = static_cast<S *>(d_blk), *diag = static_cast<S *>(d_diag), *inv = static_cast<S *>(d_inv);
+    uint32_t *nbr = static_cast<uint32_t *>(d_nbr);
+    double *part = static_cast<double *>(d_part);
+
+    const uint32_t rgrid = ntiles;
+    hipLaunchKernelGGL(rows_relayout<S>, dim3(rgrid), dim3(kBlock), 0, stream, n,
+                       static_cast<const uint32_t *>(s_nbr), static_cast<const float *>(s_blk),
+                       static_cast<const float *>(s_diag), static_cast<const float *>(s_inv),
+                       static_cast<const float *>(s_rhs), nbr, blk, diag, inv, r, dx, p0, st, kF64 ? 0 : 1);
+    VX_HIP(hipGetLastError());
+    if (max_iterations > 0) {
+        hipLaunchKernelGGL(rows_init<S>, dim3(grid), dim3(kBlock), 0, stream, n, per_xcd, ntiles, inv, r, z, st, part);
+        VX_HIP(hipGetLastError());
+
+        RowsGraph &g = g_graph[kF64 ? 1 : 0];
+        const void *key[4] = {d_blk, d_p0, d_part, d_state};
+        bool reuse = g.exec && g.n == n && std::memcmp(key, g.key, sizeof key) == 0;
+        if (!reuse) {
+            if (g.exec) {
+                VX_HIP(hipGraphExecDestroy(g.exec));
+                g.exec = nullptr;
+            }
+            hipGraph_t graph;
+            VX_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+            for (uint32_t t = 0; t < kChunk; ++t) {
+                S *pold = (t % 2 == 0) ? p0 : p1;
+                S *pnew = (t % 2 == 0) ? p1 : p0;
+                hipLaunchKernelGGL(rows_spmv<S>, dim3(grid), dim3(kBlock), 0, stream, n, per_xcd, ntiles, nbr, blk,
+                                   diag, z, pold, pnew, ap, st, part);
+                hipLaunchKernelGGL(rows_update<S>, dim3(grid), dim3(kBlock), 0, stream, n, per_xcd, ntiles, inv, dx,
+                                   r, z, pnew, ap, st, part);
+            }
+            VX_HIP(hipStreamEndCapture(stream, &graph));
+            hipError_t e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (e != hipSuccess) return fail(VX_ERR_DEVICE, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+            g.n = n;
+            std::memcpy(g.key, key, sizeof key);
+        }
+        hipEvent_t ev[2];
+        VX_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+        VX_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+        uint32_t launched = 0, chunk = 0;
+        int32_t rc = VX_OK;
+        while (launched < max_iterations) {
+            hipError_t e = hipGraphLaunch(g.exec, stream);
+            if (e == hipSuccess) e = hipMemcpyAsync(&pinned[chunk % 2], d_state, sizeof hs, hipMemcpyDeviceToHost, stream);
+            if (e == hipSuccess) e = hipEventRecord(ev[chunk % 2], stream);
+            if (e != hipSuccess) {
+                rc = fail(VX_ERR_DEVICE, "PCG graph launch failed: %s", hipGetErrorString(e));
+                break;
+            }
+            launched += kChunk;
+            if (chunk >= 1) {
+                e = hipEventSynchronize(ev[(chunk - 1) % 2]);
+                if (e != hipSuccess) {
+                    rc = fail(VX_ERR_DEVICE, "PCG status read failed: %s", hipGetErrorString(e));
+                    break;
+                }
+                if (pinned[(chunk - 1) % 2].done) break;
+            }
+            chunk += 1;
+        }
+        (void)hipStreamSynchronize(stream);
+        (void)hipEventDestroy(ev[0]);
+        (void)hipEventDestroy(ev[1]);
+        if (rc != VX_OK) return rc;
+    }
+    hipLaunchKernelGGL(rows_pack_out<S>, dim3(rgrid), dim3(kBlock), 0, stream, n, dx, static_cast<float *>(d_out));
+    VX_HIP(hipGetLastError());
+    VX_HIP(hipMemcpyAsync(&pinned[2], d_state, sizeof hs, hipMemcpyDeviceToHost, stream));
+    VX_HIP(hipStreamSynchronize(stream));
+    const RowsState fin = pinned[2];
+    if (fin.invalid) return fail(VX_ERR_INVALID_ARGUMENT, "neighbour index out of range (n=%u)", n);
+    if (out_iterations) *out_iterations = fin.iters;
+    if (out_relative_residual) *out_relative_residual = float(fin.rel);
+    const char *dev = "HIP PCG";
+    if (fin.done == kNonFinite || !std::isfinite(fin.rr_checked) || fin.rr_checked < 0.0)
+        return fail(VX_ERR_NONFINITE_RESIDUAL, "%s produced a non-finite residual norm", dev);
+    if (!std::isfinite(fin.rel) || fin.rel > fin.tol)
+        return fail(VX_ERR_NOT_CONVERGED,
+                    "%s did not converge: iterations=%u, relative residual=%.3e, tolerance=%.3e", dev, fin.iters,
+                    fin.rel, double(tolerance));
+    VX_HIP(hipMemcpy(out_correction, d_out, 3 * N * 4, hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < 3 * N; ++k)
+        if (!std::isfinite(out_correction[k])) return fail(VX_ERR_NONFINITE_CORRECTION, "%s produced a non-finite correction", dev);
+    return VX_OK;
+}
+
+}  // namespace
+}  // namespace vx
+
+extern "C" int32_t vx_solve_pcg(const float *rhs, const uint32_t *neighbors, const float *diagonal, const float *blocks,
+                                const float *inverse_diagonal, uint32_t n, uint32_t max_iterations, float tolerance,
+                                float *out_correction, uint32_t *out_iterations, float *out_relative_residual) {
+    return vx::solve_rows<float>(rhs, neighbors, diagonal, blocks, inverse_diagonal, n, max_iterations, tolerance,
+                                 out_correction, out_iterations, out_relative_residual);
+}
+
+extern "C" int32_t vx_solve_pcg_f64(const float *rhs, const uint32_t *neighbors, const float *diagonal,
+                                    const float *blocks, const float *inverse_diagonal, uint32_t n,
+                                    uint32_t max_iterations, float tolerance, float *out_correction,
+                                    uint32_t *out_iterations, float *out_relative_residual) {
+    return vx::solve_rows<double>(rhs, neighbors, diagonal, blocks, inverse_diagonal, n, max_iterations, tolerance,
+                                  out_correction, out_iterations, out_relative_residual);
+}

@@ -1,0 +1,100 @@
+"""ctypes declarations of include/vx_solver.h."""
+from __future__ import annotations
+
+import ctypes
+
+c_fp = ctypes.POINTER(ctypes.c_float)
+c_dp = ctypes.POINTER(ctypes.c_double)
+c_u32p = ctypes.POINTER(ctypes.c_uint32)
+c_u8p = ctypes.POINTER(ctypes.c_uint8)
+
+VX_OK = 0
+VX_ERR_INIT = 1
+VX_ERR_TOO_LARGE = 2
+VX_ERR_NONFINITE_RESIDUAL = 3
+VX_ERR_NOT_CONVERGED = 4
+VX_ERR_NONFINITE_CORRECTION = 5
+VX_ERR_INVALID_ARGUMENT = 6
+VX_ERR_DEVICE = 7
+VX_EMPTY = 0xFFFFFFFF
+
+VX_SOLVE_MIXED = 0
+VX_SOLVE_F64 = 1
+
+
+class SceneDesc(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_uint32), ("ny", ctypes.c_uint32), ("nz", ctypes.c_uint32),
+        ("positions", c_fp), ("rotations", c_fp), ("velocities", c_fp), ("mass", c_fp),
+        ("kinematic", c_u8p), ("young", c_fp), ("young_perp", c_fp), ("shear", c_fp), ("solid", c_fp),
+        ("fibers", c_fp), ("anisotropy", c_fp), ("delta_temp", c_fp), ("alpha_par", c_fp),
+        ("alpha_perp", c_fp), ("neighbor", c_u32p), ("damage", c_fp), ("kscale", c_fp),
+        ("formation_offset", c_fp), ("plastic_strain", c_fp), ("targets", c_fp), ("loads", c_fp),
+    ]
+
+
+class SolveParams(ctypes.Structure):
+    _fields_ = [
+        ("tolerance", ctypes.c_double), ("inner_tolerance", ctypes.c_double),
+        ("max_inner", ctypes.c_uint32), ("max_outer", ctypes.c_uint32),
+        ("warm_start", ctypes.c_uint32), ("mode", ctypes.c_uint32),
+    ]
+
+
+class StepParams(ctypes.Structure):
+    pass
+
+
+class SolveReport(ctypes.Structure):
+    _fields_ = [
+        ("inner_iterations", ctypes.c_uint32), ("outer_sweeps", ctypes.c_uint32),
+        ("active_rows", ctypes.c_uint32), ("status", ctypes.c_uint32),
+        ("relative_residual", ctypes.c_double), ("rhs_norm", ctypes.c_double),
+    ]
+
+
+StepParams._fields_ = [
+    ("fixed_dt", ctypes.c_double), ("gravity", ctypes.c_double * 3), ("substeps", ctypes.c_uint32),
+    ("corotation_iterations", ctypes.c_uint32), ("corotation_tolerance", ctypes.c_double),
+    ("solve", SolveParams),
+]
+
+
+class StepReport(ctypes.Structure):
+    _fields_ = [
+        ("solves", ctypes.c_uint32), ("inner_iterations", ctypes.c_uint32), ("outer_sweeps", ctypes.c_uint32),
+        ("active_rows", ctypes.c_uint32), ("last_relative_residual", ctypes.c_double),
+        ("last_angle", ctypes.c_double), ("seconds", ctypes.c_double),
+    ]
+
+
+# (name, restype, argtypes) for every exported function of include/vx_solver.h.
+SIGNATURES = [
+    ("vx_last_error", ctypes.c_char_p, []),
+    ("vx_init", ctypes.c_int32, []),
+    ("vx_solve_pcg", ctypes.c_int32,
+     [c_fp, c_u32p, c_fp, c_fp, c_fp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float, c_fp, c_u32p, c_fp]),
+    ("vx_solve_pcg_f64", ctypes.c_int32,
+     [c_fp, c_u32p, c_fp, c_fp, c_fp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float, c_fp, c_u32p, c_fp]),
+    ("vx_scene_create", ctypes.c_int32, [ctypes.POINTER(SceneDesc), ctypes.POINTER(ctypes.c_void_p)]),
+    ("vx_scene_destroy", ctypes.c_int32, [ctypes.c_void_p]),
+    ("vx_scene_predict", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_double, c_dp]),
+    ("vx_scene_fit_rotations", ctypes.c_int32, [ctypes.c_void_p, c_dp]),
+    ("vx_scene_assemble", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_double]),
+    ("vx_scene_solve", ctypes.c_int32,
+     [ctypes.c_void_p, ctypes.POINTER(SolveParams), ctypes.POINTER(SolveReport)]),
+    ("vx_scene_apply", ctypes.c_int32, [ctypes.c_void_p]),
+    ("vx_scene_finish_substep", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_double]),
+    ("vx_scene_download", ctypes.c_int32, [ctypes.c_void_p, c_dp, c_dp, c_dp, c_dp]),
+    ("vx_scene_export_rows", ctypes.c_int32, [ctypes.c_void_p, c_u32p, c_u32p, c_dp, c_dp, c_dp, c_u32p]),
+    ("vx_scene_time_spmv", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint32, c_dp]),
+    ("vx_scene_step", ctypes.c_int32,
+     [ctypes.c_void_p, ctypes.POINTER(StepParams), ctypes.POINTER(StepReport)]),
+]
+
+
+def declare(L: ctypes.CDLL) -> None:
+    for name, restype, argtypes in SIGNATURES:
+        fn = getattr(L, name)
+        fn.restype = restype
+        fn.argtypes = argtypes

@@ -1,0 +1,145 @@
+"""Device-resident structured scenes (vx_scene_* of include/vx_solver.h).
+
+A `GridScene` uploads a `vxsim.scenes.Scene` box once and then drives the
+solve part of a physics step on the device: predict, rotation fit
+(src/solver.rs:327-417), bond assembly (src/solver.rs:1542-1636) and the
+mixed-precision PCG (FP32 inner block-Jacobi PCG, FP64 refinement).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import lib
+from ._abi import (SceneDesc, SolveParams, SolveReport, StepParams, StepReport, VX_SOLVE_F64, VX_SOLVE_MIXED, c_dp,
+                   c_fp, c_u8p, c_u32p)
+from .solver import SolverError, check
+
+MIXED = VX_SOLVE_MIXED
+F64 = VX_SOLVE_F64
+
+
+def solve_params(tolerance=1e-9, inner_tolerance=1e-4, max_inner=4000, max_outer=12, warm_start=False, mode=MIXED):
+    return SolveParams(float(tolerance), float(inner_tolerance), int(max_inner), int(max_outer), 1 if warm_start else 0,
+                       int(mode))
+
+
+def _report(r) -> dict:
+    return {f[0]: getattr(r, f[0]) for f in r._fields_}
+
+
+class GridScene:
+    def __init__(self, scene):
+        nx, ny, nz = scene.dims
+        if nx * ny * nz != scene.n:
+            raise ValueError("scene is not a full box")
+        keep = []
+
+        def f(a, shape=None):
+            if a is None:
+                return None
+            arr = np.ascontiguousarray(a, dtype=np.float32)
+            keep.append(arr)
+            return arr.ctypes.data_as(c_fp)
+
+        kin = np.ascontiguousarray(scene.kinematic, np.uint8)
+        nbr = np.ascontiguousarray(scene.neighbor, np.uint32)
+        keep += [kin, nbr]
+        desc = SceneDesc(
+            nx, ny, nz, f(scene.positions), f(scene.rotations), f(scene.velocities), f(scene.mass),
+            kin.ctypes.data_as(c_u8p), f(scene.young), f(scene.young_perp), f(scene.shear), f(scene.solid),
+            f(scene.fibers), f(scene.anisotropy), f(scene.delta_temp), f(scene.alpha_par), f(scene.alpha_perp),
+            nbr.ctypes.data_as(c_u32p), f(scene.damage), f(scene.kscale), f(scene.formation_offset),
+            f(scene.plastic_strain), f(scene.targets), f(scene.loads))
+        handle = ctypes.c_void_p()
+        check(lib().vx_scene_create(ctypes.byref(desc), ctypes.byref(handle)))
+        self.handle = handle
+        self.n = scene.n
+        self.dims = (nx, ny, nz)
+        self.active = scene.active_count()
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().vx_scene_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- stages ---------------------------------------------------------------
+    def predict(self, h, gravity=(0.0, 0.0, 0.0)):
+        g = (ctypes.c_double * 3)(*gravity)
+        check(lib().vx_scene_predict(self.handle, float(h), g))
+
+    def fit_rotations(self) -> float:
+        a = ctypes.c_double(0.0)
+        check(lib().vx_scene_fit_rotations(self.handle, ctypes.byref(a)))
+        return float(a.value)
+
+    def assemble(self, h):
+        check(lib().vx_scene_assemble(self.handle, float(h)))
+
+    def solve(self, params=None, raise_on_error=True, **kw) -> dict:
+        p = params if params is not None else solve_params(**kw)
+        rep = SolveReport()
+        status = lib().vx_scene_solve(self.handle, ctypes.byref(p), ctypes.byref(rep))
+        if status and raise_on_error:
+            check(status)
+        out = _report(rep)
+        out["status"] = status
+        return out
+
+    def apply(self):
+        check(lib().vx_scene_apply(self.handle))
+
+    def finish_substep(self, h):
+        check(lib().vx_scene_finish_substep(self.handle, float(h)))
+
+    def step(self, fixed_dt=0.001, gravity=(0.0, 0.0, -9.81), substeps=1, corotation_iterations=2,
+             corotation_tolerance=1e-5, solve=None) -> dict:
+        sp = StepParams(float(fixed_dt), (ctypes.c_double * 3)(*gravity), int(substeps), int(corotation_iterations),
+                        float(corotation_tolerance), solve if solve is not None else solve_params())
+        rep = StepReport()
+        check(lib().vx_scene_step(self.handle, ctypes.byref(sp), ctypes.byref(rep)))
+        return _report(rep)
+
+    # -- data -----------------------------------------------------------------
+    def download(self) -> dict:
+        n = self.n
+        pos = np.empty((n, 3))
+        rot = np.empty((n, 4))
+        vel = np.empty((n, 3))
+        x = np.empty((n, 3))
+        check(lib().vx_scene_download(self.handle, pos.ctypes.data_as(c_dp), rot.ctypes.data_as(c_dp),
+                                      vel.ctypes.data_as(c_dp), x.ctypes.data_as(c_dp)))
+        return dict(positions=pos, rotations=rot, velocities=vel, correction=x)
+
+    def export_rows(self) -> dict:
+        na = self.active
+        active = np.empty(na, np.uint32)
+        nbr = np.empty((na, 6), np.uint32)
+        blocks = np.empty((na, 54))
+        diag = np.empty((na, 9))
+        rhs = np.empty((na, 3))
+        cnt = ctypes.c_uint32(0)
+        check(lib().vx_scene_export_rows(self.handle, active.ctypes.data_as(c_u32p), nbr.ctypes.data_as(c_u32p),
+                                         blocks.ctypes.data_as(c_dp), diag.ctypes.data_as(c_dp),
+                                         rhs.ctypes.data_as(c_dp), ctypes.byref(cnt)))
+        if cnt.value != na:
+            raise SolverError(6, "active row count mismatch %d != %d" % (cnt.value, na))
+        return dict(active=active, neighbors=nbr, blocks=blocks, diagonal=diag, rhs=rhs)
+
+    def time_spmv(self, reps=50) -> float:
+        us = ctypes.c_double(0.0)
+        check(lib().vx_scene_time_spmv(self.handle, int(reps), ctypes.byref(us)))
+        return float(us.value)
