@@ -1,0 +1,202 @@
+#!/usr/bin/env python3
+"""Benchmark of the corotational elastic PCG solve on one MI355X.
+
+Workload: the single large mixed-material scene (64^3 = 262,144 REVs, bottom
+layer fixed, gravity, +0.2 N on 5% of the top layer; materials per 8^3
+region, per-bond stiffness jitter), 1 ms steps with one substep and up to two
+corotation passes. One bench step is one physics step of the solve part on
+the device: predict, rotation fit, and per pass bond assembly + FP32-inner /
+FP64-refined block-Jacobi PCG to the stated relative residual + apply, then
+velocities.
+
+value = DOF-PCG-iterations/s = sum over solves of 3 * active rows *
+(inner iterations + outer FP64 residual sweeps) / wall time of the timed
+steps. Prints one JSON line (rank 0, N = 1).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "voxel-sim_amd"))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+SOLVE_TOL = 1e-9        # outer FP64 relative true residual
+INNER_TOL = 1e-4        # FP32 inner relative residual per refinement sweep
+PEAK_HBM_GBS = 8000.0   # MI355X HBM3E
+SPMV_BYTES_PER_ROW = 145  # z 12 + p_old 12 + D 24 + 3 bond blocks 72 + mask 1 + p_new 12 + Ap 12
+
+
+def cpu_model() -> str:
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name"):
+                return line.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    return platform.processor() or "unknown"
+
+
+def host_threads() -> int:
+    try:
+        return len(os.sched_getaffinity(0))
+    except Exception:
+        return os.cpu_count() or 1
+
+
+def cpu_baseline(scene, budget_s: float = 12.0):
+    """The oracle's restatement of the reference CPU PCG (src/solver.rs:
+    1833-1911: f32 vectors, f64 dots, f32 alpha/beta, block-Jacobi in the
+    Cholesky form) on the same 64^3 system, OpenMP over all host cores with
+    no serial cutoff, capped at a bounded number of iterations."""
+    from oracle import oracle as O
+    os_ = O.OracleScene(scene)
+    h = scene.config.fixed_dt / scene.config.substeps
+    g = np.array(scene.config.gravity)
+    kin = scene.kinematic != 0
+    os_.positions[~kin] += g * h * h
+    t0 = time.perf_counter()
+    O.fit_rotations(os_, O.F32)
+    sy = O.assemble(os_, O.F32, h, os_.positions.copy(), scene.free_mask(), scene.loads)
+    assemble_s = time.perf_counter() - t0
+    threads = min(host_threads(), 16)
+    results = {}
+    for label, nthreads, iters in (("all", threads, 400), ("one", 1, 60)):
+        _, it, res, sec = sy.solve_cpu(O.BJ, iters, 1e-30, nthreads, 0)
+        results[label] = (it, sec, res)
+        if sec > budget_s:
+            break
+    it, sec, _ = results["all"]
+    value = 3.0 * sy.na * it / sec
+    out = {"value": value, "unit": "DOF-PCG-iters/s", "cores": threads, "kind": "port",
+           "sample": "oracle f32 CPU PCG with block-Jacobi on the 64^3 system (%d active rows), "
+                     "%d iterations timed, OpenMP %d threads, no serial cutoff" % (sy.na, it, threads),
+           "cpu_model": cpu_model(), "assemble_s": assemble_s}
+    if "one" in results:
+        it1, sec1, _ = results["one"]
+        out["single_thread_value"] = 3.0 * sy.na * it1 / sec1
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-compare", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (independent scenes shard "
+              "without communication; a single scene would need a slab decomposition). Run with --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from vxsim import grid, scenes, solver
+
+    scene = scenes.large_64cube()
+    cfg = scene.config
+    sp = grid.solve_params(tolerance=SOLVE_TOL, inner_tolerance=INNER_TOL, max_inner=8000, max_outer=12)
+    gs = grid.GridScene(scene)
+    step_kw = dict(fixed_dt=cfg.fixed_dt, gravity=cfg.gravity, substeps=cfg.substeps,
+                   corotation_iterations=cfg.corotation_iterations, corotation_tolerance=cfg.corotation_tolerance,
+                   solve=sp)
+    for _ in range(args.warmup):
+        gs.step(**step_kw)
+    reps = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        reps.append(gs.step(**step_kw))
+    wall = time.perf_counter() - t0
+    active = scene.active_count()
+    iters = sum(r["inner_iterations"] + r["outer_sweeps"] for r in reps)
+    solves = sum(r["solves"] for r in reps)
+    value = 3.0 * active * iters / wall
+    ms_per_step = 1000.0 * wall / args.steps
+
+    # Roofline of the dominant kernel, the FP32 structured bond SpMV, timed
+    # with HIP events on the library stream after the timed steps.
+    gs.assemble(cfg.fixed_dt / cfg.substeps)
+    spmv_us = gs.time_spmv(200)
+    spmv_bytes = SPMV_BYTES_PER_ROW * scene.n
+    achieved = spmv_bytes / (spmv_us * 1e-6) / 1e9
+
+    extra = {}
+    if not args.no_compare:
+        # Plain FP64 GPU solves of the same assembled system: the structured
+        # FP64 PCG and the drop-in FP64 solve over the reference row packing
+        # (host buffers, PCIe upload included).
+        gs.predict(cfg.fixed_dt / cfg.substeps, cfg.gravity)
+        gs.fit_rotations()
+        gs.assemble(cfg.fixed_dt / cfg.substeps)
+        t = time.perf_counter()
+        rm = gs.solve(tolerance=SOLVE_TOL, inner_tolerance=INNER_TOL, max_inner=8000, max_outer=12)
+        tm = time.perf_counter() - t
+        t = time.perf_counter()
+        r64 = gs.solve(tolerance=SOLVE_TOL, max_inner=20000, mode=grid.F64)
+        t64 = time.perf_counter() - t
+        mixed_rate = 3.0 * active * (rm["inner_iterations"] + rm["outer_sweeps"]) / tm
+        f64_rate = 3.0 * active * (r64["inner_iterations"] + r64["outer_sweeps"]) / t64
+        rows = gs.export_rows()
+        inv = np.linalg.inv(rows["diagonal"].reshape(-1, 3, 3)).reshape(-1, 9)
+        args64 = (rows["rhs"], rows["neighbors"], rows["diagonal"], rows["blocks"], inv)
+        solver.solve_pcg_f64(*args64, 20000, 1e-6)  # warm the buffers and graph
+        t = time.perf_counter()
+        _, it_drop, _ = solver.solve_pcg_f64(*args64, 20000, 1e-6)
+        t_drop = time.perf_counter() - t
+        drop_rate = 3.0 * active * it_drop / t_drop
+        extra = {
+            "mixed_solve": {"dof_iters_per_s": mixed_rate, "ms": 1000 * tm, "inner": rm["inner_iterations"],
+                            "sweeps": rm["outer_sweeps"], "rel_residual": rm["relative_residual"]},
+            "plain_fp64_structured": {"dof_iters_per_s": f64_rate, "ms": 1000 * t64, "iters": r64["inner_iterations"],
+                                      "rel_residual": r64["relative_residual"]},
+            "plain_fp64_dropin_rows": {"dof_iters_per_s": drop_rate, "ms": 1000 * t_drop, "iters": it_drop,
+                                       "tolerance": 1e-6, "note": "vx_solve_pcg_f64, host buffers, PCIe upload included"},
+            "mixed_vs_plain_fp64_structured": mixed_rate / f64_rate,
+            "mixed_vs_plain_fp64_dropin": mixed_rate / drop_rate,
+            "time_to_solution_vs_fp64_structured": t64 / tm,
+        }
+    gs.close()
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(scene)
+
+    line = {
+        "metric": "DOF-PCG-iters/s",
+        "value": value,
+        "unit": "DOF-PCG-iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 inner / f64 refinement and reductions",
+        "data": "synthetic",
+        "config": {"workload": "large_64x64x64_mixed: 262144 REVs, %d active rows, 1 ms step, 1 substep, "
+                               "<=%d corotation passes, outer rel. residual %g, inner %g" %
+                               (active, cfg.corotation_iterations, SOLVE_TOL, INNER_TOL),
+                   "active_rows": active, "dof": 3 * active, "solves": solves,
+                   "iterations": iters, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": "grid_spmv<float>", "achieved": achieved, "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                     "bytes_per_launch": spmv_bytes, "avg_us": spmv_us},
+        "cpu_baseline": cpu,
+    }
+    line.update(extra)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

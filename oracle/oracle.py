@@ -88,6 +88,10 @@ def lib():
         L.vxo_matvec.argtypes = [ctypes.c_int, ctypes.c_uint32, _u32p, _dp, _dp, _dp, _dp]
         L.vxo_step.argtypes = [ctypes.c_int, ctypes.POINTER(_Scene), ctypes.POINTER(_StepConfig), _dp, _dp,
                                ctypes.POINTER(_StepReport), _dp]
+        L.vxo_dot_rows.restype = ctypes.c_double
+        L.vxo_dot_rows.argtypes = [ctypes.c_uint32, _dp, _dp]
+        L.vxo_guarded_ratio.restype = ctypes.c_double
+        L.vxo_guarded_ratio.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double]
         _lib = L
     return _lib
 
@@ -265,6 +269,16 @@ def matvec(rows, x, mode=F64):
     lib().vxo_matvec(mode, n, _p(nbr, _u32p), _p(_f64(rows["diagonal"]), _dp), _p(_f64(rows["blocks"]), _dp),
                      _p(_f64(x), _dp), _p(y, _dp))
     return y
+
+
+def dot_rows(a, b):
+    a = _f64(a).ravel()
+    b = _f64(b).ravel()
+    return float(lib().vxo_dot_rows(a.size // 3, _p(a, _dp), _p(b, _dp)))
+
+
+def guarded_ratio(num, den, mode=F32):
+    return float(lib().vxo_guarded_ratio(mode, num, den))
 
 
 def step(oscene: OracleScene, cfg, mode=F64, precond=BJ, exact=False, exact_tolerance=1e-12,

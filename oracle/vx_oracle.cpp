@@ -1124,6 +1124,15 @@ double vxo_solve_exact(uint32_t n, const double *rhs, const uint32_t *nbr, const
     return rel;
 }
 
+double vxo_dot_rows(uint32_t n, const double *a, const double *b) {
+    std::vector<float> x(a, a + 3 * size_t(n)), y(b, b + 3 * size_t(n));
+    return dot_rows_f64<float>(n, x.data(), y.data());
+}
+
+double vxo_guarded_ratio(int mode, double num, double den) {
+    return mode == VXO_F32 ? double(guarded_ratio<float>(num, den)) : guarded_ratio<double>(num, den);
+}
+
 // Solve part of step_with_axis_targets_and_loads (src/solver.rs:446-570,
 // 668-683): predict, impose targets, fit rotations, corotation passes of
 // assemble + PCG + apply, then velocities.

@@ -153,6 +153,13 @@ double vxo_solve_exact(uint32_t n, const double *rhs, const uint32_t *neighbors,
 void vxo_matvec(int mode, uint32_t n, const uint32_t *neighbors, const double *diagonal,
                 const double *blocks, const double *x, double *y);
 
+/* dot_product_single semantics (src/cuda/pcg.ptx:563-637): rows of three f32
+ * values (inputs rounded to f32), f64 products, f64 sum. */
+double vxo_dot_rows(uint32_t n, const double *a, const double *b);
+/* scalar_ratio / dot_reduce_ratio_single (src/cuda/pcg.ptx:437-466, 639-724):
+ * num/den cast to the vector type when den > 1e-30, else 0. */
+double vxo_guarded_ratio(int mode, double num, double den);
+
 /* ---- step driver --------------------------------------------------------- */
 typedef struct vxo_step_config {
     double fixed_dt;

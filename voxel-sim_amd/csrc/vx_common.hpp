@@ -16,7 +16,7 @@
 
 namespace vx {
 
-constexpr int kBlock = 256;     // 4 wave64s per workgroup
+constexpr int kBlock = 512;     // 8 wave64s per workgroup
 constexpr int kWaves = kBlock / 64;
 constexpr int kXcds = 8;        // MI355X: 8 XCDs, workgroups dispatched round-robin
 
@@ -70,6 +70,11 @@ inline uint32_t xcd_grid(uint32_t tiles, uint32_t *per_xcd) {
 }
 
 // ---- reductions -------------------------------------------------------------
+// Cross-workgroup sums never use atomics or fences: each kernel writes one
+// partial per workgroup with a plain store, and the kernels that need the
+// total re-sum the partials in every workgroup (the kernel boundary makes
+// them visible). The order of the sum is fixed, so every workgroup gets the
+// same bits and results are reproducible run to run.
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -81,62 +86,65 @@ __device__ __forceinline__ double wave_max(double v) {
     return v;
 }
 
-// Workgroup sum of N values (fixed order, deterministic). Result valid in
-// thread 0.
+// Workgroup sum of N values, broadcast to every thread.
 template <int N>
-__device__ __forceinline__ void block_sum(double (&v)[N], double (*lds)[kWaves]) {
+__device__ __forceinline__ void block_sum_all(double (&v)[N], double (*lds)[kWaves]) {
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    __syncthreads();  // a previous block_sum's reader is done with lds
+    __syncthreads();  // a previous use of lds is finished
 #pragma unroll
     for (int k = 0; k < N; ++k) {
         double s = wave_sum(v[k]);
         if (lane == 0) lds[k][wave] = s;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
 #pragma unroll
-        for (int k = 0; k < N; ++k) {
-            double s = lds[k][0];
+    for (int k = 0; k < N; ++k) {
+        double s = lds[k][0];
 #pragma unroll
-            for (int w = 1; w < kWaves; ++w) s += lds[k][w];
-            v[k] = s;
-        }
+        for (int w = 1; w < kWaves; ++w) s += lds[k][w];
+        v[k] = s;
     }
 }
 
-// Publishes this workgroup's N partial sums and returns true in every thread
-// of the workgroup that arrives last; that workgroup then sees all partials.
-// Partials are laid out partials[k * stride + block].
+// Store this workgroup's N partials: partials[slot[k] * stride + blockIdx.x].
 template <int N>
-__device__ __forceinline__ bool publish_partials(const double (&v)[N], double *partials, uint32_t stride,
-                                                 uint32_t *ticket, uint32_t nblocks, bool *lds_last) {
+__device__ __forceinline__ void store_partials(double (&v)[N], double *partials, const int (&slot)[N], uint32_t stride,
+                                               double (*lds)[kWaves]) {
+    block_sum_all<N>(v, lds);
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int k = 0; k < N; ++k) partials[size_t(k) * stride + blockIdx.x] = v[k];
-        uint32_t prev = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        *lds_last = (prev == nblocks - 1);
+        for (int k = 0; k < N; ++k) partials[size_t(slot[k]) * stride + blockIdx.x] = v[k];
     }
-    __syncthreads();
-    bool last = *lds_last;
-    if (last) __threadfence();
-    return last;
 }
 
-// In the last workgroup: deterministic sum of N partial arrays.
+// Sum N partial arrays of `count` entries (written by the previous kernel)
+// within each wave: no LDS and no barrier, so loads issued before it stay in
+// flight. Every wave runs the same fixed-order sum and gets the same bits.
 template <int N>
-__device__ __forceinline__ void reduce_partials(double (&out)[N], const double *partials, uint32_t stride,
-                                                uint32_t nblocks, double (*lds)[kWaves]) {
-    double v[N];
+__device__ __forceinline__ void wave_sum_partials(double (&out)[N], const double *partials, const int (&slot)[N],
+                                                  uint32_t stride, uint32_t count) {
+    const uint32_t lane = threadIdx.x & 63;
 #pragma unroll
     for (int k = 0; k < N; ++k) {
         double s = 0.0;
-        for (uint32_t b = threadIdx.x; b < nblocks; b += kBlock) s += partials[size_t(k) * stride + b];
-        v[k] = s;
+        for (uint32_t b = lane; b < count; b += 64) s += partials[size_t(slot[k]) * stride + b];
+        out[k] = wave_sum(s);
     }
-    block_sum<N>(v, lds);
+}
+
+// Sum N partial arrays of `count` entries (written by the previous kernel),
+// broadcast to every thread.
+template <int N>
+__device__ __forceinline__ void sum_partials(double (&out)[N], const double *partials, const int (&slot)[N],
+                                             uint32_t stride, uint32_t count, double (*lds)[kWaves]) {
 #pragma unroll
-    for (int k = 0; k < N; ++k) out[k] = v[k];
+    for (int k = 0; k < N; ++k) {
+        double s = 0.0;
+        for (uint32_t b = threadIdx.x; b < count; b += kBlock) s += partials[size_t(slot[k]) * stride + b];
+        out[k] = s;
+    }
+    block_sum_all<N>(out, lds);
 }
 
 }  // namespace vx

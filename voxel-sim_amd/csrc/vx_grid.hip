@@ -147,17 +147,20 @@ struct Geo {
     __device__ __forceinline__ uint32_t stride(int a) const { return a == 0 ? 1u : (a == 1 ? nx : nx * ny); }
 };
 
+// Device-resident solve scalars. Each field is written by workgroup 0 of one
+// kernel and read only by later kernels.
 struct GridState {
-    double rho, alpha, beta;
-    double inner_rr0;      // ||r32||^2 at the start of the inner solve
-    double inner_tol;
-    double rr64, bnorm2, outer_rel;
-    double rr_inner;
+    double rho[2];         // r.z after k inner updates, slot k & 1 (grid_spmv)
+    double rr0;            // r.r at the start of the inner solve (grid_spmv, k = 0)
+    double inner_tol;      // relative inner tolerance
+    double rr_last;        // r.r seen by the last check
     uint64_t angle_bits;   // max rotation change as ordered bits of a non-negative double
-    uint32_t iters, done, max_iter, sweeps;
-    uint32_t ticket_a, ticket_b, ticket_c, pad;
+    uint32_t k_spmv;       // inner updates completed (grid_update, grid_residual)
+    uint32_t k_update;     // iteration of the current direction (grid_spmv)
+    uint32_t iters, done, max_iter, pad;
 };
 enum : uint32_t { kRun = 0, kConv = 1, kNonFin = 2, kCap = 3 };
+enum : int { kPap = 0, kRz = 1, kRr = 2, kRr64 = 3, kBb = 4 };  // partial slots
 
 struct DevScene {
     Geo g;
@@ -196,6 +199,7 @@ struct vx_scene {
     hipGraphExec_t inner32 = nullptr, inner64 = nullptr;
     hipEvent_t ev[2] = {nullptr, nullptr};
     vx::GridState *pinned = nullptr;  // 3 status slots
+    double *host_partials = nullptr;  // pinned, 2 * grid
 };
 
 namespace vx {
@@ -472,21 +476,59 @@ template <> struct Inner<double> {
     static __device__ __forceinline__ const double *dinv(const DevScene &s) { return s.dinv64; }
 };
 
-// y = A p (structured 7-point 3x3-block operator), p formed on load as
-// z + beta p_old for this row and its neighbours; also p.Ap -> alpha.
+// Inner convergence check of the previous update (every iteration:
+// r.r <= tol^2 r0.r0), beta, then y = A p on the structured 7-point
+// 3x3-block operator with p = z + beta p_old formed on load for this row and
+// its neighbours; partials of p.Ap.
 template <class S>
 __global__ __launch_bounds__(kBlock) void grid_spmv(DevScene G, uint32_t per_xcd, uint32_t ntiles, const S *__restrict__ z,
                                                     const S *__restrict__ p_old, S *__restrict__ p_new, S *__restrict__ ap) {
     GridState *st = G.st;
-    if (st->done) return;
-    __shared__ double lds[1][kWaves];
-    __shared__ bool last;
+    __shared__ double lds[2][kWaves];
+    // Scalars first: the state words and the previous update's partials are
+    // loaded together, then summed across the workgroup.
+    const uint32_t done_in = st->done;
+    const uint32_t k = st->k_spmv;
+    const uint32_t max_iter = st->max_iter;
+    const double rho_s[2] = {st->rho[0], st->rho[1]};
+    const double rr0_s = st->rr0, tol = st->inner_tol;
+    const uint32_t nb = gridDim.x;
+    double sums[2];
+    const int in_slots[2] = {kRz, kRr};
+    sum_partials<2>(sums, G.partials, in_slots, nb, nb, lds);
+    if (done_in) return;
+    const double rho = sums[0], rr = sums[1];
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    const double rr0 = k == 0 ? rr : rr0_s;
+    uint32_t done = kRun;
+    if (!isfinite(rr))
+        done = kNonFin;
+    else if (!(rr > tol * tol * rr0))
+        done = kConv;
+    else if (k >= max_iter)
+        done = kCap;
+    if (lead) {
+        st->rr_last = rr;
+        if (k == 0) st->rr0 = rr;
+    }
+    if (done) {
+        if (lead) {
+            st->done = done;
+            st->iters = k;
+        }
+        return;
+    }
+    const double rho_prev = k == 0 ? 0.0 : rho_s[(k - 1) & 1];
+    const S beta = (k == 0 || !(rho_prev > 1e-300)) ? S(0) : S(rho / rho_prev);
+    if (lead) {
+        st->rho[k & 1] = rho;
+        st->k_update = k;
+    }
     const size_t n = G.g.n;
     const uint32_t tile = xcd_tile(blockIdx.x, per_xcd);
     const uint32_t v = tile * kBlock + threadIdx.x;
     double part[1] = {0.0};
     if (tile < ntiles && v < n) {
-        const S beta = S(st->beta);
         const S *__restrict__ K = Inner<S>::k(G);
         uint32_t x, y, zc;
         G.g.coords(v, x, y, zc);
@@ -529,34 +571,37 @@ __global__ __launch_bounds__(kBlock) void grid_spmv(DevScene G, uint32_t per_xcd
         part[0] = __builtin_fma(double(p[1]), double(yv[1]), part[0]);
         part[0] = __builtin_fma(double(p[2]), double(yv[2]), part[0]);
     }
-    block_sum<1>(part, lds);
-    if (publish_partials<1>(part, G.partials, gridDim.x, &st->ticket_a, gridDim.x, &last)) {
-        double tot[1];
-        reduce_partials<1>(tot, G.partials, gridDim.x, gridDim.x, lds);
-        if (threadIdx.x == 0) {
-            st->alpha = tot[0] > 1e-300 ? double(S(st->rho / tot[0])) : 0.0;
-            st->ticket_a = 0;
-        }
-    }
+    const int out_slots[1] = {kPap};
+    store_partials<1>(part, G.partials, out_slots, nb, lds);
 }
 
-// d += alpha p (d = alpha p on the first iteration); r -= alpha Ap;
-// z = Dinv r; beta; inner convergence every iteration.
+// alpha = rho / p.Ap; d += alpha p (d = alpha p on the first iteration);
+// r -= alpha Ap; z = Dinv r; partials of r.z and r.r.
 template <class S>
 __global__ __launch_bounds__(kBlock) void grid_update(DevScene G, uint32_t per_xcd, uint32_t ntiles, S *__restrict__ d,
                                                       S *__restrict__ r, S *__restrict__ z, const S *__restrict__ p,
                                                       const S *__restrict__ ap) {
     GridState *st = G.st;
-    if (st->done) return;
     __shared__ double lds[2][kWaves];
-    __shared__ bool last;
+    const uint32_t done_in = st->done;
+    const uint32_t k = st->k_update;
+    const double rho_s[2] = {st->rho[0], st->rho[1]};
+    const uint32_t nb = gridDim.x;
+    double pap[1];
+    const int in_slots[1] = {kPap};
+    sum_partials<1>(pap, G.partials, in_slots, nb, nb, lds);
+    if (done_in) return;
+    const S alpha = pap[0] > 1e-300 ? S(rho_s[k & 1] / pap[0]) : S(0);
+    const bool first = k == 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->k_spmv = k + 1;
+        st->iters = k + 1;
+    }
     const size_t n = G.g.n;
     const uint32_t tile = xcd_tile(blockIdx.x, per_xcd);
     const uint32_t v = tile * kBlock + threadIdx.x;
     double part[2] = {0.0, 0.0};
     if (tile < ntiles && v < n) {
-        const S alpha = S(st->alpha);
-        const bool first = st->iters == 0;
         S ri[3], zi[3];
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
@@ -571,38 +616,23 @@ __global__ __launch_bounds__(kBlock) void grid_update(DevScene G, uint32_t per_x
         part[0] = __builtin_fma(double(ri[2]), double(zi[2]), __builtin_fma(double(ri[1]), double(zi[1]), double(ri[0]) * double(zi[0])));
         part[1] = __builtin_fma(double(ri[2]), double(ri[2]), __builtin_fma(double(ri[1]), double(ri[1]), double(ri[0]) * double(ri[0])));
     }
-    block_sum<2>(part, lds);
-    if (publish_partials<2>(part, G.partials + 2 * size_t(gridDim.x), gridDim.x, &st->ticket_b, gridDim.x, &last)) {
-        double tot[2];
-        reduce_partials<2>(tot, G.partials + 2 * size_t(gridDim.x), gridDim.x, gridDim.x, lds);
-        if (threadIdx.x == 0) {
-            const double rho_next = tot[0];
-            st->beta = st->rho > 1e-300 ? double(S(rho_next / st->rho)) : 0.0;
-            st->rho = rho_next;
-            st->rr_inner = tot[1];
-            const uint32_t it = st->iters + 1;
-            st->iters = it;
-            if (!isfinite(tot[1]))
-                st->done = kNonFin;
-            else if (tot[1] <= st->inner_tol * st->inner_tol * st->inner_rr0)
-                st->done = kConv;
-            else if (it >= st->max_iter)
-                st->done = kCap;
-            st->ticket_b = 0;
-        }
-    }
+    const int out_slots[2] = {kRz, kRr};
+    store_partials<2>(part, G.partials, out_slots, nb, lds);
 }
 
 // Outer FP64 refinement step: x_new = x_old + d (own and neighbour rows),
-// r64 = b - A64 x_new; the inner system starts from r = f32(r64), z = Dinv r,
-// p_old = 0, beta = 0. Dots: ||r64||^2, ||b||^2, r.z, r.r.
+// r64 = b - A64 x_new; the inner system starts from r = r64 in the inner
+// precision, z = Dinv r, p_old = 0. Partials of ||r64||^2, ||b||^2, r.z, r.r.
 template <class S>
 __global__ __launch_bounds__(kBlock) void grid_residual(DevScene G, uint32_t per_xcd, uint32_t ntiles, const double *__restrict__ x_old,
                                                         double *__restrict__ x_new, const S *__restrict__ d, int use_d,
                                                         S *__restrict__ r, S *__restrict__ z, S *__restrict__ p_old) {
     GridState *st = G.st;
     __shared__ double lds[4][kWaves];
-    __shared__ bool last;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->k_spmv = 0;
+        st->done = kRun;
+    }
     const size_t n = G.g.n;
     const uint32_t tile = xcd_tile(blockIdx.x, per_xcd);
     const uint32_t v = tile * kBlock + threadIdx.x;
@@ -638,16 +668,15 @@ __global__ __launch_bounds__(kBlock) void grid_residual(DevScene G, uint32_t per
         const uint8_t m = G.mask[v];
         const double f[3] = {double(m & 1), double(m >> 1 & 1), double(m >> 2 & 1)};
         S rs[3], zs[3];
-        double r64[3];
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
             const double b = G.rhs[q * n + v];
-            r64[q] = f[q] * (b - (yv[q] - acc[q]));
+            const double r64 = f[q] * (b - (yv[q] - acc[q]));
             x_new[q * n + v] = xv[q];
-            rs[q] = S(r64[q]);
+            rs[q] = S(r64);
             r[q * n + v] = rs[q];
             p_old[q * n + v] = S(0);
-            part[0] = __builtin_fma(r64[q], r64[q], part[0]);
+            part[0] = __builtin_fma(r64, r64, part[0]);
             part[1] = __builtin_fma(b, b, part[1]);
         }
         symv<S>(Inner<S>::dinv(G), n, v, rs, zs);
@@ -658,26 +687,8 @@ __global__ __launch_bounds__(kBlock) void grid_residual(DevScene G, uint32_t per
             part[3] = __builtin_fma(double(rs[q]), double(rs[q]), part[3]);
         }
     }
-    block_sum<4>(part, lds);
-    if (publish_partials<4>(part, G.partials + 4 * size_t(gridDim.x), gridDim.x, &st->ticket_c, gridDim.x, &last)) {
-        double tot[4];
-        reduce_partials<4>(tot, G.partials + 4 * size_t(gridDim.x), gridDim.x, gridDim.x, lds);
-        if (threadIdx.x == 0) {
-            st->rr64 = tot[0];
-            st->bnorm2 = tot[1];
-            st->outer_rel = tot[1] > 0.0 ? sqrt(tot[0] / tot[1]) : 0.0;
-            st->rho = tot[2];
-            st->inner_rr0 = tot[3];
-            st->rr_inner = tot[3];
-            st->beta = 0.0;
-            st->alpha = 0.0;
-            st->iters = 0;
-            st->sweeps += 1;
-            st->done = (tot[3] > 0.0 && isfinite(tot[0])) ? kRun : kConv;
-            if (!isfinite(tot[0])) st->done = kNonFin;
-            st->ticket_c = 0;
-        }
-    }
+    const int out_slots[4] = {kRr64, kBb, kRz, kRr};
+    store_partials<4>(part, G.partials, out_slots, gridDim.x, lds);
 }
 
 // positions += x on the active rows (src/solver.rs:1921-1923).
@@ -750,7 +761,9 @@ int32_t capture_inner(vx_scene *sc, hipGraphExec_t *exec) {
 int32_t run_inner(vx_scene *sc, hipGraphExec_t exec, uint32_t max_inner) {
     Context &ctx = context();
     uint32_t launched = 0, chunk = 0;
-    while (launched < max_inner) {
+    // The check after the last update runs in the following grid_spmv, so
+    // replay until one spmv past the cap has been issued.
+    while (launched <= max_inner) {
         VX_HIP(hipGraphLaunch(exec, ctx.stream));
         VX_HIP(hipMemcpyAsync(&sc->pinned[chunk % 2], sc->s.st, sizeof(GridState), hipMemcpyDeviceToHost, ctx.stream));
         VX_HIP(hipEventRecord(sc->ev[chunk % 2], ctx.stream));
@@ -799,17 +812,25 @@ int32_t solve_t(vx_scene *sc, const vx_solve_params *prm, vx_solve_report *rep) 
                            (const S *)d, s > 0 ? 1 : 0, r, z, p0);
         VX_HIP(hipGetLastError());
         sc->x_in_a = !sc->x_in_a;
-        GridState st;
-        int32_t c = read_state(sc, &st);
-        if (c != VX_OK) return c;
+        // ||r64||^2 and ||b||^2 from the residual kernel's partials, summed
+        // on the host in a fixed order.
+        VX_HIP(hipMemcpyAsync(sc->host_partials, G.partials + size_t(kRr64) * sc->grid, 2 * sizeof(double) * sc->grid,
+                              hipMemcpyDeviceToHost, ctx.stream));
+        VX_HIP(hipStreamSynchronize(ctx.stream));
+        double rr64 = 0.0, bb = 0.0;
+        for (uint32_t b = 0; b < sc->grid; ++b) {
+            rr64 += sc->host_partials[b];
+            bb += sc->host_partials[sc->grid + b];
+        }
         sweeps += 1;
-        rel = st.outer_rel;
-        if (rep) rep->rhs_norm = std::sqrt(st.bnorm2);
-        if (!std::isfinite(st.rr64)) {
+        rel = bb > 0.0 ? std::sqrt(rr64 / bb) : 0.0;
+        if (rep) rep->rhs_norm = std::sqrt(bb);
+        if (!std::isfinite(rr64)) {
             status = fail(VX_ERR_NONFINITE_RESIDUAL, "HIP PCG produced a non-finite residual norm");
             break;
         }
-        if (rel <= prm->tolerance || st.done == kConv) break;
+        int32_t c = VX_OK;
+        if (rel <= prm->tolerance) break;
         if (s == max_outer) break;
         c = run_inner(sc, exec, prm->max_inner);
         if (c != VX_OK) return c;
@@ -1007,6 +1028,7 @@ extern "C" int32_t vx_scene_create(const vx_scene_desc *desc, vx_scene **out) {
         hipMemset(G.st, 0, sizeof(GridState)) != hipSuccess)
         return fail_free(fail(VX_ERR_DEVICE, "scene upload failed"));
     if (hipHostMalloc((void **)&sc->pinned, 3 * sizeof(GridState), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&sc->host_partials, 2 * sizeof(double) * sc->grid, hipHostMallocDefault) != hipSuccess ||
         hipEventCreateWithFlags(&sc->ev[0], hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&sc->ev[1], hipEventDisableTiming) != hipSuccess)
         return fail_free(fail(VX_ERR_DEVICE, "scene host resources failed"));
@@ -1025,6 +1047,7 @@ extern "C" int32_t vx_scene_destroy(vx_scene *sc) {
     for (auto &e : sc->ev)
         if (e) (void)hipEventDestroy(e);
     if (sc->pinned) (void)hipHostFree(sc->pinned);
+    if (sc->host_partials) (void)hipHostFree(sc->host_partials);
     if (sc->alloc) (void)hipFree(sc->alloc);
     delete sc;
     return VX_OK;
@@ -1208,12 +1231,18 @@ extern "C" int32_t vx_scene_time_spmv(vx_scene *sc, uint32_t reps, double *out_u
     Lock lock;
     Context &ctx = context();
     DevScene &G = sc->s;
-    // The timing launches run with a cleared status word (not done) and
-    // leave the solve vectors' contents unspecified.
+    // The timed launches run as first inner iterations (k = 0, nonzero
+    // residual partials, not done); they leave the solve vectors' contents
+    // unspecified.
     GridState init;
     std::memset(&init, 0, sizeof init);
+    init.inner_tol = 0.0;
+    init.max_iter = 1u << 30;
     sc->pinned[2] = init;
     VX_HIP(hipMemcpyAsync(G.st, &sc->pinned[2], sizeof init, hipMemcpyHostToDevice, ctx.stream));
+    std::vector<double> ones(2 * size_t(sc->grid), 1.0);
+    VX_HIP(hipMemcpyAsync(G.partials + size_t(kRz) * sc->grid, ones.data(), sizeof(double) * ones.size(), hipMemcpyHostToDevice,
+                          ctx.stream));
     std::vector<hipEvent_t> ev(2 * reps);
     for (auto &e : ev) VX_HIP(hipEventCreate(&e));
     for (uint32_t k = 0; k < reps; ++k) {

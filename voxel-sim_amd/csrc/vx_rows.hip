@@ -7,15 +7,17 @@
 // solve (the boundary hands over host slices) and re-laid out on the device
 // into structure-of-arrays planes, so every PCG kernel streams coalesced
 // 256-byte wave loads. One iteration is two kernels:
-//   rows_spmv   : p = z + beta*p_old formed on load (own row and neighbour
-//                 rows), Ap = A p, per-workgroup f64 partials of p.Ap; the last
-//                 workgroup computes alpha on the device;
-//   rows_update : dx += alpha p, r -= alpha Ap, z = Dinv r, partials of r.z
-//                 and r.r; the last workgroup computes beta and checks
-//                 convergence on the device.
+//   rows_spmv   : re-sums the previous update's r.z / r.r partials, checks
+//                 convergence, forms beta, then p = z + beta*p_old on load
+//                 (own row and neighbour rows), Ap = A p and per-workgroup
+//                 f64 partials of p.Ap;
+//   rows_update : re-sums the p.Ap partials into alpha, dx += alpha p,
+//                 r -= alpha Ap, z = Dinv r, partials of r.z and r.r.
+// Scalars are rebuilt from partials inside each kernel (no atomics, no
+// fences); workgroup 0 records them in the device state for the next kernel.
 // Sixteen iterations are captured once into a hipGraph and replayed; the
 // host reads a pinned status word one graph behind, so the device never
-// waits for the host. Scalars never leave the device.
+// waits for the host.
 #include "vx_common.hpp"
 
 #include <cmath>
@@ -27,19 +29,21 @@ namespace {
 
 constexpr uint32_t kChunk = 16;  // iterations per graph replay = reference check interval
 
-// Device-resident PCG scalars.
+// Device-resident PCG scalars. Each field is written by workgroup 0 of one
+// kernel and read by the following kernels only.
 struct RowsState {
-    double rho;         // r.z of the current iterate
+    double rho[2];      // r.z after k updates, slot k & 1 (written by rows_spmv)
     double rr_checked;  // r.r at the last convergence check
     double initial;     // ||rhs||
-    double alpha, beta; // exact values of the vector type
     double tol;         // tolerance in the vector type
     double rel;         // relative residual at the last check
+    uint32_t k_spmv;    // updates completed (written by rows_update)
+    uint32_t k_update;  // iteration of the current direction (written by rows_spmv)
     uint32_t iters, done, max_iter, check_every;
-    uint32_t invalid;
-    uint32_t ticket_a, ticket_b, pad;
+    uint32_t invalid, pad;
 };
 enum : uint32_t { kRunning = 0, kConverged = 1, kNonFinite = 2, kCapReached = 3 };
+enum : int { kPap = 0, kRz = 1, kRr = 2 };  // partial slots
 
 template <class S> __device__ __forceinline__ S fma_s(S a, S b, S c);
 template <> __device__ __forceinline__ float fma_s<float>(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
@@ -131,17 +135,16 @@ __device__ __forceinline__ void apply_inv(const S *__restrict__ inv, size_t N, u
     }
 }
 
-// z = Dinv r and rho = r.z; beta = 0 for the first direction.
+// z = Dinv r; partials of r.z and r.r for the first direction.
 template <class S>
 __global__ __launch_bounds__(kBlock) void rows_init(uint32_t n, uint32_t per_xcd, uint32_t ntiles,
                                                     const S *__restrict__ inv, const S *__restrict__ r,
-                                                    S *__restrict__ z, RowsState *st, double *partials) {
-    __shared__ double lds[1][kWaves];
-    __shared__ bool last;
+                                                    S *__restrict__ z, double *partials) {
+    __shared__ double lds[2][kWaves];
     const size_t N = n;
     const uint32_t tile = xcd_tile(blockIdx.x, per_xcd);
     const uint32_t i = tile * kBlock + threadIdx.x;
-    double v[1] = {0.0};
+    double v[2] = {0.0, 0.0};
     if (tile < ntiles && i < n) {
         S ri[3] = {r[i], r[N + i], r[2 * N + i]};
         S zi[3];
@@ -150,58 +153,120 @@ __global__ __launch_bounds__(kBlock) void rows_init(uint32_t n, uint32_t per_xcd
         z[N + i] = zi[1];
         z[2 * N + i] = zi[2];
         v[0] = row_dot<S>(ri, zi);
+        v[1] = row_dot<S>(ri, ri);
     }
-    block_sum<1>(v, lds);
-    if (publish_partials<1>(v, partials, gridDim.x, &st->ticket_a, gridDim.x, &last)) {
-        double tot[1];
-        reduce_partials<1>(tot, partials, gridDim.x, gridDim.x, lds);
-        if (threadIdx.x == 0) {
-            st->rho = tot[0];
-            st->beta = 0.0;
-            st->ticket_a = 0;
-        }
-    }
+    const int slots[2] = {kRz, kRr};
+    store_partials<2>(v, partials, slots, gridDim.x, lds);
 }
 
-// Ap = A p with p = z + beta*p_old formed on load; alpha = rho / p.Ap.
+// Convergence check of the previous update (every check_every iterations and
+// at the cap, `<=`, src/cuda_solver.rs:458-477), beta = rho_k / rho_{k-1},
+// then Ap = A p with p = z + beta*p_old formed on load; partials of p.Ap.
 template <class S>
 __global__ __launch_bounds__(kBlock) void rows_spmv(uint32_t n, uint32_t per_xcd, uint32_t ntiles,
                                                     const uint32_t *__restrict__ nbr, const S *__restrict__ blk,
                                                     const S *__restrict__ diag, const S *__restrict__ z,
                                                     const S *__restrict__ p_old, S *__restrict__ p_new,
                                                     S *__restrict__ ap, RowsState *st, double *partials) {
-    if (st->done) return;
-    __shared__ double lds[1][kWaves];
-    __shared__ bool last;
+    __shared__ double lds[2][kWaves];
+    // Scalars first: state words and the previous update's partials load
+    // together and are summed across the workgroup.
+    const uint32_t done_in = st->done;
+    const uint32_t k = st->k_spmv;
+    const uint32_t max_iter = st->max_iter, check_every = st->check_every;
+    const double rho_s[2] = {st->rho[0], st->rho[1]};
+    const double initial = st->initial, tol = st->tol;
+    const uint32_t nb = gridDim.x;
+    double sums[2];
+    const int in_slots[2] = {kRz, kRr};
+    sum_partials<2>(sums, partials, in_slots, nb, nb, lds);
+    if (done_in) return;
     const size_t N = n;
     const uint32_t tile = xcd_tile(blockIdx.x, per_xcd);
     const uint32_t i = tile * kBlock + threadIdx.x;
+    const bool row = tile < ntiles && i < n;
+    // The neighbour gathers (index load, then vector loads) and the row's own
+    // vectors.
+    uint32_t jj[6];
+    S zz[7][3], pp[7][3];
+#pragma unroll
+    for (int t = 0; t < 7; ++t)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) zz[t][c] = pp[t][c] = S(0);
+#pragma unroll
+    for (int s = 0; s < 6; ++s) jj[s] = VX_EMPTY;
+    if (row) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            zz[0][c] = z[c * N + i];
+            pp[0][c] = p_old[c * N + i];
+        }
+#pragma unroll
+        for (int s = 0; s < 6; ++s) jj[s] = nbr[s * N + i];
+#pragma unroll
+        for (int s = 0; s < 6; ++s) {
+            if (jj[s] == VX_EMPTY) continue;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                zz[1 + s][c] = z[c * N + jj[s]];
+                pp[1 + s][c] = p_old[c * N + jj[s]];
+            }
+        }
+    }
+    // Convergence check of the previous update (every check_every iterations
+    // and at the cap, `<=`, src/cuda_solver.rs:458-477); beta = rho_k / rho_{k-1}.
+    const double rho = sums[0], rr = sums[1];
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    uint32_t done = kRunning;
+    if (k > 0 && ((check_every && k % check_every == 0) || k == max_iter)) {
+        if (!isfinite(rr) || rr < 0.0) {
+            done = kNonFinite;
+        } else {
+            const S rel = S(sqrt(rr) / initial);
+            if (lead) st->rel = double(rel);
+            if (double(rel) <= tol) done = kConverged;
+        }
+        if (lead) st->rr_checked = rr;
+    }
+    if (!done && k >= max_iter) done = kCapReached;
+    if (done) {
+        if (lead) {
+            st->done = done;
+            st->iters = k;
+        }
+        return;
+    }
+    const S beta = k == 0 ? S(0) : S(guarded_ratio<S>(rho, rho_s[(k - 1) & 1]));
+    if (lead) {
+        st->rho[k & 1] = rho;
+        st->k_update = k;
+    }
+    // p = z + beta p_old on load, y = D p + sum_s B_s p_s in the reference
+    // kernel's fused multiply-add order.
     double v[1] = {0.0};
-    if (tile < ntiles && i < n) {
-        const S beta = S(st->beta);
+    if (row) {
         S p[3], y[3];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) p[c] = fma_s<S>(beta, p_old[c * N + i], z[c * N + i]);
+        for (int c = 0; c < 3; ++c) p[c] = fma_s<S>(beta, pp[0][c], zz[0][c]);
 #pragma unroll
-        for (int row = 0; row < 3; ++row) {
-            S a = diag[(3 * row) * N + i] * p[0];
-            a = fma_s<S>(diag[(3 * row + 1) * N + i], p[1], a);
-            a = fma_s<S>(diag[(3 * row + 2) * N + i], p[2], a);
-            y[row] = a;
+        for (int r = 0; r < 3; ++r) {
+            S a = diag[(3 * r) * N + i] * p[0];
+            a = fma_s<S>(diag[(3 * r + 1) * N + i], p[1], a);
+            a = fma_s<S>(diag[(3 * r + 2) * N + i], p[2], a);
+            y[r] = a;
         }
 #pragma unroll
         for (int s = 0; s < 6; ++s) {
-            const uint32_t j = nbr[s * N + i];
-            if (j == VX_EMPTY) continue;
+            if (jj[s] == VX_EMPTY) continue;
             S pj[3];
 #pragma unroll
-            for (int c = 0; c < 3; ++c) pj[c] = fma_s<S>(beta, p_old[c * N + j], z[c * N + j]);
+            for (int c = 0; c < 3; ++c) pj[c] = fma_s<S>(beta, pp[1 + s][c], zz[1 + s][c]);
 #pragma unroll
-            for (int row = 0; row < 3; ++row) {
-                const S *b = blk + (9 * s + 3 * row) * N + i;
-                y[row] = fma_s<S>(b[0], pj[0], y[row]);
-                y[row] = fma_s<S>(b[N], pj[1], y[row]);
-                y[row] = fma_s<S>(b[2 * N], pj[2], y[row]);
+            for (int r = 0; r < 3; ++r) {
+                const S *b = blk + (9 * s + 3 * r) * N + i;
+                y[r] = fma_s<S>(b[0], pj[0], y[r]);
+                y[r] = fma_s<S>(b[N], pj[1], y[r]);
+                y[r] = fma_s<S>(b[2 * N], pj[2], y[r]);
             }
         }
 #pragma unroll
@@ -211,39 +276,53 @@ __global__ __launch_bounds__(kBlock) void rows_spmv(uint32_t n, uint32_t per_xcd
         }
         v[0] = row_dot<S>(p, y);
     }
-    block_sum<1>(v, lds);
-    if (publish_partials<1>(v, partials, gridDim.x, &st->ticket_a, gridDim.x, &last)) {
-        double tot[1];
-        reduce_partials<1>(tot, partials, gridDim.x, gridDim.x, lds);
-        if (threadIdx.x == 0) {
-            st->alpha = guarded_ratio<S>(st->rho, tot[0]);
-            st->ticket_a = 0;
-        }
-    }
+    const int out_slots[1] = {kPap};
+    store_partials<1>(v, partials, out_slots, nb, lds);
 }
 
-// dx += alpha p; r -= alpha Ap; z = Dinv r; beta = r.z / rho; the residual
-// check every check_every iterations and at the cap.
+// alpha = rho / p.Ap (guarded); dx += alpha p; r -= alpha Ap; z = Dinv r;
+// partials of r.z and r.r.
 template <class S>
 __global__ __launch_bounds__(kBlock) void rows_update(uint32_t n, uint32_t per_xcd, uint32_t ntiles,
                                                       const S *__restrict__ inv, S *__restrict__ dx,
                                                       S *__restrict__ r, S *__restrict__ z,
                                                       const S *__restrict__ p, const S *__restrict__ ap,
                                                       RowsState *st, double *partials) {
-    if (st->done) return;
     __shared__ double lds[2][kWaves];
-    __shared__ bool last;
+    const uint32_t done_in = st->done;
+    const uint32_t k = st->k_update;
+    const double rho_s[2] = {st->rho[0], st->rho[1]};
+    const uint32_t nb = gridDim.x;
+    double pap[1];
+    const int in_slots[1] = {kPap};
+    sum_partials<1>(pap, partials, in_slots, nb, nb, lds);
+    if (done_in) return;
     const size_t N = n;
     const uint32_t tile = xcd_tile(blockIdx.x, per_xcd);
     const uint32_t i = tile * kBlock + threadIdx.x;
+    const bool row = tile < ntiles && i < n;
+    S pv[3], av[3], rv[3], dv[3];
+    if (row) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            pv[c] = p[c * N + i];
+            av[c] = ap[c * N + i];
+            rv[c] = r[c * N + i];
+            dv[c] = dx[c * N + i];
+        }
+    }
+    const S alpha = S(guarded_ratio<S>(rho_s[k & 1], pap[0]));
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->k_spmv = k + 1;
+        st->iters = k + 1;
+    }
     double v[2] = {0.0, 0.0};
-    if (tile < ntiles && i < n) {
-        const S alpha = S(st->alpha);
+    if (row) {
         S ri[3], zi[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            dx[c * N + i] = fma_s<S>(alpha, p[c * N + i], dx[c * N + i]);
-            ri[c] = fma_s<S>(-alpha, ap[c * N + i], r[c * N + i]);
+            dx[c * N + i] = fma_s<S>(alpha, pv[c], dv[c]);
+            ri[c] = fma_s<S>(-alpha, av[c], rv[c]);
             r[c * N + i] = ri[c];
         }
         apply_inv<S>(inv, N, i, ri, zi);
@@ -252,31 +331,8 @@ __global__ __launch_bounds__(kBlock) void rows_update(uint32_t n, uint32_t per_x
         v[0] = row_dot<S>(ri, zi);
         v[1] = row_dot<S>(ri, ri);
     }
-    block_sum<2>(v, lds);
-    if (publish_partials<2>(v, partials, gridDim.x, &st->ticket_b, gridDim.x, &last)) {
-        double tot[2];
-        reduce_partials<2>(tot, partials, gridDim.x, gridDim.x, lds);
-        if (threadIdx.x == 0) {
-            const double rho_next = tot[0];
-            st->beta = guarded_ratio<S>(rho_next, st->rho);
-            st->rho = rho_next;
-            const uint32_t it = st->iters + 1;
-            st->iters = it;
-            if ((st->check_every && it % st->check_every == 0) || it == st->max_iter) {
-                const double rr = tot[1];
-                st->rr_checked = rr;
-                if (!isfinite(rr) || rr < 0.0) {
-                    st->done = kNonFinite;
-                } else {
-                    const S rel = S(sqrt(rr) / st->initial);
-                    st->rel = double(rel);
-                    if (double(rel) <= st->tol) st->done = kConverged;
-                }
-            }
-            if (!st->done && it >= st->max_iter) st->done = kCapReached;
-            st->ticket_b = 0;
-        }
-    }
+    const int out_slots[2] = {kRz, kRr};
+    store_partials<2>(v, partials, out_slots, nb, lds);
 }
 
 // SoA correction -> caller's AoS f32 layout.
@@ -357,7 +413,7 @@ int32_t solve_rows(const float *rhs, const uint32_t *neighbors, const float *dia
     VX_BUF("p0", 3 * N * sizeof(S), &d_p0);
     VX_BUF("p1", 3 * N * sizeof(S), &d_p1);
     VX_BUF("ap", 3 * N * sizeof(S), &d_ap);
-    VX_BUF("partials", 2 * size_t(grid) * 8, &d_part);
+    VX_BUF("partials", 4 * size_t(grid) * 8, &d_part);
     VX_BUF("state", sizeof(RowsState), &d_state);
     VX_BUF("out", 3 * N * 4, &d_out);
 #undef VX_BUF
@@ -374,6 +430,8 @@ int32_t solve_rows(const float *rhs, const uint32_t *neighbors, const float *dia
     hs.tol = kF64 ? double(tolerance) : double(float(tolerance));
     hs.rel = INFINITY;
     hs.rr_checked = INFINITY;
+    hs.k_spmv = 0;
+    hs.k_update = 0;
     hs.max_iter = max_iterations;
     hs.check_every = kChunk;
     hs.done = max_iterations == 0 ? kCapReached : kRunning;
@@ -395,7 +453,7 @@ int32_t solve_rows(const float *rhs, const uint32_t *neighbors, const float *dia
                        static_cast<const float *>(s_rhs), nbr, blk, diag, inv, r, dx, p0, st, kF64 ? 0 : 1);
     VX_HIP(hipGetLastError());
     if (max_iterations > 0) {
-        hipLaunchKernelGGL(rows_init<S>, dim3(grid), dim3(kBlock), 0, stream, n, per_xcd, ntiles, inv, r, z, st, part);
+        hipLaunchKernelGGL(rows_init<S>, dim3(grid), dim3(kBlock), 0, stream, n, per_xcd, ntiles, inv, r, z, part);
         VX_HIP(hipGetLastError());
 
         RowsGraph &g = g_graph[kF64 ? 1 : 0];
@@ -428,7 +486,9 @@ int32_t solve_rows(const float *rhs, const uint32_t *neighbors, const float *dia
         VX_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
         uint32_t launched = 0, chunk = 0;
         int32_t rc = VX_OK;
-        while (launched < max_iterations) {
+        // The check after the last update runs in the following rows_spmv, so
+        // replay until one spmv past the cap has been issued.
+        while (launched <= max_iterations) {
             hipError_t e = hipGraphLaunch(g.exec, stream);
             if (e == hipSuccess) e = hipMemcpyAsync(&pinned[chunk % 2], d_state, sizeof hs, hipMemcpyDeviceToHost, stream);
             if (e == hipSuccess) e = hipEventRecord(ev[chunk % 2], stream);
