@@ -534,42 +534,68 @@ __global__ __launch_bounds__(kBlock) void grid_spmv(DevScene G, uint32_t per_xcd
         G.g.coords(v, x, y, zc);
         const uint32_t c[3] = {x, y, zc};
         const uint32_t ext[3] = {G.g.nx, G.g.ny, G.g.nz};
-        S p[3];
+        // Load everything first (zeros where a neighbour is absent) so the
+        // loads of all six neighbours are in flight together; compute after.
+        S zz[7][3], pp[7][3], Dv[6], Kp[3][6], Km[3][6];
+        const uint8_t m = G.mask[v];
 #pragma unroll
-        for (int q = 0; q < 3; ++q) p[q] = fmaS<S>(beta, p_old[q * n + v], z[q * n + v]);
-        S yv[3];
-        symv<S>(Inner<S>::d(G), n, v, p, yv);
-        S acc[3] = {S(0), S(0), S(0)};
+        for (int q = 0; q < 3; ++q) {
+            zz[0][q] = z[q * n + v];
+            pp[0][q] = p_old[q * n + v];
+        }
+#pragma unroll
+        for (int q = 0; q < 6; ++q) Dv[q] = Inner<S>::d(G)[q * n + v];
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             const uint32_t s = G.g.stride(a);
-            if (c[a] > 0) {
-                const uint32_t u = v - s;
-                S pu[3];
+            const S *Ka = K + size_t(a) * 6 * n;
+            const bool lo = c[a] > 0, hi = c[a] + 1 < ext[a];
+            const uint32_t um = lo ? v - s : v, up = hi ? v + s : v;
 #pragma unroll
-                for (int q = 0; q < 3; ++q) pu[q] = fmaS<S>(beta, p_old[q * n + u], z[q * n + u]);
-                symv_acc<S>(K + size_t(a) * 6 * n, n, u, pu, acc);
+            for (int q = 0; q < 3; ++q) {
+                zz[1 + 2 * a][q] = lo ? z[q * n + um] : S(0);
+                pp[1 + 2 * a][q] = lo ? p_old[q * n + um] : S(0);
+                zz[2 + 2 * a][q] = hi ? z[q * n + up] : S(0);
+                pp[2 + 2 * a][q] = hi ? p_old[q * n + up] : S(0);
             }
-            if (c[a] + 1 < ext[a]) {
-                const uint32_t u = v + s;
-                S pu[3];
 #pragma unroll
-                for (int q = 0; q < 3; ++q) pu[q] = fmaS<S>(beta, p_old[q * n + u], z[q * n + u]);
-                symv_acc<S>(K + size_t(a) * 6 * n, n, v, pu, acc);
+            for (int q = 0; q < 6; ++q) {
+                Km[a][q] = lo ? Ka[q * n + um] : S(0);
+                Kp[a][q] = Ka[q * n + v];  // zero when the +axis bond is absent
             }
         }
-        const uint8_t m = G.mask[v];
-        yv[0] -= (m & 1) ? acc[0] : S(0);
-        yv[1] -= (m & 2) ? acc[1] : S(0);
-        yv[2] -= (m & 4) ? acc[2] : S(0);
+        S p[7][3];
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            p_new[q * n + v] = p[q];
-            ap[q * n + v] = yv[q];
+        for (int t = 0; t < 7; ++t)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) p[t][q] = fmaS<S>(beta, pp[t][q], zz[t][q]);
+        S y0 = fmaS<S>(Dv[2], p[0][2], fmaS<S>(Dv[1], p[0][1], Dv[0] * p[0][0]));
+        S y1 = fmaS<S>(Dv[4], p[0][2], fmaS<S>(Dv[3], p[0][1], Dv[1] * p[0][0]));
+        S y2 = fmaS<S>(Dv[5], p[0][2], fmaS<S>(Dv[4], p[0][1], Dv[2] * p[0][0]));
+        S a0 = S(0), a1 = S(0), a2 = S(0);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const S *km = Km[a], *kp = Kp[a];
+            const S *pm = p[1 + 2 * a], *pq = p[2 + 2 * a];
+            a0 = fmaS<S>(km[2], pm[2], fmaS<S>(km[1], pm[1], fmaS<S>(km[0], pm[0], a0)));
+            a1 = fmaS<S>(km[4], pm[2], fmaS<S>(km[3], pm[1], fmaS<S>(km[1], pm[0], a1)));
+            a2 = fmaS<S>(km[5], pm[2], fmaS<S>(km[4], pm[1], fmaS<S>(km[2], pm[0], a2)));
+            a0 = fmaS<S>(kp[2], pq[2], fmaS<S>(kp[1], pq[1], fmaS<S>(kp[0], pq[0], a0)));
+            a1 = fmaS<S>(kp[4], pq[2], fmaS<S>(kp[3], pq[1], fmaS<S>(kp[1], pq[0], a1)));
+            a2 = fmaS<S>(kp[5], pq[2], fmaS<S>(kp[4], pq[1], fmaS<S>(kp[2], pq[0], a2)));
         }
-        part[0] = double(p[0]) * double(yv[0]);
-        part[0] = __builtin_fma(double(p[1]), double(yv[1]), part[0]);
-        part[0] = __builtin_fma(double(p[2]), double(yv[2]), part[0]);
+        if (m & 1) y0 -= a0;
+        if (m & 2) y1 -= a1;
+        if (m & 4) y2 -= a2;
+        p_new[v] = p[0][0];
+        p_new[n + v] = p[0][1];
+        p_new[2 * n + v] = p[0][2];
+        ap[v] = y0;
+        ap[n + v] = y1;
+        ap[2 * n + v] = y2;
+        part[0] = double(p[0][0]) * double(y0);
+        part[0] = __builtin_fma(double(p[0][1]), double(y1), part[0]);
+        part[0] = __builtin_fma(double(p[0][2]), double(y2), part[0]);
     }
     const int out_slots[1] = {kPap};
     store_partials<1>(part, G.partials, out_slots, nb, lds);
@@ -602,15 +628,26 @@ __global__ __launch_bounds__(kBlock) void grid_update(DevScene G, uint32_t per_x
     const uint32_t v = tile * kBlock + threadIdx.x;
     double part[2] = {0.0, 0.0};
     if (tile < ntiles && v < n) {
+        S pv[3], av[3], rv[3], dv[3], Di[6];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            pv[q] = p[q * n + v];
+            av[q] = ap[q * n + v];
+            rv[q] = r[q * n + v];
+            dv[q] = first ? S(0) : d[q * n + v];
+        }
+#pragma unroll
+        for (int q = 0; q < 6; ++q) Di[q] = Inner<S>::dinv(G)[q * n + v];
         S ri[3], zi[3];
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
-            const S pq = p[q * n + v];
-            d[q * n + v] = first ? alpha * pq : fmaS<S>(alpha, pq, d[q * n + v]);
-            ri[q] = fmaS<S>(-alpha, ap[q * n + v], r[q * n + v]);
+            d[q * n + v] = fmaS<S>(alpha, pv[q], dv[q]);
+            ri[q] = fmaS<S>(-alpha, av[q], rv[q]);
             r[q * n + v] = ri[q];
         }
-        symv<S>(Inner<S>::dinv(G), n, v, ri, zi);
+        zi[0] = fmaS<S>(Di[2], ri[2], fmaS<S>(Di[1], ri[1], Di[0] * ri[0]));
+        zi[1] = fmaS<S>(Di[4], ri[2], fmaS<S>(Di[3], ri[1], Di[1] * ri[0]));
+        zi[2] = fmaS<S>(Di[5], ri[2], fmaS<S>(Di[4], ri[1], Di[2] * ri[0]));
 #pragma unroll
         for (int q = 0; q < 3; ++q) z[q * n + v] = zi[q];
         part[0] = __builtin_fma(double(ri[2]), double(zi[2]), __builtin_fma(double(ri[1]), double(zi[1]), double(ri[0]) * double(zi[0])));

@@ -185,34 +185,6 @@ __global__ __launch_bounds__(kBlock) void rows_spmv(uint32_t n, uint32_t per_xcd
     const uint32_t tile = xcd_tile(blockIdx.x, per_xcd);
     const uint32_t i = tile * kBlock + threadIdx.x;
     const bool row = tile < ntiles && i < n;
-    // The neighbour gathers (index load, then vector loads) and the row's own
-    // vectors.
-    uint32_t jj[6];
-    S zz[7][3], pp[7][3];
-#pragma unroll
-    for (int t = 0; t < 7; ++t)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) zz[t][c] = pp[t][c] = S(0);
-#pragma unroll
-    for (int s = 0; s < 6; ++s) jj[s] = VX_EMPTY;
-    if (row) {
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            zz[0][c] = z[c * N + i];
-            pp[0][c] = p_old[c * N + i];
-        }
-#pragma unroll
-        for (int s = 0; s < 6; ++s) jj[s] = nbr[s * N + i];
-#pragma unroll
-        for (int s = 0; s < 6; ++s) {
-            if (jj[s] == VX_EMPTY) continue;
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                zz[1 + s][c] = z[c * N + jj[s]];
-                pp[1 + s][c] = p_old[c * N + jj[s]];
-            }
-        }
-    }
     // Convergence check of the previous update (every check_every iterations
     // and at the cap, `<=`, src/cuda_solver.rs:458-477); beta = rho_k / rho_{k-1}.
     const double rho = sums[0], rr = sums[1];
@@ -242,17 +214,41 @@ __global__ __launch_bounds__(kBlock) void rows_spmv(uint32_t n, uint32_t per_xcd
         st->k_update = k;
     }
     // p = z + beta p_old on load, y = D p + sum_s B_s p_s in the reference
-    // kernel's fused multiply-add order.
+    // kernel's fused multiply-add order. All loads are issued before any
+    // arithmetic (absent slots read the row itself and are skipped).
     double v[1] = {0.0};
     if (row) {
+        uint32_t jj[6];
+#pragma unroll
+        for (int s = 0; s < 6; ++s) jj[s] = nbr[s * N + i];
+        S zz[7][3], pp[7][3], dg[9];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            zz[0][c] = z[c * N + i];
+            pp[0][c] = p_old[c * N + i];
+        }
+#pragma unroll
+        for (int s = 0; s < 6; ++s) {
+            const uint32_t j = jj[s] == VX_EMPTY ? i : jj[s];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                zz[1 + s][c] = z[c * N + j];
+                pp[1 + s][c] = p_old[c * N + j];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 9; ++q) dg[q] = diag[q * N + i];
+        S bb[54];
+#pragma unroll
+        for (int q = 0; q < 54; ++q) bb[q] = blk[q * N + i];
         S p[3], y[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) p[c] = fma_s<S>(beta, pp[0][c], zz[0][c]);
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
-            S a = diag[(3 * r) * N + i] * p[0];
-            a = fma_s<S>(diag[(3 * r + 1) * N + i], p[1], a);
-            a = fma_s<S>(diag[(3 * r + 2) * N + i], p[2], a);
+            S a = dg[3 * r] * p[0];
+            a = fma_s<S>(dg[3 * r + 1], p[1], a);
+            a = fma_s<S>(dg[3 * r + 2], p[2], a);
             y[r] = a;
         }
 #pragma unroll
@@ -263,10 +259,10 @@ __global__ __launch_bounds__(kBlock) void rows_spmv(uint32_t n, uint32_t per_xcd
             for (int c = 0; c < 3; ++c) pj[c] = fma_s<S>(beta, pp[1 + s][c], zz[1 + s][c]);
 #pragma unroll
             for (int r = 0; r < 3; ++r) {
-                const S *b = blk + (9 * s + 3 * r) * N + i;
+                const S *b = bb + 9 * s + 3 * r;
                 y[r] = fma_s<S>(b[0], pj[0], y[r]);
-                y[r] = fma_s<S>(b[N], pj[1], y[r]);
-                y[r] = fma_s<S>(b[2 * N], pj[2], y[r]);
+                y[r] = fma_s<S>(b[1], pj[1], y[r]);
+                y[r] = fma_s<S>(b[2], pj[2], y[r]);
             }
         }
 #pragma unroll
