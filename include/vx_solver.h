@@ -206,6 +206,57 @@ typedef struct vx_step_report {
 
 int32_t vx_scene_step(vx_scene *scene, const vx_step_params *params, vx_step_report *report);
 
+/* ---- batched and served scenes (build extensions) ----------------------- */
+/*
+ * A batch is an HBM arena of 512-REV tiles shared by many independent box
+ * scenes of uneven size. Every kernel of the solve (predict, fit, assembly,
+ * FP64 residual, FP32 inner PCG, apply, velocities) is launched once per
+ * scheduler tick for all scenes; each scene advances through its own steps,
+ * corotation passes and refinement sweeps with per-scene scalars, early exit
+ * and convergence decided on the device. Scenes may be admitted while others
+ * run (continuous batching) as long as a contiguous tile range is free, and
+ * released when done. The reference has no counterpart.
+ */
+typedef struct vx_batch vx_batch;
+
+#define VX_SCENE_FREE 0
+#define VX_SCENE_RUNNING 1
+#define VX_SCENE_DONE 2
+#define VX_SCENE_FAILED 3
+
+typedef struct vx_batch_scene_report {
+    uint32_t state;              /* VX_SCENE_* */
+    uint32_t status;             /* VX_* status of a failed scene */
+    uint32_t steps_done;
+    uint32_t solves;
+    uint64_t inner_iterations;   /* summed over solves */
+    uint64_t outer_sweeps;       /* summed over solves */
+    uint32_t rows;               /* REVs */
+    uint32_t active_rows;
+    double last_relative_residual;
+    double last_angle;
+} vx_batch_scene_report;
+
+/* HBM bytes per arena row (for admission budgets). */
+uint64_t vx_batch_bytes_per_row(void);
+/* capacity_rows is rounded up to whole tiles; xcd_chunk_tiles is the tile
+ * interleave across the 8 XCDs (0 = choose from the capacity). */
+int32_t vx_batch_create(uint64_t capacity_rows, uint32_t max_scenes, uint32_t xcd_chunk_tiles, vx_batch **out);
+int32_t vx_batch_destroy(vx_batch *batch);
+/* Largest contiguous free range and total free space, in rows. */
+int32_t vx_batch_free_rows(vx_batch *batch, uint64_t *largest, uint64_t *total);
+/* Admit a scene that will run `steps` fixed steps with `params`. Returns
+ * VX_ERR_TOO_LARGE (nothing admitted) when no contiguous range fits. */
+int32_t vx_batch_admit(vx_batch *batch, const vx_scene_desc *desc, const vx_step_params *params, uint32_t steps,
+                       uint32_t *out_slot);
+/* One scheduler tick for all running scenes; writes how many still run. */
+int32_t vx_batch_tick(vx_batch *batch, uint32_t *out_running);
+int32_t vx_batch_report(vx_batch *batch, uint32_t slot, vx_batch_scene_report *report);
+/* Copy one scene's state out (any pointer may be NULL). */
+int32_t vx_batch_download(vx_batch *batch, uint32_t slot, double *positions, double *rotations, double *velocities);
+/* Free a finished (or failed) scene's tiles and slot. */
+int32_t vx_batch_release(vx_batch *batch, uint32_t slot);
+
 #ifdef __cplusplus
 }
 #endif

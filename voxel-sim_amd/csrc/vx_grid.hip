@@ -1,29 +1,41 @@
-// Device-resident structured scenes: rotation fit, bond assembly and the
-// mixed-precision PCG solve on an nx*ny*nz REV box.
+// Device-resident structured scenes on a tile arena: rotation fit, bond
+// assembly and the mixed-precision PCG solve for one or many REV boxes.
 //
-// HBM layout (all structure-of-arrays planes, plane q of a Q-wide field at
-// base + q*n, voxel v = x + nx*(y + ny*z)):
-//   state      positions/predicted/previous/velocity/drift 3 x f64,
-//              rotations 4 x f64, material constants f32, bond flags u8
-//   operator   per +axis bond a: k (symmetric 3x3, 6 values) in f64 and f32,
-//              bond force 3 x f64; per voxel: projected diagonal D (6 values)
-//              in f64 and f32, its inverse in f64 and f32, rhs 3 x f64, free
-//              mask u8
-//   solve      x (FP64 solution, double-buffered), d, r, z, p (x2), Ap in the
-//              inner precision
-// Each bond block is stored once, on its -end voxel; the SpMV of a row reads
-// its own three +axis blocks and the -axis neighbours' blocks (the matrix is
-// symmetric, A_ji = A_ij^T = A_ij). The free mask is applied to the row
-// output; every PCG vector is zero on fixed DOFs, so no column projection
-// is needed.
+// HBM layout. An arena holds `cap_tiles` tiles of kBlock (512) REV rows;
+// every field is a structure-of-arrays plane of R = cap_tiles * kBlock
+// entries (plane q of a Q-wide field at base + q*R). A scene owns a
+// contiguous range of tiles and its REV v = x + nx*(y + ny*z) lives at arena
+// row voff + v. Per REV: positions / predicted / previous / velocity / drift
+// (3 x f64), rotation (4 x f64), material constants (f32), flags (u8).
+// Operator: per +axis bond a, k (symmetric 3x3, 6 values) in f64 and f32 and
+// the bond force (3 x f64), stored on the bond's -end REV; per REV the
+// projected diagonal D and its inverse (6 values, f64 and f32), the rhs
+// (3 x f64) and the free mask (u8). Solve: the FP64 solution (two ping-pong
+// buffers plus a warm-start copy) and the inner vectors d, r, z, p (x2), Ap
+// in f32 (and f64 for the plain FP64 mode).
+//
+// The SpMV of a row reads its own three +axis blocks and the -axis
+// neighbours' blocks (A_ji = A_ij^T = A_ij for a symmetric k); the free mask
+// is applied to the row output and every PCG vector is zero on fixed DOFs.
+//
+// Scheduling. Kernels are launched over the whole arena once per tick; a
+// workgroup maps to one tile, looks up the owning scene and acts only when
+// that scene's phase bit for the kernel is set. Per-scene reductions sum the
+// scene's per-tile partials (written with plain stores by the previous kernel)
+// inside every workgroup, in a fixed order, so scalars are reproducible and
+// need no atomics or fences. Tiles are interleaved over the 8 XCDs in chunks
+// of `chunk` tiles (physical workgroup b runs on XCD b % 8).
 //
 // The algorithm restates src/solver.rs:27-110 (frames, compliance,
-// stiffness, thermal rest), :327-417 (rotation fit), :1325-1345 (constraint)
-// and :1552-1636 (assembly and projection); the PCG semantics follow
+// stiffness, thermal rest), :327-417 (rotation fit), :1325-1345
+// (constraint), :1552-1636 (assembly and projection), :475-570 and 668-683
+// (substep, corotation loop with early exit, velocities); the PCG follows
 // src/solver.rs:1833-1911 with block-Jacobi, run as FP32 inner solves inside
 // an FP64 true-residual refinement loop.
 #include "vx_common.hpp"
+#include "vx_mech.hpp"
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -32,240 +44,198 @@
 namespace vx {
 namespace {
 
-constexpr double kCell = 0.01;  // src/scene.rs:10
-constexpr uint32_t kInnerChunk = 16;
+constexpr uint32_t kInnerChunk = 16;  // inner iterations per graph replay
+constexpr uint32_t kFreeTile = 0xFFFFFFFFu;
 
-// ---------------------------------------------------------------------------
-// FP64 small linear algebra (columns c0, c1, c2 like the reference's Mat3).
-// ---------------------------------------------------------------------------
-struct d3 {
-    double x, y, z;
-};
-__device__ __forceinline__ d3 mk(double x, double y, double z) { return d3{x, y, z}; }
-__device__ __forceinline__ d3 operator+(d3 a, d3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
-__device__ __forceinline__ d3 operator-(d3 a, d3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
-__device__ __forceinline__ d3 operator*(d3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
-__device__ __forceinline__ double dot3(d3 a, d3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
-__device__ __forceinline__ d3 cross3(d3 a, d3 b) {
-    return {a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y};
-}
-__device__ __forceinline__ d3 normalize_or_zero(d3 a) {
-    double rcp = 1.0 / sqrt(dot3(a, a));
-    if (isfinite(rcp) && rcp > 0.0) return a * rcp;
-    return {0.0, 0.0, 0.0};
-}
-struct m3 {
-    d3 c0, c1, c2;
-};
-__device__ __forceinline__ d3 col(const m3 &m, int i) { return i == 0 ? m.c0 : (i == 1 ? m.c1 : m.c2); }
-__device__ __forceinline__ d3 mul(const m3 &m, d3 v) { return m.c0 * v.x + m.c1 * v.y + m.c2 * v.z; }
-__device__ __forceinline__ d3 mul_t(const m3 &m, d3 v) { return {dot3(m.c0, v), dot3(m.c1, v), dot3(m.c2, v)}; }
-__device__ __forceinline__ double det3(const m3 &m) { return dot3(m.c2, cross3(m.c0, m.c1)); }
-__device__ __forceinline__ m3 transpose3(const m3 &m) {
-    return {{m.c0.x, m.c1.x, m.c2.x}, {m.c0.y, m.c1.y, m.c2.y}, {m.c0.z, m.c1.z, m.c2.z}};
-}
-// Inverse by cross products of column pairs (rows of the inverse).
-__device__ __forceinline__ m3 inverse3m(const m3 &m) {
-    d3 t0 = cross3(m.c1, m.c2), t1 = cross3(m.c2, m.c0), t2 = cross3(m.c0, m.c1);
-    double inv = 1.0 / dot3(m.c2, t2);
-    return transpose3(m3{t0 * inv, t1 * inv, t2 * inv});
-}
-
-struct q4 {
-    double x, y, z, w;
-};
-__device__ __forceinline__ double qdot(q4 a, q4 b) { return ((a.x * b.x + a.y * b.y) + a.z * b.z) + a.w * b.w; }
-__device__ __forceinline__ q4 qnormalize(q4 q) {
-    double r = 1.0 / sqrt(qdot(q, q));
-    return {q.x * r, q.y * r, q.z * r, q.w * r};
-}
-__device__ __forceinline__ d3 qrotate(q4 q, d3 v) {
-    d3 b{q.x, q.y, q.z};
-    return v * (q.w * q.w - dot3(b, b)) + b * (dot3(v, b) * 2.0) + cross3(b, v) * (q.w * 2.0);
-}
-__device__ __forceinline__ m3 mat_from_quat(q4 q) {
-    double x2 = q.x + q.x, y2 = q.y + q.y, z2 = q.z + q.z;
-    double xx = q.x * x2, xy = q.x * y2, xz = q.x * z2, yy = q.y * y2, yz = q.y * z2, zz = q.z * z2;
-    double wx = q.w * x2, wy = q.w * y2, wz = q.w * z2;
-    return {{1.0 - (yy + zz), xy + wz, xz - wy}, {xy - wz, 1.0 - (xx + zz), yz + wx}, {xz + wy, yz - wx, 1.0 - (xx + yy)}};
-}
-// Shepperd's method: pick the largest of w^2, x^2, y^2, z^2 for the root.
-__device__ __forceinline__ q4 quat_from_mat(const m3 &m) {
-    double m00 = m.c0.x, m01 = m.c0.y, m02 = m.c0.z, m10 = m.c1.x, m11 = m.c1.y, m12 = m.c1.z;
-    double m20 = m.c2.x, m21 = m.c2.y, m22 = m.c2.z;
-    if (m22 <= 0.0) {
-        double dif10 = m11 - m00, omm22 = 1.0 - m22;
-        if (dif10 <= 0.0) {
-            double f = omm22 - dif10, s = 0.5 / sqrt(f);
-            return {f * s, (m01 + m10) * s, (m02 + m20) * s, (m12 - m21) * s};
-        }
-        double f = omm22 + dif10, s = 0.5 / sqrt(f);
-        return {(m01 + m10) * s, f * s, (m12 + m21) * s, (m20 - m02) * s};
-    }
-    double sum10 = m11 + m00, opm22 = 1.0 + m22;
-    if (sum10 <= 0.0) {
-        double f = opm22 - sum10, s = 0.5 / sqrt(f);
-        return {(m02 + m20) * s, (m12 + m21) * s, f * s, (m01 - m10) * s};
-    }
-    double f = opm22 + sum10, s = 0.5 / sqrt(f);
-    return {(m12 - m21) * s, (m20 - m02) * s, (m01 - m10) * s, f * s};
-}
-// Half-way slerp with the shortest-arc flip; normalized lerp above 0.9995.
-__device__ __forceinline__ q4 slerp_half(q4 a, q4 b) {
-    double d = qdot(a, b);
-    if (d < 0.0) {
-        b = {-b.x, -b.y, -b.z, -b.w};
-        d = -d;
-    }
-    if (d > 0.9995) return qnormalize({a.x + (b.x - a.x) * 0.5, a.y + (b.y - a.y) * 0.5, a.z + (b.z - a.z) * 0.5, a.w + (b.w - a.w) * 0.5});
-    double th = acos(d);
-    double s1 = sin(th * 0.5), s2 = sin(th * 0.5), inv = 1.0 / sin(th);
-    return {(a.x * s1 + b.x * s2) * inv, (a.y * s1 + b.y * s2) * inv, (a.z * s1 + b.z * s2) * inv, (a.w * s1 + b.w * s2) * inv};
-}
-// basis(axis, q): rotated e_axis, e_axis+1, e_axis+2 (src/solver.rs:27-34).
-__device__ __forceinline__ m3 basis(int axis, q4 q) {
-    const d3 e[3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
-    return {qrotate(q, e[axis]), qrotate(q, e[(axis + 1) % 3]), qrotate(q, e[(axis + 2) % 3])};
-}
-
-// Symmetric 3x3 stored as (xx, xy, xz, yy, yz, zz).
-struct s6 {
-    double a[6];
+// Phase bits of a scene for one tick.
+enum : uint32_t {
+    B_PREDICT = 1u << 0,
+    B_FIT = 1u << 1,      // fit after predict
+    B_ASSEMBLE = 1u << 2,
+    B_RESID0 = 1u << 3,   // first residual of a solve (x0 = 0 or warm start)
+    B_RESID = 1u << 4,    // refinement residual (x += d first)
+    B_INNER = 1u << 5,
+    B_APPLY = 1u << 6,
+    B_FITB = 1u << 7,     // fit after apply, records the angle
+    B_FINISH = 1u << 8,
+    B_WARM_USE = 1u << 9,
+    B_WARM_SAVE = 1u << 10,
+    B_F64 = 1u << 11,     // inner solve in FP64 (plain FP64 mode)
 };
 
-// ---------------------------------------------------------------------------
-// Scene storage
-// ---------------------------------------------------------------------------
-struct Geo {
-    uint32_t nx, ny, nz, n;
-    __device__ __forceinline__ void coords(uint32_t v, uint32_t &x, uint32_t &y, uint32_t &z) const {
-        x = v % nx;
-        uint32_t t = v / nx;
-        y = t % ny;
-        z = t / ny;
-    }
-    __device__ __forceinline__ uint32_t stride(int a) const { return a == 0 ? 1u : (a == 1 ? nx : nx * ny); }
-};
-
-// Device-resident solve scalars. Each field is written by workgroup 0 of one
-// kernel and read only by later kernels.
-struct GridState {
-    double rho[2];         // r.z after k inner updates, slot k & 1 (grid_spmv)
-    double rr0;            // r.r at the start of the inner solve (grid_spmv, k = 0)
-    double inner_tol;      // relative inner tolerance
-    double rr_last;        // r.r seen by the last check
-    uint64_t angle_bits;   // max rotation change as ordered bits of a non-negative double
-    uint32_t k_spmv;       // inner updates completed (grid_update, grid_residual)
-    uint32_t k_update;     // iteration of the current direction (grid_spmv)
-    uint32_t iters, done, max_iter, pad;
-};
+enum : int { kPap = 0, kRz = 1, kRr = 2, kRr64 = 3, kBb = 4, kAngle = 5, kSlots = 6 };  // partial kinds
 enum : uint32_t { kRun = 0, kConv = 1, kNonFin = 2, kCap = 3 };
-enum : int { kPap = 0, kRz = 1, kRr = 2, kRr64 = 3, kBb = 4 };  // partial slots
 
-struct DevScene {
-    Geo g;
+struct SlotGeo {
+    uint32_t nx, ny, nz, n;
+    uint32_t tile0, ntiles, voff, active;
+};
+struct SlotParams {
+    double h, gx, gy, gz;
+    double tol, inner_tol;
+    uint32_t max_inner, max_outer;
+};
+// Written by the device; each field by one workgroup of one kernel and read
+// by later kernels (and by the host after a tick).
+struct SlotState {
+    double rho[2], rr0, rr_last;         // inner PCG (rho by iteration parity)
+    double rr64, bb, rel, angle;         // outer residual, corotation angle
+    uint32_t k_spmv, k_update, iters, done;
+    uint32_t sweeps, outer_done, xpar, inner_total;
+    uint32_t nonfinite, pad[3];
+};
+
+struct DevArena {
+    uint32_t R, cap_tiles, chunk, max_slots;
+    const uint32_t *tile_slot;
+    const SlotGeo *geo;
+    const SlotParams *prm;
+    const uint32_t *bits;
+    SlotState *st;
+    double *partials;  // kSlots * cap_tiles
     // state
-    double *pos, *pred, *prev, *vel, *drift, *rot;
-    float *mass, *young, *young_perp, *shear, *solid, *fiber, *aniso, *dtemp, *a_par, *a_perp;
-    uint8_t *kinematic, *bonds, *tmask;  // bonds: bit a = +axis bond present; tmask: bit a = axis targeted
-    double *targets;
-    float *loads;
-    float *damage, *kscale, *offset, *plastic;  // 3*n, 3*n, 9*n ([a][c][v]), 9*n
+    double *pos, *pred, *prev, *vel, *drift, *rot, *xa, *xb, *xw, *targets;
+    float *mass, *young, *young_perp, *shear, *solid, *fiber, *aniso, *dtemp, *a_par, *a_perp, *loads;
+    uint8_t *kinematic, *bonds, *tmask, *mask;
+    float *damage, *kscale, *offset, *plastic;  // 3R, 3R, 9R ([a][c]), 9R
     // operator
-    double *k64, *force;  // 18*n ([a][q][v]), 9*n ([a][c][v])
-    float *k32;           // 18*n
-    double *d64, *dinv64, *rhs;  // 6*n, 6*n, 3*n
-    float *d32, *dinv32;         // 6*n, 6*n
-    uint8_t *mask;
-    // solve
-    double *xa, *xb;             // FP64 solution, ping-pong
-    float *d, *r, *z, *p0, *p1, *ap;  // FP32 inner
-    double *dd, *rd, *zd, *p0d, *p1d, *apd;  // FP64 inner (plain FP64 mode)
-    double *partials;
-    GridState *st;
+    double *k64, *force, *d64, *dinv64, *rhs;  // 18R, 9R, 6R, 6R, 3R
+    float *k32, *d32, *dinv32;                  // 18R, 6R, 6R
+    // inner vectors
+    float *d, *r, *z, *p0, *p1, *ap;
+    double *dd, *rd, *zd, *p0d, *p1d, *apd;
 };
 
-}  // namespace
-}  // namespace vx
-
-struct vx_scene {
-    vx::DevScene s;
-    uint32_t active = 0;
-    uint32_t per_xcd = 0, ntiles = 0, grid = 0;
-    void *alloc = nullptr;
-    size_t alloc_bytes = 0;
-    bool x_in_a = true;           // which ping-pong buffer holds the current solution
-    bool has_solution = false;
-    hipGraphExec_t inner32 = nullptr, inner64 = nullptr;
-    hipEvent_t ev[2] = {nullptr, nullptr};
-    vx::GridState *pinned = nullptr;  // 3 status slots
-    double *host_partials = nullptr;  // pinned, 2 * grid
-};
-
-namespace vx {
-namespace {
-
-// ---------------------------------------------------------------------------
-// Kernels
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ d3 ld3(const double *p, size_t n, uint32_t v) { return {p[v], p[n + v], p[2 * n + v]}; }
-__device__ __forceinline__ void st3(double *p, size_t n, uint32_t v, d3 a) {
-    p[v] = a.x;
-    p[n + v] = a.y;
-    p[2 * n + v] = a.z;
+// ---- tile context -----------------------------------------------------------
+// Physical workgroup b runs on XCD b % 8; logical tiles are dealt to the XCDs
+// in chunks of `chunk` consecutive tiles.
+__device__ __forceinline__ uint32_t logical_tile(uint32_t b, uint32_t chunk) {
+    const uint32_t r = b / kXcds, x = b % kXcds;
+    return ((r / chunk) * kXcds + x) * chunk + (r % chunk);
 }
-__device__ __forceinline__ q4 ldq(const double *p, size_t n, uint32_t v) { return {p[v], p[n + v], p[2 * n + v], p[3 * n + v]}; }
 
+struct Tile {
+    uint32_t t, slot, bits;
+    SlotGeo g;
+    uint32_t local, v;  // scene-local row and arena row of this thread
+    bool row;
+};
+
+__device__ __forceinline__ bool tile_for(const DevArena &A, uint32_t need, Tile &c) {
+    c.t = logical_tile(blockIdx.x, A.chunk);
+    if (c.t >= A.cap_tiles) return false;
+    c.slot = A.tile_slot[c.t];
+    if (c.slot == kFreeTile) return false;
+    c.bits = A.bits[c.slot];
+    if (!(c.bits & need)) return false;
+    c.g = A.geo[c.slot];
+    c.local = (c.t - c.g.tile0) * kBlock + threadIdx.x;
+    c.row = c.local < c.g.n;
+    c.v = c.g.voff + c.local;
+    return true;
+}
+
+__device__ __forceinline__ void coords(const SlotGeo &g, uint32_t local, uint32_t (&c)[3]) {
+    c[0] = local % g.nx;
+    const uint32_t t = local / g.nx;
+    c[1] = t % g.ny;
+    c[2] = t / g.ny;
+}
+__device__ __forceinline__ uint32_t stride_of(const SlotGeo &g, int a) { return a == 0 ? 1u : (a == 1 ? g.nx : g.nx * g.ny); }
+__device__ __forceinline__ uint32_t extent_of(const SlotGeo &g, int a) { return a == 0 ? g.nx : (a == 1 ? g.ny : g.nz); }
+
+__device__ __forceinline__ d3 ld3(const double *p, size_t R, uint32_t v) { return {p[v], p[R + v], p[2 * R + v]}; }
+__device__ __forceinline__ void st3(double *p, size_t R, uint32_t v, d3 a) {
+    p[v] = a.x;
+    p[R + v] = a.y;
+    p[2 * R + v] = a.z;
+}
+__device__ __forceinline__ q4 ldq(const double *p, size_t R, uint32_t v) { return {p[v], p[R + v], p[2 * R + v], p[3 * R + v]}; }
+__device__ __forceinline__ Rev ld_rev(const DevArena &A, uint32_t v) {
+    const size_t R = A.R;
+    return Rev{A.young[v], A.young_perp[v], A.shear[v], A.solid[v], A.aniso[v],
+               mk(A.fiber[v], A.fiber[R + v], A.fiber[2 * R + v]), A.dtemp[v], A.a_par[v], A.a_perp[v]};
+}
+
+// Per-scene sum of N partial kinds over the scene's tiles, broadcast to every
+// thread of the workgroup (fixed order).
+template <int N>
+__device__ __forceinline__ void scene_sum(double (&out)[N], const DevArena &A, const SlotGeo &g, const int (&kind)[N],
+                                          double (*lds)[kWaves]) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        double s = 0.0;
+        for (uint32_t b = threadIdx.x; b < g.ntiles; b += kBlock) s += A.partials[size_t(kind[k]) * A.cap_tiles + g.tile0 + b];
+        out[k] = s;
+    }
+    block_sum_all<N>(out, lds);
+}
+template <int N>
+__device__ __forceinline__ void tile_store(double (&v)[N], const DevArena &A, uint32_t t, const int (&kind)[N],
+                                           double (*lds)[kWaves]) {
+    block_sum_all<N>(v, lds);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) A.partials[size_t(kind[k]) * A.cap_tiles + t] = v[k];
+    }
+}
+
+// ---- physics kernels -------------------------------------------------------
 // Predict one substep (src/solver.rs:494-523, 542).
-__global__ __launch_bounds__(kBlock) void grid_predict(DevScene S, double h, double gx, double gy, double gz) {
-    const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
-    const size_t n = S.g.n;
-    if (v >= n) return;
-    d3 p = ld3(S.pos, n, v);
-    st3(S.prev, n, v, p);
-    const uint8_t tm = S.tmask[v];
-    if (!S.kinematic[v]) {
-        d3 vel = ld3(S.vel, n, v) + mk(gx, gy, gz) * h;
+__global__ __launch_bounds__(kBlock) void k_predict(DevArena A) {
+    Tile c;
+    if (!tile_for(A, B_PREDICT, c) || !c.row) return;
+    const size_t R = A.R;
+    const uint32_t v = c.v;
+    const SlotParams p = A.prm[c.slot];
+    d3 x = ld3(A.pos, R, v);
+    st3(A.prev, R, v, x);
+    const uint8_t tm = A.tmask[v];
+    if (!A.kinematic[v]) {
+        d3 vel = ld3(A.vel, R, v) + mk(p.gx, p.gy, p.gz) * p.h;
         if (tm & 1) vel.x = 0.0;
         if (tm & 2) vel.y = 0.0;
         if (tm & 4) vel.z = 0.0;
-        st3(S.vel, n, v, vel);
-        st3(S.drift, n, v, vel);
-        p = p + vel * h;
+        st3(A.vel, R, v, vel);
+        st3(A.drift, R, v, vel);
+        x = x + vel * p.h;
     } else {
-        st3(S.drift, n, v, mk(0, 0, 0));
+        st3(A.drift, R, v, mk(0, 0, 0));
     }
-    if (tm & 1) p.x = S.targets[v];
-    if (tm & 2) p.y = S.targets[n + v];
-    if (tm & 4) p.z = S.targets[2 * n + v];
-    st3(S.pos, n, v, p);
-    st3(S.pred, n, v, p);
+    if (tm & 1) x.x = A.targets[v];
+    if (tm & 2) x.y = A.targets[R + v];
+    if (tm & 4) x.z = A.targets[2 * R + v];
+    st3(A.pos, R, v, x);
+    st3(A.pred, R, v, x);
 }
 
-// Rotation fit (src/solver.rs:327-417) in FP64.
-__global__ __launch_bounds__(kBlock) void grid_fit(DevScene S) {
-    const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
-    const size_t n = S.g.n;
+// Rotation fit (src/solver.rs:327-417) in FP64; with `record` the per-tile
+// largest angle change goes to the kAngle partials.
+__global__ __launch_bounds__(kBlock) void k_fit(DevArena A, uint32_t need, int record) {
+    Tile c;
+    if (!tile_for(A, need, c)) return;
+    __shared__ double lds[1][kWaves];
+    const size_t R = A.R;
     double angle = 0.0;
-    if (v < n && !S.kinematic[v]) {
-        uint32_t x, y, z;
-        S.g.coords(v, x, y, z);
-        const uint32_t c[3] = {x, y, z};
-        const q4 prev = ldq(S.rot, n, v);
+    if (c.row && !A.kinematic[c.v]) {
+        const uint32_t v = c.v;
+        uint32_t cc[3];
+        coords(c.g, c.local, cc);
+        const q4 prev = ldq(A.rot, R, v);
         const m3 old = mat_from_quat(prev);
-        const d3 pv = ld3(S.pos, n, v);
+        const d3 pv = ld3(A.pos, R, v);
         m3 f;
         d3 *cols[3] = {&f.c0, &f.c1, &f.c2};
         for (int a = 0; a < 3; ++a) {
             d3 sum{0, 0, 0};
             double cnt = 0.0;
-            const uint32_t s = S.g.stride(a);
-            if (c[a] > 0 && (S.bonds[v - s] >> a & 1)) {
-                sum = sum + (pv - ld3(S.pos, n, v - s)) * (1.0 / kCell);
+            const uint32_t s = stride_of(c.g, a);
+            if (cc[a] > 0 && (A.bonds[v - s] >> a & 1)) {
+                sum = sum + (pv - ld3(A.pos, R, v - s)) * (1.0 / kCell);
                 cnt += 1.0;
             }
-            if (S.bonds[v] >> a & 1) {
-                sum = sum + (ld3(S.pos, n, v + s) - pv) * (1.0 / kCell);
+            if (A.bonds[v] >> a & 1) {
+                sum = sum + (ld3(A.pos, R, v + s) - pv) * (1.0 / kCell);
                 cnt += 1.0;
             }
             *cols[a] = cnt > 0.0 ? sum * (1.0 / cnt) : col(old, a);
@@ -287,178 +257,119 @@ __global__ __launch_bounds__(kBlock) void grid_fit(DevScene S) {
             q4 q = prev;
             if (dot3(ex, ex) > 0.5 && dot3(ey, ey) > 0.5) {
                 q = qnormalize(quat_from_mat(m3{ex, ey, cross3(ex, ey)}));
-                S.rot[v] = q.x;
-                S.rot[n + v] = q.y;
-                S.rot[2 * n + v] = q.z;
-                S.rot[3 * n + v] = q.w;
+                A.rot[v] = q.x;
+                A.rot[R + v] = q.y;
+                A.rot[2 * R + v] = q.z;
+                A.rot[3 * R + v] = q.w;
             }
             angle = 2.0 * acos(fmin(fmax(fabs(qdot(prev, q)), 0.0), 1.0));
         }
     }
+    if (!record) return;
     angle = wave_max(angle);
-    if ((threadIdx.x & 63) == 0 && angle > 0.0)
-        atomicMax(reinterpret_cast<unsigned long long *>(&S.st->angle_bits), (unsigned long long)__double_as_longlong(angle));
+    if ((threadIdx.x & 63) == 0) lds[0][threadIdx.x >> 6] = angle;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = lds[0][0];
+        for (int w = 1; w < kWaves; ++w) m = fmax(m, lds[0][w]);
+        A.partials[size_t(kAngle) * A.cap_tiles + c.t] = m;
+    }
 }
 
-// tensor_compliance (src/solver.rs:37-63) of voxel v in the bond frame.
-__device__ __forceinline__ bool compliance(const DevScene &S, uint32_t v, const m3 &F, q4 rv, double (&C)[6]) {
-    const size_t n = S.g.n;
-    double solid = fmin(fmax(double(S.solid[v]), 1e-4), 1.0);
-    double E = S.young[v], Ep = S.young_perp[v], G = S.shear[v], an = S.aniso[v];
-    d3 fib = mul_t(F, qrotate(rv, mk(S.fiber[v], S.fiber[n + v], S.fiber[2 * n + v])));
-    double tr = E + an * (Ep - E);
-    double nrm = tr + (E - tr) * fib.x * fib.x;
-    d3 dg = mk(nrm, G, G) * solid;
-    if (!(isfinite(dg.x) && isfinite(dg.y) && isfinite(dg.z)) || fmin(fmin(dg.x, dg.y), dg.z) <= 0.0) return false;
-    d3 inv{1.0 / dg.x, 1.0 / dg.y, 1.0 / dg.z};
-    d3 sf{fib.x * inv.x, fib.y * inv.y, fib.z * inv.z};
-    double alpha = G * an * 0.2;
-    double c = alpha / (1.0 + alpha * dot3(fib, sf));
-    C[0] = inv.x - sf.x * sf.x * c;
-    C[1] = -sf.x * sf.y * c;
-    C[2] = -sf.x * sf.z * c;
-    C[3] = inv.y - sf.y * sf.y * c;
-    C[4] = -sf.y * sf.z * c;
-    C[5] = inv.z - sf.z * sf.z * c;
-    return true;
-}
-
-__device__ __forceinline__ d3 endpoint_expansion(const DevScene &S, uint32_t v, q4 rv, d3 dir) {
-    const size_t n = S.g.n;
-    d3 fib = normalize_or_zero(qrotate(rv, mk(S.fiber[v], S.fiber[n + v], S.fiber[2 * n + v])));
-    double ap = S.a_par[v], aq = S.a_perp[v];
-    d3 ad = dir * aq + fib * ((ap - aq) * dot3(fib, dir));
-    return ad * double(S.dtemp[v]);
-}
-
-// Per +axis bond: rotated stiffness k and force k (p_j - p_i - rest)
-// (src/solver.rs:1325-1345, 1595-1619).
-__global__ __launch_bounds__(kBlock) void grid_bonds(DevScene S) {
-    const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
-    const size_t n = S.g.n;
-    if (v >= n) return;
-    const q4 qi = ldq(S.rot, n, v);
-    const d3 pi = ld3(S.pos, n, v);
-    const uint8_t mi = S.mask[v];
+// Per +axis bond: rotated stiffness k and force k (p_j - p_i - rest); zero
+// for absent bonds and bonds whose both endpoints are fully fixed
+// (src/solver.rs:1595-1619).
+__global__ __launch_bounds__(kBlock) void k_bonds(DevArena A) {
+    Tile c;
+    if (!tile_for(A, B_ASSEMBLE, c) || !c.row) return;
+    const size_t R = A.R;
+    const uint32_t v = c.v;
+    const q4 qi = ldq(A.rot, R, v);
+    const d3 pi = ld3(A.pos, R, v);
+    const uint8_t mi = A.mask[v];
+    const Rev ri = ld_rev(A, v);
     for (int a = 0; a < 3; ++a) {
         double k[6] = {0, 0, 0, 0, 0, 0};
         d3 force{0, 0, 0};
-        const uint32_t j = v + S.g.stride(a);
-        if ((S.bonds[v] >> a & 1) && (mi | S.mask[j])) {
-            const q4 qj = ldq(S.rot, n, j);
-            const m3 F = basis(a, slerp_half(qi, qj));
-            double Ca[6], Cb[6];
-            if (compliance(S, v, F, qi, Ca) && compliance(S, j, F, qj, Cb)) {
-                // K = (Ca + Cb)^-1 * 2*AREA/CELL, symmetric by cofactors.
-                double s0 = Ca[0] + Cb[0], s1 = Ca[1] + Cb[1], s2 = Ca[2] + Cb[2];
-                double s3 = Ca[3] + Cb[3], s4 = Ca[4] + Cb[4], s5 = Ca[5] + Cb[5];
-                double c00 = s3 * s5 - s4 * s4, c01 = s2 * s4 - s1 * s5, c02 = s1 * s4 - s2 * s3;
-                double c11 = s0 * s5 - s2 * s2, c12 = s1 * s2 - s0 * s4, c22 = s0 * s3 - s1 * s1;
-                double det = s0 * c00 + s1 * c01 + s2 * c02;
-                const size_t b = size_t(a) * n + v;
-                double scale = (2.0 * kCell * kCell / kCell) / det * double(S.kscale[b]) *
-                               fmax(1.0 - double(S.damage[b]), 0.001);
-                double K[3][3] = {{c00 * scale, c01 * scale, c02 * scale},
-                                  {c01 * scale, c11 * scale, c12 * scale},
-                                  {c02 * scale, c12 * scale, c22 * scale}};
-                // k = F K F^T
-                double Fm[3][3] = {{F.c0.x, F.c1.x, F.c2.x}, {F.c0.y, F.c1.y, F.c2.y}, {F.c0.z, F.c1.z, F.c2.z}};
-                double T[3][3];
-                for (int r = 0; r < 3; ++r)
-                    for (int c = 0; c < 3; ++c) T[r][c] = Fm[r][0] * K[0][c] + Fm[r][1] * K[1][c] + Fm[r][2] * K[2][c];
-                int q = 0;
-                for (int r = 0; r < 3; ++r)
-                    for (int c = r; c < 3; ++c) k[q++] = T[r][0] * Fm[c][0] + T[r][1] * Fm[c][1] + T[r][2] * Fm[c][2];
-            }
-            // rest = F (thermal rest + formation offset + plastic * CELL)
-            d3 dir = normalize_or_zero(F.c0);
-            d3 therm = mul_t(F, dir * kCell + (endpoint_expansion(S, v, qi, dir) + endpoint_expansion(S, j, qj, dir)) * (0.5 * kCell));
-            const size_t o = size_t(a) * 3 * n;
-            d3 off = mk(S.offset[o + v], S.offset[o + n + v], S.offset[o + 2 * n + v]);
-            d3 pl = mk(S.plastic[o + v], S.plastic[o + n + v], S.plastic[o + 2 * n + v]);
-            d3 rest = mul(F, therm + off + pl * kCell);
-            d3 e = ld3(S.pos, n, j) - pi - rest;
+        const uint32_t j = v + stride_of(c.g, a);
+        if ((A.bonds[v] >> a & 1) && (mi | A.mask[j])) {
+            const size_t b = size_t(a) * R + v;
+            const size_t o = size_t(a) * 3 * R;
+            d3 rest;
+            bond_constraint(a, ri, qi, ld_rev(A, j), ldq(A.rot, R, j),
+                            double(A.kscale[b]) * fmax(1.0 - double(A.damage[b]), 0.001),
+                            mk(A.offset[o + v], A.offset[o + R + v], A.offset[o + 2 * R + v]),
+                            mk(A.plastic[o + v], A.plastic[o + R + v], A.plastic[o + 2 * R + v]), k, rest);
+            d3 e = ld3(A.pos, R, j) - pi - rest;
             force = {k[0] * e.x + k[1] * e.y + k[2] * e.z, k[1] * e.x + k[3] * e.y + k[4] * e.z,
                      k[2] * e.x + k[4] * e.y + k[5] * e.z};
         }
         for (int q = 0; q < 6; ++q) {
-            S.k64[(size_t(a) * 6 + q) * n + v] = k[q];
-            S.k32[(size_t(a) * 6 + q) * n + v] = float(k[q]);
+            A.k64[(size_t(a) * 6 + q) * R + v] = k[q];
+            A.k32[(size_t(a) * 6 + q) * R + v] = float(k[q]);
         }
-        S.force[(size_t(a) * 3 + 0) * n + v] = force.x;
-        S.force[(size_t(a) * 3 + 1) * n + v] = force.y;
-        S.force[(size_t(a) * 3 + 2) * n + v] = force.z;
+        A.force[(size_t(a) * 3 + 0) * R + v] = force.x;
+        A.force[(size_t(a) * 3 + 1) * R + v] = force.y;
+        A.force[(size_t(a) * 3 + 2) * R + v] = force.z;
     }
 }
 
-__device__ __forceinline__ void sym_inverse(const double (&a)[6], double (&o)[6]) {
-    double c00 = a[3] * a[5] - a[4] * a[4], c01 = a[2] * a[4] - a[1] * a[5], c02 = a[1] * a[4] - a[2] * a[3];
-    double c11 = a[0] * a[5] - a[2] * a[2], c12 = a[1] * a[2] - a[0] * a[4], c22 = a[0] * a[3] - a[1] * a[1];
-    double inv = 1.0 / (a[0] * c00 + a[1] * c01 + a[2] * c02);
-    o[0] = c00 * inv;
-    o[1] = c01 * inv;
-    o[2] = c02 * inv;
-    o[3] = c11 * inv;
-    o[4] = c12 * inv;
-    o[5] = c22 * inv;
-}
-
-// Per voxel: D = m/h^2 I + sum k, rhs = (pred - p) m/h^2 + load +- forces,
-// projection D <- P D P + I - P, rhs <- P rhs (src/solver.rs:1576-1636).
-__global__ __launch_bounds__(kBlock) void grid_rows(DevScene S, double h) {
-    const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
-    const size_t n = S.g.n;
-    if (v >= n) return;
-    uint32_t x, y, z;
-    S.g.coords(v, x, y, z);
-    const uint32_t c[3] = {x, y, z};
-    const double m = double(S.mass[v]) / (h * h);
+// Per REV: D = m/h^2 I + sum k, rhs = (pred - p) m/h^2 + load +- forces in
+// constraint order (-axis bond, then +axis bond, per axis), projection
+// D <- P D P + I - P, rhs <- P rhs (src/solver.rs:1576-1636).
+__global__ __launch_bounds__(kBlock) void k_rows(DevArena A) {
+    Tile c;
+    if (!tile_for(A, B_ASSEMBLE, c) || !c.row) return;
+    const size_t R = A.R;
+    const uint32_t v = c.v;
+    const double h = A.prm[c.slot].h;
+    uint32_t cc[3];
+    coords(c.g, c.local, cc);
+    const double m = double(A.mass[v]) / (h * h);
     double D[6] = {m, 0, 0, m, 0, m};
-    d3 rhs = (ld3(S.pred, n, v) - ld3(S.pos, n, v)) * m;
-    if (!S.kinematic[v]) rhs = rhs + mk(S.loads[v], S.loads[n + v], S.loads[2 * n + v]);
+    d3 rhs = (ld3(A.pred, R, v) - ld3(A.pos, R, v)) * m;
+    if (!A.kinematic[v]) rhs = rhs + mk(A.loads[v], A.loads[R + v], A.loads[2 * R + v]);
     for (int a = 0; a < 3; ++a) {
-        const uint32_t s = S.g.stride(a);
-        if (c[a] > 0) {  // bond (v - s -> v): this voxel is its +end
-            const uint32_t u = v - s;
-            for (int q = 0; q < 6; ++q) D[q] += S.k64[(size_t(a) * 6 + q) * n + u];
-            rhs = rhs - mk(S.force[(size_t(a) * 3) * n + u], S.force[(size_t(a) * 3 + 1) * n + u], S.force[(size_t(a) * 3 + 2) * n + u]);
+        if (cc[a] > 0) {
+            const uint32_t u = v - stride_of(c.g, a);
+            for (int q = 0; q < 6; ++q) D[q] += A.k64[(size_t(a) * 6 + q) * R + u];
+            rhs = rhs - ld3(A.force + size_t(a) * 3 * R, R, u);
         }
-        for (int q = 0; q < 6; ++q) D[q] += S.k64[(size_t(a) * 6 + q) * n + v];
-        rhs = rhs + mk(S.force[(size_t(a) * 3) * n + v], S.force[(size_t(a) * 3 + 1) * n + v], S.force[(size_t(a) * 3 + 2) * n + v]);
+        for (int q = 0; q < 6; ++q) D[q] += A.k64[(size_t(a) * 6 + q) * R + v];
+        rhs = rhs + ld3(A.force + size_t(a) * 3 * R, R, v);
     }
-    const uint8_t mk_ = S.mask[v];
-    const double f[3] = {double(mk_ & 1), double(mk_ >> 1 & 1), double(mk_ >> 2 & 1)};
-    // P D P + I - P on the symmetric storage
+    const uint8_t fm = A.mask[v];
+    const double f[3] = {double(fm & 1), double(fm >> 1 & 1), double(fm >> 2 & 1)};
     const int idx[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
     double Dp[6];
     for (int r = 0; r < 3; ++r)
-        for (int cc = r; cc < 3; ++cc) Dp[idx[r][cc]] = f[r] * D[idx[r][cc]] * f[cc] + (r == cc ? 1.0 - f[r] : 0.0);
+        for (int q = r; q < 3; ++q) Dp[idx[r][q]] = f[r] * D[idx[r][q]] * f[q] + (r == q ? 1.0 - f[r] : 0.0);
     double Di[6];
     sym_inverse(Dp, Di);
     for (int q = 0; q < 6; ++q) {
-        S.d64[q * n + v] = Dp[q];
-        S.d32[q * n + v] = float(Dp[q]);
-        S.dinv64[q * n + v] = Di[q];
-        S.dinv32[q * n + v] = float(Di[q]);
+        A.d64[q * R + v] = Dp[q];
+        A.d32[q * R + v] = float(Dp[q]);
+        A.dinv64[q * R + v] = Di[q];
+        A.dinv32[q * R + v] = float(Di[q]);
     }
-    st3(S.rhs, n, v, mk(rhs.x * f[0], rhs.y * f[1], rhs.z * f[2]));
+    st3(A.rhs, R, v, mk(rhs.x * f[0], rhs.y * f[1], rhs.z * f[2]));
 }
 
-// ---- PCG kernels on the structured operator --------------------------------
+// ---- PCG kernels -------------------------------------------------------------
 template <class S> __device__ __forceinline__ S fmaS(S a, S b, S c);
 template <> __device__ __forceinline__ float fmaS<float>(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 template <> __device__ __forceinline__ double fmaS<double>(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
 template <class S>
-__device__ __forceinline__ void symv(const S *__restrict__ m, size_t n, uint32_t v, const S (&x)[3], S (&y)[3]) {
-    const S a0 = m[v], a1 = m[n + v], a2 = m[2 * n + v], a3 = m[3 * n + v], a4 = m[4 * n + v], a5 = m[5 * n + v];
+__device__ __forceinline__ void symv(const S *__restrict__ m, size_t R, uint32_t v, const S (&x)[3], S (&y)[3]) {
+    const S a0 = m[v], a1 = m[R + v], a2 = m[2 * R + v], a3 = m[3 * R + v], a4 = m[4 * R + v], a5 = m[5 * R + v];
     y[0] = fmaS<S>(a2, x[2], fmaS<S>(a1, x[1], a0 * x[0]));
     y[1] = fmaS<S>(a4, x[2], fmaS<S>(a3, x[1], a1 * x[0]));
     y[2] = fmaS<S>(a5, x[2], fmaS<S>(a4, x[1], a2 * x[0]));
 }
 template <class S>
-__device__ __forceinline__ void symv_acc(const S *__restrict__ m, size_t n, uint32_t v, const S (&x)[3], S (&y)[3]) {
-    const S a0 = m[v], a1 = m[n + v], a2 = m[2 * n + v], a3 = m[3 * n + v], a4 = m[4 * n + v], a5 = m[5 * n + v];
+__device__ __forceinline__ void symv_acc(const S *__restrict__ m, size_t R, uint32_t v, const S (&x)[3], S (&y)[3]) {
+    const S a0 = m[v], a1 = m[R + v], a2 = m[2 * R + v], a3 = m[3 * R + v], a4 = m[4 * R + v], a5 = m[5 * R + v];
     y[0] = fmaS<S>(a2, x[2], fmaS<S>(a1, x[1], fmaS<S>(a0, x[0], y[0])));
     y[1] = fmaS<S>(a4, x[2], fmaS<S>(a3, x[1], fmaS<S>(a1, x[0], y[1])));
     y[2] = fmaS<S>(a5, x[2], fmaS<S>(a4, x[1], fmaS<S>(a2, x[0], y[2])));
@@ -466,46 +377,61 @@ __device__ __forceinline__ void symv_acc(const S *__restrict__ m, size_t n, uint
 
 template <class S> struct Inner;
 template <> struct Inner<float> {
-    static __device__ __forceinline__ const float *k(const DevScene &s) { return s.k32; }
-    static __device__ __forceinline__ const float *d(const DevScene &s) { return s.d32; }
-    static __device__ __forceinline__ const float *dinv(const DevScene &s) { return s.dinv32; }
+    static constexpr uint32_t kBit = 0;
+    static __device__ __forceinline__ const float *k(const DevArena &a) { return a.k32; }
+    static __device__ __forceinline__ const float *d(const DevArena &a) { return a.d32; }
+    static __device__ __forceinline__ const float *dinv(const DevArena &a) { return a.dinv32; }
+    static __device__ __forceinline__ float *dv(const DevArena &a) { return a.d; }
+    static __device__ __forceinline__ float *r(const DevArena &a) { return a.r; }
+    static __device__ __forceinline__ float *z(const DevArena &a) { return a.z; }
+    static __device__ __forceinline__ float *p(const DevArena &a, int i) { return i ? a.p1 : a.p0; }
+    static __device__ __forceinline__ float *ap(const DevArena &a) { return a.ap; }
 };
 template <> struct Inner<double> {
-    static __device__ __forceinline__ const double *k(const DevScene &s) { return s.k64; }
-    static __device__ __forceinline__ const double *d(const DevScene &s) { return s.d64; }
-    static __device__ __forceinline__ const double *dinv(const DevScene &s) { return s.dinv64; }
+    static constexpr uint32_t kBit = B_F64;
+    static __device__ __forceinline__ const double *k(const DevArena &a) { return a.k64; }
+    static __device__ __forceinline__ const double *d(const DevArena &a) { return a.d64; }
+    static __device__ __forceinline__ const double *dinv(const DevArena &a) { return a.dinv64; }
+    static __device__ __forceinline__ double *dv(const DevArena &a) { return a.dd; }
+    static __device__ __forceinline__ double *r(const DevArena &a) { return a.rd; }
+    static __device__ __forceinline__ double *z(const DevArena &a) { return a.zd; }
+    static __device__ __forceinline__ double *p(const DevArena &a, int i) { return i ? a.p1d : a.p0d; }
+    static __device__ __forceinline__ double *ap(const DevArena &a) { return a.apd; }
 };
 
-// Inner convergence check of the previous update (every iteration:
-// r.r <= tol^2 r0.r0), beta, then y = A p on the structured 7-point
-// 3x3-block operator with p = z + beta p_old formed on load for this row and
-// its neighbours; partials of p.Ap.
 template <class S>
-__global__ __launch_bounds__(kBlock) void grid_spmv(DevScene G, uint32_t per_xcd, uint32_t ntiles, const S *__restrict__ z,
-                                                    const S *__restrict__ p_old, S *__restrict__ p_new, S *__restrict__ ap) {
-    GridState *st = G.st;
+__device__ __forceinline__ bool inner_tile(const DevArena &A, Tile &c) {
+    return tile_for(A, B_INNER, c) && ((c.bits & B_F64) == Inner<S>::kBit);
+}
+
+// Inner convergence check of the previous update (r.r <= tol^2 r0.r0 every
+// iteration), beta, then y = A p on the 7-point 3x3-block operator with
+// p = z + beta p_old formed on load for the row and its neighbours; partials
+// of p.Ap. `odd` selects the p ping-pong buffers.
+template <class S>
+__global__ __launch_bounds__(kBlock) void k_spmv(DevArena A, int odd) {
+    Tile c;
+    if (!inner_tile<S>(A, c)) return;
     __shared__ double lds[2][kWaves];
-    // Scalars first: the state words and the previous update's partials are
-    // loaded together, then summed across the workgroup.
+    SlotState *st = A.st + c.slot;
     const uint32_t done_in = st->done;
     const uint32_t k = st->k_spmv;
-    const uint32_t max_iter = st->max_iter;
     const double rho_s[2] = {st->rho[0], st->rho[1]};
-    const double rr0_s = st->rr0, tol = st->inner_tol;
-    const uint32_t nb = gridDim.x;
+    const double rr0_s = st->rr0;
+    const SlotParams prm = A.prm[c.slot];
     double sums[2];
-    const int in_slots[2] = {kRz, kRr};
-    sum_partials<2>(sums, G.partials, in_slots, nb, nb, lds);
+    const int kinds[2] = {kRz, kRr};
+    scene_sum<2>(sums, A, c.g, kinds, lds);
     if (done_in) return;
     const double rho = sums[0], rr = sums[1];
-    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    const bool lead = c.t == c.g.tile0 && threadIdx.x == 0;
     const double rr0 = k == 0 ? rr : rr0_s;
     uint32_t done = kRun;
     if (!isfinite(rr))
         done = kNonFin;
-    else if (!(rr > tol * tol * rr0))
+    else if (!(rr > prm.inner_tol * prm.inner_tol * rr0))
         done = kConv;
-    else if (k >= max_iter)
+    else if (k >= prm.max_inner)
         done = kCap;
     if (lead) {
         st->rr_last = rr;
@@ -524,44 +450,43 @@ __global__ __launch_bounds__(kBlock) void grid_spmv(DevScene G, uint32_t per_xcd
         st->rho[k & 1] = rho;
         st->k_update = k;
     }
-    const size_t n = G.g.n;
-    const uint32_t tile = xcd_tile(blockIdx.x, per_xcd);
-    const uint32_t v = tile * kBlock + threadIdx.x;
+    const size_t R = A.R;
+    const S *__restrict__ z = Inner<S>::z(A);
+    const S *__restrict__ p_old = Inner<S>::p(A, odd);
+    S *__restrict__ p_new = Inner<S>::p(A, !odd);
     double part[1] = {0.0};
-    if (tile < ntiles && v < n) {
-        const S *__restrict__ K = Inner<S>::k(G);
-        uint32_t x, y, zc;
-        G.g.coords(v, x, y, zc);
-        const uint32_t c[3] = {x, y, zc};
-        const uint32_t ext[3] = {G.g.nx, G.g.ny, G.g.nz};
-        // Load everything first (zeros where a neighbour is absent) so the
-        // loads of all six neighbours are in flight together; compute after.
+    if (c.row) {
+        const S *__restrict__ K = Inner<S>::k(A);
+        const uint32_t v = c.v;
+        uint32_t cc[3];
+        coords(c.g, c.local, cc);
+        // All loads first (zeros where a neighbour is absent), compute after.
         S zz[7][3], pp[7][3], Dv[6], Kp[3][6], Km[3][6];
-        const uint8_t m = G.mask[v];
+        const uint8_t m = A.mask[v];
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
-            zz[0][q] = z[q * n + v];
-            pp[0][q] = p_old[q * n + v];
+            zz[0][q] = z[q * R + v];
+            pp[0][q] = p_old[q * R + v];
         }
 #pragma unroll
-        for (int q = 0; q < 6; ++q) Dv[q] = Inner<S>::d(G)[q * n + v];
+        for (int q = 0; q < 6; ++q) Dv[q] = Inner<S>::d(A)[q * R + v];
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            const uint32_t s = G.g.stride(a);
-            const S *Ka = K + size_t(a) * 6 * n;
-            const bool lo = c[a] > 0, hi = c[a] + 1 < ext[a];
+            const uint32_t s = stride_of(c.g, a);
+            const S *Ka = K + size_t(a) * 6 * R;
+            const bool lo = cc[a] > 0, hi = cc[a] + 1 < extent_of(c.g, a);
             const uint32_t um = lo ? v - s : v, up = hi ? v + s : v;
 #pragma unroll
             for (int q = 0; q < 3; ++q) {
-                zz[1 + 2 * a][q] = lo ? z[q * n + um] : S(0);
-                pp[1 + 2 * a][q] = lo ? p_old[q * n + um] : S(0);
-                zz[2 + 2 * a][q] = hi ? z[q * n + up] : S(0);
-                pp[2 + 2 * a][q] = hi ? p_old[q * n + up] : S(0);
+                zz[1 + 2 * a][q] = lo ? z[q * R + um] : S(0);
+                pp[1 + 2 * a][q] = lo ? p_old[q * R + um] : S(0);
+                zz[2 + 2 * a][q] = hi ? z[q * R + up] : S(0);
+                pp[2 + 2 * a][q] = hi ? p_old[q * R + up] : S(0);
             }
 #pragma unroll
             for (int q = 0; q < 6; ++q) {
-                Km[a][q] = lo ? Ka[q * n + um] : S(0);
-                Kp[a][q] = Ka[q * n + v];  // zero when the +axis bond is absent
+                Km[a][q] = lo ? Ka[q * R + um] : S(0);
+                Kp[a][q] = Ka[q * R + v];  // zero when the +axis bond is absent
             }
         }
         S p[7][3];
@@ -587,489 +512,924 @@ __global__ __launch_bounds__(kBlock) void grid_spmv(DevScene G, uint32_t per_xcd
         if (m & 1) y0 -= a0;
         if (m & 2) y1 -= a1;
         if (m & 4) y2 -= a2;
+        S *__restrict__ ap = Inner<S>::ap(A);
         p_new[v] = p[0][0];
-        p_new[n + v] = p[0][1];
-        p_new[2 * n + v] = p[0][2];
+        p_new[R + v] = p[0][1];
+        p_new[2 * R + v] = p[0][2];
         ap[v] = y0;
-        ap[n + v] = y1;
-        ap[2 * n + v] = y2;
+        ap[R + v] = y1;
+        ap[2 * R + v] = y2;
         part[0] = double(p[0][0]) * double(y0);
         part[0] = __builtin_fma(double(p[0][1]), double(y1), part[0]);
         part[0] = __builtin_fma(double(p[0][2]), double(y2), part[0]);
     }
-    const int out_slots[1] = {kPap};
-    store_partials<1>(part, G.partials, out_slots, nb, lds);
+    const int out[1] = {kPap};
+    tile_store<1>(part, A, c.t, out, lds);
 }
 
 // alpha = rho / p.Ap; d += alpha p (d = alpha p on the first iteration);
 // r -= alpha Ap; z = Dinv r; partials of r.z and r.r.
 template <class S>
-__global__ __launch_bounds__(kBlock) void grid_update(DevScene G, uint32_t per_xcd, uint32_t ntiles, S *__restrict__ d,
-                                                      S *__restrict__ r, S *__restrict__ z, const S *__restrict__ p,
-                                                      const S *__restrict__ ap) {
-    GridState *st = G.st;
+__global__ __launch_bounds__(kBlock) void k_update(DevArena A, int odd) {
+    Tile c;
+    if (!inner_tile<S>(A, c)) return;
     __shared__ double lds[2][kWaves];
+    SlotState *st = A.st + c.slot;
     const uint32_t done_in = st->done;
     const uint32_t k = st->k_update;
     const double rho_s[2] = {st->rho[0], st->rho[1]};
-    const uint32_t nb = gridDim.x;
     double pap[1];
-    const int in_slots[1] = {kPap};
-    sum_partials<1>(pap, G.partials, in_slots, nb, nb, lds);
+    const int kinds[1] = {kPap};
+    scene_sum<1>(pap, A, c.g, kinds, lds);
     if (done_in) return;
     const S alpha = pap[0] > 1e-300 ? S(rho_s[k & 1] / pap[0]) : S(0);
     const bool first = k == 0;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (c.t == c.g.tile0 && threadIdx.x == 0) {
         st->k_spmv = k + 1;
         st->iters = k + 1;
     }
-    const size_t n = G.g.n;
-    const uint32_t tile = xcd_tile(blockIdx.x, per_xcd);
-    const uint32_t v = tile * kBlock + threadIdx.x;
+    const size_t R = A.R;
     double part[2] = {0.0, 0.0};
-    if (tile < ntiles && v < n) {
+    if (c.row) {
+        const uint32_t v = c.v;
+        const S *__restrict__ p = Inner<S>::p(A, !odd);
+        const S *__restrict__ ap = Inner<S>::ap(A);
+        S *__restrict__ d = Inner<S>::dv(A);
+        S *__restrict__ r = Inner<S>::r(A);
+        S *__restrict__ z = Inner<S>::z(A);
+        const S *__restrict__ dinv = Inner<S>::dinv(A);
         S pv[3], av[3], rv[3], dv[3], Di[6];
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
-            pv[q] = p[q * n + v];
-            av[q] = ap[q * n + v];
-            rv[q] = r[q * n + v];
-            dv[q] = first ? S(0) : d[q * n + v];
+            pv[q] = p[q * R + v];
+            av[q] = ap[q * R + v];
+            rv[q] = r[q * R + v];
+            dv[q] = first ? S(0) : d[q * R + v];
         }
 #pragma unroll
-        for (int q = 0; q < 6; ++q) Di[q] = Inner<S>::dinv(G)[q * n + v];
+        for (int q = 0; q < 6; ++q) Di[q] = dinv[q * R + v];
         S ri[3], zi[3];
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
-            d[q * n + v] = fmaS<S>(alpha, pv[q], dv[q]);
+            d[q * R + v] = fmaS<S>(alpha, pv[q], dv[q]);
             ri[q] = fmaS<S>(-alpha, av[q], rv[q]);
-            r[q * n + v] = ri[q];
+            r[q * R + v] = ri[q];
         }
         zi[0] = fmaS<S>(Di[2], ri[2], fmaS<S>(Di[1], ri[1], Di[0] * ri[0]));
         zi[1] = fmaS<S>(Di[4], ri[2], fmaS<S>(Di[3], ri[1], Di[1] * ri[0]));
         zi[2] = fmaS<S>(Di[5], ri[2], fmaS<S>(Di[4], ri[1], Di[2] * ri[0]));
 #pragma unroll
-        for (int q = 0; q < 3; ++q) z[q * n + v] = zi[q];
+        for (int q = 0; q < 3; ++q) z[q * R + v] = zi[q];
         part[0] = __builtin_fma(double(ri[2]), double(zi[2]), __builtin_fma(double(ri[1]), double(zi[1]), double(ri[0]) * double(zi[0])));
         part[1] = __builtin_fma(double(ri[2]), double(ri[2]), __builtin_fma(double(ri[1]), double(ri[1]), double(ri[0]) * double(ri[0])));
     }
-    const int out_slots[2] = {kRz, kRr};
-    store_partials<2>(part, G.partials, out_slots, nb, lds);
+    const int out[2] = {kRz, kRr};
+    tile_store<2>(part, A, c.t, out, lds);
 }
 
-// Outer FP64 refinement step: x_new = x_old + d (own and neighbour rows),
-// r64 = b - A64 x_new; the inner system starts from r = r64 in the inner
-// precision, z = Dinv r, p_old = 0. Partials of ||r64||^2, ||b||^2, r.z, r.r.
+// Outer FP64 refinement step. B_RESID0 starts a solve from zero (or the warm
+// start copy); B_RESID first adds the inner correction: x_new = x_old + d on
+// own and neighbour rows. r64 = b - A64 x_new; the inner system starts from
+// r = r64 in the inner precision, z = Dinv r, p_old = 0. Partials of
+// ||r64||^2, ||b||^2, r.z, r.r. x_new goes to the other ping-pong buffer.
 template <class S>
-__global__ __launch_bounds__(kBlock) void grid_residual(DevScene G, uint32_t per_xcd, uint32_t ntiles, const double *__restrict__ x_old,
-                                                        double *__restrict__ x_new, const S *__restrict__ d, int use_d,
-                                                        S *__restrict__ r, S *__restrict__ z, S *__restrict__ p_old) {
-    GridState *st = G.st;
+__global__ __launch_bounds__(kBlock) void k_residual(DevArena A) {
+    Tile c;
+    if (!tile_for(A, B_RESID0 | B_RESID, c) || ((c.bits & B_F64) != Inner<S>::kBit)) return;
     __shared__ double lds[4][kWaves];
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        st->k_spmv = 0;
-        st->done = kRun;
-    }
-    const size_t n = G.g.n;
-    const uint32_t tile = xcd_tile(blockIdx.x, per_xcd);
-    const uint32_t v = tile * kBlock + threadIdx.x;
+    const size_t R = A.R;
+    const uint32_t xpar = A.st[c.slot].xpar;
+    const bool first = c.bits & B_RESID0;
+    const bool warm = first && (c.bits & B_WARM_USE);
+    const double *__restrict__ x_old = warm ? A.xw : (xpar ? A.xb : A.xa);
+    double *__restrict__ x_new = xpar ? A.xa : A.xb;
+    const S *__restrict__ d = Inner<S>::dv(A);
     double part[4] = {0.0, 0.0, 0.0, 0.0};
-    if (tile < ntiles && v < n) {
-        uint32_t x, y, zc;
-        G.g.coords(v, x, y, zc);
-        const uint32_t c[3] = {x, y, zc};
-        const uint32_t ext[3] = {G.g.nx, G.g.ny, G.g.nz};
+    if (c.row) {
+        const uint32_t v = c.v;
+        uint32_t cc[3];
+        coords(c.g, c.local, cc);
         auto xval = [&](uint32_t u, double (&o)[3]) {
 #pragma unroll
-            for (int q = 0; q < 3; ++q) o[q] = x_old[q * n + u] + (use_d ? double(d[q * n + u]) : 0.0);
+            for (int q = 0; q < 3; ++q)
+                o[q] = first ? (warm ? x_old[q * R + u] : 0.0) : x_old[q * R + u] + double(d[q * R + u]);
         };
         double xv[3];
         xval(v, xv);
         double yv[3];
-        symv<double>(G.d64, n, v, xv, yv);
+        symv<double>(A.d64, R, v, xv, yv);
         double acc[3] = {0, 0, 0};
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            const uint32_t s = G.g.stride(a);
-            if (c[a] > 0) {
+            const uint32_t s = stride_of(c.g, a);
+            if (cc[a] > 0) {
                 double xu[3];
                 xval(v - s, xu);
-                symv_acc<double>(G.k64 + size_t(a) * 6 * n, n, v - s, xu, acc);
+                symv_acc<double>(A.k64 + size_t(a) * 6 * R, R, v - s, xu, acc);
             }
-            if (c[a] + 1 < ext[a]) {
+            if (cc[a] + 1 < extent_of(c.g, a)) {
                 double xu[3];
                 xval(v + s, xu);
-                symv_acc<double>(G.k64 + size_t(a) * 6 * n, n, v, xu, acc);
+                symv_acc<double>(A.k64 + size_t(a) * 6 * R, R, v, xu, acc);
             }
         }
-        const uint8_t m = G.mask[v];
+        const uint8_t m = A.mask[v];
         const double f[3] = {double(m & 1), double(m >> 1 & 1), double(m >> 2 & 1)};
         S rs[3], zs[3];
+        S *__restrict__ r = Inner<S>::r(A);
+        S *__restrict__ z = Inner<S>::z(A);
+        S *__restrict__ p0 = Inner<S>::p(A, 0);
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
-            const double b = G.rhs[q * n + v];
+            const double b = A.rhs[q * R + v];
             const double r64 = f[q] * (b - (yv[q] - acc[q]));
-            x_new[q * n + v] = xv[q];
+            x_new[q * R + v] = xv[q];
             rs[q] = S(r64);
-            r[q * n + v] = rs[q];
-            p_old[q * n + v] = S(0);
+            r[q * R + v] = rs[q];
+            p0[q * R + v] = S(0);
             part[0] = __builtin_fma(r64, r64, part[0]);
             part[1] = __builtin_fma(b, b, part[1]);
         }
-        symv<S>(Inner<S>::dinv(G), n, v, rs, zs);
+        symv<S>(Inner<S>::dinv(A), R, v, rs, zs);
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
-            z[q * n + v] = zs[q];
+            z[q * R + v] = zs[q];
             part[2] = __builtin_fma(double(rs[q]), double(zs[q]), part[2]);
             part[3] = __builtin_fma(double(rs[q]), double(rs[q]), part[3]);
         }
     }
-    const int out_slots[4] = {kRr64, kBb, kRz, kRr};
-    store_partials<4>(part, G.partials, out_slots, gridDim.x, lds);
+    const int out[4] = {kRr64, kBb, kRz, kRr};
+    tile_store<4>(part, A, c.t, out, lds);
 }
 
-// positions += x on the active rows (src/solver.rs:1921-1923).
-__global__ __launch_bounds__(kBlock) void grid_apply(DevScene S, const double *__restrict__ x) {
-    const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
-    const size_t n = S.g.n;
-    if (v >= n || S.mask[v] == 0) return;
-    for (int q = 0; q < 3; ++q) S.pos[q * n + v] += x[q * n + v];
+// One workgroup per scene after k_residual: the outer relative residual, the
+// refinement decision (converged, or out of sweeps), the ping-pong flip and
+// the reset of the inner solve.
+__global__ __launch_bounds__(kBlock) void k_reduce_residual(DevArena A) {
+    const uint32_t slot = blockIdx.x;
+    const uint32_t bits = A.bits[slot];
+    if (!(bits & (B_RESID0 | B_RESID))) return;
+    __shared__ double lds[2][kWaves];
+    const SlotGeo g = A.geo[slot];
+    if (g.n == 0) return;
+    double s[2];
+    const int kinds[2] = {kRr64, kBb};
+    scene_sum<2>(s, A, g, kinds, lds);
+    if (threadIdx.x != 0) return;
+    SlotState *st = A.st + slot;
+    const SlotParams p = A.prm[slot];
+    const bool first = bits & B_RESID0;
+    const uint32_t sweeps = (first ? 0u : st->sweeps) + 1;
+    const double rel = s[1] > 0.0 ? sqrt(s[0] / s[1]) : 0.0;
+    st->inner_total = first ? 0u : st->inner_total + st->iters;
+    st->rr64 = s[0];
+    st->bb = s[1];
+    st->rel = rel;
+    st->sweeps = sweeps;
+    st->xpar ^= 1u;
+    st->nonfinite = !isfinite(s[0]);
+    const bool outer_done = !isfinite(s[0]) || rel <= p.tol || sweeps > p.max_outer;
+    st->outer_done = outer_done;
+    st->k_spmv = 0;
+    st->k_update = 0;
+    st->iters = 0;
+    st->done = outer_done ? kConv : kRun;
+}
+
+// positions += x on the free rows (src/solver.rs:1821-1823, 1921-1923); with
+// B_WARM_SAVE the correction is also kept as the next solve's warm start.
+__global__ __launch_bounds__(kBlock) void k_apply(DevArena A) {
+    Tile c;
+    if (!tile_for(A, B_APPLY | B_WARM_SAVE, c) || !c.row) return;
+    const size_t R = A.R;
+    const uint32_t v = c.v;
+    const double *x = A.st[c.slot].xpar ? A.xb : A.xa;
+    const bool apply = (c.bits & B_APPLY) && A.mask[v];
+    const bool save = c.bits & B_WARM_SAVE;
+    for (int q = 0; q < 3; ++q) {
+        const double xv = x[q * R + v];
+        if (apply) A.pos[q * R + v] += xv;
+        if (save) A.xw[q * R + v] = xv;
+    }
+}
+
+// One workgroup per scene: largest rotation change of the fit after apply.
+__global__ __launch_bounds__(kBlock) void k_reduce_angle(DevArena A) {
+    const uint32_t slot = blockIdx.x;
+    if (!(A.bits[slot] & B_FITB)) return;
+    const SlotGeo g = A.geo[slot];
+    if (g.n == 0) return;
+    double m = 0.0;
+    for (uint32_t b = threadIdx.x; b < g.ntiles; b += kBlock)
+        m = fmax(m, A.partials[size_t(kAngle) * A.cap_tiles + g.tile0 + b]);
+    __shared__ double lds[kWaves];
+    m = wave_max(m);
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double best = lds[0];
+        for (int w = 1; w < kWaves; ++w) best = fmax(best, lds[w]);
+        A.st[slot].angle = best;
+    }
 }
 
 // Velocities after a substep (src/solver.rs:668-683), no contact.
-__global__ __launch_bounds__(kBlock) void grid_finish(DevScene S, double h) {
-    const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
-    const size_t n = S.g.n;
-    if (v >= n || S.kinematic[v]) return;
-    const uint8_t tm = S.tmask[v];
+__global__ __launch_bounds__(kBlock) void k_finish(DevArena A) {
+    Tile c;
+    if (!tile_for(A, B_FINISH, c) || !c.row) return;
+    const size_t R = A.R;
+    const uint32_t v = c.v;
+    if (A.kinematic[v]) return;
+    const double h = A.prm[c.slot].h;
+    const uint8_t tm = A.tmask[v];
     for (int q = 0; q < 3; ++q) {
-        const double p = S.pos[q * n + v];
+        const double p = A.pos[q * R + v];
         if (tm >> q & 1)
-            S.vel[q * n + v] = (p - S.prev[q * n + v]) / h;
+            A.vel[q * R + v] = (p - A.prev[q * R + v]) / h;
         else
-            S.vel[q * n + v] = S.drift[q * n + v] + (p - S.pred[q * n + v]) / h;
+            A.vel[q * R + v] = A.drift[q * R + v] + (p - A.pred[q * R + v]) / h;
     }
 }
 
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
-int32_t check_scene(vx_scene *sc) {
-    if (!sc) return fail(VX_ERR_INVALID_ARGUMENT, "null scene");
-    return VX_OK;
-}
+inline uint32_t tiles_for(uint64_t rows) { return uint32_t((rows + kBlock - 1) / kBlock); }
 
-inline uint32_t blocks_for(uint32_t n) { return (n + kBlock - 1) / kBlock; }
+// HBM bytes per arena row: the planes carved in create_batch.
+constexpr size_t kRowBytes = 24 * 5 + 32 + 24 * 3 + 24 + 4 * 9 + 12 + 12 + 4 + 12 + 12 + 36 + 36 + 144 + 72 + 48 + 48 + 24 +
+                             72 + 24 + 24 + 6 * 12 + 6 * 24;
 
-int32_t read_state(vx_scene *sc, GridState *out) {
-    Context &ctx = context();
-    VX_HIP(hipMemcpyAsync(&sc->pinned[2], sc->s.st, sizeof(GridState), hipMemcpyDeviceToHost, ctx.stream));
-    VX_HIP(hipStreamSynchronize(ctx.stream));
-    *out = sc->pinned[2];
-    return VX_OK;
-}
+enum Phase : uint32_t { P_IDLE = 0, P_START, P_SOLVE, P_APPLIED, P_FINISHED, P_DONE, P_FAILED };
 
-template <class S>
-int32_t capture_inner(vx_scene *sc, hipGraphExec_t *exec) {
-    Context &ctx = context();
-    DevScene &G = sc->s;
-    S *d, *r, *z, *p0, *p1, *ap;
-    if (sizeof(S) == 4) {
-        d = (S *)G.d, r = (S *)G.r, z = (S *)G.z, p0 = (S *)G.p0, p1 = (S *)G.p1, ap = (S *)G.ap;
-    } else {
-        d = (S *)G.dd, r = (S *)G.rd, z = (S *)G.zd, p0 = (S *)G.p0d, p1 = (S *)G.p1d, ap = (S *)G.apd;
-    }
-    hipGraph_t graph;
-    VX_HIP(hipStreamBeginCapture(ctx.stream, hipStreamCaptureModeThreadLocal));
-    for (uint32_t t = 0; t < kInnerChunk; ++t) {
-        S *pold = (t % 2 == 0) ? p0 : p1;
-        S *pnew = (t % 2 == 0) ? p1 : p0;
-        hipLaunchKernelGGL(grid_spmv<S>, dim3(sc->grid), dim3(kBlock), 0, ctx.stream, G, sc->per_xcd, sc->ntiles, z, pold, pnew, ap);
-        hipLaunchKernelGGL(grid_update<S>, dim3(sc->grid), dim3(kBlock), 0, ctx.stream, G, sc->per_xcd, sc->ntiles, d, r, z, pnew, ap);
-    }
-    VX_HIP(hipStreamEndCapture(ctx.stream, &graph));
-    hipError_t e = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(graph);
-    if (e != hipSuccess) return fail(VX_ERR_DEVICE, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
-    return VX_OK;
-}
+struct SlotHost {
+    bool used = false;
+    SlotGeo geo{};
+    vx_step_params params{};
+    uint32_t steps = 0, steps_done = 0, substep = 0, pass = 0;
+    Phase phase = P_IDLE;
+    bool warm_ok = false;
+    bool finish_next = false;
+    uint32_t status = VX_OK;
+    uint32_t solves = 0;
+    uint64_t inner = 0, sweeps = 0;
+    double last_rel = 0.0, last_angle = 0.0;
+    bool manual = false;  // driven by the vx_scene_* stage calls
+};
 
-// Inner PCG: replay 16-iteration graphs, reading the status one graph behind.
-int32_t run_inner(vx_scene *sc, hipGraphExec_t exec, uint32_t max_inner) {
-    Context &ctx = context();
-    uint32_t launched = 0, chunk = 0;
-    // The check after the last update runs in the following grid_spmv, so
-    // replay until one spmv past the cap has been issued.
-    while (launched <= max_inner) {
-        VX_HIP(hipGraphLaunch(exec, ctx.stream));
-        VX_HIP(hipMemcpyAsync(&sc->pinned[chunk % 2], sc->s.st, sizeof(GridState), hipMemcpyDeviceToHost, ctx.stream));
-        VX_HIP(hipEventRecord(sc->ev[chunk % 2], ctx.stream));
-        launched += kInnerChunk;
-        if (chunk >= 1) {
-            VX_HIP(hipEventSynchronize(sc->ev[(chunk - 1) % 2]));
-            if (sc->pinned[(chunk - 1) % 2].done) break;
-        }
-        chunk += 1;
-    }
-    return VX_OK;
-}
+}  // namespace
+}  // namespace vx
 
-template <class S>
-int32_t solve_t(vx_scene *sc, const vx_solve_params *prm, vx_solve_report *rep) {
-    Context &ctx = context();
-    DevScene &G = sc->s;
-    const bool f32 = sizeof(S) == 4;
-    hipGraphExec_t &exec = f32 ? sc->inner32 : sc->inner64;
-    if (!exec) {
-        int32_t c = capture_inner<S>(sc, &exec);
-        if (c != VX_OK) return c;
-    }
-    S *d = f32 ? (S *)G.d : (S *)G.dd;
-    S *r = f32 ? (S *)G.r : (S *)G.rd;
-    S *z = f32 ? (S *)G.z : (S *)G.zd;
-    S *p0 = f32 ? (S *)G.p0 : (S *)G.p0d;
-    // Plain FP64: one sweep solved straight to the outer tolerance.
-    const double inner_tol = f32 ? prm->inner_tolerance : prm->tolerance * 0.5;
-    const uint32_t max_outer = f32 ? prm->max_outer : 2;
-    GridState init;
-    std::memset(&init, 0, sizeof init);
-    init.inner_tol = inner_tol;
-    init.max_iter = prm->max_inner;
-    sc->pinned[2] = init;
-    VX_HIP(hipMemcpyAsync(G.st, &sc->pinned[2], sizeof init, hipMemcpyHostToDevice, ctx.stream));
-    const bool warm = prm->warm_start && sc->has_solution;
-    if (!warm) VX_HIP(hipMemsetAsync(sc->x_in_a ? G.xa : G.xb, 0, sizeof(double) * 3 * G.g.n, ctx.stream));
-    uint32_t inner_total = 0, sweeps = 0;
-    double rel = INFINITY;
-    int32_t status = VX_OK;
-    for (uint32_t s = 0; s <= max_outer; ++s) {
-        double *xo = sc->x_in_a ? G.xa : G.xb;
-        double *xn = sc->x_in_a ? G.xb : G.xa;
-        hipLaunchKernelGGL(grid_residual<S>, dim3(sc->grid), dim3(kBlock), 0, ctx.stream, G, sc->per_xcd, sc->ntiles, xo, xn,
-                           (const S *)d, s > 0 ? 1 : 0, r, z, p0);
-        VX_HIP(hipGetLastError());
-        sc->x_in_a = !sc->x_in_a;
-        // ||r64||^2 and ||b||^2 from the residual kernel's partials, summed
-        // on the host in a fixed order.
-        VX_HIP(hipMemcpyAsync(sc->host_partials, G.partials + size_t(kRr64) * sc->grid, 2 * sizeof(double) * sc->grid,
-                              hipMemcpyDeviceToHost, ctx.stream));
-        VX_HIP(hipStreamSynchronize(ctx.stream));
-        double rr64 = 0.0, bb = 0.0;
-        for (uint32_t b = 0; b < sc->grid; ++b) {
-            rr64 += sc->host_partials[b];
-            bb += sc->host_partials[sc->grid + b];
-        }
-        sweeps += 1;
-        rel = bb > 0.0 ? std::sqrt(rr64 / bb) : 0.0;
-        if (rep) rep->rhs_norm = std::sqrt(bb);
-        if (!std::isfinite(rr64)) {
-            status = fail(VX_ERR_NONFINITE_RESIDUAL, "HIP PCG produced a non-finite residual norm");
-            break;
-        }
-        int32_t c = VX_OK;
-        if (rel <= prm->tolerance) break;
-        if (s == max_outer) break;
-        c = run_inner(sc, exec, prm->max_inner);
-        if (c != VX_OK) return c;
-        GridState after;
-        c = read_state(sc, &after);
-        if (c != VX_OK) return c;
-        inner_total += after.iters;
-        if (after.done == kNonFin) {
-            status = fail(VX_ERR_NONFINITE_RESIDUAL, "HIP PCG produced a non-finite residual norm");
-            break;
-        }
-    }
-    sc->has_solution = true;
-    if (rep) {
-        rep->inner_iterations = inner_total;
-        rep->outer_sweeps = sweeps;
-        rep->active_rows = sc->active;
-        rep->relative_residual = rel;
-        rep->status = uint32_t(status);
-    }
-    if (status == VX_OK && !(rel <= prm->tolerance))
-        status = fail(VX_ERR_NOT_CONVERGED, "HIP PCG did not converge: inner iterations=%u, sweeps=%u, relative residual=%.3e, tolerance=%.3e",
-                      inner_total, sweeps, rel, prm->tolerance);
-    if (rep) rep->status = uint32_t(status);
-    return status;
-}
+struct vx_batch {
+    vx::DevArena A{};
+    uint32_t cap_tiles = 0, chunk = 1, max_slots = 0;
+    void *alloc = nullptr;
+    size_t alloc_bytes = 0;
+    std::vector<vx::SlotHost> slots;
+    std::vector<uint32_t> owner;  // host copy of tile_slot
+    uint32_t *d_tile_slot = nullptr, *d_bits = nullptr;
+    vx::SlotGeo *d_geo = nullptr;
+    vx::SlotParams *d_prm = nullptr;
+    vx::SlotState *d_st = nullptr;
+    uint32_t *h_bits = nullptr;          // pinned
+    vx::SlotState *h_st = nullptr;       // pinned
+    hipGraphExec_t g_inner[2] = {nullptr, nullptr};  // f32, f64
+    bool owner_dirty = true;
+};
+
+struct vx_scene {
+    vx_batch *batch = nullptr;
+    uint32_t slot = 0;
+    uint32_t active = 0;
+};
+
+namespace vx {
+namespace {
 
 struct Lock {
     std::lock_guard<std::recursive_mutex> g;
     Lock() : g(context().mutex) {}
 };
 
+int32_t upload_tables(vx_batch *b) {
+    Context &ctx = context();
+    if (b->owner_dirty) {
+        VX_HIP(hipMemcpyAsync(b->d_tile_slot, b->owner.data(), sizeof(uint32_t) * b->cap_tiles, hipMemcpyHostToDevice, ctx.stream));
+        VX_HIP(hipStreamSynchronize(ctx.stream));
+        b->owner_dirty = false;
+    }
+    return VX_OK;
+}
+
+int32_t capture_inner(vx_batch *b, int f64) {
+    Context &ctx = context();
+    hipGraph_t graph;
+    VX_HIP(hipStreamBeginCapture(ctx.stream, hipStreamCaptureModeThreadLocal));
+    for (uint32_t t = 0; t < kInnerChunk; ++t) {
+        if (f64) {
+            hipLaunchKernelGGL(k_spmv<double>, dim3(b->cap_tiles), dim3(kBlock), 0, ctx.stream, b->A, int(t & 1));
+            hipLaunchKernelGGL(k_update<double>, dim3(b->cap_tiles), dim3(kBlock), 0, ctx.stream, b->A, int(t & 1));
+        } else {
+            hipLaunchKernelGGL(k_spmv<float>, dim3(b->cap_tiles), dim3(kBlock), 0, ctx.stream, b->A, int(t & 1));
+            hipLaunchKernelGGL(k_update<float>, dim3(b->cap_tiles), dim3(kBlock), 0, ctx.stream, b->A, int(t & 1));
+        }
+    }
+    VX_HIP(hipStreamEndCapture(ctx.stream, &graph));
+    hipError_t e = hipGraphInstantiate(&b->g_inner[f64], graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail(VX_ERR_DEVICE, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+    return VX_OK;
+}
+
+// Launch one tick for the bits in h_bits: the pre-inner kernels, one inner
+// chunk per precision in use, the post-inner kernels; then read back all
+// scene states.
+int32_t launch_tick(vx_batch *b) {
+    Context &ctx = context();
+    hipStream_t s = ctx.stream;
+    uint32_t any = 0;
+    bool f32 = false, f64 = false;
+    for (uint32_t i = 0; i < b->max_slots; ++i) {
+        if (!b->slots[i].used) {
+            b->h_bits[i] = 0;
+            continue;
+        }
+        any |= b->h_bits[i];
+        if (b->h_bits[i] & B_INNER) (b->h_bits[i] & B_F64 ? f64 : f32) = true;
+    }
+    if (int32_t c = upload_tables(b)) return c;
+    VX_HIP(hipMemcpyAsync(b->d_bits, b->h_bits, sizeof(uint32_t) * b->max_slots, hipMemcpyHostToDevice, s));
+    const dim3 grid(b->cap_tiles), blk(kBlock), sg(b->max_slots);
+    if (any & B_PREDICT) hipLaunchKernelGGL(k_predict, grid, blk, 0, s, b->A);
+    if (any & B_FIT) hipLaunchKernelGGL(k_fit, grid, blk, 0, s, b->A, uint32_t(B_FIT), 0);
+    if (any & B_ASSEMBLE) {
+        hipLaunchKernelGGL(k_bonds, grid, blk, 0, s, b->A);
+        hipLaunchKernelGGL(k_rows, grid, blk, 0, s, b->A);
+    }
+    if (any & (B_RESID0 | B_RESID)) {
+        hipLaunchKernelGGL(k_residual<float>, grid, blk, 0, s, b->A);
+        if (any & B_F64) hipLaunchKernelGGL(k_residual<double>, grid, blk, 0, s, b->A);
+        hipLaunchKernelGGL(k_reduce_residual, sg, blk, 0, s, b->A);
+    }
+    VX_HIP(hipGetLastError());
+    for (int prec = 0; prec < 2; ++prec) {
+        if (!(prec ? f64 : f32)) continue;
+        if (!b->g_inner[prec])
+            if (int32_t c = capture_inner(b, prec)) return c;
+        VX_HIP(hipGraphLaunch(b->g_inner[prec], s));
+    }
+    if (any & (B_APPLY | B_WARM_SAVE)) hipLaunchKernelGGL(k_apply, grid, blk, 0, s, b->A);
+    if (any & B_FITB) {
+        hipLaunchKernelGGL(k_fit, grid, blk, 0, s, b->A, uint32_t(B_FITB), 1);
+        hipLaunchKernelGGL(k_reduce_angle, sg, blk, 0, s, b->A);
+    }
+    if (any & B_FINISH) hipLaunchKernelGGL(k_finish, grid, blk, 0, s, b->A);
+    VX_HIP(hipGetLastError());
+    VX_HIP(hipMemcpyAsync(b->h_st, b->d_st, sizeof(SlotState) * b->max_slots, hipMemcpyDeviceToHost, s));
+    VX_HIP(hipStreamSynchronize(s));
+    return VX_OK;
+}
+
+uint32_t prec_bits(const vx_step_params &p) { return p.solve.mode == VX_SOLVE_F64 ? uint32_t(B_F64) : 0u; }
+
+// Bits of an automatically driven scene for its next tick.
+uint32_t next_bits(const SlotHost &h, const SlotState &st) {
+    const uint32_t prec = prec_bits(h.params);
+    switch (h.phase) {
+    case P_START:
+    case P_FINISHED: {
+        uint32_t b = B_PREDICT | B_FIT | B_ASSEMBLE | B_RESID0 | B_INNER | prec;
+        if (h.params.solve.warm_start && h.warm_ok) b |= B_WARM_USE;
+        return b;
+    }
+    case P_SOLVE:
+        if (st.outer_done) {
+            uint32_t b = B_APPLY | prec;
+            if (h.pass == 0 && h.params.solve.warm_start) b |= B_WARM_SAVE;
+            // The reference refits only when another pass may follow
+            // (src/solver.rs:561-564).
+            b |= (h.pass + 1 < h.params.corotation_iterations) ? uint32_t(B_FITB) : uint32_t(B_FINISH);
+            return b;
+        }
+        if (st.done != kRun) return B_RESID | B_INNER | prec;
+        return B_INNER | prec;
+    case P_APPLIED:
+        if (h.finish_next) return B_FINISH;
+        return B_ASSEMBLE | B_RESID0 | B_INNER | prec;
+    default:
+        return 0;
+    }
+}
+
+void finish_substep(SlotHost &h) {
+    h.pass = 0;
+    h.finish_next = false;
+    h.substep += 1;
+    if (h.substep == h.params.substeps) {
+        h.substep = 0;
+        h.steps_done += 1;
+    }
+    h.phase = h.steps_done >= h.steps ? P_DONE : P_FINISHED;
+}
+
+// Advance an automatically driven scene after a tick that ran `bits`.
+void advance(SlotHost &h, const SlotState &st, uint32_t bits) {
+    if (bits & (B_RESID0 | B_RESID)) h.last_rel = st.rel;
+    if (st.nonfinite || st.done == kNonFin) {
+        h.phase = P_FAILED;
+        h.status = VX_ERR_NONFINITE_RESIDUAL;
+        return;
+    }
+    if (bits & B_APPLY) {
+        h.solves += 1;
+        h.inner += st.inner_total;
+        h.sweeps += st.sweeps;
+        if (h.pass == 0 && h.params.solve.warm_start) h.warm_ok = true;
+        if (bits & B_FINISH) {
+            finish_substep(h);
+            return;
+        }
+        // Refitted after apply: stop the passes early when the rotations and
+        // the solve have both settled (src/solver.rs:565-569).
+        h.last_angle = st.angle;
+        h.pass += 1;
+        h.phase = P_APPLIED;
+        h.finish_next = st.angle <= h.params.corotation_tolerance && st.rel <= h.params.solve.tolerance;
+        return;
+    }
+    if (bits & B_FINISH) {
+        finish_substep(h);
+        return;
+    }
+    if (bits & (B_PREDICT | B_ASSEMBLE)) h.phase = P_SOLVE;
+    if (h.phase == P_SOLVE && st.outer_done && !(st.rel <= h.params.solve.tolerance)) {
+        h.phase = P_FAILED;
+        h.status = VX_ERR_NOT_CONVERGED;
+    }
+}
+
+int32_t create_batch(uint64_t capacity_rows, uint32_t max_scenes, uint32_t chunk, vx_batch **out) {
+    if (!out || capacity_rows == 0 || max_scenes == 0) return fail(VX_ERR_INVALID_ARGUMENT, "invalid batch size");
+    *out = nullptr;
+    const uint64_t tiles = tiles_for(capacity_rows);
+    if (chunk == 0) chunk = uint32_t(std::max<uint64_t>(1, (tiles + kXcds - 1) / kXcds));
+    const uint64_t unit = uint64_t(kXcds) * chunk;
+    const uint64_t cap = (tiles + unit - 1) / unit * unit;
+    if (cap * kBlock >= (1ull << 31))
+        return fail(VX_ERR_TOO_LARGE, "arena of %llu rows exceeds the index range", (unsigned long long)(cap * kBlock));
+    if (int32_t c = ensure_init()) return c;
+    Lock lock;
+    vx_batch *b = new vx_batch();
+    b->cap_tiles = uint32_t(cap);
+    b->chunk = chunk;
+    b->max_slots = max_scenes;
+    b->slots.assign(max_scenes, SlotHost{});
+    b->owner.assign(b->cap_tiles, kFreeTile);
+    DevArena &A = b->A;
+    const size_t R = size_t(b->cap_tiles) * kBlock;
+    A.R = uint32_t(R);
+    A.cap_tiles = b->cap_tiles;
+    A.chunk = chunk;
+    A.max_slots = max_scenes;
+    struct Item {
+        void **ptr;
+        size_t bytes;
+    };
+    std::vector<Item> items = {
+        {(void **)&A.pos, 24 * R}, {(void **)&A.pred, 24 * R}, {(void **)&A.prev, 24 * R}, {(void **)&A.vel, 24 * R},
+        {(void **)&A.drift, 24 * R}, {(void **)&A.rot, 32 * R}, {(void **)&A.xa, 24 * R}, {(void **)&A.xb, 24 * R},
+        {(void **)&A.xw, 24 * R}, {(void **)&A.targets, 24 * R}, {(void **)&A.mass, 4 * R}, {(void **)&A.young, 4 * R},
+        {(void **)&A.young_perp, 4 * R}, {(void **)&A.shear, 4 * R}, {(void **)&A.solid, 4 * R}, {(void **)&A.fiber, 12 * R},
+        {(void **)&A.aniso, 4 * R}, {(void **)&A.dtemp, 4 * R}, {(void **)&A.a_par, 4 * R}, {(void **)&A.a_perp, 4 * R},
+        {(void **)&A.loads, 12 * R}, {(void **)&A.kinematic, R}, {(void **)&A.bonds, R}, {(void **)&A.tmask, R},
+        {(void **)&A.mask, R}, {(void **)&A.damage, 12 * R}, {(void **)&A.kscale, 12 * R}, {(void **)&A.offset, 36 * R},
+        {(void **)&A.plastic, 36 * R}, {(void **)&A.k64, 144 * R}, {(void **)&A.force, 72 * R}, {(void **)&A.d64, 48 * R},
+        {(void **)&A.dinv64, 48 * R}, {(void **)&A.rhs, 24 * R}, {(void **)&A.k32, 72 * R}, {(void **)&A.d32, 24 * R},
+        {(void **)&A.dinv32, 24 * R}, {(void **)&A.d, 12 * R}, {(void **)&A.r, 12 * R}, {(void **)&A.z, 12 * R},
+        {(void **)&A.p0, 12 * R}, {(void **)&A.p1, 12 * R}, {(void **)&A.ap, 12 * R}, {(void **)&A.dd, 24 * R},
+        {(void **)&A.rd, 24 * R}, {(void **)&A.zd, 24 * R}, {(void **)&A.p0d, 24 * R}, {(void **)&A.p1d, 24 * R},
+        {(void **)&A.apd, 24 * R}, {(void **)&A.partials, 8 * kSlots * size_t(b->cap_tiles)},
+        {(void **)&b->d_tile_slot, 4 * size_t(b->cap_tiles)}, {(void **)&b->d_bits, 4 * size_t(max_scenes)},
+        {(void **)&b->d_geo, sizeof(SlotGeo) * max_scenes}, {(void **)&b->d_prm, sizeof(SlotParams) * max_scenes},
+        {(void **)&b->d_st, sizeof(SlotState) * max_scenes},
+    };
+    size_t total = 0;
+    for (auto &it : items) total += (it.bytes + 255) & ~size_t(255);
+    hipError_t e = hipMalloc(&b->alloc, total);
+    if (e != hipSuccess) {
+        delete b;
+        return fail(e == hipErrorOutOfMemory ? VX_ERR_TOO_LARGE : VX_ERR_DEVICE, "arena allocation of %zu bytes failed: %s", total,
+                    hipGetErrorString(e));
+    }
+    b->alloc_bytes = total;
+    size_t off = 0;
+    for (auto &it : items) {
+        *it.ptr = static_cast<char *>(b->alloc) + off;
+        off += (it.bytes + 255) & ~size_t(255);
+    }
+    A.tile_slot = b->d_tile_slot;
+    A.geo = b->d_geo;
+    A.prm = b->d_prm;
+    A.bits = b->d_bits;
+    A.st = b->d_st;
+    Context &ctx = context();
+    if (hipMemsetAsync(b->d_geo, 0, sizeof(SlotGeo) * max_scenes, ctx.stream) != hipSuccess ||
+        hipMemsetAsync(b->d_st, 0, sizeof(SlotState) * max_scenes, ctx.stream) != hipSuccess ||
+        hipMemsetAsync(A.partials, 0, 8 * kSlots * size_t(b->cap_tiles), ctx.stream) != hipSuccess ||
+        hipStreamSynchronize(ctx.stream) != hipSuccess ||
+        hipHostMalloc((void **)&b->h_bits, sizeof(uint32_t) * max_scenes, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&b->h_st, sizeof(SlotState) * max_scenes, hipHostMallocDefault) != hipSuccess) {
+        (void)hipFree(b->alloc);
+        delete b;
+        return fail(VX_ERR_DEVICE, "arena set-up failed");
+    }
+    std::memset(b->h_bits, 0, sizeof(uint32_t) * max_scenes);
+    std::memset(b->h_st, 0, sizeof(SlotState) * max_scenes);
+    *out = b;
+    return VX_OK;
+}
+
+// Validate a box description; fills per-REV bond / target / free masks.
+int32_t check_desc(const vx_scene_desc *d, std::vector<uint8_t> &bonds, std::vector<uint8_t> &tmask,
+                   std::vector<uint8_t> &mask, uint32_t *active) {
+    if (!d) return fail(VX_ERR_INVALID_ARGUMENT, "null scene description");
+    const uint64_t n64 = uint64_t(d->nx) * d->ny * d->nz;
+    if (n64 == 0) return fail(VX_ERR_INVALID_ARGUMENT, "empty scene");
+    if (n64 >= (1ull << 31)) return fail(VX_ERR_TOO_LARGE, "scene of %llu REVs exceeds the index range", (unsigned long long)n64);
+    if (!d->positions || !d->rotations || !d->velocities || !d->mass || !d->kinematic || !d->young || !d->young_perp ||
+        !d->shear || !d->solid || !d->fibers || !d->anisotropy || !d->delta_temp || !d->alpha_par || !d->alpha_perp ||
+        !d->neighbor)
+        return fail(VX_ERR_INVALID_ARGUMENT, "missing scene array");
+    const uint32_t n = uint32_t(n64), nx = d->nx, ny = d->ny;
+    bonds.assign(n, 0);
+    tmask.assign(n, 0);
+    mask.assign(n, 0);
+    for (uint32_t a = 0; a < 3; ++a) {
+        const uint32_t s = a == 0 ? 1 : (a == 1 ? nx : nx * ny);
+        for (uint32_t v = 0; v < n; ++v) {
+            const uint32_t j = d->neighbor[size_t(a) * n + v];
+            if (j == VX_EMPTY) continue;
+            const uint32_t c = a == 0 ? v % nx : (a == 1 ? (v / nx) % ny : v / (nx * ny));
+            const uint32_t e = a == 0 ? nx : (a == 1 ? ny : d->nz);
+            if (c + 1 >= e || j != v + s) return fail(VX_ERR_INVALID_ARGUMENT, "bond %u of axis %u is not a lattice bond", v, a);
+            bonds[v] |= uint8_t(1u << a);
+        }
+    }
+    uint32_t act = 0;
+    for (uint32_t v = 0; v < n; ++v) {
+        uint8_t m = 0;
+        for (int a = 0; a < 3; ++a) {
+            const bool targeted = d->targets && !std::isnan(d->targets[3 * size_t(v) + a]);
+            if (targeted) tmask[v] |= uint8_t(1u << a);
+            if (!d->kinematic[v] && !targeted) m |= uint8_t(1u << a);
+        }
+        mask[v] = m;
+        act += m ? 1 : 0;
+    }
+    *active = act;
+    return VX_OK;
+}
+
+// First fit of `need` contiguous free tiles; cap_tiles when none.
+uint32_t find_tiles(const vx_batch *b, uint32_t need) {
+    uint32_t run = 0;
+    for (uint32_t t = 0; t < b->cap_tiles; ++t) {
+        run = b->owner[t] == kFreeTile ? run + 1 : 0;
+        if (run == need) return t + 1 - need;
+    }
+    return b->cap_tiles;
+}
+
+int32_t put(void *dst, const void *src, size_t bytes) {
+    hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+    return e == hipSuccess ? VX_OK : fail(VX_ERR_DEVICE, "scene upload failed: %s", hipGetErrorString(e));
+}
+
+int32_t upload_scene(vx_batch *b, const vx_scene_desc *d, const SlotGeo &g, const std::vector<uint8_t> &bonds,
+                     const std::vector<uint8_t> &tmask, const std::vector<uint8_t> &mask) {
+    DevArena &A = b->A;
+    const size_t R = A.R, n = g.n, voff = g.voff;
+    std::vector<double> buf(n);
+    std::vector<float> fbuf(n);
+    // f32 AoS (comps per REV) -> f64 planes
+    auto up_d = [&](double *dst, const float *src, int comps, bool nan_to_zero) -> int32_t {
+        for (int q = 0; q < comps; ++q) {
+            for (size_t v = 0; v < n; ++v) {
+                const double x = src ? double(src[comps * v + q]) : 0.0;
+                buf[v] = (nan_to_zero && std::isnan(x)) ? 0.0 : x;
+            }
+            if (int32_t c = put(dst + q * R + voff, buf.data(), 8 * n)) return c;
+        }
+        return VX_OK;
+    };
+    auto up_f = [&](float *dst, const float *src, int comps, float dflt) -> int32_t {
+        for (int q = 0; q < comps; ++q) {
+            for (size_t v = 0; v < n; ++v) fbuf[v] = src ? src[comps * v + q] : dflt;
+            if (int32_t c = put(dst + q * R + voff, fbuf.data(), 4 * n)) return c;
+        }
+        return VX_OK;
+    };
+    // per-bond [axis][voxel] scalars and [axis][voxel][3] vectors
+    auto up_bond = [&](float *dst, const float *src, float dflt) -> int32_t {
+        for (int a = 0; a < 3; ++a) {
+            for (size_t v = 0; v < n; ++v) fbuf[v] = src ? src[a * n + v] : dflt;
+            if (int32_t c = put(dst + a * R + voff, fbuf.data(), 4 * n)) return c;
+        }
+        return VX_OK;
+    };
+    auto up_bond_vec = [&](float *dst, const float *src) -> int32_t {
+        for (int a = 0; a < 3; ++a)
+            for (int q = 0; q < 3; ++q) {
+                for (size_t v = 0; v < n; ++v) fbuf[v] = src ? src[(a * n + v) * 3 + q] : 0.0f;
+                if (int32_t c = put(dst + (a * 3 + q) * R + voff, fbuf.data(), 4 * n)) return c;
+            }
+        return VX_OK;
+    };
+    int32_t rc;
+#define VX_UP(x)                   \
+    do {                           \
+        if ((rc = (x))) return rc; \
+    } while (0)
+    VX_UP(up_d(A.pos, d->positions, 3, false));
+    VX_UP(up_d(A.pred, d->positions, 3, false));
+    VX_UP(up_d(A.prev, d->positions, 3, false));
+    VX_UP(up_d(A.vel, d->velocities, 3, false));
+    VX_UP(up_d(A.drift, nullptr, 3, false));
+    VX_UP(up_d(A.rot, d->rotations, 4, false));
+    VX_UP(up_d(A.targets, d->targets, 3, true));
+    VX_UP(up_d(A.xa, nullptr, 3, false));
+    VX_UP(up_d(A.xb, nullptr, 3, false));
+    VX_UP(up_d(A.xw, nullptr, 3, false));
+    VX_UP(up_f(A.mass, d->mass, 1, 0));
+    VX_UP(up_f(A.young, d->young, 1, 0));
+    VX_UP(up_f(A.young_perp, d->young_perp, 1, 0));
+    VX_UP(up_f(A.shear, d->shear, 1, 0));
+    VX_UP(up_f(A.solid, d->solid, 1, 1));
+    VX_UP(up_f(A.fiber, d->fibers, 3, 0));
+    VX_UP(up_f(A.aniso, d->anisotropy, 1, 0));
+    VX_UP(up_f(A.dtemp, d->delta_temp, 1, 0));
+    VX_UP(up_f(A.a_par, d->alpha_par, 1, 0));
+    VX_UP(up_f(A.a_perp, d->alpha_perp, 1, 0));
+    VX_UP(up_f(A.loads, d->loads, 3, 0));
+    VX_UP(up_bond(A.damage, d->damage, 0.0f));
+    VX_UP(up_bond(A.kscale, d->kscale, 1.0f));
+    VX_UP(up_bond_vec(A.offset, d->formation_offset));
+    VX_UP(up_bond_vec(A.plastic, d->plastic_strain));
+    VX_UP(put(A.kinematic + voff, d->kinematic, n));
+    VX_UP(put(A.bonds + voff, bonds.data(), n));
+    VX_UP(put(A.tmask + voff, tmask.data(), n));
+    VX_UP(put(A.mask + voff, mask.data(), n));
+    // Inner vectors start at zero so every plane a first iteration reads is
+    // defined.
+    std::fill(fbuf.begin(), fbuf.end(), 0.0f);
+    std::fill(buf.begin(), buf.end(), 0.0);
+    for (float *p : {A.d, A.r, A.z, A.p0, A.p1, A.ap})
+        for (int q = 0; q < 3; ++q) VX_UP(put(p + q * R + voff, fbuf.data(), 4 * n));
+    for (double *p : {A.dd, A.rd, A.zd, A.p0d, A.p1d, A.apd})
+        for (int q = 0; q < 3; ++q) VX_UP(put(p + q * R + voff, buf.data(), 8 * n));
+#undef VX_UP
+    return VX_OK;
+}
+
+SlotParams device_params(double h, const double *g, const vx_solve_params *s) {
+    SlotParams sp{};
+    sp.h = h;
+    sp.gx = g ? g[0] : 0.0;
+    sp.gy = g ? g[1] : 0.0;
+    sp.gz = g ? g[2] : 0.0;
+    if (s) {
+        // Plain FP64: the inner solve goes straight to (half) the outer
+        // tolerance and one more sweep confirms the true residual.
+        const bool f64 = s->mode == VX_SOLVE_F64;
+        sp.tol = s->tolerance;
+        sp.inner_tol = f64 ? s->tolerance * 0.5 : s->inner_tolerance;
+        sp.max_inner = s->max_inner;
+        sp.max_outer = f64 ? 2 : s->max_outer;
+    }
+    return sp;
+}
+
+int32_t admit(vx_batch *b, const vx_scene_desc *d, const vx_step_params *p, uint32_t steps, bool manual, uint32_t *out_slot) {
+    std::vector<uint8_t> bonds, tmask, mask;
+    uint32_t active = 0;
+    if (int32_t c = check_desc(d, bonds, tmask, mask, &active)) return c;
+    if (!manual && (p->substeps == 0 || p->corotation_iterations == 0 || !(p->fixed_dt > 0.0) || p->solve.max_inner == 0 ||
+                    !(p->solve.tolerance > 0.0)))
+        return fail(VX_ERR_INVALID_ARGUMENT, "invalid step parameters");
+    uint32_t slot = b->max_slots;
+    for (uint32_t i = 0; i < b->max_slots; ++i)
+        if (!b->slots[i].used) {
+            slot = i;
+            break;
+        }
+    if (slot == b->max_slots) return fail(VX_ERR_TOO_LARGE, "no free scene slot");
+    const uint32_t n = d->nx * d->ny * d->nz;
+    const uint32_t need = tiles_for(n);
+    const uint32_t t0 = find_tiles(b, need);
+    if (t0 == b->cap_tiles) return fail(VX_ERR_TOO_LARGE, "no contiguous free range of %u tiles", need);
+    const SlotGeo g{d->nx, d->ny, d->nz, n, t0, need, t0 * kBlock, active};
+    if (int32_t c = upload_scene(b, d, g, bonds, tmask, mask)) return c;
+    SlotHost &h = b->slots[slot];
+    h = SlotHost{};
+    h.used = true;
+    h.geo = g;
+    h.manual = manual;
+    if (p) h.params = *p;
+    h.steps = steps;
+    h.phase = manual ? P_IDLE : (steps ? P_START : P_DONE);
+    for (uint32_t t = t0; t < t0 + need; ++t) b->owner[t] = slot;
+    b->owner_dirty = true;
+    const SlotParams sp = p ? device_params(p->fixed_dt / p->substeps, p->gravity, &p->solve) : SlotParams{};
+    const SlotState st{};
+    VX_HIP(hipMemcpy(b->d_geo + slot, &g, sizeof g, hipMemcpyHostToDevice));
+    VX_HIP(hipMemcpy(b->d_prm + slot, &sp, sizeof sp, hipMemcpyHostToDevice));
+    VX_HIP(hipMemcpy(b->d_st + slot, &st, sizeof st, hipMemcpyHostToDevice));
+    b->h_st[slot] = st;
+    *out_slot = slot;
+    return VX_OK;
+}
+
+// Device parameters of one slot; host copies keep the substep length in
+// params.fixed_dt and the gravity for later stage calls.
+int32_t set_params(vx_batch *b, uint32_t slot, double h, const double *g, const vx_solve_params *s) {
+    const SlotParams sp = device_params(h, g, s);
+    vx_step_params &hp = b->slots[slot].params;
+    hp.fixed_dt = h;
+    if (g)
+        for (int i = 0; i < 3; ++i) hp.gravity[i] = g[i];
+    if (s) hp.solve = *s;
+    VX_HIP(hipMemcpy(b->d_prm + slot, &sp, sizeof sp, hipMemcpyHostToDevice));
+    return VX_OK;
+}
+
+// One tick with fixed bits for one manually driven slot.
+int32_t manual_tick(vx_batch *b, uint32_t slot, uint32_t bits) {
+    for (uint32_t i = 0; i < b->max_slots; ++i) b->h_bits[i] = 0;
+    b->h_bits[slot] = bits;
+    return launch_tick(b);
+}
+
+int32_t download_planes(const double *dev, size_t R, size_t voff, size_t n, int comps, double *out) {
+    std::vector<double> buf(n);
+    for (int q = 0; q < comps; ++q) {
+        VX_HIP(hipMemcpy(buf.data(), dev + q * R + voff, 8 * n, hipMemcpyDeviceToHost));
+        for (size_t v = 0; v < n; ++v) out[comps * v + q] = buf[v];
+    }
+    return VX_OK;
+}
+
+int32_t scene_ok(vx_scene *sc) { return sc && sc->batch ? VX_OK : fail(VX_ERR_INVALID_ARGUMENT, "null scene"); }
+
 }  // namespace
 }  // namespace vx
 
 using namespace vx;
 
+// ---- batch API ----------------------------------------------------------------
+extern "C" uint64_t vx_batch_bytes_per_row(void) { return uint64_t(kRowBytes); }
+
+extern "C" int32_t vx_batch_create(uint64_t capacity_rows, uint32_t max_scenes, uint32_t xcd_chunk_tiles, vx_batch **out) {
+    clear_error();
+    return create_batch(capacity_rows, max_scenes, xcd_chunk_tiles, out);
+}
+
+extern "C" int32_t vx_batch_destroy(vx_batch *b) {
+    clear_error();
+    if (!b) return VX_OK;
+    Lock lock;
+    (void)hipStreamSynchronize(context().stream);
+    for (auto &g : b->g_inner)
+        if (g) (void)hipGraphExecDestroy(g);
+    if (b->h_bits) (void)hipHostFree(b->h_bits);
+    if (b->h_st) (void)hipHostFree(b->h_st);
+    if (b->alloc) (void)hipFree(b->alloc);
+    delete b;
+    return VX_OK;
+}
+
+extern "C" int32_t vx_batch_free_rows(vx_batch *b, uint64_t *largest, uint64_t *total) {
+    clear_error();
+    if (!b) return fail(VX_ERR_INVALID_ARGUMENT, "null batch");
+    Lock lock;
+    uint64_t run = 0, best = 0, tot = 0;
+    for (uint32_t t = 0; t < b->cap_tiles; ++t) {
+        if (b->owner[t] == kFreeTile) {
+            run += 1;
+            tot += 1;
+            best = std::max(best, run);
+        } else {
+            run = 0;
+        }
+    }
+    if (largest) *largest = best * kBlock;
+    if (total) *total = tot * kBlock;
+    return VX_OK;
+}
+
+extern "C" int32_t vx_batch_admit(vx_batch *b, const vx_scene_desc *d, const vx_step_params *p, uint32_t steps, uint32_t *out_slot) {
+    clear_error();
+    if (!b || !p || !out_slot) return fail(VX_ERR_INVALID_ARGUMENT, "null argument");
+    Lock lock;
+    return admit(b, d, p, steps, false, out_slot);
+}
+
+extern "C" int32_t vx_batch_tick(vx_batch *b, uint32_t *out_running) {
+    clear_error();
+    if (!b) return fail(VX_ERR_INVALID_ARGUMENT, "null batch");
+    Lock lock;
+    uint32_t running = 0;
+    for (uint32_t i = 0; i < b->max_slots; ++i) {
+        const SlotHost &h = b->slots[i];
+        b->h_bits[i] = (h.used && !h.manual) ? next_bits(h, b->h_st[i]) : 0u;
+        running += b->h_bits[i] ? 1 : 0;
+    }
+    if (running) {
+        if (int32_t c = launch_tick(b)) return c;
+        for (uint32_t i = 0; i < b->max_slots; ++i) {
+            SlotHost &h = b->slots[i];
+            if (h.used && !h.manual && b->h_bits[i]) advance(h, b->h_st[i], b->h_bits[i]);
+        }
+    }
+    uint32_t still = 0;
+    for (uint32_t i = 0; i < b->max_slots; ++i) {
+        const SlotHost &h = b->slots[i];
+        if (h.used && !h.manual && h.phase != P_DONE && h.phase != P_FAILED) still += 1;
+    }
+    if (out_running) *out_running = still;
+    return VX_OK;
+}
+
+extern "C" int32_t vx_batch_report(vx_batch *b, uint32_t slot, vx_batch_scene_report *r) {
+    clear_error();
+    if (!b || !r || slot >= b->max_slots) return fail(VX_ERR_INVALID_ARGUMENT, "invalid slot");
+    Lock lock;
+    const SlotHost &h = b->slots[slot];
+    std::memset(r, 0, sizeof *r);
+    if (!h.used) return VX_OK;
+    r->state = h.phase == P_DONE ? VX_SCENE_DONE : (h.phase == P_FAILED ? VX_SCENE_FAILED : VX_SCENE_RUNNING);
+    r->status = h.status;
+    r->steps_done = h.steps_done;
+    r->solves = h.solves;
+    r->inner_iterations = h.inner;
+    r->outer_sweeps = h.sweeps;
+    r->rows = h.geo.n;
+    r->active_rows = h.geo.active;
+    r->last_relative_residual = h.last_rel;
+    r->last_angle = h.last_angle;
+    return VX_OK;
+}
+
+extern "C" int32_t vx_batch_download(vx_batch *b, uint32_t slot, double *positions, double *rotations, double *velocities) {
+    clear_error();
+    if (!b || slot >= b->max_slots || !b->slots[slot].used) return fail(VX_ERR_INVALID_ARGUMENT, "invalid slot");
+    Lock lock;
+    VX_HIP(hipStreamSynchronize(context().stream));
+    const SlotGeo &g = b->slots[slot].geo;
+    const size_t R = b->A.R;
+    if (positions)
+        if (int32_t c = download_planes(b->A.pos, R, g.voff, g.n, 3, positions)) return c;
+    if (rotations)
+        if (int32_t c = download_planes(b->A.rot, R, g.voff, g.n, 4, rotations)) return c;
+    if (velocities)
+        if (int32_t c = download_planes(b->A.vel, R, g.voff, g.n, 3, velocities)) return c;
+    return VX_OK;
+}
+
+extern "C" int32_t vx_batch_release(vx_batch *b, uint32_t slot) {
+    clear_error();
+    if (!b || slot >= b->max_slots) return fail(VX_ERR_INVALID_ARGUMENT, "invalid slot");
+    Lock lock;
+    SlotHost &h = b->slots[slot];
+    if (!h.used) return VX_OK;
+    for (uint32_t t = h.geo.tile0; t < h.geo.tile0 + h.geo.ntiles; ++t) b->owner[t] = kFreeTile;
+    b->owner_dirty = true;
+    h = SlotHost{};
+    b->h_bits[slot] = 0;
+    const SlotGeo g{};
+    VX_HIP(hipMemcpy(b->d_geo + slot, &g, sizeof g, hipMemcpyHostToDevice));
+    return VX_OK;
+}
+
+// ---- single-scene API (a one-scene arena driven stage by stage) ------------------
 extern "C" int32_t vx_scene_create(const vx_scene_desc *desc, vx_scene **out) {
     clear_error();
     if (!desc || !out) return fail(VX_ERR_INVALID_ARGUMENT, "null argument");
     *out = nullptr;
-    const uint64_t n64 = uint64_t(desc->nx) * desc->ny * desc->nz;
-    if (n64 == 0) return fail(VX_ERR_INVALID_ARGUMENT, "empty scene");
-    if (n64 >= (1ull << 31)) return fail(VX_ERR_TOO_LARGE, "scene of %llu REVs exceeds the index range", (unsigned long long)n64);
-    if (!desc->positions || !desc->rotations || !desc->velocities || !desc->mass || !desc->kinematic || !desc->young ||
-        !desc->young_perp || !desc->shear || !desc->solid || !desc->fibers || !desc->anisotropy || !desc->delta_temp ||
-        !desc->alpha_par || !desc->alpha_perp || !desc->neighbor)
-        return fail(VX_ERR_INVALID_ARGUMENT, "missing scene array");
-    const uint32_t n = uint32_t(n64);
-    const uint32_t nx = desc->nx, ny = desc->ny;
-    // Bonds must be lattice bonds v -> v + stride_a.
-    std::vector<uint8_t> bonds(n, 0), tmask(n, 0), mask(n, 0);
-    for (uint32_t a = 0; a < 3; ++a) {
-        const uint32_t s = a == 0 ? 1 : (a == 1 ? nx : nx * ny);
-        for (uint32_t v = 0; v < n; ++v) {
-            uint32_t j = desc->neighbor[size_t(a) * n + v];
-            if (j == VX_EMPTY) continue;
-            uint32_t c = a == 0 ? v % nx : (a == 1 ? (v / nx) % ny : v / (nx * ny));
-            uint32_t e = a == 0 ? nx : (a == 1 ? ny : desc->nz);
-            if (c + 1 >= e || j != v + s) return fail(VX_ERR_INVALID_ARGUMENT, "bond %u of axis %u is not a lattice bond", v, a);
-            bonds[v] |= uint8_t(1u << a);
-        }
-    }
-    uint32_t active = 0;
-    for (uint32_t v = 0; v < n; ++v) {
-        uint8_t m = 0;
-        for (int a = 0; a < 3; ++a) {
-            bool targeted = desc->targets && !std::isnan(desc->targets[3 * size_t(v) + a]);
-            if (targeted) tmask[v] |= uint8_t(1u << a);
-            if (!desc->kinematic[v] && !targeted) m |= uint8_t(1u << a);
-        }
-        mask[v] = m;
-        active += m ? 1 : 0;
-    }
-    int32_t c = ensure_init();
-    if (c != VX_OK) return c;
+    const uint64_t n = uint64_t(desc->nx) * desc->ny * desc->nz;
+    if (n == 0) return fail(VX_ERR_INVALID_ARGUMENT, "empty scene");
+    if (n >= (1ull << 31)) return fail(VX_ERR_TOO_LARGE, "scene of %llu REVs exceeds the index range", (unsigned long long)n);
     Lock lock;
-    Context &ctx = context();
+    vx_batch *b = nullptr;
+    if (int32_t c = create_batch(n, 1, 0, &b)) return c;
+    uint32_t slot = 0;
+    if (int32_t c = admit(b, desc, nullptr, 0, true, &slot)) {
+        vx_batch_destroy(b);
+        return c;
+    }
     vx_scene *sc = new vx_scene();
-    DevScene &G = sc->s;
-    G.g = Geo{desc->nx, desc->ny, desc->nz, n};
-    sc->active = active;
-    sc->ntiles = blocks_for(n);
-    sc->grid = xcd_grid(sc->ntiles, &sc->per_xcd);
-    // One allocation, carved into 256-byte aligned planes.
-    struct Item {
-        void **ptr;
-        size_t bytes;
-    };
-    const size_t N = n;
-    std::vector<Item> items = {
-        {(void **)&G.pos, 24 * N}, {(void **)&G.pred, 24 * N}, {(void **)&G.prev, 24 * N}, {(void **)&G.vel, 24 * N},
-        {(void **)&G.drift, 24 * N}, {(void **)&G.rot, 32 * N}, {(void **)&G.mass, 4 * N}, {(void **)&G.young, 4 * N},
-        {(void **)&G.young_perp, 4 * N}, {(void **)&G.shear, 4 * N}, {(void **)&G.solid, 4 * N}, {(void **)&G.fiber, 12 * N},
-        {(void **)&G.aniso, 4 * N}, {(void **)&G.dtemp, 4 * N}, {(void **)&G.a_par, 4 * N}, {(void **)&G.a_perp, 4 * N},
-        {(void **)&G.kinematic, N}, {(void **)&G.bonds, N}, {(void **)&G.tmask, N}, {(void **)&G.targets, 24 * N},
-        {(void **)&G.loads, 12 * N}, {(void **)&G.damage, 12 * N}, {(void **)&G.kscale, 12 * N}, {(void **)&G.offset, 36 * N},
-        {(void **)&G.plastic, 36 * N}, {(void **)&G.k64, 144 * N}, {(void **)&G.force, 72 * N}, {(void **)&G.k32, 72 * N},
-        {(void **)&G.d64, 48 * N}, {(void **)&G.dinv64, 48 * N}, {(void **)&G.rhs, 24 * N}, {(void **)&G.d32, 24 * N},
-        {(void **)&G.dinv32, 24 * N}, {(void **)&G.mask, N}, {(void **)&G.xa, 24 * N}, {(void **)&G.xb, 24 * N},
-        {(void **)&G.d, 12 * N}, {(void **)&G.r, 12 * N}, {(void **)&G.z, 12 * N}, {(void **)&G.p0, 12 * N},
-        {(void **)&G.p1, 12 * N}, {(void **)&G.ap, 12 * N}, {(void **)&G.dd, 24 * N}, {(void **)&G.rd, 24 * N},
-        {(void **)&G.zd, 24 * N}, {(void **)&G.p0d, 24 * N}, {(void **)&G.p1d, 24 * N}, {(void **)&G.apd, 24 * N},
-        {(void **)&G.partials, 8 * 8 * size_t(sc->grid)}, {(void **)&G.st, sizeof(GridState)},
-    };
-    size_t total = 0;
-    for (auto &it : items) total += (it.bytes + 255) & ~size_t(255);
-    hipError_t e = hipMalloc(&sc->alloc, total);
-    if (e != hipSuccess) {
-        delete sc;
-        return fail(e == hipErrorOutOfMemory ? VX_ERR_TOO_LARGE : VX_ERR_DEVICE, "scene allocation of %zu bytes failed: %s", total,
-                    hipGetErrorString(e));
-    }
-    sc->alloc_bytes = total;
-    size_t off = 0;
-    for (auto &it : items) {
-        *it.ptr = static_cast<char *>(sc->alloc) + off;
-        off += (it.bytes + 255) & ~size_t(255);
-    }
-    auto fail_free = [&](int32_t code) {
-        (void)hipFree(sc->alloc);
-        delete sc;
-        return code;
-    };
-    // Host staging: AoS inputs -> SoA planes (f64 state).
-    std::vector<double> buf(4 * N);
-    auto up_d = [&](double *dst, const float *src, int comps) -> int32_t {
-        for (int q = 0; q < comps; ++q)
-            for (size_t v = 0; v < N; ++v) buf[q * N + v] = src ? double(src[comps * v + q]) : 0.0;
-        hipError_t er = hipMemcpy(dst, buf.data(), sizeof(double) * comps * N, hipMemcpyHostToDevice);
-        return er == hipSuccess ? VX_OK : fail(VX_ERR_DEVICE, "upload failed: %s", hipGetErrorString(er));
-    };
-    std::vector<float> fbuf(9 * N);
-    auto up_f = [&](float *dst, const float *src, int comps, float dflt) -> int32_t {
-        for (int q = 0; q < comps; ++q)
-            for (size_t v = 0; v < N; ++v) fbuf[q * N + v] = src ? src[comps * v + q] : dflt;
-        hipError_t er = hipMemcpy(dst, fbuf.data(), sizeof(float) * comps * N, hipMemcpyHostToDevice);
-        return er == hipSuccess ? VX_OK : fail(VX_ERR_DEVICE, "upload failed: %s", hipGetErrorString(er));
-    };
-    // Per-bond planes come [axis][voxel] (scalars) or [axis][voxel][3]
-    // (vectors); the device wants [axis][component][voxel].
-    auto up_bond_vec = [&](float *dst, const float *src) -> int32_t {
-        for (int a = 0; a < 3; ++a)
-            for (int q = 0; q < 3; ++q)
-                for (size_t v = 0; v < N; ++v) fbuf[(a * 3 + q) * N + v] = src ? src[(a * N + v) * 3 + q] : 0.0f;
-        hipError_t er = hipMemcpy(dst, fbuf.data(), sizeof(float) * 9 * N, hipMemcpyHostToDevice);
-        return er == hipSuccess ? VX_OK : fail(VX_ERR_DEVICE, "upload failed: %s", hipGetErrorString(er));
-    };
-    auto up_bond = [&](float *dst, const float *src, float dflt) -> int32_t {
-        for (size_t k = 0; k < 3 * N; ++k) fbuf[k] = src ? src[k] : dflt;
-        hipError_t er = hipMemcpy(dst, fbuf.data(), sizeof(float) * 3 * N, hipMemcpyHostToDevice);
-        return er == hipSuccess ? VX_OK : fail(VX_ERR_DEVICE, "upload failed: %s", hipGetErrorString(er));
-    };
-    int32_t rc = VX_OK;
-#define UP(x)                          \
-    do {                               \
-        rc = (x);                      \
-        if (rc != VX_OK) return fail_free(rc); \
-    } while (0)
-    UP(up_d(G.pos, desc->positions, 3));
-    UP(up_d(G.pred, desc->positions, 3));
-    UP(up_d(G.prev, desc->positions, 3));
-    UP(up_d(G.vel, desc->velocities, 3));
-    UP(up_d(G.drift, nullptr, 3));
-    UP(up_d(G.rot, desc->rotations, 4));
-    {
-        std::vector<float> tg(3 * N);
-        for (size_t k = 0; k < 3 * N; ++k) tg[k] = desc->targets ? desc->targets[k] : 0.0f;
-        for (size_t k = 0; k < 3 * N; ++k)
-            if (std::isnan(tg[k])) tg[k] = 0.0f;
-        UP(up_d(G.targets, tg.data(), 3));
-    }
-    UP(up_f(G.mass, desc->mass, 1, 0));
-    UP(up_f(G.young, desc->young, 1, 0));
-    UP(up_f(G.young_perp, desc->young_perp, 1, 0));
-    UP(up_f(G.shear, desc->shear, 1, 0));
-    UP(up_f(G.solid, desc->solid, 1, 1));
-    UP(up_f(G.fiber, desc->fibers, 3, 0));
-    UP(up_f(G.aniso, desc->anisotropy, 1, 0));
-    UP(up_f(G.dtemp, desc->delta_temp, 1, 0));
-    UP(up_f(G.a_par, desc->alpha_par, 1, 0));
-    UP(up_f(G.a_perp, desc->alpha_perp, 1, 0));
-    UP(up_f(G.loads, desc->loads, 3, 0));
-    UP(up_bond(G.damage, desc->damage, 0.0f));
-    UP(up_bond(G.kscale, desc->kscale, 1.0f));
-    UP(up_bond_vec(G.offset, desc->formation_offset));
-    UP(up_bond_vec(G.plastic, desc->plastic_strain));
-#undef UP
-    if (hipMemcpy(G.kinematic, desc->kinematic, N, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(G.bonds, bonds.data(), N, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(G.tmask, tmask.data(), N, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(G.mask, mask.data(), N, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemset(G.xa, 0, 24 * N) != hipSuccess || hipMemset(G.xb, 0, 24 * N) != hipSuccess ||
-        hipMemset(G.d, 0, 12 * N) != hipSuccess || hipMemset(G.dd, 0, 24 * N) != hipSuccess ||
-        hipMemset(G.st, 0, sizeof(GridState)) != hipSuccess)
-        return fail_free(fail(VX_ERR_DEVICE, "scene upload failed"));
-    if (hipHostMalloc((void **)&sc->pinned, 3 * sizeof(GridState), hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void **)&sc->host_partials, 2 * sizeof(double) * sc->grid, hipHostMallocDefault) != hipSuccess ||
-        hipEventCreateWithFlags(&sc->ev[0], hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&sc->ev[1], hipEventDisableTiming) != hipSuccess)
-        return fail_free(fail(VX_ERR_DEVICE, "scene host resources failed"));
-    (void)ctx;
+    sc->batch = b;
+    sc->slot = slot;
+    sc->active = b->slots[slot].geo.active;
     *out = sc;
     return VX_OK;
 }
@@ -1077,111 +1437,111 @@ extern "C" int32_t vx_scene_create(const vx_scene_desc *desc, vx_scene **out) {
 extern "C" int32_t vx_scene_destroy(vx_scene *sc) {
     clear_error();
     if (!sc) return VX_OK;
-    Lock lock;
-    (void)hipStreamSynchronize(context().stream);
-    if (sc->inner32) (void)hipGraphExecDestroy(sc->inner32);
-    if (sc->inner64) (void)hipGraphExecDestroy(sc->inner64);
-    for (auto &e : sc->ev)
-        if (e) (void)hipEventDestroy(e);
-    if (sc->pinned) (void)hipHostFree(sc->pinned);
-    if (sc->host_partials) (void)hipHostFree(sc->host_partials);
-    if (sc->alloc) (void)hipFree(sc->alloc);
+    vx_batch_destroy(sc->batch);
     delete sc;
     return VX_OK;
 }
 
 extern "C" int32_t vx_scene_predict(vx_scene *sc, double h, const double *g) {
     clear_error();
-    if (int32_t c = check_scene(sc)) return c;
+    if (int32_t c = scene_ok(sc)) return c;
+    if (!(h > 0.0)) return fail(VX_ERR_INVALID_ARGUMENT, "substep length must be positive");
     Lock lock;
-    const double gx = g ? g[0] : 0.0, gy = g ? g[1] : 0.0, gz = g ? g[2] : 0.0;
-    hipLaunchKernelGGL(grid_predict, dim3(blocks_for(sc->s.g.n)), dim3(kBlock), 0, context().stream, sc->s, h, gx, gy, gz);
-    VX_HIP(hipGetLastError());
-    return VX_OK;
+    vx_batch *b = sc->batch;
+    if (int32_t c = set_params(b, sc->slot, h, g, &b->slots[sc->slot].params.solve)) return c;
+    return manual_tick(b, sc->slot, B_PREDICT);
 }
 
 extern "C" int32_t vx_scene_fit_rotations(vx_scene *sc, double *out_max_angle) {
     clear_error();
-    if (int32_t c = check_scene(sc)) return c;
+    if (int32_t c = scene_ok(sc)) return c;
     Lock lock;
-    Context &ctx = context();
-    VX_HIP(hipMemsetAsync(&sc->s.st->angle_bits, 0, sizeof(uint64_t), ctx.stream));
-    hipLaunchKernelGGL(grid_fit, dim3(blocks_for(sc->s.g.n)), dim3(kBlock), 0, ctx.stream, sc->s);
-    VX_HIP(hipGetLastError());
-    if (out_max_angle) {
-        GridState st;
-        if (int32_t c = read_state(sc, &st)) return c;
-        double a;
-        std::memcpy(&a, &st.angle_bits, sizeof a);
-        *out_max_angle = a;
-    }
+    if (int32_t c = manual_tick(sc->batch, sc->slot, B_FITB)) return c;
+    if (out_max_angle) *out_max_angle = sc->batch->h_st[sc->slot].angle;
     return VX_OK;
 }
 
 extern "C" int32_t vx_scene_assemble(vx_scene *sc, double h) {
     clear_error();
-    if (int32_t c = check_scene(sc)) return c;
+    if (int32_t c = scene_ok(sc)) return c;
     if (!(h > 0.0)) return fail(VX_ERR_INVALID_ARGUMENT, "substep length must be positive");
     Lock lock;
-    Context &ctx = context();
-    hipLaunchKernelGGL(grid_bonds, dim3(blocks_for(sc->s.g.n)), dim3(kBlock), 0, ctx.stream, sc->s);
-    VX_HIP(hipGetLastError());
-    hipLaunchKernelGGL(grid_rows, dim3(blocks_for(sc->s.g.n)), dim3(kBlock), 0, ctx.stream, sc->s, h);
-    VX_HIP(hipGetLastError());
-    return VX_OK;
+    vx_batch *b = sc->batch;
+    const vx_step_params p = b->slots[sc->slot].params;
+    if (int32_t c = set_params(b, sc->slot, h, p.gravity, &p.solve)) return c;
+    return manual_tick(b, sc->slot, B_ASSEMBLE);
 }
 
 extern "C" int32_t vx_scene_solve(vx_scene *sc, const vx_solve_params *prm, vx_solve_report *rep) {
     clear_error();
-    if (int32_t c = check_scene(sc)) return c;
+    if (int32_t c = scene_ok(sc)) return c;
     if (!prm || prm->max_inner == 0 || !(prm->tolerance > 0.0)) return fail(VX_ERR_INVALID_ARGUMENT, "invalid solve parameters");
     Lock lock;
+    vx_batch *b = sc->batch;
+    SlotHost &h = b->slots[sc->slot];
+    const vx_step_params p = h.params;
+    if (int32_t c = set_params(b, sc->slot, p.fixed_dt, p.gravity, prm)) return c;
+    const uint32_t prec = prm->mode == VX_SOLVE_F64 ? uint32_t(B_F64) : 0u;
+    uint32_t bits = B_RESID0 | B_INNER | prec | ((prm->warm_start && h.warm_ok) ? uint32_t(B_WARM_USE) : 0u);
     if (rep) std::memset(rep, 0, sizeof *rep);
-    if (prm->mode == VX_SOLVE_F64) return solve_t<double>(sc, prm, rep);
-    return solve_t<float>(sc, prm, rep);
+    const SlotState *st = b->h_st + sc->slot;
+    for (;;) {
+        if (int32_t c = manual_tick(b, sc->slot, bits)) return c;
+        if (st->nonfinite || st->done == kNonFin) {
+            if (rep) rep->status = VX_ERR_NONFINITE_RESIDUAL;
+            return fail(VX_ERR_NONFINITE_RESIDUAL, "HIP PCG produced a non-finite residual norm");
+        }
+        if (st->outer_done) break;
+        bits = (st->done != kRun ? uint32_t(B_RESID) : 0u) | B_INNER | prec;
+    }
+    // The solution becomes the warm start of a later solve of this scene.
+    if (int32_t c = manual_tick(b, sc->slot, B_WARM_SAVE)) return c;
+    h.warm_ok = true;
+    const double rel = st->rel;
+    if (rep) {
+        rep->inner_iterations = st->inner_total;
+        rep->outer_sweeps = st->sweeps;
+        rep->active_rows = sc->active;
+        rep->relative_residual = rel;
+        rep->rhs_norm = std::sqrt(st->bb);
+    }
+    int32_t status = VX_OK;
+    if (!(rel <= prm->tolerance))
+        status = fail(VX_ERR_NOT_CONVERGED, "HIP PCG did not converge: inner iterations=%u, sweeps=%u, relative residual=%.3e, tolerance=%.3e",
+                      st->inner_total, st->sweeps, rel, prm->tolerance);
+    if (rep) rep->status = uint32_t(status);
+    return status;
 }
 
 extern "C" int32_t vx_scene_apply(vx_scene *sc) {
     clear_error();
-    if (int32_t c = check_scene(sc)) return c;
+    if (int32_t c = scene_ok(sc)) return c;
     Lock lock;
-    hipLaunchKernelGGL(grid_apply, dim3(blocks_for(sc->s.g.n)), dim3(kBlock), 0, context().stream, sc->s,
-                       (const double *)(sc->x_in_a ? sc->s.xa : sc->s.xb));
-    VX_HIP(hipGetLastError());
-    return VX_OK;
+    return manual_tick(sc->batch, sc->slot, B_APPLY);
 }
 
 extern "C" int32_t vx_scene_finish_substep(vx_scene *sc, double h) {
     clear_error();
-    if (int32_t c = check_scene(sc)) return c;
+    if (int32_t c = scene_ok(sc)) return c;
+    if (!(h > 0.0)) return fail(VX_ERR_INVALID_ARGUMENT, "substep length must be positive");
     Lock lock;
-    hipLaunchKernelGGL(grid_finish, dim3(blocks_for(sc->s.g.n)), dim3(kBlock), 0, context().stream, sc->s, h);
-    VX_HIP(hipGetLastError());
-    return VX_OK;
-}
-
-static int32_t download_planes(const double *dev, size_t n, int comps, double *out) {
-    std::vector<double> buf(comps * n);
-    VX_HIP(hipMemcpyAsync(buf.data(), dev, sizeof(double) * comps * n, hipMemcpyDeviceToHost, context().stream));
-    VX_HIP(hipStreamSynchronize(context().stream));
-    for (size_t v = 0; v < n; ++v)
-        for (int q = 0; q < comps; ++q) out[comps * v + q] = buf[q * n + v];
-    return VX_OK;
+    vx_batch *b = sc->batch;
+    const vx_step_params p = b->slots[sc->slot].params;
+    if (int32_t c = set_params(b, sc->slot, h, p.gravity, &p.solve)) return c;
+    return manual_tick(b, sc->slot, B_FINISH);
 }
 
 extern "C" int32_t vx_scene_download(vx_scene *sc, double *positions, double *rotations, double *velocities, double *correction) {
     clear_error();
-    if (int32_t c = check_scene(sc)) return c;
+    if (int32_t c = scene_ok(sc)) return c;
     Lock lock;
-    const size_t n = sc->s.g.n;
-    if (positions)
-        if (int32_t c = download_planes(sc->s.pos, n, 3, positions)) return c;
-    if (rotations)
-        if (int32_t c = download_planes(sc->s.rot, n, 4, rotations)) return c;
-    if (velocities)
-        if (int32_t c = download_planes(sc->s.vel, n, 3, velocities)) return c;
-    if (correction)
-        if (int32_t c = download_planes(sc->x_in_a ? sc->s.xa : sc->s.xb, n, 3, correction)) return c;
+    vx_batch *b = sc->batch;
+    if (int32_t c = vx_batch_download(b, sc->slot, positions, rotations, velocities)) return c;
+    if (correction) {
+        const SlotGeo &g = b->slots[sc->slot].geo;
+        const double *x = b->h_st[sc->slot].xpar ? b->A.xb : b->A.xa;
+        if (int32_t c = download_planes(x, b->A.R, g.voff, g.n, 3, correction)) return c;
+    }
     return VX_OK;
 }
 
@@ -1192,25 +1552,24 @@ extern "C" int32_t vx_scene_download(vx_scene *sc, double *positions, double *ro
 extern "C" int32_t vx_scene_export_rows(vx_scene *sc, uint32_t *active_out, uint32_t *neighbors, double *blocks, double *diagonal,
                                         double *rhs_out, uint32_t *out_active_count) {
     clear_error();
-    if (int32_t c = check_scene(sc)) return c;
+    if (int32_t c = scene_ok(sc)) return c;
     Lock lock;
-    const DevScene &G = sc->s;
-    const size_t n = G.g.n;
+    vx_batch *b = sc->batch;
+    const SlotGeo &G = b->slots[sc->slot].geo;
+    const size_t n = G.n, R = b->A.R, voff = G.voff;
+    VX_HIP(hipStreamSynchronize(context().stream));
     std::vector<double> k(18 * n), d(6 * n), rhs(3 * n);
-    std::vector<uint8_t> mask(n), bonds(n), kin(n);
-    Context &ctx = context();
-    VX_HIP(hipMemcpyAsync(k.data(), G.k64, 8 * 18 * n, hipMemcpyDeviceToHost, ctx.stream));
-    VX_HIP(hipMemcpyAsync(d.data(), G.d64, 8 * 6 * n, hipMemcpyDeviceToHost, ctx.stream));
-    VX_HIP(hipMemcpyAsync(rhs.data(), G.rhs, 8 * 3 * n, hipMemcpyDeviceToHost, ctx.stream));
-    VX_HIP(hipMemcpyAsync(mask.data(), G.mask, n, hipMemcpyDeviceToHost, ctx.stream));
-    VX_HIP(hipMemcpyAsync(bonds.data(), G.bonds, n, hipMemcpyDeviceToHost, ctx.stream));
-    VX_HIP(hipMemcpyAsync(kin.data(), G.kinematic, n, hipMemcpyDeviceToHost, ctx.stream));
-    VX_HIP(hipStreamSynchronize(ctx.stream));
+    std::vector<uint8_t> mask(n), bonds(n);
+    for (int q = 0; q < 18; ++q) VX_HIP(hipMemcpy(k.data() + q * n, b->A.k64 + q * R + voff, 8 * n, hipMemcpyDeviceToHost));
+    for (int q = 0; q < 6; ++q) VX_HIP(hipMemcpy(d.data() + q * n, b->A.d64 + q * R + voff, 8 * n, hipMemcpyDeviceToHost));
+    for (int q = 0; q < 3; ++q) VX_HIP(hipMemcpy(rhs.data() + q * n, b->A.rhs + q * R + voff, 8 * n, hipMemcpyDeviceToHost));
+    VX_HIP(hipMemcpy(mask.data(), b->A.mask + voff, n, hipMemcpyDeviceToHost));
+    VX_HIP(hipMemcpy(bonds.data(), b->A.bonds + voff, n, hipMemcpyDeviceToHost));
     std::vector<uint32_t> aidx(n, VX_EMPTY);
     uint32_t na = 0;
     for (size_t v = 0; v < n; ++v)
         if (mask[v]) aidx[v] = na++;
-    const uint32_t nx = G.g.nx, ny = G.g.ny;
+    const uint32_t nx = G.nx, ny = G.ny;
     const int sym[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
     for (size_t v = 0; v < n; ++v) {
         if (!mask[v]) continue;
@@ -1221,11 +1580,11 @@ extern "C" int32_t vx_scene_export_rows(vx_scene *sc, uint32_t *active_out, uint
                 for (int c = 0; c < 3; ++c) diagonal[9 * size_t(slot) + 3 * r + c] = d[sym[r][c] * n + v];
         if (rhs_out)
             for (int q = 0; q < 3; ++q) rhs_out[3 * size_t(slot) + q] = rhs[q * n + v];
-        uint32_t e = 0;
         if (neighbors)
             for (int s = 0; s < 6; ++s) neighbors[6 * size_t(slot) + s] = VX_EMPTY;
         if (blocks)
             for (int s = 0; s < 54; ++s) blocks[54 * size_t(slot) + s] = 0.0;
+        uint32_t e = 0;
         const uint32_t c3[3] = {uint32_t(v % nx), uint32_t((v / nx) % ny), uint32_t(v / (size_t(nx) * ny))};
         for (int a = 0; a < 3; ++a) {
             const size_t s = a == 0 ? 1 : (a == 1 ? nx : size_t(nx) * ny);
@@ -1240,7 +1599,6 @@ extern "C" int32_t vx_scene_export_rows(vx_scene *sc, uint32_t *active_out, uint
                     other = v + s;
                 }
                 if (!(bonds[owner] >> a & 1)) continue;
-                // Constraint enters the system (one endpoint has a free axis).
                 if (!(mask[v] | mask[other])) continue;
                 if (e >= 6) continue;
                 const uint32_t nb = aidx[other];
@@ -1256,36 +1614,44 @@ extern "C" int32_t vx_scene_export_rows(vx_scene *sc, uint32_t *active_out, uint
             }
         }
     }
-    (void)kin;
     if (out_active_count) *out_active_count = na;
     return VX_OK;
 }
 
 extern "C" int32_t vx_scene_time_spmv(vx_scene *sc, uint32_t reps, double *out_us) {
     clear_error();
-    if (int32_t c = check_scene(sc)) return c;
+    if (int32_t c = scene_ok(sc)) return c;
     if (reps == 0 || !out_us) return fail(VX_ERR_INVALID_ARGUMENT, "reps must be positive");
     Lock lock;
+    vx_batch *b = sc->batch;
     Context &ctx = context();
-    DevScene &G = sc->s;
-    // The timed launches run as first inner iterations (k = 0, nonzero
-    // residual partials, not done); they leave the solve vectors' contents
-    // unspecified.
-    GridState init;
-    std::memset(&init, 0, sizeof init);
-    init.inner_tol = 0.0;
-    init.max_iter = 1u << 30;
-    sc->pinned[2] = init;
-    VX_HIP(hipMemcpyAsync(G.st, &sc->pinned[2], sizeof init, hipMemcpyHostToDevice, ctx.stream));
-    std::vector<double> ones(2 * size_t(sc->grid), 1.0);
-    VX_HIP(hipMemcpyAsync(G.partials + size_t(kRz) * sc->grid, ones.data(), sizeof(double) * ones.size(), hipMemcpyHostToDevice,
-                          ctx.stream));
+    // The timed launches run as first FP32 inner iterations of the scene (k = 0,
+    // unit residual partials, never done); they keep the solution buffers and
+    // leave the inner vectors' contents unspecified.
+    SlotState st = b->h_st[sc->slot];
+    st.k_spmv = 0;
+    st.k_update = 0;
+    st.done = kRun;
+    st.rho[0] = st.rho[1] = 0.0;
+    const vx_step_params p = b->slots[sc->slot].params;
+    SlotParams sp = device_params(p.fixed_dt, p.gravity, &p.solve);
+    sp.inner_tol = 0.0;
+    sp.max_inner = 1u << 30;
+    const SlotGeo &g = b->slots[sc->slot].geo;
+    VX_HIP(hipMemcpy(b->d_st + sc->slot, &st, sizeof st, hipMemcpyHostToDevice));
+    VX_HIP(hipMemcpy(b->d_prm + sc->slot, &sp, sizeof sp, hipMemcpyHostToDevice));
+    const std::vector<double> ones(g.ntiles, 1.0);
+    VX_HIP(hipMemcpy(b->A.partials + size_t(kRz) * b->cap_tiles + g.tile0, ones.data(), 8 * g.ntiles, hipMemcpyHostToDevice));
+    VX_HIP(hipMemcpy(b->A.partials + size_t(kRr) * b->cap_tiles + g.tile0, ones.data(), 8 * g.ntiles, hipMemcpyHostToDevice));
+    for (uint32_t i = 0; i < b->max_slots; ++i) b->h_bits[i] = 0;
+    b->h_bits[sc->slot] = B_INNER;
+    if (int32_t c = upload_tables(b)) return c;
+    VX_HIP(hipMemcpyAsync(b->d_bits, b->h_bits, sizeof(uint32_t) * b->max_slots, hipMemcpyHostToDevice, ctx.stream));
     std::vector<hipEvent_t> ev(2 * reps);
     for (auto &e : ev) VX_HIP(hipEventCreate(&e));
     for (uint32_t k = 0; k < reps; ++k) {
         VX_HIP(hipEventRecord(ev[2 * k], ctx.stream));
-        hipLaunchKernelGGL(grid_spmv<float>, dim3(sc->grid), dim3(kBlock), 0, ctx.stream, G, sc->per_xcd, sc->ntiles, G.z,
-                           (k % 2) ? G.p1 : G.p0, (k % 2) ? G.p0 : G.p1, G.ap);
+        hipLaunchKernelGGL(k_spmv<float>, dim3(b->cap_tiles), dim3(kBlock), 0, ctx.stream, b->A, int(k & 1));
         VX_HIP(hipEventRecord(ev[2 * k + 1], ctx.stream));
     }
     VX_HIP(hipStreamSynchronize(ctx.stream));
@@ -1297,48 +1663,59 @@ extern "C" int32_t vx_scene_time_spmv(vx_scene *sc, uint32_t reps, double *out_u
     }
     for (auto &e : ev) (void)hipEventDestroy(e);
     *out_us = 1000.0 * total / reps;
-    return VX_OK;
+    // Restore the scene's state word (solution buffer parity) and parameters.
+    st = b->h_st[sc->slot];
+    VX_HIP(hipMemcpy(b->d_st + sc->slot, &st, sizeof st, hipMemcpyHostToDevice));
+    return set_params(b, sc->slot, p.fixed_dt, p.gravity, &p.solve);
 }
 
 extern "C" int32_t vx_scene_step(vx_scene *sc, const vx_step_params *prm, vx_step_report *rep) {
     clear_error();
-    if (int32_t c = check_scene(sc)) return c;
-    if (!prm || prm->substeps == 0 || prm->corotation_iterations == 0 || !(prm->fixed_dt > 0.0))
+    if (int32_t c = scene_ok(sc)) return c;
+    if (!prm || prm->substeps == 0 || prm->corotation_iterations == 0 || !(prm->fixed_dt > 0.0) || prm->solve.max_inner == 0 ||
+        !(prm->solve.tolerance > 0.0))
         return fail(VX_ERR_INVALID_ARGUMENT, "invalid step parameters");
     Lock lock;
-    vx_step_report out;
-    std::memset(&out, 0, sizeof out);
-    out.active_rows = sc->active;
+    vx_batch *b = sc->batch;
+    SlotHost &h = b->slots[sc->slot];
     const auto t0 = std::chrono::steady_clock::now();
-    const double h = prm->fixed_dt / prm->substeps;
+    // Drive the slot through one step with the per-scene state machine.
+    const bool warm_ok = h.warm_ok;
+    if (int32_t c = set_params(b, sc->slot, prm->fixed_dt / prm->substeps, prm->gravity, &prm->solve)) return c;
+    h.params = *prm;
+    h.manual = false;
+    h.steps = 1;
+    h.steps_done = 0;
+    h.substep = 0;
+    h.pass = 0;
+    h.finish_next = false;
+    h.solves = 0;
+    h.inner = 0;
+    h.sweeps = 0;
+    h.status = VX_OK;
+    h.warm_ok = warm_ok;
+    h.phase = P_START;
+    uint32_t running = 1;
     int32_t status = VX_OK;
-    for (uint32_t sub = 0; sub < prm->substeps && status == VX_OK; ++sub) {
-        if ((status = vx_scene_predict(sc, h, prm->gravity))) break;
-        if ((status = vx_scene_fit_rotations(sc, nullptr))) break;
-        for (uint32_t pass = 0; pass < prm->corotation_iterations; ++pass) {
-            if ((status = vx_scene_assemble(sc, h))) break;
-            vx_solve_report sr;
-            status = vx_scene_solve(sc, &prm->solve, &sr);
-            out.solves += 1;
-            out.inner_iterations += sr.inner_iterations;
-            out.outer_sweeps += sr.outer_sweeps;
-            out.last_relative_residual = sr.relative_residual;
-            if (status) break;
-            if ((status = vx_scene_apply(sc))) break;
-            if (pass + 1 == prm->corotation_iterations) break;
-            double angle = 0.0;
-            if ((status = vx_scene_fit_rotations(sc, &angle))) break;
-            out.last_angle = angle;
-            if (angle <= prm->corotation_tolerance && sr.relative_residual <= prm->solve.tolerance) break;
-        }
-        if (status) break;
-        status = vx_scene_finish_substep(sc, h);
+    while (running) {
+        if ((status = vx_batch_tick(b, &running))) break;
     }
-    if (status == VX_OK) {
-        hipError_t e = hipStreamSynchronize(context().stream);
-        if (e != hipSuccess) status = fail(VX_ERR_DEVICE, "step synchronize failed: %s", hipGetErrorString(e));
+    h.manual = true;
+    if (status == VX_OK && h.phase == P_FAILED)
+        status = fail(int32_t(h.status), h.status == VX_ERR_NONFINITE_RESIDUAL ? "HIP PCG produced a non-finite residual norm"
+                                                                                 : "HIP PCG did not converge within the refinement sweeps");
+    if (rep) {
+        std::memset(rep, 0, sizeof *rep);
+        rep->solves = h.solves;
+        rep->inner_iterations = uint32_t(h.inner);
+        rep->outer_sweeps = uint32_t(h.sweeps);
+        rep->active_rows = h.geo.active;
+        rep->last_relative_residual = h.last_rel;
+        rep->last_angle = h.last_angle;
+        rep->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
-    out.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    if (rep) *rep = out;
+    // Later stage calls see the substep length and the solve parameters.
+    h.params.fixed_dt = prm->fixed_dt / prm->substeps;
+    h.phase = P_IDLE;
     return status;
 }

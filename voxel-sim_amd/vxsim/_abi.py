@@ -21,6 +21,11 @@ VX_EMPTY = 0xFFFFFFFF
 VX_SOLVE_MIXED = 0
 VX_SOLVE_F64 = 1
 
+VX_SCENE_FREE = 0
+VX_SCENE_RUNNING = 1
+VX_SCENE_DONE = 2
+VX_SCENE_FAILED = 3
+
 
 class SceneDesc(ctypes.Structure):
     _fields_ = [
@@ -68,6 +73,15 @@ class StepReport(ctypes.Structure):
     ]
 
 
+class BatchSceneReport(ctypes.Structure):
+    _fields_ = [
+        ("state", ctypes.c_uint32), ("status", ctypes.c_uint32), ("steps_done", ctypes.c_uint32),
+        ("solves", ctypes.c_uint32), ("inner_iterations", ctypes.c_uint64), ("outer_sweeps", ctypes.c_uint64),
+        ("rows", ctypes.c_uint32), ("active_rows", ctypes.c_uint32), ("last_relative_residual", ctypes.c_double),
+        ("last_angle", ctypes.c_double),
+    ]
+
+
 # (name, restype, argtypes) for every exported function of include/vx_solver.h.
 SIGNATURES = [
     ("vx_last_error", ctypes.c_char_p, []),
@@ -90,6 +104,18 @@ SIGNATURES = [
     ("vx_scene_time_spmv", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint32, c_dp]),
     ("vx_scene_step", ctypes.c_int32,
      [ctypes.c_void_p, ctypes.POINTER(StepParams), ctypes.POINTER(StepReport)]),
+    ("vx_batch_bytes_per_row", ctypes.c_uint64, []),
+    ("vx_batch_create", ctypes.c_int32,
+     [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_void_p)]),
+    ("vx_batch_destroy", ctypes.c_int32, [ctypes.c_void_p]),
+    ("vx_batch_free_rows", ctypes.c_int32,
+     [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]),
+    ("vx_batch_admit", ctypes.c_int32,
+     [ctypes.c_void_p, ctypes.POINTER(SceneDesc), ctypes.POINTER(StepParams), ctypes.c_uint32, c_u32p]),
+    ("vx_batch_tick", ctypes.c_int32, [ctypes.c_void_p, c_u32p]),
+    ("vx_batch_report", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(BatchSceneReport)]),
+    ("vx_batch_download", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint32, c_dp, c_dp, c_dp]),
+    ("vx_batch_release", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint32]),
 ]
 
 

@@ -29,29 +29,43 @@ def _report(r) -> dict:
     return {f[0]: getattr(r, f[0]) for f in r._fields_}
 
 
+def scene_desc(scene):
+    """A vx_scene_desc over the scene's arrays; returns (desc, arrays to keep
+    alive while the description is in use)."""
+    nx, ny, nz = scene.dims
+    if nx * ny * nz != scene.n:
+        raise ValueError("scene is not a full box")
+    keep = []
+
+    def f(a):
+        if a is None:
+            return None
+        arr = np.ascontiguousarray(a, dtype=np.float32)
+        keep.append(arr)
+        return arr.ctypes.data_as(c_fp)
+
+    kin = np.ascontiguousarray(scene.kinematic, np.uint8)
+    nbr = np.ascontiguousarray(scene.neighbor, np.uint32)
+    keep += [kin, nbr]
+    desc = SceneDesc(
+        nx, ny, nz, f(scene.positions), f(scene.rotations), f(scene.velocities), f(scene.mass),
+        kin.ctypes.data_as(c_u8p), f(scene.young), f(scene.young_perp), f(scene.shear), f(scene.solid),
+        f(scene.fibers), f(scene.anisotropy), f(scene.delta_temp), f(scene.alpha_par), f(scene.alpha_perp),
+        nbr.ctypes.data_as(c_u32p), f(scene.damage), f(scene.kscale), f(scene.formation_offset),
+        f(scene.plastic_strain), f(scene.targets), f(scene.loads))
+    return desc, keep
+
+
+def step_params(fixed_dt=0.001, gravity=(0.0, 0.0, -9.81), substeps=1, corotation_iterations=2,
+                corotation_tolerance=1e-5, solve=None) -> StepParams:
+    return StepParams(float(fixed_dt), (ctypes.c_double * 3)(*gravity), int(substeps), int(corotation_iterations),
+                      float(corotation_tolerance), solve if solve is not None else solve_params())
+
+
 class GridScene:
     def __init__(self, scene):
+        desc, keep = scene_desc(scene)
         nx, ny, nz = scene.dims
-        if nx * ny * nz != scene.n:
-            raise ValueError("scene is not a full box")
-        keep = []
-
-        def f(a, shape=None):
-            if a is None:
-                return None
-            arr = np.ascontiguousarray(a, dtype=np.float32)
-            keep.append(arr)
-            return arr.ctypes.data_as(c_fp)
-
-        kin = np.ascontiguousarray(scene.kinematic, np.uint8)
-        nbr = np.ascontiguousarray(scene.neighbor, np.uint32)
-        keep += [kin, nbr]
-        desc = SceneDesc(
-            nx, ny, nz, f(scene.positions), f(scene.rotations), f(scene.velocities), f(scene.mass),
-            kin.ctypes.data_as(c_u8p), f(scene.young), f(scene.young_perp), f(scene.shear), f(scene.solid),
-            f(scene.fibers), f(scene.anisotropy), f(scene.delta_temp), f(scene.alpha_par), f(scene.alpha_perp),
-            nbr.ctypes.data_as(c_u32p), f(scene.damage), f(scene.kscale), f(scene.formation_offset),
-            f(scene.plastic_strain), f(scene.targets), f(scene.loads))
         handle = ctypes.c_void_p()
         check(lib().vx_scene_create(ctypes.byref(desc), ctypes.byref(handle)))
         self.handle = handle
@@ -107,8 +121,7 @@ class GridScene:
 
     def step(self, fixed_dt=0.001, gravity=(0.0, 0.0, -9.81), substeps=1, corotation_iterations=2,
              corotation_tolerance=1e-5, solve=None) -> dict:
-        sp = StepParams(float(fixed_dt), (ctypes.c_double * 3)(*gravity), int(substeps), int(corotation_iterations),
-                        float(corotation_tolerance), solve if solve is not None else solve_params())
+        sp = step_params(fixed_dt, gravity, substeps, corotation_iterations, corotation_tolerance, solve)
         rep = StepReport()
         check(lib().vx_scene_step(self.handle, ctypes.byref(sp), ctypes.byref(rep)))
         return _report(rep)
