@@ -132,18 +132,18 @@ typedef struct vx_scene vx_scene;
 
 typedef struct vx_solve_params {
     double tolerance;          /* outer FP64 relative true residual ||b - A x|| / ||b|| */
-    double inner_tolerance;    /* FP32 inner PCG relative residual per refinement sweep */
-    uint32_t max_inner;        /* inner iteration cap per sweep */
-    uint32_t max_outer;        /* refinement sweeps cap */
+    double inner_tolerance;    /* FP32 residual drop between FP64 residual replacements (1e-3) */
+    uint32_t max_inner;        /* inner iteration cap per solve */
+    uint32_t max_outer;        /* FP64 residual replacements cap */
     uint32_t warm_start;       /* 1: start from the scene's last correction */
     uint32_t mode;             /* VX_SOLVE_MIXED or VX_SOLVE_F64 */
 } vx_solve_params;
 
-#define VX_SOLVE_MIXED 0 /* FP32 inner PCG + FP64 refinement */
+#define VX_SOLVE_MIXED 0 /* FP32 PCG with FP64 residual replacement (reliable updates) */
 #define VX_SOLVE_F64 1   /* plain FP64 PCG on the structured layout */
 
 typedef struct vx_solve_report {
-    uint32_t inner_iterations; /* summed over sweeps */
+    uint32_t inner_iterations; /* inner PCG iterations of the solve */
     uint32_t outer_sweeps;     /* FP64 true-residual evaluations */
     uint32_t active_rows;
     uint32_t status;

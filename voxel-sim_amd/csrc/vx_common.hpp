@@ -86,18 +86,29 @@ __device__ __forceinline__ double wave_max(double v) {
     return v;
 }
 
+// Workgroup barrier that orders LDS only. Cross-thread data in these
+// reductions moves through LDS alone, so the barrier waits for LDS traffic
+// (lgkmcnt) but not for global loads or stores still in flight (a full
+// __syncthreads() would wait for vmcnt(0) and serialise the row loads issued
+// before it).
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Workgroup sum of N values, broadcast to every thread.
 template <int N>
 __device__ __forceinline__ void block_sum_all(double (&v)[N], double (*lds)[kWaves]) {
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    __syncthreads();  // a previous use of lds is finished
+    lds_barrier();  // a previous use of lds is finished
 #pragma unroll
     for (int k = 0; k < N; ++k) {
         double s = wave_sum(v[k]);
         if (lane == 0) lds[k][wave] = s;
     }
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int k = 0; k < N; ++k) {
         double s = lds[k][0];
@@ -115,21 +126,6 @@ __device__ __forceinline__ void store_partials(double (&v)[N], double *partials,
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int k = 0; k < N; ++k) partials[size_t(slot[k]) * stride + blockIdx.x] = v[k];
-    }
-}
-
-// Sum N partial arrays of `count` entries (written by the previous kernel)
-// within each wave: no LDS and no barrier, so loads issued before it stay in
-// flight. Every wave runs the same fixed-order sum and gets the same bits.
-template <int N>
-__device__ __forceinline__ void wave_sum_partials(double (&out)[N], const double *partials, const int (&slot)[N],
-                                                  uint32_t stride, uint32_t count) {
-    const uint32_t lane = threadIdx.x & 63;
-#pragma unroll
-    for (int k = 0; k < N; ++k) {
-        double s = 0.0;
-        for (uint32_t b = lane; b < count; b += 64) s += partials[size_t(slot[k]) * stride + b];
-        out[k] = wave_sum(s);
     }
 }
 

@@ -53,7 +53,9 @@ enum : uint32_t {
     B_FIT = 1u << 1,      // fit after predict
     B_ASSEMBLE = 1u << 2,
     B_RESID0 = 1u << 3,   // first residual of a solve (x0 = 0 or warm start)
-    B_RESID = 1u << 4,    // refinement residual (x += d first)
+    // B_INNER: one chunk of inner iterations, preceded by the refinement
+    // residual (x += d first) when the scene's inner solve has ended and its
+    // outer loop has not; that decision is taken on the device.
     B_INNER = 1u << 5,
     B_APPLY = 1u << 6,
     B_FITB = 1u << 7,     // fit after apply, records the angle
@@ -66,9 +68,11 @@ enum : uint32_t {
 enum : int { kPap = 0, kRz = 1, kRr = 2, kRr64 = 3, kBb = 4, kAngle = 5, kSlots = 6 };  // partial kinds
 enum : uint32_t { kRun = 0, kConv = 1, kNonFin = 2, kCap = 3 };
 
+// Box geometry of a scene and its tile range; one copy per slot and one per
+// tile (so a workgroup finds everything about its tile in one load).
 struct SlotGeo {
     uint32_t nx, ny, nz, n;
-    uint32_t tile0, ntiles, voff, active;
+    uint32_t tile0, ntiles, voff, slot;
 };
 struct SlotParams {
     double h, gx, gy, gz;
@@ -82,15 +86,19 @@ struct SlotState {
     double rr64, bb, rel, angle;         // outer residual, corotation angle
     uint32_t k_spmv, k_update, iters, done;
     uint32_t sweeps, outer_done, xpar, inner_total;
-    uint32_t nonfinite, pad[3];
+    uint32_t nonfinite;
+    uint32_t seg;   // iteration index at the last residual replacement
+    uint32_t pbuf;  // p buffer holding the latest search direction
+    uint32_t pad;
 };
 
 struct DevArena {
     uint32_t R, cap_tiles, chunk, max_slots;
-    const uint32_t *tile_slot;
-    const SlotGeo *geo;
+    const SlotGeo *tiles;  // per tile; slot == kFreeTile for a free tile
+    const SlotGeo *geo;    // per slot
     const SlotParams *prm;
-    const uint32_t *bits;
+    const uint32_t *bits;       // per slot, this tick
+    uint32_t *tile_bits;        // per tile: the owning slot's bits (0 if free)
     SlotState *st;
     double *partials;  // kSlots * cap_tiles
     // state
@@ -121,18 +129,31 @@ struct Tile {
     bool row;
 };
 
-__device__ __forceinline__ bool tile_for(const DevArena &A, uint32_t need, Tile &c) {
+// Copy each slot's bits of this tick to its tiles, so a workgroup reads its
+// tile's descriptor and bits side by side.
+__global__ __launch_bounds__(kBlock) void spread_bits(DevArena A) {
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= A.cap_tiles) return;
+    const uint32_t slot = A.tiles[t].slot;
+    A.tile_bits[t] = slot == kFreeTile ? 0u : A.bits[slot];
+}
+
+// The tile of this workgroup and its scene; false for a free tile. The
+// scene's phase bits are loaded but not tested.
+__device__ __forceinline__ bool tile_base(const DevArena &A, Tile &c) {
     c.t = logical_tile(blockIdx.x, A.chunk);
     if (c.t >= A.cap_tiles) return false;
-    c.slot = A.tile_slot[c.t];
+    c.g = A.tiles[c.t];
+    c.bits = A.tile_bits[c.t];
+    c.slot = c.g.slot;
     if (c.slot == kFreeTile) return false;
-    c.bits = A.bits[c.slot];
-    if (!(c.bits & need)) return false;
-    c.g = A.geo[c.slot];
     c.local = (c.t - c.g.tile0) * kBlock + threadIdx.x;
     c.row = c.local < c.g.n;
     c.v = c.g.voff + c.local;
     return true;
+}
+__device__ __forceinline__ bool tile_for(const DevArena &A, uint32_t need, Tile &c) {
+    return tile_base(A, c) && (c.bits & need);
 }
 
 __device__ __forceinline__ void coords(const SlotGeo &g, uint32_t local, uint32_t (&c)[3]) {
@@ -158,16 +179,29 @@ __device__ __forceinline__ Rev ld_rev(const DevArena &A, uint32_t v) {
 }
 
 // Per-scene sum of N partial kinds over the scene's tiles, broadcast to every
-// thread of the workgroup (fixed order).
+// thread of the workgroup (fixed order). scene_load is the per-thread part;
+// block_sum_all completes it.
+template <int N>
+__device__ __forceinline__ void scene_load(double (&out)[N], const DevArena &A, const SlotGeo &g, const int (&kind)[N]) {
+    // One load per thread covers scenes of up to kBlock tiles without a loop
+    // (a loop would wait for each load before the row loads are issued).
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        const double *part = A.partials + size_t(kind[k]) * A.cap_tiles + g.tile0;
+        out[k] = threadIdx.x < g.ntiles ? part[threadIdx.x] : 0.0;
+    }
+    if (g.ntiles > kBlock) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            const double *part = A.partials + size_t(kind[k]) * A.cap_tiles + g.tile0;
+            for (uint32_t b = threadIdx.x + kBlock; b < g.ntiles; b += kBlock) out[k] += part[b];
+        }
+    }
+}
 template <int N>
 __device__ __forceinline__ void scene_sum(double (&out)[N], const DevArena &A, const SlotGeo &g, const int (&kind)[N],
                                           double (*lds)[kWaves]) {
-#pragma unroll
-    for (int k = 0; k < N; ++k) {
-        double s = 0.0;
-        for (uint32_t b = threadIdx.x; b < g.ntiles; b += kBlock) s += A.partials[size_t(kind[k]) * A.cap_tiles + g.tile0 + b];
-        out[k] = s;
-    }
+    scene_load<N>(out, A, g, kind);
     block_sum_all<N>(out, lds);
 }
 template <int N>
@@ -182,7 +216,7 @@ __device__ __forceinline__ void tile_store(double (&v)[N], const DevArena &A, ui
 
 // ---- physics kernels -------------------------------------------------------
 // Predict one substep (src/solver.rs:494-523, 542).
-__global__ __launch_bounds__(kBlock) void k_predict(DevArena A) {
+__global__ __launch_bounds__(kBlock) void substep_predict(DevArena A) {
     Tile c;
     if (!tile_for(A, B_PREDICT, c) || !c.row) return;
     const size_t R = A.R;
@@ -211,7 +245,7 @@ __global__ __launch_bounds__(kBlock) void k_predict(DevArena A) {
 
 // Rotation fit (src/solver.rs:327-417) in FP64; with `record` the per-tile
 // largest angle change goes to the kAngle partials.
-__global__ __launch_bounds__(kBlock) void k_fit(DevArena A, uint32_t need, int record) {
+__global__ __launch_bounds__(kBlock) void fit_rotations(DevArena A, uint32_t need, int record) {
     Tile c;
     if (!tile_for(A, need, c)) return;
     __shared__ double lds[1][kWaves];
@@ -268,7 +302,7 @@ __global__ __launch_bounds__(kBlock) void k_fit(DevArena A, uint32_t need, int r
     if (!record) return;
     angle = wave_max(angle);
     if ((threadIdx.x & 63) == 0) lds[0][threadIdx.x >> 6] = angle;
-    __syncthreads();
+    lds_barrier();
     if (threadIdx.x == 0) {
         double m = lds[0][0];
         for (int w = 1; w < kWaves; ++w) m = fmax(m, lds[0][w]);
@@ -279,7 +313,7 @@ __global__ __launch_bounds__(kBlock) void k_fit(DevArena A, uint32_t need, int r
 // Per +axis bond: rotated stiffness k and force k (p_j - p_i - rest); zero
 // for absent bonds and bonds whose both endpoints are fully fixed
 // (src/solver.rs:1595-1619).
-__global__ __launch_bounds__(kBlock) void k_bonds(DevArena A) {
+__global__ __launch_bounds__(kBlock) void assemble_bonds(DevArena A) {
     Tile c;
     if (!tile_for(A, B_ASSEMBLE, c) || !c.row) return;
     const size_t R = A.R;
@@ -317,7 +351,7 @@ __global__ __launch_bounds__(kBlock) void k_bonds(DevArena A) {
 // Per REV: D = m/h^2 I + sum k, rhs = (pred - p) m/h^2 + load +- forces in
 // constraint order (-axis bond, then +axis bond, per axis), projection
 // D <- P D P + I - P, rhs <- P rhs (src/solver.rs:1576-1636).
-__global__ __launch_bounds__(kBlock) void k_rows(DevArena A) {
+__global__ __launch_bounds__(kBlock) void assemble_rows(DevArena A) {
     Tile c;
     if (!tile_for(A, B_ASSEMBLE, c) || !c.row) return;
     const size_t R = A.R;
@@ -400,42 +434,104 @@ template <> struct Inner<double> {
 };
 
 template <class S>
-__device__ __forceinline__ bool inner_tile(const DevArena &A, Tile &c) {
-    return tile_for(A, B_INNER, c) && ((c.bits & B_F64) == Inner<S>::kBit);
+__device__ __forceinline__ bool inner_bits(uint32_t bits) {
+    return (bits & B_INNER) && ((bits & B_F64) == Inner<S>::kBit);
 }
 
 // Inner convergence check of the previous update (r.r <= tol^2 r0.r0 every
 // iteration), beta, then y = A p on the 7-point 3x3-block operator with
 // p = z + beta p_old formed on load for the row and its neighbours; partials
 // of p.Ap. `odd` selects the p ping-pong buffers.
+//
+// Issue order: the partials of the previous kernel first, then every row
+// load (zeros where a neighbour is absent), then the partial sums. The row
+// loads do not depend on beta, so they are in flight while the workgroup
+// reduces the partials (LDS-only barriers do not wait for them).
 template <class S>
-__global__ __launch_bounds__(kBlock) void k_spmv(DevArena A, int odd) {
+__global__ __launch_bounds__(kBlock) void pcg_bond_spmv(DevArena A, int odd) {
     Tile c;
-    if (!inner_tile<S>(A, c)) return;
+    if (!tile_base(A, c) || !inner_bits<S>(c.bits)) return;
     __shared__ double lds[2][kWaves];
     SlotState *st = A.st + c.slot;
     const uint32_t done_in = st->done;
     const uint32_t k = st->k_spmv;
+    const uint32_t seg = st->seg;
     const double rho_s[2] = {st->rho[0], st->rho[1]};
-    const double rr0_s = st->rr0;
+    const double rr0_s = st->rr0, bb = st->bb;
     const SlotParams prm = A.prm[c.slot];
     double sums[2];
     const int kinds[2] = {kRz, kRr};
-    scene_sum<2>(sums, A, c.g, kinds, lds);
+    scene_load<2>(sums, A, c.g, kinds);
+    __builtin_amdgcn_sched_barrier(0);
+    const size_t R = A.R;
+    const S *__restrict__ z = Inner<S>::z(A);
+    const S *__restrict__ p_old = Inner<S>::p(A, odd);
+    const S *__restrict__ K = Inner<S>::k(A);
+    const uint32_t v = c.row ? c.v : c.g.voff;  // rows past the scene load row 0 and discard
+    uint32_t cc[3];
+    coords(c.g, c.row ? c.local : 0u, cc);
+    S zz[7][3], pp[7][3], Dv[6], Kp[3][6], Km[3][6];
+    const uint8_t m = A.mask[v];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        zz[0][q] = z[q * R + v];
+        pp[0][q] = p_old[q * R + v];
+    }
+#pragma unroll
+    for (int q = 0; q < 6; ++q) Dv[q] = Inner<S>::d(A)[q * R + v];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const uint32_t s = stride_of(c.g, a);
+        const S *Ka = K + size_t(a) * 6 * R;
+        const bool lo = cc[a] > 0, hi = cc[a] + 1 < extent_of(c.g, a);
+        const uint32_t um = lo ? v - s : v, up = hi ? v + s : v;
+        // Absent neighbours load the row itself (in bounds, cached) and are
+        // zeroed after the loads, so no load sits behind a branch.
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            zz[1 + 2 * a][q] = z[q * R + um];
+            pp[1 + 2 * a][q] = p_old[q * R + um];
+            zz[2 + 2 * a][q] = z[q * R + up];
+            pp[2 + 2 * a][q] = p_old[q * R + up];
+        }
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            Km[a][q] = Ka[q * R + um];
+            Kp[a][q] = Ka[q * R + v];  // zero when the +axis bond is absent
+        }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const bool lo = cc[a] > 0, hi = cc[a] + 1 < extent_of(c.g, a);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            zz[1 + 2 * a][q] = lo ? zz[1 + 2 * a][q] : S(0);
+            pp[1 + 2 * a][q] = lo ? pp[1 + 2 * a][q] : S(0);
+            zz[2 + 2 * a][q] = hi ? zz[2 + 2 * a][q] : S(0);
+            pp[2 + 2 * a][q] = hi ? pp[2 + 2 * a][q] : S(0);
+        }
+#pragma unroll
+        for (int q = 0; q < 6; ++q) Km[a][q] = lo ? Km[a][q] : S(0);
+    }
+    block_sum_all<2>(sums, lds);
     if (done_in) return;
     const double rho = sums[0], rr = sums[1];
     const bool lead = c.t == c.g.tile0 && threadIdx.x == 0;
-    const double rr0 = k == 0 ? rr : rr0_s;
+    // A segment (the iterations since the last FP64 residual replacement)
+    // ends when its recurrence residual has dropped by inner_tol, or when it
+    // already meets the outer tolerance; the FP64 residual then decides.
+    const double rr0 = k == seg ? rr : rr0_s;
     uint32_t done = kRun;
     if (!isfinite(rr))
         done = kNonFin;
-    else if (!(rr > prm.inner_tol * prm.inner_tol * rr0))
+    else if (!(rr > prm.inner_tol * prm.inner_tol * rr0) || !(rr > prm.tol * prm.tol * bb))
         done = kConv;
     else if (k >= prm.max_inner)
         done = kCap;
     if (lead) {
         st->rr_last = rr;
-        if (k == 0) st->rr0 = rr;
+        if (k == seg) st->rr0 = rr;
     }
     if (done) {
         if (lead) {
@@ -449,69 +545,34 @@ __global__ __launch_bounds__(kBlock) void k_spmv(DevArena A, int odd) {
     if (lead) {
         st->rho[k & 1] = rho;
         st->k_update = k;
+        st->pbuf = odd ? 0u : 1u;
     }
-    const size_t R = A.R;
-    const S *__restrict__ z = Inner<S>::z(A);
-    const S *__restrict__ p_old = Inner<S>::p(A, odd);
-    S *__restrict__ p_new = Inner<S>::p(A, !odd);
     double part[1] = {0.0};
+    S p[7][3];
+#pragma unroll
+    for (int t = 0; t < 7; ++t)
+#pragma unroll
+        for (int q = 0; q < 3; ++q) p[t][q] = fmaS<S>(beta, pp[t][q], zz[t][q]);
+    S y0 = fmaS<S>(Dv[2], p[0][2], fmaS<S>(Dv[1], p[0][1], Dv[0] * p[0][0]));
+    S y1 = fmaS<S>(Dv[4], p[0][2], fmaS<S>(Dv[3], p[0][1], Dv[1] * p[0][0]));
+    S y2 = fmaS<S>(Dv[5], p[0][2], fmaS<S>(Dv[4], p[0][1], Dv[2] * p[0][0]));
+    S a0 = S(0), a1 = S(0), a2 = S(0);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const S *km = Km[a], *kp = Kp[a];
+        const S *pm = p[1 + 2 * a], *pq = p[2 + 2 * a];
+        a0 = fmaS<S>(km[2], pm[2], fmaS<S>(km[1], pm[1], fmaS<S>(km[0], pm[0], a0)));
+        a1 = fmaS<S>(km[4], pm[2], fmaS<S>(km[3], pm[1], fmaS<S>(km[1], pm[0], a1)));
+        a2 = fmaS<S>(km[5], pm[2], fmaS<S>(km[4], pm[1], fmaS<S>(km[2], pm[0], a2)));
+        a0 = fmaS<S>(kp[2], pq[2], fmaS<S>(kp[1], pq[1], fmaS<S>(kp[0], pq[0], a0)));
+        a1 = fmaS<S>(kp[4], pq[2], fmaS<S>(kp[3], pq[1], fmaS<S>(kp[1], pq[0], a1)));
+        a2 = fmaS<S>(kp[5], pq[2], fmaS<S>(kp[4], pq[1], fmaS<S>(kp[2], pq[0], a2)));
+    }
+    if (m & 1) y0 -= a0;
+    if (m & 2) y1 -= a1;
+    if (m & 4) y2 -= a2;
     if (c.row) {
-        const S *__restrict__ K = Inner<S>::k(A);
-        const uint32_t v = c.v;
-        uint32_t cc[3];
-        coords(c.g, c.local, cc);
-        // All loads first (zeros where a neighbour is absent), compute after.
-        S zz[7][3], pp[7][3], Dv[6], Kp[3][6], Km[3][6];
-        const uint8_t m = A.mask[v];
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            zz[0][q] = z[q * R + v];
-            pp[0][q] = p_old[q * R + v];
-        }
-#pragma unroll
-        for (int q = 0; q < 6; ++q) Dv[q] = Inner<S>::d(A)[q * R + v];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            const uint32_t s = stride_of(c.g, a);
-            const S *Ka = K + size_t(a) * 6 * R;
-            const bool lo = cc[a] > 0, hi = cc[a] + 1 < extent_of(c.g, a);
-            const uint32_t um = lo ? v - s : v, up = hi ? v + s : v;
-#pragma unroll
-            for (int q = 0; q < 3; ++q) {
-                zz[1 + 2 * a][q] = lo ? z[q * R + um] : S(0);
-                pp[1 + 2 * a][q] = lo ? p_old[q * R + um] : S(0);
-                zz[2 + 2 * a][q] = hi ? z[q * R + up] : S(0);
-                pp[2 + 2 * a][q] = hi ? p_old[q * R + up] : S(0);
-            }
-#pragma unroll
-            for (int q = 0; q < 6; ++q) {
-                Km[a][q] = lo ? Ka[q * R + um] : S(0);
-                Kp[a][q] = Ka[q * R + v];  // zero when the +axis bond is absent
-            }
-        }
-        S p[7][3];
-#pragma unroll
-        for (int t = 0; t < 7; ++t)
-#pragma unroll
-            for (int q = 0; q < 3; ++q) p[t][q] = fmaS<S>(beta, pp[t][q], zz[t][q]);
-        S y0 = fmaS<S>(Dv[2], p[0][2], fmaS<S>(Dv[1], p[0][1], Dv[0] * p[0][0]));
-        S y1 = fmaS<S>(Dv[4], p[0][2], fmaS<S>(Dv[3], p[0][1], Dv[1] * p[0][0]));
-        S y2 = fmaS<S>(Dv[5], p[0][2], fmaS<S>(Dv[4], p[0][1], Dv[2] * p[0][0]));
-        S a0 = S(0), a1 = S(0), a2 = S(0);
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            const S *km = Km[a], *kp = Kp[a];
-            const S *pm = p[1 + 2 * a], *pq = p[2 + 2 * a];
-            a0 = fmaS<S>(km[2], pm[2], fmaS<S>(km[1], pm[1], fmaS<S>(km[0], pm[0], a0)));
-            a1 = fmaS<S>(km[4], pm[2], fmaS<S>(km[3], pm[1], fmaS<S>(km[1], pm[0], a1)));
-            a2 = fmaS<S>(km[5], pm[2], fmaS<S>(km[4], pm[1], fmaS<S>(km[2], pm[0], a2)));
-            a0 = fmaS<S>(kp[2], pq[2], fmaS<S>(kp[1], pq[1], fmaS<S>(kp[0], pq[0], a0)));
-            a1 = fmaS<S>(kp[4], pq[2], fmaS<S>(kp[3], pq[1], fmaS<S>(kp[1], pq[0], a1)));
-            a2 = fmaS<S>(kp[5], pq[2], fmaS<S>(kp[4], pq[1], fmaS<S>(kp[2], pq[0], a2)));
-        }
-        if (m & 1) y0 -= a0;
-        if (m & 2) y1 -= a1;
-        if (m & 4) y2 -= a2;
+        S *__restrict__ p_new = Inner<S>::p(A, !odd);
         S *__restrict__ ap = Inner<S>::ap(A);
         p_new[v] = p[0][0];
         p_new[R + v] = p[0][1];
@@ -527,51 +588,57 @@ __global__ __launch_bounds__(kBlock) void k_spmv(DevArena A, int odd) {
     tile_store<1>(part, A, c.t, out, lds);
 }
 
-// alpha = rho / p.Ap; d += alpha p (d = alpha p on the first iteration);
-// r -= alpha Ap; z = Dinv r; partials of r.z and r.r.
+// alpha = rho / p.Ap; d += alpha p (d = alpha p on the first iteration of a
+// segment);
+// r -= alpha Ap; z = Dinv r; partials of r.z and r.r. Row loads are issued
+// before the p.Ap partial sum, as in the SpMV.
 template <class S>
-__global__ __launch_bounds__(kBlock) void k_update(DevArena A, int odd) {
+__global__ __launch_bounds__(kBlock) void pcg_update(DevArena A, int odd) {
     Tile c;
-    if (!inner_tile<S>(A, c)) return;
+    if (!tile_base(A, c) || !inner_bits<S>(c.bits)) return;
     __shared__ double lds[2][kWaves];
     SlotState *st = A.st + c.slot;
     const uint32_t done_in = st->done;
     const uint32_t k = st->k_update;
+    const uint32_t seg = st->seg;
     const double rho_s[2] = {st->rho[0], st->rho[1]};
     double pap[1];
     const int kinds[1] = {kPap};
-    scene_sum<1>(pap, A, c.g, kinds, lds);
+    scene_load<1>(pap, A, c.g, kinds);
+    __builtin_amdgcn_sched_barrier(0);
+    const size_t R = A.R;
+    const uint32_t v = c.row ? c.v : c.g.voff;
+    const S *__restrict__ p = Inner<S>::p(A, !odd);
+    const S *__restrict__ ap = Inner<S>::ap(A);
+    S *__restrict__ d = Inner<S>::dv(A);
+    S *__restrict__ r = Inner<S>::r(A);
+    S *__restrict__ z = Inner<S>::z(A);
+    const S *__restrict__ dinv = Inner<S>::dinv(A);
+    S pv[3], av[3], rv[3], dv[3], Di[6];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        pv[q] = p[q * R + v];
+        av[q] = ap[q * R + v];
+        rv[q] = r[q * R + v];
+        dv[q] = d[q * R + v];
+    }
+#pragma unroll
+    for (int q = 0; q < 6; ++q) Di[q] = dinv[q * R + v];
+    __builtin_amdgcn_sched_barrier(0);
+    block_sum_all<1>(pap, lds);
     if (done_in) return;
     const S alpha = pap[0] > 1e-300 ? S(rho_s[k & 1] / pap[0]) : S(0);
-    const bool first = k == 0;
+    const bool first = k == seg;  // d restarts after each replacement
     if (c.t == c.g.tile0 && threadIdx.x == 0) {
         st->k_spmv = k + 1;
         st->iters = k + 1;
     }
-    const size_t R = A.R;
     double part[2] = {0.0, 0.0};
     if (c.row) {
-        const uint32_t v = c.v;
-        const S *__restrict__ p = Inner<S>::p(A, !odd);
-        const S *__restrict__ ap = Inner<S>::ap(A);
-        S *__restrict__ d = Inner<S>::dv(A);
-        S *__restrict__ r = Inner<S>::r(A);
-        S *__restrict__ z = Inner<S>::z(A);
-        const S *__restrict__ dinv = Inner<S>::dinv(A);
-        S pv[3], av[3], rv[3], dv[3], Di[6];
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            pv[q] = p[q * R + v];
-            av[q] = ap[q * R + v];
-            rv[q] = r[q * R + v];
-            dv[q] = first ? S(0) : d[q * R + v];
-        }
-#pragma unroll
-        for (int q = 0; q < 6; ++q) Di[q] = dinv[q * R + v];
         S ri[3], zi[3];
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
-            d[q * R + v] = fmaS<S>(alpha, pv[q], dv[q]);
+            d[q * R + v] = fmaS<S>(alpha, pv[q], first ? S(0) : dv[q]);
             ri[q] = fmaS<S>(-alpha, av[q], rv[q]);
             r[q * R + v] = ri[q];
         }
@@ -587,20 +654,38 @@ __global__ __launch_bounds__(kBlock) void k_update(DevArena A, int odd) {
     tile_store<2>(part, A, c.t, out, lds);
 }
 
-// Outer FP64 refinement step. B_RESID0 starts a solve from zero (or the warm
-// start copy); B_RESID first adds the inner correction: x_new = x_old + d on
-// own and neighbour rows. r64 = b - A64 x_new; the inner system starts from
-// r = r64 in the inner precision, z = Dinv r, p_old = 0. Partials of
-// ||r64||^2, ||b||^2, r.z, r.r. x_new goes to the other ping-pong buffer.
+// Whether the residual kernels act on a scene: `forced` launches run the
+// first residual of a solve (B_RESID0); the launch at the head of an inner
+// chunk runs the refinement residual once the inner solve has ended and the
+// outer loop has not.
+__device__ __forceinline__ bool resid_runs(uint32_t bits, const SlotState *st, int forced) {
+    if (forced) return bits & B_RESID0;
+    return (bits & B_INNER) && st->done != kRun && !st->outer_done;
+}
+
+// FP64 residual replacement (mixed-precision PCG with reliable updates). The
+// first residual of a solve starts from zero (or the warm start copy) and
+// clears the search direction. A later one folds the segment's inner
+// correction into the solution, x_new = x_old + d (own and neighbour rows),
+// and keeps the search direction, moved to p buffer 0 where the next chunk's
+// SpMV reads it. r64 = b - A64 x_new replaces the inner residual: r = r64 in
+// the inner precision, z = Dinv r. Partials of ||r64||^2, ||b||^2, r.z, r.r.
+// x_new goes to the other ping-pong buffer.
 template <class S>
-__global__ __launch_bounds__(kBlock) void k_residual(DevArena A) {
+__global__ __launch_bounds__(kBlock) void refine_residual(DevArena A, int forced) {
     Tile c;
-    if (!tile_for(A, B_RESID0 | B_RESID, c) || ((c.bits & B_F64) != Inner<S>::kBit)) return;
+    if (!tile_base(A, c) || ((c.bits & B_F64) != Inner<S>::kBit)) return;
+    if (!resid_runs(c.bits, A.st + c.slot, forced)) return;
     __shared__ double lds[4][kWaves];
     const size_t R = A.R;
-    const uint32_t xpar = A.st[c.slot].xpar;
-    const bool first = c.bits & B_RESID0;
+    const SlotState *sst = A.st + c.slot;
+    const uint32_t xpar = sst->xpar;
+    const bool first = forced;
     const bool warm = first && (c.bits & B_WARM_USE);
+    // A segment that ended before its first update left d from the previous
+    // segment, which x_old already holds.
+    const bool use_d = !first && sst->iters > sst->seg;
+    const uint32_t pbuf = sst->pbuf;
     const double *__restrict__ x_old = warm ? A.xw : (xpar ? A.xb : A.xa);
     double *__restrict__ x_new = xpar ? A.xa : A.xb;
     const S *__restrict__ d = Inner<S>::dv(A);
@@ -612,7 +697,7 @@ __global__ __launch_bounds__(kBlock) void k_residual(DevArena A) {
         auto xval = [&](uint32_t u, double (&o)[3]) {
 #pragma unroll
             for (int q = 0; q < 3; ++q)
-                o[q] = first ? (warm ? x_old[q * R + u] : 0.0) : x_old[q * R + u] + double(d[q * R + u]);
+                o[q] = first ? (warm ? x_old[q * R + u] : 0.0) : x_old[q * R + u] + (use_d ? double(d[q * R + u]) : 0.0);
         };
         double xv[3];
         xval(v, xv);
@@ -639,6 +724,7 @@ __global__ __launch_bounds__(kBlock) void k_residual(DevArena A) {
         S *__restrict__ r = Inner<S>::r(A);
         S *__restrict__ z = Inner<S>::z(A);
         S *__restrict__ p0 = Inner<S>::p(A, 0);
+        const S *__restrict__ p1 = Inner<S>::p(A, 1);
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
             const double b = A.rhs[q * R + v];
@@ -646,7 +732,10 @@ __global__ __launch_bounds__(kBlock) void k_residual(DevArena A) {
             x_new[q * R + v] = xv[q];
             rs[q] = S(r64);
             r[q * R + v] = rs[q];
-            p0[q * R + v] = S(0);
+            if (first)
+                p0[q * R + v] = S(0);
+            else if (pbuf)
+                p0[q * R + v] = p1[q * R + v];
             part[0] = __builtin_fma(r64, r64, part[0]);
             part[1] = __builtin_fma(b, b, part[1]);
         }
@@ -662,13 +751,17 @@ __global__ __launch_bounds__(kBlock) void k_residual(DevArena A) {
     tile_store<4>(part, A, c.t, out, lds);
 }
 
-// One workgroup per scene after k_residual: the outer relative residual, the
-// refinement decision (converged, or out of sweeps), the ping-pong flip and
-// the reset of the inner solve.
-__global__ __launch_bounds__(kBlock) void k_reduce_residual(DevArena A) {
+// One workgroup per scene after refine_residual (same `forced`): the outer
+// relative residual, the decision (converged, or out of sweeps), the
+// ping-pong flip and the start of the next inner segment. The iteration
+// count k runs on across segments, so beta after a replacement is
+// (r.z)_new / (r.z) of the last iteration.
+template <class S>
+__global__ __launch_bounds__(kBlock) void refine_decide(DevArena A, int forced) {
     const uint32_t slot = blockIdx.x;
     const uint32_t bits = A.bits[slot];
-    if (!(bits & (B_RESID0 | B_RESID))) return;
+    SlotState *st = A.st + slot;
+    if ((bits & B_F64) != Inner<S>::kBit || !resid_runs(bits, st, forced)) return;
     __shared__ double lds[2][kWaves];
     const SlotGeo g = A.geo[slot];
     if (g.n == 0) return;
@@ -676,12 +769,16 @@ __global__ __launch_bounds__(kBlock) void k_reduce_residual(DevArena A) {
     const int kinds[2] = {kRr64, kBb};
     scene_sum<2>(s, A, g, kinds, lds);
     if (threadIdx.x != 0) return;
-    SlotState *st = A.st + slot;
     const SlotParams p = A.prm[slot];
-    const bool first = bits & B_RESID0;
+    const bool first = forced;
     const uint32_t sweeps = (first ? 0u : st->sweeps) + 1;
     const double rel = s[1] > 0.0 ? sqrt(s[0] / s[1]) : 0.0;
-    st->inner_total = first ? 0u : st->inner_total + st->iters;
+    const uint32_t k = first ? 0u : st->k_spmv;
+    // Restart the search direction (beta = 0 at the next iteration) when the
+    // true residual has drifted from the segment's recurrence residual,
+    // ||r_true|| > 3 ||r_recurrence||: the old direction no longer fits it.
+    if (!first && k > 0 && s[0] > 9.0 * st->rr_last) st->rho[(k - 1) & 1] = 0.0;
+    st->inner_total = first ? 0u : st->iters;
     st->rr64 = s[0];
     st->bb = s[1];
     st->rel = rel;
@@ -690,15 +787,17 @@ __global__ __launch_bounds__(kBlock) void k_reduce_residual(DevArena A) {
     st->nonfinite = !isfinite(s[0]);
     const bool outer_done = !isfinite(s[0]) || rel <= p.tol || sweeps > p.max_outer;
     st->outer_done = outer_done;
-    st->k_spmv = 0;
-    st->k_update = 0;
-    st->iters = 0;
+    st->k_spmv = k;
+    st->k_update = k;
+    st->iters = k;
+    st->seg = k;
+    st->pbuf = 0;
     st->done = outer_done ? kConv : kRun;
 }
 
 // positions += x on the free rows (src/solver.rs:1821-1823, 1921-1923); with
 // B_WARM_SAVE the correction is also kept as the next solve's warm start.
-__global__ __launch_bounds__(kBlock) void k_apply(DevArena A) {
+__global__ __launch_bounds__(kBlock) void apply_correction(DevArena A) {
     Tile c;
     if (!tile_for(A, B_APPLY | B_WARM_SAVE, c) || !c.row) return;
     const size_t R = A.R;
@@ -714,7 +813,7 @@ __global__ __launch_bounds__(kBlock) void k_apply(DevArena A) {
 }
 
 // One workgroup per scene: largest rotation change of the fit after apply.
-__global__ __launch_bounds__(kBlock) void k_reduce_angle(DevArena A) {
+__global__ __launch_bounds__(kBlock) void reduce_angle(DevArena A) {
     const uint32_t slot = blockIdx.x;
     if (!(A.bits[slot] & B_FITB)) return;
     const SlotGeo g = A.geo[slot];
@@ -725,7 +824,7 @@ __global__ __launch_bounds__(kBlock) void k_reduce_angle(DevArena A) {
     __shared__ double lds[kWaves];
     m = wave_max(m);
     if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = m;
-    __syncthreads();
+    lds_barrier();
     if (threadIdx.x == 0) {
         double best = lds[0];
         for (int w = 1; w < kWaves; ++w) best = fmax(best, lds[w]);
@@ -734,7 +833,7 @@ __global__ __launch_bounds__(kBlock) void k_reduce_angle(DevArena A) {
 }
 
 // Velocities after a substep (src/solver.rs:668-683), no contact.
-__global__ __launch_bounds__(kBlock) void k_finish(DevArena A) {
+__global__ __launch_bounds__(kBlock) void finish_velocities(DevArena A) {
     Tile c;
     if (!tile_for(A, B_FINISH, c) || !c.row) return;
     const size_t R = A.R;
@@ -765,6 +864,7 @@ enum Phase : uint32_t { P_IDLE = 0, P_START, P_SOLVE, P_APPLIED, P_FINISHED, P_D
 struct SlotHost {
     bool used = false;
     SlotGeo geo{};
+    uint32_t active = 0;
     vx_step_params params{};
     uint32_t steps = 0, steps_done = 0, substep = 0, pass = 0;
     Phase phase = P_IDLE;
@@ -777,6 +877,15 @@ struct SlotHost {
     bool manual = false;  // driven by the vx_scene_* stage calls
 };
 
+// One launched tick: the bits it ran with and the scene states read back
+// after it (both pinned), and the event that marks its end.
+struct Flight {
+    uint32_t *bits = nullptr;
+    SlotState *st = nullptr;
+    hipEvent_t done = nullptr;
+    bool active = false;
+};
+
 }  // namespace
 }  // namespace vx
 
@@ -786,15 +895,17 @@ struct vx_batch {
     void *alloc = nullptr;
     size_t alloc_bytes = 0;
     std::vector<vx::SlotHost> slots;
-    std::vector<uint32_t> owner;  // host copy of tile_slot
-    uint32_t *d_tile_slot = nullptr, *d_bits = nullptr;
-    vx::SlotGeo *d_geo = nullptr;
+    std::vector<vx::SlotGeo> tiles;    // host copy of the per-tile table
+    std::vector<vx::SlotState> known;  // last state read back per slot
+    std::vector<uint32_t> inflight;    // bits of each slot in the tick in flight
+    vx::SlotGeo *d_tiles = nullptr, *d_geo = nullptr;
+    uint32_t *d_bits = nullptr;
     vx::SlotParams *d_prm = nullptr;
     vx::SlotState *d_st = nullptr;
-    uint32_t *h_bits = nullptr;          // pinned
-    vx::SlotState *h_st = nullptr;       // pinned
+    // Two ticks alternate: fl[cur] is filled next, fl[cur ^ 1] may be in flight.
+    vx::Flight fl[2];
+    int cur = 0;
     hipGraphExec_t g_inner[2] = {nullptr, nullptr};  // f32, f64
-    bool owner_dirty = true;
 };
 
 struct vx_scene {
@@ -811,27 +922,27 @@ struct Lock {
     Lock() : g(context().mutex) {}
 };
 
-int32_t upload_tables(vx_batch *b) {
-    Context &ctx = context();
-    if (b->owner_dirty) {
-        VX_HIP(hipMemcpyAsync(b->d_tile_slot, b->owner.data(), sizeof(uint32_t) * b->cap_tiles, hipMemcpyHostToDevice, ctx.stream));
-        VX_HIP(hipStreamSynchronize(ctx.stream));
-        b->owner_dirty = false;
-    }
-    return VX_OK;
-}
-
+// One inner chunk: the refinement residual of scenes whose inner solve has
+// ended (device-decided), then kInnerChunk iterations.
 int32_t capture_inner(vx_batch *b, int f64) {
     Context &ctx = context();
     hipGraph_t graph;
+    const dim3 grid(b->cap_tiles), blk(kBlock), sg(b->max_slots);
     VX_HIP(hipStreamBeginCapture(ctx.stream, hipStreamCaptureModeThreadLocal));
+    if (f64) {
+        hipLaunchKernelGGL(refine_residual<double>, grid, blk, 0, ctx.stream, b->A, 0);
+        hipLaunchKernelGGL(refine_decide<double>, sg, blk, 0, ctx.stream, b->A, 0);
+    } else {
+        hipLaunchKernelGGL(refine_residual<float>, grid, blk, 0, ctx.stream, b->A, 0);
+        hipLaunchKernelGGL(refine_decide<float>, sg, blk, 0, ctx.stream, b->A, 0);
+    }
     for (uint32_t t = 0; t < kInnerChunk; ++t) {
         if (f64) {
-            hipLaunchKernelGGL(k_spmv<double>, dim3(b->cap_tiles), dim3(kBlock), 0, ctx.stream, b->A, int(t & 1));
-            hipLaunchKernelGGL(k_update<double>, dim3(b->cap_tiles), dim3(kBlock), 0, ctx.stream, b->A, int(t & 1));
+            hipLaunchKernelGGL(pcg_bond_spmv<double>, grid, blk, 0, ctx.stream, b->A, int(t & 1));
+            hipLaunchKernelGGL(pcg_update<double>, grid, blk, 0, ctx.stream, b->A, int(t & 1));
         } else {
-            hipLaunchKernelGGL(k_spmv<float>, dim3(b->cap_tiles), dim3(kBlock), 0, ctx.stream, b->A, int(t & 1));
-            hipLaunchKernelGGL(k_update<float>, dim3(b->cap_tiles), dim3(kBlock), 0, ctx.stream, b->A, int(t & 1));
+            hipLaunchKernelGGL(pcg_bond_spmv<float>, grid, blk, 0, ctx.stream, b->A, int(t & 1));
+            hipLaunchKernelGGL(pcg_update<float>, grid, blk, 0, ctx.stream, b->A, int(t & 1));
         }
     }
     VX_HIP(hipStreamEndCapture(ctx.stream, &graph));
@@ -841,58 +952,63 @@ int32_t capture_inner(vx_batch *b, int f64) {
     return VX_OK;
 }
 
-// Launch one tick for the bits in h_bits: the pre-inner kernels, one inner
-// chunk per precision in use, the post-inner kernels; then read back all
-// scene states.
-int32_t launch_tick(vx_batch *b) {
+// Enqueue one tick for the bits in f.bits: the pre-inner kernels, one inner
+// chunk per precision in use, the post-inner kernels, then the read-back of
+// all scene states into f.st. Does not wait.
+int32_t launch_tick(vx_batch *b, Flight &f) {
     Context &ctx = context();
     hipStream_t s = ctx.stream;
     uint32_t any = 0;
-    bool f32 = false, f64 = false;
+    bool inner[2] = {false, false}, resid0[2] = {false, false};
     for (uint32_t i = 0; i < b->max_slots; ++i) {
-        if (!b->slots[i].used) {
-            b->h_bits[i] = 0;
-            continue;
-        }
-        any |= b->h_bits[i];
-        if (b->h_bits[i] & B_INNER) (b->h_bits[i] & B_F64 ? f64 : f32) = true;
+        if (!b->slots[i].used) f.bits[i] = 0;
+        const uint32_t bi = f.bits[i];
+        any |= bi;
+        const int prec = (bi & B_F64) ? 1 : 0;
+        if (bi & B_INNER) inner[prec] = true;
+        if (bi & B_RESID0) resid0[prec] = true;
     }
-    if (int32_t c = upload_tables(b)) return c;
-    VX_HIP(hipMemcpyAsync(b->d_bits, b->h_bits, sizeof(uint32_t) * b->max_slots, hipMemcpyHostToDevice, s));
+    VX_HIP(hipMemcpyAsync(b->d_bits, f.bits, sizeof(uint32_t) * b->max_slots, hipMemcpyHostToDevice, s));
     const dim3 grid(b->cap_tiles), blk(kBlock), sg(b->max_slots);
-    if (any & B_PREDICT) hipLaunchKernelGGL(k_predict, grid, blk, 0, s, b->A);
-    if (any & B_FIT) hipLaunchKernelGGL(k_fit, grid, blk, 0, s, b->A, uint32_t(B_FIT), 0);
+    hipLaunchKernelGGL(spread_bits, dim3((b->cap_tiles + kBlock - 1) / kBlock), blk, 0, s, b->A);
+    if (any & B_PREDICT) hipLaunchKernelGGL(substep_predict, grid, blk, 0, s, b->A);
+    if (any & B_FIT) hipLaunchKernelGGL(fit_rotations, grid, blk, 0, s, b->A, uint32_t(B_FIT), 0);
     if (any & B_ASSEMBLE) {
-        hipLaunchKernelGGL(k_bonds, grid, blk, 0, s, b->A);
-        hipLaunchKernelGGL(k_rows, grid, blk, 0, s, b->A);
+        hipLaunchKernelGGL(assemble_bonds, grid, blk, 0, s, b->A);
+        hipLaunchKernelGGL(assemble_rows, grid, blk, 0, s, b->A);
     }
-    if (any & (B_RESID0 | B_RESID)) {
-        hipLaunchKernelGGL(k_residual<float>, grid, blk, 0, s, b->A);
-        if (any & B_F64) hipLaunchKernelGGL(k_residual<double>, grid, blk, 0, s, b->A);
-        hipLaunchKernelGGL(k_reduce_residual, sg, blk, 0, s, b->A);
+    if (resid0[0]) {
+        hipLaunchKernelGGL(refine_residual<float>, grid, blk, 0, s, b->A, 1);
+        hipLaunchKernelGGL(refine_decide<float>, sg, blk, 0, s, b->A, 1);
+    }
+    if (resid0[1]) {
+        hipLaunchKernelGGL(refine_residual<double>, grid, blk, 0, s, b->A, 1);
+        hipLaunchKernelGGL(refine_decide<double>, sg, blk, 0, s, b->A, 1);
     }
     VX_HIP(hipGetLastError());
     for (int prec = 0; prec < 2; ++prec) {
-        if (!(prec ? f64 : f32)) continue;
+        if (!inner[prec]) continue;
         if (!b->g_inner[prec])
             if (int32_t c = capture_inner(b, prec)) return c;
         VX_HIP(hipGraphLaunch(b->g_inner[prec], s));
     }
-    if (any & (B_APPLY | B_WARM_SAVE)) hipLaunchKernelGGL(k_apply, grid, blk, 0, s, b->A);
+    if (any & (B_APPLY | B_WARM_SAVE)) hipLaunchKernelGGL(apply_correction, grid, blk, 0, s, b->A);
     if (any & B_FITB) {
-        hipLaunchKernelGGL(k_fit, grid, blk, 0, s, b->A, uint32_t(B_FITB), 1);
-        hipLaunchKernelGGL(k_reduce_angle, sg, blk, 0, s, b->A);
+        hipLaunchKernelGGL(fit_rotations, grid, blk, 0, s, b->A, uint32_t(B_FITB), 1);
+        hipLaunchKernelGGL(reduce_angle, sg, blk, 0, s, b->A);
     }
-    if (any & B_FINISH) hipLaunchKernelGGL(k_finish, grid, blk, 0, s, b->A);
+    if (any & B_FINISH) hipLaunchKernelGGL(finish_velocities, grid, blk, 0, s, b->A);
     VX_HIP(hipGetLastError());
-    VX_HIP(hipMemcpyAsync(b->h_st, b->d_st, sizeof(SlotState) * b->max_slots, hipMemcpyDeviceToHost, s));
-    VX_HIP(hipStreamSynchronize(s));
+    VX_HIP(hipMemcpyAsync(f.st, b->d_st, sizeof(SlotState) * b->max_slots, hipMemcpyDeviceToHost, s));
+    VX_HIP(hipEventRecord(f.done, s));
+    f.active = true;
     return VX_OK;
 }
 
 uint32_t prec_bits(const vx_step_params &p) { return p.solve.mode == VX_SOLVE_F64 ? uint32_t(B_F64) : 0u; }
 
-// Bits of an automatically driven scene for its next tick.
+// Bits of an automatically driven scene for its next tick, from its latest
+// read-back state.
 uint32_t next_bits(const SlotHost &h, const SlotState &st) {
     const uint32_t prec = prec_bits(h.params);
     switch (h.phase) {
@@ -911,7 +1027,6 @@ uint32_t next_bits(const SlotHost &h, const SlotState &st) {
             b |= (h.pass + 1 < h.params.corotation_iterations) ? uint32_t(B_FITB) : uint32_t(B_FINISH);
             return b;
         }
-        if (st.done != kRun) return B_RESID | B_INNER | prec;
         return B_INNER | prec;
     case P_APPLIED:
         if (h.finish_next) return B_FINISH;
@@ -934,7 +1049,7 @@ void finish_substep(SlotHost &h) {
 
 // Advance an automatically driven scene after a tick that ran `bits`.
 void advance(SlotHost &h, const SlotState &st, uint32_t bits) {
-    if (bits & (B_RESID0 | B_RESID)) h.last_rel = st.rel;
+    if (bits & (B_RESID0 | B_INNER)) h.last_rel = st.rel;
     if (st.nonfinite || st.done == kNonFin) {
         h.phase = P_FAILED;
         h.status = VX_ERR_NONFINITE_RESIDUAL;
@@ -968,6 +1083,38 @@ void advance(SlotHost &h, const SlotState &st, uint32_t bits) {
     }
 }
 
+// Wait for a launched tick and take its states in: the scenes it ran adopt
+// their new state and automatic scenes advance their phase.
+int32_t retire(vx_batch *b, Flight &f) {
+    if (!f.active) return VX_OK;
+    f.active = false;
+    VX_HIP(hipEventSynchronize(f.done));
+    for (uint32_t i = 0; i < b->max_slots; ++i) {
+        if (!f.bits[i] || !b->slots[i].used) continue;
+        b->known[i] = f.st[i];
+        if (!b->slots[i].manual) advance(b->slots[i], b->known[i], f.bits[i]);
+    }
+    return VX_OK;
+}
+
+// Retire the tick in flight, if any; afterwards nothing of this arena runs.
+int32_t drain(vx_batch *b) {
+    Flight &f = b->fl[b->cur ^ 1];
+    if (int32_t c = retire(b, f)) return c;
+    std::fill(b->inflight.begin(), b->inflight.end(), 0u);
+    return VX_OK;
+}
+
+// A synchronous tick with fixed bits for one slot.
+int32_t manual_tick(vx_batch *b, uint32_t slot, uint32_t bits) {
+    if (int32_t c = drain(b)) return c;
+    Flight &f = b->fl[b->cur];
+    std::fill(f.bits, f.bits + b->max_slots, 0u);
+    f.bits[slot] = bits;
+    if (int32_t c = launch_tick(b, f)) return c;
+    return retire(b, f);
+}
+
 int32_t create_batch(uint64_t capacity_rows, uint32_t max_scenes, uint32_t chunk, vx_batch **out) {
     if (!out || capacity_rows == 0 || max_scenes == 0) return fail(VX_ERR_INVALID_ARGUMENT, "invalid batch size");
     *out = nullptr;
@@ -984,7 +1131,11 @@ int32_t create_batch(uint64_t capacity_rows, uint32_t max_scenes, uint32_t chunk
     b->chunk = chunk;
     b->max_slots = max_scenes;
     b->slots.assign(max_scenes, SlotHost{});
-    b->owner.assign(b->cap_tiles, kFreeTile);
+    SlotGeo free_tile{};
+    free_tile.slot = kFreeTile;
+    b->tiles.assign(b->cap_tiles, free_tile);
+    b->known.assign(max_scenes, SlotState{});
+    b->inflight.assign(max_scenes, 0u);
     DevArena &A = b->A;
     const size_t R = size_t(b->cap_tiles) * kBlock;
     A.R = uint32_t(R);
@@ -1009,7 +1160,8 @@ int32_t create_batch(uint64_t capacity_rows, uint32_t max_scenes, uint32_t chunk
         {(void **)&A.p0, 12 * R}, {(void **)&A.p1, 12 * R}, {(void **)&A.ap, 12 * R}, {(void **)&A.dd, 24 * R},
         {(void **)&A.rd, 24 * R}, {(void **)&A.zd, 24 * R}, {(void **)&A.p0d, 24 * R}, {(void **)&A.p1d, 24 * R},
         {(void **)&A.apd, 24 * R}, {(void **)&A.partials, 8 * kSlots * size_t(b->cap_tiles)},
-        {(void **)&b->d_tile_slot, 4 * size_t(b->cap_tiles)}, {(void **)&b->d_bits, 4 * size_t(max_scenes)},
+        {(void **)&b->d_tiles, sizeof(SlotGeo) * b->cap_tiles}, {(void **)&A.tile_bits, 4 * size_t(b->cap_tiles)},
+        {(void **)&b->d_bits, 4 * size_t(max_scenes)},
         {(void **)&b->d_geo, sizeof(SlotGeo) * max_scenes}, {(void **)&b->d_prm, sizeof(SlotParams) * max_scenes},
         {(void **)&b->d_st, sizeof(SlotState) * max_scenes},
     };
@@ -1027,24 +1179,40 @@ int32_t create_batch(uint64_t capacity_rows, uint32_t max_scenes, uint32_t chunk
         *it.ptr = static_cast<char *>(b->alloc) + off;
         off += (it.bytes + 255) & ~size_t(255);
     }
-    A.tile_slot = b->d_tile_slot;
+    A.tiles = b->d_tiles;
     A.geo = b->d_geo;
     A.prm = b->d_prm;
     A.bits = b->d_bits;
     A.st = b->d_st;
     Context &ctx = context();
-    if (hipMemsetAsync(b->d_geo, 0, sizeof(SlotGeo) * max_scenes, ctx.stream) != hipSuccess ||
-        hipMemsetAsync(b->d_st, 0, sizeof(SlotState) * max_scenes, ctx.stream) != hipSuccess ||
-        hipMemsetAsync(A.partials, 0, 8 * kSlots * size_t(b->cap_tiles), ctx.stream) != hipSuccess ||
-        hipStreamSynchronize(ctx.stream) != hipSuccess ||
-        hipHostMalloc((void **)&b->h_bits, sizeof(uint32_t) * max_scenes, hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void **)&b->h_st, sizeof(SlotState) * max_scenes, hipHostMallocDefault) != hipSuccess) {
+    // Every slot starts idle: no inner work (done = converged) until its
+    // first residual.
+    std::vector<SlotState> st0(max_scenes);
+    for (auto &s : st0) s.done = kConv;
+    bool ok = hipMemcpy(b->d_tiles, b->tiles.data(), sizeof(SlotGeo) * b->cap_tiles, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(b->d_st, st0.data(), sizeof(SlotState) * max_scenes, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemsetAsync(b->d_geo, 0, sizeof(SlotGeo) * max_scenes, ctx.stream) == hipSuccess &&
+              hipMemsetAsync(b->d_bits, 0, 4 * size_t(max_scenes), ctx.stream) == hipSuccess &&
+              hipMemsetAsync(A.tile_bits, 0, 4 * size_t(b->cap_tiles), ctx.stream) == hipSuccess &&
+              hipMemsetAsync(A.partials, 0, 8 * kSlots * size_t(b->cap_tiles), ctx.stream) == hipSuccess &&
+              hipStreamSynchronize(ctx.stream) == hipSuccess;
+    for (auto &f : b->fl) {
+        ok = ok && hipHostMalloc((void **)&f.bits, sizeof(uint32_t) * max_scenes, hipHostMallocDefault) == hipSuccess &&
+             hipHostMalloc((void **)&f.st, sizeof(SlotState) * max_scenes, hipHostMallocDefault) == hipSuccess &&
+             hipEventCreateWithFlags(&f.done, hipEventDisableTiming) == hipSuccess;
+        if (f.bits) std::memset(f.bits, 0, sizeof(uint32_t) * max_scenes);
+    }
+    b->known = st0;
+    if (!ok) {
+        for (auto &f : b->fl) {
+            if (f.bits) (void)hipHostFree(f.bits);
+            if (f.st) (void)hipHostFree(f.st);
+            if (f.done) (void)hipEventDestroy(f.done);
+        }
         (void)hipFree(b->alloc);
         delete b;
         return fail(VX_ERR_DEVICE, "arena set-up failed");
     }
-    std::memset(b->h_bits, 0, sizeof(uint32_t) * max_scenes);
-    std::memset(b->h_st, 0, sizeof(SlotState) * max_scenes);
     *out = b;
     return VX_OK;
 }
@@ -1094,7 +1262,7 @@ int32_t check_desc(const vx_scene_desc *d, std::vector<uint8_t> &bonds, std::vec
 uint32_t find_tiles(const vx_batch *b, uint32_t need) {
     uint32_t run = 0;
     for (uint32_t t = 0; t < b->cap_tiles; ++t) {
-        run = b->owner[t] == kFreeTile ? run + 1 : 0;
+        run = b->tiles[t].slot == kFreeTile ? run + 1 : 0;
         if (run == need) return t + 1 - need;
     }
     return b->cap_tiles;
@@ -1103,6 +1271,14 @@ uint32_t find_tiles(const vx_batch *b, uint32_t need) {
 int32_t put(void *dst, const void *src, size_t bytes) {
     hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
     return e == hipSuccess ? VX_OK : fail(VX_ERR_DEVICE, "scene upload failed: %s", hipGetErrorString(e));
+}
+
+// Set the per-tile table entries of tiles [t0, t0 + count) to `g` (or free).
+int32_t set_tiles(vx_batch *b, uint32_t t0, uint32_t count, const SlotGeo *g) {
+    SlotGeo free_tile{};
+    free_tile.slot = kFreeTile;
+    for (uint32_t t = t0; t < t0 + count; ++t) b->tiles[t] = g ? *g : free_tile;
+    return put(b->d_tiles + t0, b->tiles.data() + t0, sizeof(SlotGeo) * count);
 }
 
 int32_t upload_scene(vx_batch *b, const vx_scene_desc *d, const SlotGeo &g, const std::vector<uint8_t> &bonds,
@@ -1227,24 +1403,28 @@ int32_t admit(vx_batch *b, const vx_scene_desc *d, const vx_step_params *p, uint
     const uint32_t need = tiles_for(n);
     const uint32_t t0 = find_tiles(b, need);
     if (t0 == b->cap_tiles) return fail(VX_ERR_TOO_LARGE, "no contiguous free range of %u tiles", need);
-    const SlotGeo g{d->nx, d->ny, d->nz, n, t0, need, t0 * kBlock, active};
+    // Nothing of the arena may run while its tables change.
+    if (int32_t c = drain(b)) return c;
+    const SlotGeo g{d->nx, d->ny, d->nz, n, t0, need, t0 * kBlock, slot};
     if (int32_t c = upload_scene(b, d, g, bonds, tmask, mask)) return c;
+    const SlotParams sp = p ? device_params(p->fixed_dt / p->substeps, p->gravity, &p->solve) : SlotParams{};
+    SlotState st{};
+    st.done = kConv;
+    VX_HIP(hipMemcpy(b->d_geo + slot, &g, sizeof g, hipMemcpyHostToDevice));
+    VX_HIP(hipMemcpy(b->d_prm + slot, &sp, sizeof sp, hipMemcpyHostToDevice));
+    VX_HIP(hipMemcpy(b->d_st + slot, &st, sizeof st, hipMemcpyHostToDevice));
+    if (int32_t c = set_tiles(b, t0, need, &g)) return c;
     SlotHost &h = b->slots[slot];
     h = SlotHost{};
     h.used = true;
     h.geo = g;
+    h.active = active;
     h.manual = manual;
     if (p) h.params = *p;
     h.steps = steps;
     h.phase = manual ? P_IDLE : (steps ? P_START : P_DONE);
-    for (uint32_t t = t0; t < t0 + need; ++t) b->owner[t] = slot;
-    b->owner_dirty = true;
-    const SlotParams sp = p ? device_params(p->fixed_dt / p->substeps, p->gravity, &p->solve) : SlotParams{};
-    const SlotState st{};
-    VX_HIP(hipMemcpy(b->d_geo + slot, &g, sizeof g, hipMemcpyHostToDevice));
-    VX_HIP(hipMemcpy(b->d_prm + slot, &sp, sizeof sp, hipMemcpyHostToDevice));
-    VX_HIP(hipMemcpy(b->d_st + slot, &st, sizeof st, hipMemcpyHostToDevice));
-    b->h_st[slot] = st;
+    b->known[slot] = st;
+    b->inflight[slot] = 0;
     *out_slot = slot;
     return VX_OK;
 }
@@ -1252,6 +1432,7 @@ int32_t admit(vx_batch *b, const vx_scene_desc *d, const vx_step_params *p, uint
 // Device parameters of one slot; host copies keep the substep length in
 // params.fixed_dt and the gravity for later stage calls.
 int32_t set_params(vx_batch *b, uint32_t slot, double h, const double *g, const vx_solve_params *s) {
+    if (int32_t c = drain(b)) return c;
     const SlotParams sp = device_params(h, g, s);
     vx_step_params &hp = b->slots[slot].params;
     hp.fixed_dt = h;
@@ -1260,13 +1441,6 @@ int32_t set_params(vx_batch *b, uint32_t slot, double h, const double *g, const 
     if (s) hp.solve = *s;
     VX_HIP(hipMemcpy(b->d_prm + slot, &sp, sizeof sp, hipMemcpyHostToDevice));
     return VX_OK;
-}
-
-// One tick with fixed bits for one manually driven slot.
-int32_t manual_tick(vx_batch *b, uint32_t slot, uint32_t bits) {
-    for (uint32_t i = 0; i < b->max_slots; ++i) b->h_bits[i] = 0;
-    b->h_bits[slot] = bits;
-    return launch_tick(b);
 }
 
 int32_t download_planes(const double *dev, size_t R, size_t voff, size_t n, int comps, double *out) {
@@ -1297,11 +1471,15 @@ extern "C" int32_t vx_batch_destroy(vx_batch *b) {
     clear_error();
     if (!b) return VX_OK;
     Lock lock;
+    (void)drain(b);
     (void)hipStreamSynchronize(context().stream);
     for (auto &g : b->g_inner)
         if (g) (void)hipGraphExecDestroy(g);
-    if (b->h_bits) (void)hipHostFree(b->h_bits);
-    if (b->h_st) (void)hipHostFree(b->h_st);
+    for (auto &f : b->fl) {
+        if (f.bits) (void)hipHostFree(f.bits);
+        if (f.st) (void)hipHostFree(f.st);
+        if (f.done) (void)hipEventDestroy(f.done);
+    }
     if (b->alloc) (void)hipFree(b->alloc);
     delete b;
     return VX_OK;
@@ -1313,7 +1491,7 @@ extern "C" int32_t vx_batch_free_rows(vx_batch *b, uint64_t *largest, uint64_t *
     Lock lock;
     uint64_t run = 0, best = 0, tot = 0;
     for (uint32_t t = 0; t < b->cap_tiles; ++t) {
-        if (b->owner[t] == kFreeTile) {
+        if (b->tiles[t].slot == kFreeTile) {
             run += 1;
             tot += 1;
             best = std::max(best, run);
@@ -1333,28 +1511,38 @@ extern "C" int32_t vx_batch_admit(vx_batch *b, const vx_scene_desc *d, const vx_
     return admit(b, d, p, steps, false, out_slot);
 }
 
+// One scheduler tick with one tick of lookahead: the new tick is decided from
+// the states read back after the tick before the one in flight, and enqueued
+// before that one is waited for. A scene whose tick in flight is a pure
+// inner chunk continues with another chunk (which does nothing if its solve
+// has ended in the meantime); a scene whose tick in flight changes its phase
+// sits this tick out.
 extern "C" int32_t vx_batch_tick(vx_batch *b, uint32_t *out_running) {
     clear_error();
     if (!b) return fail(VX_ERR_INVALID_ARGUMENT, "null batch");
     Lock lock;
-    uint32_t running = 0;
+    Flight &f = b->fl[b->cur];
+    Flight &prev = b->fl[b->cur ^ 1];
+    uint32_t launched = 0;
     for (uint32_t i = 0; i < b->max_slots; ++i) {
         const SlotHost &h = b->slots[i];
-        b->h_bits[i] = (h.used && !h.manual) ? next_bits(h, b->h_st[i]) : 0u;
-        running += b->h_bits[i] ? 1 : 0;
+        uint32_t bits = 0;
+        if (h.used && !h.manual && !(b->inflight[i] & ~uint32_t(B_INNER | B_F64))) bits = next_bits(h, b->known[i]);
+        f.bits[i] = bits;
+        launched += bits ? 1 : 0;
     }
-    if (running) {
-        if (int32_t c = launch_tick(b)) return c;
-        for (uint32_t i = 0; i < b->max_slots; ++i) {
-            SlotHost &h = b->slots[i];
-            if (h.used && !h.manual && b->h_bits[i]) advance(h, b->h_st[i], b->h_bits[i]);
-        }
-    }
+    if (launched)
+        if (int32_t c = launch_tick(b, f)) return c;
+    if (int32_t c = retire(b, prev)) return c;
+    for (uint32_t i = 0; i < b->max_slots; ++i) b->inflight[i] = launched ? f.bits[i] : 0u;
+    b->cur ^= 1;
     uint32_t still = 0;
     for (uint32_t i = 0; i < b->max_slots; ++i) {
         const SlotHost &h = b->slots[i];
         if (h.used && !h.manual && h.phase != P_DONE && h.phase != P_FAILED) still += 1;
     }
+    if (!still)
+        if (int32_t c = drain(b)) return c;
     if (out_running) *out_running = still;
     return VX_OK;
 }
@@ -1373,7 +1561,7 @@ extern "C" int32_t vx_batch_report(vx_batch *b, uint32_t slot, vx_batch_scene_re
     r->inner_iterations = h.inner;
     r->outer_sweeps = h.sweeps;
     r->rows = h.geo.n;
-    r->active_rows = h.geo.active;
+    r->active_rows = h.active;
     r->last_relative_residual = h.last_rel;
     r->last_angle = h.last_angle;
     return VX_OK;
@@ -1383,7 +1571,7 @@ extern "C" int32_t vx_batch_download(vx_batch *b, uint32_t slot, double *positio
     clear_error();
     if (!b || slot >= b->max_slots || !b->slots[slot].used) return fail(VX_ERR_INVALID_ARGUMENT, "invalid slot");
     Lock lock;
-    VX_HIP(hipStreamSynchronize(context().stream));
+    if (int32_t c = drain(b)) return c;
     const SlotGeo &g = b->slots[slot].geo;
     const size_t R = b->A.R;
     if (positions)
@@ -1401,10 +1589,9 @@ extern "C" int32_t vx_batch_release(vx_batch *b, uint32_t slot) {
     Lock lock;
     SlotHost &h = b->slots[slot];
     if (!h.used) return VX_OK;
-    for (uint32_t t = h.geo.tile0; t < h.geo.tile0 + h.geo.ntiles; ++t) b->owner[t] = kFreeTile;
-    b->owner_dirty = true;
+    if (int32_t c = drain(b)) return c;
+    if (int32_t c = set_tiles(b, h.geo.tile0, h.geo.ntiles, nullptr)) return c;
     h = SlotHost{};
-    b->h_bits[slot] = 0;
     const SlotGeo g{};
     VX_HIP(hipMemcpy(b->d_geo + slot, &g, sizeof g, hipMemcpyHostToDevice));
     return VX_OK;
@@ -1429,7 +1616,7 @@ extern "C" int32_t vx_scene_create(const vx_scene_desc *desc, vx_scene **out) {
     vx_scene *sc = new vx_scene();
     sc->batch = b;
     sc->slot = slot;
-    sc->active = b->slots[slot].geo.active;
+    sc->active = b->slots[slot].active;
     *out = sc;
     return VX_OK;
 }
@@ -1457,7 +1644,7 @@ extern "C" int32_t vx_scene_fit_rotations(vx_scene *sc, double *out_max_angle) {
     if (int32_t c = scene_ok(sc)) return c;
     Lock lock;
     if (int32_t c = manual_tick(sc->batch, sc->slot, B_FITB)) return c;
-    if (out_max_angle) *out_max_angle = sc->batch->h_st[sc->slot].angle;
+    if (out_max_angle) *out_max_angle = sc->batch->known[sc->slot].angle;
     return VX_OK;
 }
 
@@ -1484,7 +1671,7 @@ extern "C" int32_t vx_scene_solve(vx_scene *sc, const vx_solve_params *prm, vx_s
     const uint32_t prec = prm->mode == VX_SOLVE_F64 ? uint32_t(B_F64) : 0u;
     uint32_t bits = B_RESID0 | B_INNER | prec | ((prm->warm_start && h.warm_ok) ? uint32_t(B_WARM_USE) : 0u);
     if (rep) std::memset(rep, 0, sizeof *rep);
-    const SlotState *st = b->h_st + sc->slot;
+    const SlotState *st = &b->known[sc->slot];
     for (;;) {
         if (int32_t c = manual_tick(b, sc->slot, bits)) return c;
         if (st->nonfinite || st->done == kNonFin) {
@@ -1492,7 +1679,7 @@ extern "C" int32_t vx_scene_solve(vx_scene *sc, const vx_solve_params *prm, vx_s
             return fail(VX_ERR_NONFINITE_RESIDUAL, "HIP PCG produced a non-finite residual norm");
         }
         if (st->outer_done) break;
-        bits = (st->done != kRun ? uint32_t(B_RESID) : 0u) | B_INNER | prec;
+        bits = B_INNER | prec;
     }
     // The solution becomes the warm start of a later solve of this scene.
     if (int32_t c = manual_tick(b, sc->slot, B_WARM_SAVE)) return c;
@@ -1539,7 +1726,7 @@ extern "C" int32_t vx_scene_download(vx_scene *sc, double *positions, double *ro
     if (int32_t c = vx_batch_download(b, sc->slot, positions, rotations, velocities)) return c;
     if (correction) {
         const SlotGeo &g = b->slots[sc->slot].geo;
-        const double *x = b->h_st[sc->slot].xpar ? b->A.xb : b->A.xa;
+        const double *x = b->known[sc->slot].xpar ? b->A.xb : b->A.xa;
         if (int32_t c = download_planes(x, b->A.R, g.voff, g.n, 3, correction)) return c;
     }
     return VX_OK;
@@ -1555,9 +1742,9 @@ extern "C" int32_t vx_scene_export_rows(vx_scene *sc, uint32_t *active_out, uint
     if (int32_t c = scene_ok(sc)) return c;
     Lock lock;
     vx_batch *b = sc->batch;
+    if (int32_t c = drain(b)) return c;
     const SlotGeo &G = b->slots[sc->slot].geo;
     const size_t n = G.n, R = b->A.R, voff = G.voff;
-    VX_HIP(hipStreamSynchronize(context().stream));
     std::vector<double> k(18 * n), d(6 * n), rhs(3 * n);
     std::vector<uint8_t> mask(n), bonds(n);
     for (int q = 0; q < 18; ++q) VX_HIP(hipMemcpy(k.data() + q * n, b->A.k64 + q * R + voff, 8 * n, hipMemcpyDeviceToHost));
@@ -1624,13 +1811,16 @@ extern "C" int32_t vx_scene_time_spmv(vx_scene *sc, uint32_t reps, double *out_u
     if (reps == 0 || !out_us) return fail(VX_ERR_INVALID_ARGUMENT, "reps must be positive");
     Lock lock;
     vx_batch *b = sc->batch;
+    if (int32_t c = drain(b)) return c;
     Context &ctx = context();
     // The timed launches run as first FP32 inner iterations of the scene (k = 0,
     // unit residual partials, never done); they keep the solution buffers and
     // leave the inner vectors' contents unspecified.
-    SlotState st = b->h_st[sc->slot];
+    SlotState st = b->known[sc->slot];
     st.k_spmv = 0;
     st.k_update = 0;
+    st.seg = 0;
+    st.iters = 0;
     st.done = kRun;
     st.rho[0] = st.rho[1] = 0.0;
     const vx_step_params p = b->slots[sc->slot].params;
@@ -1643,15 +1833,16 @@ extern "C" int32_t vx_scene_time_spmv(vx_scene *sc, uint32_t reps, double *out_u
     const std::vector<double> ones(g.ntiles, 1.0);
     VX_HIP(hipMemcpy(b->A.partials + size_t(kRz) * b->cap_tiles + g.tile0, ones.data(), 8 * g.ntiles, hipMemcpyHostToDevice));
     VX_HIP(hipMemcpy(b->A.partials + size_t(kRr) * b->cap_tiles + g.tile0, ones.data(), 8 * g.ntiles, hipMemcpyHostToDevice));
-    for (uint32_t i = 0; i < b->max_slots; ++i) b->h_bits[i] = 0;
-    b->h_bits[sc->slot] = B_INNER;
-    if (int32_t c = upload_tables(b)) return c;
-    VX_HIP(hipMemcpyAsync(b->d_bits, b->h_bits, sizeof(uint32_t) * b->max_slots, hipMemcpyHostToDevice, ctx.stream));
+    std::vector<uint32_t> bits(b->max_slots, 0u);
+    bits[sc->slot] = B_INNER;
+    VX_HIP(hipMemcpy(b->d_bits, bits.data(), sizeof(uint32_t) * b->max_slots, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(spread_bits, dim3((b->cap_tiles + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx.stream, b->A);
+    VX_HIP(hipGetLastError());
     std::vector<hipEvent_t> ev(2 * reps);
     for (auto &e : ev) VX_HIP(hipEventCreate(&e));
     for (uint32_t k = 0; k < reps; ++k) {
         VX_HIP(hipEventRecord(ev[2 * k], ctx.stream));
-        hipLaunchKernelGGL(k_spmv<float>, dim3(b->cap_tiles), dim3(kBlock), 0, ctx.stream, b->A, int(k & 1));
+        hipLaunchKernelGGL(pcg_bond_spmv<float>, dim3(b->cap_tiles), dim3(kBlock), 0, ctx.stream, b->A, int(k & 1));
         VX_HIP(hipEventRecord(ev[2 * k + 1], ctx.stream));
     }
     VX_HIP(hipStreamSynchronize(ctx.stream));
@@ -1664,7 +1855,7 @@ extern "C" int32_t vx_scene_time_spmv(vx_scene *sc, uint32_t reps, double *out_u
     for (auto &e : ev) (void)hipEventDestroy(e);
     *out_us = 1000.0 * total / reps;
     // Restore the scene's state word (solution buffer parity) and parameters.
-    st = b->h_st[sc->slot];
+    st = b->known[sc->slot];
     VX_HIP(hipMemcpy(b->d_st + sc->slot, &st, sizeof st, hipMemcpyHostToDevice));
     return set_params(b, sc->slot, p.fixed_dt, p.gravity, &p.solve);
 }
@@ -1680,8 +1871,8 @@ extern "C" int32_t vx_scene_step(vx_scene *sc, const vx_step_params *prm, vx_ste
     SlotHost &h = b->slots[sc->slot];
     const auto t0 = std::chrono::steady_clock::now();
     // Drive the slot through one step with the per-scene state machine.
-    const bool warm_ok = h.warm_ok;
     if (int32_t c = set_params(b, sc->slot, prm->fixed_dt / prm->substeps, prm->gravity, &prm->solve)) return c;
+    const bool warm_ok = h.warm_ok;
     h.params = *prm;
     h.manual = false;
     h.steps = 1;
@@ -1700,6 +1891,7 @@ extern "C" int32_t vx_scene_step(vx_scene *sc, const vx_step_params *prm, vx_ste
     while (running) {
         if ((status = vx_batch_tick(b, &running))) break;
     }
+    (void)drain(b);
     h.manual = true;
     if (status == VX_OK && h.phase == P_FAILED)
         status = fail(int32_t(h.status), h.status == VX_ERR_NONFINITE_RESIDUAL ? "HIP PCG produced a non-finite residual norm"
@@ -1709,7 +1901,7 @@ extern "C" int32_t vx_scene_step(vx_scene *sc, const vx_step_params *prm, vx_ste
         rep->solves = h.solves;
         rep->inner_iterations = uint32_t(h.inner);
         rep->outer_sweeps = uint32_t(h.sweeps);
-        rep->active_rows = h.geo.active;
+        rep->active_rows = h.active;
         rep->last_relative_residual = h.last_rel;
         rep->last_angle = h.last_angle;
         rep->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

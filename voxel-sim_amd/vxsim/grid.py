@@ -20,7 +20,7 @@ MIXED = VX_SOLVE_MIXED
 F64 = VX_SOLVE_F64
 
 
-def solve_params(tolerance=1e-9, inner_tolerance=1e-4, max_inner=4000, max_outer=12, warm_start=False, mode=MIXED):
+def solve_params(tolerance=1e-9, inner_tolerance=1e-3, max_inner=4000, max_outer=12, warm_start=False, mode=MIXED):
     return SolveParams(float(tolerance), float(inner_tolerance), int(max_inner), int(max_outer), 1 if warm_start else 0,
                        int(mode))
 
