@@ -34,7 +34,7 @@ INNER_TOL = 1e-3        # FP32 residual drop between FP64 residual replacements
 PEAK_HBM_GBS = 8000.0   # MI355X HBM3E
 SPMV_BYTES_PER_ROW = 145  # z 12 + p_old 12 + D 24 + 3 bond blocks 72 + mask 1 + p_new 12 + Ap 12
 SPMV_KERNEL = "pcg_bond_spmv<float>"
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pcg_bond_spmv_64cube_traffic.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pcg_bond_spmv_64cube_traffic.txt")
 COPY_BYTES = 1 << 30      # calibration copy for the HBM byte counters
 
 

@@ -95,8 +95,9 @@ int32_t vx_solve_pcg_f64(const float *rhs, const uint32_t *neighbors, const floa
 /*
  * A box scene of nx*ny*nz REVs kept resident in HBM: SoA state, on-device
  * rotation fit (src/solver.rs:327-417), bond assembly
- * (src/solver.rs:1542-1636), and a mixed-precision PCG (FP32 inner
- * block-Jacobi PCG with f64 reductions, FP64 true-residual refinement). The
+ * (src/solver.rs:1542-1636), and a mixed-precision PCG (FP32 block-Jacobi
+ * PCG with f64 reductions whose residual is replaced by the FP64 true
+ * residual every time it has dropped by inner_tolerance). The
  * reference has no counterpart beyond solve_pcg; these entry points carry the
  * same status/error convention.
  *

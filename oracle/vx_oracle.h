@@ -22,6 +22,12 @@
  * f64 dot accumulation, f32 alpha/beta). Mode VXO_F64 is the FP64 oracle:
  * the same formulas evaluated in double.
  *
+ * Parity unpinned at bit level: the f32 steps the reference takes through
+ * glam 0.27 (Quat::slerp, Quat::from_mat3, Mat3::from_quat, Mat3::inverse;
+ * src/solver.rs:101, 365, 383, 398, 1326, 1765) and rayon's f64 summation
+ * order cannot be run or read here. They are pinned to tolerance only, by
+ * the closed forms, reference tests and invariants checked in tests/.
+ *
  * Every array argument is double (or an integer type) regardless of mode; in
  * VXO_F32 mode inputs are rounded to float on entry and results are exact
  * float values widened to double.

@@ -438,6 +438,23 @@ __device__ __forceinline__ bool inner_bits(uint32_t bits) {
     return (bits & B_INNER) && ((bits & B_F64) == Inner<S>::kBit);
 }
 
+// The tile of an inner-iteration kernel. ONE: the arena holds a single scene
+// whose geometry comes in the kernel arguments (no descriptor load before
+// the row loads), and the chunk is only launched while that scene iterates.
+template <class S, bool ONE>
+__device__ __forceinline__ bool inner_tile(const DevArena &A, const SlotGeo &one, Tile &c) {
+    if (!ONE) return tile_base(A, c) && inner_bits<S>(c.bits);
+    c.t = logical_tile(blockIdx.x, A.chunk);
+    if (c.t >= one.ntiles) return false;
+    c.g = one;
+    c.slot = one.slot;
+    c.bits = B_INNER | Inner<S>::kBit;
+    c.local = c.t * kBlock + threadIdx.x;
+    c.row = c.local < c.g.n;
+    c.v = c.g.voff + c.local;
+    return true;
+}
+
 // Inner convergence check of the previous update (r.r <= tol^2 r0.r0 every
 // iteration), beta, then y = A p on the 7-point 3x3-block operator with
 // p = z + beta p_old formed on load for the row and its neighbours; partials
@@ -447,10 +464,10 @@ __device__ __forceinline__ bool inner_bits(uint32_t bits) {
 // load (zeros where a neighbour is absent), then the partial sums. The row
 // loads do not depend on beta, so they are in flight while the workgroup
 // reduces the partials (LDS-only barriers do not wait for them).
-template <class S>
-__global__ __launch_bounds__(kBlock) void pcg_bond_spmv(DevArena A, int odd) {
+template <class S, bool ONE>
+__global__ __launch_bounds__(kBlock) void pcg_bond_spmv(DevArena A, SlotGeo one, int odd) {
     Tile c;
-    if (!tile_base(A, c) || !inner_bits<S>(c.bits)) return;
+    if (!inner_tile<S, ONE>(A, one, c)) return;
     __shared__ double lds[2][kWaves];
     SlotState *st = A.st + c.slot;
     const uint32_t done_in = st->done;
@@ -592,10 +609,10 @@ __global__ __launch_bounds__(kBlock) void pcg_bond_spmv(DevArena A, int odd) {
 // segment);
 // r -= alpha Ap; z = Dinv r; partials of r.z and r.r. Row loads are issued
 // before the p.Ap partial sum, as in the SpMV.
-template <class S>
-__global__ __launch_bounds__(kBlock) void pcg_update(DevArena A, int odd) {
+template <class S, bool ONE>
+__global__ __launch_bounds__(kBlock) void pcg_update(DevArena A, SlotGeo one, int odd) {
     Tile c;
-    if (!tile_base(A, c) || !inner_bits<S>(c.bits)) return;
+    if (!inner_tile<S, ONE>(A, one, c)) return;
     __shared__ double lds[2][kWaves];
     SlotState *st = A.st + c.slot;
     const uint32_t done_in = st->done;
@@ -906,6 +923,7 @@ struct vx_batch {
     vx::Flight fl[2];
     int cur = 0;
     hipGraphExec_t g_inner[2] = {nullptr, nullptr};  // f32, f64
+    bool single = false;  // one-scene arena of a vx_scene (never re-admits)
 };
 
 struct vx_scene {
@@ -936,13 +954,23 @@ int32_t capture_inner(vx_batch *b, int f64) {
         hipLaunchKernelGGL(refine_residual<float>, grid, blk, 0, ctx.stream, b->A, 0);
         hipLaunchKernelGGL(refine_decide<float>, sg, blk, 0, ctx.stream, b->A, 0);
     }
+    // A one-scene arena (vx_scene_*) never changes its scene: its geometry
+    // goes into the kernel arguments.
+    const SlotGeo one = b->single ? b->slots[0].geo : SlotGeo{};
     for (uint32_t t = 0; t < kInnerChunk; ++t) {
-        if (f64) {
-            hipLaunchKernelGGL(pcg_bond_spmv<double>, grid, blk, 0, ctx.stream, b->A, int(t & 1));
-            hipLaunchKernelGGL(pcg_update<double>, grid, blk, 0, ctx.stream, b->A, int(t & 1));
+        const int odd = int(t & 1);
+        if (f64 && b->single) {
+            hipLaunchKernelGGL((pcg_bond_spmv<double, true>), grid, blk, 0, ctx.stream, b->A, one, odd);
+            hipLaunchKernelGGL((pcg_update<double, true>), grid, blk, 0, ctx.stream, b->A, one, odd);
+        } else if (f64) {
+            hipLaunchKernelGGL((pcg_bond_spmv<double, false>), grid, blk, 0, ctx.stream, b->A, one, odd);
+            hipLaunchKernelGGL((pcg_update<double, false>), grid, blk, 0, ctx.stream, b->A, one, odd);
+        } else if (b->single) {
+            hipLaunchKernelGGL((pcg_bond_spmv<float, true>), grid, blk, 0, ctx.stream, b->A, one, odd);
+            hipLaunchKernelGGL((pcg_update<float, true>), grid, blk, 0, ctx.stream, b->A, one, odd);
         } else {
-            hipLaunchKernelGGL(pcg_bond_spmv<float>, grid, blk, 0, ctx.stream, b->A, int(t & 1));
-            hipLaunchKernelGGL(pcg_update<float>, grid, blk, 0, ctx.stream, b->A, int(t & 1));
+            hipLaunchKernelGGL((pcg_bond_spmv<float, false>), grid, blk, 0, ctx.stream, b->A, one, odd);
+            hipLaunchKernelGGL((pcg_update<float, false>), grid, blk, 0, ctx.stream, b->A, one, odd);
         }
     }
     VX_HIP(hipStreamEndCapture(ctx.stream, &graph));
@@ -1613,6 +1641,7 @@ extern "C" int32_t vx_scene_create(const vx_scene_desc *desc, vx_scene **out) {
         vx_batch_destroy(b);
         return c;
     }
+    b->single = true;
     vx_scene *sc = new vx_scene();
     sc->batch = b;
     sc->slot = slot;
@@ -1838,21 +1867,31 @@ extern "C" int32_t vx_scene_time_spmv(vx_scene *sc, uint32_t reps, double *out_u
     VX_HIP(hipMemcpy(b->d_bits, bits.data(), sizeof(uint32_t) * b->max_slots, hipMemcpyHostToDevice));
     hipLaunchKernelGGL(spread_bits, dim3((b->cap_tiles + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx.stream, b->A);
     VX_HIP(hipGetLastError());
-    std::vector<hipEvent_t> ev(2 * reps);
-    for (auto &e : ev) VX_HIP(hipEventCreate(&e));
-    for (uint32_t k = 0; k < reps; ++k) {
-        VX_HIP(hipEventRecord(ev[2 * k], ctx.stream));
-        hipLaunchKernelGGL(pcg_bond_spmv<float>, dim3(b->cap_tiles), dim3(kBlock), 0, ctx.stream, b->A, int(k & 1));
-        VX_HIP(hipEventRecord(ev[2 * k + 1], ctx.stream));
-    }
+    // The launches run back to back inside one graph, as the solve runs them,
+    // between two events on the library stream.
+    hipGraph_t graph;
+    hipGraphExec_t exec = nullptr;
+    VX_HIP(hipStreamBeginCapture(ctx.stream, hipStreamCaptureModeThreadLocal));
+    for (uint32_t k = 0; k < reps; ++k)
+        hipLaunchKernelGGL((pcg_bond_spmv<float, true>), dim3(b->cap_tiles), dim3(kBlock), 0, ctx.stream, b->A, g, int(k & 1));
+    VX_HIP(hipStreamEndCapture(ctx.stream, &graph));
+    hipError_t ge = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ge != hipSuccess) return fail(VX_ERR_DEVICE, "hipGraphInstantiate failed: %s", hipGetErrorString(ge));
+    hipEvent_t ev[2];
+    VX_HIP(hipEventCreate(&ev[0]));
+    VX_HIP(hipEventCreate(&ev[1]));
+    VX_HIP(hipGraphLaunch(exec, ctx.stream));  // warm-up
+    VX_HIP(hipEventRecord(ev[0], ctx.stream));
+    VX_HIP(hipGraphLaunch(exec, ctx.stream));
+    VX_HIP(hipEventRecord(ev[1], ctx.stream));
     VX_HIP(hipStreamSynchronize(ctx.stream));
-    double total = 0.0;
-    for (uint32_t k = 0; k < reps; ++k) {
-        float ms = 0.0f;
-        VX_HIP(hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]));
-        total += ms;
-    }
-    for (auto &e : ev) (void)hipEventDestroy(e);
+    float ms = 0.0f;
+    VX_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+    const double total = ms;
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    (void)hipGraphExecDestroy(exec);
     *out_us = 1000.0 * total / reps;
     // Restore the scene's state word (solution buffer parity) and parameters.
     st = b->known[sc->slot];
