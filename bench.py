@@ -136,6 +136,49 @@ def batched(steps: int = 1, count: int = 256):
     return out
 
 
+def serve_bench(rate: float, seconds: float, rev_lo: float, rev_hi: float, steps: int, budget_gb: float,
+                compare: bool = True):
+    """Served request stream: Poisson arrivals, boxes log-uniform in REVs,
+    `steps` warm-started steps each, admission against an HBM budget.
+    Prints one JSON line with p50/p99 latency and aggregate DOF-PCG-iters/s,
+    beside the plain FP64 solve handling the same requests one at a time."""
+    from vxsim import batch, serve
+    t = time.perf_counter()
+    reqs = serve.request_stream(rate, seconds, rev_lo, rev_hi, steps, seed=4)
+    gen_s = time.perf_counter() - t
+    budget = int(budget_gb * (1 << 30))
+    with serve.Server.for_budget(budget, max_scenes=256) as S:
+        t0 = time.perf_counter()
+        S.run(reqs)
+        wall = time.perf_counter() - t0
+        ticks = S.ticks
+    done = [r for r in reqs if r.state == batch.DONE]
+    dof_iters = sum(3.0 * r.report["active_rows"] * (r.report["inner_iterations"] + r.report["outer_sweeps"])
+                    for r in reqs if r.report)
+    lat = serve.percentiles([r.latency for r in reqs])
+    revs = [r.scene.n for r in reqs]
+    line = {
+        "metric": "served p50/p99 solve latency", "unit": "s", "higher_is_better": False,
+        "p50": lat[50], "p99": lat[99], "requests": len(reqs), "done": len(done),
+        "dof_iters_per_s": dof_iters / wall, "wall_s": wall, "ticks": ticks,
+        "offered": {"rate_per_s": rate, "seconds": seconds, "revs_log_uniform": [rev_lo, rev_hi],
+                    "mean_revs": float(np.mean(revs)), "steps_per_request": steps,
+                    "rev_steps_per_s": rate * steps * float(np.mean(revs))},
+        "served_rev_steps_per_s": sum(r.scene.n * r.steps for r in done) / wall,
+        "hbm_budget_gb": budget_gb, "scene_generation_s": gen_s,
+        "data": "synthetic", "dtype": "f32 inner / f64 refinement and reductions",
+    }
+    if compare:
+        t0 = time.perf_counter()
+        base = serve.serve_alone_fp64(reqs)
+        bwall = time.perf_counter() - t0
+        bl = serve.percentiles([b["latency"] for b in base])
+        line["plain_fp64_one_at_a_time"] = {"p50": bl[50], "p99": bl[99], "wall_s": bwall,
+                                             "dof_iters_per_s": sum(b["dof_iters"] for b in base) / bwall,
+                                             "failed": sum(1 for b in base if not b["ok"])}
+    print(json.dumps(line))
+
+
 def pmc_driver():
     """Short run for the rocprofv3 counter passes: 20 launches of the FP32
     bond SpMV on the assembled 64^3 system, then 3 copies of 1 GiB (the
@@ -167,9 +210,20 @@ def main():
     ap.add_argument("--no-compare", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--pmc-driver", action="store_true", help="short SpMV + copy run for counter passes")
+    ap.add_argument("--serve", action="store_true", help="served request stream instead of the step benchmark")
+    ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--serve-revs", type=str, default="1e3,2e5")
+    ap.add_argument("--serve-steps", type=int, default=10)
+    ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     args = ap.parse_args()
     if args.pmc_driver:
         pmc_driver()
+        return
+    if args.serve:
+        lo, hi = (float(x) for x in args.serve_revs.split(","))
+        serve_bench(args.serve_rate, args.serve_seconds, lo, hi, args.serve_steps, args.serve_budget_gb,
+                    compare=not args.no_compare)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (independent scenes shard "

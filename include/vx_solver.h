@@ -253,6 +253,9 @@ int32_t vx_batch_admit(vx_batch *batch, const vx_scene_desc *desc, const vx_step
 /* One scheduler tick for all running scenes; writes how many still run. */
 int32_t vx_batch_tick(vx_batch *batch, uint32_t *out_running);
 int32_t vx_batch_report(vx_batch *batch, uint32_t slot, vx_batch_scene_report *report);
+/* VX_SCENE_* state of every slot (states has max_scenes entries), as known
+ * after the last tick; one call instead of a report per slot. */
+int32_t vx_batch_states(vx_batch *batch, uint32_t *states);
 /* Copy one scene's state out (any pointer may be NULL). */
 int32_t vx_batch_download(vx_batch *batch, uint32_t slot, double *positions, double *rotations, double *velocities);
 /* Free a finished (or failed) scene's tiles and slot. */

@@ -922,8 +922,10 @@ struct vx_batch {
     // Two ticks alternate: fl[cur] is filled next, fl[cur ^ 1] may be in flight.
     vx::Flight fl[2];
     int cur = 0;
-    hipGraphExec_t g_inner[2] = {nullptr, nullptr};  // f32, f64
+    // Inner chunk graphs by (precision, launch grid in tiles).
+    std::vector<std::pair<uint64_t, hipGraphExec_t>> g_inner;
     bool single = false;  // one-scene arena of a vx_scene (never re-admits)
+    uint32_t hw_tiles = 0;  // one past the highest tile in use
 };
 
 struct vx_scene {
@@ -942,10 +944,10 @@ struct Lock {
 
 // One inner chunk: the refinement residual of scenes whose inner solve has
 // ended (device-decided), then kInnerChunk iterations.
-int32_t capture_inner(vx_batch *b, int f64) {
+int32_t capture_inner(vx_batch *b, int f64, uint32_t tiles, hipGraphExec_t *out) {
     Context &ctx = context();
     hipGraph_t graph;
-    const dim3 grid(b->cap_tiles), blk(kBlock), sg(b->max_slots);
+    const dim3 grid(tiles), blk(kBlock), sg(b->max_slots);
     VX_HIP(hipStreamBeginCapture(ctx.stream, hipStreamCaptureModeThreadLocal));
     if (f64) {
         hipLaunchKernelGGL(refine_residual<double>, grid, blk, 0, ctx.stream, b->A, 0);
@@ -974,9 +976,31 @@ int32_t capture_inner(vx_batch *b, int f64) {
         }
     }
     VX_HIP(hipStreamEndCapture(ctx.stream, &graph));
-    hipError_t e = hipGraphInstantiate(&b->g_inner[f64], graph, nullptr, nullptr, 0);
+    hipError_t e = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     if (e != hipSuccess) return fail(VX_ERR_DEVICE, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+    return VX_OK;
+}
+
+// Workgroups per arena-wide launch: the tiles in use rounded up to a whole
+// number of XCD rounds (8 * chunk) times a power of two, so launches skip the
+// free end of a large arena and only a few chunk graphs are ever captured.
+uint32_t launch_tiles(const vx_batch *b) {
+    const uint64_t unit = uint64_t(kXcds) * b->chunk;
+    uint64_t g = unit;
+    while (g < b->hw_tiles) g *= 2;
+    return uint32_t(std::min<uint64_t>(g, b->cap_tiles));
+}
+
+int32_t inner_graph(vx_batch *b, int f64, uint32_t tiles, hipGraphExec_t *out) {
+    const uint64_t key = (uint64_t(tiles) << 1) | uint64_t(f64);
+    for (auto &kv : b->g_inner)
+        if (kv.first == key) {
+            *out = kv.second;
+            return VX_OK;
+        }
+    if (int32_t c = capture_inner(b, f64, tiles, out)) return c;
+    b->g_inner.emplace_back(key, *out);
     return VX_OK;
 }
 
@@ -997,8 +1021,9 @@ int32_t launch_tick(vx_batch *b, Flight &f) {
         if (bi & B_RESID0) resid0[prec] = true;
     }
     VX_HIP(hipMemcpyAsync(b->d_bits, f.bits, sizeof(uint32_t) * b->max_slots, hipMemcpyHostToDevice, s));
-    const dim3 grid(b->cap_tiles), blk(kBlock), sg(b->max_slots);
-    hipLaunchKernelGGL(spread_bits, dim3((b->cap_tiles + kBlock - 1) / kBlock), blk, 0, s, b->A);
+    const uint32_t tiles = launch_tiles(b);
+    const dim3 grid(tiles), blk(kBlock), sg(b->max_slots);
+    hipLaunchKernelGGL(spread_bits, dim3((tiles + kBlock - 1) / kBlock), blk, 0, s, b->A);
     if (any & B_PREDICT) hipLaunchKernelGGL(substep_predict, grid, blk, 0, s, b->A);
     if (any & B_FIT) hipLaunchKernelGGL(fit_rotations, grid, blk, 0, s, b->A, uint32_t(B_FIT), 0);
     if (any & B_ASSEMBLE) {
@@ -1016,9 +1041,9 @@ int32_t launch_tick(vx_batch *b, Flight &f) {
     VX_HIP(hipGetLastError());
     for (int prec = 0; prec < 2; ++prec) {
         if (!inner[prec]) continue;
-        if (!b->g_inner[prec])
-            if (int32_t c = capture_inner(b, prec)) return c;
-        VX_HIP(hipGraphLaunch(b->g_inner[prec], s));
+        hipGraphExec_t g = nullptr;
+        if (int32_t c = inner_graph(b, prec, tiles, &g)) return c;
+        VX_HIP(hipGraphLaunch(g, s));
     }
     if (any & (B_APPLY | B_WARM_SAVE)) hipLaunchKernelGGL(apply_correction, grid, blk, 0, s, b->A);
     if (any & B_FITB) {
@@ -1147,7 +1172,13 @@ int32_t create_batch(uint64_t capacity_rows, uint32_t max_scenes, uint32_t chunk
     if (!out || capacity_rows == 0 || max_scenes == 0) return fail(VX_ERR_INVALID_ARGUMENT, "invalid batch size");
     *out = nullptr;
     const uint64_t tiles = tiles_for(capacity_rows);
-    if (chunk == 0) chunk = uint32_t(std::max<uint64_t>(1, (tiles + kXcds - 1) / kXcds));
+    // Default interleave: a one-scene arena deals each XCD one contiguous
+    // eighth of the scene; a multi-scene arena deals 16-tile chunks so its
+    // launches can stop at the tiles in use.
+    if (chunk == 0) {
+        const uint64_t eighth = std::max<uint64_t>(1, (tiles + kXcds - 1) / kXcds);
+        chunk = uint32_t(max_scenes == 1 ? eighth : std::min<uint64_t>(eighth, 16));
+    }
     const uint64_t unit = uint64_t(kXcds) * chunk;
     const uint64_t cap = (tiles + unit - 1) / unit * unit;
     if (cap * kBlock >= (1ull << 31))
@@ -1306,6 +1337,12 @@ int32_t set_tiles(vx_batch *b, uint32_t t0, uint32_t count, const SlotGeo *g) {
     SlotGeo free_tile{};
     free_tile.slot = kFreeTile;
     for (uint32_t t = t0; t < t0 + count; ++t) b->tiles[t] = g ? *g : free_tile;
+    uint32_t hw = b->hw_tiles;
+    if (g)
+        hw = std::max(hw, t0 + count);
+    else
+        while (hw > 0 && b->tiles[hw - 1].slot == kFreeTile) --hw;
+    b->hw_tiles = hw;
     return put(b->d_tiles + t0, b->tiles.data() + t0, sizeof(SlotGeo) * count);
 }
 
@@ -1501,8 +1538,8 @@ extern "C" int32_t vx_batch_destroy(vx_batch *b) {
     Lock lock;
     (void)drain(b);
     (void)hipStreamSynchronize(context().stream);
-    for (auto &g : b->g_inner)
-        if (g) (void)hipGraphExecDestroy(g);
+    for (auto &kv : b->g_inner)
+        if (kv.second) (void)hipGraphExecDestroy(kv.second);
     for (auto &f : b->fl) {
         if (f.bits) (void)hipHostFree(f.bits);
         if (f.st) (void)hipHostFree(f.st);
@@ -1592,6 +1629,18 @@ extern "C" int32_t vx_batch_report(vx_batch *b, uint32_t slot, vx_batch_scene_re
     r->active_rows = h.active;
     r->last_relative_residual = h.last_rel;
     r->last_angle = h.last_angle;
+    return VX_OK;
+}
+
+extern "C" int32_t vx_batch_states(vx_batch *b, uint32_t *states) {
+    clear_error();
+    if (!b || !states) return fail(VX_ERR_INVALID_ARGUMENT, "null argument");
+    Lock lock;
+    for (uint32_t i = 0; i < b->max_slots; ++i) {
+        const SlotHost &h = b->slots[i];
+        states[i] = !h.used ? VX_SCENE_FREE
+                            : (h.phase == P_DONE ? VX_SCENE_DONE : (h.phase == P_FAILED ? VX_SCENE_FAILED : VX_SCENE_RUNNING));
+    }
     return VX_OK;
 }
 

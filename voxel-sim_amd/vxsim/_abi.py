@@ -114,6 +114,7 @@ SIGNATURES = [
      [ctypes.c_void_p, ctypes.POINTER(SceneDesc), ctypes.POINTER(StepParams), ctypes.c_uint32, c_u32p]),
     ("vx_batch_tick", ctypes.c_int32, [ctypes.c_void_p, c_u32p]),
     ("vx_batch_report", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(BatchSceneReport)]),
+    ("vx_batch_states", ctypes.c_int32, [ctypes.c_void_p, c_u32p]),
     ("vx_batch_download", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint32, c_dp, c_dp, c_dp]),
     ("vx_batch_release", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint32]),
 ]

@@ -15,7 +15,7 @@ import numpy as np
 
 from . import lib
 from ._abi import (BatchSceneReport, VX_ERR_TOO_LARGE, VX_SCENE_DONE, VX_SCENE_FAILED, VX_SCENE_FREE,
-                   VX_SCENE_RUNNING, c_dp)
+                   VX_SCENE_RUNNING, c_dp, c_u32p)
 from .grid import scene_desc, step_params
 from .solver import SolverError, check, last_error
 
@@ -95,6 +95,12 @@ class Batch:
         r = BatchSceneReport()
         check(lib().vx_batch_report(self.handle, int(slot), ctypes.byref(r)))
         return {f[0]: getattr(r, f[0]) for f in r._fields_}
+
+    def states(self) -> np.ndarray:
+        """VX_SCENE_* state of every slot after the last tick."""
+        out = np.zeros(self.max_scenes, np.uint32)
+        check(lib().vx_batch_states(self.handle, out.ctypes.data_as(c_u32p)))
+        return out
 
     def error(self, slot: int):
         """The SolverError of a failed scene, else None."""
