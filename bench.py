@@ -179,13 +179,67 @@ def serve_bench(rate: float, seconds: float, rev_lo: float, rev_hi: float, steps
     print(json.dumps(line))
 
 
-def pmc_driver():
+def large_bench(steps: int, warmup: int, compare: bool = True):
+    """The 192x192x96 wood block (3.54 M REVs, out of the 256 MB Infinity
+    Cache): whole steps, the SpMV roofline at that size, and the plain FP64
+    solve of the same system. Prints one JSON line."""
+    from vxsim import grid, scenes
+    scene = scenes.wood_block_blade()
+    cfg = scene.config
+    sp = grid.solve_params(tolerance=1e-6, inner_tolerance=INNER_TOL, max_inner=20000, max_outer=12)
+    step_kw = dict(fixed_dt=cfg.fixed_dt, gravity=cfg.gravity, substeps=cfg.substeps,
+                   corotation_iterations=cfg.corotation_iterations, corotation_tolerance=cfg.corotation_tolerance,
+                   solve=sp)
+    active = scene.active_count()
+    with grid.GridScene(scene) as gs:
+        for _ in range(warmup):
+            gs.step(**step_kw)
+        reps = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            reps.append(gs.step(**step_kw))
+        wall = time.perf_counter() - t0
+        iters = sum(r["inner_iterations"] + r["outer_sweeps"] for r in reps)
+        gs.assemble(cfg.fixed_dt / cfg.substeps)
+        spmv_us = gs.time_spmv(50)
+        extra = {}
+        if compare:
+            h = cfg.fixed_dt / cfg.substeps
+            gs.predict(h, cfg.gravity)
+            gs.fit_rotations()
+            gs.assemble(h)
+            t = time.perf_counter()
+            rm = gs.solve(tolerance=1e-6, inner_tolerance=INNER_TOL, max_inner=20000)
+            tm = time.perf_counter() - t
+            t = time.perf_counter()
+            r64 = gs.solve(tolerance=1e-6, max_inner=40000, mode=grid.F64)
+            t64 = time.perf_counter() - t
+            extra = {"mixed_solve": {"ms": 1000 * tm, "inner": rm["inner_iterations"], "sweeps": rm["outer_sweeps"],
+                                     "dof_iters_per_s": 3.0 * active * (rm["inner_iterations"] + rm["outer_sweeps"]) / tm},
+                     "plain_fp64_structured": {"ms": 1000 * t64, "iters": r64["inner_iterations"],
+                                               "dof_iters_per_s": 3.0 * active * (r64["inner_iterations"] +
+                                                                                   r64["outer_sweeps"]) / t64},
+                     "time_to_solution_vs_fp64": t64 / tm}
+    b = SPMV_BYTES_PER_ROW * scene.n
+    line = {"metric": "DOF-PCG-iters/s", "value": 3.0 * active * iters / wall, "unit": "DOF-PCG-iters/s",
+            "workload": "%s: %d REVs, %d active rows, %d steps, rel. residual 1e-6" % (scene.name, scene.n, active, steps),
+            "ms_per_step": 1000 * wall / steps, "solves": sum(r["solves"] for r in reps), "iterations": iters,
+            "roofline": {"bound": "hbm", "kernel": SPMV_KERNEL, "bytes_per_launch": b, "avg_us": spmv_us,
+                         "achieved": b / (spmv_us * 1e-6) / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                         "frac": b / (spmv_us * 1e-6) / 1e9 / PEAK_HBM_GBS},
+            "data": "synthetic", "dtype": "f32 inner / f64 refinement and reductions"}
+    line.update(extra)
+    print(json.dumps(line))
+
+
+def pmc_driver(large: bool = False):
     """Short run for the rocprofv3 counter passes: 20 launches of the FP32
-    bond SpMV on the assembled 64^3 system, then 3 copies of 1 GiB (the
-    calibration of FETCH_SIZE / WRITE_SIZE against known bytes)."""
+    bond SpMV on the assembled 64^3 system (or the 192x192x96 block), then 3
+    copies of 1 GiB (the calibration of FETCH_SIZE / WRITE_SIZE against known
+    bytes)."""
     import torch
     from vxsim import grid, scenes
-    scene = scenes.large_64cube()
+    scene = scenes.wood_block_blade() if large else scenes.large_64cube()
     cfg = scene.config
     with grid.GridScene(scene) as gs:
         h = cfg.fixed_dt / cfg.substeps
@@ -211,6 +265,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--pmc-driver", action="store_true", help="short SpMV + copy run for counter passes")
     ap.add_argument("--serve", action="store_true", help="served request stream instead of the step benchmark")
+    ap.add_argument("--large", action="store_true", help="192x192x96 block instead of the 64^3 box")
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--serve-revs", type=str, default="1e3,2e5")
@@ -218,7 +273,10 @@ def main():
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     args = ap.parse_args()
     if args.pmc_driver:
-        pmc_driver()
+        pmc_driver(args.large)
+        return
+    if args.large:
+        large_bench(args.steps, args.warmup, compare=not args.no_compare)
         return
     if args.serve:
         lo, hi = (float(x) for x in args.serve_revs.split(","))

@@ -149,3 +149,24 @@ def test_large_scene_residual_property(gpu, O, scenes):
     r = rows_o["rhs"] - O.matvec(rows_o, x[rows_g["active"]])
     assert np.linalg.norm(r) / np.linalg.norm(rows_o["rhs"]) <= 1e-9 * 1.01
     assert rep["relative_residual"] <= 1e-9
+
+
+def test_large_scene_pre_summed_reductions(gpu, O, scenes):
+    """A scene of more than 1024 tiles (96 x 96 x 64 = 1152 tiles) sums its
+    inner partials once per kernel instead of in every workgroup; its solve
+    still satisfies the oracle's FP64 system, and a warm start from its own
+    solution needs no inner iteration."""
+    s = scenes.large_64cube(dims=(96, 96, 64))
+    assert -(-s.n // 512) > 1024
+    h, g = s.config.fixed_dt, s.config.gravity
+    gs, _ = device_assembled(s, h, g)
+    rep = gs.solve(tolerance=1e-9)
+    x = gs.download()["correction"]
+    rows_g = gs.export_rows()
+    again = gs.solve(tolerance=1e-9, warm_start=True)
+    gs.close()
+    _, _, rows_o, _ = oracle_rows(O, s, h, g)
+    r = rows_o["rhs"] - O.matvec(rows_o, x[rows_g["active"]])
+    assert np.linalg.norm(r) / np.linalg.norm(rows_o["rhs"]) <= 1e-9 * 1.01
+    assert rep["relative_residual"] <= 1e-9 and rep["inner_iterations"] > 0
+    assert again["inner_iterations"] == 0 and again["outer_sweeps"] == 1

@@ -69,6 +69,31 @@ inline uint32_t xcd_grid(uint32_t tiles, uint32_t *per_xcd) {
     return *per_xcd * kXcds;
 }
 
+// ---- row loads ----------------------------------------------------------------
+// Element access through a 32-bit byte offset from a plane's base, which must
+// be the same for every lane (a plane of an SoA field). The base is pinned to
+// scalar registers (readfirstlane), so the compiler emits the SGPR-base form
+// of global_load/store (one VGPR offset shared by every plane of a row)
+// instead of a 64-bit address add per access. Offsets stay below 2^32:
+// arenas hold fewer than 2^29 rows.
+#define VX_GLOBAL __attribute__((address_space(1)))
+__device__ __forceinline__ uint64_t uniform_addr(const void *p) {
+    const uint64_t x = reinterpret_cast<uint64_t>(p);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(x));
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(uint32_t(x >> 32));
+    return (uint64_t(hi) << 32) | lo;
+}
+template <class T>
+__device__ __forceinline__ T ld_row(const T *plane, uint32_t byte_off) {
+    const VX_GLOBAL char *base = reinterpret_cast<const VX_GLOBAL char *>(uniform_addr(plane));
+    return *reinterpret_cast<const VX_GLOBAL T *>(base + byte_off);
+}
+template <class T>
+__device__ __forceinline__ void st_row(T *plane, uint32_t byte_off, T value) {
+    VX_GLOBAL char *base = reinterpret_cast<VX_GLOBAL char *>(uniform_addr(plane));
+    *reinterpret_cast<VX_GLOBAL T *>(base + byte_off) = value;
+}
+
 // ---- reductions -------------------------------------------------------------
 // Cross-workgroup sums never use atomics or fences: each kernel writes one
 // partial per workgroup with a plain store, and the kernels that need the

@@ -90,7 +90,13 @@ struct SlotState {
     uint32_t seg;   // iteration index at the last residual replacement
     uint32_t pbuf;  // p buffer holding the latest search direction
     uint32_t pad;
+    double red[3];  // pre-summed p.Ap, r.z, r.r of a large scene (scene_reduce)
 };
+
+// Scenes of more tiles than this have their inner partials summed once per
+// kernel by scene_reduce instead of by every workgroup of the consumer (whose
+// cost grows with the square of the tile count).
+constexpr uint32_t kBigTiles = 1024;
 
 struct DevArena {
     uint32_t R, cap_tiles, chunk, max_slots;
@@ -180,7 +186,8 @@ __device__ __forceinline__ Rev ld_rev(const DevArena &A, uint32_t v) {
 
 // Per-scene sum of N partial kinds over the scene's tiles, broadcast to every
 // thread of the workgroup (fixed order). scene_load is the per-thread part;
-// block_sum_all completes it.
+// block_sum_all completes it. inner_load is scene_load for the inner PCG
+// partials, which a large scene takes pre-summed from its state.
 template <int N>
 __device__ __forceinline__ void scene_load(double (&out)[N], const DevArena &A, const SlotGeo &g, const int (&kind)[N]) {
     // One load per thread covers scenes of up to kBlock tiles without a loop
@@ -197,6 +204,15 @@ __device__ __forceinline__ void scene_load(double (&out)[N], const DevArena &A, 
             for (uint32_t b = threadIdx.x + kBlock; b < g.ntiles; b += kBlock) out[k] += part[b];
         }
     }
+}
+template <int N>
+__device__ __forceinline__ void inner_load(double (&out)[N], const DevArena &A, const SlotGeo &g, const int (&kind)[N]) {
+    if (g.ntiles > kBigTiles) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) out[k] = threadIdx.x == 0 ? A.st[g.slot].red[kind[k]] : 0.0;
+        return;
+    }
+    scene_load<N>(out, A, g, kind);
 }
 template <int N>
 __device__ __forceinline__ void scene_sum(double (&out)[N], const DevArena &A, const SlotGeo &g, const int (&kind)[N],
@@ -438,16 +454,51 @@ __device__ __forceinline__ bool inner_bits(uint32_t bits) {
     return (bits & B_INNER) && ((bits & B_F64) == Inner<S>::kBit);
 }
 
+// A one-scene arena's inner kernels: the scene's geometry and the order in
+// which workgroups visit its tiles. Tiles are visited by columns of 8 tiles
+// that run through all z: with q tiles per z-plane (rounded), logical index
+// L = ((jb * nzt) + zr) * 8 + j visits tile zr * q + jb * 8 + j, so a tile's
+// +-z neighbour tiles are visited 8 indices before and after it (the same
+// XCD) instead of a whole plane away, and stay in that XCD's L2.
+struct OneScene {
+    SlotGeo g;
+    uint32_t q, nzt, chunk, grid;
+};
+
+// Planes of fewer than 16 tiles keep the plain order (q = 0): their z
+// neighbours are already within a few tiles.
+inline OneScene make_one(const SlotGeo &g) {
+    OneScene o{};
+    o.g = g;
+    const uint64_t plane = uint64_t(g.nx) * g.ny;
+    const uint32_t q = uint32_t(std::max<uint64_t>(1, (plane + kBlock / 2) / kBlock));
+    uint64_t space = g.ntiles;
+    if (q >= 16) {
+        o.q = q;
+        o.nzt = (g.ntiles + q - 1) / q;
+        space = uint64_t((q + 7) / 8) * 8 * o.nzt;
+    }
+    o.chunk = uint32_t(std::max<uint64_t>(1, (space + kXcds - 1) / kXcds));
+    o.grid = o.chunk * kXcds;
+    return o;
+}
+
+__device__ __forceinline__ uint32_t one_tile(uint32_t L, const OneScene &o) {
+    if (o.q == 0) return L;
+    const uint32_t j = L % 8, r = L / 8, zr = r % o.nzt, col = (r / o.nzt) * 8 + j;
+    return col < o.q ? zr * o.q + col : kFreeTile;
+}
+
 // The tile of an inner-iteration kernel. ONE: the arena holds a single scene
 // whose geometry comes in the kernel arguments (no descriptor load before
 // the row loads), and the chunk is only launched while that scene iterates.
 template <class S, bool ONE>
-__device__ __forceinline__ bool inner_tile(const DevArena &A, const SlotGeo &one, Tile &c) {
+__device__ __forceinline__ bool inner_tile(const DevArena &A, const OneScene &one, Tile &c) {
     if (!ONE) return tile_base(A, c) && inner_bits<S>(c.bits);
-    c.t = logical_tile(blockIdx.x, A.chunk);
-    if (c.t >= one.ntiles) return false;
-    c.g = one;
-    c.slot = one.slot;
+    c.t = one_tile(logical_tile(blockIdx.x, one.chunk), one);
+    if (c.t >= one.g.ntiles) return false;
+    c.g = one.g;
+    c.slot = one.g.slot;
     c.bits = B_INNER | Inner<S>::kBit;
     c.local = c.t * kBlock + threadIdx.x;
     c.row = c.local < c.g.n;
@@ -465,7 +516,7 @@ __device__ __forceinline__ bool inner_tile(const DevArena &A, const SlotGeo &one
 // loads do not depend on beta, so they are in flight while the workgroup
 // reduces the partials (LDS-only barriers do not wait for them).
 template <class S, bool ONE>
-__global__ __launch_bounds__(kBlock) void pcg_bond_spmv(DevArena A, SlotGeo one, int odd) {
+__global__ __launch_bounds__(kBlock) void pcg_bond_spmv(DevArena A, OneScene one, int odd) {
     Tile c;
     if (!inner_tile<S, ONE>(A, one, c)) return;
     __shared__ double lds[2][kWaves];
@@ -478,59 +529,61 @@ __global__ __launch_bounds__(kBlock) void pcg_bond_spmv(DevArena A, SlotGeo one,
     const SlotParams prm = A.prm[c.slot];
     double sums[2];
     const int kinds[2] = {kRz, kRr};
-    scene_load<2>(sums, A, c.g, kinds);
+    inner_load<2>(sums, A, c.g, kinds);
     __builtin_amdgcn_sched_barrier(0);
     const size_t R = A.R;
     const S *__restrict__ z = Inner<S>::z(A);
     const S *__restrict__ p_old = Inner<S>::p(A, odd);
     const S *__restrict__ K = Inner<S>::k(A);
+    const S *__restrict__ Dp = Inner<S>::d(A);
     const uint32_t v = c.row ? c.v : c.g.voff;  // rows past the scene load row 0 and discard
     uint32_t cc[3];
     coords(c.g, c.row ? c.local : 0u, cc);
+    // Byte offsets of the row and its neighbours. An absent neighbour loads
+    // the row itself (in bounds, cached, so no load sits behind a branch):
+    // its +axis block is zero and its -axis block is zeroed below, so it
+    // adds nothing.
+    constexpr uint32_t sz = sizeof(S);
+    const uint32_t ov = v * sz;
+    uint32_t om[3], op[3];
+    bool lo[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const uint32_t s = stride_of(c.g, a);
+        lo[a] = cc[a] > 0;
+        om[a] = lo[a] ? ov - s * sz : ov;
+        op[a] = cc[a] + 1 < extent_of(c.g, a) ? ov + s * sz : ov;
+    }
     S zz[7][3], pp[7][3], Dv[6], Kp[3][6], Km[3][6];
     const uint8_t m = A.mask[v];
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-        zz[0][q] = z[q * R + v];
-        pp[0][q] = p_old[q * R + v];
+        zz[0][q] = ld_row(z + q * R, ov);
+        pp[0][q] = ld_row(p_old + q * R, ov);
     }
 #pragma unroll
-    for (int q = 0; q < 6; ++q) Dv[q] = Inner<S>::d(A)[q * R + v];
+    for (int q = 0; q < 6; ++q) Dv[q] = ld_row(Dp + q * R, ov);
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        const uint32_t s = stride_of(c.g, a);
         const S *Ka = K + size_t(a) * 6 * R;
-        const bool lo = cc[a] > 0, hi = cc[a] + 1 < extent_of(c.g, a);
-        const uint32_t um = lo ? v - s : v, up = hi ? v + s : v;
-        // Absent neighbours load the row itself (in bounds, cached) and are
-        // zeroed after the loads, so no load sits behind a branch.
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
-            zz[1 + 2 * a][q] = z[q * R + um];
-            pp[1 + 2 * a][q] = p_old[q * R + um];
-            zz[2 + 2 * a][q] = z[q * R + up];
-            pp[2 + 2 * a][q] = p_old[q * R + up];
+            zz[1 + 2 * a][q] = ld_row(z + q * R, om[a]);
+            pp[1 + 2 * a][q] = ld_row(p_old + q * R, om[a]);
+            zz[2 + 2 * a][q] = ld_row(z + q * R, op[a]);
+            pp[2 + 2 * a][q] = ld_row(p_old + q * R, op[a]);
         }
 #pragma unroll
         for (int q = 0; q < 6; ++q) {
-            Km[a][q] = Ka[q * R + um];
-            Kp[a][q] = Ka[q * R + v];  // zero when the +axis bond is absent
+            Km[a][q] = ld_row(Ka + q * R, om[a]);
+            Kp[a][q] = ld_row(Ka + q * R, ov);  // zero when the +axis bond is absent
         }
     }
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        const bool lo = cc[a] > 0, hi = cc[a] + 1 < extent_of(c.g, a);
+    for (int a = 0; a < 3; ++a)
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            zz[1 + 2 * a][q] = lo ? zz[1 + 2 * a][q] : S(0);
-            pp[1 + 2 * a][q] = lo ? pp[1 + 2 * a][q] : S(0);
-            zz[2 + 2 * a][q] = hi ? zz[2 + 2 * a][q] : S(0);
-            pp[2 + 2 * a][q] = hi ? pp[2 + 2 * a][q] : S(0);
-        }
-#pragma unroll
-        for (int q = 0; q < 6; ++q) Km[a][q] = lo ? Km[a][q] : S(0);
-    }
+        for (int q = 0; q < 6; ++q) Km[a][q] = lo[a] ? Km[a][q] : S(0);
     block_sum_all<2>(sums, lds);
     if (done_in) return;
     const double rho = sums[0], rr = sums[1];
@@ -591,12 +644,12 @@ __global__ __launch_bounds__(kBlock) void pcg_bond_spmv(DevArena A, SlotGeo one,
     if (c.row) {
         S *__restrict__ p_new = Inner<S>::p(A, !odd);
         S *__restrict__ ap = Inner<S>::ap(A);
-        p_new[v] = p[0][0];
-        p_new[R + v] = p[0][1];
-        p_new[2 * R + v] = p[0][2];
-        ap[v] = y0;
-        ap[R + v] = y1;
-        ap[2 * R + v] = y2;
+        st_row(p_new, ov, p[0][0]);
+        st_row(p_new + R, ov, p[0][1]);
+        st_row(p_new + 2 * R, ov, p[0][2]);
+        st_row(ap, ov, y0);
+        st_row(ap + R, ov, y1);
+        st_row(ap + 2 * R, ov, y2);
         part[0] = double(p[0][0]) * double(y0);
         part[0] = __builtin_fma(double(p[0][1]), double(y1), part[0]);
         part[0] = __builtin_fma(double(p[0][2]), double(y2), part[0]);
@@ -610,7 +663,7 @@ __global__ __launch_bounds__(kBlock) void pcg_bond_spmv(DevArena A, SlotGeo one,
 // r -= alpha Ap; z = Dinv r; partials of r.z and r.r. Row loads are issued
 // before the p.Ap partial sum, as in the SpMV.
 template <class S, bool ONE>
-__global__ __launch_bounds__(kBlock) void pcg_update(DevArena A, SlotGeo one, int odd) {
+__global__ __launch_bounds__(kBlock) void pcg_update(DevArena A, OneScene one, int odd) {
     Tile c;
     if (!inner_tile<S, ONE>(A, one, c)) return;
     __shared__ double lds[2][kWaves];
@@ -621,7 +674,7 @@ __global__ __launch_bounds__(kBlock) void pcg_update(DevArena A, SlotGeo one, in
     const double rho_s[2] = {st->rho[0], st->rho[1]};
     double pap[1];
     const int kinds[1] = {kPap};
-    scene_load<1>(pap, A, c.g, kinds);
+    inner_load<1>(pap, A, c.g, kinds);
     __builtin_amdgcn_sched_barrier(0);
     const size_t R = A.R;
     const uint32_t v = c.row ? c.v : c.g.voff;
@@ -631,16 +684,17 @@ __global__ __launch_bounds__(kBlock) void pcg_update(DevArena A, SlotGeo one, in
     S *__restrict__ r = Inner<S>::r(A);
     S *__restrict__ z = Inner<S>::z(A);
     const S *__restrict__ dinv = Inner<S>::dinv(A);
+    const uint32_t ov = v * uint32_t(sizeof(S));
     S pv[3], av[3], rv[3], dv[3], Di[6];
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-        pv[q] = p[q * R + v];
-        av[q] = ap[q * R + v];
-        rv[q] = r[q * R + v];
-        dv[q] = d[q * R + v];
+        pv[q] = ld_row(p + q * R, ov);
+        av[q] = ld_row(ap + q * R, ov);
+        rv[q] = ld_row(r + q * R, ov);
+        dv[q] = ld_row(d + q * R, ov);
     }
 #pragma unroll
-    for (int q = 0; q < 6; ++q) Di[q] = dinv[q * R + v];
+    for (int q = 0; q < 6; ++q) Di[q] = ld_row(dinv + q * R, ov);
     __builtin_amdgcn_sched_barrier(0);
     block_sum_all<1>(pap, lds);
     if (done_in) return;
@@ -655,15 +709,15 @@ __global__ __launch_bounds__(kBlock) void pcg_update(DevArena A, SlotGeo one, in
         S ri[3], zi[3];
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
-            d[q * R + v] = fmaS<S>(alpha, pv[q], first ? S(0) : dv[q]);
+            st_row(d + q * R, ov, fmaS<S>(alpha, pv[q], first ? S(0) : dv[q]));
             ri[q] = fmaS<S>(-alpha, av[q], rv[q]);
-            r[q * R + v] = ri[q];
+            st_row(r + q * R, ov, ri[q]);
         }
         zi[0] = fmaS<S>(Di[2], ri[2], fmaS<S>(Di[1], ri[1], Di[0] * ri[0]));
         zi[1] = fmaS<S>(Di[4], ri[2], fmaS<S>(Di[3], ri[1], Di[1] * ri[0]));
         zi[2] = fmaS<S>(Di[5], ri[2], fmaS<S>(Di[4], ri[1], Di[2] * ri[0]));
 #pragma unroll
-        for (int q = 0; q < 3; ++q) z[q * R + v] = zi[q];
+        for (int q = 0; q < 3; ++q) st_row(z + q * R, ov, zi[q]);
         part[0] = __builtin_fma(double(ri[2]), double(zi[2]), __builtin_fma(double(ri[1]), double(zi[1]), double(ri[0]) * double(zi[0])));
         part[1] = __builtin_fma(double(ri[2]), double(ri[2]), __builtin_fma(double(ri[1]), double(ri[1]), double(ri[0]) * double(ri[0])));
     }
@@ -812,6 +866,34 @@ __global__ __launch_bounds__(kBlock) void refine_decide(DevArena A, int forced) 
     st->done = outer_done ? kConv : kRun;
 }
 
+// One workgroup per large scene: sums its partials of kinds [k0, k0 + nk)
+// into red[] for the next inner kernel (forced: after the first residual of
+// a solve; else inside an inner chunk).
+template <class S>
+__global__ __launch_bounds__(kBlock) void scene_reduce(DevArena A, int k0, int nk, int forced) {
+    const uint32_t slot = blockIdx.x;
+    const SlotGeo g = A.geo[slot];
+    if (g.n == 0 || g.ntiles <= kBigTiles) return;
+    const uint32_t bits = A.bits[slot];
+    if ((bits & B_F64) != Inner<S>::kBit || !(bits & (forced ? uint32_t(B_RESID0) : uint32_t(B_INNER)))) return;
+    __shared__ double lds[2][kWaves];
+    SlotState *st = A.st + slot;
+    if (nk == 2) {
+        double s[2];
+        const int kinds[2] = {k0, k0 + 1};
+        scene_sum<2>(s, A, g, kinds, lds);
+        if (threadIdx.x == 0) {
+            st->red[k0] = s[0];
+            st->red[k0 + 1] = s[1];
+        }
+    } else {
+        double s[1];
+        const int kinds[1] = {k0};
+        scene_sum<1>(s, A, g, kinds, lds);
+        if (threadIdx.x == 0) st->red[k0] = s[0];
+    }
+}
+
 // positions += x on the free rows (src/solver.rs:1821-1823, 1921-1923); with
 // B_WARM_SAVE the correction is also kept as the next solve's warm start.
 __global__ __launch_bounds__(kBlock) void apply_correction(DevArena A) {
@@ -944,10 +1026,17 @@ struct Lock {
 
 // One inner chunk: the refinement residual of scenes whose inner solve has
 // ended (device-decided), then kInnerChunk iterations.
-int32_t capture_inner(vx_batch *b, int f64, uint32_t tiles, hipGraphExec_t *out) {
+int32_t capture_inner(vx_batch *b, int f64, uint32_t tiles, bool big, hipGraphExec_t *out) {
     Context &ctx = context();
     hipGraph_t graph;
     const dim3 grid(tiles), blk(kBlock), sg(b->max_slots);
+    auto reduce = [&](int k0, int nk) {
+        if (!big) return;
+        if (f64)
+            hipLaunchKernelGGL(scene_reduce<double>, sg, blk, 0, ctx.stream, b->A, k0, nk, 0);
+        else
+            hipLaunchKernelGGL(scene_reduce<float>, sg, blk, 0, ctx.stream, b->A, k0, nk, 0);
+    };
     VX_HIP(hipStreamBeginCapture(ctx.stream, hipStreamCaptureModeThreadLocal));
     if (f64) {
         hipLaunchKernelGGL(refine_residual<double>, grid, blk, 0, ctx.stream, b->A, 0);
@@ -956,24 +1045,31 @@ int32_t capture_inner(vx_batch *b, int f64, uint32_t tiles, hipGraphExec_t *out)
         hipLaunchKernelGGL(refine_residual<float>, grid, blk, 0, ctx.stream, b->A, 0);
         hipLaunchKernelGGL(refine_decide<float>, sg, blk, 0, ctx.stream, b->A, 0);
     }
+    reduce(kRz, 2);
     // A one-scene arena (vx_scene_*) never changes its scene: its geometry
     // goes into the kernel arguments.
-    const SlotGeo one = b->single ? b->slots[0].geo : SlotGeo{};
+    const OneScene one = b->single ? make_one(b->slots[0].geo) : OneScene{};
+    const dim3 grid1(b->single ? one.grid : tiles);
     for (uint32_t t = 0; t < kInnerChunk; ++t) {
         const int odd = int(t & 1);
-        if (f64 && b->single) {
-            hipLaunchKernelGGL((pcg_bond_spmv<double, true>), grid, blk, 0, ctx.stream, b->A, one, odd);
-            hipLaunchKernelGGL((pcg_update<double, true>), grid, blk, 0, ctx.stream, b->A, one, odd);
-        } else if (f64) {
+        if (f64 && b->single)
+            hipLaunchKernelGGL((pcg_bond_spmv<double, true>), grid1, blk, 0, ctx.stream, b->A, one, odd);
+        else if (f64)
             hipLaunchKernelGGL((pcg_bond_spmv<double, false>), grid, blk, 0, ctx.stream, b->A, one, odd);
-            hipLaunchKernelGGL((pcg_update<double, false>), grid, blk, 0, ctx.stream, b->A, one, odd);
-        } else if (b->single) {
-            hipLaunchKernelGGL((pcg_bond_spmv<float, true>), grid, blk, 0, ctx.stream, b->A, one, odd);
-            hipLaunchKernelGGL((pcg_update<float, true>), grid, blk, 0, ctx.stream, b->A, one, odd);
-        } else {
+        else if (b->single)
+            hipLaunchKernelGGL((pcg_bond_spmv<float, true>), grid1, blk, 0, ctx.stream, b->A, one, odd);
+        else
             hipLaunchKernelGGL((pcg_bond_spmv<float, false>), grid, blk, 0, ctx.stream, b->A, one, odd);
+        reduce(kPap, 1);
+        if (f64 && b->single)
+            hipLaunchKernelGGL((pcg_update<double, true>), grid1, blk, 0, ctx.stream, b->A, one, odd);
+        else if (f64)
+            hipLaunchKernelGGL((pcg_update<double, false>), grid, blk, 0, ctx.stream, b->A, one, odd);
+        else if (b->single)
+            hipLaunchKernelGGL((pcg_update<float, true>), grid1, blk, 0, ctx.stream, b->A, one, odd);
+        else
             hipLaunchKernelGGL((pcg_update<float, false>), grid, blk, 0, ctx.stream, b->A, one, odd);
-        }
+        reduce(kRz, 2);
     }
     VX_HIP(hipStreamEndCapture(ctx.stream, &graph));
     hipError_t e = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
@@ -992,14 +1088,22 @@ uint32_t launch_tiles(const vx_batch *b) {
     return uint32_t(std::min<uint64_t>(g, b->cap_tiles));
 }
 
+// Whether any admitted scene takes its inner partials through scene_reduce.
+bool any_big(const vx_batch *b) {
+    for (const auto &h : b->slots)
+        if (h.used && h.geo.ntiles > kBigTiles) return true;
+    return false;
+}
+
 int32_t inner_graph(vx_batch *b, int f64, uint32_t tiles, hipGraphExec_t *out) {
-    const uint64_t key = (uint64_t(tiles) << 1) | uint64_t(f64);
+    const bool big = any_big(b);
+    const uint64_t key = (uint64_t(tiles) << 2) | (uint64_t(big) << 1) | uint64_t(f64);
     for (auto &kv : b->g_inner)
         if (kv.first == key) {
             *out = kv.second;
             return VX_OK;
         }
-    if (int32_t c = capture_inner(b, f64, tiles, out)) return c;
+    if (int32_t c = capture_inner(b, f64, tiles, big, out)) return c;
     b->g_inner.emplace_back(key, *out);
     return VX_OK;
 }
@@ -1030,13 +1134,16 @@ int32_t launch_tick(vx_batch *b, Flight &f) {
         hipLaunchKernelGGL(assemble_bonds, grid, blk, 0, s, b->A);
         hipLaunchKernelGGL(assemble_rows, grid, blk, 0, s, b->A);
     }
+    const bool big = any_big(b);
     if (resid0[0]) {
         hipLaunchKernelGGL(refine_residual<float>, grid, blk, 0, s, b->A, 1);
         hipLaunchKernelGGL(refine_decide<float>, sg, blk, 0, s, b->A, 1);
+        if (big) hipLaunchKernelGGL(scene_reduce<float>, sg, blk, 0, s, b->A, int(kRz), 2, 1);
     }
     if (resid0[1]) {
         hipLaunchKernelGGL(refine_residual<double>, grid, blk, 0, s, b->A, 1);
         hipLaunchKernelGGL(refine_decide<double>, sg, blk, 0, s, b->A, 1);
+        if (big) hipLaunchKernelGGL(scene_reduce<double>, sg, blk, 0, s, b->A, int(kRz), 2, 1);
     }
     VX_HIP(hipGetLastError());
     for (int prec = 0; prec < 2; ++prec) {
@@ -1181,7 +1288,7 @@ int32_t create_batch(uint64_t capacity_rows, uint32_t max_scenes, uint32_t chunk
     }
     const uint64_t unit = uint64_t(kXcds) * chunk;
     const uint64_t cap = (tiles + unit - 1) / unit * unit;
-    if (cap * kBlock >= (1ull << 31))
+    if (cap * kBlock >= (1ull << 29))
         return fail(VX_ERR_TOO_LARGE, "arena of %llu rows exceeds the index range", (unsigned long long)(cap * kBlock));
     if (int32_t c = ensure_init()) return c;
     Lock lock;
@@ -1900,6 +2007,7 @@ extern "C" int32_t vx_scene_time_spmv(vx_scene *sc, uint32_t reps, double *out_u
     st.seg = 0;
     st.iters = 0;
     st.done = kRun;
+    st.red[kRz] = st.red[kRr] = double(b->slots[sc->slot].geo.ntiles);
     st.rho[0] = st.rho[1] = 0.0;
     const vx_step_params p = b->slots[sc->slot].params;
     SlotParams sp = device_params(p.fixed_dt, p.gravity, &p.solve);
@@ -1916,13 +2024,14 @@ extern "C" int32_t vx_scene_time_spmv(vx_scene *sc, uint32_t reps, double *out_u
     VX_HIP(hipMemcpy(b->d_bits, bits.data(), sizeof(uint32_t) * b->max_slots, hipMemcpyHostToDevice));
     hipLaunchKernelGGL(spread_bits, dim3((b->cap_tiles + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx.stream, b->A);
     VX_HIP(hipGetLastError());
+    const OneScene one = make_one(g);
     // The launches run back to back inside one graph, as the solve runs them,
     // between two events on the library stream.
     hipGraph_t graph;
     hipGraphExec_t exec = nullptr;
     VX_HIP(hipStreamBeginCapture(ctx.stream, hipStreamCaptureModeThreadLocal));
     for (uint32_t k = 0; k < reps; ++k)
-        hipLaunchKernelGGL((pcg_bond_spmv<float, true>), dim3(b->cap_tiles), dim3(kBlock), 0, ctx.stream, b->A, g, int(k & 1));
+        hipLaunchKernelGGL((pcg_bond_spmv<float, true>), dim3(one.grid), dim3(kBlock), 0, ctx.stream, b->A, one, int(k & 1));
     VX_HIP(hipStreamEndCapture(ctx.stream, &graph));
     hipError_t ge = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);

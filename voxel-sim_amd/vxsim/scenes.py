@@ -298,6 +298,31 @@ def large_64cube(seed: int = 2, dims=(64, 64, 64)) -> Scene:
     return s
 
 
+def wood_block_blade(seed: int = 5, dims=(192, 192, 96), blade=(40, 5, 20), speed=5.0) -> Scene:
+    """The solve of the large cutting scene at its real size: a wood block
+    (grain Z, per-bond stiffness jitter U[0.95, 1.05]) with the bottom layer
+    fixed and gravity. The rigid iron blade (realised 40x5x20 REVs) driven at
+    `speed` m/s in -Z is represented by its footprint on the top face: the
+    top-layer REVs under the blade are axis-targeted in Z one step's travel
+    down (contact projection and fracture are not part of this solve)."""
+    rng = SplitMix64(seed)
+    n = dims[0] * dims[1] * dims[2]
+    s = _box(dims, np.zeros(n, np.int64), np.full(n, 2), 1.0, np.zeros(n, np.uint8), ["wood"])
+    s.kscale = (0.95 + 0.1 * rng.uniform(3 * n)).astype(np.float32).reshape(3, n)
+    z = s.keys[:, 2]
+    s.kinematic = (z == z.min()).astype(np.uint8)
+    x = s.keys[:, 0] + dims[0] // 2
+    y = s.keys[:, 1] + dims[1] // 2
+    bx0, by0 = (dims[0] - blade[0]) // 2, (dims[1] - blade[1]) // 2
+    under = (z == z.max()) & (x >= bx0) & (x < bx0 + blade[0]) & (y >= by0) & (y < by0 + blade[1])
+    dt = 0.001
+    s.targets[under, 2] = s.positions[under, 2] - np.float32(speed * dt)
+    s.config = StepConfig(fixed_dt=dt, substeps=1, iterations=1000, corotation_iterations=6,
+                          solver_relative_tolerance=1e-6, gravity=(0.0, 0.0, -9.81))
+    s.name = "wood_block_%dx%dx%d_blade" % tuple(dims)
+    return s
+
+
 def batch_boxes(count: int = 256, seed: int = 3, lo: int = 8, hi: int = 48):
     """Independent boxes with integer edges U[lo, hi] per axis, regional
     materials, one random face fixed and initial velocities N(0, 0.05 m/s)
