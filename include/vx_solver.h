@@ -207,6 +207,10 @@ typedef struct vx_step_report {
 
 int32_t vx_scene_step(vx_scene *scene, const vx_step_params *params, vx_step_report *report);
 
+/* Inner PCG iterations per scheduling tick (one hipGraph replay; default 16,
+ * at most 256). 0 keeps the current value. */
+int32_t vx_scene_set_schedule(vx_scene *scene, uint32_t iterations_per_tick);
+
 /* ---- batched and served scenes (build extensions) ----------------------- */
 /*
  * A batch is an HBM arena of 512-REV tiles shared by many independent box
@@ -244,6 +248,9 @@ uint64_t vx_batch_bytes_per_row(void);
  * interleave across the 8 XCDs (0 = choose from the capacity). */
 int32_t vx_batch_create(uint64_t capacity_rows, uint32_t max_scenes, uint32_t xcd_chunk_tiles, vx_batch **out);
 int32_t vx_batch_destroy(vx_batch *batch);
+/* Schedule of an arena: tiles per XCD chunk of the workgroup-to-tile
+ * interleave and inner iterations per tick; 0 keeps a value. */
+int32_t vx_batch_set_schedule(vx_batch *batch, uint32_t xcd_chunk_tiles, uint32_t iterations_per_tick);
 /* Largest contiguous free range and total free space, in rows. */
 int32_t vx_batch_free_rows(vx_batch *batch, uint64_t *largest, uint64_t *total);
 /* Admit a scene that will run `steps` fixed steps with `params`. Returns

@@ -1,0 +1,40 @@
+"""Host-side logic that needs no device: schedule table lookup, request
+stream generation, latency percentiles, scene generators."""
+import json
+
+import numpy as np
+
+
+def test_schedule_lookup_buckets_and_defaults(tmp_path):
+    from vxsim import tune
+    assert [tune.bucket_of(x) for x in (1, 1.9e4, 2e4, 1e5, 3e6, 1e9)] == [0, 0, 1, 1, 3, 3]
+    assert tune.lookup(5e4, "single", table={}) == tune.DEFAULT["single"]
+    t = {"single": {"1": {"iters": 32}}, "batch": {"0": {"chunk": 8, "iters": 8}}}
+    assert tune.lookup(5e4, "single", table=t) == {"iters": 32}
+    assert tune.lookup(5e3, "batch", table=t) == {"chunk": 8, "iters": 8}
+    assert tune.lookup(5e5, "batch", table=t) == tune.DEFAULT["batch"]
+    path = tmp_path / "t.json"
+    path.write_text(json.dumps(t))
+    assert tune.load_table(str(path))["single"]["1"]["iters"] == 32
+
+
+def test_request_stream_is_seeded_and_in_range():
+    from vxsim import serve
+    a = serve.request_stream(rate=50.0, seconds=0.5, rev_lo=200, rev_hi=5000, steps=2, seed=3)
+    b = serve.request_stream(rate=50.0, seconds=0.5, rev_lo=200, rev_hi=5000, steps=2, seed=3)
+    assert len(a) == len(b) > 5
+    assert [r.arrival for r in a] == [r.arrival for r in b]
+    assert all(np.array_equal(x.scene.positions, y.scene.positions) for x, y in zip(a, b))
+    assert all(0 < r.arrival < 0.5 for r in a) and sorted(r.arrival for r in a) == [r.arrival for r in a]
+    assert all(100 <= r.scene.n <= 10000 and r.scene.kinematic.any() for r in a)
+    p = serve.percentiles([1.0, 2.0, 3.0, 4.0, None])
+    assert p[50] == 2.5 and 3.9 < p[99] <= 4.0
+
+
+def test_large_block_scene():
+    from vxsim import scenes
+    s = scenes.wood_block_blade(dims=(24, 20, 10), blade=(6, 2, 4))
+    targeted = ~np.isnan(s.targets[:, 2])
+    assert targeted.sum() == 6 * 2 and np.isnan(s.targets[targeted, :2]).all()
+    assert (s.keys[targeted, 2] == 9).all() and (s.kinematic[s.keys[:, 2] == 0] == 1).all()
+    assert s.bond_count() == 23 * 20 * 10 + 24 * 19 * 10 + 24 * 20 * 9

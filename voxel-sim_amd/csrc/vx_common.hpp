@@ -100,10 +100,35 @@ __device__ __forceinline__ void st_row(T *plane, uint32_t byte_off, T value) {
 // total re-sum the partials in every workgroup (the kernel boundary makes
 // them visible). The order of the sum is fixed, so every workgroup gets the
 // same bits and results are reproducible run to run.
+// One DPP step on a double: the two 32-bit halves are moved with the same
+// DPP control (lanes without a source, or masked off, read 0) and added.
+template <int CTRL, int ROW_MASK, int BANK_MASK>
+__device__ __forceinline__ double dpp_shifted(double v) {
+    const uint64_t x = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = __builtin_amdgcn_update_dpp(0u, uint32_t(x), CTRL, ROW_MASK, BANK_MASK, true);
+    const uint32_t hi = __builtin_amdgcn_update_dpp(0u, uint32_t(x >> 32), CTRL, ROW_MASK, BANK_MASK, true);
+    return __builtin_bit_cast(double, (uint64_t(hi) << 32) | lo);
+}
+
+// Sum over the 64 lanes of a wave, returned to every lane. DPP row shifts
+// and row broadcasts stay in the VALU (a shuffle butterfly would make six
+// dependent LDS round trips); the order is fixed, so the bits are
+// reproducible. Steps: lane i of each 16-lane row gathers lanes i-3..i of the
+// original values (row_shr 1, 2, 3), then i-7..i (row_shr 4 on banks 1-3) and
+// the row prefix (row_shr 8 on banks 2-3); row_bcast 15 and 31 carry the row
+// totals upward, and lane 63 ends with the wave total.
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
+    double s = v + dpp_shifted<0x111, 0xf, 0xf>(v);  // row_shr:1
+    s += dpp_shifted<0x112, 0xf, 0xf>(v);            // row_shr:2
+    s += dpp_shifted<0x113, 0xf, 0xf>(v);            // row_shr:3
+    s += dpp_shifted<0x114, 0xf, 0xe>(s);            // row_shr:4, banks 1-3
+    s += dpp_shifted<0x118, 0xf, 0xc>(s);            // row_shr:8, banks 2-3
+    s += dpp_shifted<0x142, 0xa, 0xf>(s);            // row_bcast:15, rows 1, 3
+    s += dpp_shifted<0x143, 0xc, 0xf>(s);            // row_bcast:31, rows 2, 3
+    const uint64_t x = __builtin_bit_cast(uint64_t, s);
+    const uint32_t lo = __builtin_amdgcn_readlane(uint32_t(x), 63);
+    const uint32_t hi = __builtin_amdgcn_readlane(uint32_t(x >> 32), 63);
+    return __builtin_bit_cast(double, (uint64_t(hi) << 32) | lo);
 }
 __device__ __forceinline__ double wave_max(double v) {
 #pragma unroll

@@ -44,7 +44,7 @@
 namespace vx {
 namespace {
 
-constexpr uint32_t kInnerChunk = 16;  // inner iterations per graph replay
+constexpr uint32_t kInnerChunk = 16;  // default inner iterations per tick (graph replay)
 constexpr uint32_t kFreeTile = 0xFFFFFFFFu;
 
 // Phase bits of a scene for one tick.
@@ -1008,6 +1008,7 @@ struct vx_batch {
     std::vector<std::pair<uint64_t, hipGraphExec_t>> g_inner;
     bool single = false;  // one-scene arena of a vx_scene (never re-admits)
     uint32_t hw_tiles = 0;  // one past the highest tile in use
+    uint32_t iters_per_tick = vx::kInnerChunk;
 };
 
 struct vx_scene {
@@ -1025,7 +1026,7 @@ struct Lock {
 };
 
 // One inner chunk: the refinement residual of scenes whose inner solve has
-// ended (device-decided), then kInnerChunk iterations.
+// ended (device-decided), then the arena's iterations per tick.
 int32_t capture_inner(vx_batch *b, int f64, uint32_t tiles, bool big, hipGraphExec_t *out) {
     Context &ctx = context();
     hipGraph_t graph;
@@ -1050,7 +1051,7 @@ int32_t capture_inner(vx_batch *b, int f64, uint32_t tiles, bool big, hipGraphEx
     // goes into the kernel arguments.
     const OneScene one = b->single ? make_one(b->slots[0].geo) : OneScene{};
     const dim3 grid1(b->single ? one.grid : tiles);
-    for (uint32_t t = 0; t < kInnerChunk; ++t) {
+    for (uint32_t t = 0; t < b->iters_per_tick; ++t) {
         const int odd = int(t & 1);
         if (f64 && b->single)
             hipLaunchKernelGGL((pcg_bond_spmv<double, true>), grid1, blk, 0, ctx.stream, b->A, one, odd);
@@ -1085,7 +1086,9 @@ uint32_t launch_tiles(const vx_batch *b) {
     const uint64_t unit = uint64_t(kXcds) * b->chunk;
     uint64_t g = unit;
     while (g < b->hw_tiles) g *= 2;
-    return uint32_t(std::min<uint64_t>(g, b->cap_tiles));
+    // Stay a whole number of XCD rounds (logical_tile is a bijection on
+    // [0, g) only then); tiles past the arena's capacity exit at once.
+    return uint32_t(std::min<uint64_t>(g, (b->cap_tiles + unit - 1) / unit * unit));
 }
 
 // Whether any admitted scene takes its inner partials through scene_reduce.
@@ -1631,6 +1634,25 @@ int32_t scene_ok(vx_scene *sc) { return sc && sc->batch ? VX_OK : fail(VX_ERR_IN
 
 using namespace vx;
 
+// Drop the captured chunk graphs (their shape changed).
+static void reset_graphs(vx_batch *b) {
+    for (auto &kv : b->g_inner)
+        if (kv.second) (void)hipGraphExecDestroy(kv.second);
+    b->g_inner.clear();
+}
+
+static int32_t set_schedule(vx_batch *b, uint32_t chunk, uint32_t iters) {
+    if (iters > 256) return fail(VX_ERR_INVALID_ARGUMENT, "at most 256 iterations per tick");
+    if (int32_t c = drain(b)) return c;
+    if (chunk) {
+        b->chunk = chunk;
+        b->A.chunk = chunk;
+    }
+    if (iters) b->iters_per_tick = iters;
+    reset_graphs(b);
+    return VX_OK;
+}
+
 // ---- batch API ----------------------------------------------------------------
 extern "C" uint64_t vx_batch_bytes_per_row(void) { return uint64_t(kRowBytes); }
 
@@ -1645,8 +1667,7 @@ extern "C" int32_t vx_batch_destroy(vx_batch *b) {
     Lock lock;
     (void)drain(b);
     (void)hipStreamSynchronize(context().stream);
-    for (auto &kv : b->g_inner)
-        if (kv.second) (void)hipGraphExecDestroy(kv.second);
+    reset_graphs(b);
     for (auto &f : b->fl) {
         if (f.bits) (void)hipHostFree(f.bits);
         if (f.st) (void)hipHostFree(f.st);
@@ -1655,6 +1676,13 @@ extern "C" int32_t vx_batch_destroy(vx_batch *b) {
     if (b->alloc) (void)hipFree(b->alloc);
     delete b;
     return VX_OK;
+}
+
+extern "C" int32_t vx_batch_set_schedule(vx_batch *b, uint32_t xcd_chunk_tiles, uint32_t iterations_per_tick) {
+    clear_error();
+    if (!b) return fail(VX_ERR_INVALID_ARGUMENT, "null batch");
+    Lock lock;
+    return set_schedule(b, xcd_chunk_tiles, iterations_per_tick);
 }
 
 extern "C" int32_t vx_batch_free_rows(vx_batch *b, uint64_t *largest, uint64_t *total) {
@@ -1804,6 +1832,13 @@ extern "C" int32_t vx_scene_create(const vx_scene_desc *desc, vx_scene **out) {
     sc->active = b->slots[slot].active;
     *out = sc;
     return VX_OK;
+}
+
+extern "C" int32_t vx_scene_set_schedule(vx_scene *sc, uint32_t iterations_per_tick) {
+    clear_error();
+    if (int32_t c = scene_ok(sc)) return c;
+    Lock lock;
+    return set_schedule(sc->batch, 0, iterations_per_tick);
 }
 
 extern "C" int32_t vx_scene_destroy(vx_scene *sc) {

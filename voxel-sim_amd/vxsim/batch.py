@@ -33,12 +33,24 @@ def rows_for_bytes(budget: int) -> int:
 
 
 class Batch:
-    def __init__(self, capacity_rows: int, max_scenes: int = 256, xcd_chunk_tiles: int = 0):
+    def __init__(self, capacity_rows: int, max_scenes: int = 256, xcd_chunk_tiles: int = 0,
+                 scene_revs_hint: float = None):
+        """An arena of `capacity_rows` rows for up to `max_scenes` scenes. With
+        `scene_revs_hint` (typical REVs per scene) and no explicit chunk, the
+        schedule (XCD interleave, iterations per tick) comes from the schedule
+        table (vxsim.tune)."""
         h = ctypes.c_void_p()
         check(lib().vx_batch_create(int(capacity_rows), int(max_scenes), int(xcd_chunk_tiles), ctypes.byref(h)))
         self.handle = h
         self.max_scenes = int(max_scenes)
         self._rows = {}
+        if scene_revs_hint is not None and max_scenes > 1:
+            from . import tune
+            sched = tune.lookup(scene_revs_hint, kind="batch")
+            self.set_schedule(0 if xcd_chunk_tiles else sched["chunk"], sched["iters"])
+
+    def set_schedule(self, xcd_chunk_tiles: int = 0, iterations_per_tick: int = 0):
+        check(lib().vx_batch_set_schedule(self.handle, int(xcd_chunk_tiles), int(iterations_per_tick)))
 
     def close(self):
         if getattr(self, "handle", None):

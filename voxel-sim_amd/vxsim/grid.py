@@ -63,7 +63,9 @@ def step_params(fixed_dt=0.001, gravity=(0.0, 0.0, -9.81), substeps=1, corotatio
 
 
 class GridScene:
-    def __init__(self, scene):
+    def __init__(self, scene, tuned: bool = True):
+        """Upload `scene`; with `tuned`, the iterations per tick come from the
+        schedule table for the scene's size (vxsim.tune)."""
         desc, keep = scene_desc(scene)
         nx, ny, nz = scene.dims
         handle = ctypes.c_void_p()
@@ -72,6 +74,9 @@ class GridScene:
         self.n = scene.n
         self.dims = (nx, ny, nz)
         self.active = scene.active_count()
+        if tuned:
+            from . import tune
+            self.set_schedule(tune.lookup(scene.n, kind="single")["iters"])
 
     def close(self):
         if getattr(self, "handle", None):
@@ -151,6 +156,10 @@ class GridScene:
         if cnt.value != na:
             raise SolverError(6, "active row count mismatch %d != %d" % (cnt.value, na))
         return dict(active=active, neighbors=nbr, blocks=blocks, diagonal=diag, rhs=rhs)
+
+    def set_schedule(self, iterations_per_tick: int):
+        """Inner PCG iterations per scheduling tick (graph replay)."""
+        check(lib().vx_scene_set_schedule(self.handle, int(iterations_per_tick)))
 
     def time_spmv(self, reps=50) -> float:
         us = ctypes.c_double(0.0)
