@@ -52,6 +52,34 @@ class _StepReport(ctypes.Structure):
                 ("solver_residual", ctypes.c_double), ("last_angle", ctypes.c_double)]
 
 
+class _FractureConfig(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_double) for k in ("damage_rate", "energy_release", "fatigue_cycles",
+                                               "fatigue_exponent", "fatigue_threshold", "plastic_heat_fraction")]
+
+
+class _Bond(ctypes.Structure):
+    _fields_ = [("neighbor", ctypes.c_uint32), ("plastic_strain", ctypes.c_double * 3)] + [
+        (k, ctypes.c_double) for k in ("damage", "equivalent_plastic_strain", "plastic_work", "fatigue_damage",
+                                       "previous_stress_ratio", "reversal_stress_ratio", "loading_increment")]
+
+
+class _Dissipation(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_double) for k in ("plastic_heat", "stored_plastic", "surface_work", "damage_heat",
+                                               "released")]
+
+
+class _FractureState(ctypes.Structure):
+    _fields_ = [("neighbor", _u32p), ("damage", _dp), ("plastic_strain", _dp),
+                ("equivalent_plastic_strain", _dp), ("plastic_work", _dp), ("fatigue_damage", _dp),
+                ("previous_stress_ratio", _dp), ("reversal_stress_ratio", _dp), ("loading_increment", _dp),
+                ("strength", _dp), ("config", _FractureConfig), ("dissipated", _Dissipation),
+                ("broken", ctypes.c_uint32)]
+
+
+HISTORY_FIELDS = ("equivalent_plastic_strain", "plastic_work", "fatigue_damage", "previous_stress_ratio",
+                  "reversal_stress_ratio", "loading_increment")
+BOND_FIELDS = ("damage",) + HISTORY_FIELDS
+
 _lib = None
 
 
@@ -88,6 +116,12 @@ def lib():
         L.vxo_matvec.argtypes = [ctypes.c_int, ctypes.c_uint32, _u32p, _dp, _dp, _dp, _dp]
         L.vxo_step.argtypes = [ctypes.c_int, ctypes.POINTER(_Scene), ctypes.POINTER(_StepConfig), _dp, _dp,
                                ctypes.POINTER(_StepReport), _dp]
+        L.vxo_pair_limits.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
+        L.vxo_fatigue.argtypes = [ctypes.c_int, ctypes.POINTER(_Bond), ctypes.c_double, ctypes.POINTER(_FractureConfig)]
+        L.vxo_fracture_update.argtypes = [ctypes.c_int, ctypes.POINTER(_Bond), _dp, _dp, _dp, ctypes.c_double,
+                                          ctypes.POINTER(_FractureConfig), ctypes.POINTER(_Dissipation)]
+        L.vxo_step_fracture.argtypes = [ctypes.c_int, ctypes.POINTER(_Scene), ctypes.POINTER(_StepConfig), _dp, _dp,
+                                        ctypes.POINTER(_FractureState), ctypes.POINTER(_StepReport)]
         L.vxo_dot_rows.restype = ctypes.c_double
         L.vxo_dot_rows.argtypes = [ctypes.c_uint32, _dp, _dp]
         L.vxo_guarded_ratio.restype = ctypes.c_double
@@ -123,11 +157,13 @@ class OracleScene:
         self.delta_temp = _f64(scene.delta_temp)
         self.alpha_par = _f64(scene.alpha_par)
         self.alpha_perp = _f64(scene.alpha_perp)
-        self.neighbor = np.ascontiguousarray(scene.neighbor, dtype=np.uint32)
-        self.damage = _f64(scene.damage)
+        # Topology, damage and plastic strain are copies: the fracture pass
+        # writes them.
+        self.neighbor = np.array(scene.neighbor, dtype=np.uint32, order="C")
+        self.damage = np.array(scene.damage, dtype=np.float64, order="C")
         self.kscale = _f64(scene.kscale)
         self.formation_offset = _f64(scene.formation_offset)
-        self.plastic_strain = _f64(scene.plastic_strain)
+        self.plastic_strain = np.array(scene.plastic_strain, dtype=np.float64, order="C")
         self.struct = _Scene(
             scene.n,
             _p(self.positions, _dp), _p(self.rotations, _dp), _p(self.velocities, _dp),
@@ -298,3 +334,80 @@ def step(oscene: OracleScene, cfg, mode=F64, precond=BJ, exact=False, exact_tole
     if want_forces:
         out["last_force"] = forces
     return out
+
+
+# ---- fracture --------------------------------------------------------------
+def _fcfg(c):
+    return _FractureConfig(c.damage_rate, c.energy_release, c.fatigue_cycles, c.fatigue_exponent,
+                           c.fatigue_threshold, c.plastic_heat_fraction)
+
+
+def pair_limits(a, b, alignment, anisotropy, mode=F64):
+    """pair_limits (src/fracture.rs:77-133) for two strength rows."""
+    out = np.empty(8)
+    lib().vxo_pair_limits(mode, _p(_f64(a), _dp), _p(_f64(b), _dp), _p(_f64(alignment), _dp),
+                          _p(_f64(anisotropy), _dp), _p(out, _dp))
+    return out
+
+
+def bond(neighbor=1, **kw):
+    b = _Bond()
+    b.neighbor = neighbor
+    for k, v in kw.items():
+        if k == "plastic_strain":
+            b.plastic_strain[:] = list(v)
+        else:
+            setattr(b, k, v)
+    return b
+
+
+def bond_dict(b):
+    d = {k: getattr(b, k) for k, _ in _Bond._fields_ if k != "plastic_strain"}
+    d["plastic_strain"] = np.array(b.plastic_strain[:])
+    return d
+
+
+def fatigue(b, signed_ratio, cfg, mode=F64):
+    lib().vxo_fatigue(mode, ctypes.byref(b), float(signed_ratio), ctypes.byref(_fcfg(cfg)))
+    return b
+
+
+def fracture_update(b, u_total, k, limits, dt, cfg, mode=F64):
+    """fracture::update (src/fracture.rs:199-292) on a _Bond in place; returns
+    the dissipation dict."""
+    d = _Dissipation()
+    lib().vxo_fracture_update(mode, ctypes.byref(b), _p(_f64(u_total), _dp), _p(_f64(np.asarray(k)).ravel(), _dp),
+                              _p(_f64(limits), _dp), float(dt), ctypes.byref(_fcfg(cfg)), ctypes.byref(d))
+    return {k: getattr(d, k) for k, _ in _Dissipation._fields_}
+
+
+class FractureHistory:
+    """Bond history beyond the scene (3 x n each), zero for a new scene."""
+
+    def __init__(self, n):
+        for k in HISTORY_FIELDS:
+            setattr(self, k, np.zeros((3, n)))
+
+
+def step_fracture(oscene: OracleScene, cfg, fcfg, history: FractureHistory, strength, mode=F64, precond=BJ,
+                  exact=False, exact_tolerance=1e-12, targets=None, loads=None):
+    """One fixed step (solve part) with the fracture pass of every substep
+    (src/solver.rs:627-665). Updates oscene.neighbor / damage / plastic_strain
+    and `history`; returns the step report with the broken-bond count and
+    the dissipated energies of the step."""
+    c = _StepConfig(cfg.fixed_dt, cfg.substeps, cfg.iterations, cfg.corotation_iterations,
+                    cfg.solver_relative_tolerance, cfg.corotation_tolerance,
+                    (ctypes.c_double * 3)(*cfg.gravity), precond, 1 if exact else 0, exact_tolerance)
+    st = np.ascontiguousarray(strength, dtype=np.float64)
+    fs = _FractureState(_p(oscene.neighbor, _u32p), _p(oscene.damage, _dp), _p(oscene.plastic_strain, _dp),
+                        *[_p(getattr(history, k), _dp) for k in HISTORY_FIELDS], _p(st, _dp), _fcfg(fcfg),
+                        _Dissipation(), 0)
+    rep = _StepReport()
+    tg = None if targets is None else _f64(targets).ravel()
+    ld = None if loads is None else _f64(loads).ravel()
+    lib().vxo_step_fracture(mode, ctypes.byref(oscene.struct), ctypes.byref(c), _p(tg, _dp), _p(ld, _dp),
+                            ctypes.byref(fs), ctypes.byref(rep))
+    return dict(pcg_iterations=rep.pcg_iterations, passes=rep.passes, solver_residual=rep.solver_residual,
+                last_angle=rep.last_angle, broken=fs.broken,
+                dissipated={k: getattr(fs.dissipated, k) for k, _ in _Dissipation._fields_})
+

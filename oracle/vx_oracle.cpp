@@ -286,6 +286,153 @@ template <class T> Constraint<T> make_constraint(const Scene<T> &sc, uint32_t i,
     return c;
 }
 
+// ---------------------------------------------------------------------------
+// Fracture law (src/fracture.rs) in scalar type T; energies in double.
+// ---------------------------------------------------------------------------
+template <class T> struct Limits {
+    T tension, compression, shear, yield, hardening, plastic_failure, ductility, gc;
+};
+
+// directional (src/fracture.rs:74-76).
+template <class T> T directional(T parallel, T perpendicular, T alignment, T anisotropy) {
+    T a = std::min(std::max(std::fabs(alignment), T(0)), T(1));
+    return parallel + anisotropy * (T(1) - a) * (perpendicular - parallel);
+}
+
+// pair_limits (src/fracture.rs:77-133) for strength rows a, b.
+template <class T> Limits<T> pair_limits(const double *a, const double *b, const T (&alignment)[2],
+                                          const T (&anisotropy)[2]) {
+    auto one = [&](const double *m, int k) {
+        Limits<T> l;
+        const int mode = int(m[VXO_S_MODE]);
+        l.ductility = mode == VXO_BRITTLE ? T(0)
+                      : mode == VXO_DUCTILE ? T(1)
+                                            : (T(1) - anisotropy[k]) * T(0.5) + anisotropy[k] * (alignment[k] * alignment[k]);
+        l.tension = directional(T(m[VXO_S_TENSION]), T(m[VXO_S_TENSION_PERP]), alignment[k], anisotropy[k]);
+        l.compression = directional(T(m[VXO_S_COMPRESSION]), T(m[VXO_S_COMPRESSION_PERP]), alignment[k], anisotropy[k]);
+        l.shear = directional(T(m[VXO_S_SHEAR]), T(m[VXO_S_SHEAR_PERP]), alignment[k], anisotropy[k]);
+        l.yield = T(m[VXO_S_YIELD]);
+        l.hardening = T(m[VXO_S_HARDENING]);
+        l.plastic_failure = T(m[VXO_S_PLASTIC_FAILURE]);
+        l.gc = directional(T(m[VXO_S_GC]), T(m[VXO_S_GC_PERP]), alignment[k], anisotropy[k]);
+        return l;
+    };
+    Limits<T> x = one(a, 0), y = one(b, 1);
+    return Limits<T>{std::min(x.tension, y.tension),     std::min(x.compression, y.compression),
+                     std::min(x.shear, y.shear),         std::min(x.yield, y.yield),
+                     std::min(x.hardening, y.hardening), std::min(x.plastic_failure, y.plastic_failure),
+                     std::min(x.ductility, y.ductility), std::min(x.gc, y.gc)};
+}
+
+template <class T> struct BondT {
+    uint32_t neighbor;
+    V3<T> plastic;
+    T damage, eq_plastic, plastic_work, fatigue_damage, previous, reversal, increment;
+};
+template <class T> BondT<T> bond_in(const vxo_bond &b) {
+    return BondT<T>{b.neighbor, V3<T>{T(b.plastic_strain[0]), T(b.plastic_strain[1]), T(b.plastic_strain[2])},
+                    T(b.damage), T(b.equivalent_plastic_strain), T(b.plastic_work), T(b.fatigue_damage),
+                    T(b.previous_stress_ratio), T(b.reversal_stress_ratio), T(b.loading_increment)};
+}
+template <class T> void bond_out(const BondT<T> &t, vxo_bond &b) {
+    b.neighbor = t.neighbor;
+    b.plastic_strain[0] = double(t.plastic.x);
+    b.plastic_strain[1] = double(t.plastic.y);
+    b.plastic_strain[2] = double(t.plastic.z);
+    b.damage = double(t.damage);
+    b.equivalent_plastic_strain = double(t.eq_plastic);
+    b.plastic_work = double(t.plastic_work);
+    b.fatigue_damage = double(t.fatigue_damage);
+    b.previous_stress_ratio = double(t.previous);
+    b.reversal_stress_ratio = double(t.reversal);
+    b.loading_increment = double(t.increment);
+}
+
+// energy (src/fracture.rs:175-177).
+template <class T> double bond_energy(V3<T> u, const M3<T> &k, T damage) {
+    return double(std::max(T(0.5) * (T(1) - damage) * dot(u, k * u), T(0)));
+}
+
+// fatigue (src/fracture.rs:178-196): only stress reversals consume life.
+template <class T> void fatigue(BondT<T> &b, T signed_ratio, const vxo_fracture_config *c) {
+    const T delta = signed_ratio - b.previous;
+    if (std::fabs(delta) > T(1e-6)) {
+        if (b.increment * delta < T(0)) {
+            const T amplitude = std::fabs(b.previous - b.reversal) * T(0.5);
+            const T threshold = T(c->fatigue_threshold);
+            if (amplitude > threshold) {
+                const T level = std::min((amplitude - threshold) / (T(1) - threshold), T(1));
+                b.fatigue_damage = std::min(
+                    b.fatigue_damage + T(0.5) * T(std::pow(level, T(c->fatigue_exponent))) / T(c->fatigue_cycles), T(1));
+            }
+            b.reversal = b.previous;
+        }
+        b.increment = delta;
+    }
+    b.previous = signed_ratio;
+}
+
+// update (src/fracture.rs:199-292).
+template <class T>
+vxo_dissipation fracture_update(BondT<T> &b, V3<T> u_total, const M3<T> &k, const Limits<T> &l, T dt,
+                                const vxo_fracture_config *c) {
+    vxo_dissipation r{0, 0, 0, 0, 0};
+    if (b.neighbor == kEmpty) return r;
+    const T cell = T(kCell), area = cell * cell;
+    const T old_damage = b.damage;
+    V3<T> u = u_total - b.plastic * cell;
+    const double before = bond_energy(u, k, old_damage);
+    const V3<T> trial = (k * u) / area;
+    if (l.ductility > T(1e-5)) {
+        const T equivalent = std::sqrt(dot(trial, trial));
+        const T yield_now = l.yield + l.hardening * b.eq_plastic;
+        if (equivalent > yield_now) {
+            const V3<T> direction = trial / equivalent;
+            const V3<T> compliance_direction = (inverse(k) * (direction * area)) / cell;
+            const T elastic_modulus = T(1) / std::max(std::sqrt(dot(compliance_direction, compliance_direction)), T(1e-20));
+            const T dep = (equivalent - yield_now) / (elastic_modulus + l.hardening) * l.ductility;
+            const V3<T> dp = normalize_or_zero(compliance_direction) * dep;
+            b.plastic = b.plastic + dp;
+            b.eq_plastic += dep;
+            u = u_total - b.plastic * cell;
+            const double lost = std::max(before - bond_energy(u, k, old_damage), 0.0);
+            b.plastic_work += T(lost);
+            r.plastic_heat = lost * double(T(c->plastic_heat_fraction));
+            r.stored_plastic = lost - r.plastic_heat;
+        }
+    }
+    const V3<T> stress = (k * u) / area;
+    const T normal_limit = stress.x >= T(0) ? l.tension : l.compression;
+    const T sx = stress.x / std::max(normal_limit, T(1));
+    const T sh = std::max(l.shear, T(1));
+    const T ratio = std::sqrt(sx * sx + (stress.y * stress.y + stress.z * stress.z) / (sh * sh));
+    T principal;
+    if (std::fabs(stress.x) >= std::max(std::fabs(stress.y), std::fabs(stress.z)))
+        principal = stress.x;
+    else if (std::fabs(stress.y) >= std::fabs(stress.z))
+        principal = stress.y;
+    else
+        principal = stress.z;
+    fatigue(b, std::copysign(ratio, principal), c);
+    if (ratio >= T(1)) {
+        if (l.ductility <= T(1e-5))
+            b.damage = T(1);
+        else
+            b.damage = std::min(b.damage + T(c->damage_rate) * dt * std::min(ratio, T(10)), T(1));
+    }
+    if (l.ductility > T(1e-5) && b.eq_plastic >= std::max(l.plastic_failure, T(1e-6)))
+        b.damage = std::min(b.damage + T(c->damage_rate) * dt, T(1));
+    if (b.fatigue_damage >= T(1)) b.damage = T(1);
+    const double lost = std::max(bond_energy(u, k, old_damage) - bond_energy(u, k, b.damage), 0.0);
+    const double budget = double(l.gc) * double(area) * double(b.damage - old_damage);
+    r.surface_work = std::min(lost, budget);
+    const double available = lost - r.surface_work;
+    r.released = available * double(T(c->energy_release));
+    r.damage_heat = available - r.released;
+    if (b.damage >= T(1)) b.neighbor = kEmpty;
+    return r;
+}
+
 // fit_rotations (src/solver.rs:327-417).
 template <class T> T fit_rotations(vxo_scene *s) {
     Scene<T> sc{s};
@@ -1133,12 +1280,123 @@ double vxo_guarded_ratio(int mode, double num, double den) {
     return mode == VXO_F32 ? double(guarded_ratio<float>(num, den)) : guarded_ratio<double>(num, den);
 }
 
+void vxo_pair_limits(int mode, const double *a, const double *b, const double alignment[2], const double anisotropy[2],
+                     double limits[VXO_LIMITS]) {
+    auto put = [&](auto l) {
+        const double v[VXO_LIMITS] = {double(l.tension), double(l.compression), double(l.shear), double(l.yield),
+                                      double(l.hardening), double(l.plastic_failure), double(l.ductility), double(l.gc)};
+        std::memcpy(limits, v, sizeof v);
+    };
+    if (mode == VXO_F32) {
+        const float al[2] = {float(alignment[0]), float(alignment[1])}, an[2] = {float(anisotropy[0]), float(anisotropy[1])};
+        put(pair_limits<float>(a, b, al, an));
+    } else {
+        const double al[2] = {alignment[0], alignment[1]}, an[2] = {anisotropy[0], anisotropy[1]};
+        put(pair_limits<double>(a, b, al, an));
+    }
+}
+
+template <class T> static void fatigue_t(vxo_bond *bond, double signed_ratio, const vxo_fracture_config *c) {
+    BondT<T> b = bond_in<T>(*bond);
+    fatigue(b, T(signed_ratio), c);
+    bond_out(b, *bond);
+}
+void vxo_fatigue(int mode, vxo_bond *bond, double signed_ratio, const vxo_fracture_config *c) {
+    if (mode == VXO_F32)
+        fatigue_t<float>(bond, signed_ratio, c);
+    else
+        fatigue_t<double>(bond, signed_ratio, c);
+}
+
+template <class T>
+static void fracture_update_t(vxo_bond *bond, const double u_total[3], const double k[9], const double lim[VXO_LIMITS],
+                              double dt, const vxo_fracture_config *c, vxo_dissipation *out) {
+    BondT<T> b = bond_in<T>(*bond);
+    M3<T> km;
+    for (int r = 0; r < 3; ++r)
+        for (int col = 0; col < 3; ++col) km.c[col][r] = T(k[3 * r + col]);
+    const Limits<T> l{T(lim[0]), T(lim[1]), T(lim[2]), T(lim[3]), T(lim[4]), T(lim[5]), T(lim[6]), T(lim[7])};
+    vxo_dissipation d = fracture_update(b, V3<T>{T(u_total[0]), T(u_total[1]), T(u_total[2])}, km, l, T(dt), c);
+    bond_out(b, *bond);
+    if (out) *out = d;
+}
+void vxo_fracture_update(int mode, vxo_bond *bond, const double u_total[3], const double k[9],
+                         const double limits[VXO_LIMITS], double dt, const vxo_fracture_config *c, vxo_dissipation *out) {
+    if (mode == VXO_F32)
+        fracture_update_t<float>(bond, u_total, k, limits, dt, c, out);
+    else
+        fracture_update_t<double>(bond, u_total, k, limits, dt, c, out);
+}
+
+// The fracture pass of a substep (src/solver.rs:627-665): every constraint of
+// the last assembly whose bond is intact and whose -end REV is not kinematic,
+// with bond_limits (src/solver.rs:124-161; thermal strength scale 1) and the
+// elastic displacement in the constraint's frame (src/solver.rs:106-110).
+template <class T>
+static void fracture_pass(const System<T> &sys, vxo_scene *s, vxo_fracture_state *fr, T h) {
+    Scene<T> sc{s};
+    const uint32_t n = s->n;
+    for (const Constraint<T> &c : sys.constraints) {
+        const uint32_t i = c.i, j = c.j;
+        const size_t b = size_t(c.axis) * n + i;
+        if (fr->neighbor[b] == kEmpty || s->kinematic[i]) continue;
+        const V3<T> direction = c.frame.c[0];
+        T alignment[2], anisotropy[2];
+        const uint32_t ends[2] = {i, j};
+        T solid = T(1);
+        for (int e = 0; e < 2; ++e) {
+            alignment[e] = dot(direction, qrotate(sc.rot(ends[e]), sc.fib(ends[e])));
+            anisotropy[e] = T(s->anisotropy[ends[e]]);
+            solid = std::min(solid, std::min(std::max(T(s->solid[ends[e]]), T(0)), T(1)));
+        }
+        Limits<T> l = pair_limits<T>(fr->strength + size_t(VXO_STRENGTH) * i, fr->strength + size_t(VXO_STRENGTH) * j,
+                                     alignment, anisotropy);
+        const T scale = solid * solid;
+        l.tension *= scale;
+        l.compression *= scale;
+        l.shear *= scale;
+        l.yield *= scale;
+        l.hardening *= scale;
+        l.gc *= scale;
+        const V3<T> off = s->formation_offset ? sc.bond3(s->formation_offset, c.axis, i) : V3<T>{0, 0, 0};
+        const V3<T> u = transpose(c.frame) * (sc.pos(j) - sc.pos(i)) - bond_thermal_rest(sc, i, j, c.frame) - off;
+        vxo_bond bond;
+        bond.neighbor = fr->neighbor[b];
+        for (int a = 0; a < 3; ++a) bond.plastic_strain[a] = fr->plastic_strain[3 * b + a];
+        bond.damage = fr->damage[b];
+        bond.equivalent_plastic_strain = fr->equivalent_plastic_strain[b];
+        bond.plastic_work = fr->plastic_work[b];
+        bond.fatigue_damage = fr->fatigue_damage[b];
+        bond.previous_stress_ratio = fr->previous_stress_ratio[b];
+        bond.reversal_stress_ratio = fr->reversal_stress_ratio[b];
+        bond.loading_increment = fr->loading_increment[b];
+        BondT<T> bt = bond_in<T>(bond);
+        const vxo_dissipation d = fracture_update(bt, u, c.material_k, l, h, &fr->config);
+        bond_out(bt, bond);
+        fr->neighbor[b] = bond.neighbor;
+        for (int a = 0; a < 3; ++a) fr->plastic_strain[3 * b + a] = bond.plastic_strain[a];
+        fr->damage[b] = bond.damage;
+        fr->equivalent_plastic_strain[b] = bond.equivalent_plastic_strain;
+        fr->plastic_work[b] = bond.plastic_work;
+        fr->fatigue_damage[b] = bond.fatigue_damage;
+        fr->previous_stress_ratio[b] = bond.previous_stress_ratio;
+        fr->reversal_stress_ratio[b] = bond.reversal_stress_ratio;
+        fr->loading_increment[b] = bond.loading_increment;
+        fr->dissipated.plastic_heat += d.plastic_heat;
+        fr->dissipated.stored_plastic += d.stored_plastic;
+        fr->dissipated.surface_work += d.surface_work;
+        fr->dissipated.damage_heat += d.damage_heat;
+        fr->dissipated.released += d.released;
+        if (bond.neighbor == kEmpty) fr->broken += 1;
+    }
+}
+
 // Solve part of step_with_axis_targets_and_loads (src/solver.rs:446-570,
 // 668-683): predict, impose targets, fit rotations, corotation passes of
 // assemble + PCG + apply, then velocities.
 template <class T>
 static void step_t(vxo_scene *s, const vxo_step_config *cfg, const double *targets, const double *loads,
-                   vxo_step_report *report, double *last_force) {
+                   vxo_step_report *report, double *last_force, vxo_fracture_state *fracture = nullptr) {
     const uint32_t n = s->n;
     const T h = T(cfg->fixed_dt) / T(cfg->substeps);
     const V3<T> g{T(cfg->gravity[0]), T(cfg->gravity[1]), T(cfg->gravity[2])};
@@ -1207,6 +1465,7 @@ static void step_t(vxo_scene *s, const vxo_step_config *cfg, const double *targe
             rep.last_angle = double(angle);
             if (double(angle) <= cfg->corotation_tolerance && residual <= cfg->solver_relative_tolerance) break;
         }
+        if (fracture) fracture_pass<T>(sys, s, fracture, h);
         // Velocities from the solved positions (no contact): src/solver.rs:668-683.
         for (uint32_t i = 0; i < n; ++i) {
             if (s->kinematic[i]) continue;
@@ -1250,3 +1509,10 @@ void vxo_step(int mode, vxo_scene *s, const vxo_step_config *cfg, const double *
         step_t<double>(s, cfg, targets, loads, report, last_force);
 }
 
+void vxo_step_fracture(int mode, vxo_scene *s, const vxo_step_config *cfg, const double *targets, const double *loads,
+                       vxo_fracture_state *fracture, vxo_step_report *report) {
+    if (mode == VXO_F32)
+        step_t<float>(s, cfg, targets, loads, report, nullptr, fracture);
+    else
+        step_t<double>(s, cfg, targets, loads, report, nullptr, fracture);
+}

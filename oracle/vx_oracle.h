@@ -166,6 +166,62 @@ double vxo_dot_rows(uint32_t n, const double *a, const double *b);
  * num/den cast to the vector type when den > 1e-30, else 0. */
 double vxo_guarded_ratio(int mode, double num, double den);
 
+/* ---- fracture (src/fracture.rs:73-133, 199-292; src/solver.rs:106-161, 627-665) */
+#define VXO_BRITTLE 0
+#define VXO_DUCTILE 1
+#define VXO_MIXED 2
+/* Per-REV strength (Material, src/material.rs:27-62): VXO_STRENGTH doubles
+ * per REV in this order. */
+enum {
+    VXO_S_TENSION, VXO_S_TENSION_PERP, VXO_S_COMPRESSION, VXO_S_COMPRESSION_PERP, VXO_S_SHEAR,
+    VXO_S_SHEAR_PERP, VXO_S_YIELD, VXO_S_HARDENING, VXO_S_PLASTIC_FAILURE, VXO_S_MODE, VXO_S_GC,
+    VXO_S_GC_PERP, VXO_STRENGTH
+};
+/* Limits (src/fracture.rs:63-72), 8 doubles in this order. */
+enum { VXO_L_TENSION, VXO_L_COMPRESSION, VXO_L_SHEAR, VXO_L_YIELD, VXO_L_HARDENING, VXO_L_PLASTIC_FAILURE,
+       VXO_L_DUCTILITY, VXO_L_GC, VXO_LIMITS };
+
+typedef struct vxo_fracture_config { /* FractureConfig (src/fracture.rs:11-25) */
+    double damage_rate, energy_release, fatigue_cycles, fatigue_exponent, fatigue_threshold, plastic_heat_fraction;
+} vxo_fracture_config;
+
+typedef struct vxo_bond { /* Bond history (src/bond.rs:4-16) */
+    uint32_t neighbor;
+    double plastic_strain[3];
+    double damage, equivalent_plastic_strain, plastic_work, fatigue_damage;
+    double previous_stress_ratio, reversal_stress_ratio, loading_increment;
+} vxo_bond;
+
+typedef struct vxo_dissipation { /* Dissipation (src/fracture.rs:134-140), J */
+    double plastic_heat, stored_plastic, surface_work, damage_heat, released;
+} vxo_dissipation;
+
+/* pair_limits (src/fracture.rs:73-133): a, b are VXO_STRENGTH-value rows. */
+void vxo_pair_limits(int mode, const double *a, const double *b, const double alignment[2],
+                     const double anisotropy[2], double limits[VXO_LIMITS]);
+/* fatigue (src/fracture.rs:178-196). */
+void vxo_fatigue(int mode, vxo_bond *bond, double signed_ratio, const vxo_fracture_config *c);
+/* update (src/fracture.rs:199-292): u_total in the bond frame (m), k the
+ * undamaged local stiffness (row-major 3x3, N/m). */
+void vxo_fracture_update(int mode, vxo_bond *bond, const double u_total[3], const double k[9],
+                         const double limits[VXO_LIMITS], double dt, const vxo_fracture_config *c,
+                         vxo_dissipation *out);
+
+/* Fracture state for vxo_step: the same neighbor / damage / plastic_strain
+ * arrays as the scene (written here), the remaining bond history (3*n each)
+ * and the strength table (VXO_STRENGTH*n). thermal strength scale is 1 (no
+ * temperature model): strength is scaled by min(solid_i, solid_j)^2. */
+typedef struct vxo_fracture_state {
+    uint32_t *neighbor;
+    double *damage, *plastic_strain;
+    double *equivalent_plastic_strain, *plastic_work, *fatigue_damage;
+    double *previous_stress_ratio, *reversal_stress_ratio, *loading_increment;
+    const double *strength;
+    vxo_fracture_config config;
+    vxo_dissipation dissipated;     /* accumulated */
+    uint32_t broken;                /* accumulated */
+} vxo_fracture_state;
+
 /* ---- step driver --------------------------------------------------------- */
 typedef struct vxo_step_config {
     double fixed_dt;
@@ -195,6 +251,11 @@ typedef struct vxo_step_report {
  * last assembly evaluated at the final positions (src/solver.rs:627-640). */
 void vxo_step(int mode, vxo_scene *s, const vxo_step_config *cfg, const double *targets,
               const double *loads, vxo_step_report *report, double *last_force);
+/* vxo_step plus, per substep after the corotation passes and before the
+ * velocities, the fracture pass over the last assembly's constraints
+ * (src/solver.rs:627-665). Bonds that break leave the topology. */
+void vxo_step_fracture(int mode, vxo_scene *s, const vxo_step_config *cfg, const double *targets,
+                       const double *loads, vxo_fracture_state *fracture, vxo_step_report *report);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,35 @@
+"""The C ABI driven from C, as the Rust caller would drive it
+(src/solver.rs:1778-1808): Ok uses the device correction, a nonzero status
+takes the caller's CPU fallback. Compiled with gcc against include/ and the
+in-tree library."""
+import os
+import subprocess
+
+import pytest
+
+from conftest import ROOT, gpu_present
+
+LIBDIR = os.path.join(ROOT, "voxel-sim_amd", "vxsim")
+
+
+@pytest.fixture(scope="module")
+def harness(tmp_path_factory):
+    out = str(tmp_path_factory.mktemp("c_abi") / "harness")
+    subprocess.run(["gcc", "-O2", "-std=c11", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "c_abi", "harness.c"), "-L", LIBDIR, "-lvxsim",
+                    "-Wl,-rpath," + LIBDIR, "-lm", "-o", out], check=True)
+    return out
+
+
+@pytest.mark.skipif(gpu_present(), reason="a HIP device is present")
+def test_c_caller_takes_fallback_without_device(harness):
+    r = subprocess.run([harness], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.startswith("fallback 1 ")
+
+
+@pytest.mark.gpu
+def test_c_caller_uses_device_solution(gpu, harness):
+    r = subprocess.run([harness], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.startswith("device ")

@@ -1,0 +1,196 @@
+"""The oracle's fracture law against the reference's own fracture tests
+(src/fracture.rs tests module): brittle failure, ductile yield before
+tearing, energy accounting, fatigue only under reversals, directional limits
+of wood. Run in the reference's f32 arithmetic and in FP64."""
+import math
+
+import numpy as np
+import pytest
+
+from vxsim.scenes import MATERIALS, FractureConfig
+
+CELL = 0.01
+AREA = CELL * CELL
+EMPTY = 0xFFFFFFFF
+MODES = [0, 1]  # oracle F32 (reference arithmetic), F64
+
+
+def k_of(m):
+    """Mat3::from_diagonal((young, shear, shear) * AREA / CELL) (src/fracture.rs test helper)."""
+    return np.diag([m.young * AREA / CELL, m.shear * AREA / CELL, m.shear * AREA / CELL])
+
+
+def energy(u, k, damage):
+    return max(0.5 * (1.0 - damage) * float(np.dot(u, k @ u)), 0.0)
+
+
+def strength(name):
+    return np.array(MATERIALS[name].strength, dtype=np.float64)
+
+
+def tensile(name):
+    return MATERIALS[name].strength[0]
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_brittle_failure_has_no_plasticity(O, mode):
+    m = MATERIALS["stone"]
+    lim = O.pair_limits(strength("stone"), strength("stone"), [1, 1], [0, 0], mode)
+    b = O.bond(1)
+    O.fracture_update(b, [1.1 * tensile("stone") / m.young * CELL, 0, 0], k_of(m), lim, 0.001, FractureConfig(), mode)
+    assert b.neighbor == EMPTY
+    assert list(b.plastic_strain) == [0.0, 0.0, 0.0]
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_metal_yields_heats_and_only_then_tears(O, mode):
+    m = MATERIALS["iron"]
+    lim = O.pair_limits(strength("iron"), strength("iron"), [1, 1], [0, 0], mode)
+    b = O.bond(1)
+    heat = 0.0
+    for n in range(1, 301):
+        d = O.fracture_update(b, [n * 0.001 * CELL, 0, 0], k_of(m), lim, 0.001, FractureConfig(), mode)
+        heat += d["plastic_heat"]
+        if n == 2:
+            assert b.neighbor != EMPTY and b.equivalent_plastic_strain > 0
+        if b.neighbor == EMPTY:
+            break
+    assert b.neighbor == EMPTY
+    assert heat > 0 and b.equivalent_plastic_strain > 0.1
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_energy_removed_is_not_duplicated(O, mode):
+    m = MATERIALS["stone"]
+    u = np.array([2.0 * tensile("stone") / m.young * CELL, 0, 0])
+    before = energy(u, k_of(m), 0.0)
+    b = O.bond(1)
+    d = O.fracture_update(b, u, k_of(m), O.pair_limits(strength("stone"), strength("stone"), [1, 1], [0, 0], mode),
+                          0.001, FractureConfig(), mode)
+    assert abs(sum(d.values()) - before) < 1e-8 + 1e-6 * before * (mode == 0)
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_sustained_subcritical_load_does_not_count_as_cycles(O, mode):
+    b = O.bond(1)
+    for _ in range(10000):
+        O.fatigue(b, 0.8, FractureConfig(), mode)
+    assert b.fatigue_damage == 0.0
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_cyclic_subcritical_load_accumulates_damage(O, mode):
+    c = FractureConfig(fatigue_cycles=10.0, fatigue_exponent=1.0)
+    b = O.bond(1)
+    for _ in range(50):
+        O.fatigue(b, 0.8, c, mode)
+        O.fatigue(b, -0.8, c, mode)
+    assert b.fatigue_damage == 1.0
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_cyclic_subcritical_stress_breaks_bond_through_constitutive_update(O, mode):
+    m = MATERIALS["iron"]
+    lim = O.pair_limits(strength("iron"), strength("iron"), [1, 1], [0, 0], mode)
+    c = FractureConfig(fatigue_cycles=2.0, fatigue_exponent=1.0, fatigue_threshold=0.1)
+    b = O.bond(1)
+    amplitude = 0.4 * tensile("iron") / m.young * CELL
+    for _ in range(100):
+        d = O.fracture_update(b, [amplitude, 0, 0], k_of(m), lim, 0.001, c, mode)
+        assert b.neighbor == 1 and b.fatigue_damage == 0.0 and sum(d.values()) == 0.0
+    dissipated = 0.0
+    failure = None
+    for _ in range(20):
+        for sign in (-1.0, 1.0):
+            d = O.fracture_update(b, [sign * amplitude, 0, 0], k_of(m), lim, 0.001, c, mode)
+            dissipated += sum(d.values())
+            if b.neighbor == EMPTY:
+                failure = d
+                break
+        if b.neighbor == EMPTY:
+            break
+    assert b.neighbor == EMPTY and b.damage == 1.0 and b.fatigue_damage == 1.0
+    assert dissipated > 0.0 and failure is not None
+    released = energy(np.array([amplitude, 0, 0]), k_of(m), 0.0)
+    assert abs(sum(failure.values()) - released) <= released * 1e-6
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_wood_has_tension_and_compression_anisotropy(O, mode):
+    w = strength("wood_builtin")
+    along = O.pair_limits(w, w, [1, 1], [1, 1], mode)
+    across = O.pair_limits(w, w, [0, 0], [1, 1], mode)
+    assert along[0] / across[0] >= 10 and along[1] / across[1] >= 10
+    assert across[6] == 0.0 and along[6] == 1.0
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_wood_fracture_energy_follows_bond_alignment(O, mode):
+    w = strength("wood_builtin")
+    gc = lambda al, an: O.pair_limits(w, w, [al, al], [an, an], mode)[7]  # noqa: E731
+    assert (gc(1, 1), gc(0, 1), gc(0.5, 1), gc(0, 0)) == (350.0, 80.0, 215.0, 350.0)
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_mixed_wood_shear_failure_does_not_wait_for_higher_yield_stress(O, mode):
+    m = MATERIALS["wood_builtin"]
+    w = strength("wood_builtin")
+    lim = O.pair_limits(w, w, [0.5, 0.5], [1, 1], mode)
+    assert lim[6] > 0.0
+    b = O.bond(1)
+    O.fracture_update(b, [0, 1.1 * lim[2] / m.shear * CELL, 0], k_of(m), lim, 0.001, FractureConfig(), mode)
+    assert b.equivalent_plastic_strain == 0.0
+    assert b.damage > 0.0 and b.neighbor == 1
+
+
+def test_directional_limit_closed_form(O):
+    """directional(par, perp, a, an) = par + an (1 - |a|) (perp - par), and the
+    pair takes the smaller of the two endpoint limits."""
+    w, s = strength("wood_builtin"), strength("stone")
+    lim = O.pair_limits(w, s, [0.25, 1.0], [1.0, 0.0])
+    tension_w = 100e6 + 1.0 * (1 - 0.25) * (2.5e6 - 100e6)
+    assert lim[0] == pytest.approx(min(tension_w, 7e6))
+    assert lim[1] == pytest.approx(min(50e6 + 0.75 * (2e6 - 50e6), 120e6))
+    assert lim[6] == 0.0  # stone is brittle
+    assert math.isclose(lim[7], min(350 + 0.75 * (80 - 350), 80))
+
+
+def stone_column(scenes, pull):
+    """3x3x6 stone REVs, bottom layer held, top layer targeted `pull` metres
+    up in Z (all axes targeted), no gravity."""
+    from vxsim.scenes import StepConfig
+    s = scenes.uniform_box((3, 3, 6), "stone")
+    z = s.keys[:, 2]
+    s.targets[z == z.min()] = s.positions[z == z.min()]
+    s.targets[z == z.max()] = s.positions[z == z.max()] + np.array([0, 0, pull], np.float32)
+    s.config = StepConfig(fixed_dt=0.001, substeps=1, iterations=1000, corotation_iterations=4,
+                          solver_relative_tolerance=1e-10, gravity=(0.0, 0.0, 0.0))
+    return s
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("pull", [2e-6, 5e-5])
+def test_step_fracture_pass_follows_the_solve(O, scenes, mode, pull):
+    """The fracture pass runs after the corotation passes (src/solver.rs:627-665):
+    the step's positions and velocities equal those of the plain step; a
+    sub-critical pull changes no bond, a pull of ~7x the tensile strain breaks
+    every Z bond and only those, each releasing gc * CELL^2 of surface work."""
+    s = stone_column(scenes, pull)
+    a = O.OracleScene(s)
+    nb0 = a.neighbor.copy()
+    rep = O.step_fracture(a, s.config, FractureConfig(), O.FractureHistory(s.n), s.strength, mode,
+                          targets=s.targets, loads=s.loads)
+    b = O.OracleScene(s)
+    O.step(b, s.config, mode, O.BJ, targets=s.targets, loads=s.loads)
+    assert np.array_equal(a.positions, b.positions) and np.array_equal(a.velocities, b.velocities)
+    changed = np.argwhere(a.neighbor != nb0)
+    z_bonds = int(np.count_nonzero(nb0[2] != EMPTY))
+    if pull < 1e-5:
+        assert rep["broken"] == 0 and changed.size == 0
+        assert sum(rep["dissipated"].values()) == 0.0
+    else:
+        assert rep["broken"] == z_bonds == 45
+        assert set(changed[:, 0]) == {2} and np.all(a.neighbor[2][nb0[2] != EMPTY] == EMPTY)
+        gc = MATERIALS["stone"].strength[11]
+        assert rep["dissipated"]["surface_work"] == pytest.approx(45 * gc * AREA, rel=1e-6)
+        assert rep["dissipated"]["plastic_heat"] == 0.0

@@ -62,9 +62,16 @@ class SplitMix64:
         return out[:size]
 
 
+BRITTLE, DUCTILE, MIXED = 0, 1, 2  # FractureMode (src/material.rs:9-18)
+# Order of the per-REV strength row (Material fields, src/material.rs:27-62).
+STRENGTH_FIELDS = ("tension", "tension_perp", "compression", "compression_perp", "shear", "shear_perp",
+                   "yield", "hardening", "plastic_failure", "mode", "gc", "gc_perp")
+
+
 @dataclasses.dataclass(frozen=True)
 class Material:
-    """Elastic and thermal constants used by the solve (src/material.rs:20-63)."""
+    """Elastic, thermal and strength constants used by the solve and the
+    fracture law (src/material.rs:20-63)."""
 
     name: str
     density: float
@@ -73,17 +80,35 @@ class Material:
     shear: float
     alpha_par: float = 0.0
     alpha_perp: float = 0.0
+    strength: tuple = (0.0,) * 12  # STRENGTH_FIELDS order, Pa / - / J m^-2
 
 
-# Built-in values of src/material.rs:105-150 for iron and stone. Wood uses the
-# values of the synthetic workloads (600 kg/m^3, 12 / 0.8 GPa), with the
-# built-in shear modulus and expansion (src/material.rs:96-101, 258-262).
+# Built-in values of src/material.rs:105-150, 277-295 for iron and stone.
+# Wood uses the values of the synthetic workloads (600 kg/m^3, 12 / 0.8 GPa),
+# with the built-in shear modulus, expansion and strength (src/material.rs:
+# 96-116, 277-286): limits 100 / 50 / 12 MPa along the grain and 2.5 / 2 / 3
+# MPa across (tension / compression / shear), yield 25 MPa, mixed mode.
+_WOOD_STRENGTH = (100e6, 2.5e6, 50e6, 2e6, 12e6, 3e6, 25e6, 2e9, 0.04, MIXED, 350.0, 80.0)
 MATERIALS = {
-    "wood": Material("wood", 600.0, 12e9, 0.8e9, 0.65e9, 3.0e-6, 30.0e-6),
-    "wood_builtin": Material("wood_builtin", 650.0, 10e9, 0.6e9, 0.65e9, 3.0e-6, 30.0e-6),
-    "iron": Material("iron", 7870.0, 200e9, 200e9, 77e9, 12.0e-6, 12.0e-6),
-    "stone": Material("stone", 2700.0, 50e9, 50e9, 20e9, 8.0e-6, 8.0e-6),
+    "wood": Material("wood", 600.0, 12e9, 0.8e9, 0.65e9, 3.0e-6, 30.0e-6, _WOOD_STRENGTH),
+    "wood_builtin": Material("wood_builtin", 650.0, 10e9, 0.6e9, 0.65e9, 3.0e-6, 30.0e-6, _WOOD_STRENGTH),
+    "iron": Material("iron", 7870.0, 200e9, 200e9, 77e9, 12.0e-6, 12.0e-6,
+                     (350e6, 350e6, 350e6, 350e6, 200e6, 200e6, 200e6, 1e9, 0.15, DUCTILE, 30000.0, 30000.0)),
+    "stone": Material("stone", 2700.0, 50e9, 50e9, 20e9, 8.0e-6, 8.0e-6,
+                      (7e6, 7e6, 120e6, 120e6, 12e6, 12e6, 100e6, 0.0, 0.0, BRITTLE, 80.0, 80.0)),
 }
+
+
+@dataclasses.dataclass
+class FractureConfig:
+    """FractureConfig of scenes/wood_chop.ron (src/fracture.rs:11-25)."""
+
+    damage_rate: float = 35.0
+    energy_release: float = 0.005
+    fatigue_cycles: float = 1000.0
+    fatigue_exponent: float = 3.0
+    fatigue_threshold: float = 0.5
+    plastic_heat_fraction: float = 0.9
 
 AXES = np.eye(3, dtype=np.float32)
 
@@ -131,6 +156,7 @@ class Scene:
     keys: np.ndarray           # (n, 3) int32 lattice keys
     config: StepConfig = dataclasses.field(default_factory=StepConfig)
     name: str = ""
+    strength: np.ndarray = None  # (n, 12) STRENGTH_FIELDS per REV
 
     @property
     def n(self) -> int:
@@ -213,6 +239,7 @@ def _box(dims, material_ids, grain_axis, anisotropy, kinematic, materials) -> Sc
         targets=np.full((n, 3), np.nan, np.float32),
         loads=np.zeros((n, 3), np.float32),
         keys=keys,
+        strength=np.array([m.strength for m in table], np.float32)[mid],
     )
 
 
