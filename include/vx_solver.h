@@ -127,7 +127,29 @@ typedef struct vx_scene_desc {
     const float *plastic_strain;   /* 3*n*3, may be NULL (zero) */
     const float *targets;          /* n*3 world coordinates, NaN = free axis; may be NULL */
     const float *loads;            /* n*3 persistent applied loads; may be NULL */
+    /* Fracture inputs. strength: n*12 per-REV strength rows (Material
+     * fields, src/material.rs:20-63) in VX_S_* order; required when a step
+     * runs the fracture pass, else may be NULL. bond_history: 3*n*6 per +axis
+     * bond in VX_H_* order, may be NULL (a new scene: zero). */
+    const float *strength;
+    const float *bond_history;
 } vx_scene_desc;
+
+/* Strength row fields (Material fields, src/material.rs:20-63). S_MODE is
+ * VX_BRITTLE, VX_DUCTILE or VX_MIXED (FractureMode, src/material.rs:4-8). */
+enum {
+    VX_S_TENSION, VX_S_TENSION_PERP, VX_S_COMPRESSION, VX_S_COMPRESSION_PERP, VX_S_SHEAR, VX_S_SHEAR_PERP,
+    VX_S_YIELD, VX_S_HARDENING, VX_S_PLASTIC_FAILURE, VX_S_MODE, VX_S_GC, VX_S_GC_PERP, VX_STRENGTH_FIELDS
+};
+#define VX_BRITTLE 0
+#define VX_DUCTILE 1
+#define VX_MIXED 2
+/* Bond history fields beyond damage and plastic strain (Bond,
+ * src/bond.rs:4-16). */
+enum {
+    VX_H_EQUIVALENT_PLASTIC_STRAIN, VX_H_PLASTIC_WORK, VX_H_FATIGUE_DAMAGE, VX_H_PREVIOUS_STRESS_RATIO,
+    VX_H_REVERSAL_STRESS_RATIO, VX_H_LOADING_INCREMENT, VX_HISTORY_FIELDS
+};
 
 typedef struct vx_scene vx_scene;
 
@@ -184,8 +206,35 @@ int32_t vx_scene_time_spmv(vx_scene *scene, uint32_t reps, double *out_us);
 /* One fixed step of the solve part of step_with_axis_targets_and_loads
  * (src/solver.rs:475-570, 668-683) on the device: per substep predict, fit
  * rotations, up to corotation_iterations passes of assemble + solve + apply
- * with the reference's early exit, then velocities. Contact, tool, fracture
- * and heat are not part of it. */
+ * with the reference's early exit, then (with fracture.enabled) the fracture
+ * pass, then velocities. Contact, tool and heat are not part of it. */
+/* FractureConfig (src/fracture.rs:12-25); enabled = 0 skips the pass. With
+ * the pass on, every substep ends with fracture::update (src/fracture.rs:
+ * 210-292) of each intact bond of a non-kinematic REV, after the corotation
+ * passes and before the velocities (src/solver.rs:627-665); a bond whose
+ * damage reaches one leaves the lattice. */
+typedef struct vx_fracture_params {
+    uint32_t enabled;
+    uint32_t reserved;
+    double damage_rate;
+    double energy_release;
+    double fatigue_cycles;
+    double fatigue_exponent;
+    double fatigue_threshold;
+    double plastic_heat_fraction;
+} vx_fracture_params;
+
+/* Energies dissipated by the fracture pass (Dissipation, src/fracture.rs:
+ * 135-141) and the bonds it broke. */
+typedef struct vx_fracture_report {
+    uint64_t bonds_broken;
+    double plastic_heat;
+    double stored_plastic;
+    double surface_work;
+    double damage_heat;
+    double released;
+} vx_fracture_report;
+
 typedef struct vx_step_params {
     double fixed_dt;
     double gravity[3];
@@ -193,6 +242,7 @@ typedef struct vx_step_params {
     uint32_t corotation_iterations;
     double corotation_tolerance;
     vx_solve_params solve;
+    vx_fracture_params fracture;
 } vx_step_params;
 
 typedef struct vx_step_report {
@@ -203,9 +253,16 @@ typedef struct vx_step_report {
     double last_relative_residual;
     double last_angle;
     double seconds;              /* device wall time of the step */
+    vx_fracture_report fracture; /* summed over the step's substeps */
 } vx_step_report;
 
 int32_t vx_scene_step(vx_scene *scene, const vx_step_params *params, vx_step_report *report);
+
+/* Copy the bond state out (any pointer may be NULL): neighbor 3*n (v +
+ * stride or VX_EMPTY once broken), damage 3*n, plastic_strain 3*n*3 and
+ * history 3*n*6 (VX_H_* order), all per +axis bond [axis][voxel]. */
+int32_t vx_scene_download_bonds(vx_scene *scene, uint32_t *neighbor, float *damage, float *plastic_strain,
+                                float *history);
 
 /* Inner PCG iterations per scheduling tick (one hipGraph replay; default 16,
  * at most 256). 0 keeps the current value. */
@@ -240,6 +297,7 @@ typedef struct vx_batch_scene_report {
     uint32_t active_rows;
     double last_relative_residual;
     double last_angle;
+    vx_fracture_report fracture; /* since admission */
 } vx_batch_scene_report;
 
 /* HBM bytes per arena row (for admission budgets). */
@@ -265,6 +323,9 @@ int32_t vx_batch_report(vx_batch *batch, uint32_t slot, vx_batch_scene_report *r
 int32_t vx_batch_states(vx_batch *batch, uint32_t *states);
 /* Copy one scene's state out (any pointer may be NULL). */
 int32_t vx_batch_download(vx_batch *batch, uint32_t slot, double *positions, double *rotations, double *velocities);
+/* Bond state of one scene, as vx_scene_download_bonds. */
+int32_t vx_batch_download_bonds(vx_batch *batch, uint32_t slot, uint32_t *neighbor, float *damage,
+                                float *plastic_strain, float *history);
 /* Free a finished (or failed) scene's tiles and slot. */
 int32_t vx_batch_release(vx_batch *batch, uint32_t slot);
 

@@ -343,7 +343,7 @@ def _fcfg(c):
 
 
 def pair_limits(a, b, alignment, anisotropy, mode=F64):
-    """pair_limits (src/fracture.rs:77-133) for two strength rows."""
+    """pair_limits (src/fracture.rs:76-133) for two strength rows."""
     out = np.empty(8)
     lib().vxo_pair_limits(mode, _p(_f64(a), _dp), _p(_f64(b), _dp), _p(_f64(alignment), _dp),
                           _p(_f64(anisotropy), _dp), _p(out, _dp))
@@ -373,7 +373,7 @@ def fatigue(b, signed_ratio, cfg, mode=F64):
 
 
 def fracture_update(b, u_total, k, limits, dt, cfg, mode=F64):
-    """fracture::update (src/fracture.rs:199-292) on a _Bond in place; returns
+    """fracture::update (src/fracture.rs:210-292) on a _Bond in place; returns
     the dissipation dict."""
     d = _Dissipation()
     lib().vxo_fracture_update(mode, ctypes.byref(b), _p(_f64(u_total), _dp), _p(_f64(np.asarray(k)).ravel(), _dp),

@@ -293,13 +293,13 @@ template <class T> struct Limits {
     T tension, compression, shear, yield, hardening, plastic_failure, ductility, gc;
 };
 
-// directional (src/fracture.rs:74-76).
+// directional (src/fracture.rs:73-75).
 template <class T> T directional(T parallel, T perpendicular, T alignment, T anisotropy) {
     T a = std::min(std::max(std::fabs(alignment), T(0)), T(1));
     return parallel + anisotropy * (T(1) - a) * (perpendicular - parallel);
 }
 
-// pair_limits (src/fracture.rs:77-133) for strength rows a, b.
+// pair_limits (src/fracture.rs:76-133) for strength rows a, b.
 template <class T> Limits<T> pair_limits(const double *a, const double *b, const T (&alignment)[2],
                                           const T (&anisotropy)[2]) {
     auto one = [&](const double *m, int k) {
@@ -348,12 +348,12 @@ template <class T> void bond_out(const BondT<T> &t, vxo_bond &b) {
     b.loading_increment = double(t.increment);
 }
 
-// energy (src/fracture.rs:175-177).
+// energy (src/fracture.rs:186-188).
 template <class T> double bond_energy(V3<T> u, const M3<T> &k, T damage) {
     return double(std::max(T(0.5) * (T(1) - damage) * dot(u, k * u), T(0)));
 }
 
-// fatigue (src/fracture.rs:178-196): only stress reversals consume life.
+// fatigue (src/fracture.rs:190-207): only stress reversals consume life.
 template <class T> void fatigue(BondT<T> &b, T signed_ratio, const vxo_fracture_config *c) {
     const T delta = signed_ratio - b.previous;
     if (std::fabs(delta) > T(1e-6)) {
@@ -372,7 +372,7 @@ template <class T> void fatigue(BondT<T> &b, T signed_ratio, const vxo_fracture_
     b.previous = signed_ratio;
 }
 
-// update (src/fracture.rs:199-292).
+// update (src/fracture.rs:210-292).
 template <class T>
 vxo_dissipation fracture_update(BondT<T> &b, V3<T> u_total, const M3<T> &k, const Limits<T> &l, T dt,
                                 const vxo_fracture_config *c) {

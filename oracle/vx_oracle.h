@@ -166,11 +166,11 @@ double vxo_dot_rows(uint32_t n, const double *a, const double *b);
  * num/den cast to the vector type when den > 1e-30, else 0. */
 double vxo_guarded_ratio(int mode, double num, double den);
 
-/* ---- fracture (src/fracture.rs:73-133, 199-292; src/solver.rs:106-161, 627-665) */
+/* ---- fracture (src/fracture.rs:73-133, 210-292; src/solver.rs:106-161, 627-665) */
 #define VXO_BRITTLE 0
 #define VXO_DUCTILE 1
 #define VXO_MIXED 2
-/* Per-REV strength (Material, src/material.rs:27-62): VXO_STRENGTH doubles
+/* Per-REV strength (Material, src/material.rs:20-63): VXO_STRENGTH doubles
  * per REV in this order. */
 enum {
     VXO_S_TENSION, VXO_S_TENSION_PERP, VXO_S_COMPRESSION, VXO_S_COMPRESSION_PERP, VXO_S_SHEAR,
@@ -192,16 +192,16 @@ typedef struct vxo_bond { /* Bond history (src/bond.rs:4-16) */
     double previous_stress_ratio, reversal_stress_ratio, loading_increment;
 } vxo_bond;
 
-typedef struct vxo_dissipation { /* Dissipation (src/fracture.rs:134-140), J */
+typedef struct vxo_dissipation { /* Dissipation (src/fracture.rs:135-141), J */
     double plastic_heat, stored_plastic, surface_work, damage_heat, released;
 } vxo_dissipation;
 
 /* pair_limits (src/fracture.rs:73-133): a, b are VXO_STRENGTH-value rows. */
 void vxo_pair_limits(int mode, const double *a, const double *b, const double alignment[2],
                      const double anisotropy[2], double limits[VXO_LIMITS]);
-/* fatigue (src/fracture.rs:178-196). */
+/* fatigue (src/fracture.rs:190-207). */
 void vxo_fatigue(int mode, vxo_bond *bond, double signed_ratio, const vxo_fracture_config *c);
-/* update (src/fracture.rs:199-292): u_total in the bond frame (m), k the
+/* update (src/fracture.rs:210-292): u_total in the bond frame (m), k the
  * undamaged local stiffness (row-major 3x3, N/m). */
 void vxo_fracture_update(int mode, vxo_bond *bond, const double u_total[3], const double k[9],
                          const double limits[VXO_LIMITS], double dt, const vxo_fracture_config *c,

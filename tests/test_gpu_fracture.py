@@ -1,0 +1,161 @@
+"""GPU parity of the fracture pass (src/solver.rs:627-665 with
+fracture::update, src/fracture.rs:210-292) against the FP64 oracle, through
+the C ABI: the set of broken bonds is identical, the dissipated energies and
+the bond history (damage, plastic strain, equivalent plastic strain, plastic
+work, fatigue) agree to the solve's accuracy, and the steps after a break
+(assembled without the broken bonds, with the plastic rest offsets) keep
+matching. Bond state is stored in float32, the format of the reference's
+Bond, on the device and (rounded between steps) in the oracle."""
+import numpy as np
+import pytest
+
+from vxsim.scenes import FractureConfig, HISTORY_FIELDS
+
+pytestmark = pytest.mark.gpu
+EMPTY = 0xFFFFFFFF
+STEPS = 3
+ENERGIES = ("plastic_heat", "stored_plastic", "surface_work", "damage_heat", "released")
+
+
+def run_device(s, steps, fracture=FractureConfig(), tol=1e-11):
+    from vxsim import grid
+    cfg = s.config
+    sp = grid.solve_params(tolerance=tol, inner_tolerance=1e-3, max_inner=20000)
+    reps = []
+    with grid.GridScene(s) as gs:
+        for _ in range(steps):
+            reps.append(gs.step(fixed_dt=cfg.fixed_dt, gravity=cfg.gravity, substeps=cfg.substeps,
+                                corotation_iterations=cfg.corotation_iterations,
+                                corotation_tolerance=cfg.corotation_tolerance, solve=sp, fracture=fracture))
+        return reps, gs.download(), gs.download_bonds()
+
+
+def run_oracle(O, s, steps):
+    """FP64 oracle steps; between steps the bond state is rounded to float32,
+    the storage format of the reference's Bond and of the device planes."""
+    os_ = O.OracleScene(s)
+    hist = O.FractureHistory(s.n)
+    reps = []
+    for _ in range(steps):
+        reps.append(O.step_fracture(os_, s.config, FractureConfig(), hist, s.strength, O.F64, exact=True,
+                                    exact_tolerance=1e-12, targets=s.targets, loads=s.loads))
+        for a in [os_.damage, os_.plastic_strain] + [getattr(hist, k) for k in HISTORY_FIELDS]:
+            a[...] = a.astype(np.float32)
+    return reps, os_, hist
+
+
+def check_parity(O, s, steps=STEPS):
+    reps_d, out, bonds = run_device(s, steps)
+    reps_o, os_, hist = run_oracle(O, s, steps)
+    broken_total = 0
+    # Energies against the run's energy scale: a later step's small residual
+    # flow is a difference of nearly equal bond energies.
+    scale = max(max(sum(ro["dissipated"].values()) for ro in reps_o), 1e-30)
+    for k, (rd, ro) in enumerate(zip(reps_d, reps_o)):
+        fr = rd["fracture"]
+        assert fr["bonds_broken"] == ro["broken"], (s.name, k)
+        broken_total += ro["broken"]
+        for e in ENERGIES:
+            assert abs(fr[e] - ro["dissipated"][e]) <= 1e-6 * scale + 1e-12, (s.name, k, e, fr[e], ro["dissipated"][e])
+    # the same bonds are gone
+    assert np.array_equal(bonds["neighbor"], os_.neighbor), s.name
+    # bond history to the solve's accuracy (float32 storage on the device)
+    intact = os_.neighbor != EMPTY
+    np.testing.assert_allclose(bonds["damage"], os_.damage, rtol=1e-5, atol=1e-7)
+    pl_o = os_.plastic_strain
+    np.testing.assert_allclose(bonds["plastic_strain"], pl_o, rtol=1e-5, atol=1e-6 * max(np.abs(pl_o).max(), 1e-12))
+    ratio_scale = np.abs(hist.previous_stress_ratio).max()
+    for f, name in enumerate(HISTORY_FIELDS):
+        ref = getattr(hist, name)
+        got = bonds["history"][..., f]
+        # The stress-ratio fields (and the increment, a difference of two
+        # ratios) carry the ratio's absolute error.
+        scale = ratio_scale if name.endswith(("stress_ratio", "increment")) else np.abs(ref).max()
+        np.testing.assert_allclose(got[intact], ref[intact], rtol=1e-5, atol=1e-6 * max(scale, 1e-12), err_msg=name)
+    # positions after the breaks (the later steps run without the broken bonds)
+    start = s.positions.astype(np.float64)
+    disp_o = os_.positions - start
+    disp = out["positions"] - start
+    assert np.linalg.norm(disp - disp_o) <= 1e-6 * np.linalg.norm(disp_o), s.name
+    return broken_total, hist
+
+
+@pytest.mark.parametrize("material,pull", [("stone", 2e-6), ("stone", 5e-5), ("iron", 1e-3), ("iron", 8e-3)])
+def test_pulled_column_matches_oracle(gpu, O, scenes, material, pull):
+    s = scenes.pulled_column(material, pull)
+    broken, hist = check_parity(O, s)
+    if material == "stone":
+        assert broken == (45 if pull > 1e-5 else 0)
+    else:
+        assert broken == 0 and hist.equivalent_plastic_strain.max() > 0.0
+
+
+def test_pulled_mixed_box_matches_oracle(gpu, O, scenes):
+    """Brittle breaks, cross-grain wood breaks and iron / along-grain wood
+    plastic flow in one 12^3 box, three steps."""
+    s = scenes.pulled_mixed((12, 12, 12), 4, 4e-4)
+    broken, hist = check_parity(O, s)
+    assert broken > 500 and np.count_nonzero(hist.equivalent_plastic_strain) > 100
+
+
+def test_fracture_off_leaves_bonds_alone(gpu, scenes):
+    """Without the pass the step is the plain step: same positions bit for
+    bit, no bond touched, nothing reported."""
+    s = scenes.pulled_column("stone", 5e-5)
+    reps, out, bonds = run_device(s, 2, fracture=None)
+    plain = s.copy()
+    plain.strength = None
+    reps_p, out_p, _ = run_device(plain, 2, fracture=None)
+    assert np.array_equal(out["positions"], out_p["positions"])
+    assert np.array_equal(bonds["neighbor"], s.neighbor)
+    assert not bonds["damage"].any() and not bonds["history"].any()
+    assert all(r["fracture"]["bonds_broken"] == 0 and r["fracture"]["surface_work"] == 0.0 for r in reps)
+
+
+def test_fracture_needs_strength(gpu, scenes):
+    from vxsim import grid
+    from vxsim.solver import SolverError
+    s = scenes.pulled_column("stone", 5e-5)
+    s.strength = None
+    with grid.GridScene(s) as gs:
+        with pytest.raises(SolverError):
+            gs.step(fixed_dt=0.001, gravity=(0, 0, 0), fracture=FractureConfig())
+
+
+def test_batched_fracture_same_alone_and_in_mix(gpu, scenes):
+    """Scenes with and without the fracture pass share one arena; each one's
+    positions and bonds are bit-identical to the scene stepped alone."""
+    from vxsim import batch, grid
+    cases = [(scenes.pulled_mixed((10, 9, 11), 5, 4e-4), FractureConfig()),
+             (scenes.pulled_column("stone", 5e-5), FractureConfig()),
+             (scenes.pulled_mixed((8, 12, 7), 6, 4e-4), None),
+             (scenes.pulled_column("iron", 8e-3, dims=(4, 3, 9)), FractureConfig())]
+    sp = grid.solve_params(tolerance=1e-9, inner_tolerance=1e-3, max_inner=20000)
+    params = []
+    with batch.Batch(capacity_rows=16 * 512, max_scenes=8) as B:
+        slots = []
+        for s, f in cases:
+            c = s.config
+            p = grid.step_params(c.fixed_dt, c.gravity, c.substeps, c.corotation_iterations,
+                                 c.corotation_tolerance, sp, fracture=f)
+            params.append(p)
+            slots.append(B.admit(s, 2, p))
+        B.run()
+        got = [(B.download(k), B.download_bonds(k), B.report(k)) for k in slots]
+    for (s, f), (out, bonds, rep) in zip(cases, got):
+        assert rep["state"] == batch.DONE
+        c = s.config
+        with grid.GridScene(s) as gs:
+            total = 0
+            for _ in range(2):
+                r = gs.step(fixed_dt=c.fixed_dt, gravity=c.gravity, substeps=c.substeps,
+                            corotation_iterations=c.corotation_iterations,
+                            corotation_tolerance=c.corotation_tolerance, solve=sp, fracture=f)
+                total += r["fracture"]["bonds_broken"]
+            ref, ref_b = gs.download(), gs.download_bonds()
+        assert np.array_equal(out["positions"], ref["positions"]), s.name
+        for key in ("neighbor", "damage", "plastic_strain", "history"):
+            assert np.array_equal(bonds[key], ref_b[key]), (s.name, key)
+        assert rep["fracture"]["bonds_broken"] == total
+        if f is None:
+            assert total == 0

@@ -155,27 +155,15 @@ def test_directional_limit_closed_form(O):
     assert math.isclose(lim[7], min(350 + 0.75 * (80 - 350), 80))
 
 
-def stone_column(scenes, pull):
-    """3x3x6 stone REVs, bottom layer held, top layer targeted `pull` metres
-    up in Z (all axes targeted), no gravity."""
-    from vxsim.scenes import StepConfig
-    s = scenes.uniform_box((3, 3, 6), "stone")
-    z = s.keys[:, 2]
-    s.targets[z == z.min()] = s.positions[z == z.min()]
-    s.targets[z == z.max()] = s.positions[z == z.max()] + np.array([0, 0, pull], np.float32)
-    s.config = StepConfig(fixed_dt=0.001, substeps=1, iterations=1000, corotation_iterations=4,
-                          solver_relative_tolerance=1e-10, gravity=(0.0, 0.0, 0.0))
-    return s
-
-
 @pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("pull", [2e-6, 5e-5])
 def test_step_fracture_pass_follows_the_solve(O, scenes, mode, pull):
     """The fracture pass runs after the corotation passes (src/solver.rs:627-665):
     the step's positions and velocities equal those of the plain step; a
     sub-critical pull changes no bond, a pull of ~7x the tensile strain breaks
-    every Z bond and only those, each releasing gc * CELL^2 of surface work."""
-    s = stone_column(scenes, pull)
+    every Z bond and only those, each releasing gc * CELL^2 of surface work.
+    (A 3x3x6 stone column, bottom held, top pulled in Z.)"""
+    s = scenes.pulled_column("stone", pull)
     a = O.OracleScene(s)
     nb0 = a.neighbor.copy()
     rep = O.step_fracture(a, s.config, FractureConfig(), O.FractureHistory(s.n), s.strength, mode,

@@ -63,9 +63,13 @@ enum : uint32_t {
     B_WARM_USE = 1u << 9,
     B_WARM_SAVE = 1u << 10,
     B_F64 = 1u << 11,     // inner solve in FP64 (plain FP64 mode)
+    B_FRACTURE = 1u << 12,  // fracture pass of the substep, before the velocities
 };
 
-enum : int { kPap = 0, kRz = 1, kRr = 2, kRr64 = 3, kBb = 4, kAngle = 5, kSlots = 6 };  // partial kinds
+// Partial kinds; kFrac..kFrac+5 are the fracture pass's dissipated energies
+// (plastic heat, stored plastic, surface work, damage heat, released) and
+// its count of broken bonds.
+enum : int { kPap = 0, kRz = 1, kRr = 2, kRr64 = 3, kBb = 4, kAngle = 5, kFrac = 6, kFracN = 6, kSlots = 12 };
 enum : uint32_t { kRun = 0, kConv = 1, kNonFin = 2, kCap = 3 };
 
 // Box geometry of a scene and its tile range; one copy per slot and one per
@@ -78,6 +82,8 @@ struct SlotParams {
     double h, gx, gy, gz;
     double tol, inner_tol;
     uint32_t max_inner, max_outer;
+    uint32_t fracture, pad;  // fracture: 1 when the scene runs the fracture pass
+    FractureCfg fcfg;
 };
 // Written by the device; each field by one workgroup of one kernel and read
 // by later kernels (and by the host after a tick).
@@ -91,6 +97,7 @@ struct SlotState {
     uint32_t pbuf;  // p buffer holding the latest search direction
     uint32_t pad;
     double red[3];  // pre-summed p.Ap, r.z, r.r of a large scene (scene_reduce)
+    double frac[kFracN];  // the last fracture pass: energies and broken bonds
 };
 
 // Scenes of more tiles than this have their inner partials summed once per
@@ -112,6 +119,10 @@ struct DevArena {
     float *mass, *young, *young_perp, *shear, *solid, *fiber, *aniso, *dtemp, *a_par, *a_perp, *loads;
     uint8_t *kinematic, *bonds, *tmask, *mask;
     float *damage, *kscale, *offset, *plastic;  // 3R, 3R, 9R ([a][c]), 9R
+    // fracture: strength rows (12R), bond history (18R, [a][field]) and the
+    // rotations of the last assembly (4R), which fix each bond's frame
+    float *strength, *hist;
+    double *rot_asm;
     // operator
     double *k64, *force, *d64, *dinv64, *rhs;  // 18R, 9R, 6R, 6R, 3R
     float *k32, *d32, *dinv32;                  // 18R, 6R, 6R
@@ -338,6 +349,14 @@ __global__ __launch_bounds__(kBlock) void assemble_bonds(DevArena A) {
     const d3 pi = ld3(A.pos, R, v);
     const uint8_t mi = A.mask[v];
     const Rev ri = ld_rev(A, v);
+    if (A.prm[c.slot].fracture) {
+        // The fracture pass evaluates every bond in the frame of the last
+        // assembly (src/solver.rs:627-665).
+        A.rot_asm[v] = qi.x;
+        A.rot_asm[R + v] = qi.y;
+        A.rot_asm[2 * R + v] = qi.z;
+        A.rot_asm[3 * R + v] = qi.w;
+    }
     for (int a = 0; a < 3; ++a) {
         double k[6] = {0, 0, 0, 0, 0, 0};
         d3 force{0, 0, 0};
@@ -931,6 +950,88 @@ __global__ __launch_bounds__(kBlock) void reduce_angle(DevArena A) {
     }
 }
 
+// Fracture pass of a substep (src/solver.rs:627-665 with fracture::update,
+// src/fracture.rs:210-292): for every intact +axis bond of a non-kinematic
+// REV, after the corotation passes and before the velocities. The bond frame
+// and the material stiffness (kscale included, damage not) are those of the
+// last assembly; fibre alignment and the thermal rest length use the current
+// rotations. A bond whose damage reaches one is removed from the lattice.
+// Per tile: the dissipated energies and the count of broken bonds.
+__global__ __launch_bounds__(kBlock) void fracture_bonds(DevArena A) {
+    Tile c;
+    if (!tile_for(A, B_FRACTURE, c)) return;
+    __shared__ double lds[kFracN][kWaves];
+    double acc[kFracN] = {0, 0, 0, 0, 0, 0};
+    const size_t R = A.R;
+    const uint32_t v = c.v;
+    const uint8_t bits0 = c.row ? A.bonds[v] : uint8_t(0);
+    if (c.row && bits0 && !A.kinematic[v]) {
+        const SlotParams &p = A.prm[c.slot];
+        const FractureCfg cfg = p.fcfg;
+        const Rev ri = ld_rev(A, v);
+        const q4 ai = ldq(A.rot_asm, R, v), qi = ldq(A.rot, R, v);
+        const d3 pi = ld3(A.pos, R, v);
+        float si[kStrength];
+        for (int q = 0; q < kStrength; ++q) si[q] = A.strength[size_t(q) * R + v];
+        const double solid_i = fmin(fmax(ri.solid, 0.0), 1.0);
+        uint8_t bits = bits0;
+        for (int a = 0; a < 3; ++a) {
+            if (!(bits0 >> a & 1)) continue;
+            const uint32_t j = v + stride_of(c.g, a);
+            const size_t b = size_t(a) * R + v;
+            const Rev rj = ld_rev(A, j);
+            const q4 aj = ldq(A.rot_asm, R, j), qj = ldq(A.rot, R, j);
+            const m3 F = basis(a, slerp_half(ai, aj));
+            double K[6];
+            local_stiffness(F, ri, ai, rj, aj, double(A.kscale[b]), K);
+            float sj[kStrength];
+            for (int q = 0; q < kStrength; ++q) sj[q] = A.strength[size_t(q) * R + j];
+            const double solid = fmin(1.0, fmin(solid_i, fmin(fmax(rj.solid, 0.0), 1.0)));
+            const Limits l = pair_limits(endpoint_limits(si, dot3(F.c0, qrotate(qi, ri.fiber)), ri.aniso),
+                                         endpoint_limits(sj, dot3(F.c0, qrotate(qj, rj.fiber)), rj.aniso), solid * solid);
+            const size_t o = size_t(a) * 3 * R;
+            const d3 off = mk(A.offset[o + v], A.offset[o + R + v], A.offset[o + 2 * R + v]);
+            const d3 u = mul_t(F, ld3(A.pos, R, j) - pi) - thermal_rest_local(F, ri, qi, rj, qj) - off;
+            d3 plastic = mk(A.plastic[o + v], A.plastic[o + R + v], A.plastic[o + 2 * R + v]);
+            double damage = A.damage[b];
+            double h[kHistory];
+            for (int f = 0; f < kHistory; ++f) h[f] = A.hist[(size_t(a) * kHistory + f) * R + v];
+            Dissipation d;
+            const bool broken = fracture_update(plastic, damage, h, u, K, l, p.h, cfg, d);
+            A.plastic[o + v] = float(plastic.x);
+            A.plastic[o + R + v] = float(plastic.y);
+            A.plastic[o + 2 * R + v] = float(plastic.z);
+            A.damage[b] = float(damage);
+            for (int f = 0; f < kHistory; ++f) A.hist[(size_t(a) * kHistory + f) * R + v] = float(h[f]);
+            for (int q = 0; q < 5; ++q) acc[q] += d.e[q];
+            if (broken) {
+                bits &= uint8_t(~(1u << a));
+                acc[5] += 1.0;
+            }
+        }
+        if (bits != bits0) A.bonds[v] = bits;
+    }
+    int kinds[kFracN];
+    for (int q = 0; q < kFracN; ++q) kinds[q] = kFrac + q;
+    tile_store<kFracN>(acc, A, c.t, kinds, lds);
+}
+
+// One workgroup per scene: the fracture pass's totals over the scene's tiles
+// (fixed order) into its state.
+__global__ __launch_bounds__(kBlock) void reduce_fracture(DevArena A) {
+    const uint32_t slot = blockIdx.x;
+    if (!(A.bits[slot] & B_FRACTURE)) return;
+    const SlotGeo g = A.geo[slot];
+    if (g.n == 0) return;
+    __shared__ double lds[kFracN][kWaves];
+    double acc[kFracN] = {0, 0, 0, 0, 0, 0};
+    for (uint32_t b = threadIdx.x; b < g.ntiles; b += kBlock)
+        for (int q = 0; q < kFracN; ++q) acc[q] += A.partials[size_t(kFrac + q) * A.cap_tiles + g.tile0 + b];
+    block_sum_all<kFracN>(acc, lds);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < kFracN; ++q) A.st[slot].frac[q] = acc[q];
+}
+
 // Velocities after a substep (src/solver.rs:668-683), no contact.
 __global__ __launch_bounds__(kBlock) void finish_velocities(DevArena A) {
     Tile c;
@@ -956,7 +1057,7 @@ inline uint32_t tiles_for(uint64_t rows) { return uint32_t((rows + kBlock - 1) /
 
 // HBM bytes per arena row: the planes carved in create_batch.
 constexpr size_t kRowBytes = 24 * 5 + 32 + 24 * 3 + 24 + 4 * 9 + 12 + 12 + 4 + 12 + 12 + 36 + 36 + 144 + 72 + 48 + 48 + 24 +
-                             72 + 24 + 24 + 6 * 12 + 6 * 24;
+                             72 + 24 + 24 + 6 * 12 + 6 * 24 + 4 * kStrength + 12 * kHistory + 32;
 
 enum Phase : uint32_t { P_IDLE = 0, P_START, P_SOLVE, P_APPLIED, P_FINISHED, P_DONE, P_FAILED };
 
@@ -974,6 +1075,8 @@ struct SlotHost {
     uint64_t inner = 0, sweeps = 0;
     double last_rel = 0.0, last_angle = 0.0;
     bool manual = false;  // driven by the vx_scene_* stage calls
+    bool has_strength = false;
+    vx_fracture_report frac{};  // fracture totals since admission (a vx_scene: this step)
 };
 
 // One launched tick: the bits it ran with and the scene states read back
@@ -1160,6 +1263,10 @@ int32_t launch_tick(vx_batch *b, Flight &f) {
         hipLaunchKernelGGL(fit_rotations, grid, blk, 0, s, b->A, uint32_t(B_FITB), 1);
         hipLaunchKernelGGL(reduce_angle, sg, blk, 0, s, b->A);
     }
+    if (any & B_FRACTURE) {
+        hipLaunchKernelGGL(fracture_bonds, grid, blk, 0, s, b->A);
+        hipLaunchKernelGGL(reduce_fracture, sg, blk, 0, s, b->A);
+    }
     if (any & B_FINISH) hipLaunchKernelGGL(finish_velocities, grid, blk, 0, s, b->A);
     VX_HIP(hipGetLastError());
     VX_HIP(hipMemcpyAsync(f.st, b->d_st, sizeof(SlotState) * b->max_slots, hipMemcpyDeviceToHost, s));
@@ -1169,6 +1276,8 @@ int32_t launch_tick(vx_batch *b, Flight &f) {
 }
 
 uint32_t prec_bits(const vx_step_params &p) { return p.solve.mode == VX_SOLVE_F64 ? uint32_t(B_F64) : 0u; }
+// The end of a substep: the fracture pass (when on) and the velocities.
+uint32_t finish_bits(const vx_step_params &p) { return p.fracture.enabled ? uint32_t(B_FINISH | B_FRACTURE) : uint32_t(B_FINISH); }
 
 // Bits of an automatically driven scene for its next tick, from its latest
 // read-back state.
@@ -1187,12 +1296,12 @@ uint32_t next_bits(const SlotHost &h, const SlotState &st) {
             if (h.pass == 0 && h.params.solve.warm_start) b |= B_WARM_SAVE;
             // The reference refits only when another pass may follow
             // (src/solver.rs:561-564).
-            b |= (h.pass + 1 < h.params.corotation_iterations) ? uint32_t(B_FITB) : uint32_t(B_FINISH);
+            b |= (h.pass + 1 < h.params.corotation_iterations) ? uint32_t(B_FITB) : finish_bits(h.params);
             return b;
         }
         return B_INNER | prec;
     case P_APPLIED:
-        if (h.finish_next) return B_FINISH;
+        if (h.finish_next) return finish_bits(h.params);
         return B_ASSEMBLE | B_RESID0 | B_INNER | prec;
     default:
         return 0;
@@ -1213,6 +1322,14 @@ void finish_substep(SlotHost &h) {
 // Advance an automatically driven scene after a tick that ran `bits`.
 void advance(SlotHost &h, const SlotState &st, uint32_t bits) {
     if (bits & (B_RESID0 | B_INNER)) h.last_rel = st.rel;
+    if (bits & B_FRACTURE) {
+        h.frac.plastic_heat += st.frac[0];
+        h.frac.stored_plastic += st.frac[1];
+        h.frac.surface_work += st.frac[2];
+        h.frac.damage_heat += st.frac[3];
+        h.frac.released += st.frac[4];
+        h.frac.bonds_broken += uint64_t(st.frac[5]);
+    }
     if (st.nonfinite || st.done == kNonFin) {
         h.phase = P_FAILED;
         h.status = VX_ERR_NONFINITE_RESIDUAL;
@@ -1328,7 +1445,8 @@ int32_t create_batch(uint64_t capacity_rows, uint32_t max_scenes, uint32_t chunk
         {(void **)&A.dinv32, 24 * R}, {(void **)&A.d, 12 * R}, {(void **)&A.r, 12 * R}, {(void **)&A.z, 12 * R},
         {(void **)&A.p0, 12 * R}, {(void **)&A.p1, 12 * R}, {(void **)&A.ap, 12 * R}, {(void **)&A.dd, 24 * R},
         {(void **)&A.rd, 24 * R}, {(void **)&A.zd, 24 * R}, {(void **)&A.p0d, 24 * R}, {(void **)&A.p1d, 24 * R},
-        {(void **)&A.apd, 24 * R}, {(void **)&A.partials, 8 * kSlots * size_t(b->cap_tiles)},
+        {(void **)&A.apd, 24 * R}, {(void **)&A.strength, 4 * kStrength * R}, {(void **)&A.hist, 12 * kHistory * R},
+        {(void **)&A.rot_asm, 32 * R}, {(void **)&A.partials, 8 * kSlots * size_t(b->cap_tiles)},
         {(void **)&b->d_tiles, sizeof(SlotGeo) * b->cap_tiles}, {(void **)&A.tile_bits, 4 * size_t(b->cap_tiles)},
         {(void **)&b->d_bits, 4 * size_t(max_scenes)},
         {(void **)&b->d_geo, sizeof(SlotGeo) * max_scenes}, {(void **)&b->d_prm, sizeof(SlotParams) * max_scenes},
@@ -1526,6 +1644,13 @@ int32_t upload_scene(vx_batch *b, const vx_scene_desc *d, const SlotGeo &g, cons
     VX_UP(up_bond(A.kscale, d->kscale, 1.0f));
     VX_UP(up_bond_vec(A.offset, d->formation_offset));
     VX_UP(up_bond_vec(A.plastic, d->plastic_strain));
+    VX_UP(up_f(A.strength, d->strength, kStrength, 0.0f));
+    for (int a = 0; a < 3; ++a)
+        for (int f = 0; f < kHistory; ++f) {
+            for (size_t v = 0; v < n; ++v) fbuf[v] = d->bond_history ? d->bond_history[(a * n + v) * kHistory + f] : 0.0f;
+            VX_UP(put(A.hist + (a * kHistory + f) * R + voff, fbuf.data(), 4 * n));
+        }
+    VX_UP(up_d(A.rot_asm, d->rotations, 4, false));
     VX_UP(put(A.kinematic + voff, d->kinematic, n));
     VX_UP(put(A.bonds + voff, bonds.data(), n));
     VX_UP(put(A.tmask + voff, tmask.data(), n));
@@ -1542,8 +1667,13 @@ int32_t upload_scene(vx_batch *b, const vx_scene_desc *d, const SlotGeo &g, cons
     return VX_OK;
 }
 
-SlotParams device_params(double h, const double *g, const vx_solve_params *s) {
+SlotParams device_params(double h, const double *g, const vx_solve_params *s, const vx_fracture_params *f) {
     SlotParams sp{};
+    if (f && f->enabled) {
+        sp.fracture = 1;
+        sp.fcfg = FractureCfg{f->damage_rate,       f->energy_release,    f->fatigue_cycles,
+                              f->fatigue_exponent, f->fatigue_threshold, f->plastic_heat_fraction};
+    }
     sp.h = h;
     sp.gx = g ? g[0] : 0.0;
     sp.gy = g ? g[1] : 0.0;
@@ -1582,7 +1712,9 @@ int32_t admit(vx_batch *b, const vx_scene_desc *d, const vx_step_params *p, uint
     if (int32_t c = drain(b)) return c;
     const SlotGeo g{d->nx, d->ny, d->nz, n, t0, need, t0 * kBlock, slot};
     if (int32_t c = upload_scene(b, d, g, bonds, tmask, mask)) return c;
-    const SlotParams sp = p ? device_params(p->fixed_dt / p->substeps, p->gravity, &p->solve) : SlotParams{};
+    if (p && p->fracture.enabled && !d->strength)
+        return fail(VX_ERR_INVALID_ARGUMENT, "the fracture pass needs the scene's strength rows");
+    const SlotParams sp = p ? device_params(p->fixed_dt / p->substeps, p->gravity, &p->solve, &p->fracture) : SlotParams{};
     SlotState st{};
     st.done = kConv;
     VX_HIP(hipMemcpy(b->d_geo + slot, &g, sizeof g, hipMemcpyHostToDevice));
@@ -1595,6 +1727,7 @@ int32_t admit(vx_batch *b, const vx_scene_desc *d, const vx_step_params *p, uint
     h.geo = g;
     h.active = active;
     h.manual = manual;
+    h.has_strength = d->strength != nullptr;
     if (p) h.params = *p;
     h.steps = steps;
     h.phase = manual ? P_IDLE : (steps ? P_START : P_DONE);
@@ -1606,14 +1739,16 @@ int32_t admit(vx_batch *b, const vx_scene_desc *d, const vx_step_params *p, uint
 
 // Device parameters of one slot; host copies keep the substep length in
 // params.fixed_dt and the gravity for later stage calls.
-int32_t set_params(vx_batch *b, uint32_t slot, double h, const double *g, const vx_solve_params *s) {
+int32_t set_params(vx_batch *b, uint32_t slot, double h, const double *g, const vx_solve_params *s,
+                   const vx_fracture_params *f = nullptr) {
     if (int32_t c = drain(b)) return c;
-    const SlotParams sp = device_params(h, g, s);
     vx_step_params &hp = b->slots[slot].params;
+    const SlotParams sp = device_params(h, g, s, f ? f : &hp.fracture);
     hp.fixed_dt = h;
     if (g)
         for (int i = 0; i < 3; ++i) hp.gravity[i] = g[i];
     if (s) hp.solve = *s;
+    if (f) hp.fracture = *f;
     VX_HIP(hipMemcpy(b->d_prm + slot, &sp, sizeof sp, hipMemcpyHostToDevice));
     return VX_OK;
 }
@@ -1764,6 +1899,7 @@ extern "C" int32_t vx_batch_report(vx_batch *b, uint32_t slot, vx_batch_scene_re
     r->active_rows = h.active;
     r->last_relative_residual = h.last_rel;
     r->last_angle = h.last_angle;
+    r->fracture = h.frac;
     return VX_OK;
 }
 
@@ -1793,6 +1929,51 @@ extern "C" int32_t vx_batch_download(vx_batch *b, uint32_t slot, double *positio
     if (velocities)
         if (int32_t c = download_planes(b->A.vel, R, g.voff, g.n, 3, velocities)) return c;
     return VX_OK;
+}
+
+// Bond state of a slot: neighbour from the bond bits, damage, plastic strain
+// and history planes, re-laid out per bond as [axis][voxel][...].
+static int32_t download_bonds(vx_batch *b, uint32_t slot, uint32_t *neighbor, float *damage, float *plastic, float *history) {
+    if (int32_t c = drain(b)) return c;
+    const SlotGeo &g = b->slots[slot].geo;
+    const DevArena &A = b->A;
+    const size_t R = A.R, n = g.n, voff = g.voff;
+    std::vector<float> buf(n);
+    auto get = [&](const float *plane) -> int32_t {
+        VX_HIP(hipMemcpy(buf.data(), plane + voff, 4 * n, hipMemcpyDeviceToHost));
+        return VX_OK;
+    };
+    if (neighbor) {
+        std::vector<uint8_t> bits(n);
+        VX_HIP(hipMemcpy(bits.data(), A.bonds + voff, n, hipMemcpyDeviceToHost));
+        for (int a = 0; a < 3; ++a) {
+            const uint32_t stride = a == 0 ? 1u : (a == 1 ? g.nx : g.nx * g.ny);
+            for (size_t v = 0; v < n; ++v) neighbor[a * n + v] = (bits[v] >> a & 1) ? uint32_t(v) + stride : VX_EMPTY;
+        }
+    }
+    for (int a = 0; a < 3; ++a) {
+        if (damage) {
+            if (int32_t c = get(A.damage + a * R)) return c;
+            std::memcpy(damage + a * n, buf.data(), 4 * n);
+        }
+        for (int q = 0; plastic && q < 3; ++q) {
+            if (int32_t c = get(A.plastic + (a * 3 + q) * R)) return c;
+            for (size_t v = 0; v < n; ++v) plastic[(a * n + v) * 3 + q] = buf[v];
+        }
+        for (int f = 0; history && f < kHistory; ++f) {
+            if (int32_t c = get(A.hist + (a * kHistory + f) * R)) return c;
+            for (size_t v = 0; v < n; ++v) history[(a * n + v) * kHistory + f] = buf[v];
+        }
+    }
+    return VX_OK;
+}
+
+extern "C" int32_t vx_batch_download_bonds(vx_batch *b, uint32_t slot, uint32_t *neighbor, float *damage, float *plastic_strain,
+                                           float *history) {
+    clear_error();
+    if (!b || slot >= b->max_slots || !b->slots[slot].used) return fail(VX_ERR_INVALID_ARGUMENT, "invalid slot");
+    Lock lock;
+    return download_bonds(b, slot, neighbor, damage, plastic_strain, history);
 }
 
 extern "C" int32_t vx_batch_release(vx_batch *b, uint32_t slot) {
@@ -1952,6 +2133,13 @@ extern "C" int32_t vx_scene_download(vx_scene *sc, double *positions, double *ro
     return VX_OK;
 }
 
+extern "C" int32_t vx_scene_download_bonds(vx_scene *sc, uint32_t *neighbor, float *damage, float *plastic_strain, float *history) {
+    clear_error();
+    if (int32_t c = scene_ok(sc)) return c;
+    Lock lock;
+    return download_bonds(sc->batch, sc->slot, neighbor, damage, plastic_strain, history);
+}
+
 // Reference row packing (src/solver.rs:1652-1672, 1731-1777) of the assembled
 // structured operator: slots in incidence order (-x, +x, -y, +y, -z, +z with
 // absent bonds compacted out), off-diagonal blocks -P_i k P_j, zero blocks and
@@ -2045,7 +2233,7 @@ extern "C" int32_t vx_scene_time_spmv(vx_scene *sc, uint32_t reps, double *out_u
     st.red[kRz] = st.red[kRr] = double(b->slots[sc->slot].geo.ntiles);
     st.rho[0] = st.rho[1] = 0.0;
     const vx_step_params p = b->slots[sc->slot].params;
-    SlotParams sp = device_params(p.fixed_dt, p.gravity, &p.solve);
+    SlotParams sp = device_params(p.fixed_dt, p.gravity, &p.solve, &p.fracture);
     sp.inner_tol = 0.0;
     sp.max_inner = 1u << 30;
     const SlotGeo &g = b->slots[sc->slot].geo;
@@ -2101,9 +2289,11 @@ extern "C" int32_t vx_scene_step(vx_scene *sc, const vx_step_params *prm, vx_ste
     Lock lock;
     vx_batch *b = sc->batch;
     SlotHost &h = b->slots[sc->slot];
+    if (prm->fracture.enabled && !h.has_strength)
+        return fail(VX_ERR_INVALID_ARGUMENT, "the fracture pass needs the scene's strength rows");
     const auto t0 = std::chrono::steady_clock::now();
     // Drive the slot through one step with the per-scene state machine.
-    if (int32_t c = set_params(b, sc->slot, prm->fixed_dt / prm->substeps, prm->gravity, &prm->solve)) return c;
+    if (int32_t c = set_params(b, sc->slot, prm->fixed_dt / prm->substeps, prm->gravity, &prm->solve, &prm->fracture)) return c;
     const bool warm_ok = h.warm_ok;
     h.params = *prm;
     h.manual = false;
@@ -2115,6 +2305,7 @@ extern "C" int32_t vx_scene_step(vx_scene *sc, const vx_step_params *prm, vx_ste
     h.solves = 0;
     h.inner = 0;
     h.sweeps = 0;
+    h.frac = vx_fracture_report{};
     h.status = VX_OK;
     h.warm_ok = warm_ok;
     h.phase = P_START;
@@ -2137,6 +2328,7 @@ extern "C" int32_t vx_scene_step(vx_scene *sc, const vx_step_params *prm, vx_ste
         rep->last_relative_residual = h.last_rel;
         rep->last_angle = h.last_angle;
         rep->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        rep->fracture = h.frac;
     }
     // Later stage calls see the substep length and the solve parameters.
     h.params.fixed_dt = prm->fixed_dt / prm->substeps;

@@ -155,6 +155,36 @@ __device__ __forceinline__ d3 endpoint_expansion(const Rev &m, q4 rv, d3 dir) {
     return ad * m.dtemp;
 }
 
+// Bond stiffness in the bond frame F, K = (Ca + Cb)^-1 * 2*AREA/CELL * scale
+// (src/solver.rs:65-73), symmetric by cofactors; false (K = 0) when either
+// compliance is invalid.
+__device__ __forceinline__ bool local_stiffness(const m3 &F, const Rev &mi, q4 qi, const Rev &mj, q4 qj, double scale,
+                                                double (&K)[6]) {
+    double Ca[6], Cb[6];
+    for (int q = 0; q < 6; ++q) K[q] = 0.0;
+    if (!(compliance(mi, F, qi, Ca) && compliance(mj, F, qj, Cb))) return false;
+    double s0 = Ca[0] + Cb[0], s1 = Ca[1] + Cb[1], s2 = Ca[2] + Cb[2];
+    double s3 = Ca[3] + Cb[3], s4 = Ca[4] + Cb[4], s5 = Ca[5] + Cb[5];
+    double c00 = s3 * s5 - s4 * s4, c01 = s2 * s4 - s1 * s5, c02 = s1 * s4 - s2 * s3;
+    double c11 = s0 * s5 - s2 * s2, c12 = s1 * s2 - s0 * s4, c22 = s0 * s3 - s1 * s1;
+    double det = s0 * c00 + s1 * c01 + s2 * c02;
+    double sc = (2.0 * kCell * kCell / kCell) / det * scale;
+    K[0] = c00 * sc;
+    K[1] = c01 * sc;
+    K[2] = c02 * sc;
+    K[3] = c11 * sc;
+    K[4] = c12 * sc;
+    K[5] = c22 * sc;
+    return true;
+}
+
+// Thermal rest length vector of the bond in its own frame F
+// (bond_thermal_rest, src/solver.rs:78-95), endpoints rotated by qi, qj.
+__device__ __forceinline__ d3 thermal_rest_local(const m3 &F, const Rev &mi, q4 qi, const Rev &mj, q4 qj) {
+    d3 dir = normalize_or_zero(F.c0);
+    return mul_t(F, dir * kCell + (endpoint_expansion(mi, qi, dir) + endpoint_expansion(mj, qj, dir)) * (0.5 * kCell));
+}
+
 // Rotated bond stiffness k = F (K * scale) F^T (symmetric, 6 values) and the
 // rest vector rest = F (thermal + offset + plastic CELL) of the +axis bond
 // i -> j (src/solver.rs:65-95, 1325-1345). scale = kscale * max(1 - damage,
@@ -163,16 +193,9 @@ __device__ __forceinline__ void bond_constraint(int axis, const Rev &mi, q4 qi, 
                                                 d3 offset, d3 plastic, double (&k)[6], d3 &rest) {
     const m3 F = basis(axis, slerp_half(qi, qj));
     for (int q = 0; q < 6; ++q) k[q] = 0.0;
-    double Ca[6], Cb[6];
-    if (compliance(mi, F, qi, Ca) && compliance(mj, F, qj, Cb)) {
-        // K = (Ca + Cb)^-1 * 2*AREA/CELL, symmetric by cofactors.
-        double s0 = Ca[0] + Cb[0], s1 = Ca[1] + Cb[1], s2 = Ca[2] + Cb[2];
-        double s3 = Ca[3] + Cb[3], s4 = Ca[4] + Cb[4], s5 = Ca[5] + Cb[5];
-        double c00 = s3 * s5 - s4 * s4, c01 = s2 * s4 - s1 * s5, c02 = s1 * s4 - s2 * s3;
-        double c11 = s0 * s5 - s2 * s2, c12 = s1 * s2 - s0 * s4, c22 = s0 * s3 - s1 * s1;
-        double det = s0 * c00 + s1 * c01 + s2 * c02;
-        double sc = (2.0 * kCell * kCell / kCell) / det * scale;
-        double K[3][3] = {{c00 * sc, c01 * sc, c02 * sc}, {c01 * sc, c11 * sc, c12 * sc}, {c02 * sc, c12 * sc, c22 * sc}};
+    double Ks[6];
+    if (local_stiffness(F, mi, qi, mj, qj, scale, Ks)) {
+        double K[3][3] = {{Ks[0], Ks[1], Ks[2]}, {Ks[1], Ks[3], Ks[4]}, {Ks[2], Ks[4], Ks[5]}};
         double Fm[3][3] = {{F.c0.x, F.c1.x, F.c2.x}, {F.c0.y, F.c1.y, F.c2.y}, {F.c0.z, F.c1.z, F.c2.z}};
         double T[3][3];
         for (int r = 0; r < 3; ++r)
@@ -181,9 +204,157 @@ __device__ __forceinline__ void bond_constraint(int axis, const Rev &mi, q4 qi, 
         for (int r = 0; r < 3; ++r)
             for (int c = r; c < 3; ++c) k[q++] = T[r][0] * Fm[c][0] + T[r][1] * Fm[c][1] + T[r][2] * Fm[c][2];
     }
-    d3 dir = normalize_or_zero(F.c0);
-    d3 therm = mul_t(F, dir * kCell + (endpoint_expansion(mi, qi, dir) + endpoint_expansion(mj, qj, dir)) * (0.5 * kCell));
-    rest = mul(F, therm + offset + plastic * kCell);
+    rest = mul(F, thermal_rest_local(F, mi, qi, mj, qj) + offset + plastic * kCell);
+}
+
+// ---- fracture law (src/fracture.rs) -----------------------------------------
+// Per-REV strength row (Material fields, src/material.rs:20-63).
+enum : int {
+    S_TENSION, S_TENSION_PERP, S_COMPRESSION, S_COMPRESSION_PERP, S_SHEAR, S_SHEAR_PERP, S_YIELD, S_HARDENING,
+    S_PLASTIC_FAILURE, S_MODE, S_GC, S_GC_PERP, kStrength
+};
+enum : int { M_BRITTLE = 0, M_DUCTILE = 1, M_MIXED = 2 };
+// Per-bond history beyond damage and plastic strain (Bond, src/bond.rs:4-16).
+enum : int { H_EQ_PLASTIC, H_PLASTIC_WORK, H_FATIGUE, H_PREVIOUS, H_REVERSAL, H_INCREMENT, kHistory };
+
+struct Limits {
+    double tension, compression, shear, yield, hardening, plastic_failure, ductility, gc;
+};
+struct FractureCfg {
+    double damage_rate, energy_release, fatigue_cycles, fatigue_exponent, fatigue_threshold, heat_fraction;
+};
+// plastic_heat, stored_plastic, surface_work, damage_heat, released
+struct Dissipation {
+    double e[5];
+};
+
+// directional (src/fracture.rs:73-75).
+__device__ __forceinline__ double directional(double par, double perp, double alignment, double anisotropy) {
+    const double a = fmin(fmax(fabs(alignment), 0.0), 1.0);
+    return par + anisotropy * (1.0 - a) * (perp - par);
+}
+
+// One endpoint's limits (src/fracture.rs:76-127); s is its strength row.
+__device__ __forceinline__ Limits endpoint_limits(const float (&s)[kStrength], double alignment, double anisotropy) {
+    Limits l;
+    const int mode = int(s[S_MODE]);
+    l.ductility = mode == M_BRITTLE ? 0.0
+                  : mode == M_DUCTILE ? 1.0
+                                      : (1.0 - anisotropy) * 0.5 + anisotropy * (alignment * alignment);
+    l.tension = directional(s[S_TENSION], s[S_TENSION_PERP], alignment, anisotropy);
+    l.compression = directional(s[S_COMPRESSION], s[S_COMPRESSION_PERP], alignment, anisotropy);
+    l.shear = directional(s[S_SHEAR], s[S_SHEAR_PERP], alignment, anisotropy);
+    l.yield = s[S_YIELD];
+    l.hardening = s[S_HARDENING];
+    l.plastic_failure = s[S_PLASTIC_FAILURE];
+    l.gc = directional(s[S_GC], s[S_GC_PERP], alignment, anisotropy);
+    return l;
+}
+
+// pair_limits (src/fracture.rs:128-133): the weaker endpoint per field,
+// scaled by the smaller solid fraction squared.
+__device__ __forceinline__ Limits pair_limits(const Limits &a, const Limits &b, double scale) {
+    return Limits{fmin(a.tension, b.tension) * scale,
+                  fmin(a.compression, b.compression) * scale,
+                  fmin(a.shear, b.shear) * scale,
+                  fmin(a.yield, b.yield) * scale,
+                  fmin(a.hardening, b.hardening) * scale,
+                  fmin(a.plastic_failure, b.plastic_failure),
+                  fmin(a.ductility, b.ductility),
+                  fmin(a.gc, b.gc) * scale};
+}
+
+__device__ __forceinline__ d3 div3(d3 a, double s) { return {a.x / s, a.y / s, a.z / s}; }
+
+// Symmetric 3x3 (xx, xy, xz, yy, yz, zz) times a vector.
+__device__ __forceinline__ d3 sym_mul(const double (&k)[6], d3 u) {
+    return {k[0] * u.x + k[1] * u.y + k[2] * u.z, k[1] * u.x + k[3] * u.y + k[4] * u.z, k[2] * u.x + k[4] * u.y + k[5] * u.z};
+}
+
+// Elastic energy of a bond (src/fracture.rs:186-188).
+__device__ __forceinline__ double bond_energy(d3 u, const double (&k)[6], double damage) {
+    return fmax(0.5 * (1.0 - damage) * dot3(u, sym_mul(k, u)), 0.0);
+}
+
+// Fatigue counting (src/fracture.rs:190-207): only reversals of the signed
+// stress ratio consume life.
+__device__ __forceinline__ void fatigue(double (&h)[kHistory], double signed_ratio, const FractureCfg &c) {
+    const double delta = signed_ratio - h[H_PREVIOUS];
+    if (fabs(delta) > 1e-6) {
+        if (h[H_INCREMENT] * delta < 0.0) {
+            const double amplitude = fabs(h[H_PREVIOUS] - h[H_REVERSAL]) * 0.5;
+            if (amplitude > c.fatigue_threshold) {
+                const double level = fmin((amplitude - c.fatigue_threshold) / (1.0 - c.fatigue_threshold), 1.0);
+                h[H_FATIGUE] = fmin(h[H_FATIGUE] + 0.5 * pow(level, c.fatigue_exponent) / c.fatigue_cycles, 1.0);
+            }
+            h[H_REVERSAL] = h[H_PREVIOUS];
+        }
+        h[H_INCREMENT] = delta;
+    }
+    h[H_PREVIOUS] = signed_ratio;
+}
+
+// Constitutive update of one bond (src/fracture.rs:210-292): plastic return
+// mapping along the trial traction, the stress ratio against the limits,
+// fatigue, damage growth, and the split of the energy lost. u_total is the
+// bond extension in its frame, k the material stiffness (undamaged). Returns
+// true when the bond breaks (damage reaches one).
+__device__ __forceinline__ bool fracture_update(d3 &plastic, double &damage, double (&h)[kHistory], d3 u_total,
+                                                const double (&k)[6], const Limits &l, double dt, const FractureCfg &c,
+                                                Dissipation &r) {
+    for (int q = 0; q < 5; ++q) r.e[q] = 0.0;
+    const double area = kCell * kCell;
+    const double old_damage = damage;
+    d3 u = u_total - plastic * kCell;
+    const double before = bond_energy(u, k, old_damage);
+    const d3 trial = div3(sym_mul(k, u), area);
+    if (l.ductility > 1e-5) {
+        const double equivalent = sqrt(dot3(trial, trial));
+        const double yield_now = l.yield + l.hardening * h[H_EQ_PLASTIC];
+        if (equivalent > yield_now) {
+            const d3 direction = div3(trial, equivalent);
+            double kinv[6];
+            sym_inverse(k, kinv);
+            const d3 cdir = div3(sym_mul(kinv, direction * area), kCell);
+            const double modulus = 1.0 / fmax(sqrt(dot3(cdir, cdir)), 1e-20);
+            const double dep = (equivalent - yield_now) / (modulus + l.hardening) * l.ductility;
+            plastic = plastic + normalize_or_zero(cdir) * dep;
+            h[H_EQ_PLASTIC] += dep;
+            u = u_total - plastic * kCell;
+            const double lost = fmax(before - bond_energy(u, k, old_damage), 0.0);
+            h[H_PLASTIC_WORK] += lost;
+            r.e[0] = lost * c.heat_fraction;
+            r.e[1] = lost - r.e[0];
+        }
+    }
+    const d3 stress = div3(sym_mul(k, u), area);
+    const double normal_limit = stress.x >= 0.0 ? l.tension : l.compression;
+    const double sx = stress.x / fmax(normal_limit, 1.0);
+    const double sh = fmax(l.shear, 1.0);
+    const double ratio = sqrt(sx * sx + (stress.y * stress.y + stress.z * stress.z) / (sh * sh));
+    double principal;
+    if (fabs(stress.x) >= fmax(fabs(stress.y), fabs(stress.z)))
+        principal = stress.x;
+    else if (fabs(stress.y) >= fabs(stress.z))
+        principal = stress.y;
+    else
+        principal = stress.z;
+    fatigue(h, copysign(ratio, principal), c);
+    if (ratio >= 1.0) {
+        if (l.ductility <= 1e-5)
+            damage = 1.0;
+        else
+            damage = fmin(damage + c.damage_rate * dt * fmin(ratio, 10.0), 1.0);
+    }
+    if (l.ductility > 1e-5 && h[H_EQ_PLASTIC] >= fmax(l.plastic_failure, 1e-6)) damage = fmin(damage + c.damage_rate * dt, 1.0);
+    if (h[H_FATIGUE] >= 1.0) damage = 1.0;
+    const double lost = fmax(bond_energy(u, k, old_damage) - bond_energy(u, k, damage), 0.0);
+    const double budget = l.gc * area * (damage - old_damage);
+    r.e[2] = fmin(lost, budget);
+    const double available = lost - r.e[2];
+    r.e[4] = available * c.energy_release;
+    r.e[3] = available - r.e[4];
+    return damage >= 1.0;
 }
 
 }  // namespace vx

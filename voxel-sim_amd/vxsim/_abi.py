@@ -35,7 +35,27 @@ class SceneDesc(ctypes.Structure):
         ("fibers", c_fp), ("anisotropy", c_fp), ("delta_temp", c_fp), ("alpha_par", c_fp),
         ("alpha_perp", c_fp), ("neighbor", c_u32p), ("damage", c_fp), ("kscale", c_fp),
         ("formation_offset", c_fp), ("plastic_strain", c_fp), ("targets", c_fp), ("loads", c_fp),
+        ("strength", c_fp), ("bond_history", c_fp),
     ]
+
+
+class FractureParams(ctypes.Structure):
+    _fields_ = [
+        ("enabled", ctypes.c_uint32), ("reserved", ctypes.c_uint32), ("damage_rate", ctypes.c_double),
+        ("energy_release", ctypes.c_double), ("fatigue_cycles", ctypes.c_double),
+        ("fatigue_exponent", ctypes.c_double), ("fatigue_threshold", ctypes.c_double),
+        ("plastic_heat_fraction", ctypes.c_double),
+    ]
+
+
+class FractureReport(ctypes.Structure):
+    _fields_ = [
+        ("bonds_broken", ctypes.c_uint64), ("plastic_heat", ctypes.c_double), ("stored_plastic", ctypes.c_double),
+        ("surface_work", ctypes.c_double), ("damage_heat", ctypes.c_double), ("released", ctypes.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {f[0]: getattr(self, f[0]) for f in self._fields_}
 
 
 class SolveParams(ctypes.Structure):
@@ -61,7 +81,7 @@ class SolveReport(ctypes.Structure):
 StepParams._fields_ = [
     ("fixed_dt", ctypes.c_double), ("gravity", ctypes.c_double * 3), ("substeps", ctypes.c_uint32),
     ("corotation_iterations", ctypes.c_uint32), ("corotation_tolerance", ctypes.c_double),
-    ("solve", SolveParams),
+    ("solve", SolveParams), ("fracture", FractureParams),
 ]
 
 
@@ -69,7 +89,7 @@ class StepReport(ctypes.Structure):
     _fields_ = [
         ("solves", ctypes.c_uint32), ("inner_iterations", ctypes.c_uint32), ("outer_sweeps", ctypes.c_uint32),
         ("active_rows", ctypes.c_uint32), ("last_relative_residual", ctypes.c_double),
-        ("last_angle", ctypes.c_double), ("seconds", ctypes.c_double),
+        ("last_angle", ctypes.c_double), ("seconds", ctypes.c_double), ("fracture", FractureReport),
     ]
 
 
@@ -78,7 +98,7 @@ class BatchSceneReport(ctypes.Structure):
         ("state", ctypes.c_uint32), ("status", ctypes.c_uint32), ("steps_done", ctypes.c_uint32),
         ("solves", ctypes.c_uint32), ("inner_iterations", ctypes.c_uint64), ("outer_sweeps", ctypes.c_uint64),
         ("rows", ctypes.c_uint32), ("active_rows", ctypes.c_uint32), ("last_relative_residual", ctypes.c_double),
-        ("last_angle", ctypes.c_double),
+        ("last_angle", ctypes.c_double), ("fracture", FractureReport),
     ]
 
 
@@ -119,6 +139,8 @@ SIGNATURES = [
     ("vx_batch_states", ctypes.c_int32, [ctypes.c_void_p, c_u32p]),
     ("vx_batch_download", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint32, c_dp, c_dp, c_dp]),
     ("vx_batch_release", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint32]),
+    ("vx_scene_download_bonds", ctypes.c_int32, [ctypes.c_void_p, c_u32p, c_fp, c_fp, c_fp]),
+    ("vx_batch_download_bonds", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint32, c_u32p, c_fp, c_fp, c_fp]),
 ]
 
 

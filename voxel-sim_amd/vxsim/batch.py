@@ -16,7 +16,7 @@ import numpy as np
 from . import lib
 from ._abi import (BatchSceneReport, VX_ERR_TOO_LARGE, VX_SCENE_DONE, VX_SCENE_FAILED, VX_SCENE_FREE,
                    VX_SCENE_RUNNING, c_dp, c_u32p)
-from .grid import scene_desc, step_params
+from .grid import _report, download_bonds, scene_desc, step_params
 from .solver import SolverError, check, last_error
 
 FREE, RUNNING, DONE, FAILED = VX_SCENE_FREE, VX_SCENE_RUNNING, VX_SCENE_DONE, VX_SCENE_FAILED
@@ -106,7 +106,7 @@ class Batch:
     def report(self, slot: int) -> dict:
         r = BatchSceneReport()
         check(lib().vx_batch_report(self.handle, int(slot), ctypes.byref(r)))
-        return {f[0]: getattr(r, f[0]) for f in r._fields_}
+        return _report(r)
 
     def states(self) -> np.ndarray:
         """VX_SCENE_* state of every slot after the last tick."""
@@ -129,6 +129,9 @@ class Batch:
         check(lib().vx_batch_download(self.handle, int(slot), pos.ctypes.data_as(c_dp), rot.ctypes.data_as(c_dp),
                                       vel.ctypes.data_as(c_dp)))
         return dict(positions=pos, rotations=rot, velocities=vel)
+
+    def download_bonds(self, slot: int) -> dict:
+        return download_bonds(lib().vx_batch_download_bonds, self.handle, int(slot), n=self._rows[int(slot)])
 
     def release(self, slot: int):
         check(lib().vx_batch_release(self.handle, int(slot)))

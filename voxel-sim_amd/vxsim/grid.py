@@ -12,8 +12,8 @@ import ctypes
 import numpy as np
 
 from . import lib
-from ._abi import (SceneDesc, SolveParams, SolveReport, StepParams, StepReport, VX_SOLVE_F64, VX_SOLVE_MIXED, c_dp,
-                   c_fp, c_u8p, c_u32p)
+from ._abi import (FractureParams, SceneDesc, SolveParams, SolveReport, StepParams, StepReport, VX_SOLVE_F64,
+                   VX_SOLVE_MIXED, c_dp, c_fp, c_u8p, c_u32p)
 from .solver import SolverError, check
 
 MIXED = VX_SOLVE_MIXED
@@ -26,7 +26,21 @@ def solve_params(tolerance=1e-9, inner_tolerance=1e-3, max_inner=4000, max_outer
 
 
 def _report(r) -> dict:
-    return {f[0]: getattr(r, f[0]) for f in r._fields_}
+    out = {}
+    for f in r._fields_:
+        v = getattr(r, f[0])
+        out[f[0]] = _report(v) if isinstance(v, ctypes.Structure) else v
+    return out
+
+
+def fracture_params(config=None) -> FractureParams:
+    """vx_fracture_params from a `vxsim.scenes.FractureConfig` (None: the
+    pass is off)."""
+    if config is None:
+        return FractureParams()
+    c = config
+    return FractureParams(1, 0, float(c.damage_rate), float(c.energy_release), float(c.fatigue_cycles),
+                          float(c.fatigue_exponent), float(c.fatigue_threshold), float(c.plastic_heat_fraction))
 
 
 def scene_desc(scene):
@@ -52,14 +66,30 @@ def scene_desc(scene):
         kin.ctypes.data_as(c_u8p), f(scene.young), f(scene.young_perp), f(scene.shear), f(scene.solid),
         f(scene.fibers), f(scene.anisotropy), f(scene.delta_temp), f(scene.alpha_par), f(scene.alpha_perp),
         nbr.ctypes.data_as(c_u32p), f(scene.damage), f(scene.kscale), f(scene.formation_offset),
-        f(scene.plastic_strain), f(scene.targets), f(scene.loads))
+        f(scene.plastic_strain), f(scene.targets), f(scene.loads), f(getattr(scene, "strength", None)),
+        f(getattr(scene, "bond_history", None)))
     return desc, keep
 
 
 def step_params(fixed_dt=0.001, gravity=(0.0, 0.0, -9.81), substeps=1, corotation_iterations=2,
-                corotation_tolerance=1e-5, solve=None) -> StepParams:
+                corotation_tolerance=1e-5, solve=None, fracture=None) -> StepParams:
+    """vx_step_params; `fracture` is a `vxsim.scenes.FractureConfig` that
+    turns the fracture pass on (None: off)."""
     return StepParams(float(fixed_dt), (ctypes.c_double * 3)(*gravity), int(substeps), int(corotation_iterations),
-                      float(corotation_tolerance), solve if solve is not None else solve_params())
+                      float(corotation_tolerance), solve if solve is not None else solve_params(),
+                      fracture_params(fracture))
+
+
+def download_bonds(fn, *args, n: int) -> dict:
+    """Bond state through vx_scene_download_bonds / vx_batch_download_bonds:
+    neighbor (3, n), damage (3, n), plastic_strain (3, n, 3), history (3, n, 6)."""
+    nb = np.empty((3, n), np.uint32)
+    dmg = np.empty((3, n), np.float32)
+    pl = np.empty((3, n, 3), np.float32)
+    hist = np.empty((3, n, 6), np.float32)
+    check(fn(*args, nb.ctypes.data_as(c_u32p), dmg.ctypes.data_as(c_fp), pl.ctypes.data_as(c_fp),
+             hist.ctypes.data_as(c_fp)))
+    return dict(neighbor=nb, damage=dmg, plastic_strain=pl, history=hist)
 
 
 class GridScene:
@@ -125,8 +155,8 @@ class GridScene:
         check(lib().vx_scene_finish_substep(self.handle, float(h)))
 
     def step(self, fixed_dt=0.001, gravity=(0.0, 0.0, -9.81), substeps=1, corotation_iterations=2,
-             corotation_tolerance=1e-5, solve=None) -> dict:
-        sp = step_params(fixed_dt, gravity, substeps, corotation_iterations, corotation_tolerance, solve)
+             corotation_tolerance=1e-5, solve=None, fracture=None) -> dict:
+        sp = step_params(fixed_dt, gravity, substeps, corotation_iterations, corotation_tolerance, solve, fracture)
         rep = StepReport()
         check(lib().vx_scene_step(self.handle, ctypes.byref(sp), ctypes.byref(rep)))
         return _report(rep)
@@ -141,6 +171,9 @@ class GridScene:
         check(lib().vx_scene_download(self.handle, pos.ctypes.data_as(c_dp), rot.ctypes.data_as(c_dp),
                                       vel.ctypes.data_as(c_dp), x.ctypes.data_as(c_dp)))
         return dict(positions=pos, rotations=rot, velocities=vel, correction=x)
+
+    def download_bonds(self) -> dict:
+        return download_bonds(lib().vx_scene_download_bonds, self.handle, n=self.n)
 
     def export_rows(self) -> dict:
         na = self.active

@@ -62,10 +62,13 @@ class SplitMix64:
         return out[:size]
 
 
-BRITTLE, DUCTILE, MIXED = 0, 1, 2  # FractureMode (src/material.rs:9-18)
-# Order of the per-REV strength row (Material fields, src/material.rs:27-62).
+BRITTLE, DUCTILE, MIXED = 0, 1, 2  # FractureMode (src/material.rs:4-8)
+# Order of the per-REV strength row (Material fields, src/material.rs:20-63).
 STRENGTH_FIELDS = ("tension", "tension_perp", "compression", "compression_perp", "shear", "shear_perp",
                    "yield", "hardening", "plastic_failure", "mode", "gc", "gc_perp")
+# Per-bond history beyond damage and plastic strain (Bond, src/bond.rs:4-16).
+HISTORY_FIELDS = ("equivalent_plastic_strain", "plastic_work", "fatigue_damage", "previous_stress_ratio",
+                  "reversal_stress_ratio", "loading_increment")
 
 
 @dataclasses.dataclass(frozen=True)
@@ -157,6 +160,7 @@ class Scene:
     config: StepConfig = dataclasses.field(default_factory=StepConfig)
     name: str = ""
     strength: np.ndarray = None  # (n, 12) STRENGTH_FIELDS per REV
+    bond_history: np.ndarray = None  # (3, n, 6) HISTORY_FIELDS per +axis bond; None = zero
 
     @property
     def n(self) -> int:
@@ -265,6 +269,34 @@ def tension_10cube() -> Scene:
     s.config = StepConfig(fixed_dt=0.001, substeps=1, iterations=1000, corotation_iterations=4,
                           solver_relative_tolerance=1e-10, gravity=(0.0, 0.0, 0.0))
     s.name = "tension_10x10x10"
+    return s
+
+
+def pulled_column(material: str, pull: float, dims=(3, 3, 6), grain: int = 2) -> Scene:
+    """A box of one material with its bottom layer held and its top layer
+    axis-targeted `pull` metres up in Z (all axes targeted), no gravity: the
+    fracture fixture of the reference's tension tests at REV scale."""
+    s = uniform_box(dims, material, grain=grain)
+    z = s.keys[:, 2]
+    s.targets[z == z.min()] = s.positions[z == z.min()]
+    s.targets[z == z.max()] = s.positions[z == z.max()] + np.array([0, 0, pull], np.float32)
+    s.config = StepConfig(fixed_dt=0.001, substeps=1, iterations=1000, corotation_iterations=2,
+                          solver_relative_tolerance=1e-10, gravity=(0.0, 0.0, 0.0))
+    s.name = "pulled_%s_%dx%dx%d" % ((material,) + tuple(dims))
+    return s
+
+
+def pulled_mixed(dims, seed: int, pull: float) -> Scene:
+    """A regional wood/iron/stone box (`mixed_box`, bottom face fixed) whose
+    top layer is axis-targeted `pull` metres up in Z, with gravity: brittle
+    stone and cross-grain wood bonds break, iron and along-grain wood yield."""
+    s = mixed_box(dims, seed, "z-")
+    z = s.keys[:, 2]
+    top = z == z.max()
+    s.targets[top] = s.positions[top] + np.array([0, 0, pull], np.float32)
+    s.config = StepConfig(fixed_dt=0.001, substeps=1, iterations=1000, corotation_iterations=2,
+                          solver_relative_tolerance=1e-10, gravity=(0.0, 0.0, -9.81))
+    s.name = "pulled_mixed_%dx%dx%d" % tuple(dims)
     return s
 
 
