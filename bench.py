@@ -232,6 +232,46 @@ def large_bench(steps: int, warmup: int, compare: bool = True):
     print(json.dumps(line))
 
 
+def cut_bench(steps: int, dims=(192, 192, 96)):
+    """The cutting scene with fracture: the 192x192x96 wood block, the blade's
+    40x5 edge driven 5 mm per 1 ms step through per-step axis targets, up to
+    six corotation passes and the fracture pass every step (broken bonds leave
+    the lattice before the next assembly). Prints one JSON line with the step
+    time, the solve throughput and the bonds broken per step."""
+    from vxsim import grid, scenes
+    blade = (40, 5, 20)
+    scene = scenes.wood_block_blade(dims=dims, blade=blade)
+    cfg = scene.config
+    sp = grid.solve_params(tolerance=1e-6, inner_tolerance=INNER_TOL, max_inner=40000, max_outer=12)
+    reps = []
+    walls = []
+    with grid.GridScene(scene) as gs:
+        for k in range(steps):
+            t = time.perf_counter()
+            gs.set_targets_and_loads(scenes.blade_targets(scene, blade, 5.0, cfg.fixed_dt, k), scene.loads)
+            r = gs.step(fixed_dt=cfg.fixed_dt, gravity=cfg.gravity, substeps=cfg.substeps,
+                        corotation_iterations=cfg.corotation_iterations, corotation_tolerance=cfg.corotation_tolerance,
+                        solve=sp, fracture=scenes.FractureConfig())
+            walls.append(time.perf_counter() - t)
+            r["active"] = gs.active
+            reps.append(r)
+            print("cut step %d: %.1f ms, %d solves, %d inner iterations, %d bonds broken" %
+                  (k, 1000 * walls[-1], r["solves"], r["inner_iterations"], r["fracture"]["bonds_broken"]),
+                  file=sys.stderr, flush=True)
+    dof_iters = sum(3.0 * r["active"] * (r["inner_iterations"] + r["outer_sweeps"]) for r in reps)
+    line = {"metric": "DOF-PCG-iters/s", "value": dof_iters / sum(walls), "unit": "DOF-PCG-iters/s",
+            "workload": "%s with fracture: %d REVs, blade edge %dx%d REVs at 5 m/s, %d steps of 1 ms, <=%d passes, "
+                        "rel. residual 1e-6" % (scene.name, scene.n, blade[0], blade[1], steps,
+                                               cfg.corotation_iterations),
+            "ms_per_step": [round(1000 * w, 2) for w in walls], "solves": [r["solves"] for r in reps],
+            "inner_iterations": [r["inner_iterations"] for r in reps],
+            "bonds_broken": [int(r["fracture"]["bonds_broken"]) for r in reps],
+            "dissipated_j": {e: sum(r["fracture"][e] for r in reps) for e in
+                             ("plastic_heat", "stored_plastic", "surface_work", "damage_heat", "released")},
+            "data": "synthetic", "dtype": "f32 inner / f64 refinement, reductions, assembly and fracture"}
+    print(json.dumps(line))
+
+
 def pmc_driver(large: bool = False):
     """Short run for the rocprofv3 counter passes: 20 launches of the FP32
     bond SpMV on the assembled 64^3 system (or the 192x192x96 block), then 3
@@ -266,14 +306,23 @@ def main():
     ap.add_argument("--pmc-driver", action="store_true", help="short SpMV + copy run for counter passes")
     ap.add_argument("--serve", action="store_true", help="served request stream instead of the step benchmark")
     ap.add_argument("--large", action="store_true", help="192x192x96 block instead of the 64^3 box")
+    ap.add_argument("--cut", action="store_true", help="192x192x96 block cut by the blade with fracture")
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--serve-revs", type=str, default="1e3,2e5")
     ap.add_argument("--serve-steps", type=int, default=10)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (independent scenes shard "
+              "without communication; a single scene would need a slab decomposition). Run with --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
     if args.pmc_driver:
         pmc_driver(args.large)
+        return
+    if args.cut:
+        cut_bench(args.steps)
         return
     if args.large:
         large_bench(args.steps, args.warmup, compare=not args.no_compare)
@@ -283,11 +332,6 @@ def main():
         serve_bench(args.serve_rate, args.serve_seconds, lo, hi, args.serve_steps, args.serve_budget_gb,
                     compare=not args.no_compare)
         return
-    if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (independent scenes shard "
-              "without communication; a single scene would need a slab decomposition). Run with --gpus 1.",
-              file=sys.stderr)
-        sys.exit(2)
 
     from vxsim import grid, scenes, solver
 

@@ -258,6 +258,13 @@ typedef struct vx_step_report {
 
 int32_t vx_scene_step(vx_scene *scene, const vx_step_params *params, vx_step_report *report);
 
+/* Per-step inputs of step_with_axis_targets_and_loads (src/solver.rs:
+ * 440-482): replace the scene's axis targets (n*3 world coordinates, NaN =
+ * free axis; NULL: none) and applied loads (n*3 newtons; NULL: zero) for
+ * the following steps. Targets must be finite where not NaN. The warm start
+ * is dropped, since the solved axes may change. */
+int32_t vx_scene_set_targets_and_loads(vx_scene *scene, const float *targets, const float *loads);
+
 /* Copy the bond state out (any pointer may be NULL): neighbor 3*n (v +
  * stride or VX_EMPTY once broken), damage 3*n, plastic_strain 3*n*3 and
  * history 3*n*6 (VX_H_* order), all per +axis bond [axis][voxel]. */

@@ -1504,6 +1504,30 @@ int32_t create_batch(uint64_t capacity_rows, uint32_t max_scenes, uint32_t chunk
     return VX_OK;
 }
 
+// Per REV: targeted axes (tmask) and solved axes (mask: not kinematic, not
+// targeted); active = REVs with a solved axis (src/solver.rs:1554-1565).
+// Targets must be finite where they are not NaN (src/solver.rs:455-459).
+int32_t free_masks(const uint8_t *kinematic, const float *targets, uint32_t n, std::vector<uint8_t> &tmask,
+                   std::vector<uint8_t> &mask, uint32_t *active) {
+    tmask.assign(n, 0);
+    mask.assign(n, 0);
+    uint32_t act = 0;
+    for (uint32_t v = 0; v < n; ++v) {
+        uint8_t m = 0;
+        for (int a = 0; a < 3; ++a) {
+            const float t = targets ? targets[3 * size_t(v) + a] : NAN;
+            const bool targeted = !std::isnan(t);
+            if (targeted && !std::isfinite(t)) return fail(VX_ERR_INVALID_ARGUMENT, "target of REV %u axis %d is not finite", v, a);
+            if (targeted) tmask[v] |= uint8_t(1u << a);
+            if (!kinematic[v] && !targeted) m |= uint8_t(1u << a);
+        }
+        mask[v] = m;
+        act += m ? 1 : 0;
+    }
+    *active = act;
+    return VX_OK;
+}
+
 // Validate a box description; fills per-REV bond / target / free masks.
 int32_t check_desc(const vx_scene_desc *d, std::vector<uint8_t> &bonds, std::vector<uint8_t> &tmask,
                    std::vector<uint8_t> &mask, uint32_t *active) {
@@ -1517,8 +1541,6 @@ int32_t check_desc(const vx_scene_desc *d, std::vector<uint8_t> &bonds, std::vec
         return fail(VX_ERR_INVALID_ARGUMENT, "missing scene array");
     const uint32_t n = uint32_t(n64), nx = d->nx, ny = d->ny;
     bonds.assign(n, 0);
-    tmask.assign(n, 0);
-    mask.assign(n, 0);
     for (uint32_t a = 0; a < 3; ++a) {
         const uint32_t s = a == 0 ? 1 : (a == 1 ? nx : nx * ny);
         for (uint32_t v = 0; v < n; ++v) {
@@ -1530,19 +1552,7 @@ int32_t check_desc(const vx_scene_desc *d, std::vector<uint8_t> &bonds, std::vec
             bonds[v] |= uint8_t(1u << a);
         }
     }
-    uint32_t act = 0;
-    for (uint32_t v = 0; v < n; ++v) {
-        uint8_t m = 0;
-        for (int a = 0; a < 3; ++a) {
-            const bool targeted = d->targets && !std::isnan(d->targets[3 * size_t(v) + a]);
-            if (targeted) tmask[v] |= uint8_t(1u << a);
-            if (!d->kinematic[v] && !targeted) m |= uint8_t(1u << a);
-        }
-        mask[v] = m;
-        act += m ? 1 : 0;
-    }
-    *active = act;
-    return VX_OK;
+    return free_masks(d->kinematic, d->targets, n, tmask, mask, active);
 }
 
 // First fit of `need` contiguous free tiles; cap_tiles when none.
@@ -2138,6 +2148,42 @@ extern "C" int32_t vx_scene_download_bonds(vx_scene *sc, uint32_t *neighbor, flo
     if (int32_t c = scene_ok(sc)) return c;
     Lock lock;
     return download_bonds(sc->batch, sc->slot, neighbor, damage, plastic_strain, history);
+}
+
+// Replace the scene's axis targets and applied loads for the following steps
+// (the per-step inputs of step_with_axis_targets_and_loads,
+// src/solver.rs:440-482). The warm start is dropped: the solved axes change.
+extern "C" int32_t vx_scene_set_targets_and_loads(vx_scene *sc, const float *targets, const float *loads) {
+    clear_error();
+    if (int32_t c = scene_ok(sc)) return c;
+    Lock lock;
+    vx_batch *b = sc->batch;
+    if (int32_t c = drain(b)) return c;
+    SlotHost &h = b->slots[sc->slot];
+    const SlotGeo &g = h.geo;
+    const DevArena &A = b->A;
+    const size_t n = g.n, voff = g.voff, R = A.R;
+    std::vector<uint8_t> kin(n), tmask, mask;
+    VX_HIP(hipMemcpy(kin.data(), A.kinematic + voff, n, hipMemcpyDeviceToHost));
+    uint32_t active = 0;
+    if (int32_t c = free_masks(kin.data(), targets, uint32_t(n), tmask, mask, &active)) return c;
+    std::vector<double> buf(n);
+    std::vector<float> fbuf(n);
+    for (int q = 0; q < 3; ++q) {
+        for (size_t v = 0; v < n; ++v) {
+            const float t = targets ? targets[3 * v + q] : NAN;
+            buf[v] = std::isnan(t) ? 0.0 : double(t);
+            fbuf[v] = loads ? loads[3 * v + q] : 0.0f;
+        }
+        if (int32_t c = put(A.targets + q * R + voff, buf.data(), 8 * n)) return c;
+        if (int32_t c = put(A.loads + q * R + voff, fbuf.data(), 4 * n)) return c;
+    }
+    if (int32_t c = put(A.tmask + voff, tmask.data(), n)) return c;
+    if (int32_t c = put(A.mask + voff, mask.data(), n)) return c;
+    h.active = active;
+    sc->active = active;
+    h.warm_ok = false;
+    return VX_OK;
 }
 
 // Reference row packing (src/solver.rs:1652-1672, 1731-1777) of the assembled

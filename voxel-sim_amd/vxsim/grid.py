@@ -104,6 +104,7 @@ class GridScene:
         self.n = scene.n
         self.dims = (nx, ny, nz)
         self.active = scene.active_count()
+        self._kinematic = np.asarray(scene.kinematic, np.uint8).copy()
         if tuned:
             from . import tune
             self.set_schedule(tune.lookup(scene.n, kind="single")["iters"])
@@ -171,6 +172,16 @@ class GridScene:
         check(lib().vx_scene_download(self.handle, pos.ctypes.data_as(c_dp), rot.ctypes.data_as(c_dp),
                                       vel.ctypes.data_as(c_dp), x.ctypes.data_as(c_dp)))
         return dict(positions=pos, rotations=rot, velocities=vel, correction=x)
+
+    def set_targets_and_loads(self, targets=None, loads=None):
+        """Axis targets (n, 3), NaN = free, and applied loads (n, 3) for the
+        following steps (None: no targets / zero loads). Drops the warm start."""
+        t = None if targets is None else np.ascontiguousarray(targets, np.float32).reshape(self.n, 3)
+        l_ = None if loads is None else np.ascontiguousarray(loads, np.float32).reshape(self.n, 3)
+        check(lib().vx_scene_set_targets_and_loads(self.handle, None if t is None else t.ctypes.data_as(c_fp),
+                                                   None if l_ is None else l_.ctypes.data_as(c_fp)))
+        free = (np.isnan(t) if t is not None else np.ones((self.n, 3), bool)) & (self._kinematic == 0)[:, None]
+        self.active = int(np.count_nonzero(free.any(axis=1)))
 
     def download_bonds(self) -> dict:
         return download_bonds(lib().vx_scene_download_bonds, self.handle, n=self.n)

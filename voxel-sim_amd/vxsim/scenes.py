@@ -370,16 +370,34 @@ def wood_block_blade(seed: int = 5, dims=(192, 192, 96), blade=(40, 5, 20), spee
     s.kscale = (0.95 + 0.1 * rng.uniform(3 * n)).astype(np.float32).reshape(3, n)
     z = s.keys[:, 2]
     s.kinematic = (z == z.min()).astype(np.uint8)
-    x = s.keys[:, 0] + dims[0] // 2
-    y = s.keys[:, 1] + dims[1] // 2
-    bx0, by0 = (dims[0] - blade[0]) // 2, (dims[1] - blade[1]) // 2
-    under = (z == z.max()) & (x >= bx0) & (x < bx0 + blade[0]) & (y >= by0) & (y < by0 + blade[1])
     dt = 0.001
-    s.targets[under, 2] = s.positions[under, 2] - np.float32(speed * dt)
+    s.targets = blade_targets(s, blade, speed, dt, 0)
     s.config = StepConfig(fixed_dt=dt, substeps=1, iterations=1000, corotation_iterations=6,
                           solver_relative_tolerance=1e-6, gravity=(0.0, 0.0, -9.81))
     s.name = "wood_block_%dx%dx%d_blade" % tuple(dims)
     return s
+
+
+def blade_footprint(s: Scene, blade=(40, 5, 20)) -> np.ndarray:
+    """Top-layer REVs under the blade's blade[0] x blade[1] edge, centred on
+    the box's top face."""
+    nx, ny, _ = s.dims
+    z = s.keys[:, 2]
+    x = s.keys[:, 0] - s.keys[:, 0].min()
+    y = s.keys[:, 1] - s.keys[:, 1].min()
+    bx0, by0 = (nx - blade[0]) // 2, (ny - blade[1]) // 2
+    return (z == z.max()) & (x >= bx0) & (x < bx0 + blade[0]) & (y >= by0) & (y < by0 + blade[1])
+
+
+def blade_targets(s: Scene, blade=(40, 5, 20), speed=5.0, dt=0.001, step: int = 0) -> np.ndarray:
+    """Axis targets of step `step` (0-based) of a blade driven at `speed` m/s
+    in -Z: the footprint REVs are targeted in Z (step + 1) steps of travel
+    below their rest height; every other axis is free."""
+    t = np.full((s.n, 3), np.nan, np.float32)
+    under = blade_footprint(s, blade)
+    rest_z = s.keys[:, 2].astype(np.float32) * CELL + np.float32(CELL * np.float32(0.5))  # as _box places REVs
+    t[under, 2] = rest_z[under] - np.float32(speed * dt * (step + 1))
+    return t
 
 
 def batch_boxes(count: int = 256, seed: int = 3, lo: int = 8, hi: int = 48):
