@@ -33,6 +33,10 @@ SOLVE_TOL = 1e-9        # outer FP64 relative true residual
 INNER_TOL = 1e-3        # FP32 residual drop between FP64 residual replacements
 PEAK_HBM_GBS = 8000.0   # MI355X HBM3E
 SPMV_BYTES_PER_ROW = 145  # z 12 + p_old 12 + D 24 + 3 bond blocks 72 + mask 1 + p_new 12 + Ap 12
+# fracture_bonds per REV row: reads bond bits 1 + kinematic 1 + material 44 + rotation and assembly rotation 64 +
+# position 24 + strength row 48 + per +axis bond (kscale 4, offset 12, plastic 12, damage 4, history 24) x 3 = 350;
+# writes plastic 36 + damage 12 + history 72 = 120
+FRACTURE_BYTES_PER_ROW = 470
 SPMV_KERNEL = "pcg_bond_spmv<float>"
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pcg_bond_spmv_64cube_traffic.txt")
 COPY_BYTES = 1 << 30      # calibration copy for the HBM byte counters
@@ -255,9 +259,9 @@ def cut_bench(steps: int, dims=(192, 192, 96)):
             walls.append(time.perf_counter() - t)
             r["active"] = gs.active
             reps.append(r)
-            print("cut step %d: %.1f ms, %d solves, %d inner iterations, %d bonds broken" %
-                  (k, 1000 * walls[-1], r["solves"], r["inner_iterations"], r["fracture"]["bonds_broken"]),
-                  file=sys.stderr, flush=True)
+            print("cut step %d: %.1f ms, %d solves, %d inner iterations, %d bonds broken, fracture pass %.3f ms" %
+                  (k, 1000 * walls[-1], r["solves"], r["inner_iterations"], r["fracture"]["bonds_broken"],
+                   1000 * r["fracture"]["seconds"]), file=sys.stderr, flush=True)
     dof_iters = sum(3.0 * r["active"] * (r["inner_iterations"] + r["outer_sweeps"]) for r in reps)
     line = {"metric": "DOF-PCG-iters/s", "value": dof_iters / sum(walls), "unit": "DOF-PCG-iters/s",
             "workload": "%s with fracture: %d REVs, blade edge %dx%d REVs at 5 m/s, %d steps of 1 ms, <=%d passes, "
@@ -266,6 +270,12 @@ def cut_bench(steps: int, dims=(192, 192, 96)):
             "ms_per_step": [round(1000 * w, 2) for w in walls], "solves": [r["solves"] for r in reps],
             "inner_iterations": [r["inner_iterations"] for r in reps],
             "bonds_broken": [int(r["fracture"]["bonds_broken"]) for r in reps],
+            "fracture_pass": {"kernels": "fracture_bonds + reduce_fracture",
+                              "ms_per_step": [round(1000 * r["fracture"]["seconds"], 4) for r in reps],
+                              "bytes_per_row": FRACTURE_BYTES_PER_ROW,
+                              "achieved_gbs": FRACTURE_BYTES_PER_ROW * scene.n / max(
+                                  sum(r["fracture"]["seconds"] for r in reps) / len(reps), 1e-12) / 1e9,
+                              "peak_gbs": PEAK_HBM_GBS},
             "dissipated_j": {e: sum(r["fracture"][e] for r in reps) for e in
                              ("plastic_heat", "stored_plastic", "surface_work", "damage_heat", "released")},
             "data": "synthetic", "dtype": "f32 inner / f64 refinement, reductions, assembly and fracture"}

@@ -233,6 +233,7 @@ typedef struct vx_fracture_report {
     double surface_work;
     double damage_heat;
     double released;
+    double seconds;              /* device time of the fracture kernels (HIP events) */
 } vx_fracture_report;
 
 typedef struct vx_step_params {

@@ -53,6 +53,7 @@ def check_parity(O, s, steps=STEPS):
     scale = max(max(sum(ro["dissipated"].values()) for ro in reps_o), 1e-30)
     for k, (rd, ro) in enumerate(zip(reps_d, reps_o)):
         fr = rd["fracture"]
+        assert fr["seconds"] > 0.0  # the pass ran (HIP-event timed)
         assert fr["bonds_broken"] == ro["broken"], (s.name, k)
         broken_total += ro["broken"]
         for e in ENERGIES:
@@ -194,7 +195,8 @@ def test_fracture_off_leaves_bonds_alone(gpu, scenes):
     assert np.array_equal(out["positions"], out_p["positions"])
     assert np.array_equal(bonds["neighbor"], s.neighbor)
     assert not bonds["damage"].any() and not bonds["history"].any()
-    assert all(r["fracture"]["bonds_broken"] == 0 and r["fracture"]["surface_work"] == 0.0 for r in reps)
+    assert all(r["fracture"]["bonds_broken"] == 0 and r["fracture"]["surface_work"] == 0.0 and
+               r["fracture"]["seconds"] == 0.0 for r in reps)
 
 
 def test_fracture_needs_strength(gpu, scenes):

@@ -1085,7 +1085,9 @@ struct Flight {
     uint32_t *bits = nullptr;
     SlotState *st = nullptr;
     hipEvent_t done = nullptr;
+    hipEvent_t frac0 = nullptr, frac1 = nullptr;  // around the fracture kernels
     bool active = false;
+    bool frac_timed = false;
 };
 
 }  // namespace
@@ -1263,9 +1265,12 @@ int32_t launch_tick(vx_batch *b, Flight &f) {
         hipLaunchKernelGGL(fit_rotations, grid, blk, 0, s, b->A, uint32_t(B_FITB), 1);
         hipLaunchKernelGGL(reduce_angle, sg, blk, 0, s, b->A);
     }
-    if (any & B_FRACTURE) {
+    f.frac_timed = (any & B_FRACTURE) != 0;
+    if (f.frac_timed) {
+        VX_HIP(hipEventRecord(f.frac0, s));
         hipLaunchKernelGGL(fracture_bonds, grid, blk, 0, s, b->A);
         hipLaunchKernelGGL(reduce_fracture, sg, blk, 0, s, b->A);
+        VX_HIP(hipEventRecord(f.frac1, s));
     }
     if (any & B_FINISH) hipLaunchKernelGGL(finish_velocities, grid, blk, 0, s, b->A);
     VX_HIP(hipGetLastError());
@@ -1369,8 +1374,12 @@ int32_t retire(vx_batch *b, Flight &f) {
     if (!f.active) return VX_OK;
     f.active = false;
     VX_HIP(hipEventSynchronize(f.done));
+    float frac_ms = 0.0f;
+    if (f.frac_timed) VX_HIP(hipEventElapsedTime(&frac_ms, f.frac0, f.frac1));
     for (uint32_t i = 0; i < b->max_slots; ++i) {
         if (!f.bits[i] || !b->slots[i].used) continue;
+        // The fracture launches are shared by every scene in the tick.
+        if (f.bits[i] & B_FRACTURE) b->slots[i].frac.seconds += 1e-3 * double(frac_ms);
         b->known[i] = f.st[i];
         if (!b->slots[i].manual) advance(b->slots[i], b->known[i], f.bits[i]);
     }
@@ -1486,7 +1495,8 @@ int32_t create_batch(uint64_t capacity_rows, uint32_t max_scenes, uint32_t chunk
     for (auto &f : b->fl) {
         ok = ok && hipHostMalloc((void **)&f.bits, sizeof(uint32_t) * max_scenes, hipHostMallocDefault) == hipSuccess &&
              hipHostMalloc((void **)&f.st, sizeof(SlotState) * max_scenes, hipHostMallocDefault) == hipSuccess &&
-             hipEventCreateWithFlags(&f.done, hipEventDisableTiming) == hipSuccess;
+             hipEventCreateWithFlags(&f.done, hipEventDisableTiming) == hipSuccess &&
+             hipEventCreate(&f.frac0) == hipSuccess && hipEventCreate(&f.frac1) == hipSuccess;
         if (f.bits) std::memset(f.bits, 0, sizeof(uint32_t) * max_scenes);
     }
     b->known = st0;
@@ -1495,6 +1505,8 @@ int32_t create_batch(uint64_t capacity_rows, uint32_t max_scenes, uint32_t chunk
             if (f.bits) (void)hipHostFree(f.bits);
             if (f.st) (void)hipHostFree(f.st);
             if (f.done) (void)hipEventDestroy(f.done);
+            if (f.frac0) (void)hipEventDestroy(f.frac0);
+            if (f.frac1) (void)hipEventDestroy(f.frac1);
         }
         (void)hipFree(b->alloc);
         delete b;
@@ -1817,6 +1829,8 @@ extern "C" int32_t vx_batch_destroy(vx_batch *b) {
         if (f.bits) (void)hipHostFree(f.bits);
         if (f.st) (void)hipHostFree(f.st);
         if (f.done) (void)hipEventDestroy(f.done);
+        if (f.frac0) (void)hipEventDestroy(f.frac0);
+        if (f.frac1) (void)hipEventDestroy(f.frac1);
     }
     if (b->alloc) (void)hipFree(b->alloc);
     delete b;

@@ -52,6 +52,7 @@ class FractureReport(ctypes.Structure):
     _fields_ = [
         ("bonds_broken", ctypes.c_uint64), ("plastic_heat", ctypes.c_double), ("stored_plastic", ctypes.c_double),
         ("surface_work", ctypes.c_double), ("damage_heat", ctypes.c_double), ("released", ctypes.c_double),
+        ("seconds", ctypes.c_double),
     ]
 
     def as_dict(self) -> dict:
