@@ -100,6 +100,8 @@ struct SlotState {
     double frac[kFracN];  // the last fracture pass: energies and broken bonds
 };
 
+static_assert(sizeof(SlotState) % 8 == 0, "SlotState is copied in 8-byte words");
+
 // Scenes of more tiles than this have their inner partials summed once per
 // kernel by scene_reduce instead of by every workgroup of the consumer (whose
 // cost grows with the square of the tile count).
@@ -110,7 +112,7 @@ struct DevArena {
     const SlotGeo *tiles;  // per tile; slot == kFreeTile for a free tile
     const SlotGeo *geo;    // per slot
     const SlotParams *prm;
-    const uint32_t *bits;       // per slot, this tick
+    uint32_t *bits;             // per slot, this tick
     uint32_t *tile_bits;        // per tile: the owning slot's bits (0 if free)
     SlotState *st;
     double *partials;  // kSlots * cap_tiles
@@ -148,11 +150,22 @@ struct Tile {
 
 // Copy each slot's bits of this tick to its tiles, so a workgroup reads its
 // tile's descriptor and bits side by side.
-__global__ __launch_bounds__(kBlock) void spread_bits(DevArena A) {
+// Phase bits of up to kArgSlots scenes passed by value, so that a tick needs
+// no host-to-device copy before its first kernel; n = 0: the bits are
+// already in A.bits.
+constexpr uint32_t kArgSlots = 256;
+struct BitsArg {
+    uint32_t n;
+    uint32_t bits[kArgSlots];
+};
+
+// Per-slot bits (from the argument) and per-tile bits of this tick.
+__global__ __launch_bounds__(kBlock) void spread_bits(DevArena A, BitsArg arg) {
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t < arg.n) A.bits[t] = arg.bits[t];
     if (t >= A.cap_tiles) return;
     const uint32_t slot = A.tiles[t].slot;
-    A.tile_bits[t] = slot == kFreeTile ? 0u : A.bits[slot];
+    A.tile_bits[t] = slot == kFreeTile ? 0u : (arg.n ? arg.bits[slot] : A.bits[slot]);
 }
 
 // The tile of this workgroup and its scene; false for a free tile. The
@@ -1032,6 +1045,15 @@ __global__ __launch_bounds__(kBlock) void reduce_fracture(DevArena A) {
         for (int q = 0; q < kFracN; ++q) A.st[slot].frac[q] = acc[q];
 }
 
+// The scenes' states into pinned host memory at the end of a tick; the host
+// reads them after the tick's event.
+__global__ __launch_bounds__(kBlock) void publish_state(const SlotState *__restrict__ src, SlotState *dst, uint32_t n) {
+    const uint64_t *s = reinterpret_cast<const uint64_t *>(src);
+    uint64_t *d = reinterpret_cast<uint64_t *>(dst);
+    const uint32_t words = n * uint32_t(sizeof(SlotState) / 8);
+    for (uint32_t i = threadIdx.x; i < words; i += kBlock) d[i] = s[i];
+}
+
 // Velocities after a substep (src/solver.rs:668-683), no contact.
 __global__ __launch_bounds__(kBlock) void finish_velocities(DevArena A) {
     Tile c;
@@ -1083,7 +1105,8 @@ struct SlotHost {
 // after it (both pinned), and the event that marks its end.
 struct Flight {
     uint32_t *bits = nullptr;
-    SlotState *st = nullptr;
+    SlotState *st = nullptr;      // pinned, coherent: written by publish_state
+    SlotState *st_dev = nullptr;  // its device address
     hipEvent_t done = nullptr;
     hipEvent_t frac0 = nullptr, frac1 = nullptr;  // around the fracture kernels
     bool active = false;
@@ -1114,6 +1137,11 @@ struct vx_batch {
     bool single = false;  // one-scene arena of a vx_scene (never re-admits)
     uint32_t hw_tiles = 0;  // one past the highest tile in use
     uint32_t iters_per_tick = vx::kInnerChunk;
+    // Bits and launch grid of the last spread_bits enqueued; a tick with the
+    // same bits and grid and an unchanged tile table skips it.
+    std::vector<uint32_t> spread_last;
+    uint32_t spread_tiles = 0;
+    bool spread_dirty = true;
 };
 
 struct vx_scene {
@@ -1232,10 +1260,22 @@ int32_t launch_tick(vx_batch *b, Flight &f) {
         if (bi & B_INNER) inner[prec] = true;
         if (bi & B_RESID0) resid0[prec] = true;
     }
-    VX_HIP(hipMemcpyAsync(b->d_bits, f.bits, sizeof(uint32_t) * b->max_slots, hipMemcpyHostToDevice, s));
     const uint32_t tiles = launch_tiles(b);
     const dim3 grid(tiles), blk(kBlock), sg(b->max_slots);
-    hipLaunchKernelGGL(spread_bits, dim3((tiles + kBlock - 1) / kBlock), blk, 0, s, b->A);
+    if (b->spread_dirty || tiles != b->spread_tiles || !std::equal(f.bits, f.bits + b->max_slots, b->spread_last.begin())) {
+        BitsArg arg{};
+        if (b->max_slots <= kArgSlots) {
+            arg.n = b->max_slots;
+            std::copy(f.bits, f.bits + b->max_slots, arg.bits);
+        } else {
+            VX_HIP(hipMemcpyAsync(b->d_bits, f.bits, sizeof(uint32_t) * b->max_slots, hipMemcpyHostToDevice, s));
+        }
+        const uint32_t nb = std::max((tiles + kBlock - 1) / kBlock, (arg.n + kBlock - 1) / kBlock);
+        hipLaunchKernelGGL(spread_bits, dim3(nb), blk, 0, s, b->A, arg);
+        std::copy(f.bits, f.bits + b->max_slots, b->spread_last.begin());
+        b->spread_tiles = tiles;
+        b->spread_dirty = false;
+    }
     if (any & B_PREDICT) hipLaunchKernelGGL(substep_predict, grid, blk, 0, s, b->A);
     if (any & B_FIT) hipLaunchKernelGGL(fit_rotations, grid, blk, 0, s, b->A, uint32_t(B_FIT), 0);
     if (any & B_ASSEMBLE) {
@@ -1274,7 +1314,8 @@ int32_t launch_tick(vx_batch *b, Flight &f) {
     }
     if (any & B_FINISH) hipLaunchKernelGGL(finish_velocities, grid, blk, 0, s, b->A);
     VX_HIP(hipGetLastError());
-    VX_HIP(hipMemcpyAsync(f.st, b->d_st, sizeof(SlotState) * b->max_slots, hipMemcpyDeviceToHost, s));
+    hipLaunchKernelGGL(publish_state, dim3(1), blk, 0, s, b->d_st, f.st_dev, b->max_slots);
+    VX_HIP(hipGetLastError());
     VX_HIP(hipEventRecord(f.done, s));
     f.active = true;
     return VX_OK;
@@ -1431,6 +1472,7 @@ int32_t create_batch(uint64_t capacity_rows, uint32_t max_scenes, uint32_t chunk
     b->tiles.assign(b->cap_tiles, free_tile);
     b->known.assign(max_scenes, SlotState{});
     b->inflight.assign(max_scenes, 0u);
+    b->spread_last.assign(max_scenes, 0u);
     DevArena &A = b->A;
     const size_t R = size_t(b->cap_tiles) * kBlock;
     A.R = uint32_t(R);
@@ -1494,7 +1536,8 @@ int32_t create_batch(uint64_t capacity_rows, uint32_t max_scenes, uint32_t chunk
               hipStreamSynchronize(ctx.stream) == hipSuccess;
     for (auto &f : b->fl) {
         ok = ok && hipHostMalloc((void **)&f.bits, sizeof(uint32_t) * max_scenes, hipHostMallocDefault) == hipSuccess &&
-             hipHostMalloc((void **)&f.st, sizeof(SlotState) * max_scenes, hipHostMallocDefault) == hipSuccess &&
+             hipHostMalloc((void **)&f.st, sizeof(SlotState) * max_scenes, hipHostMallocCoherent) == hipSuccess &&
+             hipHostGetDevicePointer((void **)&f.st_dev, f.st, 0) == hipSuccess &&
              hipEventCreateWithFlags(&f.done, hipEventDisableTiming) == hipSuccess &&
              hipEventCreate(&f.frac0) == hipSuccess && hipEventCreate(&f.frac1) == hipSuccess;
         if (f.bits) std::memset(f.bits, 0, sizeof(uint32_t) * max_scenes);
@@ -1593,6 +1636,7 @@ int32_t set_tiles(vx_batch *b, uint32_t t0, uint32_t count, const SlotGeo *g) {
     else
         while (hw > 0 && b->tiles[hw - 1].slot == kFreeTile) --hw;
     b->hw_tiles = hw;
+    b->spread_dirty = true;
     return put(b->d_tiles + t0, b->tiles.data() + t0, sizeof(SlotGeo) * count);
 }
 
@@ -2305,7 +2349,8 @@ extern "C" int32_t vx_scene_time_spmv(vx_scene *sc, uint32_t reps, double *out_u
     std::vector<uint32_t> bits(b->max_slots, 0u);
     bits[sc->slot] = B_INNER;
     VX_HIP(hipMemcpy(b->d_bits, bits.data(), sizeof(uint32_t) * b->max_slots, hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(spread_bits, dim3((b->cap_tiles + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx.stream, b->A);
+    hipLaunchKernelGGL(spread_bits, dim3((b->cap_tiles + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx.stream, b->A, BitsArg{});
+    b->spread_dirty = true;
     VX_HIP(hipGetLastError());
     const OneScene one = make_one(g);
     // The launches run back to back inside one graph, as the solve runs them,
