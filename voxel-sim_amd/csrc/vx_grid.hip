@@ -487,11 +487,16 @@ __device__ __forceinline__ bool inner_bits(uint32_t bits) {
 }
 
 // A one-scene arena's inner kernels: the scene's geometry and the order in
-// which workgroups visit its tiles. Tiles are visited by columns of 8 tiles
-// that run through all z: with q tiles per z-plane (rounded), logical index
-// L = ((jb * nzt) + zr) * 8 + j visits tile zr * q + jb * 8 + j, so a tile's
-// +-z neighbour tiles are visited 8 indices before and after it (the same
-// XCD) instead of a whole plane away, and stay in that XCD's L2.
+// which workgroups visit its tiles. Tiles are visited by groups of kZCols
+// tile columns that run through all z: with q tiles per z-plane (rounded),
+// logical index L = ((jb * nzt) + zr) * kZCols + j visits tile
+// zr * q + jb * kZCols + j, so a tile's +-z neighbour tiles are visited
+// kZCols indices before and after it (the same XCD) instead of a whole plane
+// away, and stay in that XCD's L2. kZCols = 0 keeps the plain order.
+#ifndef VX_ZCOLS
+#define VX_ZCOLS 8
+#endif
+constexpr uint32_t kZCols = VX_ZCOLS;
 struct OneScene {
     SlotGeo g;
     uint32_t q, nzt, chunk, grid;
@@ -505,10 +510,10 @@ inline OneScene make_one(const SlotGeo &g) {
     const uint64_t plane = uint64_t(g.nx) * g.ny;
     const uint32_t q = uint32_t(std::max<uint64_t>(1, (plane + kBlock / 2) / kBlock));
     uint64_t space = g.ntiles;
-    if (q >= 16) {
+    if (kZCols && q >= 16) {
         o.q = q;
         o.nzt = (g.ntiles + q - 1) / q;
-        space = uint64_t((q + 7) / 8) * 8 * o.nzt;
+        space = uint64_t((q + kZCols - 1) / kZCols) * kZCols * o.nzt;
     }
     o.chunk = uint32_t(std::max<uint64_t>(1, (space + kXcds - 1) / kXcds));
     o.grid = o.chunk * kXcds;
@@ -516,9 +521,13 @@ inline OneScene make_one(const SlotGeo &g) {
 }
 
 __device__ __forceinline__ uint32_t one_tile(uint32_t L, const OneScene &o) {
-    if (o.q == 0) return L;
-    const uint32_t j = L % 8, r = L / 8, zr = r % o.nzt, col = (r / o.nzt) * 8 + j;
-    return col < o.q ? zr * o.q + col : kFreeTile;
+    if constexpr (kZCols == 0) {
+        return L;
+    } else {
+        if (o.q == 0) return L;
+        const uint32_t j = L % kZCols, r = L / kZCols, zr = r % o.nzt, col = (r / o.nzt) * kZCols + j;
+        return col < o.q ? zr * o.q + col : kFreeTile;
+    }
 }
 
 // The tile of an inner-iteration kernel. ONE: the arena holds a single scene
