@@ -157,9 +157,13 @@ class GridScene:
 
     def step(self, fixed_dt=0.001, gravity=(0.0, 0.0, -9.81), substeps=1, corotation_iterations=2,
              corotation_tolerance=1e-5, solve=None, fracture=None) -> dict:
-        sp = step_params(fixed_dt, gravity, substeps, corotation_iterations, corotation_tolerance, solve, fracture)
+        return self.step_with(step_params(fixed_dt, gravity, substeps, corotation_iterations, corotation_tolerance,
+                                          solve, fracture))
+
+    def step_with(self, params: StepParams) -> dict:
+        """One fixed step with a prepared vx_step_params."""
         rep = StepReport()
-        check(lib().vx_scene_step(self.handle, ctypes.byref(sp), ctypes.byref(rep)))
+        check(lib().vx_scene_step(self.handle, ctypes.byref(params), ctypes.byref(rep)))
         return _report(rep)
 
     # -- data -----------------------------------------------------------------
