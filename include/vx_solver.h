@@ -273,7 +273,7 @@ int32_t vx_scene_download_bonds(vx_scene *scene, uint32_t *neighbor, float *dama
                                 float *history);
 
 /* Inner PCG iterations per scheduling tick (one hipGraph replay; default 16,
- * at most 256). 0 keeps the current value. */
+ * even, at most 256). 0 keeps the current value. */
 int32_t vx_scene_set_schedule(vx_scene *scene, uint32_t iterations_per_tick);
 
 /* ---- batched and served scenes (build extensions) ----------------------- */
@@ -315,7 +315,8 @@ uint64_t vx_batch_bytes_per_row(void);
 int32_t vx_batch_create(uint64_t capacity_rows, uint32_t max_scenes, uint32_t xcd_chunk_tiles, vx_batch **out);
 int32_t vx_batch_destroy(vx_batch *batch);
 /* Schedule of an arena: tiles per XCD chunk of the workgroup-to-tile
- * interleave and inner iterations per tick; 0 keeps a value. */
+ * interleave and inner iterations per tick (even, at most 256); 0 keeps a
+ * value. */
 int32_t vx_batch_set_schedule(vx_batch *batch, uint32_t xcd_chunk_tiles, uint32_t iterations_per_tick);
 /* Largest contiguous free range and total free space, in rows. */
 int32_t vx_batch_free_rows(vx_batch *batch, uint64_t *largest, uint64_t *total);

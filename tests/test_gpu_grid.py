@@ -170,3 +170,24 @@ def test_large_scene_pre_summed_reductions(gpu, O, scenes):
     assert np.linalg.norm(r) / np.linalg.norm(rows_o["rhs"]) <= 1e-9 * 1.01
     assert rep["relative_residual"] <= 1e-9 and rep["inner_iterations"] > 0
     assert again["inner_iterations"] == 0 and again["outer_sweeps"] == 1
+
+
+def test_schedule_validation(gpu, scenes):
+    """Iterations per tick must be even (each chunk graph starts on p buffer 0)
+    and at most 256; every accepted schedule converges to the tolerance."""
+    from vxsim import grid
+    from vxsim.solver import SolverError
+    s = scenes.large_64cube(dims=(12, 10, 9))
+    cfg = s.config
+    with grid.GridScene(s, tuned=False) as gs:
+        for bad in (7, 258):
+            with pytest.raises(SolverError):
+                gs.set_schedule(bad)
+        for iters in (2, 8, 32):
+            gs.set_schedule(iters)
+            h = cfg.fixed_dt / cfg.substeps
+            gs.predict(h, cfg.gravity)
+            gs.fit_rotations()
+            gs.assemble(h)
+            r = gs.solve(tolerance=1e-9)
+            assert r["relative_residual"] <= 1e-9, iters

@@ -1853,6 +1853,9 @@ static void reset_graphs(vx_batch *b) {
 
 static int32_t set_schedule(vx_batch *b, uint32_t chunk, uint32_t iters) {
     if (iters > 256) return fail(VX_ERR_INVALID_ARGUMENT, "at most 256 iterations per tick");
+    // Each chunk graph alternates the p buffers starting from buffer 0, so a
+    // chunk must end on the buffer the next one starts from.
+    if (iters & 1) return fail(VX_ERR_INVALID_ARGUMENT, "iterations per tick must be even");
     if (int32_t c = drain(b)) return c;
     if (chunk) {
         b->chunk = chunk;
