@@ -91,6 +91,16 @@ def test_pulled_column_matches_oracle(gpu, O, scenes, material, pull):
         assert broken == 0 and hist.equivalent_plastic_strain.max() > 0.0
 
 
+def test_fracture_every_substep_matches_oracle(gpu, O, scenes):
+    """Four substeps per step (the reference's default SceneConfig): the pass
+    runs after each substep's corotation passes, with the substep length as
+    its time step; later substeps assemble without the bonds broken before."""
+    s = scenes.pulled_mixed((10, 10, 8), 9, 3e-4)
+    s.config.substeps = 4
+    broken, hist = check_parity(O, s, steps=2)
+    assert broken > 50
+
+
 def test_pulled_mixed_box_matches_oracle(gpu, O, scenes):
     """Brittle breaks, cross-grain wood breaks and iron / along-grain wood
     plastic flow in one 12^3 box, three steps."""
