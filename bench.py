@@ -390,12 +390,28 @@ def main():
         f64_rate = 3.0 * active * (r64["inner_iterations"] + r64["outer_sweeps"]) / t64
         rows = gs.export_rows()
         inv = np.linalg.inv(rows["diagonal"].reshape(-1, 3, 3)).reshape(-1, 9)
-        args64 = (rows["rhs"], rows["neighbors"], rows["diagonal"], rows["blocks"], inv)
-        solver.solve_pcg_f64(*args64, 20000, 1e-6)  # warm the buffers and graph
+        # The reference's row arrays as the caller holds them (f32, u32), built
+        # before the timed calls so that only the library call is timed.
+        row_args = (np.ascontiguousarray(rows["rhs"], np.float32), np.ascontiguousarray(rows["neighbors"], np.uint32),
+                np.ascontiguousarray(rows["diagonal"], np.float32), np.ascontiguousarray(rows["blocks"], np.float32),
+                np.ascontiguousarray(inv, np.float32))
+        solver.solve_pcg_f64(*row_args, 20000, 1e-6)  # warm the buffers and graph
         t = time.perf_counter()
-        _, it_drop, _ = solver.solve_pcg_f64(*args64, 20000, 1e-6)
+        _, it_drop, _ = solver.solve_pcg_f64(*row_args, 20000, 1e-6)
         t_drop = time.perf_counter() - t
         drop_rate = 3.0 * active * it_drop / t_drop
+        # The drop-in itself: the reference's f32 block-Jacobi PCG semantics
+        # (src/cuda_solver.rs:366-500) at its default tolerance.
+        try:
+            solver.solve_pcg(*row_args, 20000, 1e-5)
+            t = time.perf_counter()
+            _, it32, res32 = solver.solve_pcg(*row_args, 20000, 1e-5)
+            t32 = time.perf_counter() - t
+            drop32 = {"dof_iters_per_s": 3.0 * active * it32 / t32, "ms": 1000 * t32, "iters": it32,
+                      "relative_residual": res32, "tolerance": 1e-5,
+                      "note": "vx_solve_pcg, the reference's f32 PCG semantics; host buffers, PCIe upload included"}
+        except solver.SolverError as e:
+            drop32 = {"status": e.status, "error": str(e), "tolerance": 1e-5}
         extra = {
             "mixed_solve": {"dof_iters_per_s": mixed_rate, "ms": 1000 * tm, "inner": rm["inner_iterations"],
                             "sweeps": rm["outer_sweeps"], "rel_residual": rm["relative_residual"]},
@@ -403,6 +419,7 @@ def main():
                                       "rel_residual": r64["relative_residual"]},
             "plain_fp64_dropin_rows": {"dof_iters_per_s": drop_rate, "ms": 1000 * t_drop, "iters": it_drop,
                                        "tolerance": 1e-6, "note": "vx_solve_pcg_f64, host buffers, PCIe upload included"},
+            "dropin_rows_f32": drop32,
             "mixed_vs_plain_fp64_structured": mixed_rate / f64_rate,
             "mixed_vs_plain_fp64_dropin": mixed_rate / drop_rate,
             "time_to_solution_vs_fp64_structured": t64 / tm,

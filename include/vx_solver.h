@@ -199,8 +199,9 @@ int32_t vx_scene_download(vx_scene *scene, double *positions, double *rotations,
  * voxel index of each active row. Any pointer may be NULL. */
 int32_t vx_scene_export_rows(vx_scene *scene, uint32_t *active, uint32_t *neighbors, double *blocks,
                              double *diagonal, double *rhs, uint32_t *out_active_count);
-/* Mean duration (microseconds) of the FP32 structured bond SpMV kernel over
- * `reps` launches on the library stream, each bracketed by HIP events. */
+/* Mean duration (microseconds) of the FP32 structured bond SpMV kernel in a
+ * graph of `reps` back-to-back launches on the library stream, timed with HIP
+ * events; the median of five replays of that graph. */
 int32_t vx_scene_time_spmv(vx_scene *scene, uint32_t reps, double *out_us);
 
 /* One fixed step of the solve part of step_with_axis_targets_and_loads

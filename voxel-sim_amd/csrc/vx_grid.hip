@@ -2376,21 +2376,23 @@ extern "C" int32_t vx_scene_time_spmv(vx_scene *sc, uint32_t reps, double *out_u
     hipError_t ge = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     if (ge != hipSuccess) return fail(VX_ERR_DEVICE, "hipGraphInstantiate failed: %s", hipGetErrorString(ge));
-    hipEvent_t ev[2];
-    VX_HIP(hipEventCreate(&ev[0]));
-    VX_HIP(hipEventCreate(&ev[1]));
+    // Five timed replays after a warm-up one; the median replay is reported.
+    constexpr int kReplays = 5;
+    hipEvent_t ev[kReplays + 1];
+    for (auto &e : ev) VX_HIP(hipEventCreate(&e));
     VX_HIP(hipGraphLaunch(exec, ctx.stream));  // warm-up
     VX_HIP(hipEventRecord(ev[0], ctx.stream));
-    VX_HIP(hipGraphLaunch(exec, ctx.stream));
-    VX_HIP(hipEventRecord(ev[1], ctx.stream));
+    for (int i = 0; i < kReplays; ++i) {
+        VX_HIP(hipGraphLaunch(exec, ctx.stream));
+        VX_HIP(hipEventRecord(ev[i + 1], ctx.stream));
+    }
     VX_HIP(hipStreamSynchronize(ctx.stream));
-    float ms = 0.0f;
-    VX_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
-    const double total = ms;
-    (void)hipEventDestroy(ev[0]);
-    (void)hipEventDestroy(ev[1]);
+    float ms[kReplays];
+    for (int i = 0; i < kReplays; ++i) VX_HIP(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
+    std::sort(ms, ms + kReplays);
+    for (auto &e : ev) (void)hipEventDestroy(e);
     (void)hipGraphExecDestroy(exec);
-    *out_us = 1000.0 * total / reps;
+    *out_us = 1000.0 * double(ms[kReplays / 2]) / reps;
     // Restore the scene's state word (solution buffer parity) and parameters.
     st = b->known[sc->slot];
     VX_HIP(hipMemcpy(b->d_st + sc->slot, &st, sizeof st, hipMemcpyHostToDevice));
