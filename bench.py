@@ -218,7 +218,28 @@ def large_bench(steps: int, warmup: int, compare: bool = True):
             t = time.perf_counter()
             r64 = gs.solve(tolerance=1e-6, max_inner=40000, mode=grid.F64)
             t64 = time.perf_counter() - t
-            extra = {"mixed_solve": {"ms": 1000 * tm, "inner": rm["inner_iterations"], "sweeps": rm["outer_sweeps"],
+            # The drop-in over the reference row packing at this size (host
+            # buffers and PCIe upload included), the reference's f32 semantics.
+            rows = gs.export_rows()
+            inv = np.linalg.inv(rows["diagonal"].reshape(-1, 3, 3)).reshape(-1, 9)
+            row_args = (np.ascontiguousarray(rows["rhs"], np.float32),
+                        np.ascontiguousarray(rows["neighbors"], np.uint32),
+                        np.ascontiguousarray(rows["diagonal"], np.float32),
+                        np.ascontiguousarray(rows["blocks"], np.float32), np.ascontiguousarray(inv, np.float32))
+            del rows, inv
+            from vxsim import solver
+            try:
+                solver.solve_pcg(*row_args, 20000, 1e-5)
+                t = time.perf_counter()
+                _, it32, res32 = solver.solve_pcg(*row_args, 20000, 1e-5)
+                t32 = time.perf_counter() - t
+                drop32 = {"ms": 1000 * t32, "iters": it32, "relative_residual": res32, "tolerance": 1e-5,
+                          "dof_iters_per_s": 3.0 * active * it32 / t32}
+            except solver.SolverError as e:
+                drop32 = {"status": e.status, "error": str(e)}
+            del row_args
+            extra = {"dropin_rows_f32": drop32,
+                     "mixed_solve": {"ms": 1000 * tm, "inner": rm["inner_iterations"], "sweeps": rm["outer_sweeps"],
                                      "dof_iters_per_s": 3.0 * active * (rm["inner_iterations"] + rm["outer_sweeps"]) / tm},
                      "plain_fp64_structured": {"ms": 1000 * t64, "iters": r64["inner_iterations"],
                                                "dof_iters_per_s": 3.0 * active * (r64["inner_iterations"] +

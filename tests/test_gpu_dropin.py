@@ -113,3 +113,39 @@ def test_repeated_and_resized_calls_are_reproducible(gpu, c1, mixed24):
     d = call(solver, mixed24, 4000, 1e-6)
     assert np.array_equal(a[0], c[0]) and a[1] == c[1]
     assert np.array_equal(b[0], d[0]) and b[1] == d[1]
+
+
+def rows_matvec(rows, x):
+    """y = A x over the reference row packing (diagonal block + six slots), FP64."""
+    n = rows["rhs"].shape[0]
+    x = np.asarray(x, np.float64).reshape(n, 3)
+    y = np.einsum("nrc,nc->nr", rows["diagonal"].reshape(n, 3, 3).astype(np.float64), x)
+    nb = rows["neighbors"].reshape(n, 6)
+    blk = rows["blocks"].reshape(n, 6, 3, 3).astype(np.float64)
+    for s in range(6):
+        j = nb[:, s]
+        ok = j != EMPTY
+        y[ok] += np.einsum("nrc,nc->nr", blk[ok, s], x[j[ok]])
+    return y
+
+
+def test_large_grid_pre_summed_partials(gpu, O, scenes):
+    """More than 1,024 workgroups (580,608 rows): the partials of each kernel
+    are pre-summed once instead of re-summed by every workgroup. The solve
+    stops on a 16-iteration boundary with the residual it reports, that
+    residual is the FP64 true residual of the correction it returns, and the
+    correction agrees with the FP64 drop-in's solution of the same rows."""
+    from vxsim import solver
+    rows = rows_for(O, scenes.large_64cube(dims=(96, 96, 64)), O.F32)
+    f32rows = {k: (v.astype(np.float32) if v.dtype == np.float64 else v) for k, v in rows.items()}
+    n = f32rows["rhs"].shape[0]
+    assert -(-n // 512) > 1024
+    tol = 1e-4
+    dx, it, rel = call(solver, f32rows, 4000, tol)
+    assert it % 16 == 0 and rel <= tol
+    b = f32rows["rhs"].astype(np.float64)
+    true_rel = np.linalg.norm(b - rows_matvec(f32rows, dx)) / np.linalg.norm(b)
+    assert true_rel == pytest.approx(rel, rel=0.1)
+    x64, it64, rel64 = call(solver, f32rows, 20000, 1e-9, f64=True)
+    assert rel64 <= 1e-9
+    assert np.linalg.norm(dx - x64) / np.linalg.norm(x64) <= 50 * tol

@@ -41,9 +41,40 @@ struct RowsState {
     uint32_t k_update;  // iteration of the current direction (written by rows_spmv)
     uint32_t iters, done, max_iter, check_every;
     uint32_t invalid, pad;
+    double red[3];      // pre-summed partials of a big grid (rows_reduce), by slot
 };
 enum : uint32_t { kRunning = 0, kConverged = 1, kNonFinite = 2, kCapReached = 3 };
 enum : int { kPap = 0, kRz = 1, kRr = 2 };  // partial slots
+
+// Grids of more workgroups than this have their partials summed once per
+// kernel by rows_reduce instead of by every workgroup of the consumer kernel,
+// whose cost grows with the square of the grid. Both sum in the same order
+// (thread-strided, then the fixed workgroup sum), so the scalars are the same
+// bits either way.
+constexpr uint32_t kRowsBig = 1024;
+
+template <int N>
+__device__ __forceinline__ void rows_sums(double (&out)[N], const double *partials, const int (&slot)[N], uint32_t nb,
+                                          const RowsState *st, double (*lds)[kWaves]) {
+    if (nb > kRowsBig) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) out[k] = st->red[slot[k]];
+        return;
+    }
+    sum_partials<N>(out, partials, slot, nb, nb, lds);
+}
+
+// One workgroup: slots [k0, k0 + nk) of a big grid's partials into red[].
+__global__ __launch_bounds__(kBlock) void rows_reduce(const double *partials, uint32_t nb, int k0, int nk,
+                                                      RowsState *st) {
+    __shared__ double lds[2][kWaves];
+    double s[2] = {0.0, 0.0};
+    for (int k = 0; k < nk; ++k)
+        for (uint32_t b = threadIdx.x; b < nb; b += kBlock) s[k] += partials[size_t(k0 + k) * nb + b];
+    block_sum_all<2>(s, lds);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < nk; ++k) st->red[k0 + k] = s[k];
+}
 
 template <class S> __device__ __forceinline__ S fma_s(S a, S b, S c);
 template <> __device__ __forceinline__ float fma_s<float>(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
@@ -179,7 +210,7 @@ __global__ __launch_bounds__(kBlock) void rows_spmv(uint32_t n, uint32_t per_xcd
     const uint32_t nb = gridDim.x;
     double sums[2];
     const int in_slots[2] = {kRz, kRr};
-    sum_partials<2>(sums, partials, in_slots, nb, nb, lds);
+    rows_sums<2>(sums, partials, in_slots, nb, st, lds);
     if (done_in) return;
     const size_t N = n;
     const uint32_t tile = xcd_tile(blockIdx.x, per_xcd);
@@ -291,7 +322,7 @@ __global__ __launch_bounds__(kBlock) void rows_update(uint32_t n, uint32_t per_x
     const uint32_t nb = gridDim.x;
     double pap[1];
     const int in_slots[1] = {kPap};
-    sum_partials<1>(pap, partials, in_slots, nb, nb, lds);
+    rows_sums<1>(pap, partials, in_slots, nb, st, lds);
     if (done_in) return;
     const size_t N = n;
     const uint32_t tile = xcd_tile(blockIdx.x, per_xcd);
@@ -450,6 +481,8 @@ int32_t solve_rows(const float *rhs, const uint32_t *neighbors, const float *dia
     VX_HIP(hipGetLastError());
     if (max_iterations > 0) {
         hipLaunchKernelGGL(rows_init<S>, dim3(grid), dim3(kBlock), 0, stream, n, per_xcd, ntiles, inv, r, z, part);
+        const bool big = grid > kRowsBig;
+        if (big) hipLaunchKernelGGL(rows_reduce, dim3(1), dim3(kBlock), 0, stream, part, grid, int(kRz), 2, st);
         VX_HIP(hipGetLastError());
 
         RowsGraph &g = g_graph[kF64 ? 1 : 0];
@@ -467,8 +500,10 @@ int32_t solve_rows(const float *rhs, const uint32_t *neighbors, const float *dia
                 S *pnew = (t % 2 == 0) ? p1 : p0;
                 hipLaunchKernelGGL(rows_spmv<S>, dim3(grid), dim3(kBlock), 0, stream, n, per_xcd, ntiles, nbr, blk,
                                    diag, z, pold, pnew, ap, st, part);
+                if (big) hipLaunchKernelGGL(rows_reduce, dim3(1), dim3(kBlock), 0, stream, part, grid, int(kPap), 1, st);
                 hipLaunchKernelGGL(rows_update<S>, dim3(grid), dim3(kBlock), 0, stream, n, per_xcd, ntiles, inv, dx,
                                    r, z, pnew, ap, st, part);
+                if (big) hipLaunchKernelGGL(rows_reduce, dim3(1), dim3(kBlock), 0, stream, part, grid, int(kRz), 2, st);
             }
             VX_HIP(hipStreamEndCapture(stream, &graph));
             hipError_t e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
