@@ -151,9 +151,18 @@ def serve_bench(rate: float, seconds: float, rev_lo: float, rev_hi: float, steps
     reqs = serve.request_stream(rate, seconds, rev_lo, rev_hi, steps, seed=4)
     gen_s = time.perf_counter() - t
     budget = int(budget_gb * (1 << 30))
+    last = [time.perf_counter()]
+
+    def progress(server):
+        now = time.perf_counter()
+        if now - last[0] >= 10.0:
+            last[0] = now
+            print("serve: %d ticks, %d of %d requests finished" %
+                  (server.ticks, sum(1 for r in reqs if r.finished is not None), len(reqs)), file=sys.stderr, flush=True)
+
     with serve.Server.for_budget(budget, max_scenes=256) as S:
         t0 = time.perf_counter()
-        S.run(reqs)
+        S.run(reqs, on_tick=progress)
         wall = time.perf_counter() - t0
         ticks = S.ticks
     done = [r for r in reqs if r.state == batch.DONE]
