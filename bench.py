@@ -91,6 +91,14 @@ def cpu_baseline(scene, budget_s: float = 12.0):
     if "one" in results:
         it1, sec1, _ = results["one"]
         out["single_thread_value"] = 3.0 * sy.na * it1 / sec1
+    # The reference's CPU solve as configured by default (src/scene.rs:16-24,
+    # src/solver.rs:1390-1496): block IC(0), the rayon cutoff of 32,768 rows,
+    # to its default tolerance 1e-5; and block-Jacobi with the same settings.
+    for label, pc in (("ic0", O.IC0), ("block_jacobi", O.BJ)):
+        _, it_s, res_s, sec_s = sy.solve_cpu(pc, 4000, 1e-5, threads, 32768)
+        out[label + "_to_1e-5"] = {"iterations": it_s, "seconds": sec_s, "relative_residual": res_s,
+                                   "dof_iters_per_s": 3.0 * sy.na * it_s / sec_s, "threads": threads,
+                                   "parallel_cutoff": 32768}
     return out
 
 
@@ -416,6 +424,11 @@ def main():
         t = time.perf_counter()
         r64 = gs.solve(tolerance=SOLVE_TOL, max_inner=20000, mode=grid.F64)
         t64 = time.perf_counter() - t
+        # The same system to the reference's default tolerance (beside the
+        # CPU solves to 1e-5 in cpu_baseline).
+        t = time.perf_counter()
+        r5 = gs.solve(tolerance=1e-5, inner_tolerance=INNER_TOL, max_inner=8000, max_outer=12)
+        t5 = time.perf_counter() - t
         mixed_rate = 3.0 * active * (rm["inner_iterations"] + rm["outer_sweeps"]) / tm
         f64_rate = 3.0 * active * (r64["inner_iterations"] + r64["outer_sweeps"]) / t64
         rows = gs.export_rows()
@@ -450,6 +463,8 @@ def main():
             "plain_fp64_dropin_rows": {"dof_iters_per_s": drop_rate, "ms": 1000 * t_drop, "iters": it_drop,
                                        "tolerance": 1e-6, "note": "vx_solve_pcg_f64, host buffers, PCIe upload included"},
             "dropin_rows_f32": drop32,
+            "mixed_solve_to_1e-5": {"ms": 1000 * t5, "inner": r5["inner_iterations"], "sweeps": r5["outer_sweeps"],
+                                    "rel_residual": r5["relative_residual"]},
             "mixed_vs_plain_fp64_structured": mixed_rate / f64_rate,
             "mixed_vs_plain_fp64_dropin": mixed_rate / drop_rate,
             "time_to_solution_vs_fp64_structured": t64 / tm,
