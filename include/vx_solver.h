@@ -160,10 +160,25 @@ typedef struct vx_solve_params {
     uint32_t max_outer;        /* FP64 residual replacements cap */
     uint32_t warm_start;       /* 1: start from the scene's last correction */
     uint32_t mode;             /* VX_SOLVE_MIXED or VX_SOLVE_F64 */
+    uint32_t preconditioner;   /* VX_PRECOND_BLOCK_JACOBI or VX_PRECOND_RED_BLACK */
+    uint32_t reserved;         /* 0 */
 } vx_solve_params;
 
 #define VX_SOLVE_MIXED 0 /* FP32 PCG with FP64 residual replacement (reliable updates) */
 #define VX_SOLVE_F64 1   /* plain FP64 PCG on the structured layout */
+
+/* Preconditioners of the structured solve. Block-Jacobi is the reference's
+ * device preconditioner (src/cuda_solver.rs:366-500, src/cuda/pcg.ptx:104-155).
+ * Red-black: CG on the black rows' reduced system S = D_bb - A_br D_rr^-1 A_rb
+ * (REVs coloured by the parity of x + y + z) with block-Jacobi, which is CG
+ * preconditioned by red-black block symmetric Gauss-Seidel on the full system;
+ * the red rows are recovered exactly, and the tolerance applies to the full
+ * system's FP64 residual. It has no counterpart in the reference (a stronger
+ * device preconditioner than block-Jacobi, cf. the CPU block IC(0) of
+ * src/solver.rs:1390-1496). Iterations it reports are reduced-system
+ * iterations, each about 1.5 block-Jacobi iterations of work. */
+#define VX_PRECOND_BLOCK_JACOBI 0
+#define VX_PRECOND_RED_BLACK 1
 
 typedef struct vx_solve_report {
     uint32_t inner_iterations; /* inner PCG iterations of the solve */

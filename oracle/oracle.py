@@ -113,6 +113,9 @@ def lib():
                                                  ctypes.c_uint32, ctypes.c_double, ctypes.c_uint32, _dp, _u32p, _dp]
         L.vxo_solve_exact.restype = ctypes.c_double
         L.vxo_solve_exact.argtypes = [ctypes.c_uint32, _dp, _u32p, _dp, _dp, ctypes.c_double, ctypes.c_uint32, _dp, _u32p]
+        L.vxo_solve_red_black.restype = ctypes.c_double
+        L.vxo_solve_red_black.argtypes = [ctypes.c_uint32, _dp, _u32p, _dp, _dp, _u8p, ctypes.c_double, ctypes.c_double,
+                                          ctypes.c_uint32, _dp, _u32p, _dp]
         L.vxo_matvec.argtypes = [ctypes.c_int, ctypes.c_uint32, _u32p, _dp, _dp, _dp, _dp]
         L.vxo_step.argtypes = [ctypes.c_int, ctypes.POINTER(_Scene), ctypes.POINTER(_StepConfig), _dp, _dp,
                                ctypes.POINTER(_StepReport), _dp]
@@ -296,6 +299,30 @@ def solve_exact(rows, tolerance=1e-13, max_iterations=200000):
                                 _p(_f64(rows["blocks"]), _dp), tolerance, max_iterations, _p(x, _dp),
                                 ctypes.byref(it))
     return x, int(it.value), float(rel)
+
+
+def black_rows(active, dims):
+    """1 for the rows whose REV has odd x + y + z (the black colour of the
+    red-black ordering), over the active REV indices of a box."""
+    nx, ny, _ = dims
+    a = np.asarray(active, np.int64)
+    return (((a % nx) + (a // nx) % ny + a // (nx * ny)) & 1).astype(np.uint8)
+
+
+def solve_red_black(rows, black, tolerance, inner_tolerance=0.0, max_iterations=100000, history=False):
+    """FP64 CG on the red-black reduced system (vxo_solve_red_black):
+    (x, iterations, full true relative residual, ||r|| history or None)."""
+    n = rows["rhs"].shape[0]
+    x = np.zeros((n, 3))
+    it = ctypes.c_uint32()
+    nbr = np.ascontiguousarray(rows["neighbors"], np.uint32)
+    blk = np.ascontiguousarray(black, np.uint8)
+    hist = np.zeros(max_iterations + 1) if history else None
+    rel = lib().vxo_solve_red_black(n, _p(_f64(rows["rhs"]), _dp), _p(nbr, _u32p), _p(_f64(rows["diagonal"]), _dp),
+                                    _p(_f64(rows["blocks"]), _dp), _p(blk, _u8p), tolerance, inner_tolerance,
+                                    max_iterations, _p(x, _dp), ctypes.byref(it), _p(hist, _dp))
+    k = int(it.value)
+    return x, k, float(rel), (hist[:k + 1] if history else None)
 
 
 def matvec(rows, x, mode=F64):

@@ -1271,6 +1271,118 @@ double vxo_solve_exact(uint32_t n, const double *rhs, const uint32_t *nbr, const
     return rel;
 }
 
+// CG on the red-black reduced system, FP64, over the reference packing.
+// Rows are coloured (black[i] = 1 for odd x + y + z); every bond joins a red
+// and a black row, so A = [D_rr A_rb; A_br D_bb] with block-diagonal D_rr,
+// D_bb. Eliminating the red rows: S x_b = b_b - A_br D_rr^-1 b_r with
+// S = D_bb - A_br D_rr^-1 A_rb, solved by CG with block-Jacobi (D_bb^-1);
+// then x_r = D_rr^-1 (b_r - A_rb x_b). The loop checks, before each
+// iteration, r.r <= tol^2 b.b (b the full right-hand side) or
+// r.r <= inner_tol^2 r0.r0 (r0 the reduced residual at x_b = 0). Returns the
+// full system's true relative residual of the returned x. history (nullable,
+// max_iterations + 1 entries): ||r|| of the reduced system before each
+// iteration. No counterpart in the reference (its device preconditioner is
+// block-Jacobi, src/cuda_solver.rs:366-500); the algorithm is textbook
+// red-black reduction (equivalently CG preconditioned by red-black block
+// symmetric Gauss-Seidel in Eisenstat's form).
+double vxo_solve_red_black(uint32_t n, const double *rhs, const uint32_t *nbr, const double *diagonal,
+                           const double *blocks, const uint8_t *black, double tolerance, double inner_tolerance,
+                           uint32_t max_iterations, double *x, uint32_t *iterations_out, double *history) {
+    const size_t m = 3 * size_t(n);
+    std::vector<double> inv(9 * size_t(n));
+    for (uint32_t i = 0; i < n; ++i) inverse_row_major(diagonal + 9 * size_t(i), inv.data() + 9 * size_t(i));
+    auto mul3 = [](const double *a, const double *v, double *out) {
+        for (int r = 0; r < 3; ++r) out[r] = a[3 * r] * v[0] + a[3 * r + 1] * v[1] + a[3 * r + 2] * v[2];
+    };
+    // out_i = sum_e B_ie v_j over rows of one colour (the off-diagonal part).
+    auto offdiag = [&](const std::vector<double> &v, std::vector<double> &out, uint8_t colour) {
+        for (uint32_t i = 0; i < n; ++i) {
+            if (black[i] != colour) continue;
+            double acc[3] = {0, 0, 0};
+            for (int e = 0; e < 6; ++e) {
+                const uint32_t j = nbr[6 * size_t(i) + e];
+                if (j == kEmpty) continue;
+                double t[3];
+                mul3(blocks + 54 * size_t(i) + 9 * e, v.data() + 3 * size_t(j), t);
+                for (int r = 0; r < 3; ++r) acc[r] += t[r];
+            }
+            for (int r = 0; r < 3; ++r) out[3 * size_t(i) + r] = acc[r];
+        }
+    };
+    // out_i = M_i v_i over rows of one colour.
+    auto blockwise = [&](const double *mats, const std::vector<double> &v, std::vector<double> &out, uint8_t colour) {
+        for (uint32_t i = 0; i < n; ++i)
+            if (black[i] == colour) mul3(mats + 9 * size_t(i), v.data() + 3 * size_t(i), out.data() + 3 * size_t(i));
+    };
+    auto dotb = [&](const std::vector<double> &a, const std::vector<double> &b) {
+        double s = 0.0;
+        for (uint32_t i = 0; i < n; ++i)
+            if (black[i])
+                for (int r = 0; r < 3; ++r) s += a[3 * size_t(i) + r] * b[3 * size_t(i) + r];
+        return s;
+    };
+    std::vector<double> b(rhs, rhs + m), t(m, 0.0), u(m, 0.0), xs(m, 0.0);
+    double bb = 0.0;
+    for (size_t k = 0; k < m; ++k) bb += b[k] * b[k];
+    // S p on the black rows: D_bb p - A_br D_rr^-1 A_rb p.
+    auto apply_s = [&](const std::vector<double> &p, std::vector<double> &out) {
+        offdiag(p, t, 0);
+        blockwise(inv.data(), t, u, 0);
+        offdiag(u, t, 1);
+        blockwise(diagonal, p, out, 1);
+        for (uint32_t i = 0; i < n; ++i)
+            if (black[i])
+                for (int r = 0; r < 3; ++r) out[3 * size_t(i) + r] -= t[3 * size_t(i) + r];
+    };
+    // reduced right-hand side b_b - A_br D_rr^-1 b_r
+    std::vector<double> r(m, 0.0), z(m, 0.0), p(m, 0.0), ap(m, 0.0);
+    blockwise(inv.data(), b, u, 0);
+    offdiag(u, t, 1);
+    for (uint32_t i = 0; i < n; ++i)
+        if (black[i])
+            for (int q = 0; q < 3; ++q) r[3 * size_t(i) + q] = b[3 * size_t(i) + q] - t[3 * size_t(i) + q];
+    const double rr0 = dotb(r, r);
+    blockwise(inv.data(), r, z, 1);
+    double rz = dotb(r, z);
+    p = z;
+    uint32_t k = 0;
+    for (;; ++k) {
+        const double rr = dotb(r, r);
+        if (history) history[k] = std::sqrt(rr);
+        if (!(rr > tolerance * tolerance * bb) || !(rr > inner_tolerance * inner_tolerance * rr0) || k >= max_iterations)
+            break;
+        if (k > 0) {
+            blockwise(inv.data(), r, z, 1);
+            const double rz2 = dotb(r, z);
+            const double beta = rz2 / rz;
+            rz = rz2;
+            for (uint32_t i = 0; i < n; ++i)
+                if (black[i])
+                    for (int q = 0; q < 3; ++q) p[3 * size_t(i) + q] = z[3 * size_t(i) + q] + beta * p[3 * size_t(i) + q];
+        }
+        apply_s(p, ap);
+        const double alpha = rz / dotb(p, ap);
+        for (uint32_t i = 0; i < n; ++i)
+            if (black[i])
+                for (int q = 0; q < 3; ++q) {
+                    xs[3 * size_t(i) + q] += alpha * p[3 * size_t(i) + q];
+                    r[3 * size_t(i) + q] -= alpha * ap[3 * size_t(i) + q];
+                }
+    }
+    // x_r = D_rr^-1 (b_r - A_rb x_b)
+    offdiag(xs, t, 0);
+    for (uint32_t i = 0; i < n; ++i)
+        if (!black[i])
+            for (int q = 0; q < 3; ++q) u[3 * size_t(i) + q] = b[3 * size_t(i) + q] - t[3 * size_t(i) + q];
+    blockwise(inv.data(), u, xs, 0);
+    for (size_t q = 0; q < m; ++q) x[q] = xs[q];
+    if (iterations_out) *iterations_out = k;
+    matvec_packed<double>(n, nbr, diagonal, blocks, x, ap.data());
+    double s2 = 0.0;
+    for (size_t q = 0; q < m; ++q) s2 += (rhs[q] - ap[q]) * (rhs[q] - ap[q]);
+    return bb > 0.0 ? std::sqrt(s2 / bb) : 0.0;
+}
+
 double vxo_dot_rows(uint32_t n, const double *a, const double *b) {
     std::vector<float> x(a, a + 3 * size_t(n)), y(b, b + 3 * size_t(n));
     return dot_rows_f64<float>(n, x.data(), y.data());

@@ -154,6 +154,17 @@ double vxo_solve_exact(uint32_t n, const double *rhs, const uint32_t *neighbors,
                        const double *diagonal, const double *blocks, double tolerance,
                        uint32_t max_iterations, double *x, uint32_t *iterations);
 
+/* FP64 CG on the red-black reduced system S x_b = b_b - A_br D_rr^-1 b_r
+ * (black[i] = 1 marks the black rows, odd x + y + z) with block-Jacobi on
+ * D_bb, then x_r = D_rr^-1 (b_r - A_rb x_b). Stops before an iteration when
+ * ||r|| <= tolerance ||b|| or ||r|| <= inner_tolerance ||r0|| (or after
+ * max_iterations). Returns the full system's true relative residual; history
+ * (nullable, max_iterations + 1 entries) receives ||r|| before each
+ * iteration. */
+double vxo_solve_red_black(uint32_t n, const double *rhs, const uint32_t *neighbors, const double *diagonal,
+                           const double *blocks, const uint8_t *black, double tolerance, double inner_tolerance,
+                           uint32_t max_iterations, double *x, uint32_t *iterations, double *history);
+
 /* y = A x over the reference packing, in the given mode (PTX fma order in
  * VXO_F32). */
 void vxo_matvec(int mode, uint32_t n, const uint32_t *neighbors, const double *diagonal,

@@ -64,6 +64,7 @@ enum : uint32_t {
     B_WARM_SAVE = 1u << 10,
     B_F64 = 1u << 11,     // inner solve in FP64 (plain FP64 mode)
     B_FRACTURE = 1u << 12,  // fracture pass of the substep, before the velocities
+    B_RB = 1u << 13,        // red-black reduced system (red-black block SGS) instead of block-Jacobi
 };
 
 // Partial kinds; kFrac..kFrac+5 are the fracture pass's dissipated energies
@@ -500,6 +501,7 @@ constexpr uint32_t kZCols = VX_ZCOLS;
 struct OneScene {
     SlotGeo g;
     uint32_t q, nzt, chunk, grid;
+    uint32_t rb;  // the scene's bits carry B_RB
 };
 
 // Planes of fewer than 16 tiles keep the plain order (q = 0): their z
@@ -540,11 +542,276 @@ __device__ __forceinline__ bool inner_tile(const DevArena &A, const OneScene &on
     if (c.t >= one.g.ntiles) return false;
     c.g = one.g;
     c.slot = one.g.slot;
-    c.bits = B_INNER | Inner<S>::kBit;
+    c.bits = B_INNER | Inner<S>::kBit | (one.rb ? uint32_t(B_RB) : 0u);
     c.local = c.t * kBlock + threadIdx.x;
     c.row = c.local < c.g.n;
     c.v = c.g.voff + c.local;
     return true;
+}
+
+// The head of an inner iteration, from the summed r.z (rho) and r.r of the
+// previous update: false when the scene's inner solve has ended, else beta.
+// A segment (the iterations since the last FP64 residual replacement) ends
+// when its recurrence residual has dropped by inner_tol, or when it already
+// meets the outer tolerance; the FP64 residual then decides. The lead thread
+// records the iteration in the scene's state (one thread of one kernel per
+// iteration).
+template <class S>
+__device__ __forceinline__ bool inner_head(SlotState *st, const SlotParams &prm, const double (&sums)[2], uint32_t k,
+                                           uint32_t seg, double rr0_s, double bb, const double (&rho_s)[2], bool lead,
+                                           int odd, S &beta) {
+    const double rho = sums[0], rr = sums[1];
+    const double rr0 = k == seg ? rr : rr0_s;
+    uint32_t done = kRun;
+    if (!isfinite(rr))
+        done = kNonFin;
+    else if (!(rr > prm.inner_tol * prm.inner_tol * rr0) || !(rr > prm.tol * prm.tol * bb))
+        done = kConv;
+    else if (k >= prm.max_inner)
+        done = kCap;
+    if (lead) {
+        st->rr_last = rr;
+        if (k == seg) st->rr0 = rr;
+    }
+    if (done) {
+        if (lead) {
+            st->done = done;
+            st->iters = k;
+        }
+        return false;
+    }
+    const double rho_prev = k == 0 ? 0.0 : rho_s[(k - 1) & 1];
+    beta = (k == 0 || !(rho_prev > 1e-300)) ? S(0) : S(rho / rho_prev);
+    if (lead) {
+        st->rho[k & 1] = rho;
+        st->k_update = k;
+        st->pbuf = odd ? 0u : 1u;
+    }
+    return true;
+}
+
+// ---- red-black reduced system -------------------------------------------------
+// With the REVs coloured by the parity of x + y + z (red even, black odd),
+// every bond joins a red and a black row, so the red-red and black-black parts
+// of A are the block diagonals D_rr and D_bb. Eliminating the red rows leaves
+// S x_b = b_b - A_br D_rr^-1 b_r with S = D_bb - A_br D_rr^-1 A_rb. CG on S
+// with block-Jacobi (D_bb) is CG with red-black block symmetric Gauss-Seidel
+// on A, in Eisenstat's form (the preconditioned operator is diag(I, S) in the
+// scaled variables), at half the iterations of block-Jacobi on A. With the
+// off-diagonal blocks of A equal to -k: A_rb p = -sum k p_nb, so for
+// w_r = D_rr^-1 (mask * sum_nb k p_nb) the product is S p = D_bb p - mask *
+// sum_nb k w_nb, the stencil of the SpMV with w in place of p at the
+// neighbours. One iteration is three kernels: pcg_rb_red (w on the red
+// rows), the SpMV of the black rows (S p, p.Ap), the update of the black
+// rows. The inner vectors live in the black rows of their planes; w lives in
+// the red rows of the Ap planes. Every residual replacement first recovers
+// x_r = D_rr^-1 (b_r + mask * sum_nb k x_nb) in FP64 (refine_red), so the FP64
+// residual is the full system's.
+__device__ __forceinline__ bool red_row(const uint32_t (&cc)[3]) { return ((cc[0] + cc[1] + cc[2]) & 1u) == 0u; }
+
+// Loads of the red-black inner kernels: the byte offsets of the row and its
+// neighbours, absent neighbours pointing at the row itself (in bounds).
+struct NbOffsets {
+    uint32_t ov, om[3], op[3];
+    bool lo[3], hi[3];
+};
+template <class S>
+__device__ __forceinline__ NbOffsets nb_offsets(const SlotGeo &g, uint32_t v, const uint32_t (&cc)[3]) {
+    NbOffsets o;
+    o.ov = v * uint32_t(sizeof(S));
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const uint32_t s = stride_of(g, a) * uint32_t(sizeof(S));
+        o.lo[a] = cc[a] > 0;
+        o.hi[a] = cc[a] + 1 < extent_of(g, a);
+        o.om[a] = o.lo[a] ? o.ov - s : o.ov;
+        o.op[a] = o.hi[a] ? o.ov + s : o.ov;
+    }
+    return o;
+}
+
+// sum over the six bonds of k u_nb (u zero where the neighbour is absent).
+template <class S>
+__device__ __forceinline__ void bond_sum(const S (&Km)[3][6], const S (&Kp)[3][6], const S (&u)[6][3], S (&acc)[3]) {
+    acc[0] = acc[1] = acc[2] = S(0);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const S *km = Km[a], *kp = Kp[a];
+        const S *pm = u[2 * a], *pq = u[2 * a + 1];
+        acc[0] = fmaS<S>(km[2], pm[2], fmaS<S>(km[1], pm[1], fmaS<S>(km[0], pm[0], acc[0])));
+        acc[1] = fmaS<S>(km[4], pm[2], fmaS<S>(km[3], pm[1], fmaS<S>(km[1], pm[0], acc[1])));
+        acc[2] = fmaS<S>(km[5], pm[2], fmaS<S>(km[4], pm[1], fmaS<S>(km[2], pm[0], acc[2])));
+        acc[0] = fmaS<S>(kp[2], pq[2], fmaS<S>(kp[1], pq[1], fmaS<S>(kp[0], pq[0], acc[0])));
+        acc[1] = fmaS<S>(kp[4], pq[2], fmaS<S>(kp[3], pq[1], fmaS<S>(kp[1], pq[0], acc[1])));
+        acc[2] = fmaS<S>(kp[5], pq[2], fmaS<S>(kp[4], pq[1], fmaS<S>(kp[2], pq[0], acc[2])));
+    }
+}
+
+// The six bond blocks of a row: its own +axis blocks and the -axis
+// neighbours' (whatever they hold where the neighbour is absent; the vector
+// they multiply is zero there).
+template <class S>
+__device__ __forceinline__ void ld_blocks(const S *__restrict__ K, size_t R, const NbOffsets &o, S (&Km)[3][6], S (&Kp)[3][6]) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const S *Ka = K + size_t(a) * 6 * R;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            Km[a][q] = ld_row(Ka + q * R, o.om[a]);
+            Kp[a][q] = ld_row(Ka + q * R, o.ov);
+        }
+    }
+}
+
+// The inner iteration's first kernel on the red rows: the inner check and
+// beta (the SpMV takes the same from the same partials next, and records
+// them), p = z + beta p_old formed at the six black neighbours, and
+// w = D_rr^-1 (mask * sum k p_nb) into the red rows of the Ap planes.
+template <class S, bool ONE>
+__global__ __launch_bounds__(kBlock) void pcg_rb_red(DevArena A, OneScene one, int odd) {
+    Tile c;
+    if (!inner_tile<S, ONE>(A, one, c) || !(c.bits & B_RB)) return;
+    __shared__ double lds[2][kWaves];
+    SlotState *st = A.st + c.slot;
+    const uint32_t done_in = st->done;
+    const uint32_t k = st->k_spmv;
+    const uint32_t seg = st->seg;
+    const double rho_s[2] = {st->rho[0], st->rho[1]};
+    const double rr0_s = st->rr0, bb = st->bb;
+    const SlotParams prm = A.prm[c.slot];
+    double sums[2];
+    const int kinds[2] = {kRz, kRr};
+    inner_load<2>(sums, A, c.g, kinds);
+    __builtin_amdgcn_sched_barrier(0);
+    const size_t R = A.R;
+    const S *__restrict__ z = Inner<S>::z(A);
+    const S *__restrict__ p_old = Inner<S>::p(A, odd);
+    const S *__restrict__ Di = Inner<S>::dinv(A);
+    const uint32_t v = c.row ? c.v : c.g.voff;
+    uint32_t cc[3];
+    coords(c.g, c.row ? c.local : 0u, cc);
+    const NbOffsets o = nb_offsets<S>(c.g, v, cc);
+    S zz[6][3], pp[6][3], Dv[6], Kp[3][6], Km[3][6];
+    const uint8_t m = A.mask[v];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) Dv[q] = ld_row(Di + q * R, o.ov);
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            zz[2 * a][q] = ld_row(z + q * R, o.om[a]);
+            pp[2 * a][q] = ld_row(p_old + q * R, o.om[a]);
+            zz[2 * a + 1][q] = ld_row(z + q * R, o.op[a]);
+            pp[2 * a + 1][q] = ld_row(p_old + q * R, o.op[a]);
+        }
+    ld_blocks<S>(Inner<S>::k(A), R, o, Km, Kp);
+    __builtin_amdgcn_sched_barrier(0);
+    block_sum_all<2>(sums, lds);
+    if (done_in) return;
+    S beta;
+    if (!inner_head<S>(st, prm, sums, k, seg, rr0_s, bb, rho_s, false, odd, beta)) return;
+    S u[6][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            u[2 * a][q] = o.lo[a] ? fmaS<S>(beta, pp[2 * a][q], zz[2 * a][q]) : S(0);
+            u[2 * a + 1][q] = o.hi[a] ? fmaS<S>(beta, pp[2 * a + 1][q], zz[2 * a + 1][q]) : S(0);
+        }
+    S s[3];
+    bond_sum<S>(Km, Kp, u, s);
+    if (!(m & 1)) s[0] = S(0);
+    if (!(m & 2)) s[1] = S(0);
+    if (!(m & 4)) s[2] = S(0);
+    if (c.row && red_row(cc)) {
+        S *__restrict__ w = Inner<S>::ap(A);
+        st_row(w, o.ov, fmaS<S>(Dv[2], s[2], fmaS<S>(Dv[1], s[1], Dv[0] * s[0])));
+        st_row(w + R, o.ov, fmaS<S>(Dv[4], s[2], fmaS<S>(Dv[3], s[1], Dv[1] * s[0])));
+        st_row(w + 2 * R, o.ov, fmaS<S>(Dv[5], s[2], fmaS<S>(Dv[4], s[1], Dv[2] * s[0])));
+    }
+}
+
+// The SpMV of the reduced system on the black rows: the inner check and
+// beta (recorded by the lead thread), p = z + beta p_old on the row,
+// y = D p - mask * sum k w_nb with w from the red neighbours' Ap rows; stores
+// p and y = S p on the black rows, partials of p.Ap.
+template <class S>
+__device__ __forceinline__ void spmv_black(const DevArena &A, const Tile &c, int odd, double (*lds)[kWaves]) {
+    SlotState *st = A.st + c.slot;
+    const uint32_t done_in = st->done;
+    const uint32_t k = st->k_spmv;
+    const uint32_t seg = st->seg;
+    const double rho_s[2] = {st->rho[0], st->rho[1]};
+    const double rr0_s = st->rr0, bb = st->bb;
+    const SlotParams prm = A.prm[c.slot];
+    double sums[2];
+    const int kinds[2] = {kRz, kRr};
+    inner_load<2>(sums, A, c.g, kinds);
+    __builtin_amdgcn_sched_barrier(0);
+    const size_t R = A.R;
+    const S *__restrict__ z = Inner<S>::z(A);
+    const S *__restrict__ p_old = Inner<S>::p(A, odd);
+    const S *__restrict__ Dp = Inner<S>::d(A);
+    const S *w = Inner<S>::ap(A);
+    const uint32_t v = c.row ? c.v : c.g.voff;
+    uint32_t cc[3];
+    coords(c.g, c.row ? c.local : 0u, cc);
+    const NbOffsets o = nb_offsets<S>(c.g, v, cc);
+    S zz[3], pp[3], ww[6][3], Dv[6], Kp[3][6], Km[3][6];
+    const uint8_t m = A.mask[v];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        zz[q] = ld_row(z + q * R, o.ov);
+        pp[q] = ld_row(p_old + q * R, o.ov);
+    }
+#pragma unroll
+    for (int q = 0; q < 6; ++q) Dv[q] = ld_row(Dp + q * R, o.ov);
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            ww[2 * a][q] = ld_row(w + q * R, o.om[a]);
+            ww[2 * a + 1][q] = ld_row(w + q * R, o.op[a]);
+        }
+    ld_blocks<S>(Inner<S>::k(A), R, o, Km, Kp);
+    __builtin_amdgcn_sched_barrier(0);
+    block_sum_all<2>(sums, lds);
+    if (done_in) return;
+    S beta;
+    if (!inner_head<S>(st, prm, sums, k, seg, rr0_s, bb, rho_s, c.t == c.g.tile0 && threadIdx.x == 0, odd, beta)) return;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            if (!o.lo[a]) ww[2 * a][q] = S(0);
+            if (!o.hi[a]) ww[2 * a + 1][q] = S(0);
+        }
+    S p[3], s[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) p[q] = fmaS<S>(beta, pp[q], zz[q]);
+    bond_sum<S>(Km, Kp, ww, s);
+    S y0 = fmaS<S>(Dv[2], p[2], fmaS<S>(Dv[1], p[1], Dv[0] * p[0]));
+    S y1 = fmaS<S>(Dv[4], p[2], fmaS<S>(Dv[3], p[1], Dv[1] * p[0]));
+    S y2 = fmaS<S>(Dv[5], p[2], fmaS<S>(Dv[4], p[1], Dv[2] * p[0]));
+    if (m & 1) y0 -= s[0];
+    if (m & 2) y1 -= s[1];
+    if (m & 4) y2 -= s[2];
+    double part[1] = {0.0};
+    if (c.row && !red_row(cc)) {
+        S *__restrict__ p_new = Inner<S>::p(A, !odd);
+        S *ap = Inner<S>::ap(A);
+        st_row(p_new, o.ov, p[0]);
+        st_row(p_new + R, o.ov, p[1]);
+        st_row(p_new + 2 * R, o.ov, p[2]);
+        st_row(ap, o.ov, y0);
+        st_row(ap + R, o.ov, y1);
+        st_row(ap + 2 * R, o.ov, y2);
+        part[0] = double(p[0]) * double(y0);
+        part[0] = __builtin_fma(double(p[1]), double(y1), part[0]);
+        part[0] = __builtin_fma(double(p[2]), double(y2), part[0]);
+    }
+    const int out[1] = {kPap};
+    tile_store<1>(part, A, c.t, out, lds);
 }
 
 // Inner convergence check of the previous update (r.r <= tol^2 r0.r0 every
@@ -556,11 +823,18 @@ __device__ __forceinline__ bool inner_tile(const DevArena &A, const OneScene &on
 // load (zeros where a neighbour is absent), then the partial sums. The row
 // loads do not depend on beta, so they are in flight while the workgroup
 // reduces the partials (LDS-only barriers do not wait for them).
-template <class S, bool ONE>
+template <class S, bool ONE, bool RB>
 __global__ __launch_bounds__(kBlock) void pcg_bond_spmv(DevArena A, OneScene one, int odd) {
     Tile c;
     if (!inner_tile<S, ONE>(A, one, c)) return;
     __shared__ double lds[2][kWaves];
+    // RB: the launch may hold red-black scenes (all of them when ONE).
+    if constexpr (RB) {
+        if (ONE || (c.bits & B_RB)) {
+            spmv_black<S>(A, c, odd, lds);
+            return;
+        }
+    }
     SlotState *st = A.st + c.slot;
     const uint32_t done_in = st->done;
     const uint32_t k = st->k_spmv;
@@ -627,37 +901,8 @@ __global__ __launch_bounds__(kBlock) void pcg_bond_spmv(DevArena A, OneScene one
         for (int q = 0; q < 6; ++q) Km[a][q] = lo[a] ? Km[a][q] : S(0);
     block_sum_all<2>(sums, lds);
     if (done_in) return;
-    const double rho = sums[0], rr = sums[1];
-    const bool lead = c.t == c.g.tile0 && threadIdx.x == 0;
-    // A segment (the iterations since the last FP64 residual replacement)
-    // ends when its recurrence residual has dropped by inner_tol, or when it
-    // already meets the outer tolerance; the FP64 residual then decides.
-    const double rr0 = k == seg ? rr : rr0_s;
-    uint32_t done = kRun;
-    if (!isfinite(rr))
-        done = kNonFin;
-    else if (!(rr > prm.inner_tol * prm.inner_tol * rr0) || !(rr > prm.tol * prm.tol * bb))
-        done = kConv;
-    else if (k >= prm.max_inner)
-        done = kCap;
-    if (lead) {
-        st->rr_last = rr;
-        if (k == seg) st->rr0 = rr;
-    }
-    if (done) {
-        if (lead) {
-            st->done = done;
-            st->iters = k;
-        }
-        return;
-    }
-    const double rho_prev = k == 0 ? 0.0 : rho_s[(k - 1) & 1];
-    const S beta = (k == 0 || !(rho_prev > 1e-300)) ? S(0) : S(rho / rho_prev);
-    if (lead) {
-        st->rho[k & 1] = rho;
-        st->k_update = k;
-        st->pbuf = odd ? 0u : 1u;
-    }
+    S beta;
+    if (!inner_head<S>(st, prm, sums, k, seg, rr0_s, bb, rho_s, c.t == c.g.tile0 && threadIdx.x == 0, odd, beta)) return;
     double part[1] = {0.0};
     S p[7][3];
 #pragma unroll
@@ -703,10 +948,19 @@ __global__ __launch_bounds__(kBlock) void pcg_bond_spmv(DevArena A, OneScene one
 // segment);
 // r -= alpha Ap; z = Dinv r; partials of r.z and r.r. Row loads are issued
 // before the p.Ap partial sum, as in the SpMV.
-template <class S, bool ONE>
+template <class S, bool ONE, bool RB>
 __global__ __launch_bounds__(kBlock) void pcg_update(DevArena A, OneScene one, int odd) {
     Tile c;
     if (!inner_tile<S, ONE>(A, one, c)) return;
+    // A red-black scene updates its black rows only.
+    bool act = c.row;
+    if constexpr (RB) {
+        if (ONE || (c.bits & B_RB)) {
+            uint32_t cc[3];
+            coords(c.g, c.local, cc);
+            act = act && !red_row(cc);
+        }
+    }
     __shared__ double lds[2][kWaves];
     SlotState *st = A.st + c.slot;
     const uint32_t done_in = st->done;
@@ -746,7 +1000,7 @@ __global__ __launch_bounds__(kBlock) void pcg_update(DevArena A, OneScene one, i
         st->iters = k + 1;
     }
     double part[2] = {0.0, 0.0};
-    if (c.row) {
+    if (act) {
         S ri[3], zi[3];
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
@@ -775,6 +1029,62 @@ __device__ __forceinline__ bool resid_runs(uint32_t bits, const SlotState *st, i
     return (bits & B_INNER) && st->done != kRun && !st->outer_done;
 }
 
+// Red-black reduced system, before refine_residual (same `forced`): the new
+// FP64 solution on every row of the scene. Black rows: x_new = x_old + d (or
+// zero, or the warm start, on the first residual of a solve). Red rows, all
+// of whose neighbours are black: x_new = D_rr^-1 (b + mask * sum_nb k x_nb),
+// which zeroes the red rows of the residual b - A x.
+template <class S>
+__global__ __launch_bounds__(kBlock) void refine_red(DevArena A, int forced) {
+    Tile c;
+    if (!tile_base(A, c) || (c.bits & (B_F64 | B_RB)) != (Inner<S>::kBit | B_RB)) return;
+    if (!resid_runs(c.bits, A.st + c.slot, forced) || !c.row) return;
+    const size_t R = A.R;
+    const SlotState *sst = A.st + c.slot;
+    const uint32_t xpar = sst->xpar;
+    const bool first = forced;
+    const bool warm = first && (c.bits & B_WARM_USE);
+    const bool use_d = !first && sst->iters > sst->seg;
+    const double *__restrict__ x_old = warm ? A.xw : (xpar ? A.xb : A.xa);
+    double *__restrict__ x_new = xpar ? A.xa : A.xb;
+    const S *__restrict__ d = Inner<S>::dv(A);
+    auto xval = [&](uint32_t u, double (&o)[3]) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+            o[q] = first ? (warm ? x_old[q * R + u] : 0.0) : x_old[q * R + u] + (use_d ? double(d[q * R + u]) : 0.0);
+    };
+    const uint32_t v = c.v;
+    uint32_t cc[3];
+    coords(c.g, c.local, cc);
+    double xv[3];
+    if (!red_row(cc)) {
+        xval(v, xv);
+    } else {
+        double acc[3] = {0, 0, 0};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const uint32_t s = stride_of(c.g, a);
+            if (cc[a] > 0) {
+                double xu[3];
+                xval(v - s, xu);
+                symv_acc<double>(A.k64 + size_t(a) * 6 * R, R, v - s, xu, acc);
+            }
+            if (cc[a] + 1 < extent_of(c.g, a)) {
+                double xu[3];
+                xval(v + s, xu);
+                symv_acc<double>(A.k64 + size_t(a) * 6 * R, R, v, xu, acc);
+            }
+        }
+        const uint8_t m = A.mask[v];
+        double t[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) t[q] = A.rhs[q * R + v] + ((m >> q & 1) ? acc[q] : 0.0);
+        symv<double>(A.dinv64, R, v, t, xv);
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) x_new[q * R + v] = xv[q];
+}
+
 // FP64 residual replacement (mixed-precision PCG with reliable updates). The
 // first residual of a solve starts from zero (or the warm start copy) and
 // clears the search direction. A later one folds the segment's inner
@@ -782,7 +1092,9 @@ __device__ __forceinline__ bool resid_runs(uint32_t bits, const SlotState *st, i
 // and keeps the search direction, moved to p buffer 0 where the next chunk's
 // SpMV reads it. r64 = b - A64 x_new replaces the inner residual: r = r64 in
 // the inner precision, z = Dinv r. Partials of ||r64||^2, ||b||^2, r.z, r.r.
-// x_new goes to the other ping-pong buffer.
+// x_new goes to the other ping-pong buffer. A red-black scene's x_new is
+// already complete (refine_red); its red rows take r = z = p = 0, so the
+// inner partials cover the black rows while ||r64|| is the full system's.
 template <class S>
 __global__ __launch_bounds__(kBlock) void refine_residual(DevArena A, int forced) {
     Tile c;
@@ -801,15 +1113,18 @@ __global__ __launch_bounds__(kBlock) void refine_residual(DevArena A, int forced
     const double *__restrict__ x_old = warm ? A.xw : (xpar ? A.xb : A.xa);
     double *__restrict__ x_new = xpar ? A.xa : A.xb;
     const S *__restrict__ d = Inner<S>::dv(A);
+    const bool rb = c.bits & B_RB;
     double part[4] = {0.0, 0.0, 0.0, 0.0};
     if (c.row) {
         const uint32_t v = c.v;
         uint32_t cc[3];
         coords(c.g, c.local, cc);
+        const bool red = rb && red_row(cc);
         auto xval = [&](uint32_t u, double (&o)[3]) {
 #pragma unroll
             for (int q = 0; q < 3; ++q)
-                o[q] = first ? (warm ? x_old[q * R + u] : 0.0) : x_old[q * R + u] + (use_d ? double(d[q * R + u]) : 0.0);
+                o[q] = rb ? x_new[q * R + u]
+                          : (first ? (warm ? x_old[q * R + u] : 0.0) : x_old[q * R + u] + (use_d ? double(d[q * R + u]) : 0.0));
         };
         double xv[3];
         xval(v, xv);
@@ -841,10 +1156,10 @@ __global__ __launch_bounds__(kBlock) void refine_residual(DevArena A, int forced
         for (int q = 0; q < 3; ++q) {
             const double b = A.rhs[q * R + v];
             const double r64 = f[q] * (b - (yv[q] - acc[q]));
-            x_new[q * R + v] = xv[q];
-            rs[q] = S(r64);
+            if (!rb) x_new[q * R + v] = xv[q];
+            rs[q] = red ? S(0) : S(r64);
             r[q * R + v] = rs[q];
-            if (first)
+            if (first || red)
                 p0[q * R + v] = S(0);
             else if (pbuf)
                 p0[q * R + v] = p1[q * R + v];
@@ -1167,52 +1482,66 @@ struct Lock {
     Lock() : g(context().mutex) {}
 };
 
+// The inner kernels of one iteration for precision S: with red-black scenes
+// in the launch (rb), pcg_rb_red first and the kernels that branch per tile.
+template <class S>
+void launch_iteration(vx_batch *b, bool rb, const OneScene &one, const dim3 &grid, int odd, bool big) {
+    Context &ctx = context();
+    const dim3 blk(kBlock), sg(b->max_slots);
+    const bool single = b->single;
+    auto reduce = [&](int k0, int nk) {
+        if (big) hipLaunchKernelGGL(scene_reduce<S>, sg, blk, 0, ctx.stream, b->A, k0, nk, 0);
+    };
+    if (rb && single) {
+        hipLaunchKernelGGL((pcg_rb_red<S, true>), grid, blk, 0, ctx.stream, b->A, one, odd);
+        hipLaunchKernelGGL((pcg_bond_spmv<S, true, true>), grid, blk, 0, ctx.stream, b->A, one, odd);
+        reduce(kPap, 1);
+        hipLaunchKernelGGL((pcg_update<S, true, true>), grid, blk, 0, ctx.stream, b->A, one, odd);
+    } else if (rb) {
+        hipLaunchKernelGGL((pcg_rb_red<S, false>), grid, blk, 0, ctx.stream, b->A, one, odd);
+        hipLaunchKernelGGL((pcg_bond_spmv<S, false, true>), grid, blk, 0, ctx.stream, b->A, one, odd);
+        reduce(kPap, 1);
+        hipLaunchKernelGGL((pcg_update<S, false, true>), grid, blk, 0, ctx.stream, b->A, one, odd);
+    } else if (single) {
+        hipLaunchKernelGGL((pcg_bond_spmv<S, true, false>), grid, blk, 0, ctx.stream, b->A, one, odd);
+        reduce(kPap, 1);
+        hipLaunchKernelGGL((pcg_update<S, true, false>), grid, blk, 0, ctx.stream, b->A, one, odd);
+    } else {
+        hipLaunchKernelGGL((pcg_bond_spmv<S, false, false>), grid, blk, 0, ctx.stream, b->A, one, odd);
+        reduce(kPap, 1);
+        hipLaunchKernelGGL((pcg_update<S, false, false>), grid, blk, 0, ctx.stream, b->A, one, odd);
+    }
+    reduce(kRz, 2);
+}
+
 // One inner chunk: the refinement residual of scenes whose inner solve has
 // ended (device-decided), then the arena's iterations per tick.
-int32_t capture_inner(vx_batch *b, int f64, uint32_t tiles, bool big, hipGraphExec_t *out) {
+int32_t capture_inner(vx_batch *b, int f64, bool rb, uint32_t tiles, bool big, hipGraphExec_t *out) {
     Context &ctx = context();
     hipGraph_t graph;
     const dim3 grid(tiles), blk(kBlock), sg(b->max_slots);
-    auto reduce = [&](int k0, int nk) {
-        if (!big) return;
-        if (f64)
-            hipLaunchKernelGGL(scene_reduce<double>, sg, blk, 0, ctx.stream, b->A, k0, nk, 0);
-        else
-            hipLaunchKernelGGL(scene_reduce<float>, sg, blk, 0, ctx.stream, b->A, k0, nk, 0);
-    };
     VX_HIP(hipStreamBeginCapture(ctx.stream, hipStreamCaptureModeThreadLocal));
     if (f64) {
+        if (rb) hipLaunchKernelGGL(refine_red<double>, grid, blk, 0, ctx.stream, b->A, 0);
         hipLaunchKernelGGL(refine_residual<double>, grid, blk, 0, ctx.stream, b->A, 0);
         hipLaunchKernelGGL(refine_decide<double>, sg, blk, 0, ctx.stream, b->A, 0);
+        if (big) hipLaunchKernelGGL(scene_reduce<double>, sg, blk, 0, ctx.stream, b->A, int(kRz), 2, 0);
     } else {
+        if (rb) hipLaunchKernelGGL(refine_red<float>, grid, blk, 0, ctx.stream, b->A, 0);
         hipLaunchKernelGGL(refine_residual<float>, grid, blk, 0, ctx.stream, b->A, 0);
         hipLaunchKernelGGL(refine_decide<float>, sg, blk, 0, ctx.stream, b->A, 0);
+        if (big) hipLaunchKernelGGL(scene_reduce<float>, sg, blk, 0, ctx.stream, b->A, int(kRz), 2, 0);
     }
-    reduce(kRz, 2);
     // A one-scene arena (vx_scene_*) never changes its scene: its geometry
     // goes into the kernel arguments.
-    const OneScene one = b->single ? make_one(b->slots[0].geo) : OneScene{};
-    const dim3 grid1(b->single ? one.grid : tiles);
+    OneScene one = b->single ? make_one(b->slots[0].geo) : OneScene{};
+    one.rb = rb ? 1u : 0u;
+    const dim3 gi(b->single ? one.grid : tiles);
     for (uint32_t t = 0; t < b->iters_per_tick; ++t) {
-        const int odd = int(t & 1);
-        if (f64 && b->single)
-            hipLaunchKernelGGL((pcg_bond_spmv<double, true>), grid1, blk, 0, ctx.stream, b->A, one, odd);
-        else if (f64)
-            hipLaunchKernelGGL((pcg_bond_spmv<double, false>), grid, blk, 0, ctx.stream, b->A, one, odd);
-        else if (b->single)
-            hipLaunchKernelGGL((pcg_bond_spmv<float, true>), grid1, blk, 0, ctx.stream, b->A, one, odd);
+        if (f64)
+            launch_iteration<double>(b, rb, one, gi, int(t & 1), big);
         else
-            hipLaunchKernelGGL((pcg_bond_spmv<float, false>), grid, blk, 0, ctx.stream, b->A, one, odd);
-        reduce(kPap, 1);
-        if (f64 && b->single)
-            hipLaunchKernelGGL((pcg_update<double, true>), grid1, blk, 0, ctx.stream, b->A, one, odd);
-        else if (f64)
-            hipLaunchKernelGGL((pcg_update<double, false>), grid, blk, 0, ctx.stream, b->A, one, odd);
-        else if (b->single)
-            hipLaunchKernelGGL((pcg_update<float, true>), grid1, blk, 0, ctx.stream, b->A, one, odd);
-        else
-            hipLaunchKernelGGL((pcg_update<float, false>), grid, blk, 0, ctx.stream, b->A, one, odd);
-        reduce(kRz, 2);
+            launch_iteration<float>(b, rb, one, gi, int(t & 1), big);
     }
     VX_HIP(hipStreamEndCapture(ctx.stream, &graph));
     hipError_t e = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
@@ -1240,15 +1569,15 @@ bool any_big(const vx_batch *b) {
     return false;
 }
 
-int32_t inner_graph(vx_batch *b, int f64, uint32_t tiles, hipGraphExec_t *out) {
+int32_t inner_graph(vx_batch *b, int f64, bool rb, uint32_t tiles, hipGraphExec_t *out) {
     const bool big = any_big(b);
-    const uint64_t key = (uint64_t(tiles) << 2) | (uint64_t(big) << 1) | uint64_t(f64);
+    const uint64_t key = (uint64_t(tiles) << 3) | (uint64_t(rb) << 2) | (uint64_t(big) << 1) | uint64_t(f64);
     for (auto &kv : b->g_inner)
         if (kv.first == key) {
             *out = kv.second;
             return VX_OK;
         }
-    if (int32_t c = capture_inner(b, f64, tiles, big, out)) return c;
+    if (int32_t c = capture_inner(b, f64, rb, tiles, big, out)) return c;
     b->g_inner.emplace_back(key, *out);
     return VX_OK;
 }
@@ -1260,14 +1589,21 @@ int32_t launch_tick(vx_batch *b, Flight &f) {
     Context &ctx = context();
     hipStream_t s = ctx.stream;
     uint32_t any = 0;
-    bool inner[2] = {false, false}, resid0[2] = {false, false};
+    bool inner[2] = {false, false}, resid0[2] = {false, false}, rb[2] = {false, false}, rb0[2] = {false, false};
     for (uint32_t i = 0; i < b->max_slots; ++i) {
         if (!b->slots[i].used) f.bits[i] = 0;
         const uint32_t bi = f.bits[i];
         any |= bi;
         const int prec = (bi & B_F64) ? 1 : 0;
-        if (bi & B_INNER) inner[prec] = true;
-        if (bi & B_RESID0) resid0[prec] = true;
+        const bool red_black = bi & B_RB;
+        if (bi & B_INNER) {
+            inner[prec] = true;
+            rb[prec] = rb[prec] || red_black;
+        }
+        if (bi & B_RESID0) {
+            resid0[prec] = true;
+            rb0[prec] = rb0[prec] || red_black;
+        }
     }
     const uint32_t tiles = launch_tiles(b);
     const dim3 grid(tiles), blk(kBlock), sg(b->max_slots);
@@ -1293,11 +1629,13 @@ int32_t launch_tick(vx_batch *b, Flight &f) {
     }
     const bool big = any_big(b);
     if (resid0[0]) {
+        if (rb0[0]) hipLaunchKernelGGL(refine_red<float>, grid, blk, 0, s, b->A, 1);
         hipLaunchKernelGGL(refine_residual<float>, grid, blk, 0, s, b->A, 1);
         hipLaunchKernelGGL(refine_decide<float>, sg, blk, 0, s, b->A, 1);
         if (big) hipLaunchKernelGGL(scene_reduce<float>, sg, blk, 0, s, b->A, int(kRz), 2, 1);
     }
     if (resid0[1]) {
+        if (rb0[1]) hipLaunchKernelGGL(refine_red<double>, grid, blk, 0, s, b->A, 1);
         hipLaunchKernelGGL(refine_residual<double>, grid, blk, 0, s, b->A, 1);
         hipLaunchKernelGGL(refine_decide<double>, sg, blk, 0, s, b->A, 1);
         if (big) hipLaunchKernelGGL(scene_reduce<double>, sg, blk, 0, s, b->A, int(kRz), 2, 1);
@@ -1306,7 +1644,7 @@ int32_t launch_tick(vx_batch *b, Flight &f) {
     for (int prec = 0; prec < 2; ++prec) {
         if (!inner[prec]) continue;
         hipGraphExec_t g = nullptr;
-        if (int32_t c = inner_graph(b, prec, tiles, &g)) return c;
+        if (int32_t c = inner_graph(b, prec, rb[prec], tiles, &g)) return c;
         VX_HIP(hipGraphLaunch(g, s));
     }
     if (any & (B_APPLY | B_WARM_SAVE)) hipLaunchKernelGGL(apply_correction, grid, blk, 0, s, b->A);
@@ -1330,7 +1668,11 @@ int32_t launch_tick(vx_batch *b, Flight &f) {
     return VX_OK;
 }
 
-uint32_t prec_bits(const vx_step_params &p) { return p.solve.mode == VX_SOLVE_F64 ? uint32_t(B_F64) : 0u; }
+// Bits of every solve tick: the inner precision and the preconditioner.
+uint32_t solve_bits(const vx_solve_params &s) {
+    return (s.mode == VX_SOLVE_F64 ? uint32_t(B_F64) : 0u) | (s.preconditioner == VX_PRECOND_RED_BLACK ? uint32_t(B_RB) : 0u);
+}
+uint32_t prec_bits(const vx_step_params &p) { return solve_bits(p.solve); }
 // The end of a substep: the fracture pass (when on) and the velocities.
 uint32_t finish_bits(const vx_step_params &p) { return p.fracture.enabled ? uint32_t(B_FINISH | B_FRACTURE) : uint32_t(B_FINISH); }
 
@@ -1765,12 +2107,15 @@ SlotParams device_params(double h, const double *g, const vx_solve_params *s, co
     return sp;
 }
 
+bool solve_ok(const vx_solve_params &s) {
+    return s.max_inner > 0 && s.tolerance > 0.0 && s.mode <= VX_SOLVE_F64 && s.preconditioner <= VX_PRECOND_RED_BLACK;
+}
+
 int32_t admit(vx_batch *b, const vx_scene_desc *d, const vx_step_params *p, uint32_t steps, bool manual, uint32_t *out_slot) {
     std::vector<uint8_t> bonds, tmask, mask;
     uint32_t active = 0;
     if (int32_t c = check_desc(d, bonds, tmask, mask, &active)) return c;
-    if (!manual && (p->substeps == 0 || p->corotation_iterations == 0 || !(p->fixed_dt > 0.0) || p->solve.max_inner == 0 ||
-                    !(p->solve.tolerance > 0.0)))
+    if (!manual && (p->substeps == 0 || p->corotation_iterations == 0 || !(p->fixed_dt > 0.0) || !solve_ok(p->solve)))
         return fail(VX_ERR_INVALID_ARGUMENT, "invalid step parameters");
     uint32_t slot = b->max_slots;
     for (uint32_t i = 0; i < b->max_slots; ++i)
@@ -2143,13 +2488,13 @@ extern "C" int32_t vx_scene_assemble(vx_scene *sc, double h) {
 extern "C" int32_t vx_scene_solve(vx_scene *sc, const vx_solve_params *prm, vx_solve_report *rep) {
     clear_error();
     if (int32_t c = scene_ok(sc)) return c;
-    if (!prm || prm->max_inner == 0 || !(prm->tolerance > 0.0)) return fail(VX_ERR_INVALID_ARGUMENT, "invalid solve parameters");
+    if (!prm || !solve_ok(*prm)) return fail(VX_ERR_INVALID_ARGUMENT, "invalid solve parameters");
     Lock lock;
     vx_batch *b = sc->batch;
     SlotHost &h = b->slots[sc->slot];
     const vx_step_params p = h.params;
     if (int32_t c = set_params(b, sc->slot, p.fixed_dt, p.gravity, prm)) return c;
-    const uint32_t prec = prm->mode == VX_SOLVE_F64 ? uint32_t(B_F64) : 0u;
+    const uint32_t prec = solve_bits(*prm);
     uint32_t bits = B_RESID0 | B_INNER | prec | ((prm->warm_start && h.warm_ok) ? uint32_t(B_WARM_USE) : 0u);
     if (rep) std::memset(rep, 0, sizeof *rep);
     const SlotState *st = &b->known[sc->slot];
@@ -2371,7 +2716,7 @@ extern "C" int32_t vx_scene_time_spmv(vx_scene *sc, uint32_t reps, double *out_u
     hipGraphExec_t exec = nullptr;
     VX_HIP(hipStreamBeginCapture(ctx.stream, hipStreamCaptureModeThreadLocal));
     for (uint32_t k = 0; k < reps; ++k)
-        hipLaunchKernelGGL((pcg_bond_spmv<float, true>), dim3(one.grid), dim3(kBlock), 0, ctx.stream, b->A, one, int(k & 1));
+        hipLaunchKernelGGL((pcg_bond_spmv<float, true, false>), dim3(one.grid), dim3(kBlock), 0, ctx.stream, b->A, one, int(k & 1));
     VX_HIP(hipStreamEndCapture(ctx.stream, &graph));
     hipError_t ge = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
@@ -2402,8 +2747,7 @@ extern "C" int32_t vx_scene_time_spmv(vx_scene *sc, uint32_t reps, double *out_u
 extern "C" int32_t vx_scene_step(vx_scene *sc, const vx_step_params *prm, vx_step_report *rep) {
     clear_error();
     if (int32_t c = scene_ok(sc)) return c;
-    if (!prm || prm->substeps == 0 || prm->corotation_iterations == 0 || !(prm->fixed_dt > 0.0) || prm->solve.max_inner == 0 ||
-        !(prm->solve.tolerance > 0.0))
+    if (!prm || prm->substeps == 0 || prm->corotation_iterations == 0 || !(prm->fixed_dt > 0.0) || !solve_ok(prm->solve))
         return fail(VX_ERR_INVALID_ARGUMENT, "invalid step parameters");
     Lock lock;
     vx_batch *b = sc->batch;

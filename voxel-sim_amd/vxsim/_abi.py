@@ -20,6 +20,8 @@ VX_EMPTY = 0xFFFFFFFF
 
 VX_SOLVE_MIXED = 0
 VX_SOLVE_F64 = 1
+VX_PRECOND_BLOCK_JACOBI = 0
+VX_PRECOND_RED_BLACK = 1
 
 VX_SCENE_FREE = 0
 VX_SCENE_RUNNING = 1
@@ -64,6 +66,7 @@ class SolveParams(ctypes.Structure):
         ("tolerance", ctypes.c_double), ("inner_tolerance", ctypes.c_double),
         ("max_inner", ctypes.c_uint32), ("max_outer", ctypes.c_uint32),
         ("warm_start", ctypes.c_uint32), ("mode", ctypes.c_uint32),
+        ("preconditioner", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
     ]
 
 

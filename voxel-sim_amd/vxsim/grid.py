@@ -12,17 +12,23 @@ import ctypes
 import numpy as np
 
 from . import lib
-from ._abi import (FractureParams, SceneDesc, SolveParams, SolveReport, StepParams, StepReport, VX_SOLVE_F64,
-                   VX_SOLVE_MIXED, c_dp, c_fp, c_u8p, c_u32p)
+from ._abi import (FractureParams, SceneDesc, SolveParams, SolveReport, StepParams, StepReport, VX_PRECOND_BLOCK_JACOBI,
+                   VX_PRECOND_RED_BLACK, VX_SOLVE_F64, VX_SOLVE_MIXED, c_dp, c_fp, c_u8p, c_u32p)
 from .solver import SolverError, check
 
 MIXED = VX_SOLVE_MIXED
 F64 = VX_SOLVE_F64
+BLOCK_JACOBI = VX_PRECOND_BLOCK_JACOBI
+RED_BLACK = VX_PRECOND_RED_BLACK
 
 
-def solve_params(tolerance=1e-9, inner_tolerance=1e-3, max_inner=4000, max_outer=12, warm_start=False, mode=MIXED):
+def solve_params(tolerance=1e-9, inner_tolerance=1e-3, max_inner=4000, max_outer=12, warm_start=False, mode=MIXED,
+                 preconditioner=BLOCK_JACOBI):
+    """vx_solve_params. `preconditioner`: BLOCK_JACOBI (the reference's) or
+    RED_BLACK (CG on the red-black reduced system: red-black block symmetric
+    Gauss-Seidel, about half the iterations)."""
     return SolveParams(float(tolerance), float(inner_tolerance), int(max_inner), int(max_outer), 1 if warm_start else 0,
-                       int(mode))
+                       int(mode), int(preconditioner), 0)
 
 
 def _report(r) -> dict:
