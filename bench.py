@@ -200,6 +200,27 @@ def serve_bench(rate: float, seconds: float, rev_lo: float, rev_hi: float, steps
     print(json.dumps(line))
 
 
+def red_black_steps(scene, step_kw: dict, steps: int, warmup: int, ms_block_jacobi: float) -> dict:
+    """The same steps from the same initial scene with the red-black
+    preconditioner (its wall time per step beside block-Jacobi's)."""
+    from vxsim import grid
+    sp = step_kw["solve"]
+    kw = dict(step_kw, solve=grid.solve_params(tolerance=sp.tolerance, inner_tolerance=sp.inner_tolerance,
+                                               max_inner=sp.max_inner, max_outer=sp.max_outer,
+                                               preconditioner=grid.RED_BLACK))
+    with grid.GridScene(scene) as gs:
+        for _ in range(warmup):
+            gs.step(**kw)
+        reps = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            reps.append(gs.step(**kw))
+        wall = time.perf_counter() - t0
+    ms = 1000.0 * wall / steps
+    return {"ms_per_step": ms, "inner_iterations": sum(r["inner_iterations"] for r in reps),
+            "solves": sum(r["solves"] for r in reps), "time_vs_block_jacobi": ms_block_jacobi / ms}
+
+
 def large_bench(steps: int, warmup: int, compare: bool = True):
     """The 192x192x96 wood block (3.54 M REVs, out of the 256 MB Infinity
     Cache): whole steps, the SpMV roofline at that size, and the plain FP64
@@ -235,6 +256,12 @@ def large_bench(steps: int, warmup: int, compare: bool = True):
             t = time.perf_counter()
             r64 = gs.solve(tolerance=1e-6, max_inner=40000, mode=grid.F64)
             t64 = time.perf_counter() - t
+            rb_sp = grid.solve_params(tolerance=1e-6, inner_tolerance=INNER_TOL, max_inner=20000,
+                                      preconditioner=grid.RED_BLACK)
+            gs.solve(rb_sp)
+            t = time.perf_counter()
+            rrb = gs.solve(rb_sp)
+            trb = time.perf_counter() - t
             # The drop-in over the reference row packing at this size (host
             # buffers and PCIe upload included), the reference's f32 semantics.
             rows = gs.export_rows()
@@ -261,7 +288,10 @@ def large_bench(steps: int, warmup: int, compare: bool = True):
                      "plain_fp64_structured": {"ms": 1000 * t64, "iters": r64["inner_iterations"],
                                                "dof_iters_per_s": 3.0 * active * (r64["inner_iterations"] +
                                                                                    r64["outer_sweeps"]) / t64},
-                     "time_to_solution_vs_fp64": t64 / tm}
+                     "time_to_solution_vs_fp64": t64 / tm,
+                     "red_black_solve": {"ms": 1000 * trb, "inner": rrb["inner_iterations"],
+                                         "sweeps": rrb["outer_sweeps"], "rel_residual": rrb["relative_residual"],
+                                         "time_vs_block_jacobi": tm / trb}}
     b = SPMV_BYTES_PER_ROW * scene.n
     line = {"metric": "DOF-PCG-iters/s", "value": 3.0 * active * iters / wall, "unit": "DOF-PCG-iters/s",
             "workload": "%s: %d REVs, %d active rows, %d steps, rel. residual 1e-6" % (scene.name, scene.n, active, steps),
@@ -429,6 +459,18 @@ def main():
         t = time.perf_counter()
         r5 = gs.solve(tolerance=1e-5, inner_tolerance=INNER_TOL, max_inner=8000, max_outer=12)
         t5 = time.perf_counter() - t
+        # The red-black reduced-system preconditioner on the same system.
+        rb_sp = grid.solve_params(tolerance=SOLVE_TOL, inner_tolerance=INNER_TOL, max_inner=8000, max_outer=12,
+                                  preconditioner=grid.RED_BLACK)
+        gs.solve(rb_sp)  # captures its graph
+        t = time.perf_counter()
+        rrb = gs.solve(rb_sp)
+        trb = time.perf_counter() - t
+        rb_sp5 = grid.solve_params(tolerance=1e-5, inner_tolerance=INNER_TOL, max_inner=8000, max_outer=12,
+                                   preconditioner=grid.RED_BLACK)
+        t = time.perf_counter()
+        rrb5 = gs.solve(rb_sp5)
+        trb5 = time.perf_counter() - t
         mixed_rate = 3.0 * active * (rm["inner_iterations"] + rm["outer_sweeps"]) / tm
         f64_rate = 3.0 * active * (r64["inner_iterations"] + r64["outer_sweeps"]) / t64
         rows = gs.export_rows()
@@ -465,11 +507,19 @@ def main():
             "dropin_rows_f32": drop32,
             "mixed_solve_to_1e-5": {"ms": 1000 * t5, "inner": r5["inner_iterations"], "sweeps": r5["outer_sweeps"],
                                     "rel_residual": r5["relative_residual"]},
+            "red_black_solve": {"ms": 1000 * trb, "inner": rrb["inner_iterations"], "sweeps": rrb["outer_sweeps"],
+                                "rel_residual": rrb["relative_residual"], "tolerance": SOLVE_TOL,
+                                "time_vs_block_jacobi": tm / trb},
+            "red_black_solve_to_1e-5": {"ms": 1000 * trb5, "inner": rrb5["inner_iterations"],
+                                        "sweeps": rrb5["outer_sweeps"], "rel_residual": rrb5["relative_residual"],
+                                        "time_vs_block_jacobi": t5 / trb5},
             "mixed_vs_plain_fp64_structured": mixed_rate / f64_rate,
             "mixed_vs_plain_fp64_dropin": mixed_rate / drop_rate,
             "time_to_solution_vs_fp64_structured": t64 / tm,
         }
     gs.close()
+    if not args.no_compare:
+        extra["red_black_step"] = red_black_steps(scene, step_kw, args.steps, args.warmup, ms_per_step)
     if not args.no_batch:
         extra["batched"] = batched()
 
