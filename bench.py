@@ -102,6 +102,19 @@ def cpu_baseline(scene, budget_s: float = 12.0):
     return out
 
 
+def stream_copy(nbytes: int, reps: int = 20) -> dict:
+    """A STREAM copy on the device (vx_copy_bandwidth: the library's own
+    16-byte-vector copy kernel, median of `reps` launches timed with HIP
+    events). GB/s counts the bytes read plus written."""
+    import ctypes
+    from vxsim import lib
+    from vxsim.solver import check
+    nbytes = nbytes // 16 * 16
+    us = ctypes.c_double()
+    check(lib().vx_copy_bandwidth(nbytes, reps, ctypes.byref(us)))
+    return {"bytes_read": nbytes, "us": us.value, "gbs": 2.0 * nbytes / (us.value * 1e-6) / 1e9}
+
+
 def measured_traffic():
     """HBM bytes per SpMV launch from the committed rocprofv3 counter
     summary (FETCH_SIZE and WRITE_SIZE scaled by a 1 GiB copy), or None."""
@@ -298,7 +311,8 @@ def large_bench(steps: int, warmup: int, compare: bool = True):
             "ms_per_step": 1000 * wall / steps, "solves": sum(r["solves"] for r in reps), "iterations": iters,
             "roofline": {"bound": "hbm", "kernel": SPMV_KERNEL, "bytes_per_launch": b, "avg_us": spmv_us,
                          "achieved": b / (spmv_us * 1e-6) / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                         "frac": b / (spmv_us * 1e-6) / 1e9 / PEAK_HBM_GBS},
+                         "frac": b / (spmv_us * 1e-6) / 1e9 / PEAK_HBM_GBS,
+                         "stream_copy": {"1GiB": stream_copy(COPY_BYTES), "same_bytes": stream_copy(b // 2)}},
             "data": "synthetic", "dtype": "f32 inner / f64 refinement and reductions"}
     line.update(extra)
     print(json.dumps(line))
@@ -439,6 +453,10 @@ def main():
     spmv_us = gs.time_spmv(200)
     spmv_bytes = SPMV_BYTES_PER_ROW * scene.n
     achieved = spmv_bytes / (spmv_us * 1e-6) / 1e9
+    # What a plain copy reaches on this device: at 1 GiB (out of every cache)
+    # and with the SpMV's own byte count (half read, half written).
+    copy_big = stream_copy(COPY_BYTES)
+    copy_same = stream_copy(spmv_bytes // 2)
 
     extra = {}
     if not args.no_compare:
@@ -547,7 +565,8 @@ def main():
                    "iterations": iters, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": SPMV_KERNEL, "achieved": achieved, "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": measured_traffic(),
-                     "bytes_per_launch": spmv_bytes, "avg_us": spmv_us},
+                     "bytes_per_launch": spmv_bytes, "avg_us": spmv_us,
+                     "stream_copy": {"1GiB": copy_big, "same_bytes": copy_same}},
         "cpu_baseline": cpu,
     }
     line.update(extra)

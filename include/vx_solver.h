@@ -56,6 +56,12 @@ const char *vx_last_error(void);
 /* Initialise the device context (optional; every entry point does it). */
 int32_t vx_init(void);
 
+/* Measurement aid, no counterpart in the reference: the median duration
+ * (microseconds) of `reps` launches of a STREAM copy kernel reading `bytes`
+ * and writing as many (device buffers allocated and freed by the call), the
+ * bandwidth a plain copy reaches beside the solver kernels' rooflines. */
+int32_t vx_copy_bandwidth(uint64_t bytes, uint32_t reps, double *out_us);
+
 /* ---- drop-in solve ----------------------------------------------------- */
 /*
  * vx_solve_pcg — replaces cuda_solver::solve_pcg (src/cuda_solver.rs:301-501).

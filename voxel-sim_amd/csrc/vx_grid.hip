@@ -603,29 +603,71 @@ __device__ __forceinline__ bool inner_head(SlotState *st, const SlotParams &prm,
 // sum_nb k w_nb, the stencil of the SpMV with w in place of p at the
 // neighbours. One iteration is three kernels: pcg_rb_red (w on the red
 // rows), the SpMV of the black rows (S p, p.Ap), the update of the black
-// rows. The inner vectors live in the black rows of their planes; w lives in
-// the red rows of the Ap planes. Every residual replacement first recovers
+// rows. The inner vectors hold the black rows at colour-split positions
+// (rb_pos); w holds the red rows at theirs, in the Ap planes. Operator
+// planes (bond blocks, D, D^-1, mask) keep the natural layout. Every residual
+// replacement first recovers
 // x_r = D_rr^-1 (b_r + mask * sum_nb k x_nb) in FP64 (refine_red), so the FP64
 // residual is the full system's.
 __device__ __forceinline__ bool red_row(const uint32_t (&cc)[3]) { return ((cc[0] + cc[1] + cc[2]) & 1u) == 0u; }
+__device__ __forceinline__ uint32_t colour_of(const SlotGeo &g, uint32_t v) {
+    uint32_t cc[3];
+    coords(g, v, cc);
+    return red_row(cc) ? 0u : 1u;
+}
 
-// Loads of the red-black inner kernels: the byte offsets of the row and its
-// neighbours, absent neighbours pointing at the row itself (in bounds).
-struct NbOffsets {
-    uint32_t ov, om[3], op[3];
+// Colour-split positions of a red-black scene's inner vectors. Natural rows
+// 2m and 2m + 1 of a box always differ in colour (they share an x-line of
+// even length, or a z-plane of even size, or, with nx and ny odd, the
+// colour is the parity of v itself), so m = v >> 1 numbers the rows of each
+// colour in order. Row v of colour c sits at position rb_pos(c, v >> 1):
+// blocks of 64 red rows then 64 black rows, so a wave reads one colour's
+// values contiguously and a position stays within 64 of its natural row
+// (the tile order's locality carries over). Positions of a scene of n rows
+// stay below 128 * ceil(ceil(n / 2) / 64) <= its tile range.
+__device__ __forceinline__ uint32_t rb_pos(uint32_t colour, uint32_t m) { return ((m >> 6) << 7) | (colour << 6) | (m & 63u); }
+
+// The row a thread of a red-black inner kernel serves: its tile-local index s
+// is a colour-split position; colour and natural row follow. False when the
+// position holds no row of the wanted colour (wave-uniform colour).
+struct RbRow {
+    uint32_t colour, v;  // v: natural scene-local row
+    bool row;
+};
+__device__ __forceinline__ RbRow rb_row(const SlotGeo &g, uint32_t s) {
+    RbRow r;
+    r.colour = (s >> 6) & 1u;
+    const uint32_t m = ((s >> 7) << 6) | (s & 63u);
+    const uint32_t v0 = 2u * m;
+    r.v = v0 < g.n && colour_of(g, v0) == r.colour ? v0 : v0 + 1u;
+    r.row = r.v < g.n;
+    return r;
+}
+
+// Offsets of a red-black inner kernel's row: natural byte offsets of the row
+// and its -axis neighbours (bond blocks, D, mask), colour-split byte offsets
+// of the six neighbours (vectors); absent neighbours point at the row itself.
+struct RbOffsets {
+    uint32_t ov, om[3];   // natural
+    uint32_t nb[6];       // colour-split, neighbour 2a (-axis a) and 2a + 1 (+axis a)
     bool lo[3], hi[3];
 };
 template <class S>
-__device__ __forceinline__ NbOffsets nb_offsets(const SlotGeo &g, uint32_t v, const uint32_t (&cc)[3]) {
-    NbOffsets o;
-    o.ov = v * uint32_t(sizeof(S));
+__device__ __forceinline__ RbOffsets rb_offsets(const SlotGeo &g, uint32_t v, uint32_t self_pos) {
+    RbOffsets o;
+    uint32_t cc[3];
+    coords(g, v, cc);
+    const uint32_t other = red_row(cc) ? 1u : 0u;
+    const uint32_t sz = uint32_t(sizeof(S));
+    o.ov = (g.voff + v) * sz;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        const uint32_t s = stride_of(g, a) * uint32_t(sizeof(S));
+        const uint32_t st = stride_of(g, a);
         o.lo[a] = cc[a] > 0;
         o.hi[a] = cc[a] + 1 < extent_of(g, a);
-        o.om[a] = o.lo[a] ? o.ov - s : o.ov;
-        o.op[a] = o.hi[a] ? o.ov + s : o.ov;
+        o.om[a] = o.lo[a] ? o.ov - st * sz : o.ov;
+        o.nb[2 * a] = (g.voff + (o.lo[a] ? rb_pos(other, (v - st) >> 1) : self_pos)) * sz;
+        o.nb[2 * a + 1] = (g.voff + (o.hi[a] ? rb_pos(other, (v + st) >> 1) : self_pos)) * sz;
     }
     return o;
 }
@@ -647,11 +689,11 @@ __device__ __forceinline__ void bond_sum(const S (&Km)[3][6], const S (&Kp)[3][6
     }
 }
 
-// The six bond blocks of a row: its own +axis blocks and the -axis
-// neighbours' (whatever they hold where the neighbour is absent; the vector
-// they multiply is zero there).
+// The six bond blocks of a row (natural layout): its own +axis blocks and the
+// -axis neighbours' (whatever they hold where the neighbour is absent; the
+// vector they multiply is zero there).
 template <class S>
-__device__ __forceinline__ void ld_blocks(const S *__restrict__ K, size_t R, const NbOffsets &o, S (&Km)[3][6], S (&Kp)[3][6]) {
+__device__ __forceinline__ void ld_blocks(const S *__restrict__ K, size_t R, const RbOffsets &o, S (&Km)[3][6], S (&Kp)[3][6]) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         const S *Ka = K + size_t(a) * 6 * R;
@@ -666,7 +708,8 @@ __device__ __forceinline__ void ld_blocks(const S *__restrict__ K, size_t R, con
 // The inner iteration's first kernel on the red rows: the inner check and
 // beta (the SpMV takes the same from the same partials next, and records
 // them), p = z + beta p_old formed at the six black neighbours, and
-// w = D_rr^-1 (mask * sum k p_nb) into the red rows of the Ap planes.
+// w = D_rr^-1 (mask * sum k p_nb) into the red rows of the Ap planes. Waves
+// of black positions only take part in the partial sums.
 template <class S, bool ONE>
 __global__ __launch_bounds__(kBlock) void pcg_rb_red(DevArena A, OneScene one, int odd) {
     Tile c;
@@ -683,31 +726,32 @@ __global__ __launch_bounds__(kBlock) void pcg_rb_red(DevArena A, OneScene one, i
     const int kinds[2] = {kRz, kRr};
     inner_load<2>(sums, A, c.g, kinds);
     __builtin_amdgcn_sched_barrier(0);
+    const RbRow rr = rb_row(c.g, c.local);
+    const bool wave_on = rr.colour == 0u;  // uniform over the wave
     const size_t R = A.R;
     const S *__restrict__ z = Inner<S>::z(A);
     const S *__restrict__ p_old = Inner<S>::p(A, odd);
-    const S *__restrict__ Di = Inner<S>::dinv(A);
-    const uint32_t v = c.row ? c.v : c.g.voff;
-    uint32_t cc[3];
-    coords(c.g, c.row ? c.local : 0u, cc);
-    const NbOffsets o = nb_offsets<S>(c.g, v, cc);
     S zz[6][3], pp[6][3], Dv[6], Kp[3][6], Km[3][6];
-    const uint8_t m = A.mask[v];
+    uint8_t m = 0;
+    RbOffsets o{};
+    if (wave_on) {
+        o = rb_offsets<S>(c.g, rr.row ? rr.v : 0u, c.local);
+        const S *__restrict__ Di = Inner<S>::dinv(A);
+        m = A.mask[o.ov / uint32_t(sizeof(S))];
 #pragma unroll
-    for (int q = 0; q < 6; ++q) Dv[q] = ld_row(Di + q * R, o.ov);
+        for (int q = 0; q < 6; ++q) Dv[q] = ld_row(Di + q * R, o.ov);
 #pragma unroll
-    for (int a = 0; a < 3; ++a)
+        for (int t = 0; t < 6; ++t)
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            zz[2 * a][q] = ld_row(z + q * R, o.om[a]);
-            pp[2 * a][q] = ld_row(p_old + q * R, o.om[a]);
-            zz[2 * a + 1][q] = ld_row(z + q * R, o.op[a]);
-            pp[2 * a + 1][q] = ld_row(p_old + q * R, o.op[a]);
-        }
-    ld_blocks<S>(Inner<S>::k(A), R, o, Km, Kp);
+            for (int q = 0; q < 3; ++q) {
+                zz[t][q] = ld_row(z + q * R, o.nb[t]);
+                pp[t][q] = ld_row(p_old + q * R, o.nb[t]);
+            }
+        ld_blocks<S>(Inner<S>::k(A), R, o, Km, Kp);
+    }
     __builtin_amdgcn_sched_barrier(0);
     block_sum_all<2>(sums, lds);
-    if (done_in) return;
+    if (done_in || !wave_on) return;
     S beta;
     if (!inner_head<S>(st, prm, sums, k, seg, rr0_s, bb, rho_s, false, odd, beta)) return;
     S u[6][3];
@@ -718,23 +762,25 @@ __global__ __launch_bounds__(kBlock) void pcg_rb_red(DevArena A, OneScene one, i
             u[2 * a][q] = o.lo[a] ? fmaS<S>(beta, pp[2 * a][q], zz[2 * a][q]) : S(0);
             u[2 * a + 1][q] = o.hi[a] ? fmaS<S>(beta, pp[2 * a + 1][q], zz[2 * a + 1][q]) : S(0);
         }
-    S s[3];
-    bond_sum<S>(Km, Kp, u, s);
-    if (!(m & 1)) s[0] = S(0);
-    if (!(m & 2)) s[1] = S(0);
-    if (!(m & 4)) s[2] = S(0);
-    if (c.row && red_row(cc)) {
+    S sv[3];
+    bond_sum<S>(Km, Kp, u, sv);
+    if (!(m & 1)) sv[0] = S(0);
+    if (!(m & 2)) sv[1] = S(0);
+    if (!(m & 4)) sv[2] = S(0);
+    if (rr.row) {
         S *__restrict__ w = Inner<S>::ap(A);
-        st_row(w, o.ov, fmaS<S>(Dv[2], s[2], fmaS<S>(Dv[1], s[1], Dv[0] * s[0])));
-        st_row(w + R, o.ov, fmaS<S>(Dv[4], s[2], fmaS<S>(Dv[3], s[1], Dv[1] * s[0])));
-        st_row(w + 2 * R, o.ov, fmaS<S>(Dv[5], s[2], fmaS<S>(Dv[4], s[1], Dv[2] * s[0])));
+        const uint32_t os = c.v * uint32_t(sizeof(S));  // own colour-split position
+        st_row(w, os, fmaS<S>(Dv[2], sv[2], fmaS<S>(Dv[1], sv[1], Dv[0] * sv[0])));
+        st_row(w + R, os, fmaS<S>(Dv[4], sv[2], fmaS<S>(Dv[3], sv[1], Dv[1] * sv[0])));
+        st_row(w + 2 * R, os, fmaS<S>(Dv[5], sv[2], fmaS<S>(Dv[4], sv[1], Dv[2] * sv[0])));
     }
 }
 
 // The SpMV of the reduced system on the black rows: the inner check and
 // beta (recorded by the lead thread), p = z + beta p_old on the row,
-// y = D p - mask * sum k w_nb with w from the red neighbours' Ap rows; stores
-// p and y = S p on the black rows, partials of p.Ap.
+// y = D p - mask * sum k w_nb with w from the red neighbours' Ap positions;
+// stores p and y = S p at the row's position, partials of p.Ap. Waves of red
+// positions only take part in the partial sums.
 template <class S>
 __device__ __forceinline__ void spmv_black(const DevArena &A, const Tile &c, int odd, double (*lds)[kWaves]) {
     SlotState *st = A.st + c.slot;
@@ -748,67 +794,70 @@ __device__ __forceinline__ void spmv_black(const DevArena &A, const Tile &c, int
     const int kinds[2] = {kRz, kRr};
     inner_load<2>(sums, A, c.g, kinds);
     __builtin_amdgcn_sched_barrier(0);
+    const RbRow rr = rb_row(c.g, c.local);
+    const bool wave_on = rr.colour == 1u;
     const size_t R = A.R;
-    const S *__restrict__ z = Inner<S>::z(A);
-    const S *__restrict__ p_old = Inner<S>::p(A, odd);
-    const S *__restrict__ Dp = Inner<S>::d(A);
+    const uint32_t os = (rr.row ? c.v : c.g.voff) * uint32_t(sizeof(S));  // own colour-split position
     const S *w = Inner<S>::ap(A);
-    const uint32_t v = c.row ? c.v : c.g.voff;
-    uint32_t cc[3];
-    coords(c.g, c.row ? c.local : 0u, cc);
-    const NbOffsets o = nb_offsets<S>(c.g, v, cc);
     S zz[3], pp[3], ww[6][3], Dv[6], Kp[3][6], Km[3][6];
-    const uint8_t m = A.mask[v];
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-        zz[q] = ld_row(z + q * R, o.ov);
-        pp[q] = ld_row(p_old + q * R, o.ov);
-    }
-#pragma unroll
-    for (int q = 0; q < 6; ++q) Dv[q] = ld_row(Dp + q * R, o.ov);
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
+    uint8_t m = 0;
+    RbOffsets o{};
+    if (wave_on) {
+        o = rb_offsets<S>(c.g, rr.row ? rr.v : 0u, rr.row ? c.local : 0u);
+        const S *__restrict__ z = Inner<S>::z(A);
+        const S *__restrict__ p_old = Inner<S>::p(A, odd);
+        const S *__restrict__ Dp = Inner<S>::d(A);
+        m = A.mask[o.ov / uint32_t(sizeof(S))];
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
-            ww[2 * a][q] = ld_row(w + q * R, o.om[a]);
-            ww[2 * a + 1][q] = ld_row(w + q * R, o.op[a]);
+            zz[q] = ld_row(z + q * R, os);
+            pp[q] = ld_row(p_old + q * R, os);
         }
-    ld_blocks<S>(Inner<S>::k(A), R, o, Km, Kp);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) Dv[q] = ld_row(Dp + q * R, o.ov);
+#pragma unroll
+        for (int t = 0; t < 6; ++t)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) ww[t][q] = ld_row(w + q * R, o.nb[t]);
+        ld_blocks<S>(Inner<S>::k(A), R, o, Km, Kp);
+    }
     __builtin_amdgcn_sched_barrier(0);
     block_sum_all<2>(sums, lds);
     if (done_in) return;
     S beta;
     if (!inner_head<S>(st, prm, sums, k, seg, rr0_s, bb, rho_s, c.t == c.g.tile0 && threadIdx.x == 0, odd, beta)) return;
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            if (!o.lo[a]) ww[2 * a][q] = S(0);
-            if (!o.hi[a]) ww[2 * a + 1][q] = S(0);
-        }
-    S p[3], s[3];
-#pragma unroll
-    for (int q = 0; q < 3; ++q) p[q] = fmaS<S>(beta, pp[q], zz[q]);
-    bond_sum<S>(Km, Kp, ww, s);
-    S y0 = fmaS<S>(Dv[2], p[2], fmaS<S>(Dv[1], p[1], Dv[0] * p[0]));
-    S y1 = fmaS<S>(Dv[4], p[2], fmaS<S>(Dv[3], p[1], Dv[1] * p[0]));
-    S y2 = fmaS<S>(Dv[5], p[2], fmaS<S>(Dv[4], p[1], Dv[2] * p[0]));
-    if (m & 1) y0 -= s[0];
-    if (m & 2) y1 -= s[1];
-    if (m & 4) y2 -= s[2];
     double part[1] = {0.0};
-    if (c.row && !red_row(cc)) {
-        S *__restrict__ p_new = Inner<S>::p(A, !odd);
-        S *ap = Inner<S>::ap(A);
-        st_row(p_new, o.ov, p[0]);
-        st_row(p_new + R, o.ov, p[1]);
-        st_row(p_new + 2 * R, o.ov, p[2]);
-        st_row(ap, o.ov, y0);
-        st_row(ap + R, o.ov, y1);
-        st_row(ap + 2 * R, o.ov, y2);
-        part[0] = double(p[0]) * double(y0);
-        part[0] = __builtin_fma(double(p[1]), double(y1), part[0]);
-        part[0] = __builtin_fma(double(p[2]), double(y2), part[0]);
+    if (wave_on) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                if (!o.lo[a]) ww[2 * a][q] = S(0);
+                if (!o.hi[a]) ww[2 * a + 1][q] = S(0);
+            }
+        S p[3], sv[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) p[q] = fmaS<S>(beta, pp[q], zz[q]);
+        bond_sum<S>(Km, Kp, ww, sv);
+        S y0 = fmaS<S>(Dv[2], p[2], fmaS<S>(Dv[1], p[1], Dv[0] * p[0]));
+        S y1 = fmaS<S>(Dv[4], p[2], fmaS<S>(Dv[3], p[1], Dv[1] * p[0]));
+        S y2 = fmaS<S>(Dv[5], p[2], fmaS<S>(Dv[4], p[1], Dv[2] * p[0]));
+        if (m & 1) y0 -= sv[0];
+        if (m & 2) y1 -= sv[1];
+        if (m & 4) y2 -= sv[2];
+        if (rr.row) {
+            S *__restrict__ p_new = Inner<S>::p(A, !odd);
+            S *ap = Inner<S>::ap(A);
+            st_row(p_new, os, p[0]);
+            st_row(p_new + R, os, p[1]);
+            st_row(p_new + 2 * R, os, p[2]);
+            st_row(ap, os, y0);
+            st_row(ap + R, os, y1);
+            st_row(ap + 2 * R, os, y2);
+            part[0] = double(p[0]) * double(y0);
+            part[0] = __builtin_fma(double(p[1]), double(y1), part[0]);
+            part[0] = __builtin_fma(double(p[2]), double(y2), part[0]);
+        }
     }
     const int out[1] = {kPap};
     tile_store<1>(part, A, c.t, out, lds);
@@ -952,13 +1001,18 @@ template <class S, bool ONE, bool RB>
 __global__ __launch_bounds__(kBlock) void pcg_update(DevArena A, OneScene one, int odd) {
     Tile c;
     if (!inner_tile<S, ONE>(A, one, c)) return;
-    // A red-black scene updates its black rows only.
-    bool act = c.row;
+    // A red-black scene updates its black rows at their colour-split
+    // positions (the thread's own index); D^-1 stays in the natural layout.
+    // Waves of red positions only take part in the partial sums.
+    bool act = c.row, load = true;
+    uint32_t v = c.row ? c.v : c.g.voff, vd = v;
     if constexpr (RB) {
         if (ONE || (c.bits & B_RB)) {
-            uint32_t cc[3];
-            coords(c.g, c.local, cc);
-            act = act && !red_row(cc);
+            const RbRow rr = rb_row(c.g, c.local);
+            load = rr.colour == 1u;
+            act = load && rr.row;
+            v = c.v;
+            vd = c.g.voff + (rr.row ? rr.v : 0u);
         }
     }
     __shared__ double lds[2][kWaves];
@@ -972,24 +1026,25 @@ __global__ __launch_bounds__(kBlock) void pcg_update(DevArena A, OneScene one, i
     inner_load<1>(pap, A, c.g, kinds);
     __builtin_amdgcn_sched_barrier(0);
     const size_t R = A.R;
-    const uint32_t v = c.row ? c.v : c.g.voff;
     const S *__restrict__ p = Inner<S>::p(A, !odd);
     const S *__restrict__ ap = Inner<S>::ap(A);
     S *__restrict__ d = Inner<S>::dv(A);
     S *__restrict__ r = Inner<S>::r(A);
     S *__restrict__ z = Inner<S>::z(A);
     const S *__restrict__ dinv = Inner<S>::dinv(A);
-    const uint32_t ov = v * uint32_t(sizeof(S));
-    S pv[3], av[3], rv[3], dv[3], Di[6];
+    const uint32_t ov = v * uint32_t(sizeof(S)), ovd = vd * uint32_t(sizeof(S));
+    S pv[3] = {}, av[3] = {}, rv[3] = {}, dv[3] = {}, Di[6] = {};
+    if (load) {
 #pragma unroll
-    for (int q = 0; q < 3; ++q) {
-        pv[q] = ld_row(p + q * R, ov);
-        av[q] = ld_row(ap + q * R, ov);
-        rv[q] = ld_row(r + q * R, ov);
-        dv[q] = ld_row(d + q * R, ov);
+        for (int q = 0; q < 3; ++q) {
+            pv[q] = ld_row(p + q * R, ov);
+            av[q] = ld_row(ap + q * R, ov);
+            rv[q] = ld_row(r + q * R, ov);
+            dv[q] = ld_row(d + q * R, ov);
+        }
+#pragma unroll
+        for (int q = 0; q < 6; ++q) Di[q] = ld_row(dinv + q * R, ovd);
     }
-#pragma unroll
-    for (int q = 0; q < 6; ++q) Di[q] = ld_row(dinv + q * R, ov);
     __builtin_amdgcn_sched_barrier(0);
     block_sum_all<1>(pap, lds);
     if (done_in) return;
@@ -1048,10 +1103,12 @@ __global__ __launch_bounds__(kBlock) void refine_red(DevArena A, int forced) {
     const double *__restrict__ x_old = warm ? A.xw : (xpar ? A.xb : A.xa);
     double *__restrict__ x_new = xpar ? A.xa : A.xb;
     const S *__restrict__ d = Inner<S>::dv(A);
+    // x at black row u (arena row); d sits at u's colour-split position
     auto xval = [&](uint32_t u, double (&o)[3]) {
+        const uint32_t du = c.g.voff + rb_pos(1u, (u - c.g.voff) >> 1);
 #pragma unroll
         for (int q = 0; q < 3; ++q)
-            o[q] = first ? (warm ? x_old[q * R + u] : 0.0) : x_old[q * R + u] + (use_d ? double(d[q * R + u]) : 0.0);
+            o[q] = first ? (warm ? x_old[q * R + u] : 0.0) : x_old[q * R + u] + (use_d ? double(d[q * R + du]) : 0.0);
     };
     const uint32_t v = c.v;
     uint32_t cc[3];
@@ -1093,8 +1150,9 @@ __global__ __launch_bounds__(kBlock) void refine_red(DevArena A, int forced) {
 // SpMV reads it. r64 = b - A64 x_new replaces the inner residual: r = r64 in
 // the inner precision, z = Dinv r. Partials of ||r64||^2, ||b||^2, r.z, r.r.
 // x_new goes to the other ping-pong buffer. A red-black scene's x_new is
-// already complete (refine_red); its red rows take r = z = p = 0, so the
-// inner partials cover the black rows while ||r64|| is the full system's.
+// already complete (refine_red); its inner vectors hold the black rows (at
+// their colour-split positions), so the inner partials cover the black rows
+// while ||r64|| is the full system's.
 template <class S>
 __global__ __launch_bounds__(kBlock) void refine_residual(DevArena A, int forced) {
     Tile c;
@@ -1152,24 +1210,29 @@ __global__ __launch_bounds__(kBlock) void refine_residual(DevArena A, int forced
         S *__restrict__ z = Inner<S>::z(A);
         S *__restrict__ p0 = Inner<S>::p(A, 0);
         const S *__restrict__ p1 = Inner<S>::p(A, 1);
+        // inner vectors of a red-black scene: black rows at their colour-split
+        // positions, none for red rows
+        const uint32_t pv = rb ? c.g.voff + rb_pos(1u, c.local >> 1) : v;
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
             const double b = A.rhs[q * R + v];
             const double r64 = f[q] * (b - (yv[q] - acc[q]));
             if (!rb) x_new[q * R + v] = xv[q];
             rs[q] = red ? S(0) : S(r64);
-            r[q * R + v] = rs[q];
-            if (first || red)
-                p0[q * R + v] = S(0);
-            else if (pbuf)
-                p0[q * R + v] = p1[q * R + v];
+            if (!red) {
+                r[q * R + pv] = rs[q];
+                if (first)
+                    p0[q * R + pv] = S(0);
+                else if (pbuf)
+                    p0[q * R + pv] = p1[q * R + pv];
+            }
             part[0] = __builtin_fma(r64, r64, part[0]);
             part[1] = __builtin_fma(b, b, part[1]);
         }
         symv<S>(Inner<S>::dinv(A), R, v, rs, zs);
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
-            z[q * R + v] = zs[q];
+            if (!red) z[q * R + pv] = zs[q];
             part[2] = __builtin_fma(double(rs[q]), double(zs[q]), part[2]);
             part[3] = __builtin_fma(double(rs[q]), double(rs[q]), part[3]);
         }

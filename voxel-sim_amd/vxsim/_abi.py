@@ -110,6 +110,7 @@ class BatchSceneReport(ctypes.Structure):
 SIGNATURES = [
     ("vx_last_error", ctypes.c_char_p, []),
     ("vx_init", ctypes.c_int32, []),
+    ("vx_copy_bandwidth", ctypes.c_int32, [ctypes.c_uint64, ctypes.c_uint32, c_dp]),
     ("vx_solve_pcg", ctypes.c_int32,
      [c_fp, c_u32p, c_fp, c_fp, c_fp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float, c_fp, c_u32p, c_fp]),
     ("vx_solve_pcg_f64", ctypes.c_int32,
