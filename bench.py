@@ -125,13 +125,14 @@ def measured_traffic():
         return None
 
 
-def batched(steps: int = 1, count: int = 256):
+def batched(steps: int = 1, count: int = 256, preconditioner: int = 0):
     """256 independent boxes (edges U[8, 48], one face fixed, random initial
     velocities) on one tile arena, `steps` steps each, all scenes advanced
     together tick by tick. Returns DOF-PCG-iters/s over the timed run."""
     from vxsim import batch, grid, scenes
     boxes = scenes.batch_boxes(count=count, seed=3, lo=8, hi=48)
-    sp = grid.solve_params(tolerance=1e-6, inner_tolerance=INNER_TOL, max_inner=8000, max_outer=12)
+    sp = grid.solve_params(tolerance=1e-6, inner_tolerance=INNER_TOL, max_inner=8000, max_outer=12,
+                           preconditioner=preconditioner)
     tiles = sum(-(-s.n // batch.TILE) for s in boxes)
     out = {}
     with batch.Batch(tiles * batch.TILE, max_scenes=count) as B:
@@ -157,6 +158,7 @@ def batched(steps: int = 1, count: int = 256):
                            % (count, sum(s.n for s in boxes), sum(r["active_rows"] for r in reps), steps),
                "dof_iters_per_s": dof_iters / wall, "wall_s": wall, "ticks": ticks, "ms_per_tick": 1000 * wall / ticks,
                "admit_s": t_admit, "failed": len(bad),
+               "inner_iterations": sum(r["inner_iterations"] for r in reps),
                "max_rel_residual": max(r["last_relative_residual"] for r in reps)}
     return out
 
@@ -318,7 +320,7 @@ def large_bench(steps: int, warmup: int, compare: bool = True):
     print(json.dumps(line))
 
 
-def cut_bench(steps: int, dims=(192, 192, 96)):
+def cut_bench(steps: int, dims=(192, 192, 96), preconditioner: int = 0):
     """The cutting scene with fracture: the 192x192x96 wood block, the blade's
     40x5 edge driven 5 mm per 1 ms step through per-step axis targets, up to
     six corotation passes and the fracture pass every step (broken bonds leave
@@ -328,7 +330,8 @@ def cut_bench(steps: int, dims=(192, 192, 96)):
     blade = (40, 5, 20)
     scene = scenes.wood_block_blade(dims=dims, blade=blade)
     cfg = scene.config
-    sp = grid.solve_params(tolerance=1e-6, inner_tolerance=INNER_TOL, max_inner=40000, max_outer=12)
+    sp = grid.solve_params(tolerance=1e-6, inner_tolerance=INNER_TOL, max_inner=40000, max_outer=12,
+                           preconditioner=preconditioner)
     reps = []
     walls = []
     with grid.GridScene(scene) as gs:
@@ -349,6 +352,8 @@ def cut_bench(steps: int, dims=(192, 192, 96)):
             "workload": "%s with fracture: %d REVs, blade edge %dx%d REVs at 5 m/s, %d steps of 1 ms, <=%d passes, "
                         "rel. residual 1e-6" % (scene.name, scene.n, blade[0], blade[1], steps,
                                                cfg.corotation_iterations),
+            "preconditioner": "red_black" if preconditioner else "block_jacobi",
+            "mean_ms_per_step": 1000 * sum(walls) / len(walls),
             "ms_per_step": [round(1000 * w, 2) for w in walls], "solves": [r["solves"] for r in reps],
             "inner_iterations": [r["inner_iterations"] for r in reps],
             "bonds_broken": [int(r["fracture"]["bonds_broken"]) for r in reps],
@@ -399,6 +404,7 @@ def main():
     ap.add_argument("--serve", action="store_true", help="served request stream instead of the step benchmark")
     ap.add_argument("--large", action="store_true", help="192x192x96 block instead of the 64^3 box")
     ap.add_argument("--cut", action="store_true", help="192x192x96 block cut by the blade with fracture")
+    ap.add_argument("--red-black", action="store_true", help="--cut with the red-black preconditioner")
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--serve-revs", type=str, default="1e3,2e5")
@@ -414,7 +420,7 @@ def main():
         pmc_driver(args.large)
         return
     if args.cut:
-        cut_bench(args.steps)
+        cut_bench(args.steps, preconditioner=1 if args.red_black else 0)
         return
     if args.large:
         large_bench(args.steps, args.warmup, compare=not args.no_compare)
@@ -540,6 +546,9 @@ def main():
         extra["red_black_step"] = red_black_steps(scene, step_kw, args.steps, args.warmup, ms_per_step)
     if not args.no_batch:
         extra["batched"] = batched()
+        rb = batched(preconditioner=1)
+        rb["time_vs_block_jacobi"] = extra["batched"]["wall_s"] / rb["wall_s"]
+        extra["batched_red_black"] = rb
 
     cpu = None
     if not args.no_cpu_baseline:

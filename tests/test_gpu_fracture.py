@@ -17,10 +17,10 @@ STEPS = 3
 ENERGIES = ("plastic_heat", "stored_plastic", "surface_work", "damage_heat", "released")
 
 
-def run_device(s, steps, fracture=FractureConfig(), tol=1e-11):
+def run_device(s, steps, fracture=FractureConfig(), tol=1e-11, preconditioner=0):
     from vxsim import grid
     cfg = s.config
-    sp = grid.solve_params(tolerance=tol, inner_tolerance=1e-3, max_inner=20000)
+    sp = grid.solve_params(tolerance=tol, inner_tolerance=1e-3, max_inner=20000, preconditioner=preconditioner)
     reps = []
     with grid.GridScene(s) as gs:
         for _ in range(steps):
@@ -44,8 +44,8 @@ def run_oracle(O, s, steps):
     return reps, os_, hist
 
 
-def check_parity(O, s, steps=STEPS):
-    reps_d, out, bonds = run_device(s, steps)
+def check_parity(O, s, steps=STEPS, preconditioner=0):
+    reps_d, out, bonds = run_device(s, steps, preconditioner=preconditioner)
     reps_o, os_, hist = run_oracle(O, s, steps)
     broken_total = 0
     # Energies against the run's energy scale: a later step's small residual
@@ -101,11 +101,17 @@ def test_fracture_every_substep_matches_oracle(gpu, O, scenes):
     assert broken > 50
 
 
-def test_pulled_mixed_box_matches_oracle(gpu, O, scenes):
+# vx_solve_params.preconditioner: 0 block-Jacobi, 1 red-black reduced system
+PRECONDITIONERS = {"block_jacobi": 0, "red_black": 1}
+
+
+@pytest.mark.parametrize("precond", list(PRECONDITIONERS))
+def test_pulled_mixed_box_matches_oracle(gpu, O, scenes, precond):
     """Brittle breaks, cross-grain wood breaks and iron / along-grain wood
-    plastic flow in one 12^3 box, three steps."""
+    plastic flow in one 12^3 box, three steps; the same broken-bond set with
+    either preconditioner."""
     s = scenes.pulled_mixed((12, 12, 12), 4, 4e-4)
-    broken, hist = check_parity(O, s)
+    broken, hist = check_parity(O, s, preconditioner=PRECONDITIONERS[precond])
     assert broken > 500 and np.count_nonzero(hist.equivalent_plastic_strain) > 100
 
 
@@ -122,7 +128,8 @@ def exempt(os_, hist, bonds):
     return len(diff)
 
 
-def test_blade_cut_matches_oracle(gpu, O, scenes):
+@pytest.mark.parametrize("precond", list(PRECONDITIONERS))
+def test_blade_cut_matches_oracle(gpu, O, scenes, precond):
     """The cutting block at reduced size (24x24x16 wood, grain Z, blade edge
     8x3 REVs driven 5 mm per 1 ms step): per-step axis targets through
     vx_scene_set_targets_and_loads, six corotation passes, fracture every
@@ -132,7 +139,8 @@ def test_blade_cut_matches_oracle(gpu, O, scenes):
     blade, steps = (8, 3, 5), 2
     s = scenes.wood_block_blade(dims=(24, 24, 16), blade=blade)
     cfg = s.config
-    sp = grid.solve_params(tolerance=1e-11, inner_tolerance=1e-3, max_inner=40000)
+    sp = grid.solve_params(tolerance=1e-11, inner_tolerance=1e-3, max_inner=40000,
+                           preconditioner=PRECONDITIONERS[precond])
     reps_d = []
     with grid.GridScene(s) as gs:
         for k in range(steps):

@@ -154,3 +154,21 @@ def test_batch_mixes_preconditioners(gpu, scenes):
         ref, reps = alone(s, 2, tolerance=1e-9, **kw)
         assert r["inner_iterations"] == sum(x["inner_iterations"] for x in reps)
         assert_same(out, ref)
+
+
+def test_batch_ticks_follow_the_iterations(gpu, scenes):
+    """A red-black scene's inner chunks pipeline like block-Jacobi's (the next
+    chunk is issued while the previous one runs), so a batch that needs half
+    the iterations needs markedly fewer ticks."""
+    from vxsim import batch, grid
+    from test_gpu_batch import params
+    boxes = scenes.batch_boxes(count=4, seed=5, lo=16, hi=24)
+    total = sum(-(-s.n // batch.TILE) * batch.TILE for s in boxes)
+    ticks, iters = {}, {}
+    for name, pc in (("bj", grid.BLOCK_JACOBI), ("rb", grid.RED_BLACK)):
+        with batch.Batch(total, max_scenes=len(boxes)) as B:
+            slots = [B.admit(s, 1, params(grid, s, preconditioner=pc)) for s in boxes]
+            ticks[name] = B.run()
+            iters[name] = sum(B.report(k)["inner_iterations"] for k in slots)
+    assert iters["rb"] <= 0.6 * iters["bj"]
+    assert ticks["rb"] <= 0.8 * ticks["bj"], (ticks, iters)

@@ -2171,7 +2171,8 @@ SlotParams device_params(double h, const double *g, const vx_solve_params *s, co
 }
 
 bool solve_ok(const vx_solve_params &s) {
-    return s.max_inner > 0 && s.tolerance > 0.0 && s.mode <= VX_SOLVE_F64 && s.preconditioner <= VX_PRECOND_RED_BLACK;
+    return s.max_inner > 0 && s.tolerance > 0.0 && s.mode <= VX_SOLVE_F64 && s.preconditioner <= VX_PRECOND_RED_BLACK &&
+           s.reserved == 0;
 }
 
 int32_t admit(vx_batch *b, const vx_scene_desc *d, const vx_step_params *p, uint32_t steps, bool manual, uint32_t *out_slot) {
@@ -2340,6 +2341,7 @@ extern "C" int32_t vx_batch_admit(vx_batch *b, const vx_scene_desc *d, const vx_
 // inner chunk continues with another chunk (which does nothing if its solve
 // has ended in the meantime); a scene whose tick in flight changes its phase
 // sits this tick out.
+constexpr uint32_t kChunkBits = B_INNER | B_F64 | B_RB;  // bits of a pure inner chunk
 extern "C" int32_t vx_batch_tick(vx_batch *b, uint32_t *out_running) {
     clear_error();
     if (!b) return fail(VX_ERR_INVALID_ARGUMENT, "null batch");
@@ -2350,7 +2352,7 @@ extern "C" int32_t vx_batch_tick(vx_batch *b, uint32_t *out_running) {
     for (uint32_t i = 0; i < b->max_slots; ++i) {
         const SlotHost &h = b->slots[i];
         uint32_t bits = 0;
-        if (h.used && !h.manual && !(b->inflight[i] & ~uint32_t(B_INNER | B_F64))) bits = next_bits(h, b->known[i]);
+        if (h.used && !h.manual && !(b->inflight[i] & ~kChunkBits)) bits = next_bits(h, b->known[i]);
         f.bits[i] = bits;
         launched += bits ? 1 : 0;
     }
