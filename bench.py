@@ -164,14 +164,15 @@ def batched(steps: int = 1, count: int = 256, preconditioner: int = 0):
 
 
 def serve_bench(rate: float, seconds: float, rev_lo: float, rev_hi: float, steps: int, budget_gb: float,
-                compare: bool = True):
+                compare: bool = True, preconditioner: int = 0):
     """Served request stream: Poisson arrivals, boxes log-uniform in REVs,
     `steps` warm-started steps each, admission against an HBM budget.
     Prints one JSON line with p50/p99 latency and aggregate DOF-PCG-iters/s,
     beside the plain FP64 solve handling the same requests one at a time."""
-    from vxsim import batch, serve
+    from vxsim import batch, grid, serve
     t = time.perf_counter()
-    reqs = serve.request_stream(rate, seconds, rev_lo, rev_hi, steps, seed=4)
+    reqs = serve.request_stream(rate, seconds, rev_lo, rev_hi, steps, seed=4,
+                                solve=grid.solve_params(tolerance=1e-6, warm_start=True, preconditioner=preconditioner))
     gen_s = time.perf_counter() - t
     budget = int(budget_gb * (1 << 30))
     last = [time.perf_counter()]
@@ -196,6 +197,7 @@ def serve_bench(rate: float, seconds: float, rev_lo: float, rev_hi: float, steps
     line = {
         "metric": "served p50/p99 solve latency", "unit": "s", "higher_is_better": False,
         "p50": lat[50], "p99": lat[99], "requests": len(reqs), "done": len(done),
+        "preconditioner": "red_black" if preconditioner else "block_jacobi",
         "dof_iters_per_s": dof_iters / wall, "wall_s": wall, "ticks": ticks,
         "offered": {"rate_per_s": rate, "seconds": seconds, "revs_log_uniform": [rev_lo, rev_hi],
                     "mean_revs": float(np.mean(revs)), "steps_per_request": steps,
@@ -404,7 +406,7 @@ def main():
     ap.add_argument("--serve", action="store_true", help="served request stream instead of the step benchmark")
     ap.add_argument("--large", action="store_true", help="192x192x96 block instead of the 64^3 box")
     ap.add_argument("--cut", action="store_true", help="192x192x96 block cut by the blade with fracture")
-    ap.add_argument("--red-black", action="store_true", help="--cut with the red-black preconditioner")
+    ap.add_argument("--red-black", action="store_true", help="--cut / --serve with the red-black preconditioner")
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--serve-revs", type=str, default="1e3,2e5")
@@ -428,7 +430,7 @@ def main():
     if args.serve:
         lo, hi = (float(x) for x in args.serve_revs.split(","))
         serve_bench(args.serve_rate, args.serve_seconds, lo, hi, args.serve_steps, args.serve_budget_gb,
-                    compare=not args.no_compare)
+                    compare=not args.no_compare, preconditioner=1 if args.red_black else 0)
         return
 
     from vxsim import grid, scenes, solver
