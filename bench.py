@@ -371,13 +371,15 @@ def cut_bench(steps: int, dims=(192, 192, 96), preconditioner: int = 0):
     print(json.dumps(line))
 
 
-def pmc_driver(large: bool = False):
+def pmc_driver(large: bool = False, red_black: bool = False):
     """Short run for the rocprofv3 counter passes: 20 launches of the FP32
-    bond SpMV on the assembled 64^3 system (or the 192x192x96 block), then 3
-    copies of 1 GiB (the calibration of FETCH_SIZE / WRITE_SIZE against known
-    bytes)."""
-    import torch
-    from vxsim import grid, scenes
+    bond SpMV on the assembled 64^3 system (or the 192x192x96 block) — with
+    red_black, instead 40 iterations of the red-black solve (pcg_rb_red, its
+    black SpMV and update) on the same system — then 4 copies of 1 GiB by the
+    library's copy kernel (the calibration of FETCH_SIZE / WRITE_SIZE against
+    known bytes)."""
+    import ctypes
+    from vxsim import grid, lib, scenes
     scene = scenes.wood_block_blade() if large else scenes.large_64cube()
     cfg = scene.config
     with grid.GridScene(scene) as gs:
@@ -385,13 +387,18 @@ def pmc_driver(large: bool = False):
         gs.predict(h, cfg.gravity)
         gs.fit_rotations()
         gs.assemble(h)
-        us = gs.time_spmv(20)
-    src = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
-    dst = torch.empty_like(src)
-    for _ in range(3):
-        dst.copy_(src)
-    torch.cuda.synchronize()
-    print(json.dumps({"spmv_us": us, "copy_bytes": COPY_BYTES, "spmv_bytes": SPMV_BYTES_PER_ROW * scene.n}))
+        if red_black:
+            rep = gs.solve(grid.solve_params(tolerance=1e-14, inner_tolerance=1e-14, max_inner=40, max_outer=1,
+                                             preconditioner=grid.RED_BLACK), raise_on_error=False)
+            us = None
+            extra = {"inner_iterations": rep["inner_iterations"]}
+        else:
+            us = gs.time_spmv(20)
+            extra = {}
+    out_us = ctypes.c_double()
+    lib().vx_copy_bandwidth(COPY_BYTES, 3, ctypes.byref(out_us))
+    print(json.dumps(dict({"spmv_us": us, "copy_bytes": COPY_BYTES, "rows": scene.n,
+                           "spmv_bytes": SPMV_BYTES_PER_ROW * scene.n}, **extra)))
 
 
 def main():
@@ -419,7 +426,7 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     if args.pmc_driver:
-        pmc_driver(args.large)
+        pmc_driver(args.large, red_black=args.red_black)
         return
     if args.cut:
         cut_bench(args.steps, preconditioner=1 if args.red_black else 0)

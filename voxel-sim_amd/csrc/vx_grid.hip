@@ -504,6 +504,12 @@ struct OneScene {
     uint32_t rb;  // the scene's bits carry B_RB
 };
 
+// EARLY (template argument of the inner kernels): test the scene's done flag
+// before the row loads, so the launches left in a chunk after the inner
+// solve has ended fetch nothing. Chosen for arenas holding scenes of more
+// than kBigTiles tiles, where the saved traffic outweighs the wait for the
+// flag; at smaller sizes that wait lengthens every latency-bound launch.
+
 // Planes of fewer than 16 tiles keep the plain order (q = 0): their z
 // neighbours are already within a few tiles.
 inline OneScene make_one(const SlotGeo &g) {
@@ -710,13 +716,16 @@ __device__ __forceinline__ void ld_blocks(const S *__restrict__ K, size_t R, con
 // them), p = z + beta p_old formed at the six black neighbours, and
 // w = D_rr^-1 (mask * sum k p_nb) into the red rows of the Ap planes. Waves
 // of black positions only take part in the partial sums.
-template <class S, bool ONE>
+template <class S, bool ONE, bool EARLY>
 __global__ __launch_bounds__(kBlock) void pcg_rb_red(DevArena A, OneScene one, int odd) {
     Tile c;
     if (!inner_tile<S, ONE>(A, one, c) || !(c.bits & B_RB)) return;
     __shared__ double lds[2][kWaves];
     SlotState *st = A.st + c.slot;
     const uint32_t done_in = st->done;
+    if constexpr (EARLY) {
+        if (done_in) return;  // the scene's inner solve has ended: no row loads
+    }
     const uint32_t k = st->k_spmv;
     const uint32_t seg = st->seg;
     const double rho_s[2] = {st->rho[0], st->rho[1]};
@@ -781,10 +790,13 @@ __global__ __launch_bounds__(kBlock) void pcg_rb_red(DevArena A, OneScene one, i
 // y = D p - mask * sum k w_nb with w from the red neighbours' Ap positions;
 // stores p and y = S p at the row's position, partials of p.Ap. Waves of red
 // positions only take part in the partial sums.
-template <class S>
+template <class S, bool EARLY>
 __device__ __forceinline__ void spmv_black(const DevArena &A, const Tile &c, int odd, double (*lds)[kWaves]) {
     SlotState *st = A.st + c.slot;
     const uint32_t done_in = st->done;
+    if constexpr (EARLY) {
+        if (done_in) return;  // the scene's inner solve has ended: no row loads
+    }
     const uint32_t k = st->k_spmv;
     const uint32_t seg = st->seg;
     const double rho_s[2] = {st->rho[0], st->rho[1]};
@@ -872,7 +884,7 @@ __device__ __forceinline__ void spmv_black(const DevArena &A, const Tile &c, int
 // load (zeros where a neighbour is absent), then the partial sums. The row
 // loads do not depend on beta, so they are in flight while the workgroup
 // reduces the partials (LDS-only barriers do not wait for them).
-template <class S, bool ONE, bool RB>
+template <class S, bool ONE, bool RB, bool EARLY>
 __global__ __launch_bounds__(kBlock) void pcg_bond_spmv(DevArena A, OneScene one, int odd) {
     Tile c;
     if (!inner_tile<S, ONE>(A, one, c)) return;
@@ -880,12 +892,15 @@ __global__ __launch_bounds__(kBlock) void pcg_bond_spmv(DevArena A, OneScene one
     // RB: the launch may hold red-black scenes (all of them when ONE).
     if constexpr (RB) {
         if (ONE || (c.bits & B_RB)) {
-            spmv_black<S>(A, c, odd, lds);
+            spmv_black<S, EARLY>(A, c, odd, lds);
             return;
         }
     }
     SlotState *st = A.st + c.slot;
     const uint32_t done_in = st->done;
+    if constexpr (EARLY) {
+        if (done_in) return;  // the scene's inner solve has ended: no row loads
+    }
     const uint32_t k = st->k_spmv;
     const uint32_t seg = st->seg;
     const double rho_s[2] = {st->rho[0], st->rho[1]};
@@ -997,7 +1012,7 @@ __global__ __launch_bounds__(kBlock) void pcg_bond_spmv(DevArena A, OneScene one
 // segment);
 // r -= alpha Ap; z = Dinv r; partials of r.z and r.r. Row loads are issued
 // before the p.Ap partial sum, as in the SpMV.
-template <class S, bool ONE, bool RB>
+template <class S, bool ONE, bool RB, bool EARLY>
 __global__ __launch_bounds__(kBlock) void pcg_update(DevArena A, OneScene one, int odd) {
     Tile c;
     if (!inner_tile<S, ONE>(A, one, c)) return;
@@ -1018,6 +1033,9 @@ __global__ __launch_bounds__(kBlock) void pcg_update(DevArena A, OneScene one, i
     __shared__ double lds[2][kWaves];
     SlotState *st = A.st + c.slot;
     const uint32_t done_in = st->done;
+    if constexpr (EARLY) {
+        if (done_in) return;  // the scene's inner solve has ended: no row loads
+    }
     const uint32_t k = st->k_update;
     const uint32_t seg = st->seg;
     const double rho_s[2] = {st->rho[0], st->rho[1]};
@@ -1546,35 +1564,37 @@ struct Lock {
 };
 
 // The inner kernels of one iteration for precision S: with red-black scenes
-// in the launch (rb), pcg_rb_red first and the kernels that branch per tile.
-template <class S>
-void launch_iteration(vx_batch *b, bool rb, const OneScene &one, const dim3 &grid, int odd, bool big) {
+// in the launch (RB), pcg_rb_red first and the kernels that branch per tile.
+template <class S, bool ONE, bool RB, bool EARLY>
+void launch_iteration_t(vx_batch *b, const OneScene &one, const dim3 &grid, int odd, bool big) {
     Context &ctx = context();
     const dim3 blk(kBlock), sg(b->max_slots);
-    const bool single = b->single;
     auto reduce = [&](int k0, int nk) {
         if (big) hipLaunchKernelGGL(scene_reduce<S>, sg, blk, 0, ctx.stream, b->A, k0, nk, 0);
     };
-    if (rb && single) {
-        hipLaunchKernelGGL((pcg_rb_red<S, true>), grid, blk, 0, ctx.stream, b->A, one, odd);
-        hipLaunchKernelGGL((pcg_bond_spmv<S, true, true>), grid, blk, 0, ctx.stream, b->A, one, odd);
-        reduce(kPap, 1);
-        hipLaunchKernelGGL((pcg_update<S, true, true>), grid, blk, 0, ctx.stream, b->A, one, odd);
-    } else if (rb) {
-        hipLaunchKernelGGL((pcg_rb_red<S, false>), grid, blk, 0, ctx.stream, b->A, one, odd);
-        hipLaunchKernelGGL((pcg_bond_spmv<S, false, true>), grid, blk, 0, ctx.stream, b->A, one, odd);
-        reduce(kPap, 1);
-        hipLaunchKernelGGL((pcg_update<S, false, true>), grid, blk, 0, ctx.stream, b->A, one, odd);
-    } else if (single) {
-        hipLaunchKernelGGL((pcg_bond_spmv<S, true, false>), grid, blk, 0, ctx.stream, b->A, one, odd);
-        reduce(kPap, 1);
-        hipLaunchKernelGGL((pcg_update<S, true, false>), grid, blk, 0, ctx.stream, b->A, one, odd);
-    } else {
-        hipLaunchKernelGGL((pcg_bond_spmv<S, false, false>), grid, blk, 0, ctx.stream, b->A, one, odd);
-        reduce(kPap, 1);
-        hipLaunchKernelGGL((pcg_update<S, false, false>), grid, blk, 0, ctx.stream, b->A, one, odd);
-    }
+    if constexpr (RB) hipLaunchKernelGGL((pcg_rb_red<S, ONE, EARLY>), grid, blk, 0, ctx.stream, b->A, one, odd);
+    hipLaunchKernelGGL((pcg_bond_spmv<S, ONE, RB, EARLY>), grid, blk, 0, ctx.stream, b->A, one, odd);
+    reduce(kPap, 1);
+    hipLaunchKernelGGL((pcg_update<S, ONE, RB, EARLY>), grid, blk, 0, ctx.stream, b->A, one, odd);
     reduce(kRz, 2);
+}
+template <class S, bool EARLY>
+void launch_iteration_e(vx_batch *b, bool rb, const OneScene &one, const dim3 &grid, int odd, bool big) {
+    if (rb && b->single)
+        launch_iteration_t<S, true, true, EARLY>(b, one, grid, odd, big);
+    else if (rb)
+        launch_iteration_t<S, false, true, EARLY>(b, one, grid, odd, big);
+    else if (b->single)
+        launch_iteration_t<S, true, false, EARLY>(b, one, grid, odd, big);
+    else
+        launch_iteration_t<S, false, false, EARLY>(b, one, grid, odd, big);
+}
+template <class S>
+void launch_iteration(vx_batch *b, bool rb, const OneScene &one, const dim3 &grid, int odd, bool big) {
+    if (big)
+        launch_iteration_e<S, true>(b, rb, one, grid, odd, big);
+    else
+        launch_iteration_e<S, false>(b, rb, one, grid, odd, big);
 }
 
 // One inner chunk: the refinement residual of scenes whose inner solve has
@@ -2775,13 +2795,19 @@ extern "C" int32_t vx_scene_time_spmv(vx_scene *sc, uint32_t reps, double *out_u
     b->spread_dirty = true;
     VX_HIP(hipGetLastError());
     const OneScene one = make_one(g);
+    const bool early = g.ntiles > kBigTiles;  // as the solve's chunks launch it
     // The launches run back to back inside one graph, as the solve runs them,
     // between two events on the library stream.
     hipGraph_t graph;
     hipGraphExec_t exec = nullptr;
     VX_HIP(hipStreamBeginCapture(ctx.stream, hipStreamCaptureModeThreadLocal));
     for (uint32_t k = 0; k < reps; ++k)
-        hipLaunchKernelGGL((pcg_bond_spmv<float, true, false>), dim3(one.grid), dim3(kBlock), 0, ctx.stream, b->A, one, int(k & 1));
+        if (early)
+            hipLaunchKernelGGL((pcg_bond_spmv<float, true, false, true>), dim3(one.grid), dim3(kBlock), 0, ctx.stream, b->A, one,
+                               int(k & 1));
+        else
+            hipLaunchKernelGGL((pcg_bond_spmv<float, true, false, false>), dim3(one.grid), dim3(kBlock), 0, ctx.stream, b->A, one,
+                               int(k & 1));
     VX_HIP(hipStreamEndCapture(ctx.stream, &graph));
     hipError_t ge = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
