@@ -39,6 +39,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 namespace vx {
@@ -633,6 +634,27 @@ __device__ __forceinline__ uint32_t colour_of(const SlotGeo &g, uint32_t v) {
 // stay below 128 * ceil(ceil(n / 2) / 64) <= its tile range.
 __device__ __forceinline__ uint32_t rb_pos(uint32_t colour, uint32_t m) { return ((m >> 6) << 7) | (colour << 6) | (m & 63u); }
 
+// D and D^-1 of a mixed-precision red-black scene at colour-split positions,
+// so each colour's kernel reads them contiguously. They live in the FP64
+// inner planes dd (D) and rd (D^-1), which an FP32 inner solve does not use:
+// the six floats of position s fill the three doubles of row s, so a copy
+// stays inside its own row's slots (a batch's FP64 scenes keep theirs).
+// Refreshed by refine_red at the first residual of every solve.
+__device__ __forceinline__ void ld_sym_rb(const double *plane, size_t R, uint32_t s, float (&o)[6]) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const uint64_t t = ld_row(reinterpret_cast<const uint64_t *>(plane) + size_t(j) * R, s * 8u);
+        o[2 * j] = __uint_as_float(uint32_t(t));
+        o[2 * j + 1] = __uint_as_float(uint32_t(t >> 32));
+    }
+}
+__device__ __forceinline__ void st_sym_rb(double *plane, size_t R, uint32_t s, const float (&v)[6]) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+        st_row(reinterpret_cast<uint64_t *>(plane) + size_t(j) * R, s * 8u,
+               uint64_t(__float_as_uint(v[2 * j])) | (uint64_t(__float_as_uint(v[2 * j + 1])) << 32));
+}
+
 // The row a thread of a red-black inner kernel serves: its tile-local index s
 // is a colour-split position; colour and natural row follow. False when the
 // position holds no row of the wanted colour (wave-uniform colour).
@@ -745,10 +767,14 @@ __global__ __launch_bounds__(kBlock) void pcg_rb_red(DevArena A, OneScene one, i
     RbOffsets o{};
     if (wave_on) {
         o = rb_offsets<S>(c.g, rr.row ? rr.v : 0u, c.local);
-        const S *__restrict__ Di = Inner<S>::dinv(A);
         m = A.mask[o.ov / uint32_t(sizeof(S))];
+        if constexpr (std::is_same<S, float>::value) {
+            ld_sym_rb(A.rd, R, c.v, Dv);  // D^-1 at the own colour-split position
+        } else {
+            const S *__restrict__ Di = Inner<S>::dinv(A);
 #pragma unroll
-        for (int q = 0; q < 6; ++q) Dv[q] = ld_row(Di + q * R, o.ov);
+            for (int q = 0; q < 6; ++q) Dv[q] = ld_row(Di + q * R, o.ov);
+        }
 #pragma unroll
         for (int t = 0; t < 6; ++t)
 #pragma unroll
@@ -818,15 +844,19 @@ __device__ __forceinline__ void spmv_black(const DevArena &A, const Tile &c, int
         o = rb_offsets<S>(c.g, rr.row ? rr.v : 0u, rr.row ? c.local : 0u);
         const S *__restrict__ z = Inner<S>::z(A);
         const S *__restrict__ p_old = Inner<S>::p(A, odd);
-        const S *__restrict__ Dp = Inner<S>::d(A);
         m = A.mask[o.ov / uint32_t(sizeof(S))];
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
             zz[q] = ld_row(z + q * R, os);
             pp[q] = ld_row(p_old + q * R, os);
         }
+        if constexpr (std::is_same<S, float>::value) {
+            ld_sym_rb(A.dd, R, os / uint32_t(sizeof(S)), Dv);  // D at the own colour-split position
+        } else {
+            const S *__restrict__ Dp = Inner<S>::d(A);
 #pragma unroll
-        for (int q = 0; q < 6; ++q) Dv[q] = ld_row(Dp + q * R, o.ov);
+            for (int q = 0; q < 6; ++q) Dv[q] = ld_row(Dp + q * R, o.ov);
+        }
 #pragma unroll
         for (int t = 0; t < 6; ++t)
 #pragma unroll
@@ -1019,7 +1049,7 @@ __global__ __launch_bounds__(kBlock) void pcg_update(DevArena A, OneScene one, i
     // A red-black scene updates its black rows at their colour-split
     // positions (the thread's own index); D^-1 stays in the natural layout.
     // Waves of red positions only take part in the partial sums.
-    bool act = c.row, load = true;
+    bool act = c.row, load = true, split_d = false;
     uint32_t v = c.row ? c.v : c.g.voff, vd = v;
     if constexpr (RB) {
         if (ONE || (c.bits & B_RB)) {
@@ -1028,6 +1058,7 @@ __global__ __launch_bounds__(kBlock) void pcg_update(DevArena A, OneScene one, i
             act = load && rr.row;
             v = c.v;
             vd = c.g.voff + (rr.row ? rr.v : 0u);
+            split_d = std::is_same<S, float>::value;  // D^-1 at the own colour-split position
         }
     }
     __shared__ double lds[2][kWaves];
@@ -1060,8 +1091,17 @@ __global__ __launch_bounds__(kBlock) void pcg_update(DevArena A, OneScene one, i
             rv[q] = ld_row(r + q * R, ov);
             dv[q] = ld_row(d + q * R, ov);
         }
+        if constexpr (RB && std::is_same<S, float>::value) {
+            if (split_d) {
+                ld_sym_rb(A.rd, R, v, Di);
+            } else {
 #pragma unroll
-        for (int q = 0; q < 6; ++q) Di[q] = ld_row(dinv + q * R, ovd);
+                for (int q = 0; q < 6; ++q) Di[q] = ld_row(dinv + q * R, ovd);
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < 6; ++q) Di[q] = ld_row(dinv + q * R, ovd);
+        }
     }
     __builtin_amdgcn_sched_barrier(0);
     block_sum_all<1>(pap, lds);
@@ -1131,6 +1171,20 @@ __global__ __launch_bounds__(kBlock) void refine_red(DevArena A, int forced) {
     const uint32_t v = c.v;
     uint32_t cc[3];
     coords(c.g, c.local, cc);
+    if constexpr (std::is_same<S, float>::value) {
+        if (first) {
+            // colour-split copies of D and D^-1 (assembled since the last solve)
+            const uint32_t ps = c.g.voff + rb_pos(red_row(cc) ? 0u : 1u, c.local >> 1);
+            float dv6[6], di6[6];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                dv6[q] = A.d32[q * R + v];
+                di6[q] = A.dinv32[q * R + v];
+            }
+            st_sym_rb(A.dd, R, ps, dv6);
+            st_sym_rb(A.rd, R, ps, di6);
+        }
+    }
     double xv[3];
     if (!red_row(cc)) {
         xval(v, xv);
