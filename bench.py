@@ -421,7 +421,7 @@ def main():
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (independent scenes shard "
+        print("bench.py measures one GPU: the library has no multi-GPU path (independent scenes shard "
               "without communication; a single scene would need a slab decomposition). Run with --gpus 1.",
               file=sys.stderr)
         sys.exit(2)

@@ -1,5 +1,6 @@
 // CPU oracle for the corotational elastic PCG solve. TEST INFRASTRUCTURE ONLY:
-// the checker and the timed CPU baseline, never part of the product path.
+// what the tests compare the HIP path with, and the timed CPU baseline; never
+// part of the product path.
 // Each function cites the reference file:line it restates (paths relative to
 // the reference repository). See vx_oracle.h for the API contract.
 #include "vx_oracle.h"

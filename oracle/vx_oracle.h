@@ -1,7 +1,7 @@
 /*
  * CPU oracle for the corotational elastic PCG solve of Voxel-sim.
  *
- * TEST INFRASTRUCTURE ONLY. This library is the checker the HIP path is
+ * TEST INFRASTRUCTURE ONLY. This library is what the HIP path is
  * compared against (and the timed CPU baseline). It is never linked into,
  * loaded by, or called from the product library, and the product path never
  * falls back to it.
