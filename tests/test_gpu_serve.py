@@ -54,3 +54,18 @@ def test_admission_waits_for_free_tiles(gpu, stream):
     assert waited, "expected some requests to queue for tiles"
     p = serve.percentiles([r.latency for r in stream])
     assert 0 < p[50] <= p[99]
+
+
+def test_served_red_black_results_equal_results_alone(gpu):
+    """The same with the red-black preconditioner: warm starts kept on the
+    device, requests overlapping, results bit-identical to each alone."""
+    from vxsim import batch, grid, serve
+    reqs = serve.request_stream(rate=200.0, seconds=0.06, rev_lo=200, rev_hi=4000, steps=3, seed=13, max_requests=8,
+                                solve=grid.solve_params(tolerance=1e-6, warm_start=True, preconditioner=grid.RED_BLACK))
+    with serve.Server(capacity_rows=64 * 512, max_scenes=8, keep_results=True) as S:
+        S.run(reqs)
+    for req in reqs:
+        assert req.state == batch.DONE, (req.rid, req.status, req.report)
+        ref = alone(req)
+        for k in ("positions", "rotations", "velocities"):
+            assert np.array_equal(req.result[k], ref[k]), (req.rid, k)
