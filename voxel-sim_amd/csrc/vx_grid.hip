@@ -956,12 +956,33 @@ __global__ __launch_bounds__(kBlock) void pcg_bond_spmv(DevArena A, OneScene one
     const uint32_t ov = v * sz;
     uint32_t om[3], op[3];
     bool lo[3];
+    // Large arenas (EARLY, FP32): the tile's own z and p_old are staged in
+    // LDS, and a neighbour inside the same tile (x but at the tile's ends, y
+    // when an x-line is shorter than the tile) is read from there instead of
+    // through the caches: fewer memory requests per row. Rows past the scene
+    // stage row 0's values; no row of the scene has them as a neighbour.
+    constexpr bool kStage = EARLY && std::is_same<S, float>::value;
+    __shared__ S stage[kStage ? 6 : 1][kStage ? kBlock : 1];
+    bool inm[3] = {false, false, false}, inp[3] = {false, false, false};
+    if constexpr (kStage) {
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        const uint32_t s = stride_of(c.g, a);
-        lo[a] = cc[a] > 0;
-        om[a] = lo[a] ? ov - s * sz : ov;
-        op[a] = cc[a] + 1 < extent_of(c.g, a) ? ov + s * sz : ov;
+        for (int a = 0; a < 3; ++a) {
+            const uint32_t s = stride_of(c.g, a);
+            const bool hi = cc[a] + 1 < extent_of(c.g, a);
+            lo[a] = cc[a] > 0;
+            om[a] = lo[a] ? ov - s * sz : ov;
+            op[a] = hi ? ov + s * sz : ov;
+            inm[a] = lo[a] && threadIdx.x >= s;
+            inp[a] = hi && threadIdx.x + s < kBlock;
+        }
+    } else {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const uint32_t s = stride_of(c.g, a);
+            lo[a] = cc[a] > 0;
+            om[a] = lo[a] ? ov - s * sz : ov;
+            op[a] = cc[a] + 1 < extent_of(c.g, a) ? ov + s * sz : ov;
+        }
     }
     S zz[7][3], pp[7][3], Dv[6], Kp[3][6], Km[3][6];
     const uint8_t m = A.mask[v];
@@ -977,10 +998,14 @@ __global__ __launch_bounds__(kBlock) void pcg_bond_spmv(DevArena A, OneScene one
         const S *Ka = K + size_t(a) * 6 * R;
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
-            zz[1 + 2 * a][q] = ld_row(z + q * R, om[a]);
-            pp[1 + 2 * a][q] = ld_row(p_old + q * R, om[a]);
-            zz[2 + 2 * a][q] = ld_row(z + q * R, op[a]);
-            pp[2 + 2 * a][q] = ld_row(p_old + q * R, op[a]);
+            if (!inm[a]) {
+                zz[1 + 2 * a][q] = ld_row(z + q * R, om[a]);
+                pp[1 + 2 * a][q] = ld_row(p_old + q * R, om[a]);
+            }
+            if (!inp[a]) {
+                zz[2 + 2 * a][q] = ld_row(z + q * R, op[a]);
+                pp[2 + 2 * a][q] = ld_row(p_old + q * R, op[a]);
+            }
         }
 #pragma unroll
         for (int q = 0; q < 6; ++q) {
@@ -993,10 +1018,35 @@ __global__ __launch_bounds__(kBlock) void pcg_bond_spmv(DevArena A, OneScene one
     for (int a = 0; a < 3; ++a)
 #pragma unroll
         for (int q = 0; q < 6; ++q) Km[a][q] = lo[a] ? Km[a][q] : S(0);
-    block_sum_all<2>(sums, lds);
+    if constexpr (kStage) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            stage[q][threadIdx.x] = zz[0][q];
+            stage[3 + q][threadIdx.x] = pp[0][q];
+        }
+    }
+    block_sum_all<2>(sums, lds);  // its barriers also order the staging stores before the reads below
     if (done_in) return;
     S beta;
     if (!inner_head<S>(st, prm, sums, k, seg, rr0_s, bb, rho_s, c.t == c.g.tile0 && threadIdx.x == 0, odd, beta)) return;
+    if constexpr (kStage) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const uint32_t s = stride_of(c.g, a);
+            if (inm[a])
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    zz[1 + 2 * a][q] = stage[q][threadIdx.x - s];
+                    pp[1 + 2 * a][q] = stage[3 + q][threadIdx.x - s];
+                }
+            if (inp[a])
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    zz[2 + 2 * a][q] = stage[q][threadIdx.x + s];
+                    pp[2 + 2 * a][q] = stage[3 + q][threadIdx.x + s];
+                }
+        }
+    }
     double part[1] = {0.0};
     S p[7][3];
 #pragma unroll
