@@ -30,7 +30,7 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 SOLVE_TOL = 1e-9        # outer FP64 relative true residual
-INNER_TOL = 1e-3        # FP32 residual drop between FP64 residual replacements
+INNER_TOL = 3e-3        # FP32 residual drop between FP64 residual replacements
 PEAK_HBM_GBS = 8000.0   # MI355X HBM3E
 SPMV_BYTES_PER_ROW = 145  # z 12 + p_old 12 + D 24 + 3 bond blocks 72 + mask 1 + p_new 12 + Ap 12
 # fracture_bonds per REV row: reads bond bits 1 + kinematic 1 + material 44 + rotation and assembly rotation 64 +

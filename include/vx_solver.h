@@ -161,7 +161,7 @@ typedef struct vx_scene vx_scene;
 
 typedef struct vx_solve_params {
     double tolerance;          /* outer FP64 relative true residual ||b - A x|| / ||b|| */
-    double inner_tolerance;    /* FP32 residual drop between FP64 residual replacements (1e-3) */
+    double inner_tolerance;    /* FP32 residual drop between FP64 residual replacements (3e-3) */
     uint32_t max_inner;        /* inner iteration cap per solve */
     uint32_t max_outer;        /* FP64 residual replacements cap */
     uint32_t warm_start;       /* 1: start from the scene's last correction */

@@ -22,7 +22,7 @@ BLOCK_JACOBI = VX_PRECOND_BLOCK_JACOBI
 RED_BLACK = VX_PRECOND_RED_BLACK
 
 
-def solve_params(tolerance=1e-9, inner_tolerance=1e-3, max_inner=4000, max_outer=12, warm_start=False, mode=MIXED,
+def solve_params(tolerance=1e-9, inner_tolerance=3e-3, max_inner=4000, max_outer=12, warm_start=False, mode=MIXED,
                  preconditioner=BLOCK_JACOBI):
     """vx_solve_params. `preconditioner`: BLOCK_JACOBI (the reference's) or
     RED_BLACK (CG on the red-black reduced system: red-black block symmetric
