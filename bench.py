@@ -579,7 +579,7 @@ def main():
         "config": {"workload": "large_64x64x64_mixed: 262144 REVs, %d active rows, 1 ms step, 1 substep, "
                                "<=%d corotation passes, outer rel. residual %g, inner %g" %
                                (active, cfg.corotation_iterations, SOLVE_TOL, INNER_TOL),
-                   "active_rows": active, "dof": 3 * active, "solves": solves,
+                   "active_rows": active, "dof": 3 * active, "dof_total": 3 * scene.n, "solves": solves,
                    "iterations": iters, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": SPMV_KERNEL, "achieved": achieved, "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": measured_traffic(),
