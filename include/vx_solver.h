@@ -361,6 +361,52 @@ int32_t vx_batch_download_bonds(vx_batch *batch, uint32_t slot, uint32_t *neighb
 /* Free a finished (or failed) scene's tiles and slot. */
 int32_t vx_batch_release(vx_batch *batch, uint32_t slot);
 
+/* ---- contact projection ---------------------------------------------------
+ * Drop-in for collision::project (src/collision.rs:33-123): projects voxel
+ * pairs apart and reports the contacts, with the reference's results bit for
+ * bit (the Gauss-Seidel order over pairs sorted by (i, j) is reproduced).
+ *
+ *   positions, previous  n*3 f32, updated in place (scene.positions,
+ *                        scene.previous; the swept start is previous on entry)
+ *   rest_positions       n*3 f32 (scene.rest_positions)
+ *   mass                 n f32 (voxel.mass; unread for kinematic voxels)
+ *   kinematic            n bytes, nonzero = kinematic (voxel.kinematic)
+ *   target_axes          n bytes, bit a set = targets[i][a].is_some(); NULL
+ *                        for no targets (an empty targets slice)
+ *   bond_neighbor        3*n, [axis][voxel]: scene.bonds[axis][i].neighbor,
+ *                        0xFFFFFFFF when empty
+ *   contacts, capacity   receive the contacts sorted by (i, j), one per pair
+ *
+ * One contact is kept per pair, the first found; the reference's
+ * sort_unstable_by_key leaves that choice open when a pair's normal changes
+ * between passes. VX_ERR_TOO_LARGE when more than `capacity` contacts are
+ * found: nothing is written then except report->contacts, so a second call
+ * with a larger capacity gives the same result. VX_ERR_INVALID_ARGUMENT for
+ * a swept box with no finite bound on some axis (previous and position both
+ * non-finite there; the reference would bin such a voxel at cell 0) or one
+ * reaching 2^20 cells or more from the origin, or a bond neighbor >= n. A
+ * NaN in only one of previous and position is binned by the other, as the
+ * reference's f32::min / f32::max do. */
+typedef struct vx_contact {
+    uint32_t i, j;        /* i < j */
+    float normal[3];      /* unit axis normal pointing from i toward j */
+    uint32_t new_impact;  /* the pair started with a gap along the normal */
+} vx_contact;
+
+typedef struct vx_contact_report {
+    uint64_t candidate_pairs; /* broad-phase pairs */
+    uint64_t pairs;           /* after dropping intact local continuum pairs */
+    uint64_t contacts;        /* contacts found */
+    uint32_t passes;          /* projection passes (at most 6) */
+    uint32_t rounds;          /* device windows over the pair list */
+    double device_us;         /* device time, inputs resident, no transfers */
+} vx_contact_report;
+
+int32_t vx_contact_project(uint32_t n, float *positions, float *previous, const float *rest_positions,
+                           const float *mass, const uint8_t *kinematic, const uint8_t *target_axes,
+                           const uint32_t *bond_neighbor, vx_contact *contacts, uint64_t capacity,
+                           vx_contact_report *report);
+
 #ifdef __cplusplus
 }
 #endif

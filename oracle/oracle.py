@@ -129,6 +129,15 @@ def lib():
         L.vxo_dot_rows.argtypes = [ctypes.c_uint32, _dp, _dp]
         L.vxo_guarded_ratio.restype = ctypes.c_double
         L.vxo_guarded_ratio.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double]
+        _fp = ctypes.POINTER(ctypes.c_float)
+        _u64p = ctypes.POINTER(ctypes.c_uint64)
+        L.vxo_contact_project.restype = ctypes.c_uint64
+        L.vxo_contact_project.argtypes = [ctypes.c_uint32, _fp, _fp, _fp, _fp, _u8p, _u8p, _u32p, ctypes.c_uint64,
+                                          _u32p, _u32p, _fp, _u8p, _u64p]
+        L.vxo_contact_candidates.restype = ctypes.c_uint64
+        L.vxo_contact_candidates.argtypes = [ctypes.c_uint32, _fp, _fp, _u8p, _u32p, ctypes.c_uint64]
+        L.vxo_local_continuum_pair.restype = ctypes.c_int
+        L.vxo_local_continuum_pair.argtypes = [ctypes.c_uint32, _fp, _u32p, ctypes.c_uint32, ctypes.c_uint32]
         _lib = L
     return _lib
 
@@ -438,3 +447,70 @@ def step_fracture(oscene: OracleScene, cfg, fcfg, history: FractureHistory, stre
                 last_angle=rep.last_angle, broken=fs.broken,
                 dissipated={k: getattr(fs.dissipated, k) for k, _ in _Dissipation._fields_})
 
+
+
+# ---- contact projection (src/collision.rs:33-123) ------------------------------
+
+def _f32c(a):
+    return np.ascontiguousarray(a, dtype=np.float32).reshape(-1).copy()
+
+
+def _targets_bits(targets, n):
+    if targets is None:
+        return None
+    t = np.asarray(targets, dtype=bool).reshape(n, 3)
+    return np.ascontiguousarray(t[:, 0] * 1 + t[:, 1] * 2 + t[:, 2] * 4, dtype=np.uint8)
+
+
+def contact_project(positions, previous, rest_positions, mass, kinematic, bond_neighbor, targets=None):
+    """collision::project on the CPU: (positions, previous, contacts, stats), as vxsim.contact.project."""
+    L = lib()
+    _fp = ctypes.POINTER(ctypes.c_float)
+    pos, prev, rest = _f32c(positions), _f32c(previous), _f32c(rest_positions)
+    n = pos.size // 3
+    m = _f32c(mass)
+    kin = np.ascontiguousarray(np.asarray(kinematic).reshape(-1) != 0, dtype=np.uint8)
+    bonds = np.ascontiguousarray(bond_neighbor, dtype=np.uint32).reshape(-1)
+    tgt = _targets_bits(targets, n)
+    cap = max(1, 6 * n)
+    while True:
+        p, q = pos.copy(), prev.copy()
+        ci = np.zeros(cap, np.uint32)
+        cj = np.zeros(cap, np.uint32)
+        cn = np.zeros(3 * cap, np.float32)
+        cimp = np.zeros(cap, np.uint8)
+        stats = np.zeros(4, np.uint64)
+        k = L.vxo_contact_project(n, p.ctypes.data_as(_fp), q.ctypes.data_as(_fp), rest.ctypes.data_as(_fp),
+                                  m.ctypes.data_as(_fp), _p(kin, _u8p), _p(tgt, _u8p) if tgt is not None else None,
+                                  _p(bonds, _u32p), cap, _p(ci, _u32p), _p(cj, _u32p), cn.ctypes.data_as(_fp),
+                                  _p(cimp, _u8p), stats.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)))
+        if k <= cap:
+            break
+        cap = int(k)
+    pos, prev = p, q
+    contacts = {"i": ci[:k].copy(), "j": cj[:k].copy(), "normal": cn[:3 * k].reshape(k, 3).copy(),
+                "new_impact": cimp[:k] != 0}
+    st = {"candidate_pairs": int(stats[0]), "pairs": int(stats[1]), "passes": int(stats[2]),
+          "contacts": int(stats[3])}
+    return pos.reshape(n, 3), prev.reshape(n, 3), contacts, st
+
+
+def contact_candidates(positions, previous, kinematic):
+    """The broad phase alone: sorted (i, j) candidate pairs, shape (k, 2)."""
+    L = lib()
+    _fp = ctypes.POINTER(ctypes.c_float)
+    pos, prev = _f32c(positions), _f32c(previous)
+    n = pos.size // 3
+    kin = np.ascontiguousarray(np.asarray(kinematic).reshape(-1) != 0, dtype=np.uint8)
+    k = L.vxo_contact_candidates(n, pos.ctypes.data_as(_fp), prev.ctypes.data_as(_fp), _p(kin, _u8p), None, 0)
+    out = np.zeros(2 * max(int(k), 1), np.uint32)
+    L.vxo_contact_candidates(n, pos.ctypes.data_as(_fp), prev.ctypes.data_as(_fp), _p(kin, _u8p), _p(out, _u32p), k)
+    return out[:2 * k].reshape(int(k), 2)
+
+
+def local_continuum_pair(rest_positions, bond_neighbor, i, j):
+    L = lib()
+    rest = _f32c(rest_positions)
+    bonds = np.ascontiguousarray(bond_neighbor, dtype=np.uint32).reshape(-1)
+    return bool(L.vxo_local_continuum_pair(rest.size // 3, rest.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                           _p(bonds, _u32p), i, j))

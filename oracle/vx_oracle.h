@@ -268,6 +268,27 @@ void vxo_step(int mode, vxo_scene *s, const vxo_step_config *cfg, const double *
 void vxo_step_fracture(int mode, vxo_scene *s, const vxo_step_config *cfg, const double *targets,
                        const double *loads, vxo_fracture_state *fracture, vxo_step_report *report);
 
+/* ---- contact projection (src/collision.rs:33-123), f32 ------------------
+ * vxo_contact_project restates collision::project: positions and previous
+ * (n*3 f32) are updated in place; rest (n*3), mass (n), kinematic (n),
+ * targets (n, bit a set = axis a held by a target; NULL for none) and
+ * bond_neighbor (3*n, [axis][voxel], VXO_EMPTY when absent) are inputs.
+ * Contacts are written sorted by (i, j), up to `capacity`; the return value
+ * is the full count. stats (may be NULL): candidate pairs, pairs after the
+ * local-continuum filter, projection passes, contacts. */
+uint64_t vxo_contact_project(uint32_t n, float *positions, float *previous, const float *rest,
+                             const float *mass, const uint8_t *kinematic, const uint8_t *targets,
+                             const uint32_t *bond_neighbor, uint64_t capacity, uint32_t *out_i,
+                             uint32_t *out_j, float *out_normal, uint8_t *out_new_impact, uint64_t *stats);
+/* The broad phase alone: sorted candidate pairs (2 per entry) up to
+ * `capacity`; returns the count. `previous` is the swept start. */
+uint64_t vxo_contact_candidates(uint32_t n, const float *positions, const float *previous,
+                                const uint8_t *kinematic, uint32_t *out_pairs, uint64_t capacity);
+/* local_continuum_pair (src/collision.rs:520-550): 1 when i and j are joined
+ * by a monotone path of intact bonds inside their lattice cell. */
+int vxo_local_continuum_pair(uint32_t n, const float *rest, const uint32_t *bond_neighbor, uint32_t i,
+                             uint32_t j);
+
 #ifdef __cplusplus
 }
 #endif

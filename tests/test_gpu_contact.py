@@ -1,0 +1,206 @@
+"""Device contact projection (vx_contact_project) against the oracle's
+collision::project (src/collision.rs:33-123): positions, previous positions,
+contacts and counts bit for bit, on the reference's collision-test
+configurations, on random lattice boxes with cracks, a fixed floor, fast
+movers and targets, and on the 64^3 box; plus the boundary's error paths."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from oracle import oracle
+from vxsim import _abi, contact, lib, scenes
+from vxsim.solver import SolverError
+
+from test_oracle_contact import line_box
+
+pytestmark = pytest.mark.gpu
+
+CELL = np.float32(0.01)
+EMPTY = 0xFFFFFFFF
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float32).view(np.uint32)
+
+
+def check_parity(s, targets=None, label=""):
+    print("contact parity", label, len(s["positions"]), flush=True)
+    p, q, c, rep = contact.project(**s, targets=targets)
+    po, qo, co, st = oracle.contact_project(**s, targets=targets)
+    assert np.array_equal(bits(p), bits(po)), label
+    assert np.array_equal(bits(q), bits(qo)), label
+    for k in ("i", "j", "new_impact"):
+        assert np.array_equal(c[k], co[k]), (label, k)
+    assert np.array_equal(bits(c["normal"]), bits(co["normal"])), label
+    assert rep["candidate_pairs"] == st["candidate_pairs"], label
+    assert rep["pairs"] == st["pairs"], label
+    assert rep["passes"] == st["passes"], label
+    assert rep["contacts"] == st["contacts"], label
+    return rep, st
+
+
+def test_reference_configurations():
+    s = line_box((3, 2, 1))
+    s["bond_neighbor"][0, 0] = EMPTY
+    s["positions"][1] = s["positions"][0] + np.float32(0.5) * np.array([CELL, 0, 0], np.float32)
+    s["previous"] = s["positions"].copy()
+    check_parity(s, label="crack")
+
+    s = line_box((2, 1, 1))
+    s["bond_neighbor"][0, 0] = EMPTY
+    s["kinematic"] = np.array([False, True])
+    s["positions"][0, 0] += np.float32(3.0) * CELL
+    check_parity(s, label="tunnel")
+
+    s = line_box((2, 1, 1))
+    s["bond_neighbor"][:] = EMPTY
+    s["positions"][1, 0] += np.float32(0.005)
+    s["previous"][1] = s["positions"][1]
+    s["positions"][0, 0] += np.float32(0.002503)
+    s["positions"][1, 0] -= np.float32(0.002503)
+    check_parity(s, targets=np.zeros((2, 3), bool), label="impact")
+
+    s = line_box((3, 3, 3))
+    c, sn = np.cos(0.37), np.sin(0.37)
+    r = np.array([[c, -sn, 0], [sn, c, 0], [0, 0, 1]])
+    s["positions"] = (s["positions"].astype(np.float64) @ r.T).astype(np.float32)
+    s["previous"] = s["positions"].copy()
+    rep, _ = check_parity(s, label="rotated")
+    assert rep["contacts"] == 0
+
+    s = line_box((2, 1, 1))
+    s["bond_neighbor"][:] = EMPTY
+    s["positions"][:] = (s["positions"][0] + s["positions"][1]) * np.float32(0.5)
+    s["previous"] = s["positions"].copy()
+    check_parity(s, label="coincident")
+
+
+@pytest.mark.parametrize("dims,seed,kw", [
+    ((6, 5, 4), 1, {}),
+    ((12, 10, 8), 3, {}),
+    ((9, 7, 5), 4, {"fixed_floor": False, "jitter": 8e-4}),
+    ((16, 12, 10), 5, {"movers": 0.05, "mover_reach": 3.0}),
+    ((33, 7, 5), 6, {"broken": 0.3, "targeted": 0.2}),
+    ((20, 20, 12), 8, {"jitter": 1.5e-3, "sag": 2e-4}),
+])
+def test_random_boxes_match_the_oracle(dims, seed, kw):
+    s = scenes.contact_box(dims, seed=seed, **kw)
+    targets = s.pop("targets")
+    check_parity(s, targets=targets, label="%s seed %d" % (dims, seed))
+    check_parity(s, targets=None, label="%s seed %d no targets" % (dims, seed))
+
+
+def test_dense_pile_in_few_bins():
+    # many movable voxels crowded into a few bins: long bin runs, long dependency chains
+    rng = np.random.default_rng(11)
+    n = 400
+    s = {
+        "positions": (rng.uniform(0, 3, (n, 3)) * CELL).astype(np.float32),
+        "rest_positions": (np.stack([np.arange(n) * 3, np.zeros(n), np.zeros(n)], 1) * CELL).astype(np.float32),
+        "mass": rng.uniform(1e-4, 1e-3, n).astype(np.float32),
+        "kinematic": rng.uniform(size=n) < 0.1,
+        "bond_neighbor": np.full((3, n), EMPTY, np.uint32),
+    }
+    s["previous"] = s["positions"].copy()
+    rep, _ = check_parity(s, label="pile")
+    assert rep["rounds"] > rep["passes"]
+
+
+def test_64cube_matches_the_oracle():
+    s = scenes.contact_box((64, 64, 64), seed=2)
+    targets = s.pop("targets")
+    rep, st = check_parity(s, targets=targets, label="64^3")
+    assert st["pairs"] > 1_000_000 and st["contacts"] > 10_000
+
+
+def raw_call(s, capacity):
+    L = lib()
+    n = len(s["positions"])
+    pos = np.ascontiguousarray(s["positions"], np.float32).copy()
+    prev = np.ascontiguousarray(s["previous"], np.float32).copy()
+    rest = np.ascontiguousarray(s["rest_positions"], np.float32)
+    mass = np.ascontiguousarray(s["mass"], np.float32)
+    kin = np.ascontiguousarray(s["kinematic"], np.uint8)
+    nb = np.ascontiguousarray(s["bond_neighbor"], np.uint32)
+    out = (_abi.Contact * max(capacity, 1))()
+    rep = _abi.ContactReport()
+    st = L.vx_contact_project(n, pos.ctypes.data_as(_abi.c_fp), prev.ctypes.data_as(_abi.c_fp),
+                              rest.ctypes.data_as(_abi.c_fp), mass.ctypes.data_as(_abi.c_fp),
+                              kin.ctypes.data_as(_abi.c_u8p), None, nb.ctypes.data_as(_abi.c_u32p), out, capacity,
+                              ctypes.byref(rep))
+    return st, pos, prev, rep
+
+
+def test_capacity_too_small_writes_nothing_and_reports_the_count():
+    s = scenes.contact_box((10, 8, 6), seed=9)
+    s.pop("targets")
+    _, _, co, _ = oracle.contact_project(**s)
+    assert len(co["i"]) > 1
+    st, pos, prev, rep = raw_call(s, 1)
+    assert st == _abi.VX_ERR_TOO_LARGE
+    assert rep.contacts == len(co["i"])
+    assert np.array_equal(bits(pos), bits(s["positions"]))
+    assert np.array_equal(bits(prev), bits(s["previous"]))
+    st, pos, prev, rep = raw_call(s, len(co["i"]))
+    assert st == _abi.VX_OK
+
+
+def same_or_both_nan(a, b):
+    a, b = np.asarray(a, np.float32), np.asarray(b, np.float32)
+    nan = np.isnan(a)
+    return np.array_equal(nan, np.isnan(b)) and np.array_equal(bits(a[~nan]), bits(b[~nan]))
+
+
+def test_nan_end_position_follows_the_reference():
+    # f32::min / max skip a NaN operand, so the voxel is binned by its finite
+    # start and the projection runs on, as in the reference
+    s = line_box((3, 2, 2))
+    s["bond_neighbor"][0, :] = EMPTY
+    s["positions"][2, 1] = np.nan
+    s["positions"][1, 0] -= np.float32(0.004)
+    p, q, c, rep = contact.project(**s)
+    po, qo, co, st = oracle.contact_project(**s)
+    assert same_or_both_nan(p, po) and same_or_both_nan(q, qo)
+    for k in ("i", "j", "new_impact"):
+        assert np.array_equal(c[k], co[k])
+    assert rep["contacts"] == st["contacts"] > 0
+
+
+def test_invalid_inputs_are_rejected():
+    s = line_box((3, 2, 2))
+    s["positions"][2, 1] = np.nan
+    s["previous"][2, 1] = np.nan  # no finite bound on that axis
+    with pytest.raises(SolverError) as e:
+        contact.project(**s)
+    assert e.value.status == _abi.VX_ERR_INVALID_ARGUMENT
+    s = line_box((3, 2, 2))
+    s["positions"][0, 0] = np.float32(2.0e4)  # 2^21 cells away
+    with pytest.raises(SolverError) as e:
+        contact.project(**s)
+    assert e.value.status == _abi.VX_ERR_INVALID_ARGUMENT
+    s = line_box((3, 2, 2))
+    s["bond_neighbor"][1, 0] = 12
+    with pytest.raises(SolverError) as e:
+        contact.project(**s)
+    assert e.value.status == _abi.VX_ERR_INVALID_ARGUMENT
+
+
+def test_empty_and_all_kinematic_scenes():
+    st, _, _, rep = raw_call({"positions": np.zeros((0, 3)), "previous": np.zeros((0, 3)),
+                              "rest_positions": np.zeros((0, 3)), "mass": np.zeros(0),
+                              "kinematic": np.zeros(0, bool), "bond_neighbor": np.zeros((3, 0))}, 0)
+    assert st == _abi.VX_OK and rep.contacts == 0 and rep.passes == 1
+    s = line_box((4, 3, 2), kinematic=np.ones(24, bool))
+    s["bond_neighbor"][:] = EMPTY
+    rep, _ = check_parity(s, label="all kinematic")
+    assert rep["pairs"] == 0 and rep["contacts"] == 0
+
+
+def test_repeated_calls_reuse_buffers_and_agree():
+    big = scenes.contact_box((24, 20, 16), seed=12)
+    small = scenes.contact_box((7, 6, 5), seed=13)
+    for s in (big, small, big):
+        t = s["targets"]
+        s2 = {k: v for k, v in s.items() if k != "targets"}
+        check_parity(s2, targets=t, label="sequence %d" % len(s2["positions"]))

@@ -1,0 +1,843 @@
+// Device contact projection: the drop-in for collision::project
+// (src/collision.rs:33-123).
+//
+// The reference projects voxel pairs apart with a Gauss-Seidel sweep over
+// the candidate pairs sorted by (i, j), up to six passes, in which every
+// `separate` moves positions that later pairs read. Its result depends on
+// that order, and this file reproduces it bit for bit:
+//
+//  * Broad phase. Each voxel's swept box covers a range of unit bins
+//    (bin_bounds, src/collision.rs:432-439); two voxels are candidates when
+//    their ranges share a bin and they are not both kinematic
+//    (src/collision.rs:376-430). Every covered bin becomes an entry (bin code,
+//    voxel), the entries are radix sorted by bin, and the voxels of one bin
+//    form pairs. A pair is emitted only from the lowest-corner bin of the
+//    two ranges' intersection, so each pair appears once without a hash set.
+//    Pairs joined by intact bonds inside their lattice cell are dropped
+//    (local_continuum_pair, :520-550) and the rest radix sorted by (i, j).
+//
+//  * Projection. One workgroup of 1024 threads takes a window of the pair
+//    list in order, evaluates every pair on the current positions
+//    (swept, overlap and touching tests, :552-621, :441-460), and commits the
+//    pairs whose sequential predecessors cannot change what they read: a pair
+//    is held back when an earlier pair of the window moves one of its voxels,
+//    or when an earlier held-back pair shares a voxel that can move (not
+//    kinematic). Committed pairs touch disjoint moving voxels, so they
+//    commute with the held-back ones and the result equals the sequential
+//    sweep. Held-back pairs stay at the head of the next window in order; the
+//    first pair of a window always commits.
+//
+//  * Signed zeros. `separate` also adds exact zeros to the components off the
+//    normal (and to immovable endpoints). Adding -0 changes nothing and adding
+//    +0 only turns a -0 into +0, so those additions commute with everything:
+//    they run after the real moves of a window, as a store of +0 over -0.
+//
+// All arithmetic is f32 in the reference's order without contraction (Rust
+// never fuses a multiply and an add), with IEEE division.
+#include "vx_common.hpp"
+
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+
+#include <cstring>
+#include <vector>
+
+#pragma clang fp contract(off)
+
+namespace vx {
+namespace {
+
+constexpr float kCell = 0.01f;  // src/scene.rs:10
+constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+constexpr uint32_t kNoMark = 0xFFFFFFFFu;
+constexpr float kBinLimit = 1048576.0f;  // |bin| < 2^20: 21 bits per axis in a bin code
+constexpr int kBinBias = 1 << 20;
+constexpr int kGs = 1024;  // projection workgroup
+constexpr int kGsWaves = kGs / 64;
+constexpr int kPasses = 6;  // src/collision.rs:55
+
+// flags byte per voxel: bit 0 kinematic, bit 1 + a: axis a held by a target.
+__device__ __forceinline__ bool kinematic(uint8_t f) { return f & 1; }
+
+// ---- broad phase -------------------------------------------------------------
+
+// adjacency (src/collision.rs:506-518): slot 2a+1 is the +a neighbour, 2a the -a one.
+__global__ void k_adjacency(uint32_t n, const uint32_t *__restrict__ bond, uint32_t *__restrict__ adj,
+                            uint32_t *__restrict__ err) {
+    const uint64_t g = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (g >= uint64_t(3) * n) return;
+    const uint32_t axis = uint32_t(g / n), i = uint32_t(g % n);
+    const uint32_t j = bond[g];
+    if (j == kEmpty) return;
+    if (j >= n) {
+        atomicOr(err, 1u);
+        return;
+    }
+    adj[size_t(i) * 6 + 2 * axis + 1] = j;
+    adj[size_t(j) * 6 + 2 * axis] = i;
+}
+
+// The bin range of each voxel's swept box (src/collision.rs:383-386, 432-439).
+__global__ void k_bins(uint32_t n, const float *__restrict__ start, const float *__restrict__ pos,
+                       int32_t *__restrict__ lohi, uint64_t *__restrict__ count, uint32_t *__restrict__ err) {
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    const float m = 0.5f + 2e-4f;
+    uint64_t c = 1;
+    bool ok = true;
+    for (int a = 0; a < 3; ++a) {
+        const float s = start[size_t(v) * 3 + a], p = pos[size_t(v) * 3 + a];
+        const float lo = floorf(fminf(s, p) / kCell - m);
+        const float hi = floorf(fmaxf(s, p) / kCell + m);
+        // NaN fails both comparisons
+        if (!(lo >= -kBinLimit && hi < kBinLimit)) {
+            ok = false;
+            break;
+        }
+        lohi[size_t(v) * 6 + a] = int32_t(lo);
+        lohi[size_t(v) * 6 + 3 + a] = int32_t(hi);
+        c *= uint64_t(int64_t(hi) - int64_t(lo) + 1);
+    }
+    if (!ok) {
+        atomicOr(err, 2u);
+        c = 0;
+    }
+    count[v] = c;
+}
+
+__device__ __forceinline__ uint64_t bin_code(int x, int y, int z) {
+    return (uint64_t(uint32_t(z + kBinBias)) << 42) | (uint64_t(uint32_t(y + kBinBias)) << 21) |
+           uint64_t(uint32_t(x + kBinBias));
+}
+__device__ __forceinline__ int bin_axis(uint64_t code, int a) {
+    return int(uint32_t(code >> (21 * a)) & ((1u << 21) - 1)) - kBinBias;
+}
+
+// One entry per covered bin; `ends` is the inclusive scan of the counts.
+__global__ void k_emit_bins(uint32_t n, const int32_t *__restrict__ lohi, const uint64_t *__restrict__ count,
+                            const uint64_t *__restrict__ ends, uint64_t *__restrict__ keys,
+                            uint32_t *__restrict__ vals) {
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    uint64_t o = ends[v] - count[v];
+    const int32_t *b = lohi + size_t(v) * 6;
+    for (int z = b[2]; z <= b[5]; ++z)
+        for (int y = b[1]; y <= b[4]; ++y)
+            for (int x = b[0]; x <= b[3]; ++x) {
+                keys[o] = bin_code(x, y, z);
+                vals[o] = v;
+                ++o;
+            }
+}
+
+// local_continuum_pair (src/collision.rs:520-550): the offset of the rest
+// positions rounds to one lattice step or less per axis and a monotone path
+// of intact bonds walks it. The recursion of the reference tries the axes in
+// every order; so does this loop over the six permutations.
+__device__ bool local_continuum(const float *__restrict__ rest, const uint32_t *__restrict__ adj, uint32_t i,
+                                uint32_t j) {
+    float steps[3], frac[3];
+    for (int a = 0; a < 3; ++a) {
+        const float d = (rest[size_t(j) * 3 + a] - rest[size_t(i) * 3 + a]) / kCell;
+        steps[a] = roundf(d);
+        frac[a] = fabsf(d - steps[a]);
+    }
+    const float fmax_ = fmaxf(frac[0], fmaxf(frac[1], frac[2]));
+    const float smax = fmaxf(fabsf(steps[0]), fmaxf(fabsf(steps[1]), fabsf(steps[2])));
+    if (fmax_ > 1e-3f || smax > 1.0f) return false;
+    constexpr int perms[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+#pragma unroll
+    for (int p = 0; p < 6; ++p) {
+        uint32_t node = i;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            if (node == kEmpty) break;
+            const int a = perms[p][k];
+            if (steps[a] == 0.0f) continue;
+            node = adj[size_t(node) * 6 + 2 * a + (steps[a] > 0.0f ? 1 : 0)];
+        }
+        if (node == j) return true;
+    }
+    return false;
+}
+
+// For entry e of the bin-sorted list: the candidate pairs it forms with the
+// later entries of its bin, counted (EMIT = false) or written (EMIT = true).
+template <bool EMIT>
+__global__ void k_bin_pairs(uint64_t total, const uint64_t *__restrict__ keys, const uint32_t *__restrict__ vals,
+                            const int32_t *__restrict__ lohi, const uint8_t *__restrict__ flags,
+                            const float *__restrict__ rest, const uint32_t *__restrict__ adj,
+                            uint64_t *__restrict__ count, const uint64_t *__restrict__ ends,
+                            uint64_t *__restrict__ out, unsigned long long *__restrict__ candidates) {
+    const uint64_t e = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= total) return;
+    const uint64_t key = keys[e];
+    const uint32_t v = vals[e];
+    const bool kin_v = kinematic(flags[v]);
+    const int bx = bin_axis(key, 0), by = bin_axis(key, 1), bz = bin_axis(key, 2);
+    uint64_t kept = 0, cand = 0;
+    uint64_t o = EMIT ? ends[e] - count[e] : 0;
+    for (uint64_t f = e + 1; f < total && keys[f] == key; ++f) {
+        const uint32_t u = vals[f];
+        if (kin_v && kinematic(flags[u])) continue;
+        // emit from the lowest corner of the two ranges' intersection only
+        const int32_t *lu = lohi + size_t(u) * 6, *lv = lohi + size_t(v) * 6;
+        if (max(lu[0], lv[0]) != bx || max(lu[1], lv[1]) != by || max(lu[2], lv[2]) != bz) continue;
+        ++cand;
+        const uint32_t i = min(u, v), j = max(u, v);
+        if (local_continuum(rest, adj, i, j)) continue;
+        if (EMIT) out[o++] = (uint64_t(i) << 32) | j;
+        ++kept;
+    }
+    if (!EMIT) {
+        count[e] = kept;
+        if (cand) atomicAdd(candidates, (unsigned long long)cand);
+    }
+}
+
+// ---- narrow phase: the reference's tests, f32 ---------------------------------
+
+struct V3 {
+    float x[3];
+};
+__device__ __forceinline__ V3 ld3(const float *p, uint32_t i) {
+    return V3{{p[size_t(i) * 3], p[size_t(i) * 3 + 1], p[size_t(i) * 3 + 2]}};
+}
+__device__ __forceinline__ V3 sub3(const V3 &a, const V3 &b) {
+    return V3{{a.x[0] - b.x[0], a.x[1] - b.x[1], a.x[2] - b.x[2]}};
+}
+// glam Vec3::dot: (x*x' + y*y') + z*z'
+__device__ __forceinline__ float dot3(const V3 &a, const V3 &b) {
+    return a.x[0] * b.x[0] + a.x[1] * b.x[1] + a.x[2] * b.x[2];
+}
+// Component `axis` of v without a runtime-indexed array (which would live in memory).
+__device__ __forceinline__ float comp(const V3 &v, int axis) {
+    return axis == 0 ? v.x[0] : (axis == 1 ? v.x[1] : v.x[2]);
+}
+__device__ __forceinline__ V3 axis_normal(int axis, float sign) {
+    return V3{{axis == 0 ? sign : 0.0f, axis == 1 ? sign : 0.0f, axis == 2 ? sign : 0.0f}};
+}
+
+// swept_normal (src/collision.rs:552-596): the axis, or -1.
+__device__ int swept_normal(const V3 &si, const V3 &sj, const V3 &ei, const V3 &ej, float *sign_out) {
+    const V3 d0 = sub3(sj, si), rel = sub3(sub3(ej, sj), sub3(ei, si));
+    const float close = kCell - 1e-6f;
+    if (fabsf(d0.x[0]) <= close && fabsf(d0.x[1]) <= close && fabsf(d0.x[2]) <= close) return -1;
+    float enter = 0.0f, exit = 1.0f, sign = 1.0f;
+    int hit = -1;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if (fabsf(rel.x[a]) < 1e-12f) {
+            if (fabsf(d0.x[a]) > kCell) return -1;
+            continue;
+        }
+        const float t0 = (-kCell - d0.x[a]) / rel.x[a];
+        const float t1 = (kCell - d0.x[a]) / rel.x[a];
+        const float near_t = fminf(t0, t1), far_t = fmaxf(t0, t1);
+        if (near_t > enter || (hit < 0 && near_t >= enter)) {
+            enter = near_t;
+            hit = a;
+            sign = (d0.x[a] + rel.x[a] * enter >= 0.0f) ? 1.0f : -1.0f;
+        }
+        exit = fminf(exit, far_t);
+        if (enter > exit) return -1;
+    }
+    if (exit < 0.0f || enter > 1.0f) return -1;
+    *sign_out = sign;
+    return hit;
+}
+
+// overlap_normal (src/collision.rs:598-621): the axis, or -1.
+__device__ int overlap_normal(const V3 &a, const V3 &b, const V3 &fallback, float *sign_out, float *depth) {
+    const V3 d = sub3(b, a);
+    const float o0 = kCell - fabsf(d.x[0]), o1 = kCell - fabsf(d.x[1]), o2 = kCell - fabsf(d.x[2]);
+    if (fminf(o0, fminf(o1, o2)) <= 0.0f) return -1;
+    const int axis = (o0 <= o1 && o0 <= o2) ? 0 : (o1 <= o2 ? 1 : 2);
+    float src = 1.0f;
+    const float da = comp(d, axis), fa = comp(fallback, axis);
+    if (fabsf(da) > 1e-8f)
+        src = da;
+    else if (fabsf(fa) > 1e-8f)
+        src = fa;
+    *sign_out = signbit(src) ? -1.0f : 1.0f;  // f32::signum of a nonzero value
+    *depth = axis == 0 ? o0 : (axis == 1 ? o1 : o2);
+    return axis;
+}
+
+// touching_normal (src/collision.rs:441-460): the axis, or -1.
+__device__ int touching_normal(const V3 &a, const V3 &b, float *sign_out) {
+    const float slop = 2.0e-6f;
+    const V3 d = sub3(b, a);
+    const V3 g{{fabsf(d.x[0]) - kCell, fabsf(d.x[1]) - kCell, fabsf(d.x[2]) - kCell}};
+    const int axis = (g.x[0] >= g.x[1] && g.x[0] >= g.x[2]) ? 0 : (g.x[1] >= g.x[2] ? 1 : 2);
+    if (comp(g, axis) > slop) return -1;
+#pragma unroll
+    for (int o = 0; o < 3; ++o)
+        if (o != axis && g.x[o] >= -slop) return -1;
+    *sign_out = comp(d, axis) >= 0.0f ? 1.0f : -1.0f;
+    return axis;
+}
+
+// inverse_mass (src/collision.rs:633-640)
+__device__ __forceinline__ float inverse_mass(uint8_t f, float mass, int axis) {
+    return (kinematic(f) || ((f >> (1 + axis)) & 1)) ? 0.0f : 1.0f / mass;
+}
+
+// The outcome of one pair evaluated on the current positions.
+struct Outcome {
+    uint32_t i = 0, j = 0;
+    uint8_t code = 0;       // 0: no contact; else 0x80 | axis | 4 (normal negative) | 8 (new impact)
+    bool sep = false;       // separate() ran with a positive denominator (src/collision.rs:650-652)
+    bool movable_i = false, movable_j = false;  // not kinematic
+    uint8_t real_i = 0, real_j = 0;  // components with a nonzero delta
+    uint8_t plus0_i = 0, plus0_j = 0;  // components with a +0 delta
+    float di[3] = {0, 0, 0}, dj[3] = {0, 0, 0};
+};
+
+__device__ __forceinline__ bool is_zero(float x) { return (__float_as_uint(x) & 0x7fffffffu) == 0; }
+
+// One pass body of project (src/collision.rs:55-109) for pair (i, j).
+__device__ Outcome evaluate(uint32_t i, uint32_t j, const float *pos, const float *__restrict__ start,
+                            const float *__restrict__ mass, const uint8_t *__restrict__ flags) {
+    Outcome o;
+    o.i = i;
+    o.j = j;
+    const uint8_t fi = flags[i], fj = flags[j];
+    o.movable_i = !kinematic(fi);
+    o.movable_j = !kinematic(fj);
+    const V3 si = ld3(start, i), sj = ld3(start, j), pi = ld3(pos, i), pj = ld3(pos, j);
+    const V3 start_delta = sub3(sj, si);
+    float sign = 1.0f, amount = 0.0f, depth = 0.0f;
+    bool contact = false, impact = false, do_sep = false;
+    int axis = swept_normal(si, sj, pi, pj, &sign);
+    if (axis >= 0) {
+        const V3 nrm = axis_normal(axis, sign);
+        const float separation = dot3(sub3(pj, pi), nrm);
+        if (separation < kCell) {
+            amount = kCell - separation + 1e-6f;
+            do_sep = true;
+        }
+        contact = true;
+        impact = dot3(start_delta, nrm) > kCell + 2e-6f;
+    } else if ((axis = overlap_normal(pi, pj, start_delta, &sign, &depth)) >= 0) {
+        amount = depth + 1e-6f;
+        do_sep = true;
+        contact = true;
+        impact = dot3(start_delta, axis_normal(axis, sign)) > kCell + 2e-6f;
+    } else if ((axis = touching_normal(pi, pj, &sign)) >= 0) {
+        contact = true;
+    }
+    if (contact) o.code = uint8_t(0x80 | axis | (sign < 0.0f ? 4 : 0) | (impact ? 8 : 0));
+    if (do_sep) {
+        // separate (src/collision.rs:642-665)
+        const float wi = inverse_mass(fi, o.movable_i ? mass[i] : 1.0f, axis);
+        const float wj = inverse_mass(fj, o.movable_j ? mass[j] : 1.0f, axis);
+        const float denom = wi + wj;
+        if (denom > 0.0f) {
+            o.sep = true;
+            const float s_i = amount * wi / denom, s_j = amount * wj / denom;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const float nc = c == axis ? sign : 0.0f;
+                o.di[c] = -nc * s_i;
+                o.dj[c] = nc * s_j;
+                if (!is_zero(o.di[c])) o.real_i |= uint8_t(1 << c);
+                else if (__float_as_uint(o.di[c]) == 0u) o.plus0_i |= uint8_t(1 << c);
+                if (!is_zero(o.dj[c])) o.real_j |= uint8_t(1 << c);
+                else if (__float_as_uint(o.dj[c]) == 0u) o.plus0_j |= uint8_t(1 << c);
+            }
+        }
+    }
+    return o;
+}
+
+// ---- projection ----------------------------------------------------------------
+
+constexpr int kR = 2;                        // pairs per thread and window
+constexpr uint32_t kWin = uint32_t(kGs) * kR;  // window slots: s = r * kGs + threadIdx.x
+
+// One window slot. A slot holds a pair until the pair commits; a committed or
+// empty slot takes the next pair of the list. Pairs are ordered by their index
+// k in the sorted list, whatever slot they sit in. Flags are packed in one
+// word to keep kR slots in registers.
+enum : uint32_t {
+    S_LIVE = 1u << 0, S_FRESH = 1u << 1, S_SEP = 1u << 2, S_MOV_I = 1u << 3, S_MOV_J = 1u << 4,
+    S_REAL_I = 1u << 5, S_REAL_J = 1u << 6, S_HELD = 1u << 7, S_MARKED = 1u << 8,
+};
+constexpr int S_AXIS = 9;     // 2 bits: the normal axis
+constexpr int S_PLUS0 = 11;   // 6 bits: +0 components of i (3) and j (3)
+constexpr int S_CODE = 24;    // 8 bits: contact code
+struct Slot {
+    uint32_t k, i, j, f;
+    float di, dj;  // the deltas along the normal axis (the others are +-0)
+};
+
+__device__ __forceinline__ void set_outcome(Slot &sl, const Outcome &o) {
+    uint32_t f = sl.f & (S_LIVE | S_FRESH);
+    if (o.sep) f |= S_SEP;
+    if (o.movable_i) f |= S_MOV_I;
+    if (o.movable_j) f |= S_MOV_J;
+    if (o.real_i) f |= S_REAL_I;
+    if (o.real_j) f |= S_REAL_J;
+    f |= (uint32_t(o.plus0_i) | (uint32_t(o.plus0_j) << 3)) << S_PLUS0;
+    f |= uint32_t(o.code) << S_CODE;
+    uint32_t axis = 0;
+    float di = 0.0f, dj = 0.0f;
+    // only the normal axis can carry a nonzero delta
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        if (o.real_i & (1 << c)) {
+            axis = uint32_t(c);
+            di = o.di[c];
+        }
+        if (o.real_j & (1 << c)) {
+            axis = uint32_t(c);
+            dj = o.dj[c];
+        }
+    }
+    sl.f = f | (axis << S_AXIS);
+    sl.di = di;
+    sl.dj = dj;
+}
+
+// Wait until this wave's global stores and atomics are performed, so that the
+// barrier after it publishes them to the other waves.
+__device__ __forceinline__ void wait_stores() { __builtin_amdgcn_s_waitcnt(0x0F70); }  // vmcnt(0)
+
+__device__ __forceinline__ uint32_t mark_load(const uint32_t *m, uint32_t v) {
+    return __hip_atomic_load(m + v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void mark_clear(uint32_t *m, uint32_t v) {
+    __hip_atomic_store(m + v, kNoMark, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void add_plus0(float *p, uint32_t v, uint32_t mask) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+        if ((mask & (1u << c)) && __float_as_uint(p[size_t(v) * 3 + c]) == 0x80000000u) p[size_t(v) * 3 + c] = 0.0f;
+}
+
+// Per-wave popcounts of NF flags for each of the kR slot rows, exchanged
+// through LDS in one barrier. total[f] is the workgroup count of flag f;
+// rank[r] is the position of slot (r, thread) among the flag-0 slots in slot
+// order. The two LDS rows alternate between calls: a row is rewritten only
+// after every thread has passed the next call's barrier.
+template <int NF>
+__device__ __forceinline__ void slot_counts(const bool (&f)[NF][kR], uint32_t (*cnt)[NF][kR][kGsWaves], int &row,
+                                            uint32_t (&total)[NF], uint32_t (&rank)[kR]) {
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint64_t b0[kR];
+#pragma unroll
+    for (int q = 0; q < NF; ++q)
+#pragma unroll
+        for (int r = 0; r < kR; ++r) {
+            const uint64_t b = __ballot(f[q][r]);
+            if (q == 0) b0[r] = b;
+            if (lane == 0) cnt[row][q][r][wave] = uint32_t(__popcll(b));
+        }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NF; ++q) {
+        uint32_t t = 0;
+        for (int r = 0; r < kR; ++r)
+            for (int w = 0; w < kGsWaves; ++w) t += cnt[row][q][r][w];
+        total[q] = t;
+    }
+    uint32_t before = 0;
+#pragma unroll
+    for (int r = 0; r < kR; ++r) {
+        uint32_t in_row = 0;
+        for (int w = 0; w < kGsWaves; ++w) in_row += (uint32_t(w) < wave) ? cnt[row][0][r][w] : 0u;
+        rank[r] = before + in_row + uint32_t(__popcll(b0[r] & ((uint64_t(1) << lane) - 1)));
+        for (int w = 0; w < kGsWaves; ++w) before += cnt[row][0][r][w];
+    }
+    row ^= 1;
+}
+
+// The Gauss-Seidel passes of project (src/collision.rs:54-109) over the
+// sorted pairs, window by window.
+//   markW[v]: lowest pair index of the window whose evaluation moves voxel v;
+//   markB[v]: lowest held pair index of the window touching movable voxel v.
+// A pair k is held when markW or markB of one of its voxels is below k; the
+// held set is closed under that rule before anything commits. Both arrays
+// are all-ones between windows. The +0-over--0 stores of a window's committed
+// pairs are made at the start of the next window: pair tests do not depend on
+// the sign of a zero, and the next real moves come after that window's first
+// barrier. stats: passes, windows.
+__global__ __launch_bounds__(kGs) void k_project(const uint64_t *__restrict__ pairs, uint32_t npairs, float *pos,
+                                                 float *prev, const float *__restrict__ start,
+                                                 const float *__restrict__ mass, const uint8_t *__restrict__ flags,
+                                                 uint32_t *markW, uint32_t *markB, uint8_t *code,
+                                                 uint32_t *stats) {
+    __shared__ uint32_t cnt1[2][1][kR][kGsWaves];
+    __shared__ uint32_t cnt2[2][2][kR][kGsWaves];
+    const uint32_t t = threadIdx.x;
+    int row1 = 0, row2 = 0;
+    uint32_t passes = 0, windows = 0;
+    Slot sl[kR];
+    uint32_t fx_i[kR], fx_j[kR], fx_m[kR];  // deferred +0 stores: voxels and masks (i: bits 0-2, j: 3-5)
+#pragma unroll
+    for (int r = 0; r < kR; ++r) fx_m[r] = 0;
+    for (int pass = 0; pass < kPasses; ++pass) {
+        ++passes;
+        bool changed = false;
+#pragma unroll
+        for (int r = 0; r < kR; ++r) {
+            const uint32_t s = uint32_t(r) * kGs + t;
+            sl[r].k = s;
+            sl[r].f = (s < npairs ? S_LIVE : 0u) | S_FRESH;
+        }
+        uint32_t next = min(kWin, npairs);
+        uint32_t live_total = next;
+        while (live_total > 0) {
+            ++windows;
+#pragma unroll
+            for (int r = 0; r < kR; ++r)
+                if (fx_m[r]) {
+                    add_plus0(pos, fx_i[r], fx_m[r] & 7u);
+                    add_plus0(prev, fx_i[r], fx_m[r] & 7u);
+                    add_plus0(pos, fx_j[r], fx_m[r] >> 3);
+                    add_plus0(prev, fx_j[r], fx_m[r] >> 3);
+                    fx_m[r] = 0;
+                }
+#pragma unroll
+            for (int r = 0; r < kR; ++r) {
+                Slot &q = sl[r];
+                if (!(q.f & S_LIVE)) continue;
+                if (q.f & S_FRESH) {
+                    const uint64_t key = pairs[q.k];
+                    q.i = uint32_t(key >> 32);
+                    q.j = uint32_t(key);
+                }
+                set_outcome(q, evaluate(q.i, q.j, pos, start, mass, flags));
+                if (q.f & S_REAL_I) atomicMin(markW + q.i, q.k);
+                if (q.f & S_REAL_J) atomicMin(markW + q.j, q.k);
+            }
+            wait_stores();
+            __syncthreads();
+            bool fl1[1][kR];
+            uint32_t tot1[1], rk[kR];
+#pragma unroll
+            for (int r = 0; r < kR; ++r) {
+                Slot &q = sl[r];
+                const bool held = (q.f & S_LIVE) && (mark_load(markW, q.i) < q.k || mark_load(markW, q.j) < q.k);
+                if (held) q.f |= S_HELD;
+                fl1[0][r] = held;
+            }
+            slot_counts<1>(fl1, cnt1, row1, tot1, rk);
+            if (tot1[0] > 0) {
+                for (;;) {
+#pragma unroll
+                    for (int r = 0; r < kR; ++r) {
+                        Slot &q = sl[r];
+                        if ((q.f & S_HELD) && !(q.f & S_MARKED)) {
+                            if (q.f & S_MOV_I) atomicMin(markB + q.i, q.k);
+                            if (q.f & S_MOV_J) atomicMin(markB + q.j, q.k);
+                            q.f |= S_MARKED;
+                        }
+                    }
+                    wait_stores();
+                    __syncthreads();
+#pragma unroll
+                    for (int r = 0; r < kR; ++r) {
+                        Slot &q = sl[r];
+                        const bool newly = (q.f & S_LIVE) && !(q.f & S_HELD) &&
+                                           (mark_load(markB, q.i) < q.k || mark_load(markB, q.j) < q.k);
+                        if (newly) q.f |= S_HELD;
+                        fl1[0][r] = newly;
+                    }
+                    slot_counts<1>(fl1, cnt1, row1, tot1, rk);
+                    if (tot1[0] == 0) break;
+                }
+            }
+            bool fl2[2][kR];
+#pragma unroll
+            for (int r = 0; r < kR; ++r) {
+                Slot &q = sl[r];
+                const bool live = q.f & S_LIVE;
+                const bool commit = live && !(q.f & S_HELD);
+                const bool sep = commit && (q.f & S_SEP);
+                if (sep) {
+                    const size_t a = (q.f >> S_AXIS) & 3u;
+                    if (q.f & S_REAL_I) {
+                        pos[size_t(q.i) * 3 + a] += q.di;
+                        prev[size_t(q.i) * 3 + a] += q.di;
+                    }
+                    if (q.f & S_REAL_J) {
+                        pos[size_t(q.j) * 3 + a] += q.dj;
+                        prev[size_t(q.j) * 3 + a] += q.dj;
+                    }
+                    fx_i[r] = q.i;
+                    fx_j[r] = q.j;
+                    fx_m[r] = (q.f >> S_PLUS0) & 63u;
+                }
+                // the contact of the first pass that finds one (contacts are deduplicated per pair)
+                const uint8_t c = uint8_t(q.f >> S_CODE);
+                if (commit && c && code[q.k] == 0) code[q.k] = c;
+                if (live) {
+                    if (q.f & S_REAL_I) mark_clear(markW, q.i);
+                    if (q.f & S_REAL_J) mark_clear(markW, q.j);
+                    if (q.f & S_MARKED) {
+                        if (q.f & S_MOV_I) mark_clear(markB, q.i);
+                        if (q.f & S_MOV_J) mark_clear(markB, q.j);
+                    }
+                }
+                fl2[0][r] = !live || commit;  // the slot takes a new pair
+                fl2[1][r] = sep;
+            }
+            wait_stores();
+            uint32_t tot2[2];
+            slot_counts<2>(fl2, cnt2, row2, tot2, rk);
+            if (tot2[1] > 0) changed = true;
+            const uint32_t added = min(tot2[0], npairs - next);
+#pragma unroll
+            for (int r = 0; r < kR; ++r) {
+                Slot &q = sl[r];
+                if (fl2[0][r]) {
+                    q.k = next + rk[r];
+                    q.f = (rk[r] < added ? S_LIVE : 0u) | S_FRESH;
+                } else {
+                    q.f = S_LIVE;  // held: evaluated again on the next window
+                }
+            }
+            next += added;
+            live_total = (kWin - tot2[0]) + added;
+        }
+        if (!changed) break;
+    }
+#pragma unroll
+    for (int r = 0; r < kR; ++r)
+        if (fx_m[r]) {
+            add_plus0(pos, fx_i[r], fx_m[r] & 7u);
+            add_plus0(prev, fx_i[r], fx_m[r] & 7u);
+            add_plus0(pos, fx_j[r], fx_m[r] >> 3);
+            add_plus0(prev, fx_j[r], fx_m[r] >> 3);
+        }
+    if (t == 0) {
+        stats[0] = passes;
+        stats[1] = windows;
+    }
+}
+
+// ---- contacts out ----------------------------------------------------------------
+
+__global__ void k_contact_flags(uint32_t npairs, const uint8_t *__restrict__ code, uint64_t *__restrict__ flag) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < npairs) flag[k] = code[k] ? 1 : 0;
+}
+
+__global__ void k_contact_scatter(uint32_t npairs, const uint64_t *__restrict__ pairs,
+                                  const uint8_t *__restrict__ code, const uint64_t *__restrict__ ends,
+                                  vx_contact *__restrict__ out) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= npairs || !code[k]) return;
+    const uint8_t c = code[k];
+    vx_contact r;
+    r.i = uint32_t(pairs[k] >> 32);
+    r.j = uint32_t(pairs[k]);
+    for (int a = 0; a < 3; ++a) r.normal[a] = 0.0f;
+    r.normal[c & 3] = (c & 4) ? -1.0f : 1.0f;
+    r.new_impact = (c & 8) ? 1u : 0u;
+    out[ends[k] - 1] = r;
+}
+
+inline uint32_t blocks_for(uint64_t n, uint32_t per) { return uint32_t((n + per - 1) / per); }
+
+}  // namespace
+}  // namespace vx
+
+using namespace vx;
+
+extern "C" int32_t vx_contact_project(uint32_t n, float *positions, float *previous, const float *rest_positions,
+                                      const float *mass, const uint8_t *kinematic_flags, const uint8_t *target_axes,
+                                      const uint32_t *bond_neighbor, vx_contact *contacts, uint64_t capacity,
+                                      vx_contact_report *report) {
+    clear_error();
+    vx_contact_report rep;
+    std::memset(&rep, 0, sizeof rep);
+    if (report) *report = rep;
+    if (n > 0 && (!positions || !previous || !rest_positions || !mass || !kinematic_flags || !bond_neighbor))
+        return fail(VX_ERR_INVALID_ARGUMENT, "vx_contact_project: null input array");
+    if (capacity > 0 && !contacts) return fail(VX_ERR_INVALID_ARGUMENT, "vx_contact_project: null contacts");
+    if (n == 0) {
+        rep.passes = 1;  // one empty pass, as the reference's loop
+        if (report) *report = rep;
+        return VX_OK;
+    }
+    int32_t st = ensure_init();
+    if (st != VX_OK) return st;
+    Context &ctx = context();
+    std::lock_guard<std::recursive_mutex> lock(ctx.mutex);
+    hipStream_t s = ctx.stream;
+    const size_t N = n;
+
+    float *d_pos, *d_prev, *d_start, *d_rest, *d_mass;
+    uint8_t *d_flags;
+    uint32_t *d_bond, *d_adj, *d_err, *d_markW, *d_markB, *d_stats;
+    int32_t *d_lohi;
+    uint64_t *d_cnt, *d_ends;
+    unsigned long long *d_cand;
+#define VX_BUF(name, bytes, ptr)                                               \
+    do {                                                                       \
+        int32_t b_ = buffer(name, (bytes), reinterpret_cast<void **>(&(ptr))); \
+        if (b_ != VX_OK) return b_;                                            \
+    } while (0)
+    VX_BUF("contact.pos", 12 * N, d_pos);
+    VX_BUF("contact.prev", 12 * N, d_prev);
+    VX_BUF("contact.start", 12 * N, d_start);
+    VX_BUF("contact.rest", 12 * N, d_rest);
+    VX_BUF("contact.mass", 4 * N, d_mass);
+    VX_BUF("contact.flags", N, d_flags);
+    VX_BUF("contact.bond", 12 * N, d_bond);
+    VX_BUF("contact.adj", 24 * N, d_adj);
+    VX_BUF("contact.lohi", 24 * N, d_lohi);
+    VX_BUF("contact.count", 8 * N, d_cnt);
+    VX_BUF("contact.ends", 8 * N, d_ends);
+    VX_BUF("contact.markW", 4 * N, d_markW);
+    VX_BUF("contact.markB", 4 * N, d_markB);
+    VX_BUF("contact.small", 64, d_err);
+    d_stats = d_err + 4;
+    d_cand = reinterpret_cast<unsigned long long *>(d_err + 8);
+
+    std::vector<uint8_t> flags(N);
+    for (size_t v = 0; v < N; ++v)
+        flags[v] = uint8_t((kinematic_flags[v] ? 1 : 0) | ((target_axes ? (target_axes[v] & 7) : 0) << 1));
+    VX_HIP(hipMemcpyAsync(d_pos, positions, 12 * N, hipMemcpyHostToDevice, s));
+    VX_HIP(hipMemcpyAsync(d_prev, previous, 12 * N, hipMemcpyHostToDevice, s));
+    VX_HIP(hipMemcpyAsync(d_rest, rest_positions, 12 * N, hipMemcpyHostToDevice, s));
+    VX_HIP(hipMemcpyAsync(d_mass, mass, 4 * N, hipMemcpyHostToDevice, s));
+    VX_HIP(hipMemcpyAsync(d_flags, flags.data(), N, hipMemcpyHostToDevice, s));
+    VX_HIP(hipMemcpyAsync(d_bond, bond_neighbor, 12 * N, hipMemcpyHostToDevice, s));
+
+    hipEvent_t ev[2];
+    VX_HIP(hipEventCreate(&ev[0]));
+    VX_HIP(hipEventCreate(&ev[1]));
+    struct EventGuard {
+        hipEvent_t *e;
+        ~EventGuard() {
+            (void)hipEventDestroy(e[0]);
+            (void)hipEventDestroy(e[1]);
+        }
+    } guard{ev};
+    VX_HIP(hipEventRecord(ev[0], s));
+    // let starts = scene.previous.clone() (src/collision.rs:36)
+    VX_HIP(hipMemcpyAsync(d_start, d_prev, 12 * N, hipMemcpyDeviceToDevice, s));
+    VX_HIP(hipMemsetAsync(d_adj, 0xFF, 24 * N, s));
+    VX_HIP(hipMemsetAsync(d_markW, 0xFF, 4 * N, s));
+    VX_HIP(hipMemsetAsync(d_markB, 0xFF, 4 * N, s));
+    VX_HIP(hipMemsetAsync(d_err, 0, 64, s));
+    k_adjacency<<<blocks_for(3 * N, 256), 256, 0, s>>>(n, d_bond, d_adj, d_err);
+    k_bins<<<blocks_for(N, 256), 256, 0, s>>>(n, d_start, d_pos, d_lohi, d_cnt, d_err);
+    VX_HIP(hipGetLastError());
+
+    // scratch for rocprim: grown to the largest request
+    auto tmp = [&](size_t bytes, void **p) -> int32_t { return buffer("contact.tmp", bytes ? bytes : 16, p); };
+    size_t tb = 0;
+    void *d_tmp = nullptr;
+    VX_HIP(rocprim::inclusive_scan(nullptr, tb, d_cnt, d_ends, N, rocprim::plus<uint64_t>(), s));
+    if ((st = tmp(tb, &d_tmp)) != VX_OK) return st;
+    VX_HIP(rocprim::inclusive_scan(d_tmp, tb, d_cnt, d_ends, N, rocprim::plus<uint64_t>(), s));
+
+    uint32_t *pinned = static_cast<uint32_t *>(ctx.pinned);
+    uint64_t *pin64 = reinterpret_cast<uint64_t *>(pinned + 64);
+    VX_HIP(hipMemcpyAsync(pinned, d_err, 4, hipMemcpyDeviceToHost, s));
+    VX_HIP(hipMemcpyAsync(pin64, d_ends + (N - 1), 8, hipMemcpyDeviceToHost, s));
+    VX_HIP(hipStreamSynchronize(s));
+    if (pinned[0] & 1) return fail(VX_ERR_INVALID_ARGUMENT, "vx_contact_project: bond neighbor index out of range");
+    if (pinned[0] & 2)
+        return fail(VX_ERR_INVALID_ARGUMENT,
+                    "vx_contact_project: a swept box has no finite bound on some axis or reaches 2^20 cells from the origin");
+    const uint64_t entries = pin64[0];
+    if (entries > (uint64_t(1) << 31))
+        return fail(VX_ERR_TOO_LARGE, "vx_contact_project: %llu bin entries (swept boxes too large)",
+                    (unsigned long long)entries);
+
+    uint64_t *d_keys, *d_keys2, *d_ecount, *d_eends;
+    uint32_t *d_vals, *d_vals2;
+    VX_BUF("contact.keys", 8 * entries, d_keys);
+    VX_BUF("contact.keys2", 8 * entries, d_keys2);
+    VX_BUF("contact.vals", 4 * entries, d_vals);
+    VX_BUF("contact.vals2", 4 * entries, d_vals2);
+    VX_BUF("contact.ecount", 8 * entries, d_ecount);
+    VX_BUF("contact.eends", 8 * entries, d_eends);
+    k_emit_bins<<<blocks_for(N, 256), 256, 0, s>>>(n, d_lohi, d_cnt, d_ends, d_keys, d_vals);
+    VX_HIP(hipGetLastError());
+    tb = 0;
+    VX_HIP(rocprim::radix_sort_pairs(nullptr, tb, d_keys, d_keys2, d_vals, d_vals2, size_t(entries), 0, 63, s));
+    if ((st = tmp(tb, &d_tmp)) != VX_OK) return st;
+    VX_HIP(rocprim::radix_sort_pairs(d_tmp, tb, d_keys, d_keys2, d_vals, d_vals2, size_t(entries), 0, 63, s));
+    const uint32_t eblocks = blocks_for(entries, 256);
+    k_bin_pairs<false><<<eblocks, 256, 0, s>>>(entries, d_keys2, d_vals2, d_lohi, d_flags, d_rest, d_adj, d_ecount,
+                                               nullptr, nullptr, d_cand);
+    VX_HIP(hipGetLastError());
+    tb = 0;
+    VX_HIP(rocprim::inclusive_scan(nullptr, tb, d_ecount, d_eends, size_t(entries), rocprim::plus<uint64_t>(), s));
+    if ((st = tmp(tb, &d_tmp)) != VX_OK) return st;
+    VX_HIP(rocprim::inclusive_scan(d_tmp, tb, d_ecount, d_eends, size_t(entries), rocprim::plus<uint64_t>(), s));
+    VX_HIP(hipMemcpyAsync(pin64, d_eends + (entries - 1), 8, hipMemcpyDeviceToHost, s));
+    VX_HIP(hipMemcpyAsync(pin64 + 1, d_cand, 8, hipMemcpyDeviceToHost, s));
+    VX_HIP(hipStreamSynchronize(s));
+    const uint64_t npairs = pin64[0];
+    rep.candidate_pairs = pin64[1];
+    rep.pairs = npairs;
+    if (npairs >= (uint64_t(1) << 32))
+        return fail(VX_ERR_TOO_LARGE, "vx_contact_project: %llu contact pairs", (unsigned long long)npairs);
+    const uint32_t P = uint32_t(npairs);
+
+    uint64_t *d_pk, *d_pairs, *d_cflag, *d_cends;
+    uint8_t *d_code;
+    vx_contact *d_out;
+    VX_BUF("contact.pk", 8 * size_t(P) + 8, d_pk);
+    VX_BUF("contact.pairs", 8 * size_t(P) + 8, d_pairs);
+    VX_BUF("contact.code", size_t(P) + 8, d_code);
+    VX_BUF("contact.cflag", 8 * size_t(P) + 8, d_cflag);
+    VX_BUF("contact.cends", 8 * size_t(P) + 8, d_cends);
+    uint64_t ncontacts = 0;
+    if (P > 0) {
+        k_bin_pairs<true><<<eblocks, 256, 0, s>>>(entries, d_keys2, d_vals2, d_lohi, d_flags, d_rest, d_adj, d_ecount,
+                                                  d_eends, d_pk, nullptr);
+        VX_HIP(hipGetLastError());
+        tb = 0;
+        VX_HIP(rocprim::radix_sort_keys(nullptr, tb, d_pk, d_pairs, size_t(P), 0, 64, s));
+        if ((st = tmp(tb, &d_tmp)) != VX_OK) return st;
+        VX_HIP(rocprim::radix_sort_keys(d_tmp, tb, d_pk, d_pairs, size_t(P), 0, 64, s));
+        VX_HIP(hipMemsetAsync(d_code, 0, P, s));
+        k_project<<<1, kGs, 0, s>>>(d_pairs, P, d_pos, d_prev, d_start, d_mass, d_flags, d_markW, d_markB, d_code,
+                                    d_stats);
+        VX_HIP(hipGetLastError());
+        k_contact_flags<<<blocks_for(P, 256), 256, 0, s>>>(P, d_code, d_cflag);
+        tb = 0;
+        VX_HIP(rocprim::inclusive_scan(nullptr, tb, d_cflag, d_cends, size_t(P), rocprim::plus<uint64_t>(), s));
+        if ((st = tmp(tb, &d_tmp)) != VX_OK) return st;
+        VX_HIP(rocprim::inclusive_scan(d_tmp, tb, d_cflag, d_cends, size_t(P), rocprim::plus<uint64_t>(), s));
+        VX_HIP(hipMemcpyAsync(pin64, d_cends + (P - 1), 8, hipMemcpyDeviceToHost, s));
+        VX_HIP(hipMemcpyAsync(pinned, d_stats, 8, hipMemcpyDeviceToHost, s));
+        VX_HIP(hipEventRecord(ev[1], s));
+        VX_HIP(hipStreamSynchronize(s));
+        ncontacts = pin64[0];
+        rep.passes = pinned[0];
+        rep.rounds = pinned[1];
+    } else {
+        VX_HIP(hipEventRecord(ev[1], s));
+        VX_HIP(hipStreamSynchronize(s));
+        rep.passes = 1;
+    }
+    float ms = 0.0f;
+    VX_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+    rep.device_us = double(ms) * 1000.0;
+    rep.contacts = ncontacts;
+    if (report) *report = rep;
+    if (ncontacts > capacity)
+        return fail(VX_ERR_TOO_LARGE, "vx_contact_project: %llu contacts exceed the capacity %llu",
+                    (unsigned long long)ncontacts, (unsigned long long)capacity);
+    if (ncontacts > 0) {
+        VX_BUF("contact.out", sizeof(vx_contact) * ncontacts, d_out);
+        k_contact_scatter<<<blocks_for(P, 256), 256, 0, s>>>(P, d_pairs, d_code, d_cends, d_out);
+        VX_HIP(hipGetLastError());
+        VX_HIP(hipMemcpyAsync(contacts, d_out, sizeof(vx_contact) * ncontacts, hipMemcpyDeviceToHost, s));
+    }
+    VX_HIP(hipMemcpyAsync(positions, d_pos, 12 * N, hipMemcpyDeviceToHost, s));
+    VX_HIP(hipMemcpyAsync(previous, d_prev, 12 * N, hipMemcpyDeviceToHost, s));
+    VX_HIP(hipStreamSynchronize(s));
+#undef VX_BUF
+    return VX_OK;
+}

@@ -106,6 +106,16 @@ class BatchSceneReport(ctypes.Structure):
     ]
 
 
+class Contact(ctypes.Structure):
+    _fields_ = [("i", ctypes.c_uint32), ("j", ctypes.c_uint32), ("normal", ctypes.c_float * 3),
+                ("new_impact", ctypes.c_uint32)]
+
+
+class ContactReport(ctypes.Structure):
+    _fields_ = [("candidate_pairs", ctypes.c_uint64), ("pairs", ctypes.c_uint64), ("contacts", ctypes.c_uint64),
+                ("passes", ctypes.c_uint32), ("rounds", ctypes.c_uint32), ("device_us", ctypes.c_double)]
+
+
 # (name, restype, argtypes) for every exported function of include/vx_solver.h.
 SIGNATURES = [
     ("vx_last_error", ctypes.c_char_p, []),
@@ -147,6 +157,9 @@ SIGNATURES = [
     ("vx_scene_download_bonds", ctypes.c_int32, [ctypes.c_void_p, c_u32p, c_fp, c_fp, c_fp]),
     ("vx_scene_set_targets_and_loads", ctypes.c_int32, [ctypes.c_void_p, c_fp, c_fp]),
     ("vx_batch_download_bonds", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint32, c_u32p, c_fp, c_fp, c_fp]),
+    ("vx_contact_project", ctypes.c_int32,
+     [ctypes.c_uint32, c_fp, c_fp, c_fp, c_fp, c_u8p, c_u8p, c_u32p, ctypes.POINTER(Contact), ctypes.c_uint64,
+      ctypes.POINTER(ContactReport)]),
 ]
 
 

@@ -433,3 +433,49 @@ def rotate_scene(s: Scene, quat_xyzw) -> Scene:
     out = s.copy()
     out.positions = (s.positions.astype(np.float64) @ r.T).astype(np.float32)
     return out
+
+
+def contact_box(dims, seed: int = 7, broken: float = 0.02, jitter: float = 2e-4, sag: float = 3e-6,
+                movers: float = 0.002, mover_reach: float = 1.5, targeted: float = 0.01,
+                fixed_floor: bool = True) -> dict:
+    """Input of one contact projection (collision::project, src/collision.rs:33-123)
+    on a lattice box part way through a step: the scene fields it reads, f32.
+
+    Rest positions are the lattice centres; bonds follow the lattice rule, and
+    a fraction `broken` of them is removed (cracks). With `fixed_floor` the
+    z = 0 layer is kinematic (fixed, hence unbonded to the layer above). Each
+    movable voxel starts (previous) within jitter/2 of its rest position and
+    ends (positions) within a further `jitter`, sunk by `sag` (the load of a
+    settled block pressing its bottom layer into the floor); a fraction
+    `movers` starts up to `mover_reach` cells away instead, so its swept box
+    spans several bins. A fraction `targeted` of the movable voxels has one
+    axis held by a target. Masses are cell volumes at densities U[500, 8000].
+    """
+    nx, ny, nz = dims
+    n = nx * ny * nz
+    rng = SplitMix64(seed)
+    idx = np.arange(n, dtype=np.int64)
+    key = np.stack([idx % nx - nx // 2, (idx // nx) % ny - ny // 2, idx // (nx * ny)], axis=1)
+    rest = (key.astype(np.float32) * CELL + CELL / np.float32(2)).astype(np.float32)
+    kin = (key[:, 2] == 0) if fixed_floor else np.zeros(n, dtype=bool)
+    nb = lattice_topology(dims, kin)
+    cut = rng.uniform(3 * n).reshape(3, n) < broken
+    nb[cut & (nb != EMPTY)] = EMPTY
+    mass = (float(CELL) ** 3 * (500.0 + 7500.0 * rng.uniform(n))).astype(np.float32)
+    move = ~kin
+    prev = rest.astype(np.float64) + (0.5 * jitter) * (2.0 * rng.uniform(3 * n).reshape(n, 3) - 1.0) * move[:, None]
+    pos = prev + jitter * (2.0 * rng.uniform(3 * n).reshape(n, 3) - 1.0) * move[:, None]
+    pos[move, 2] -= sag
+    far = move & (rng.uniform(n) < movers)
+    direction = rng.normal(3 * n).reshape(n, 3)
+    direction /= np.maximum(np.linalg.norm(direction, axis=1, keepdims=True), 1e-12)
+    reach = mover_reach * float(CELL) * rng.uniform(n)
+    prev[far] = pos[far] - direction[far] * reach[far, None]
+    targets = np.zeros((n, 3), dtype=bool)
+    held = move & (rng.uniform(n) < targeted)
+    axis = (rng.uniform(n) * 3).astype(np.int64).clip(0, 2)
+    targets[held, axis[held]] = True
+    return {
+        "positions": pos.astype(np.float32), "previous": prev.astype(np.float32), "rest_positions": rest,
+        "mass": mass, "kinematic": kin, "bond_neighbor": nb, "targets": targets,
+    }
