@@ -371,6 +371,41 @@ def cut_bench(steps: int, dims=(192, 192, 96), preconditioner: int = 0):
     print(json.dumps(line))
 
 
+def contact_bench(dims=(64, 64, 64), reps: int = 3, cpu: bool = True) -> dict:
+    """Contact projection (collision::project) on the device against the oracle:
+    a lattice box with 2% of its bonds broken, a fixed floor, per-voxel jitter,
+    fast movers and targets (vxsim.scenes.contact_box). Device time with the
+    inputs resident (transfers excluded), median of `reps` calls; the oracle's
+    single-thread time for one call on the same input, and whether every
+    position, previous position and contact matched bit for bit."""
+    from vxsim import contact, scenes
+    s = scenes.contact_box(dims, seed=2)
+    targets = s.pop("targets")
+    runs = [contact.project(**s, targets=targets) for _ in range(reps)]
+    p, q, c, rep = runs[-1]
+    dev_us = sorted(r[3]["device_us"] for r in runs)[reps // 2]
+    proj_us = sorted(r[3]["projection_us"] for r in runs)[reps // 2]
+    out = {"workload": "contact_box %dx%dx%d seed 2: %d voxels" % (dims + (len(p),)),
+           "candidate_pairs": rep["candidate_pairs"], "pairs": rep["pairs"], "contacts": rep["contacts"],
+           "passes": rep["passes"], "windows": rep["rounds"], "device_ms": dev_us / 1e3,
+           "projection_ms": proj_us / 1e3, "broad_phase_ms": (dev_us - proj_us) / 1e3,
+           "us_per_window": proj_us / max(rep["rounds"], 1),
+           "pair_evaluations_per_s": rep["pairs"] * rep["passes"] / (proj_us * 1e-6)}
+    if cpu:
+        from oracle import oracle
+        t0 = time.perf_counter()
+        po, qo, co, st = oracle.contact_project(**s, targets=targets)
+        t_cpu = time.perf_counter() - t0
+        same = (np.array_equal(p.view(np.uint32), po.view(np.uint32))
+                and np.array_equal(q.view(np.uint32), qo.view(np.uint32))
+                and all(np.array_equal(c[k], co[k]) for k in ("i", "j", "new_impact"))
+                and np.array_equal(c["normal"].view(np.uint32), co["normal"].view(np.uint32)))
+        out["cpu_oracle"] = {"ms": 1000 * t_cpu, "cores": 1, "kind": "port",
+                             "device_speedup": 1000 * t_cpu / (dev_us / 1e3)}
+        out["bit_identical_to_oracle"] = bool(same)
+    return out
+
+
 def pmc_driver(large: bool = False, red_black: bool = False):
     """Short run for the rocprofv3 counter passes: 20 launches of the FP32
     bond SpMV on the assembled 64^3 system (or the 192x192x96 block) — with
@@ -414,6 +449,8 @@ def main():
     ap.add_argument("--large", action="store_true", help="192x192x96 block instead of the 64^3 box")
     ap.add_argument("--cut", action="store_true", help="192x192x96 block cut by the blade with fracture")
     ap.add_argument("--red-black", action="store_true", help="--cut / --serve with the red-black preconditioner")
+    ap.add_argument("--contact", action="store_true", help="contact projection leg only (64^3 contact box)")
+    ap.add_argument("--no-contact", action="store_true")
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--serve-revs", type=str, default="1e3,2e5")
@@ -427,6 +464,9 @@ def main():
         sys.exit(2)
     if args.pmc_driver:
         pmc_driver(args.large, red_black=args.red_black)
+        return
+    if args.contact:
+        print(json.dumps({"contact_projection": contact_bench(cpu=not args.no_cpu_baseline)}))
         return
     if args.cut:
         cut_bench(args.steps, preconditioner=1 if args.red_black else 0)
@@ -558,6 +598,9 @@ def main():
         rb = batched(preconditioner=1)
         rb["time_vs_block_jacobi"] = extra["batched"]["wall_s"] / rb["wall_s"]
         extra["batched_red_black"] = rb
+
+    if not args.no_contact:
+        extra["contact_projection"] = contact_bench(cpu=not args.no_cpu_baseline)
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -400,6 +400,7 @@ typedef struct vx_contact_report {
     uint32_t passes;          /* projection passes (at most 6) */
     uint32_t rounds;          /* device windows over the pair list */
     double device_us;         /* device time, inputs resident, no transfers */
+    double projection_us;     /* of which the Gauss-Seidel passes */
 } vx_contact_report;
 
 int32_t vx_contact_project(uint32_t n, float *positions, float *previous, const float *rest_positions,

@@ -1,5 +1,5 @@
 """The contact-projection oracle (collision::project, src/collision.rs:33-123)
-pinned by the reference's own collision tests (src/collision.rs:671-1016),
+pinned by the reference's own collision tests (src/collision.rs:667-1016),
 restated on the same configurations, by a brute-force broad phase and by
 the momentum balance of `separate`."""
 import numpy as np
@@ -53,7 +53,7 @@ def test_moving_voxel_cannot_tunnel_through_fixed_voxel():
 
 
 def test_touching_contact_persists_after_projection_slop():
-    # src/collision.rs:900-929: an impact, then the resting contact it leaves
+    # src/collision.rs:910-938: an impact, then the resting contact it leaves
     s = line_box((2, 1, 1))
     s["bond_neighbor"][:] = EMPTY  # two separate spawned voxels
     s["mass"][:] = np.float32(7870.0) * CELL ** 3
@@ -71,7 +71,7 @@ def test_touching_contact_persists_after_projection_slop():
 
 
 def test_rigidly_rotated_continuum_has_no_internal_box_contacts():
-    # src/collision.rs:671-688: a 3^3 box rotated 0.37 rad about z
+    # src/collision.rs:673-688: a 3^3 box rotated 0.37 rad about z
     s = line_box((3, 3, 3))
     c, sn = np.cos(0.37), np.sin(0.37)
     r = np.array([[c, -sn, 0], [sn, c, 0], [0, 0, 1]])
@@ -85,7 +85,7 @@ def test_rigidly_rotated_continuum_has_no_internal_box_contacts():
 
 
 def test_initially_overlapping_voxels_are_projected_to_touching():
-    # src/collision.rs:990-1015 (the projection inside that step): two voxels at one point
+    # src/collision.rs:994-1015 (the projection inside that step): two voxels at one point
     s = line_box((2, 1, 1))
     s["bond_neighbor"][:] = EMPTY
     mid = (s["positions"][0] + s["positions"][1]) * np.float32(0.5)

@@ -200,9 +200,6 @@ __global__ void k_bin_pairs(uint64_t total, const uint64_t *__restrict__ keys, c
 struct V3 {
     float x[3];
 };
-__device__ __forceinline__ V3 ld3(const float *p, uint32_t i) {
-    return V3{{p[size_t(i) * 3], p[size_t(i) * 3 + 1], p[size_t(i) * 3 + 2]}};
-}
 __device__ __forceinline__ V3 sub3(const V3 &a, const V3 &b) {
     return V3{{a.x[0] - b.x[0], a.x[1] - b.x[1], a.x[2] - b.x[2]}};
 }
@@ -296,18 +293,59 @@ struct Outcome {
 
 __device__ __forceinline__ bool is_zero(float x) { return (__float_as_uint(x) & 0x7fffffffu) == 0; }
 
+// True only when swept_normal (src/collision.rs:552-596) certainly returns
+// None, decided without its divisions: the starts already overlap on every
+// axis, or on some axis the pair starts apart and stays outside the slab
+// |d| <= CELL for the whole step. Still-standing pairs (|rel| < 1e-12) return
+// None in the reference right there; a receding pair has a slab exit time
+// below 0; an approaching pair enters after t = 1 when its gap exceeds
+// 1.002 |rel| (the margin covers the rounding of both sides, so the
+// reference's rounded entry time is above 1 as well). Either way the
+// reference returns None at that axis whatever the axes before it did.
+__device__ __forceinline__ bool swept_surely_none(const V3 &d0, const V3 &rel) {
+    const float close = kCell - 1e-6f;
+    if (fabsf(d0.x[0]) <= close && fabsf(d0.x[1]) <= close && fabsf(d0.x[2]) <= close) return true;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float d = d0.x[a], r = rel.x[a];
+        if (fabsf(r) < 1e-12f) {
+            if (fabsf(d) > kCell) return true;
+        } else if (d > kCell) {
+            if (r > 0.0f || (d - kCell) > 1.002f * -r) return true;
+        } else if (d < -kCell) {
+            if (r < 0.0f || (-kCell - d) > 1.002f * r) return true;
+        }
+    }
+    return false;
+}
+
 // One pass body of project (src/collision.rs:55-109) for pair (i, j).
-__device__ Outcome evaluate(uint32_t i, uint32_t j, const float *pos, const float *__restrict__ start,
-                            const float *__restrict__ mass, const uint8_t *__restrict__ flags) {
+// The projection reads each voxel as one 32-byte record: position (3),
+// start (3), mass, flags; two 16-byte loads from one cache line.
+constexpr int kRec = 8;  // floats per voxel record
+__device__ Outcome evaluate(uint32_t i, uint32_t j, const float *rec) {
     Outcome o;
     o.i = i;
     o.j = j;
-    const uint8_t fi = flags[i], fj = flags[j];
+    const float4 *r4 = reinterpret_cast<const float4 *>(rec);
+    const float4 a0 = r4[2 * size_t(i)], a1 = r4[2 * size_t(i) + 1];
+    const float4 b0 = r4[2 * size_t(j)], b1 = r4[2 * size_t(j) + 1];
+    const uint8_t fi = uint8_t(__float_as_uint(a1.w)), fj = uint8_t(__float_as_uint(b1.w));
     o.movable_i = !kinematic(fi);
     o.movable_j = !kinematic(fj);
-    const V3 si = ld3(start, i), sj = ld3(start, j), pi = ld3(pos, i), pj = ld3(pos, j);
+    const V3 pi{{a0.x, a0.y, a0.z}}, si{{a0.w, a1.x, a1.y}};
+    const V3 pj{{b0.x, b0.y, b0.z}}, sj{{b0.w, b1.x, b1.y}};
     const V3 start_delta = sub3(sj, si);
     float sign = 1.0f, amount = 0.0f, depth = 0.0f;
+    {
+        // most pairs are apart: no contact from any of the three tests
+        const V3 d = sub3(pj, pi);
+        const bool no_overlap =
+            fminf(kCell - fabsf(d.x[0]), fminf(kCell - fabsf(d.x[1]), kCell - fabsf(d.x[2]))) <= 0.0f;
+        if (no_overlap && touching_normal(pi, pj, &sign) < 0 &&
+            swept_surely_none(start_delta, sub3(sub3(pj, sj), sub3(pi, si))))
+            return o;
+    }
     bool contact = false, impact = false, do_sep = false;
     int axis = swept_normal(si, sj, pi, pj, &sign);
     if (axis >= 0) {
@@ -330,8 +368,8 @@ __device__ Outcome evaluate(uint32_t i, uint32_t j, const float *pos, const floa
     if (contact) o.code = uint8_t(0x80 | axis | (sign < 0.0f ? 4 : 0) | (impact ? 8 : 0));
     if (do_sep) {
         // separate (src/collision.rs:642-665)
-        const float wi = inverse_mass(fi, o.movable_i ? mass[i] : 1.0f, axis);
-        const float wj = inverse_mass(fj, o.movable_j ? mass[j] : 1.0f, axis);
+        const float wi = inverse_mass(fi, a1.z, axis);
+        const float wj = inverse_mass(fj, b1.z, axis);
         const float denom = wi + wj;
         if (denom > 0.0f) {
             o.sep = true;
@@ -404,16 +442,50 @@ __device__ __forceinline__ void set_outcome(Slot &sl, const Outcome &o) {
 // barrier after it publishes them to the other waves.
 __device__ __forceinline__ void wait_stores() { __builtin_amdgcn_s_waitcnt(0x0F70); }  // vmcnt(0)
 
-__device__ __forceinline__ uint32_t mark_load(const uint32_t *m, uint32_t v) {
-    return __hip_atomic_load(m + v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// Window marks live in an LDS hash table keyed by voxel (open addressing,
+// linear probing). A window holds kWin pairs, so at most 2 * kWin distinct
+// voxels are inserted: the table is twice that, never full, and every probe
+// sequence ends at an empty entry. hw[h]: lowest pair index of the window that
+// moves the voxel; hb[h]: lowest held pair index that touches it (movable
+// voxels only).
+constexpr uint32_t kHash = 4 * kWin;
+constexpr uint32_t kFreeKey = 0xFFFFFFFFu;  // voxel indices are below n <= 2^32 - 1
+__device__ __forceinline__ uint32_t hash_slot(uint32_t v) { return (v * 2654435761u) >> 19; }  // 13 bits
+static_assert(kHash == 8192, "hash_slot gives 13 bits");
+__device__ __forceinline__ void mark(uint32_t *hkey, uint32_t *hval, uint32_t v, uint32_t k) {
+    uint32_t h = hash_slot(v);
+    for (;;) {
+        const uint32_t was = atomicCAS(hkey + h, kFreeKey, v);
+        if (was == kFreeKey || was == v) {
+            atomicMin(hval + h, k);
+            return;
+        }
+        h = (h + 1) & (kHash - 1);
+    }
 }
-__device__ __forceinline__ void mark_clear(uint32_t *m, uint32_t v) {
-    __hip_atomic_store(m + v, kNoMark, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ uint32_t marked(const uint32_t *hkey, const uint32_t *hval, uint32_t v) {
+    uint32_t h = hash_slot(v);
+    for (;;) {
+        const uint32_t key = hkey[h];
+        if (key == v) return hval[h];
+        if (key == kFreeKey) return kNoMark;
+        h = (h + 1) & (kHash - 1);
+    }
 }
+
+template <int STRIDE>
 __device__ __forceinline__ void add_plus0(float *p, uint32_t v, uint32_t mask) {
 #pragma unroll
     for (int c = 0; c < 3; ++c)
-        if ((mask & (1u << c)) && __float_as_uint(p[size_t(v) * 3 + c]) == 0x80000000u) p[size_t(v) * 3 + c] = 0.0f;
+        if ((mask & (1u << c)) && __float_as_uint(p[size_t(v) * STRIDE + c]) == 0x80000000u)
+            p[size_t(v) * STRIDE + c] = 0.0f;
+}
+// The deferred +0 stores of one committed pair (mask: i in bits 0-2, j in 3-5).
+__device__ __forceinline__ void apply_plus0(float *rec, float *prev, uint32_t i, uint32_t j, uint32_t mask) {
+    add_plus0<kRec>(rec, i, mask & 7u);
+    add_plus0<3>(prev, i, mask & 7u);
+    add_plus0<kRec>(rec, j, mask >> 3);
+    add_plus0<3>(prev, j, mask >> 3);
 }
 
 // Per-wave popcounts of NF flags for each of the kR slot rows, exchanged
@@ -453,26 +525,43 @@ __device__ __forceinline__ void slot_counts(const bool (&f)[NF][kR], uint32_t (*
     row ^= 1;
 }
 
+// Workgroup minimum, broadcast; the two LDS rows alternate as in slot_counts.
+__device__ __forceinline__ uint32_t block_min(uint32_t v, uint32_t (*mins)[kGsWaves], int &row) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = min(v, uint32_t(__shfl_xor(int(v), off, 64)));
+    if ((threadIdx.x & 63) == 0) mins[row][threadIdx.x >> 6] = v;
+    __syncthreads();
+    uint32_t m = mins[row][0];
+    for (int w = 1; w < kGsWaves; ++w) m = min(m, mins[row][w]);
+    row ^= 1;
+    return m;
+}
+
 // The Gauss-Seidel passes of project (src/collision.rs:54-109) over the
-// sorted pairs, window by window.
-//   markW[v]: lowest pair index of the window whose evaluation moves voxel v;
-//   markB[v]: lowest held pair index of the window touching movable voxel v.
-// A pair k is held when markW or markB of one of its voxels is below k; the
-// held set is closed under that rule before anything commits. Both arrays
-// are all-ones between windows. The +0-over--0 stores of a window's committed
-// pairs are made at the start of the next window: pair tests do not depend on
-// the sign of a zero, and the next real moves come after that window's first
-// barrier. stats: passes, windows.
-__global__ __launch_bounds__(kGs) void k_project(const uint64_t *__restrict__ pairs, uint32_t npairs, float *pos,
-                                                 float *prev, const float *__restrict__ start,
-                                                 const float *__restrict__ mass, const uint8_t *__restrict__ flags,
-                                                 uint32_t *markW, uint32_t *markB, uint8_t *code,
-                                                 uint32_t *stats) {
-    __shared__ uint32_t cnt1[2][1][kR][kGsWaves];
+// sorted pairs, window by window, in one workgroup. A pair k is held when an
+// earlier pair of the window moves one of its voxels (hw below k) or an
+// earlier held pair touches one of its movable voxels (hb below k), with the
+// closure under that rule cut after one round as described below, before
+// anything commits. Four barriers per window. The pair keys of the
+// next window are fetched into LDS while the current one runs. The
+// +0-over--0 stores of a window's committed pairs are made at the start of the
+// next window: pair tests do not depend on the sign of a zero, and the next
+// real moves come after that window's barriers. stats: passes, windows.
+__global__ __launch_bounds__(kGs) void k_project(const uint64_t *__restrict__ pairs, uint32_t npairs, float *rec,
+                                                 float *prev, uint8_t *code, uint32_t *stats) {
+    __shared__ uint32_t hkey[kHash], hw[kHash], hb[kHash];
+    __shared__ uint64_t pre[2][kWin];
     __shared__ uint32_t cnt2[2][2][kR][kGsWaves];
+    __shared__ uint32_t mins[2][kGsWaves];
     const uint32_t t = threadIdx.x;
-    int row1 = 0, row2 = 0;
-    uint32_t passes = 0, windows = 0;
+    int row2 = 0, row_min = 0;
+    for (uint32_t x = t; x < kHash; x += kGs) {
+        hkey[x] = kFreeKey;
+        hw[x] = kNoMark;
+        hb[x] = kNoMark;
+    }
+    __syncthreads();
+    uint32_t passes = 0, windows = 0, buf = 0;
     Slot sl[kR];
     uint32_t fx_i[kR], fx_j[kR], fx_m[kR];  // deferred +0 stores: voxels and masks (i: bits 0-2, j: 3-5)
 #pragma unroll
@@ -484,72 +573,75 @@ __global__ __launch_bounds__(kGs) void k_project(const uint64_t *__restrict__ pa
         for (int r = 0; r < kR; ++r) {
             const uint32_t s = uint32_t(r) * kGs + t;
             sl[r].k = s;
-            sl[r].f = (s < npairs ? S_LIVE : 0u) | S_FRESH;
+            sl[r].f = s < npairs ? S_LIVE : 0u;
+            if (s < npairs) {
+                const uint64_t key = pairs[s];
+                sl[r].i = uint32_t(key >> 32);
+                sl[r].j = uint32_t(key);
+            }
         }
         uint32_t next = min(kWin, npairs);
         uint32_t live_total = next;
         while (live_total > 0) {
             ++windows;
+            // keys of the pairs the next window may take: pairs[next + x]
+            uint64_t ahead[kR];
+#pragma unroll
+            for (int r = 0; r < kR; ++r) {
+                const uint32_t x = uint32_t(r) * kGs + t;
+                ahead[r] = (x < npairs - next) ? pairs[next + x] : 0;
+            }
 #pragma unroll
             for (int r = 0; r < kR; ++r)
                 if (fx_m[r]) {
-                    add_plus0(pos, fx_i[r], fx_m[r] & 7u);
-                    add_plus0(prev, fx_i[r], fx_m[r] & 7u);
-                    add_plus0(pos, fx_j[r], fx_m[r] >> 3);
-                    add_plus0(prev, fx_j[r], fx_m[r] >> 3);
+                    apply_plus0(rec, prev, fx_i[r], fx_j[r], fx_m[r]);
                     fx_m[r] = 0;
                 }
 #pragma unroll
             for (int r = 0; r < kR; ++r) {
                 Slot &q = sl[r];
-                if (!(q.f & S_LIVE)) continue;
-                if (q.f & S_FRESH) {
-                    const uint64_t key = pairs[q.k];
-                    q.i = uint32_t(key >> 32);
-                    q.j = uint32_t(key);
-                }
-                set_outcome(q, evaluate(q.i, q.j, pos, start, mass, flags));
-                if (q.f & S_REAL_I) atomicMin(markW + q.i, q.k);
-                if (q.f & S_REAL_J) atomicMin(markW + q.j, q.k);
+                if (q.f & S_LIVE) set_outcome(q, evaluate(q.i, q.j, rec));
+                if (q.f & S_REAL_I) mark(hkey, hw, q.i, q.k);
+                if (q.f & S_REAL_J) mark(hkey, hw, q.j, q.k);
             }
-            wait_stores();
             __syncthreads();
-            bool fl1[1][kR];
-            uint32_t tot1[1], rk[kR];
+            // held: an earlier pair of the window moves one of its voxels ...
 #pragma unroll
             for (int r = 0; r < kR; ++r) {
                 Slot &q = sl[r];
-                const bool held = (q.f & S_LIVE) && (mark_load(markW, q.i) < q.k || mark_load(markW, q.j) < q.k);
-                if (held) q.f |= S_HELD;
-                fl1[0][r] = held;
-            }
-            slot_counts<1>(fl1, cnt1, row1, tot1, rk);
-            if (tot1[0] > 0) {
-                for (;;) {
-#pragma unroll
-                    for (int r = 0; r < kR; ++r) {
-                        Slot &q = sl[r];
-                        if ((q.f & S_HELD) && !(q.f & S_MARKED)) {
-                            if (q.f & S_MOV_I) atomicMin(markB + q.i, q.k);
-                            if (q.f & S_MOV_J) atomicMin(markB + q.j, q.k);
-                            q.f |= S_MARKED;
-                        }
-                    }
-                    wait_stores();
-                    __syncthreads();
-#pragma unroll
-                    for (int r = 0; r < kR; ++r) {
-                        Slot &q = sl[r];
-                        const bool newly = (q.f & S_LIVE) && !(q.f & S_HELD) &&
-                                           (mark_load(markB, q.i) < q.k || mark_load(markB, q.j) < q.k);
-                        if (newly) q.f |= S_HELD;
-                        fl1[0][r] = newly;
-                    }
-                    slot_counts<1>(fl1, cnt1, row1, tot1, rk);
-                    if (tot1[0] == 0) break;
+                if ((q.f & S_LIVE) && (marked(hkey, hw, q.i) < q.k || marked(hkey, hw, q.j) < q.k)) {
+                    q.f |= S_HELD;
+                    if (q.f & S_MOV_I) mark(hkey, hb, q.i, q.k);
+                    if (q.f & S_MOV_J) mark(hkey, hb, q.j, q.k);
                 }
             }
+            __syncthreads();
+            // ... or an earlier held pair touches one of its movable voxels. One
+            // round of that closure; if it holds new pairs, the window is cut at the
+            // smallest of them k*: pairs below k* are final (marks only hold later
+            // pairs), so holding every pair from k* on closes the held set.
+            uint32_t kmin = kNoMark;
+#pragma unroll
+            for (int r = 0; r < kR; ++r) {
+                Slot &q = sl[r];
+                const bool newly = (q.f & S_LIVE) && !(q.f & S_HELD) &&
+                                   (marked(hkey, hb, q.i) < q.k || marked(hkey, hb, q.j) < q.k);
+                if (newly) kmin = min(kmin, q.k);
+            }
+            const uint32_t kstar = block_min(kmin, mins, row_min);
+            if (kstar != kNoMark) {
+#pragma unroll
+                for (int r = 0; r < kR; ++r)
+                    if ((sl[r].f & S_LIVE) && sl[r].k >= kstar) sl[r].f |= S_HELD;
+            }
+            // every table read of the window is done: clear it for the next one
+            for (uint32_t x = t; x < kHash; x += kGs) {
+                hkey[x] = kFreeKey;
+                hw[x] = kNoMark;
+                hb[x] = kNoMark;
+            }
             bool fl2[2][kR];
+            uint32_t rk[kR];
 #pragma unroll
             for (int r = 0; r < kR; ++r) {
                 Slot &q = sl[r];
@@ -559,11 +651,11 @@ __global__ __launch_bounds__(kGs) void k_project(const uint64_t *__restrict__ pa
                 if (sep) {
                     const size_t a = (q.f >> S_AXIS) & 3u;
                     if (q.f & S_REAL_I) {
-                        pos[size_t(q.i) * 3 + a] += q.di;
+                        rec[size_t(q.i) * kRec + a] += q.di;
                         prev[size_t(q.i) * 3 + a] += q.di;
                     }
                     if (q.f & S_REAL_J) {
-                        pos[size_t(q.j) * 3 + a] += q.dj;
+                        rec[size_t(q.j) * kRec + a] += q.dj;
                         prev[size_t(q.j) * 3 + a] += q.dj;
                     }
                     fx_i[r] = q.i;
@@ -573,18 +665,12 @@ __global__ __launch_bounds__(kGs) void k_project(const uint64_t *__restrict__ pa
                 // the contact of the first pass that finds one (contacts are deduplicated per pair)
                 const uint8_t c = uint8_t(q.f >> S_CODE);
                 if (commit && c && code[q.k] == 0) code[q.k] = c;
-                if (live) {
-                    if (q.f & S_REAL_I) mark_clear(markW, q.i);
-                    if (q.f & S_REAL_J) mark_clear(markW, q.j);
-                    if (q.f & S_MARKED) {
-                        if (q.f & S_MOV_I) mark_clear(markB, q.i);
-                        if (q.f & S_MOV_J) mark_clear(markB, q.j);
-                    }
-                }
                 fl2[0][r] = !live || commit;  // the slot takes a new pair
                 fl2[1][r] = sep;
             }
-            wait_stores();
+#pragma unroll
+            for (int r = 0; r < kR; ++r) pre[buf][uint32_t(r) * kGs + t] = ahead[r];
+            wait_stores();  // the moves are performed before the next window reads positions
             uint32_t tot2[2];
             slot_counts<2>(fl2, cnt2, row2, tot2, rk);
             if (tot2[1] > 0) changed = true;
@@ -594,11 +680,17 @@ __global__ __launch_bounds__(kGs) void k_project(const uint64_t *__restrict__ pa
                 Slot &q = sl[r];
                 if (fl2[0][r]) {
                     q.k = next + rk[r];
-                    q.f = (rk[r] < added ? S_LIVE : 0u) | S_FRESH;
+                    q.f = rk[r] < added ? S_LIVE : 0u;
+                    if (rk[r] < added) {
+                        const uint64_t key = pre[buf][rk[r]];
+                        q.i = uint32_t(key >> 32);
+                        q.j = uint32_t(key);
+                    }
                 } else {
                     q.f = S_LIVE;  // held: evaluated again on the next window
                 }
             }
+            buf ^= 1;
             next += added;
             live_total = (kWin - tot2[0]) + added;
         }
@@ -606,16 +698,27 @@ __global__ __launch_bounds__(kGs) void k_project(const uint64_t *__restrict__ pa
     }
 #pragma unroll
     for (int r = 0; r < kR; ++r)
-        if (fx_m[r]) {
-            add_plus0(pos, fx_i[r], fx_m[r] & 7u);
-            add_plus0(prev, fx_i[r], fx_m[r] & 7u);
-            add_plus0(pos, fx_j[r], fx_m[r] >> 3);
-            add_plus0(prev, fx_j[r], fx_m[r] >> 3);
-        }
+        if (fx_m[r]) apply_plus0(rec, prev, fx_i[r], fx_j[r], fx_m[r]);
     if (t == 0) {
         stats[0] = passes;
         stats[1] = windows;
     }
+}
+
+// Voxel records for the projection, and the positions back out of them.
+__global__ void k_pack(uint32_t n, const float *__restrict__ pos, const float *__restrict__ start,
+                       const float *__restrict__ mass, const uint8_t *__restrict__ flags, float4 *__restrict__ rec) {
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    const size_t b = size_t(v) * 3;
+    rec[2 * size_t(v)] = make_float4(pos[b], pos[b + 1], pos[b + 2], start[b]);
+    rec[2 * size_t(v) + 1] = make_float4(start[b + 1], start[b + 2], kinematic(flags[v]) ? 1.0f : mass[v],
+                                         __uint_as_float(flags[v]));
+}
+__global__ void k_unpack(uint32_t n, const float *__restrict__ rec, float *__restrict__ pos) {
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    for (int c = 0; c < 3; ++c) pos[size_t(v) * 3 + c] = rec[size_t(v) * kRec + c];
 }
 
 // ---- contacts out ----------------------------------------------------------------
@@ -671,8 +774,9 @@ extern "C" int32_t vx_contact_project(uint32_t n, float *positions, float *previ
     const size_t N = n;
 
     float *d_pos, *d_prev, *d_start, *d_rest, *d_mass;
+    float4 *d_rec;
     uint8_t *d_flags;
-    uint32_t *d_bond, *d_adj, *d_err, *d_markW, *d_markB, *d_stats;
+    uint32_t *d_bond, *d_adj, *d_err, *d_stats;
     int32_t *d_lohi;
     uint64_t *d_cnt, *d_ends;
     unsigned long long *d_cand;
@@ -692,8 +796,7 @@ extern "C" int32_t vx_contact_project(uint32_t n, float *positions, float *previ
     VX_BUF("contact.lohi", 24 * N, d_lohi);
     VX_BUF("contact.count", 8 * N, d_cnt);
     VX_BUF("contact.ends", 8 * N, d_ends);
-    VX_BUF("contact.markW", 4 * N, d_markW);
-    VX_BUF("contact.markB", 4 * N, d_markB);
+    VX_BUF("contact.rec", 32 * N, d_rec);
     VX_BUF("contact.small", 64, d_err);
     d_stats = d_err + 4;
     d_cand = reinterpret_cast<unsigned long long *>(d_err + 8);
@@ -708,22 +811,18 @@ extern "C" int32_t vx_contact_project(uint32_t n, float *positions, float *previ
     VX_HIP(hipMemcpyAsync(d_flags, flags.data(), N, hipMemcpyHostToDevice, s));
     VX_HIP(hipMemcpyAsync(d_bond, bond_neighbor, 12 * N, hipMemcpyHostToDevice, s));
 
-    hipEvent_t ev[2];
-    VX_HIP(hipEventCreate(&ev[0]));
-    VX_HIP(hipEventCreate(&ev[1]));
+    hipEvent_t ev[4];
+    for (hipEvent_t &e : ev) VX_HIP(hipEventCreate(&e));
     struct EventGuard {
         hipEvent_t *e;
         ~EventGuard() {
-            (void)hipEventDestroy(e[0]);
-            (void)hipEventDestroy(e[1]);
+            for (int q = 0; q < 4; ++q) (void)hipEventDestroy(e[q]);
         }
     } guard{ev};
     VX_HIP(hipEventRecord(ev[0], s));
     // let starts = scene.previous.clone() (src/collision.rs:36)
     VX_HIP(hipMemcpyAsync(d_start, d_prev, 12 * N, hipMemcpyDeviceToDevice, s));
     VX_HIP(hipMemsetAsync(d_adj, 0xFF, 24 * N, s));
-    VX_HIP(hipMemsetAsync(d_markW, 0xFF, 4 * N, s));
-    VX_HIP(hipMemsetAsync(d_markB, 0xFF, 4 * N, s));
     VX_HIP(hipMemsetAsync(d_err, 0, 64, s));
     k_adjacency<<<blocks_for(3 * N, 256), 256, 0, s>>>(n, d_bond, d_adj, d_err);
     k_bins<<<blocks_for(N, 256), 256, 0, s>>>(n, d_start, d_pos, d_lohi, d_cnt, d_err);
@@ -801,9 +900,12 @@ extern "C" int32_t vx_contact_project(uint32_t n, float *positions, float *previ
         if ((st = tmp(tb, &d_tmp)) != VX_OK) return st;
         VX_HIP(rocprim::radix_sort_keys(d_tmp, tb, d_pk, d_pairs, size_t(P), 0, 64, s));
         VX_HIP(hipMemsetAsync(d_code, 0, P, s));
-        k_project<<<1, kGs, 0, s>>>(d_pairs, P, d_pos, d_prev, d_start, d_mass, d_flags, d_markW, d_markB, d_code,
-                                    d_stats);
+        k_pack<<<blocks_for(N, 256), 256, 0, s>>>(n, d_pos, d_start, d_mass, d_flags, d_rec);
+        VX_HIP(hipEventRecord(ev[2], s));
+        k_project<<<1, kGs, 0, s>>>(d_pairs, P, reinterpret_cast<float *>(d_rec), d_prev, d_code, d_stats);
         VX_HIP(hipGetLastError());
+        VX_HIP(hipEventRecord(ev[3], s));
+        k_unpack<<<blocks_for(N, 256), 256, 0, s>>>(n, reinterpret_cast<const float *>(d_rec), d_pos);
         k_contact_flags<<<blocks_for(P, 256), 256, 0, s>>>(P, d_code, d_cflag);
         tb = 0;
         VX_HIP(rocprim::inclusive_scan(nullptr, tb, d_cflag, d_cends, size_t(P), rocprim::plus<uint64_t>(), s));
@@ -824,6 +926,10 @@ extern "C" int32_t vx_contact_project(uint32_t n, float *positions, float *previ
     float ms = 0.0f;
     VX_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
     rep.device_us = double(ms) * 1000.0;
+    if (P > 0) {
+        VX_HIP(hipEventElapsedTime(&ms, ev[2], ev[3]));
+        rep.projection_us = double(ms) * 1000.0;
+    }
     rep.contacts = ncontacts;
     if (report) *report = rep;
     if (ncontacts > capacity)
