@@ -398,9 +398,12 @@ typedef struct vx_contact_report {
     uint64_t pairs;           /* after dropping intact local continuum pairs */
     uint64_t contacts;        /* contacts found */
     uint32_t passes;          /* projection passes (at most 6) */
-    uint32_t rounds;          /* device windows over the pair list */
+    uint32_t rounds;          /* device windows over the pair lists */
     double device_us;         /* device time, inputs resident, no transfers */
     double projection_us;     /* of which the Gauss-Seidel passes */
+    uint64_t evaluated;       /* pairs taken into the passes (all of them in the first) */
+    uint32_t redone;          /* passes after the first run again with more pairs */
+    uint32_t reserved;
 } vx_contact_report;
 
 int32_t vx_contact_project(uint32_t n, float *positions, float *previous, const float *rest_positions,

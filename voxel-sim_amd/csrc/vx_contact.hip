@@ -38,6 +38,8 @@
 
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
+#include <rocprim/device/device_select.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
 
 #include <cstring>
 #include <vector>
@@ -406,7 +408,7 @@ constexpr int S_AXIS = 9;     // 2 bits: the normal axis
 constexpr int S_PLUS0 = 11;   // 6 bits: +0 components of i (3) and j (3)
 constexpr int S_CODE = 24;    // 8 bits: contact code
 struct Slot {
-    uint32_t k, i, j, f;
+    uint32_t k, m, i, j, f;  // k: position in the list of the launch; m: index in the sorted pairs
     float di, dj;  // the deltas along the normal axis (the others are +-0)
 };
 
@@ -525,6 +527,10 @@ __device__ __forceinline__ void slot_counts(const bool (&f)[NF][kR], uint32_t (*
     row ^= 1;
 }
 
+// Control words of one call, in device memory (the passes are separate
+// launches enqueued without host round trips).
+enum : int { C_DONE = 0, C_OK, C_CHANGED, C_BAD, C_PASSES, C_WINDOWS, C_RESTORE, C_REDONE, C_EVALS, C_WORDS };
+
 // Workgroup minimum, broadcast; the two LDS rows alternate as in slot_counts.
 __device__ __forceinline__ uint32_t block_min(uint32_t v, uint32_t (*mins)[kGsWaves], int &row) {
 #pragma unroll
@@ -547,10 +553,15 @@ __device__ __forceinline__ uint32_t block_min(uint32_t v, uint32_t (*mins)[kGsWa
 // +0-over--0 stores of a window's committed pairs are made at the start of the
 // next window: pair tests do not depend on the sign of a zero, and the next
 // real moves come after that window's barriers. stats: passes, windows.
-__global__ __launch_bounds__(kGs) void k_project(const uint64_t *__restrict__ pairs, uint32_t npairs, float *rec,
-                                                 float *prev, uint8_t *code, uint32_t *stats) {
+__global__ __launch_bounds__(kGs) void k_project(const uint64_t *__restrict__ pairs, const uint32_t *__restrict__ list,
+                                                 const uint32_t *__restrict__ list_count, uint32_t npairs, float *rec,
+                                                 float *prev, uint8_t *code, uint8_t *sep_last, uint32_t *wfirst,
+                                                 uint32_t *wlast, uint32_t *ctl) {
+    if (ctl[C_DONE] | ctl[C_OK]) return;
+    const uint32_t count = list ? *list_count : npairs;
     __shared__ uint32_t hkey[kHash], hw[kHash], hb[kHash];
     __shared__ uint64_t pre[2][kWin];
+    __shared__ uint32_t prem[2][kWin];
     __shared__ uint32_t cnt2[2][2][kR][kGsWaves];
     __shared__ uint32_t mins[2][kGsWaves];
     const uint32_t t = threadIdx.x;
@@ -561,35 +572,41 @@ __global__ __launch_bounds__(kGs) void k_project(const uint64_t *__restrict__ pa
         hb[x] = kNoMark;
     }
     __syncthreads();
-    uint32_t passes = 0, windows = 0, buf = 0;
+    uint32_t windows = 0, buf = 0;
     Slot sl[kR];
     uint32_t fx_i[kR], fx_j[kR], fx_m[kR];  // deferred +0 stores: voxels and masks (i: bits 0-2, j: 3-5)
 #pragma unroll
     for (int r = 0; r < kR; ++r) fx_m[r] = 0;
-    for (int pass = 0; pass < kPasses; ++pass) {
-        ++passes;
+    {
         bool changed = false;
 #pragma unroll
         for (int r = 0; r < kR; ++r) {
             const uint32_t s = uint32_t(r) * kGs + t;
             sl[r].k = s;
-            sl[r].f = s < npairs ? S_LIVE : 0u;
-            if (s < npairs) {
-                const uint64_t key = pairs[s];
+            sl[r].f = s < count ? S_LIVE : 0u;
+            if (s < count) {
+                sl[r].m = list ? list[s] : s;
+                const uint64_t key = pairs[sl[r].m];
                 sl[r].i = uint32_t(key >> 32);
                 sl[r].j = uint32_t(key);
             }
         }
-        uint32_t next = min(kWin, npairs);
+        uint32_t next = min(kWin, count);
         uint32_t live_total = next;
         while (live_total > 0) {
             ++windows;
             // keys of the pairs the next window may take: pairs[next + x]
             uint64_t ahead[kR];
+            uint32_t ahead_m[kR];
 #pragma unroll
             for (int r = 0; r < kR; ++r) {
                 const uint32_t x = uint32_t(r) * kGs + t;
-                ahead[r] = (x < npairs - next) ? pairs[next + x] : 0;
+                ahead_m[r] = 0;
+                ahead[r] = 0;
+                if (x < count - next) {
+                    ahead_m[r] = list ? list[next + x] : next + x;
+                    ahead[r] = pairs[ahead_m[r]];
+                }
             }
 #pragma unroll
             for (int r = 0; r < kR; ++r)
@@ -661,20 +678,32 @@ __global__ __launch_bounds__(kGs) void k_project(const uint64_t *__restrict__ pa
                     fx_i[r] = q.i;
                     fx_j[r] = q.j;
                     fx_m[r] = (q.f >> S_PLUS0) & 63u;
+                    if (q.f & S_REAL_I) {
+                        atomicMin(wfirst + q.i, q.m);
+                        atomicMax(wlast + q.i, q.m + 1);
+                    }
+                    if (q.f & S_REAL_J) {
+                        atomicMin(wfirst + q.j, q.m);
+                        atomicMax(wlast + q.j, q.m + 1);
+                    }
                 }
                 // the contact of the first pass that finds one (contacts are deduplicated per pair)
                 const uint8_t c = uint8_t(q.f >> S_CODE);
-                if (commit && c && code[q.k] == 0) code[q.k] = c;
+                if (commit && c && code[q.m] == 0) code[q.m] = c;
+                if (commit) sep_last[q.m] = sep ? 1 : 0;
                 fl2[0][r] = !live || commit;  // the slot takes a new pair
                 fl2[1][r] = sep;
             }
 #pragma unroll
-            for (int r = 0; r < kR; ++r) pre[buf][uint32_t(r) * kGs + t] = ahead[r];
+            for (int r = 0; r < kR; ++r) {
+                pre[buf][uint32_t(r) * kGs + t] = ahead[r];
+                prem[buf][uint32_t(r) * kGs + t] = ahead_m[r];
+            }
             wait_stores();  // the moves are performed before the next window reads positions
             uint32_t tot2[2];
             slot_counts<2>(fl2, cnt2, row2, tot2, rk);
             if (tot2[1] > 0) changed = true;
-            const uint32_t added = min(tot2[0], npairs - next);
+            const uint32_t added = min(tot2[0], count - next);
 #pragma unroll
             for (int r = 0; r < kR; ++r) {
                 Slot &q = sl[r];
@@ -682,6 +711,7 @@ __global__ __launch_bounds__(kGs) void k_project(const uint64_t *__restrict__ pa
                     q.k = next + rk[r];
                     q.f = rk[r] < added ? S_LIVE : 0u;
                     if (rk[r] < added) {
+                        q.m = prem[buf][rk[r]];
                         const uint64_t key = pre[buf][rk[r]];
                         q.i = uint32_t(key >> 32);
                         q.j = uint32_t(key);
@@ -694,15 +724,124 @@ __global__ __launch_bounds__(kGs) void k_project(const uint64_t *__restrict__ pa
             next += added;
             live_total = (kWin - tot2[0]) + added;
         }
-        if (!changed) break;
+        if (changed && t == 0) ctl[C_CHANGED] = 1;
     }
 #pragma unroll
     for (int r = 0; r < kR; ++r)
         if (fx_m[r]) apply_plus0(rec, prev, fx_i[r], fx_j[r], fx_m[r]);
     if (t == 0) {
-        stats[0] = passes;
-        stats[1] = windows;
+        ctl[C_WINDOWS] += windows;
+        ctl[C_EVALS] += count;
     }
+}
+
+// ---- passes after the first ------------------------------------------------------
+//
+// A pair whose two voxels have not moved since its last evaluation, and whose
+// last evaluation did not separate, would give the same outcome again: no
+// move, no new contact (the first one is kept), no `changed`. Only the other
+// pairs need evaluating. A pass first takes as candidates the pairs that
+// separated last time or touch a voxel moved in the previous pass (moves
+// before a pair's own slot included: a superset). Moves inside the pass can
+// make more pairs due; after the candidate pass, every skipped pair is checked
+// against the first move of each of its voxels in this pass (wfirst below its
+// index means it was due). If any was, the pass is undone from the snapshot
+// and run again with those pairs added; after kAttempts it runs over every
+// pair. The result is the sequential sweep's either way.
+constexpr int kAttempts = 3;
+
+__global__ void k_pass_begin(uint32_t *ctl) {
+    if (ctl[C_DONE]) return;
+    if (!ctl[C_CHANGED]) {  // the last pass moved nothing (src/collision.rs:105-107)
+        ctl[C_DONE] = 1;
+        return;
+    }
+    ctl[C_PASSES] += 1;
+    ctl[C_OK] = 0;
+    ctl[C_CHANGED] = 0;
+    ctl[C_BAD] = 0;
+    ctl[C_RESTORE] = 0;
+}
+
+// Per voxel: the previous pass's moves, fresh move records, the snapshot.
+__global__ void k_pass_voxels(uint32_t n, const uint32_t *ctl, uint32_t *moved, uint32_t *wfirst, uint32_t *wlast,
+                              const float *__restrict__ rec, float *__restrict__ pos_s,
+                              const float *__restrict__ prev, float *__restrict__ prev_s) {
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n || ctl[C_DONE]) return;
+    moved[v] = wlast[v] != 0 ? 1u : 0u;
+    wfirst[v] = kNoMark;
+    wlast[v] = 0;
+    for (int c = 0; c < 3; ++c) {
+        pos_s[size_t(v) * 3 + c] = rec[size_t(v) * kRec + c];
+        prev_s[size_t(v) * 3 + c] = prev[size_t(v) * 3 + c];
+    }
+}
+
+// Per pair: the candidate flag and the snapshot of its contact and last outcome.
+__global__ void k_pass_pairs(uint32_t P, const uint32_t *ctl, const uint64_t *__restrict__ pairs,
+                             const uint32_t *__restrict__ moved, const uint8_t *__restrict__ sep_last,
+                             uint8_t *__restrict__ cflag, const uint8_t *__restrict__ code,
+                             uint8_t *__restrict__ code_s, uint8_t *__restrict__ sep_s) {
+    const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= P || ctl[C_DONE]) return;
+    const uint64_t key = pairs[m];
+    cflag[m] = (sep_last[m] || moved[uint32_t(key >> 32)] || moved[uint32_t(key)]) ? 1 : 0;
+    code_s[m] = code[m];
+    sep_s[m] = sep_last[m];
+}
+
+// After a candidate pass: a skipped pair with a voxel moved before its slot was due.
+__global__ void k_verify(uint32_t P, uint32_t *ctl, const uint64_t *__restrict__ pairs, uint8_t *cflag,
+                         const uint32_t *__restrict__ wfirst) {
+    const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= P || ctl[C_DONE] || ctl[C_OK] || cflag[m]) return;
+    const uint64_t key = pairs[m];
+    if (wfirst[uint32_t(key >> 32)] < m || wfirst[uint32_t(key)] < m) {
+        cflag[m] = 1;
+        atomicOr(ctl + C_BAD, 1u);
+    }
+}
+
+__global__ void k_settle(uint32_t *ctl) {
+    if (ctl[C_DONE] || ctl[C_OK]) return;
+    if (!ctl[C_BAD]) {
+        ctl[C_OK] = 1;
+    } else {
+        ctl[C_BAD] = 0;
+        ctl[C_RESTORE] = 1;
+        ctl[C_CHANGED] = 0;
+        ctl[C_REDONE] += 1;
+    }
+}
+
+__global__ void k_restore_voxels(uint32_t n, const uint32_t *ctl, float *__restrict__ rec,
+                                 const float *__restrict__ pos_s, float *__restrict__ prev,
+                                 const float *__restrict__ prev_s, uint32_t *wfirst, uint32_t *wlast) {
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n || !ctl[C_RESTORE]) return;
+    for (int c = 0; c < 3; ++c) {
+        rec[size_t(v) * kRec + c] = pos_s[size_t(v) * 3 + c];
+        prev[size_t(v) * 3 + c] = prev_s[size_t(v) * 3 + c];
+    }
+    wfirst[v] = kNoMark;
+    wlast[v] = 0;
+}
+
+__global__ void k_restore_pairs(uint32_t P, const uint32_t *ctl, uint8_t *__restrict__ code,
+                                const uint8_t *__restrict__ code_s, uint8_t *__restrict__ sep_last,
+                                const uint8_t *__restrict__ sep_s) {
+    const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= P || !ctl[C_RESTORE]) return;
+    code[m] = code_s[m];
+    sep_last[m] = sep_s[m];
+}
+
+__global__ void k_restore_done(uint32_t *ctl) { ctl[C_RESTORE] = 0; }
+
+__global__ void k_ctl_init(uint32_t *ctl) {
+    for (int w = 0; w < C_WORDS; ++w) ctl[w] = 0;
+    ctl[C_PASSES] = 1;
 }
 
 // Voxel records for the projection, and the positions back out of them.
@@ -899,11 +1038,60 @@ extern "C" int32_t vx_contact_project(uint32_t n, float *positions, float *previ
         VX_HIP(rocprim::radix_sort_keys(nullptr, tb, d_pk, d_pairs, size_t(P), 0, 64, s));
         if ((st = tmp(tb, &d_tmp)) != VX_OK) return st;
         VX_HIP(rocprim::radix_sort_keys(d_tmp, tb, d_pk, d_pairs, size_t(P), 0, 64, s));
+        uint8_t *d_sep, *d_due, *d_code_s, *d_sep_s;
+        uint32_t *d_clist, *d_wfirst, *d_wlast, *d_moved, *d_ctl;
+        float *d_pos_s, *d_prev_s;
+        VX_BUF("contact.sep", size_t(P) + 8, d_sep);
+        VX_BUF("contact.cflag", size_t(P) + 8, d_due);
+        VX_BUF("contact.code_s", size_t(P) + 8, d_code_s);
+        VX_BUF("contact.sep_s", size_t(P) + 8, d_sep_s);
+        VX_BUF("contact.clist", 4 * size_t(P) + 8, d_clist);
+        VX_BUF("contact.wfirst", 4 * N, d_wfirst);
+        VX_BUF("contact.wlast", 4 * N, d_wlast);
+        VX_BUF("contact.moved", 4 * N, d_moved);
+        VX_BUF("contact.pos_s", 12 * N, d_pos_s);
+        VX_BUF("contact.prev_s", 12 * N, d_prev_s);
+        VX_BUF("contact.ctl", 4 * (C_WORDS + 1), d_ctl);
+        uint32_t *d_ccount = d_ctl + C_WORDS;
+        float *rec = reinterpret_cast<float *>(d_rec);
         VX_HIP(hipMemsetAsync(d_code, 0, P, s));
+        VX_HIP(hipMemsetAsync(d_sep, 0, P, s));
+        VX_HIP(hipMemsetAsync(d_wfirst, 0xFF, 4 * N, s));
+        VX_HIP(hipMemsetAsync(d_wlast, 0, 4 * N, s));
+        k_ctl_init<<<1, 1, 0, s>>>(d_ctl);
         k_pack<<<blocks_for(N, 256), 256, 0, s>>>(n, d_pos, d_start, d_mass, d_flags, d_rec);
+        tb = 0;
+        VX_HIP(rocprim::select(nullptr, tb, rocprim::counting_iterator<uint32_t>(0), d_due, d_clist, d_ccount,
+                               size_t(P), s));
+        if ((st = tmp(tb, &d_tmp)) != VX_OK) return st;
         VX_HIP(hipEventRecord(ev[2], s));
-        k_project<<<1, kGs, 0, s>>>(d_pairs, P, reinterpret_cast<float *>(d_rec), d_prev, d_code, d_stats);
+        // pass 1 over every pair, then passes 2..6 over candidates (k_pass_begin stops them
+        // once a pass moved nothing)
+        k_project<<<1, kGs, 0, s>>>(d_pairs, nullptr, nullptr, P, rec, d_prev, d_code, d_sep, d_wfirst, d_wlast,
+                                    d_ctl);
         VX_HIP(hipGetLastError());
+        const uint32_t vb = blocks_for(N, 256), pb = blocks_for(P, 256);
+        for (int pass = 1; pass < kPasses; ++pass) {
+            k_pass_begin<<<1, 1, 0, s>>>(d_ctl);
+            k_pass_voxels<<<vb, 256, 0, s>>>(n, d_ctl, d_moved, d_wfirst, d_wlast, rec, d_pos_s, d_prev, d_prev_s);
+            k_pass_pairs<<<pb, 256, 0, s>>>(P, d_ctl, d_pairs, d_moved, d_sep, d_due, d_code, d_code_s, d_sep_s);
+            for (int a = 0; a < kAttempts; ++a) {
+                VX_HIP(rocprim::select(d_tmp, tb, rocprim::counting_iterator<uint32_t>(0), d_due, d_clist,
+                                       d_ccount, size_t(P), s));
+                k_project<<<1, kGs, 0, s>>>(d_pairs, d_clist, d_ccount, P, rec, d_prev, d_code, d_sep, d_wfirst,
+                                            d_wlast, d_ctl);
+                k_verify<<<pb, 256, 0, s>>>(P, d_ctl, d_pairs, d_due, d_wfirst);
+                k_settle<<<1, 1, 0, s>>>(d_ctl);
+                k_restore_voxels<<<vb, 256, 0, s>>>(n, d_ctl, rec, d_pos_s, d_prev, d_prev_s, d_wfirst, d_wlast);
+                k_restore_pairs<<<pb, 256, 0, s>>>(P, d_ctl, d_code, d_code_s, d_sep, d_sep_s);
+                k_restore_done<<<1, 1, 0, s>>>(d_ctl);
+            }
+            // no candidate pass verified: the pass over every pair (skipped when one did)
+            k_project<<<1, kGs, 0, s>>>(d_pairs, nullptr, nullptr, P, rec, d_prev, d_code, d_sep, d_wfirst,
+                                        d_wlast, d_ctl);
+            VX_HIP(hipGetLastError());
+        }
+        d_stats = d_ctl;
         VX_HIP(hipEventRecord(ev[3], s));
         k_unpack<<<blocks_for(N, 256), 256, 0, s>>>(n, reinterpret_cast<const float *>(d_rec), d_pos);
         k_contact_flags<<<blocks_for(P, 256), 256, 0, s>>>(P, d_code, d_cflag);
@@ -912,12 +1100,14 @@ extern "C" int32_t vx_contact_project(uint32_t n, float *positions, float *previ
         if ((st = tmp(tb, &d_tmp)) != VX_OK) return st;
         VX_HIP(rocprim::inclusive_scan(d_tmp, tb, d_cflag, d_cends, size_t(P), rocprim::plus<uint64_t>(), s));
         VX_HIP(hipMemcpyAsync(pin64, d_cends + (P - 1), 8, hipMemcpyDeviceToHost, s));
-        VX_HIP(hipMemcpyAsync(pinned, d_stats, 8, hipMemcpyDeviceToHost, s));
+        VX_HIP(hipMemcpyAsync(pinned, d_stats, 4 * C_WORDS, hipMemcpyDeviceToHost, s));
         VX_HIP(hipEventRecord(ev[1], s));
         VX_HIP(hipStreamSynchronize(s));
         ncontacts = pin64[0];
-        rep.passes = pinned[0];
-        rep.rounds = pinned[1];
+        rep.passes = pinned[C_PASSES];
+        rep.rounds = pinned[C_WINDOWS];
+        rep.redone = pinned[C_REDONE];
+        rep.evaluated = pinned[C_EVALS];
     } else {
         VX_HIP(hipEventRecord(ev[1], s));
         VX_HIP(hipStreamSynchronize(s));

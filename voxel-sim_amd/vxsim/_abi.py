@@ -114,7 +114,8 @@ class Contact(ctypes.Structure):
 class ContactReport(ctypes.Structure):
     _fields_ = [("candidate_pairs", ctypes.c_uint64), ("pairs", ctypes.c_uint64), ("contacts", ctypes.c_uint64),
                 ("passes", ctypes.c_uint32), ("rounds", ctypes.c_uint32), ("device_us", ctypes.c_double),
-                ("projection_us", ctypes.c_double)]
+                ("projection_us", ctypes.c_double), ("evaluated", ctypes.c_uint64), ("redone", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
 
 
 # (name, restype, argtypes) for every exported function of include/vx_solver.h.
