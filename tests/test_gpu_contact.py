@@ -112,6 +112,10 @@ def test_64cube_matches_the_oracle():
     targets = s.pop("targets")
     rep, st = check_parity(s, targets=targets, label="64^3")
     assert st["pairs"] > 1_000_000 and st["contacts"] > 10_000
+    # the passes after the first took candidates only, and at least one was
+    # undone and run again with more pairs: both paths are in this result
+    assert rep["evaluated"] < 2 * st["pairs"]
+    assert rep["redone"] >= 1
 
 
 def raw_call(s, capacity):
