@@ -16,16 +16,21 @@
 //    Pairs joined by intact bonds inside their lattice cell are dropped
 //    (local_continuum_pair, :520-550) and the rest radix sorted by (i, j).
 //
-//  * Projection. One workgroup of 1024 threads takes a window of the pair
-//    list in order, evaluates every pair on the current positions
+//  * Projection. One workgroup of 1024 threads takes a window of the smallest
+//    unprocessed pairs, evaluates every pair on the current positions
 //    (swept, overlap and touching tests, :552-621, :441-460), and commits the
 //    pairs whose sequential predecessors cannot change what they read: a pair
 //    is held back when an earlier pair of the window moves one of its voxels,
 //    or when an earlier held-back pair shares a voxel that can move (not
 //    kinematic). Committed pairs touch disjoint moving voxels, so they
 //    commute with the held-back ones and the result equals the sequential
-//    sweep. Held-back pairs stay at the head of the next window in order; the
-//    first pair of a window always commits.
+//    sweep. The window's smallest pair always commits.
+//
+//  * Later passes. A pair whose voxels have not moved since its last
+//    evaluation, which did not separate, would repeat that outcome; the passes
+//    after the first evaluate candidates only, check every skipped pair
+//    afterwards, and redo the pass from a snapshot with more pairs when one
+//    was due.
 //
 //  * Signed zeros. `separate` also adds exact zeros to the components off the
 //    normal (and to immovable endpoints). Adding -0 changes nothing and adding
