@@ -322,7 +322,36 @@ def large_bench(steps: int, warmup: int, compare: bool = True):
     print(json.dumps(line))
 
 
-def cut_bench(steps: int, dims=(192, 192, 96), preconditioner: int = 0):
+def cut_contact(scene, gs, prev_pos, targets, cpu: bool) -> dict:
+    """collision::project on the cut's own state after its last step: the
+    device positions (f64, rounded to the reference's f32), the positions at
+    the step's start as `previous`, the lattice centres as rest positions, the
+    bonds left after fracture and the blade's axis targets."""
+    from vxsim import contact
+    cur = gs.download()["positions"].astype(np.float32)
+    bonds = gs.download_bonds()["neighbor"]
+    rest = scene.positions.astype(np.float32)
+    held = ~np.isnan(targets)
+    args = dict(positions=cur, previous=prev_pos.astype(np.float32), rest_positions=rest,
+                mass=scene.mass.astype(np.float32), kinematic=scene.kinematic != 0, bond_neighbor=bonds)
+    runs = [contact.project(**args, targets=held) for _ in range(2)]
+    p, q, c, rep = runs[-1]
+    out = {"voxels": scene.n, "broken_bonds_so_far": int(3 * scene.n - np.count_nonzero(bonds != 0xFFFFFFFF)),
+           "candidate_pairs": rep["candidate_pairs"], "pairs": rep["pairs"], "contacts": rep["contacts"],
+           "passes": rep["passes"], "windows": rep["rounds"], "pair_evaluations": rep["evaluated"],
+           "device_ms": rep["device_us"] / 1e3, "projection_ms": rep["projection_us"] / 1e3}
+    if cpu:
+        from oracle import oracle
+        t0 = time.perf_counter()
+        po, qo, co, _ = oracle.contact_project(**args, targets=held)
+        out["cpu_oracle_ms"] = 1000 * (time.perf_counter() - t0)
+        out["bit_identical_to_oracle"] = bool(
+            np.array_equal(p.view(np.uint32), po.view(np.uint32)) and np.array_equal(q.view(np.uint32), qo.view(np.uint32))
+            and all(np.array_equal(c[k], co[k]) for k in ("i", "j", "new_impact")))
+    return out
+
+
+def cut_bench(steps: int, dims=(192, 192, 96), preconditioner: int = 0, contact: bool = False, cpu: bool = True):
     """The cutting scene with fracture: the 192x192x96 wood block, the blade's
     40x5 edge driven 5 mm per 1 ms step through per-step axis targets, up to
     six corotation passes and the fracture pass every step (broken bonds leave
@@ -336,10 +365,14 @@ def cut_bench(steps: int, dims=(192, 192, 96), preconditioner: int = 0):
                            preconditioner=preconditioner)
     reps = []
     walls = []
+    contact_out = None
     with grid.GridScene(scene) as gs:
         for k in range(steps):
+            if contact and k == steps - 1:
+                prev_pos = gs.download()["positions"]
             t = time.perf_counter()
-            gs.set_targets_and_loads(scenes.blade_targets(scene, blade, 5.0, cfg.fixed_dt, k), scene.loads)
+            step_targets = scenes.blade_targets(scene, blade, 5.0, cfg.fixed_dt, k)
+            gs.set_targets_and_loads(step_targets, scene.loads)
             r = gs.step(fixed_dt=cfg.fixed_dt, gravity=cfg.gravity, substeps=cfg.substeps,
                         corotation_iterations=cfg.corotation_iterations, corotation_tolerance=cfg.corotation_tolerance,
                         solve=sp, fracture=scenes.FractureConfig())
@@ -349,6 +382,8 @@ def cut_bench(steps: int, dims=(192, 192, 96), preconditioner: int = 0):
             print("cut step %d: %.1f ms, %d solves, %d inner iterations, %d bonds broken, fracture pass %.3f ms" %
                   (k, 1000 * walls[-1], r["solves"], r["inner_iterations"], r["fracture"]["bonds_broken"],
                    1000 * r["fracture"]["seconds"]), file=sys.stderr, flush=True)
+        if contact:
+            contact_out = cut_contact(scene, gs, prev_pos, step_targets, cpu)
     dof_iters = sum(3.0 * r["active"] * (r["inner_iterations"] + r["outer_sweeps"]) for r in reps)
     line = {"metric": "DOF-PCG-iters/s", "value": dof_iters / sum(walls), "unit": "DOF-PCG-iters/s",
             "workload": "%s with fracture: %d REVs, blade edge %dx%d REVs at 5 m/s, %d steps of 1 ms, <=%d passes, "
@@ -368,6 +403,8 @@ def cut_bench(steps: int, dims=(192, 192, 96), preconditioner: int = 0):
             "dissipated_j": {e: sum(r["fracture"][e] for r in reps) for e in
                              ("plastic_heat", "stored_plastic", "surface_work", "damage_heat", "released")},
             "data": "synthetic", "dtype": "f32 inner / f64 refinement, reductions, assembly and fracture"}
+    if contact_out is not None:
+        line["contact_projection_after_last_step"] = contact_out
     print(json.dumps(line))
 
 
@@ -451,6 +488,8 @@ def main():
     ap.add_argument("--red-black", action="store_true", help="--cut / --serve with the red-black preconditioner")
     ap.add_argument("--contact", action="store_true", help="contact projection leg only (64^3 contact box)")
     ap.add_argument("--no-contact", action="store_true")
+    ap.add_argument("--cut-contact", action="store_true",
+                    help="with --cut: the contact projection on the state after the last step, against the oracle")
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--serve-revs", type=str, default="1e3,2e5")
@@ -469,7 +508,8 @@ def main():
         print(json.dumps({"contact_projection": contact_bench(cpu=not args.no_cpu_baseline)}))
         return
     if args.cut:
-        cut_bench(args.steps, preconditioner=1 if args.red_black else 0)
+        cut_bench(args.steps, preconditioner=1 if args.red_black else 0, contact=args.cut_contact,
+                  cpu=not args.no_cpu_baseline)
         return
     if args.large:
         large_bench(args.steps, args.warmup, compare=not args.no_compare)

@@ -534,7 +534,9 @@ __device__ __forceinline__ void slot_counts(const bool (&f)[NF][kR], uint32_t (*
 
 // Control words of one call, in device memory (the passes are separate
 // launches enqueued without host round trips).
-enum : int { C_DONE = 0, C_OK, C_CHANGED, C_BAD, C_PASSES, C_WINDOWS, C_RESTORE, C_REDONE, C_EVALS, C_WORDS };
+enum : int {
+    C_DONE = 0, C_OK, C_CHANGED, C_BAD, C_PASSES, C_WINDOWS, C_RESTORE, C_REDONE, C_EVALS, C_CANDS, C_FULL, C_PREV, C_WORDS
+};
 
 // Workgroup minimum, broadcast; the two LDS rows alternate as in slot_counts.
 __device__ __forceinline__ uint32_t block_min(uint32_t v, uint32_t (*mins)[kGsWaves], int &row) {
@@ -562,7 +564,7 @@ __global__ __launch_bounds__(kGs) void k_project(const uint64_t *__restrict__ pa
                                                  const uint32_t *__restrict__ list_count, uint32_t npairs, float *rec,
                                                  float *prev, uint8_t *code, uint8_t *sep_last, uint32_t *wfirst,
                                                  uint32_t *wlast, uint32_t *ctl) {
-    if (ctl[C_DONE] | ctl[C_OK]) return;
+    if (ctl[C_DONE] | ctl[C_OK] | (list ? ctl[C_FULL] : 0u)) return;
     const uint32_t count = list ? *list_count : npairs;
     __shared__ uint32_t hkey[kHash], hw[kHash], hb[kHash];
     __shared__ uint64_t pre[2][kWin];
@@ -766,6 +768,16 @@ __global__ void k_pass_begin(uint32_t *ctl) {
     ctl[C_CHANGED] = 0;
     ctl[C_BAD] = 0;
     ctl[C_RESTORE] = 0;
+    ctl[C_CANDS] = 0;
+    ctl[C_FULL] = 0;
+}
+
+// Candidates past half the pairs: the pass goes over every pair at once, as
+// attempts would cost more than they skip (k_settle applies the same rule,
+// and gives up on attempts whose failures keep adding many pairs).
+__global__ void k_choose(uint32_t P, uint32_t *ctl) {
+    if (!ctl[C_DONE] && ctl[C_CANDS] > P / 2) ctl[C_FULL] = 1;
+    ctl[C_PREV] = ctl[C_CANDS];
 }
 
 // Per voxel: the previous pass's moves, fresh move records, the snapshot.
@@ -784,32 +796,46 @@ __global__ void k_pass_voxels(uint32_t n, const uint32_t *ctl, uint32_t *moved, 
 }
 
 // Per pair: the candidate flag and the snapshot of its contact and last outcome.
-__global__ void k_pass_pairs(uint32_t P, const uint32_t *ctl, const uint64_t *__restrict__ pairs,
+__global__ void k_pass_pairs(uint32_t P, uint32_t *ctl, const uint64_t *__restrict__ pairs,
                              const uint32_t *__restrict__ moved, const uint8_t *__restrict__ sep_last,
                              uint8_t *__restrict__ cflag, const uint8_t *__restrict__ code,
                              uint8_t *__restrict__ code_s, uint8_t *__restrict__ sep_s) {
     const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= P || ctl[C_DONE]) return;
-    const uint64_t key = pairs[m];
-    cflag[m] = (sep_last[m] || moved[uint32_t(key >> 32)] || moved[uint32_t(key)]) ? 1 : 0;
-    code_s[m] = code[m];
-    sep_s[m] = sep_last[m];
+    if (ctl[C_DONE]) return;
+    bool due = false;
+    if (m < P) {
+        const uint64_t key = pairs[m];
+        due = sep_last[m] || moved[uint32_t(key >> 32)] || moved[uint32_t(key)];
+        cflag[m] = due ? 1 : 0;
+        code_s[m] = code[m];
+        sep_s[m] = sep_last[m];
+    }
+    const int c = __syncthreads_count(due);
+    if (threadIdx.x == 0 && c) atomicAdd(ctl + C_CANDS, uint32_t(c));
 }
 
 // After a candidate pass: a skipped pair with a voxel moved before its slot was due.
 __global__ void k_verify(uint32_t P, uint32_t *ctl, const uint64_t *__restrict__ pairs, uint8_t *cflag,
                          const uint32_t *__restrict__ wfirst) {
     const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= P || ctl[C_DONE] || ctl[C_OK] || cflag[m]) return;
-    const uint64_t key = pairs[m];
-    if (wfirst[uint32_t(key >> 32)] < m || wfirst[uint32_t(key)] < m) {
-        cflag[m] = 1;
+    if (ctl[C_DONE] || ctl[C_OK] || ctl[C_FULL]) return;
+    bool due = false;
+    if (m < P && !cflag[m]) {
+        const uint64_t key = pairs[m];
+        if (wfirst[uint32_t(key >> 32)] < m || wfirst[uint32_t(key)] < m) {
+            cflag[m] = 1;
+            due = true;
+        }
+    }
+    const int c = __syncthreads_count(due);
+    if (threadIdx.x == 0 && c) {
         atomicOr(ctl + C_BAD, 1u);
+        atomicAdd(ctl + C_CANDS, uint32_t(c));
     }
 }
 
-__global__ void k_settle(uint32_t *ctl) {
-    if (ctl[C_DONE] || ctl[C_OK]) return;
+__global__ void k_settle(uint32_t P, uint32_t *ctl) {
+    if (ctl[C_DONE] || ctl[C_OK] || ctl[C_FULL]) return;
     if (!ctl[C_BAD]) {
         ctl[C_OK] = 1;
     } else {
@@ -817,6 +843,10 @@ __global__ void k_settle(uint32_t *ctl) {
         ctl[C_RESTORE] = 1;
         ctl[C_CHANGED] = 0;
         ctl[C_REDONE] += 1;
+        // a failed attempt that adds a quarter more pairs means moves cascade
+        // through the pass: attempts would keep failing
+        if (ctl[C_CANDS] > P / 2 || ctl[C_CANDS] - ctl[C_PREV] >= ctl[C_PREV] / 4) ctl[C_FULL] = 1;
+        ctl[C_PREV] = ctl[C_CANDS];
     }
 }
 
@@ -1080,13 +1110,14 @@ extern "C" int32_t vx_contact_project(uint32_t n, float *positions, float *previ
             k_pass_begin<<<1, 1, 0, s>>>(d_ctl);
             k_pass_voxels<<<vb, 256, 0, s>>>(n, d_ctl, d_moved, d_wfirst, d_wlast, rec, d_pos_s, d_prev, d_prev_s);
             k_pass_pairs<<<pb, 256, 0, s>>>(P, d_ctl, d_pairs, d_moved, d_sep, d_due, d_code, d_code_s, d_sep_s);
+            k_choose<<<1, 1, 0, s>>>(P, d_ctl);
             for (int a = 0; a < kAttempts; ++a) {
                 VX_HIP(rocprim::select(d_tmp, tb, rocprim::counting_iterator<uint32_t>(0), d_due, d_clist,
                                        d_ccount, size_t(P), s));
                 k_project<<<1, kGs, 0, s>>>(d_pairs, d_clist, d_ccount, P, rec, d_prev, d_code, d_sep, d_wfirst,
                                             d_wlast, d_ctl);
                 k_verify<<<pb, 256, 0, s>>>(P, d_ctl, d_pairs, d_due, d_wfirst);
-                k_settle<<<1, 1, 0, s>>>(d_ctl);
+                k_settle<<<1, 1, 0, s>>>(P, d_ctl);
                 k_restore_voxels<<<vb, 256, 0, s>>>(n, d_ctl, rec, d_pos_s, d_prev, d_prev_s, d_wfirst, d_wlast);
                 k_restore_pairs<<<pb, 256, 0, s>>>(P, d_ctl, d_code, d_code_s, d_sep, d_sep_s);
                 k_restore_done<<<1, 1, 0, s>>>(d_ctl);
