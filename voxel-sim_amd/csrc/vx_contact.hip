@@ -406,7 +406,7 @@ constexpr uint32_t kWin = uint32_t(kGs) * kR;  // window slots: s = r * kGs + th
 // k in the sorted list, whatever slot they sit in. Flags are packed in one
 // word to keep kR slots in registers.
 enum : uint32_t {
-    S_LIVE = 1u << 0, S_FRESH = 1u << 1, S_SEP = 1u << 2, S_MOV_I = 1u << 3, S_MOV_J = 1u << 4,
+    S_LIVE = 1u << 0, S_SEP = 1u << 2, S_MOV_I = 1u << 3, S_MOV_J = 1u << 4,
     S_REAL_I = 1u << 5, S_REAL_J = 1u << 6, S_HELD = 1u << 7, S_MARKED = 1u << 8,
 };
 constexpr int S_AXIS = 9;     // 2 bits: the normal axis
@@ -418,7 +418,7 @@ struct Slot {
 };
 
 __device__ __forceinline__ void set_outcome(Slot &sl, const Outcome &o) {
-    uint32_t f = sl.f & (S_LIVE | S_FRESH);
+    uint32_t f = sl.f & S_LIVE;
     if (o.sep) f |= S_SEP;
     if (o.movable_i) f |= S_MOV_I;
     if (o.movable_j) f |= S_MOV_J;
@@ -550,16 +550,20 @@ __device__ __forceinline__ uint32_t block_min(uint32_t v, uint32_t (*mins)[kGsWa
     return m;
 }
 
-// The Gauss-Seidel passes of project (src/collision.rs:54-109) over the
-// sorted pairs, window by window, in one workgroup. A pair k is held when an
-// earlier pair of the window moves one of its voxels (hw below k) or an
-// earlier held pair touches one of its movable voxels (hb below k), with the
-// closure under that rule cut after one round as described below, before
-// anything commits. Four barriers per window. The pair keys of the
-// next window are fetched into LDS while the current one runs. The
-// +0-over--0 stores of a window's committed pairs are made at the start of the
-// next window: pair tests do not depend on the sign of a zero, and the next
-// real moves come after that window's barriers. stats: passes, windows.
+// One Gauss-Seidel pass of project (src/collision.rs:54-109) over the sorted
+// pairs (list == nullptr) or over the candidate pairs list[0, *list_count),
+// window by window, in one workgroup. A pair k is held when an earlier pair
+// of the window moves one of its voxels (hw below k) or an earlier held pair
+// touches one of its movable voxels (hb below k), with the closure under that
+// rule cut after one round as described below, before anything commits. Four
+// barriers per window. The pair keys of the next window are fetched into LDS
+// while the current one runs. The +0-over--0 stores of a window's committed
+// pairs are made at the start of the next window: pair tests do not depend on
+// the sign of a zero, and the next real moves come after that window's
+// barriers. Per committed pair: its contact (first one kept), its outcome
+// (sep_last) and, per moved voxel, the first and last moving pair
+// (wfirst: index, wlast: index + 1). ctl: the control words; the launch returns at once when
+// the pass is over or settled.
 __global__ __launch_bounds__(kGs) void k_project(const uint64_t *__restrict__ pairs, const uint32_t *__restrict__ list,
                                                  const uint32_t *__restrict__ list_count, uint32_t npairs, float *rec,
                                                  float *prev, uint8_t *code, uint8_t *sep_last, uint32_t *wfirst,
@@ -950,7 +954,7 @@ extern "C" int32_t vx_contact_project(uint32_t n, float *positions, float *previ
     float *d_pos, *d_prev, *d_start, *d_rest, *d_mass;
     float4 *d_rec;
     uint8_t *d_flags;
-    uint32_t *d_bond, *d_adj, *d_err, *d_stats;
+    uint32_t *d_bond, *d_adj, *d_err;
     int32_t *d_lohi;
     uint64_t *d_cnt, *d_ends;
     unsigned long long *d_cand;
@@ -972,7 +976,6 @@ extern "C" int32_t vx_contact_project(uint32_t n, float *positions, float *previ
     VX_BUF("contact.ends", 8 * N, d_ends);
     VX_BUF("contact.rec", 32 * N, d_rec);
     VX_BUF("contact.small", 64, d_err);
-    d_stats = d_err + 4;
     d_cand = reinterpret_cast<unsigned long long *>(d_err + 8);
 
     std::vector<uint8_t> flags(N);
@@ -1127,7 +1130,6 @@ extern "C" int32_t vx_contact_project(uint32_t n, float *positions, float *previ
                                         d_wlast, d_ctl);
             VX_HIP(hipGetLastError());
         }
-        d_stats = d_ctl;
         VX_HIP(hipEventRecord(ev[3], s));
         k_unpack<<<blocks_for(N, 256), 256, 0, s>>>(n, reinterpret_cast<const float *>(d_rec), d_pos);
         k_contact_flags<<<blocks_for(P, 256), 256, 0, s>>>(P, d_code, d_cflag);
@@ -1136,7 +1138,7 @@ extern "C" int32_t vx_contact_project(uint32_t n, float *positions, float *previ
         if ((st = tmp(tb, &d_tmp)) != VX_OK) return st;
         VX_HIP(rocprim::inclusive_scan(d_tmp, tb, d_cflag, d_cends, size_t(P), rocprim::plus<uint64_t>(), s));
         VX_HIP(hipMemcpyAsync(pin64, d_cends + (P - 1), 8, hipMemcpyDeviceToHost, s));
-        VX_HIP(hipMemcpyAsync(pinned, d_stats, 4 * C_WORDS, hipMemcpyDeviceToHost, s));
+        VX_HIP(hipMemcpyAsync(pinned, d_ctl, 4 * C_WORDS, hipMemcpyDeviceToHost, s));
         VX_HIP(hipEventRecord(ev[1], s));
         VX_HIP(hipStreamSynchronize(s));
         ncontacts = pin64[0];
