@@ -208,3 +208,29 @@ def test_repeated_calls_reuse_buffers_and_agree():
         t = s["targets"]
         s2 = {k: v for k, v in s.items() if k != "targets"}
         check_parity(s2, targets=t, label="sequence %d" % len(s2["positions"]))
+
+
+def test_state_left_by_the_blade_cut_matches_the_oracle():
+    # the projection as it follows the solve: the state a device step with
+    # fracture leaves (deformed lattice, broken bonds, the blade's targets)
+    from vxsim import grid
+    blade = (10, 5, 8)
+    sc = scenes.wood_block_blade(dims=(32, 32, 20), blade=blade)
+    cfg = sc.config
+    sp = grid.solve_params(tolerance=1e-6, inner_tolerance=3e-3, max_inner=20000, max_outer=12)
+    with grid.GridScene(sc) as gs:
+        for k in range(3):
+            start = gs.download()["positions"]
+            tg = scenes.blade_targets(sc, blade, 5.0, cfg.fixed_dt, k)
+            gs.set_targets_and_loads(tg, sc.loads)
+            gs.step(fixed_dt=cfg.fixed_dt, gravity=cfg.gravity, substeps=cfg.substeps,
+                    corotation_iterations=cfg.corotation_iterations, corotation_tolerance=cfg.corotation_tolerance,
+                    solve=sp, fracture=scenes.FractureConfig())
+        end = gs.download()["positions"]
+        bonds = gs.download_bonds()["neighbor"]
+    s = {"positions": end.astype(np.float32), "previous": start.astype(np.float32),
+         "rest_positions": sc.positions.astype(np.float32), "mass": sc.mass.astype(np.float32),
+         "kinematic": sc.kinematic != 0, "bond_neighbor": bonds}
+    assert np.count_nonzero(bonds == EMPTY) > np.count_nonzero(sc.neighbor == EMPTY)  # bonds broke
+    rep, st = check_parity(s, targets=~np.isnan(tg), label="blade cut state")
+    assert st["contacts"] > 0
