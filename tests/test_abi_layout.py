@@ -20,6 +20,8 @@ STRUCTS = {
     "vx_step_params": _abi.StepParams,
     "vx_step_report": _abi.StepReport,
     "vx_batch_scene_report": _abi.BatchSceneReport,
+    "vx_contact": _abi.Contact,
+    "vx_contact_report": _abi.ContactReport,
 }
 
 
