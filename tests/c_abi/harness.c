@@ -96,5 +96,26 @@ int main(void) {
             if (fabs(dx[i * 3 + a] - expect) > 1e-5) ok = 0;
         }
     printf("\n");
+
+    /* collision::project from C: a voxel jumps three cells through a fixed
+       one (src/collision.rs:731-747); the fallback is the caller's CPU path. */
+    float pos[6] = {-0.005f, 0.005f, 0.005f, 0.005f, 0.005f, 0.005f};
+    float prev[6], rest[6], mass[2] = {6e-4f, 6e-4f};
+    uint8_t kinematic[2] = {0, 1};
+    uint32_t bonds[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
+    for (int k = 0; k < 6; ++k) prev[k] = rest[k] = pos[k];
+    const float wall = pos[3];
+    pos[0] += 3.0f * 0.01f;
+    vx_contact contacts[4];
+    vx_contact_report report;
+    status = vx_contact_project(2, pos, prev, rest, mass, kinematic, NULL, bonds, contacts, 4, &report);
+    if (status == VX_OK) {
+        const int held = report.contacts >= 1 && pos[3] == wall && wall - pos[0] >= 0.01f - 1e-6f &&
+                         contacts[0].i == 0 && contacts[0].j == 1;
+        printf("contact %s %llu\n", held ? "ok" : "wrong", (unsigned long long)report.contacts);
+        if (!held) ok = 0;
+    } else {
+        printf("contact fallback %d\n", status);
+    }
     return ok ? 0 : 1;
 }

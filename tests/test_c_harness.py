@@ -26,6 +26,7 @@ def test_c_caller_takes_fallback_without_device(harness):
     r = subprocess.run([harness], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0, r.stdout + r.stderr
     assert r.stdout.startswith("fallback 1 ")
+    assert "contact fallback 1" in r.stdout
 
 
 @pytest.mark.gpu
@@ -33,3 +34,4 @@ def test_c_caller_uses_device_solution(gpu, harness):
     r = subprocess.run([harness], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     assert r.stdout.startswith("device ")
+    assert "contact ok 1" in r.stdout
