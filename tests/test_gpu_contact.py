@@ -234,3 +234,22 @@ def test_state_left_by_the_blade_cut_matches_the_oracle():
     assert np.count_nonzero(bonds == EMPTY) > np.count_nonzero(sc.neighbor == EMPTY)  # bonds broke
     rep, st = check_parity(s, targets=~np.isnan(tg), label="blade cut state")
     assert st["contacts"] > 0
+
+
+def test_negative_zero_coordinates_keep_the_reference_bits():
+    # separate adds exact zeros; only a -0 coordinate can tell +0 from -0
+    # apart (it turns into +0), so plant -0 in positions and previous ones
+    s = scenes.contact_box((10, 8, 6), seed=21, jitter=8e-4)
+    targets = s.pop("targets")
+    rng = np.random.default_rng(21)
+    movable = np.nonzero(~s["kinematic"])[0]
+    for key in ("positions", "previous"):
+        pick = rng.choice(movable, size=60, replace=False)
+        comp = rng.integers(0, 3, size=60)
+        s[key][pick, comp] = np.float32(-0.0)
+    assert np.count_nonzero(np.signbit(s["positions"]) & (s["positions"] == 0)) > 0
+    rep, st = check_parity(s, targets=targets, label="-0 coordinates")
+    p, _, _, _ = contact.project(**s, targets=targets)
+    po, _, _, _ = oracle.contact_project(**s, targets=targets)
+    zero = (po == 0)
+    assert np.array_equal(np.signbit(p[zero]), np.signbit(po[zero]))

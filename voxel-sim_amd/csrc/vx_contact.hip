@@ -294,7 +294,7 @@ struct Outcome {
     bool sep = false;       // separate() ran with a positive denominator (src/collision.rs:650-652)
     bool movable_i = false, movable_j = false;  // not kinematic
     uint8_t real_i = 0, real_j = 0;  // components with a nonzero delta
-    uint8_t plus0_i = 0, plus0_j = 0;  // components with a +0 delta
+    uint8_t plus0_i = 0, plus0_j = 0;  // components with a +0 delta that can meet a -0
     float di[3] = {0, 0, 0}, dj[3] = {0, 0, 0};
 };
 
@@ -328,8 +328,13 @@ __device__ __forceinline__ bool swept_surely_none(const V3 &d0, const V3 &rel) {
 
 // One pass body of project (src/collision.rs:55-109) for pair (i, j).
 // The projection reads each voxel as one 32-byte record: position (3),
-// start (3), mass, flags; two 16-byte loads from one cache line.
+// start (3), mass, flags; two 16-byte loads from one cache line. The flags
+// word carries the flag byte and, from bit kNegZeroShift, the components that
+// are -0 on entry in the position or the previous position: a real move never
+// gives -0 and a +0 only turns -0 into +0, so no other component ever needs
+// the +0-over--0 store.
 constexpr int kRec = 8;  // floats per voxel record
+constexpr int kNegZeroShift = 8;
 __device__ Outcome evaluate(uint32_t i, uint32_t j, const float *rec) {
     Outcome o;
     o.i = i;
@@ -337,7 +342,10 @@ __device__ Outcome evaluate(uint32_t i, uint32_t j, const float *rec) {
     const float4 *r4 = reinterpret_cast<const float4 *>(rec);
     const float4 a0 = r4[2 * size_t(i)], a1 = r4[2 * size_t(i) + 1];
     const float4 b0 = r4[2 * size_t(j)], b1 = r4[2 * size_t(j) + 1];
-    const uint8_t fi = uint8_t(__float_as_uint(a1.w)), fj = uint8_t(__float_as_uint(b1.w));
+    const uint32_t wi_flags = __float_as_uint(a1.w), wj_flags = __float_as_uint(b1.w);
+    const uint8_t fi = uint8_t(wi_flags), fj = uint8_t(wj_flags);
+    // components that were -0 on entry (position or previous): the only ones a +0 can change
+    const uint8_t nz_i = uint8_t((wi_flags >> kNegZeroShift) & 7u), nz_j = uint8_t((wj_flags >> kNegZeroShift) & 7u);
     o.movable_i = !kinematic(fi);
     o.movable_j = !kinematic(fj);
     const V3 pi{{a0.x, a0.y, a0.z}}, si{{a0.w, a1.x, a1.y}};
@@ -387,9 +395,9 @@ __device__ Outcome evaluate(uint32_t i, uint32_t j, const float *rec) {
                 o.di[c] = -nc * s_i;
                 o.dj[c] = nc * s_j;
                 if (!is_zero(o.di[c])) o.real_i |= uint8_t(1 << c);
-                else if (__float_as_uint(o.di[c]) == 0u) o.plus0_i |= uint8_t(1 << c);
+                else if (__float_as_uint(o.di[c]) == 0u) o.plus0_i |= uint8_t((1 << c) & nz_i);
                 if (!is_zero(o.dj[c])) o.real_j |= uint8_t(1 << c);
-                else if (__float_as_uint(o.dj[c]) == 0u) o.plus0_j |= uint8_t(1 << c);
+                else if (__float_as_uint(o.dj[c]) == 0u) o.plus0_j |= uint8_t((1 << c) & nz_j);
             }
         }
     }
@@ -890,8 +898,11 @@ __global__ void k_pack(uint32_t n, const float *__restrict__ pos, const float *_
     if (v >= n) return;
     const size_t b = size_t(v) * 3;
     rec[2 * size_t(v)] = make_float4(pos[b], pos[b + 1], pos[b + 2], start[b]);
+    uint32_t nz = 0;
+    for (int c = 0; c < 3; ++c)
+        if (__float_as_uint(pos[b + c]) == 0x80000000u || __float_as_uint(start[b + c]) == 0x80000000u) nz |= 1u << c;
     rec[2 * size_t(v) + 1] = make_float4(start[b + 1], start[b + 2], kinematic(flags[v]) ? 1.0f : mass[v],
-                                         __uint_as_float(flags[v]));
+                                         __uint_as_float(uint32_t(flags[v]) | (nz << kNegZeroShift)));
 }
 __global__ void k_unpack(uint32_t n, const float *__restrict__ rec, float *__restrict__ pos) {
     const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
