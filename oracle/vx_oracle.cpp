@@ -1202,26 +1202,89 @@ void vxo_matvec(int mode, uint32_t n, const uint32_t *neighbors, const double *d
     }
 }
 
+// Rows per chunk of the exact solve's parallel loops; dot products sum the
+// chunks' partial sums in chunk order, so the result does not depend on the
+// thread count.
+constexpr size_t kExactChunk = 4096;
+
+// y = A x over the reference packing (the same arithmetic as matvec_packed),
+// rows split over OpenMP threads.
+void matvec_packed_par(uint32_t n, const uint32_t *nbr, const double *diag, const double *blocks, const double *x,
+                       double *y) {
+    const int64_t chunks = int64_t((size_t(n) + kExactChunk - 1) / kExactChunk);
+#pragma omp parallel for schedule(static)
+    for (int64_t c = 0; c < chunks; ++c) {
+        const size_t r0 = size_t(c) * kExactChunk;
+        const uint32_t rows = uint32_t(std::min(kExactChunk, size_t(n) - r0));
+        // Row i of the chunk reads neighbours anywhere in x: offset the row
+        // arrays only.
+        const uint32_t *nb = nbr + 6 * r0;
+        const double *d = diag + 9 * r0;
+        const double *b = blocks + 54 * r0;
+        for (uint32_t i = 0; i < rows; ++i) {
+            const double *di = d + 9 * size_t(i);
+            const double *xi = x + 3 * (r0 + i);
+            double acc[3];
+            for (int r = 0; r < 3; ++r) {
+                double v = di[3 * r] * xi[0];
+                v = fma_t(di[3 * r + 1], xi[1], v);
+                v = fma_t(di[3 * r + 2], xi[2], v);
+                acc[r] = v;
+            }
+            for (int sl = 0; sl < 6; ++sl) {
+                uint32_t j = nb[6 * size_t(i) + sl];
+                if (j == kEmpty) continue;
+                const double *xj = x + 3 * size_t(j);
+                const double *bl = b + 54 * size_t(i) + 9 * sl;
+                for (int r = 0; r < 3; ++r) {
+                    acc[r] = fma_t(bl[3 * r], xj[0], acc[r]);
+                    acc[r] = fma_t(bl[3 * r + 1], xj[1], acc[r]);
+                    acc[r] = fma_t(bl[3 * r + 2], xj[2], acc[r]);
+                }
+            }
+            for (int r = 0; r < 3; ++r) y[3 * (r0 + i) + r] = acc[r];
+        }
+    }
+}
+
+// FP64 block-Jacobi CG to a tight relative residual: the accuracy reference
+// ("FP64 oracle") every device solve is compared with. Not a restatement of a
+// reference function (the reference has no FP64 solve); textbook PCG with a
+// true-residual restart so the returned figure is the residual of x.
 double vxo_solve_exact(uint32_t n, const double *rhs, const uint32_t *nbr, const double *diagonal,
                        const double *blocks, double tolerance, uint32_t max_iterations, double *x,
                        uint32_t *iterations_out) {
     const size_t m = 3 * size_t(n);
+    const int64_t chunks = int64_t((size_t(n) + kExactChunk - 1) / kExactChunk);
     std::vector<double> inv(9 * size_t(n));
-    for (uint32_t i = 0; i < n; ++i) inverse_row_major(diagonal + 9 * size_t(i), inv.data() + 9 * size_t(i));
-    double bnorm = 0.0;
-    for (size_t k = 0; k < m; ++k) bnorm += rhs[k] * rhs[k];
-    bnorm = std::sqrt(bnorm);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < int64_t(n); ++i) inverse_row_major(diagonal + 9 * size_t(i), inv.data() + 9 * size_t(i));
+    std::vector<double> partial(size_t(std::max<int64_t>(chunks, 1)));
+    // Sum over rows of f(row), fixed chunks, partials added in chunk order.
+    auto chunked_sum = [&](auto &&f) {
+#pragma omp parallel for schedule(static)
+        for (int64_t c = 0; c < chunks; ++c) {
+            const size_t k0 = 3 * size_t(c) * kExactChunk, k1 = std::min(m, k0 + 3 * kExactChunk);
+            double s = 0.0;
+            for (size_t k = k0; k < k1; ++k) s += f(k);
+            partial[size_t(c)] = s;
+        }
+        double s = 0.0;
+        for (int64_t c = 0; c < chunks; ++c) s += partial[size_t(c)];
+        return s;
+    };
+    const double bnorm = std::sqrt(chunked_sum([&](size_t k) { return rhs[k] * rhs[k]; }));
     std::vector<double> r(rhs, rhs + m), z(m), p(m), ap(m);
-    for (size_t k = 0; k < m; ++k) x[k] = 0.0;
+#pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k < int64_t(m); ++k) x[k] = 0.0;
     if (iterations_out) *iterations_out = 0;
     if (bnorm == 0.0) return 0.0;
     auto dotv = [&](const std::vector<double> &a, const std::vector<double> &b) {
-        double s = 0.0;
-        for (size_t k = 0; k < m; ++k) s += a[k] * b[k];
-        return s;
+        return chunked_sum([&](size_t k) { return a[k] * b[k]; });
     };
     auto precond = [&](const std::vector<double> &in, std::vector<double> &out) {
-        for (uint32_t i = 0; i < n; ++i) {
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < int64_t(n); ++i) {
             const double *mi = inv.data() + 9 * size_t(i);
             for (int row = 0; row < 3; ++row)
                 out[3 * i + row] = mi[3 * row] * in[3 * i] + mi[3 * row + 1] * in[3 * i + 1] +
@@ -1229,12 +1292,11 @@ double vxo_solve_exact(uint32_t n, const double *rhs, const uint32_t *nbr, const
         }
     };
     auto true_rel = [&]() {
-        matvec_packed<double>(n, nbr, diagonal, blocks, x, ap.data());
-        double s = 0.0;
-        for (size_t k = 0; k < m; ++k) {
-            double d = rhs[k] - ap[k];
-            s += d * d;
-        }
+        matvec_packed_par(n, nbr, diagonal, blocks, x, ap.data());
+        const double s = chunked_sum([&](size_t k) {
+            const double d = rhs[k] - ap[k];
+            return d * d;
+        });
         return std::sqrt(s) / bnorm;
     };
     uint32_t iterations = 0;
@@ -1247,11 +1309,12 @@ double vxo_solve_exact(uint32_t n, const double *rhs, const uint32_t *nbr, const
         double rz = dotv(r, z);
         uint32_t since = 0;
         while (iterations < max_iterations && since < 5000) {
-            matvec_packed<double>(n, nbr, diagonal, blocks, p.data(), ap.data());
+            matvec_packed_par(n, nbr, diagonal, blocks, p.data(), ap.data());
             double pap = dotv(p, ap);
             if (!(pap > 0.0)) break;
             double alpha = rz / pap;
-            for (size_t k = 0; k < m; ++k) {
+#pragma omp parallel for schedule(static)
+            for (int64_t k = 0; k < int64_t(m); ++k) {
                 x[k] += alpha * p[k];
                 r[k] -= alpha * ap[k];
             }
@@ -1262,11 +1325,13 @@ double vxo_solve_exact(uint32_t n, const double *rhs, const uint32_t *nbr, const
             double rz2 = dotv(r, z);
             double beta = rz2 / rz;
             rz = rz2;
-            for (size_t k = 0; k < m; ++k) p[k] = z[k] + beta * p[k];
+#pragma omp parallel for schedule(static)
+            for (int64_t k = 0; k < int64_t(m); ++k) p[k] = z[k] + beta * p[k];
         }
         rel = true_rel();
         if (rel <= tolerance || since == 0) break;
-        for (size_t k = 0; k < m; ++k) r[k] = rhs[k] - ap[k];
+#pragma omp parallel for schedule(static)
+        for (int64_t k = 0; k < int64_t(m); ++k) r[k] = rhs[k] - ap[k];
     }
     if (iterations_out) *iterations_out = iterations;
     return rel;
