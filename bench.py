@@ -37,8 +37,14 @@ SPMV_BYTES_PER_ROW = 145  # z 12 + p_old 12 + D 24 + 3 bond blocks 72 + mask 1 +
 # position 24 + strength row 48 + per +axis bond (kscale 4, offset 12, plastic 12, damage 4, history 24) x 3 = 350;
 # writes plastic 36 + damage 12 + history 72 = 120
 FRACTURE_BYTES_PER_ROW = 470
+UPDATE_BYTES_PER_ROW = 108  # p, Ap, r, d 48 + Dinv 24 read; d, r, z 36 written
 SPMV_KERNEL = "pcg_bond_spmv<float>"
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pcg_bond_spmv_64cube_traffic.txt")
+TRAFFIC_FILE_LARGE = os.path.join(ROOT, "profiles", "pcg_bond_spmv_192x192x96_traffic.txt")
+# Relative residual of the batched and served solves: a residual of 1e-6 leaves
+# up to 1e-3 relative L2 error against the FP64 oracle on these boxes (condition
+# numbers near 1e6); 1e-10 keeps it below 1e-6 (tests/test_gpu_configs.py).
+BATCH_TOL = 1e-10
 COPY_BYTES = 1 << 30      # calibration copy for the HBM byte counters
 
 
@@ -60,6 +66,71 @@ def host_threads() -> int:
         return os.cpu_count() or 1
 
 
+def host_info() -> dict:
+    """The host cores this process may use: all CPUs it is affine to, capped by
+    OMP_NUM_THREADS where the environment sets it (a host shared by several
+    GPUs gives each GPU's processes its share that way)."""
+    affinity = host_threads()
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    threads = min(affinity, int(omp)) if omp.isdigit() and int(omp) > 0 else affinity
+    return {"nproc": os.cpu_count(), "affinity_cpus": affinity, "omp_num_threads": omp or None,
+            "threads_used": threads, "cpu_model": cpu_model()}
+
+
+def oracle_system(O, scene, h, gravity):
+    """The oracle's FP64 system of a scene's first solve: predict
+    (src/solver.rs:500-523: v += g h, targeted axes at their targets with zero
+    velocity, p += v h on free voxels), fit rotations, assemble. Returns the
+    oracle scene (positions = predicted) and the packed rows."""
+    os_ = O.OracleScene(scene)
+    t = scene.targets.astype(np.float64)
+    m = ~np.isnan(t)
+    kin = scene.kinematic != 0
+    v = os_.velocities + np.array(gravity) * h
+    v[m] = 0.0
+    os_.positions[~kin] += v[~kin] * h
+    os_.positions[m] = t[m]
+    O.fit_rotations(os_, O.F64)
+    sy = O.assemble(os_, O.F64, h, os_.positions.copy(), scene.free_mask(), scene.loads)
+    return os_, sy.rows()
+
+
+def error_vs_oracle_first_solve(scene, sp) -> dict:
+    """Relative L2 error of the workload's first device solve (the bench's
+    solve parameters) against the oracle's FP64 solution of the same system
+    (FP64 block-Jacobi CG to 1e-13), outside the timed region."""
+    from oracle import oracle as O
+    from vxsim import grid
+    cfg = scene.config
+    h = cfg.fixed_dt / cfg.substeps
+    with grid.GridScene(scene) as gs:
+        gs.predict(h, cfg.gravity)
+        gs.fit_rotations()
+        gs.assemble(h)
+        rep = gs.solve(sp)
+        x = gs.download()["correction"]
+    _, rows = oracle_system(O, scene, h, cfg.gravity)
+    t0 = time.perf_counter()
+    x_o, it_o, rel_o = O.solve_exact(rows, 1e-13)
+    xa = x[rows["active"]]
+    return {"relative_l2": float(np.linalg.norm(xa - x_o) / np.linalg.norm(x_o)),
+            "device_relative_residual": rep["relative_residual"], "oracle_relative_residual": rel_o,
+            "oracle_iterations": it_o, "oracle_seconds": time.perf_counter() - t0,
+            "what": "first solve of the workload vs the oracle's FP64 solution of the oracle-assembled system"}
+
+
+def error_vs_oracle_steps(scene, result_positions, steps: int, step_cfg) -> float:
+    """Relative L2 error of the displacement after `steps` device steps
+    against the oracle's FP64 steps with exact solves."""
+    from oracle import oracle as O
+    os_ = O.OracleScene(scene)
+    start = os_.positions.copy()
+    for _ in range(steps):
+        O.step(os_, step_cfg, O.F64, O.BJ, exact=True, exact_tolerance=1e-13, loads=scene.loads)
+    d_o = os_.positions - start
+    return float(np.linalg.norm((result_positions - start) - d_o) / np.linalg.norm(d_o))
+
+
 def cpu_baseline(scene, budget_s: float = 12.0):
     """The oracle's restatement of the reference CPU PCG (src/solver.rs:
     1833-1911: f32 vectors, f64 dots, f32 alpha/beta, block-Jacobi in the
@@ -75,7 +146,8 @@ def cpu_baseline(scene, budget_s: float = 12.0):
     O.fit_rotations(os_, O.F32)
     sy = O.assemble(os_, O.F32, h, os_.positions.copy(), scene.free_mask(), scene.loads)
     assemble_s = time.perf_counter() - t0
-    threads = min(host_threads(), 16)
+    hi = host_info()
+    threads = hi["threads_used"]
     results = {}
     for label, nthreads, iters in (("all", threads, 400), ("one", 1, 60)):
         _, it, res, sec = sy.solve_cpu(O.BJ, iters, 1e-30, nthreads, 0)
@@ -87,7 +159,7 @@ def cpu_baseline(scene, budget_s: float = 12.0):
     out = {"value": value, "unit": "DOF-PCG-iters/s", "cores": threads, "kind": "port",
            "sample": "oracle f32 CPU PCG with block-Jacobi on the 64^3 system (%d active rows), "
                      "%d iterations timed, OpenMP %d threads, no serial cutoff" % (sy.na, it, threads),
-           "cpu_model": cpu_model(), "assemble_s": assemble_s}
+           "cpu_model": hi["cpu_model"], "host": hi, "assemble_s": assemble_s}
     if "one" in results:
         it1, sec1, _ = results["one"]
         out["single_thread_value"] = 3.0 * sy.na * it1 / sec1
@@ -115,25 +187,73 @@ def stream_copy(nbytes: int, reps: int = 20) -> dict:
     return {"bytes_read": nbytes, "us": us.value, "gbs": 2.0 * nbytes / (us.value * 1e-6) / 1e9}
 
 
-def measured_traffic():
-    """HBM bytes per SpMV launch from the committed rocprofv3 counter
-    summary (FETCH_SIZE and WRITE_SIZE scaled by a 1 GiB copy), or None."""
+def measured_traffic(path=TRAFFIC_FILE):
+    """HBM bytes per SpMV launch from a committed rocprofv3 counter summary
+    (FETCH_SIZE and WRITE_SIZE scaled by a 1 GiB copy), or None."""
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(path) as f:
             return json.load(f)["traffic_bytes_per_launch"]
     except (OSError, KeyError, ValueError):
         return None
 
 
-def batched(steps: int = 1, count: int = 256, preconditioner: int = 0):
+def spmv_in_solve(gs, scene, sp) -> dict:
+    """Roofline of the FP32 bond SpMV as it runs inside a solve: one solve of
+    the scene's assembled system with every pcg_bond_spmv and pcg_update
+    dispatch timestamped (vx_scene_solve_timed: begin and end of each
+    dispatch, the figures a kernel trace reports). achieved = algorithmic
+    bytes per launch / mean dispatch duration."""
+    t = gs.solve_timed(sp)
+    st, up = t["spmv"], t["update"]
+    b = SPMV_BYTES_PER_ROW * scene.n
+    bu = UPDATE_BYTES_PER_ROW * scene.n
+    achieved = b / (st["mean_us"] * 1e-6) / 1e9
+    return {"bound": "hbm", "kernel": SPMV_KERNEL, "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": achieved / PEAK_HBM_GBS, "bytes_per_launch": b, "avg_us": st["mean_us"],
+            "median_us": st["median_us"], "min_us": st["min_us"], "max_us": st["max_us"],
+            "dispatches": st["dispatches"], "inner_iterations": t["inner_iterations"],
+            "update_kernel": {"kernel": "pcg_update<float>", "bytes_per_launch": bu, "avg_us": up["mean_us"],
+                              "achieved": bu / (up["mean_us"] * 1e-6) / 1e9,
+                              "frac": bu / (up["mean_us"] * 1e-6) / 1e9 / PEAK_HBM_GBS},
+            "timing": "mean over every pcg_bond_spmv dispatch of one solve (begin/end timestamps of each dispatch, "
+                      "vx_scene_solve_timed), including the dispatches a chunk issues after the inner solve ends"}
+
+
+def large_roofline() -> dict:
+    """The same in-solve SpMV roofline on the 192x192x96 wood block (3.54 M
+    rows, 513 MB per launch, far beyond the 256 MB Infinity Cache): the HBM
+    figure proper."""
+    from vxsim import grid, scenes
+    scene = scenes.wood_block_blade()
+    cfg = scene.config
+    h = cfg.fixed_dt / cfg.substeps
+    sp = grid.solve_params(tolerance=1e-9, inner_tolerance=INNER_TOL, max_inner=40000, max_outer=12)
+    with grid.GridScene(scene) as gs:
+        gs.predict(h, cfg.gravity)
+        gs.fit_rotations()
+        gs.assemble(h)
+        gs.solve(sp)  # warm-up (first touch of the inner planes)
+        out = spmv_in_solve(gs, scene, sp)
+    out["workload"] = "%s: %d rows per launch, one solve to relative residual 1e-9" % (scene.name, scene.n)
+    out["traffic"] = measured_traffic(TRAFFIC_FILE_LARGE)
+    out["stream_copy_same_bytes"] = stream_copy(out["bytes_per_launch"] // 2)
+    out["frac_of_copy"] = out["stream_copy_same_bytes"]["us"] / out["avg_us"]
+    return out
+
+
+def batched(steps: int = 1, count: int = 256, preconditioner: int = 0, cpu: bool = False):
     """256 independent boxes (edges U[8, 48], one face fixed, random initial
     velocities) on one tile arena, `steps` steps each, all scenes advanced
-    together tick by tick. Returns DOF-PCG-iters/s over the timed run."""
+    together tick by tick, solves to relative residual BATCH_TOL. Returns
+    DOF-PCG-iters/s over the timed run; with `cpu`, the smallest and the
+    largest box's step against the oracle's FP64 step."""
     from vxsim import batch, grid, scenes
     boxes = scenes.batch_boxes(count=count, seed=3, lo=8, hi=48)
-    sp = grid.solve_params(tolerance=1e-6, inner_tolerance=INNER_TOL, max_inner=8000, max_outer=12,
+    sp = grid.solve_params(tolerance=BATCH_TOL, inner_tolerance=INNER_TOL, max_inner=20000, max_outer=12,
                            preconditioner=preconditioner)
     tiles = sum(-(-s.n // batch.TILE) for s in boxes)
+    ns = [s.n for s in boxes]
+    probe = sorted({int(np.argmin(ns)), int(np.argmax(ns))})
     out = {}
     with batch.Batch(tiles * batch.TILE, max_scenes=count) as B:
         for timed in (False, True):
@@ -150,16 +270,75 @@ def batched(steps: int = 1, count: int = 256, preconditioner: int = 0):
             ticks = B.run()
             wall = time.perf_counter() - t0
             reps = [B.report(x) for x in slots]
+            probed = {b: B.download(slots[b])["positions"] for b in probe}
             for x in slots:
                 B.release(x)
         bad = [r for r in reps if r["state"] != batch.DONE]
         dof_iters = sum(3.0 * r["active_rows"] * (r["inner_iterations"] + r["outer_sweeps"]) for r in reps)
-        out = {"workload": "batch_256_boxes: %d scenes, %d REVs, %d active rows, %d step(s) each, rel. residual 1e-6"
-                           % (count, sum(s.n for s in boxes), sum(r["active_rows"] for r in reps), steps),
+        out = {"workload": "batch_256_boxes: %d scenes, %d REVs, %d active rows, %d step(s) each, rel. residual %g"
+                           % (count, sum(ns), sum(r["active_rows"] for r in reps), steps, BATCH_TOL),
                "dof_iters_per_s": dof_iters / wall, "wall_s": wall, "ticks": ticks, "ms_per_tick": 1000 * wall / ticks,
                "admit_s": t_admit, "failed": len(bad),
                "inner_iterations": sum(r["inner_iterations"] for r in reps),
                "max_rel_residual": max(r["last_relative_residual"] for r in reps)}
+    if cpu and steps == 1:
+        out["error_vs_fp64_oracle"] = {
+            boxes[b].name: error_vs_oracle_steps(boxes[b], probed[b], steps, boxes[b].config) for b in probe}
+    return out
+
+
+def served(count: int, compare: bool = True, cpu: bool = True, preconditioner: int = 0) -> dict:
+    """The served stream at its size mix, bounded: the first `count` arrivals
+    of Poisson arrivals at 200/s, REVs log-uniform in [1e3, 2e6], aspect
+    ratios U[0.5, 2], ten warm-started steps per request, solves to relative
+    residual BATCH_TOL, continuous admission into an arena sized from a 200 GB
+    HBM budget. p50/p99 latency (arrival to result on the host) and aggregate
+    DOF-PCG-iters/s, beside the plain FP64 structured solve handling the same
+    requests one at a time; with `cpu`, the three smallest requests' ten steps
+    against the oracle's FP64 steps."""
+    from vxsim import batch, grid, serve
+    sp = grid.solve_params(tolerance=BATCH_TOL, inner_tolerance=INNER_TOL, warm_start=True, max_inner=40000,
+                           preconditioner=preconditioner)
+    t = time.perf_counter()
+    reqs = serve.request_stream(200.0, 60.0, 1e3, 2e6, 10, seed=4, max_requests=count, solve=sp)
+    gen_s = time.perf_counter() - t
+    with serve.Server.for_budget(200 << 30, max_scenes=256, keep_results=cpu) as S:
+        t0 = time.perf_counter()
+        S.run(reqs)
+        wall = time.perf_counter() - t0
+        ticks = S.ticks
+    done = [r for r in reqs if r.state == batch.DONE]
+    dof_iters = sum(3.0 * r.report["active_rows"] * (r.report["inner_iterations"] + r.report["outer_sweeps"])
+                    for r in reqs if r.report)
+    lat = serve.percentiles([r.latency for r in reqs])
+    revs = [r.scene.n for r in reqs]
+    out = {"workload": "first %d arrivals of the served stream (Poisson 200/s, REVs log-uniform [1e3, 2e6], "
+                       "aspect U[0.5, 2], 10 warm-started steps each, rel. residual %g, 200 GB HBM budget)"
+                       % (count, BATCH_TOL),
+           "p50_s": lat[50], "p99_s": lat[99], "requests": len(reqs), "done": len(done),
+           "revs": revs, "total_revs": int(sum(revs)), "dof_iters_per_s": dof_iters / wall, "wall_s": wall,
+           "ticks": ticks, "scene_generation_s": gen_s,
+           "max_rel_residual": max(r.report["last_relative_residual"] for r in reqs if r.report)}
+    if compare:
+        t0 = time.perf_counter()
+        base = serve.serve_alone_fp64(reqs)
+        bwall = time.perf_counter() - t0
+        bl = serve.percentiles([b["latency"] for b in base])
+        bdi = sum(b["dof_iters"] for b in base) / bwall
+        out["plain_fp64_one_at_a_time"] = {"p50_s": bl[50], "p99_s": bl[99], "wall_s": bwall, "dof_iters_per_s": bdi,
+                                           "failed": sum(1 for b in base if not b["ok"])}
+        out["served_vs_plain_fp64_dof_iters"] = out["dof_iters_per_s"] / bdi
+        out["served_vs_plain_fp64_wall"] = bwall / wall
+    if cpu:
+        errs = {}
+        for r in sorted(reqs, key=lambda q: q.scene.n)[:3]:
+            cfg = r.scene.config
+            cfg.corotation_iterations = r.params.corotation_iterations
+            cfg.corotation_tolerance = r.params.corotation_tolerance
+            cfg.gravity = tuple(r.params.gravity)
+            errs["request%d_%drevs" % (r.rid, r.scene.n)] = error_vs_oracle_steps(r.scene, r.result["positions"],
+                                                                                 r.steps, cfg)
+        out["error_vs_fp64_oracle"] = errs
     return out
 
 
@@ -172,7 +351,8 @@ def serve_bench(rate: float, seconds: float, rev_lo: float, rev_hi: float, steps
     from vxsim import batch, grid, serve
     t = time.perf_counter()
     reqs = serve.request_stream(rate, seconds, rev_lo, rev_hi, steps, seed=4,
-                                solve=grid.solve_params(tolerance=1e-6, warm_start=True, preconditioner=preconditioner))
+                                solve=grid.solve_params(tolerance=BATCH_TOL, inner_tolerance=INNER_TOL, warm_start=True,
+                                                        max_inner=40000, preconditioner=preconditioner))
     gen_s = time.perf_counter() - t
     budget = int(budget_gb * (1 << 30))
     last = [time.perf_counter()]
@@ -444,12 +624,12 @@ def contact_bench(dims=(64, 64, 64), reps: int = 3, cpu: bool = True) -> dict:
 
 
 def pmc_driver(large: bool = False, red_black: bool = False):
-    """Short run for the rocprofv3 counter passes: 20 launches of the FP32
-    bond SpMV on the assembled 64^3 system (or the 192x192x96 block) — with
-    red_black, instead 40 iterations of the red-black solve (pcg_rb_red, its
-    black SpMV and update) on the same system — then 4 copies of 1 GiB by the
-    library's copy kernel (the calibration of FETCH_SIZE / WRITE_SIZE against
-    known bytes)."""
+    """Short run for the rocprofv3 counter passes: 40 inner iterations of the
+    solve of the assembled 64^3 system (or the 192x192x96 block), block-Jacobi
+    (pcg_bond_spmv, pcg_update) or with red_black the red-black reduced
+    system (pcg_rb_red, its black SpMV and update), then 3 copies of 1 GiB by
+    the library's copy kernel (the calibration of FETCH_SIZE / WRITE_SIZE
+    against known bytes)."""
     import ctypes
     from vxsim import grid, lib, scenes
     scene = scenes.wood_block_blade() if large else scenes.large_64cube()
@@ -459,18 +639,13 @@ def pmc_driver(large: bool = False, red_black: bool = False):
         gs.predict(h, cfg.gravity)
         gs.fit_rotations()
         gs.assemble(h)
-        if red_black:
-            rep = gs.solve(grid.solve_params(tolerance=1e-14, inner_tolerance=1e-14, max_inner=40, max_outer=1,
-                                             preconditioner=grid.RED_BLACK), raise_on_error=False)
-            us = None
-            extra = {"inner_iterations": rep["inner_iterations"]}
-        else:
-            us = gs.time_spmv(20)
-            extra = {}
+        rep = gs.solve(grid.solve_params(tolerance=1e-14, inner_tolerance=1e-14, max_inner=40, max_outer=1,
+                                         preconditioner=grid.RED_BLACK if red_black else grid.BLOCK_JACOBI),
+                       raise_on_error=False)
     out_us = ctypes.c_double()
     lib().vx_copy_bandwidth(COPY_BYTES, 3, ctypes.byref(out_us))
-    print(json.dumps(dict({"spmv_us": us, "copy_bytes": COPY_BYTES, "rows": scene.n,
-                           "spmv_bytes": SPMV_BYTES_PER_ROW * scene.n}, **extra)))
+    print(json.dumps({"inner_iterations": rep["inner_iterations"], "copy_bytes": COPY_BYTES, "rows": scene.n,
+                      "spmv_bytes": SPMV_BYTES_PER_ROW * scene.n}))
 
 
 def main():
@@ -495,12 +670,22 @@ def main():
     ap.add_argument("--serve-revs", type=str, default="1e3,2e5")
     ap.add_argument("--serve-steps", type=int, default=10)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
+    ap.add_argument("--serve-count", type=int, default=12,
+                    help="requests of the default served leg (the first arrivals of the 200/s stream)")
+    ap.add_argument("--no-serve", action="store_true", help="skip the default served leg")
+    ap.add_argument("--no-large-roofline", action="store_true", help="skip the 192x192x96 in-solve SpMV roofline")
+    ap.add_argument("--direct-launch", action="store_true",
+                    help="launch the PCG chunks kernel by kernel instead of replaying captured hipGraphs "
+                         "(identical results; for dispatch tracing)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the library has no multi-GPU path (independent scenes shard "
               "without communication; a single scene would need a slab decomposition). Run with --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
+    if args.direct_launch:
+        from vxsim import grid
+        grid.set_launch_mode(True)
     if args.pmc_driver:
         pmc_driver(args.large, red_black=args.red_black)
         return
@@ -524,6 +709,7 @@ def main():
 
     scene = scenes.large_64cube()
     cfg = scene.config
+    h = cfg.fixed_dt / cfg.substeps
     sp = grid.solve_params(tolerance=SOLVE_TOL, inner_tolerance=INNER_TOL, max_inner=8000, max_outer=12)
     gs = grid.GridScene(scene)
     step_kw = dict(fixed_dt=cfg.fixed_dt, gravity=cfg.gravity, substeps=cfg.substeps,
@@ -541,26 +727,32 @@ def main():
     solves = sum(r["solves"] for r in reps)
     value = 3.0 * active * iters / wall
     ms_per_step = 1000.0 * wall / args.steps
+    print("bench: %d steps of the 64^3 box, %.2f ms per step, %.3e DOF-PCG-iters/s" % (args.steps, ms_per_step, value),
+          file=sys.stderr, flush=True)
 
-    # Roofline of the dominant kernel, the FP32 structured bond SpMV, timed
-    # with HIP events on the library stream after the timed steps.
-    gs.assemble(cfg.fixed_dt / cfg.substeps)
-    spmv_us = gs.time_spmv(200)
-    spmv_bytes = SPMV_BYTES_PER_ROW * scene.n
-    achieved = spmv_bytes / (spmv_us * 1e-6) / 1e9
+    # Roofline of the dominant kernel, the FP32 structured bond SpMV, as it
+    # runs inside a solve of the next step's assembled system (after the timed
+    # steps).
+    gs.predict(h, cfg.gravity)
+    gs.fit_rotations()
+    gs.assemble(h)
+    roof = spmv_in_solve(gs, scene, sp)
+    roof["traffic"] = measured_traffic(TRAFFIC_FILE)
+    roof["cache_resident"] = ("the 64^3 planes (about 60 MB) stay in the 256 MB Infinity Cache between kernels: a "
+                              "latency-bound launch of 512 workgroups; roofline_hbm_out_of_cache is the HBM figure")
+    roof["isolated_back_to_back_us"] = gs.time_spmv(200)
     # What a plain copy reaches on this device: at 1 GiB (out of every cache)
     # and with the SpMV's own byte count (half read, half written).
-    copy_big = stream_copy(COPY_BYTES)
-    copy_same = stream_copy(spmv_bytes // 2)
+    roof["stream_copy"] = {"1GiB": stream_copy(COPY_BYTES), "same_bytes": stream_copy(roof["bytes_per_launch"] // 2)}
 
     extra = {}
     if not args.no_compare:
         # Plain FP64 GPU solves of the same assembled system: the structured
         # FP64 PCG and the drop-in FP64 solve over the reference row packing
         # (host buffers, PCIe upload included).
-        gs.predict(cfg.fixed_dt / cfg.substeps, cfg.gravity)
+        gs.predict(h, cfg.gravity)
         gs.fit_rotations()
-        gs.assemble(cfg.fixed_dt / cfg.substeps)
+        gs.assemble(h)
         t = time.perf_counter()
         rm = gs.solve(tolerance=SOLVE_TOL, inner_tolerance=INNER_TOL, max_inner=8000, max_outer=12)
         tm = time.perf_counter() - t
@@ -591,8 +783,8 @@ def main():
         # The reference's row arrays as the caller holds them (f32, u32), built
         # before the timed calls so that only the library call is timed.
         row_args = (np.ascontiguousarray(rows["rhs"], np.float32), np.ascontiguousarray(rows["neighbors"], np.uint32),
-                np.ascontiguousarray(rows["diagonal"], np.float32), np.ascontiguousarray(rows["blocks"], np.float32),
-                np.ascontiguousarray(inv, np.float32))
+                    np.ascontiguousarray(rows["diagonal"], np.float32), np.ascontiguousarray(rows["blocks"], np.float32),
+                    np.ascontiguousarray(inv, np.float32))
         solver.solve_pcg_f64(*row_args, 20000, 1e-6)  # warm the buffers and graph
         t = time.perf_counter()
         _, it_drop, _ = solver.solve_pcg_f64(*row_args, 20000, 1e-6)
@@ -631,20 +823,28 @@ def main():
             "time_to_solution_vs_fp64_structured": t64 / tm,
         }
     gs.close()
+    cpu_on = not args.no_cpu_baseline
     if not args.no_compare:
         extra["red_black_step"] = red_black_steps(scene, step_kw, args.steps, args.warmup, ms_per_step)
     if not args.no_batch:
-        extra["batched"] = batched()
+        extra["batched"] = batched(cpu=cpu_on)
         rb = batched(preconditioner=1)
         rb["time_vs_block_jacobi"] = extra["batched"]["wall_s"] / rb["wall_s"]
         extra["batched_red_black"] = rb
-
+        print("bench: batched legs done", file=sys.stderr, flush=True)
+    if not args.no_serve:
+        extra["served"] = served(args.serve_count, compare=not args.no_compare, cpu=cpu_on)
+        print("bench: served leg done", file=sys.stderr, flush=True)
+    if not args.no_large_roofline:
+        extra["roofline_hbm_out_of_cache"] = large_roofline()
+        print("bench: 192x192x96 roofline done", file=sys.stderr, flush=True)
     if not args.no_contact:
-        extra["contact_projection"] = contact_bench(cpu=not args.no_cpu_baseline)
+        extra["contact_projection"] = contact_bench(cpu=cpu_on)
 
     cpu = None
-    if not args.no_cpu_baseline:
+    if cpu_on:
         cpu = cpu_baseline(scene)
+        extra["error_vs_fp64_oracle"] = error_vs_oracle_first_solve(scene, sp)
 
     line = {
         "metric": "DOF-PCG-iters/s",
@@ -663,11 +863,9 @@ def main():
                                "<=%d corotation passes, outer rel. residual %g, inner %g" %
                                (active, cfg.corotation_iterations, SOLVE_TOL, INNER_TOL),
                    "active_rows": active, "dof": 3 * active, "dof_total": 3 * scene.n, "solves": solves,
-                   "iterations": iters, "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": SPMV_KERNEL, "achieved": achieved, "peak": PEAK_HBM_GBS,
-                     "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": measured_traffic(),
-                     "bytes_per_launch": spmv_bytes, "avg_us": spmv_us,
-                     "stream_copy": {"1GiB": copy_big, "same_bytes": copy_same}},
+                   "iterations": iters, "parallelism": "single GPU",
+                   "launch": "direct" if args.direct_launch else "graphs"},
+        "roofline": roof,
         "cpu_baseline": cpu,
     }
     line.update(extra)

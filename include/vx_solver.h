@@ -62,6 +62,15 @@ int32_t vx_init(void);
  * bandwidth a plain copy reaches beside the solver kernels' rooflines. */
 int32_t vx_copy_bandwidth(uint64_t bytes, uint32_t reps, double *out_us);
 
+/* Measurement aid, no counterpart in the reference: how the PCG iteration
+ * chunks reach the device. VX_LAUNCH_GRAPHS (the default) replays one
+ * captured hipGraph per chunk; VX_LAUNCH_DIRECT launches the same kernels with
+ * the same arguments in the same order one by one, for tools that trace
+ * kernel dispatches. Results are identical bit for bit in both modes. */
+#define VX_LAUNCH_GRAPHS 0
+#define VX_LAUNCH_DIRECT 1
+int32_t vx_set_launch_mode(int32_t mode);
+
 /* ---- drop-in solve ----------------------------------------------------- */
 /*
  * vx_solve_pcg — replaces cuda_solver::solve_pcg (src/cuda_solver.rs:301-501).
@@ -222,9 +231,30 @@ int32_t vx_scene_download(vx_scene *scene, double *positions, double *rotations,
 int32_t vx_scene_export_rows(vx_scene *scene, uint32_t *active, uint32_t *neighbors, double *blocks,
                              double *diagonal, double *rhs, uint32_t *out_active_count);
 /* Mean duration (microseconds) of the FP32 structured bond SpMV kernel in a
- * graph of `reps` back-to-back launches on the library stream, timed with HIP
- * events; the median of five replays of that graph. */
+ * graph of `reps` back-to-back launches on the library stream (launched one by
+ * one under VX_LAUNCH_DIRECT), timed with HIP events; the median of five
+ * replays. A microbenchmark: the kernel alone, without the update kernel
+ * between launches that a solve has. */
 int32_t vx_scene_time_spmv(vx_scene *scene, uint32_t reps, double *out_us);
+
+/* Durations of one kernel's dispatches, from each dispatch's own begin and
+ * end timestamps (the figures a kernel-dispatch trace reports). */
+typedef struct vx_kernel_stats {
+    uint32_t dispatches;
+    uint32_t reserved;
+    double mean_us, median_us, min_us, max_us, total_us;
+} vx_kernel_stats;
+typedef struct vx_kernel_times {
+    vx_kernel_stats spmv;    /* pcg_bond_spmv: beta, p = z + beta p_old, Ap, p.Ap partials */
+    vx_kernel_stats update;  /* pcg_update: alpha, d, r, z = D^-1 r, r.z and r.r partials */
+} vx_kernel_times;
+/* Measurement aid, no counterpart in the reference: vx_scene_solve with the
+ * inner kernels launched one by one (no graph replay), every bond SpMV and
+ * update dispatch timestamped. The solve's results are identical bit for bit
+ * to vx_scene_solve's. The dispatches include those a chunk issues after the
+ * scene's inner solve has ended (as in an untimed solve). */
+int32_t vx_scene_solve_timed(vx_scene *scene, const vx_solve_params *params, vx_solve_report *report,
+                             vx_kernel_times *times);
 
 /* One fixed step of the solve part of step_with_axis_targets_and_loads
  * (src/solver.rs:475-570, 668-683) on the device: per substep predict, fit

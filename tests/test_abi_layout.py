@@ -22,6 +22,8 @@ STRUCTS = {
     "vx_batch_scene_report": _abi.BatchSceneReport,
     "vx_contact": _abi.Contact,
     "vx_contact_report": _abi.ContactReport,
+    "vx_kernel_stats": _abi.KernelStats,
+    "vx_kernel_times": _abi.KernelTimes,
 }
 
 

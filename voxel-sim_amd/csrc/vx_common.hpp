@@ -50,6 +50,10 @@ struct Context {
     std::map<std::string, DeviceBuffer> buffers;  // named, grow-only (src/cuda_solver.rs:162-193)
     void *pinned = nullptr;                       // small pinned host area for status words
     size_t pinned_bytes = 0;
+    // Launch the PCG chunks kernel by kernel instead of replaying their
+    // captured hipGraphs (vx_set_launch_mode; the same kernels, arguments and
+    // order).
+    bool direct_launch = false;
 };
 
 Context &context();

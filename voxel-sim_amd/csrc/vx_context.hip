@@ -141,6 +141,17 @@ extern "C" int32_t vx_copy_bandwidth(uint64_t bytes, uint32_t reps, double *out_
 
 extern "C" const char *vx_last_error(void) { return vx::g_error.c_str(); }
 
+extern "C" int32_t vx_set_launch_mode(int32_t mode) {
+    using namespace vx;
+    clear_error();
+    if (mode != VX_LAUNCH_GRAPHS && mode != VX_LAUNCH_DIRECT)
+        return fail(VX_ERR_INVALID_ARGUMENT, "launch mode must be VX_LAUNCH_GRAPHS or VX_LAUNCH_DIRECT");
+    Context &ctx = context();
+    std::lock_guard<std::recursive_mutex> lock(ctx.mutex);
+    ctx.direct_launch = mode == VX_LAUNCH_DIRECT;
+    return VX_OK;
+}
+
 extern "C" int32_t vx_init(void) {
     vx::clear_error();
     return vx::ensure_init();

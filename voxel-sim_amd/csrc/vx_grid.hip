@@ -35,6 +35,8 @@
 #include "vx_common.hpp"
 #include "vx_mech.hpp"
 
+#include <hip/hip_ext.h>
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -1678,6 +1680,59 @@ struct Lock {
     Lock() : g(context().mutex) {}
 };
 
+// Per-dispatch timing of the inner kernels (vx_scene_solve_timed): each
+// timed launch carries a start and a stop event that the runtime binds to the
+// dispatch's own begin and end timestamps (hipExtLaunchKernelGGL), the
+// figures a kernel-dispatch trace reports. Only direct launches are timed.
+struct DispatchTimer {
+    static constexpr int kKinds = 2;  // 0: pcg_bond_spmv, 1: pcg_update
+    std::vector<hipEvent_t> pool;     // start/stop pairs
+    std::vector<int> kind;            // kind of each pair in use
+    size_t used = 0;
+    std::vector<float> us[kKinds];
+    int32_t take(int k, hipEvent_t *start, hipEvent_t *stop) {
+        if (2 * (used + 1) > pool.size()) {
+            for (int i = 0; i < 2; ++i) {
+                hipEvent_t e = nullptr;
+                VX_HIP(hipEventCreate(&e));
+                pool.push_back(e);
+            }
+        }
+        if (kind.size() <= used) kind.resize(used + 1);
+        kind[used] = k;
+        *start = pool[2 * used];
+        *stop = pool[2 * used + 1];
+        ++used;
+        return VX_OK;
+    }
+    // After the stream has passed every timed dispatch.
+    int32_t harvest() {
+        for (size_t i = 0; i < used; ++i) {
+            float ms = 0.0f;
+            VX_HIP(hipEventElapsedTime(&ms, pool[2 * i], pool[2 * i + 1]));
+            us[kind[i]].push_back(1e3f * ms);
+        }
+        used = 0;
+        return VX_OK;
+    }
+    ~DispatchTimer() {
+        for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+    }
+};
+DispatchTimer *g_timer = nullptr;  // set for the duration of a timed solve (under the context lock)
+
+// Launch `kernel` on the library stream, with dispatch timestamps of kind
+// `k` while a timed solve runs.
+template <class K, class... Args>
+void launch_kernel(int k, K kernel, const dim3 &grid, const dim3 &blk, Args... args) {
+    Context &ctx = context();
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    if (g_timer && k >= 0 && g_timer->take(k, &t0, &t1) == VX_OK)
+        hipExtLaunchKernelGGL(kernel, grid, blk, 0, ctx.stream, t0, t1, 0, args...);
+    else
+        hipLaunchKernelGGL(kernel, grid, blk, 0, ctx.stream, args...);
+}
+
 template <class S, bool ONE, bool RB, bool EARLY>
 void launch_iteration_t(vx_batch *b, const OneScene &one, const dim3 &grid, const OneScene &one_u, const dim3 &grid_u,
                         int odd, bool big) {
@@ -1687,9 +1742,9 @@ void launch_iteration_t(vx_batch *b, const OneScene &one, const dim3 &grid, cons
         if (big) hipLaunchKernelGGL(scene_reduce<S>, sg, blk, 0, ctx.stream, b->A, k0, nk, 0);
     };
     if constexpr (RB) hipLaunchKernelGGL((pcg_rb_red<S, ONE, EARLY>), grid_u, blk, 0, ctx.stream, b->A, one_u, odd);
-    hipLaunchKernelGGL((pcg_bond_spmv<S, ONE, RB, EARLY>), grid, blk, 0, ctx.stream, b->A, one, odd);
+    launch_kernel(0, pcg_bond_spmv<S, ONE, RB, EARLY>, grid, blk, b->A, one, odd);
     reduce(kPap, 1);
-    hipLaunchKernelGGL((pcg_update<S, ONE, RB, EARLY>), grid_u, blk, 0, ctx.stream, b->A, one_u, odd);
+    launch_kernel(1, pcg_update<S, ONE, RB, EARLY>, grid_u, blk, b->A, one_u, odd);
     reduce(kRz, 2);
 }
 // The inner kernels of one iteration for precision S: with red-black scenes
@@ -1718,12 +1773,11 @@ void launch_iteration(vx_batch *b, bool rb, const OneScene &one, const dim3 &gri
 }
 
 // One inner chunk: the refinement residual of scenes whose inner solve has
-// ended (device-decided), then the arena's iterations per tick.
-int32_t capture_inner(vx_batch *b, int f64, bool rb, uint32_t tiles, bool big, hipGraphExec_t *out) {
+// ended (device-decided), then the arena's iterations per tick. Enqueued on
+// the library stream (captured into a graph, or launched directly).
+void enqueue_inner(vx_batch *b, int f64, bool rb, uint32_t tiles, bool big) {
     Context &ctx = context();
-    hipGraph_t graph;
     const dim3 grid(tiles), blk(kBlock), sg(b->max_slots);
-    VX_HIP(hipStreamBeginCapture(ctx.stream, hipStreamCaptureModeThreadLocal));
     if (f64) {
         if (rb) hipLaunchKernelGGL(refine_red<double>, grid, blk, 0, ctx.stream, b->A, 0);
         hipLaunchKernelGGL(refine_residual<double>, grid, blk, 0, ctx.stream, b->A, 0);
@@ -1750,6 +1804,13 @@ int32_t capture_inner(vx_batch *b, int f64, bool rb, uint32_t tiles, bool big, h
         else
             launch_iteration<float>(b, rb, one, gi, one_u, gu, int(t & 1), big);
     }
+}
+
+int32_t capture_inner(vx_batch *b, int f64, bool rb, uint32_t tiles, bool big, hipGraphExec_t *out) {
+    Context &ctx = context();
+    hipGraph_t graph;
+    VX_HIP(hipStreamBeginCapture(ctx.stream, hipStreamCaptureModeThreadLocal));
+    enqueue_inner(b, f64, rb, tiles, big);
     VX_HIP(hipStreamEndCapture(ctx.stream, &graph));
     hipError_t e = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
@@ -1850,6 +1911,11 @@ int32_t launch_tick(vx_batch *b, Flight &f) {
     VX_HIP(hipGetLastError());
     for (int prec = 0; prec < 2; ++prec) {
         if (!inner[prec]) continue;
+        if (ctx.direct_launch || g_timer) {
+            enqueue_inner(b, prec, rb[prec], tiles, big);
+            VX_HIP(hipGetLastError());
+            continue;
+        }
         hipGraphExec_t g = nullptr;
         if (int32_t c = inner_graph(b, prec, rb[prec], tiles, &g)) return c;
         VX_HIP(hipGraphLaunch(g, s));
@@ -2694,11 +2760,9 @@ extern "C" int32_t vx_scene_assemble(vx_scene *sc, double h) {
     return manual_tick(b, sc->slot, B_ASSEMBLE);
 }
 
-extern "C" int32_t vx_scene_solve(vx_scene *sc, const vx_solve_params *prm, vx_solve_report *rep) {
-    clear_error();
-    if (int32_t c = scene_ok(sc)) return c;
-    if (!prm || !solve_ok(*prm)) return fail(VX_ERR_INVALID_ARGUMENT, "invalid solve parameters");
-    Lock lock;
+// One solve of a vx_scene (the caller holds the context lock); with a timer,
+// every tick's timed dispatches are harvested once the tick has retired.
+static int32_t scene_solve(vx_scene *sc, const vx_solve_params *prm, vx_solve_report *rep, DispatchTimer *timer) {
     vx_batch *b = sc->batch;
     SlotHost &h = b->slots[sc->slot];
     const vx_step_params p = h.params;
@@ -2709,6 +2773,8 @@ extern "C" int32_t vx_scene_solve(vx_scene *sc, const vx_solve_params *prm, vx_s
     const SlotState *st = &b->known[sc->slot];
     for (;;) {
         if (int32_t c = manual_tick(b, sc->slot, bits)) return c;
+        if (timer)
+            if (int32_t c = timer->harvest()) return c;
         if (st->nonfinite || st->done == kNonFin) {
             if (rep) rep->status = VX_ERR_NONFINITE_RESIDUAL;
             return fail(VX_ERR_NONFINITE_RESIDUAL, "HIP PCG produced a non-finite residual norm");
@@ -2732,6 +2798,46 @@ extern "C" int32_t vx_scene_solve(vx_scene *sc, const vx_solve_params *prm, vx_s
         status = fail(VX_ERR_NOT_CONVERGED, "HIP PCG did not converge: inner iterations=%u, sweeps=%u, relative residual=%.3e, tolerance=%.3e",
                       st->inner_total, st->sweeps, rel, prm->tolerance);
     if (rep) rep->status = uint32_t(status);
+    return status;
+}
+
+extern "C" int32_t vx_scene_solve(vx_scene *sc, const vx_solve_params *prm, vx_solve_report *rep) {
+    clear_error();
+    if (int32_t c = scene_ok(sc)) return c;
+    if (!prm || !solve_ok(*prm)) return fail(VX_ERR_INVALID_ARGUMENT, "invalid solve parameters");
+    Lock lock;
+    return scene_solve(sc, prm, rep, nullptr);
+}
+
+extern "C" int32_t vx_scene_solve_timed(vx_scene *sc, const vx_solve_params *prm, vx_solve_report *rep,
+                                        vx_kernel_times *times) {
+    clear_error();
+    if (int32_t c = scene_ok(sc)) return c;
+    if (!prm || !solve_ok(*prm) || !times) return fail(VX_ERR_INVALID_ARGUMENT, "invalid solve parameters");
+    Lock lock;
+    if (int32_t c = drain(sc->batch)) return c;
+    DispatchTimer timer;
+    g_timer = &timer;
+    const int32_t status = scene_solve(sc, prm, rep, &timer);
+    g_timer = nullptr;
+    // A failed tick may leave timed dispatches unharvested: wait for them.
+    (void)hipStreamSynchronize(context().stream);
+    if (timer.used) (void)timer.harvest();
+    std::memset(times, 0, sizeof *times);
+    for (int k = 0; k < DispatchTimer::kKinds; ++k) {
+        std::vector<float> &u = timer.us[k];
+        vx_kernel_stats &o = k == 0 ? times->spmv : times->update;
+        o.dispatches = uint32_t(u.size());
+        if (u.empty()) continue;
+        double sum = 0.0;
+        for (float x : u) sum += x;
+        std::sort(u.begin(), u.end());
+        o.mean_us = sum / double(u.size());
+        o.median_us = u[u.size() / 2];
+        o.min_us = u.front();
+        o.max_us = u.back();
+        o.total_us = sum;
+    }
     return status;
 }
 
@@ -2922,28 +3028,38 @@ extern "C" int32_t vx_scene_time_spmv(vx_scene *sc, uint32_t reps, double *out_u
     const bool early = g.ntiles > kBigTiles;  // as the solve's chunks launch it
     // The launches run back to back inside one graph, as the solve runs them,
     // between two events on the library stream.
-    hipGraph_t graph;
+    auto enqueue = [&]() {
+        for (uint32_t k = 0; k < reps; ++k)
+            if (early)
+                hipLaunchKernelGGL((pcg_bond_spmv<float, true, false, true>), dim3(one.grid), dim3(kBlock), 0, ctx.stream, b->A,
+                                   one, int(k & 1));
+            else
+                hipLaunchKernelGGL((pcg_bond_spmv<float, true, false, false>), dim3(one.grid), dim3(kBlock), 0, ctx.stream, b->A,
+                                   one, int(k & 1));
+    };
     hipGraphExec_t exec = nullptr;
-    VX_HIP(hipStreamBeginCapture(ctx.stream, hipStreamCaptureModeThreadLocal));
-    for (uint32_t k = 0; k < reps; ++k)
-        if (early)
-            hipLaunchKernelGGL((pcg_bond_spmv<float, true, false, true>), dim3(one.grid), dim3(kBlock), 0, ctx.stream, b->A, one,
-                               int(k & 1));
-        else
-            hipLaunchKernelGGL((pcg_bond_spmv<float, true, false, false>), dim3(one.grid), dim3(kBlock), 0, ctx.stream, b->A, one,
-                               int(k & 1));
-    VX_HIP(hipStreamEndCapture(ctx.stream, &graph));
-    hipError_t ge = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(graph);
-    if (ge != hipSuccess) return fail(VX_ERR_DEVICE, "hipGraphInstantiate failed: %s", hipGetErrorString(ge));
+    if (!ctx.direct_launch) {
+        hipGraph_t graph;
+        VX_HIP(hipStreamBeginCapture(ctx.stream, hipStreamCaptureModeThreadLocal));
+        enqueue();
+        VX_HIP(hipStreamEndCapture(ctx.stream, &graph));
+        hipError_t ge = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ge != hipSuccess) return fail(VX_ERR_DEVICE, "hipGraphInstantiate failed: %s", hipGetErrorString(ge));
+    }
+    auto replay = [&]() -> hipError_t {
+        if (exec) return hipGraphLaunch(exec, ctx.stream);
+        enqueue();
+        return hipGetLastError();
+    };
     // Five timed replays after a warm-up one; the median replay is reported.
     constexpr int kReplays = 5;
     hipEvent_t ev[kReplays + 1];
     for (auto &e : ev) VX_HIP(hipEventCreate(&e));
-    VX_HIP(hipGraphLaunch(exec, ctx.stream));  // warm-up
+    VX_HIP(replay());  // warm-up
     VX_HIP(hipEventRecord(ev[0], ctx.stream));
     for (int i = 0; i < kReplays; ++i) {
-        VX_HIP(hipGraphLaunch(exec, ctx.stream));
+        VX_HIP(replay());
         VX_HIP(hipEventRecord(ev[i + 1], ctx.stream));
     }
     VX_HIP(hipStreamSynchronize(ctx.stream));
@@ -2951,7 +3067,7 @@ extern "C" int32_t vx_scene_time_spmv(vx_scene *sc, uint32_t reps, double *out_u
     for (int i = 0; i < kReplays; ++i) VX_HIP(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
     std::sort(ms, ms + kReplays);
     for (auto &e : ev) (void)hipEventDestroy(e);
-    (void)hipGraphExecDestroy(exec);
+    if (exec) (void)hipGraphExecDestroy(exec);
     *out_us = 1000.0 * double(ms[kReplays / 2]) / reps;
     // Restore the scene's state word (solution buffer parity) and parameters.
     st = b->known[sc->slot];

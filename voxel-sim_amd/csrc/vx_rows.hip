@@ -485,16 +485,9 @@ int32_t solve_rows(const float *rhs, const uint32_t *neighbors, const float *dia
         if (big) hipLaunchKernelGGL(rows_reduce, dim3(1), dim3(kBlock), 0, stream, part, grid, int(kRz), 2, st);
         VX_HIP(hipGetLastError());
 
-        RowsGraph &g = g_graph[kF64 ? 1 : 0];
-        const void *key[4] = {d_blk, d_p0, d_part, d_state};
-        bool reuse = g.exec && g.n == n && std::memcmp(key, g.key, sizeof key) == 0;
-        if (!reuse) {
-            if (g.exec) {
-                VX_HIP(hipGraphExecDestroy(g.exec));
-                g.exec = nullptr;
-            }
-            hipGraph_t graph;
-            VX_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        // One chunk of kChunk iterations: captured once into a graph and
+        // replayed, or launched kernel by kernel in direct launch mode.
+        auto enqueue_chunk = [&]() {
             for (uint32_t t = 0; t < kChunk; ++t) {
                 S *pold = (t % 2 == 0) ? p0 : p1;
                 S *pnew = (t % 2 == 0) ? p1 : p0;
@@ -505,6 +498,19 @@ int32_t solve_rows(const float *rhs, const uint32_t *neighbors, const float *dia
                                    r, z, pnew, ap, st, part);
                 if (big) hipLaunchKernelGGL(rows_reduce, dim3(1), dim3(kBlock), 0, stream, part, grid, int(kRz), 2, st);
             }
+        };
+        const bool direct = ctx.direct_launch;
+        RowsGraph &g = g_graph[kF64 ? 1 : 0];
+        const void *key[4] = {d_blk, d_p0, d_part, d_state};
+        bool reuse = direct || (g.exec && g.n == n && std::memcmp(key, g.key, sizeof key) == 0);
+        if (!reuse) {
+            if (g.exec) {
+                VX_HIP(hipGraphExecDestroy(g.exec));
+                g.exec = nullptr;
+            }
+            hipGraph_t graph;
+            VX_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+            enqueue_chunk();
             VX_HIP(hipStreamEndCapture(stream, &graph));
             hipError_t e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
             (void)hipGraphDestroy(graph);
@@ -520,7 +526,13 @@ int32_t solve_rows(const float *rhs, const uint32_t *neighbors, const float *dia
         // The check after the last update runs in the following rows_spmv, so
         // replay until one spmv past the cap has been issued.
         while (launched <= max_iterations) {
-            hipError_t e = hipGraphLaunch(g.exec, stream);
+            hipError_t e = hipSuccess;
+            if (direct) {
+                enqueue_chunk();
+                e = hipGetLastError();
+            } else {
+                e = hipGraphLaunch(g.exec, stream);
+            }
             if (e == hipSuccess) e = hipMemcpyAsync(&pinned[chunk % 2], d_state, sizeof hs, hipMemcpyDeviceToHost, stream);
             if (e == hipSuccess) e = hipEventRecord(ev[chunk % 2], stream);
             if (e != hipSuccess) {

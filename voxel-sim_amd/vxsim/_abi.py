@@ -17,6 +17,8 @@ VX_ERR_NONFINITE_CORRECTION = 5
 VX_ERR_INVALID_ARGUMENT = 6
 VX_ERR_DEVICE = 7
 VX_EMPTY = 0xFFFFFFFF
+VX_LAUNCH_GRAPHS = 0
+VX_LAUNCH_DIRECT = 1
 
 VX_SOLVE_MIXED = 0
 VX_SOLVE_F64 = 1
@@ -74,6 +76,16 @@ class StepParams(ctypes.Structure):
     pass
 
 
+class KernelStats(ctypes.Structure):
+    _fields_ = [("dispatches", ctypes.c_uint32), ("reserved", ctypes.c_uint32), ("mean_us", ctypes.c_double),
+                ("median_us", ctypes.c_double), ("min_us", ctypes.c_double), ("max_us", ctypes.c_double),
+                ("total_us", ctypes.c_double)]
+
+
+class KernelTimes(ctypes.Structure):
+    _fields_ = [("spmv", KernelStats), ("update", KernelStats)]
+
+
 class SolveReport(ctypes.Structure):
     _fields_ = [
         ("inner_iterations", ctypes.c_uint32), ("outer_sweeps", ctypes.c_uint32),
@@ -123,6 +135,7 @@ SIGNATURES = [
     ("vx_last_error", ctypes.c_char_p, []),
     ("vx_init", ctypes.c_int32, []),
     ("vx_copy_bandwidth", ctypes.c_int32, [ctypes.c_uint64, ctypes.c_uint32, c_dp]),
+    ("vx_set_launch_mode", ctypes.c_int32, [ctypes.c_int32]),
     ("vx_solve_pcg", ctypes.c_int32,
      [c_fp, c_u32p, c_fp, c_fp, c_fp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float, c_fp, c_u32p, c_fp]),
     ("vx_solve_pcg_f64", ctypes.c_int32,
@@ -139,6 +152,8 @@ SIGNATURES = [
     ("vx_scene_download", ctypes.c_int32, [ctypes.c_void_p, c_dp, c_dp, c_dp, c_dp]),
     ("vx_scene_export_rows", ctypes.c_int32, [ctypes.c_void_p, c_u32p, c_u32p, c_dp, c_dp, c_dp, c_u32p]),
     ("vx_scene_time_spmv", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint32, c_dp]),
+    ("vx_scene_solve_timed", ctypes.c_int32,
+     [ctypes.c_void_p, ctypes.POINTER(SolveParams), ctypes.POINTER(SolveReport), ctypes.POINTER(KernelTimes)]),
     ("vx_scene_step", ctypes.c_int32,
      [ctypes.c_void_p, ctypes.POINTER(StepParams), ctypes.POINTER(StepReport)]),
     ("vx_scene_set_schedule", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint32]),

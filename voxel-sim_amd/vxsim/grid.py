@@ -12,7 +12,8 @@ import ctypes
 import numpy as np
 
 from . import lib
-from ._abi import (FractureParams, SceneDesc, SolveParams, SolveReport, StepParams, StepReport, VX_PRECOND_BLOCK_JACOBI,
+from ._abi import (FractureParams, KernelTimes, SceneDesc, SolveParams, SolveReport, StepParams, StepReport,
+                   VX_LAUNCH_DIRECT, VX_LAUNCH_GRAPHS, VX_PRECOND_BLOCK_JACOBI,
                    VX_PRECOND_RED_BLACK, VX_SOLVE_F64, VX_SOLVE_MIXED, c_dp, c_fp, c_u8p, c_u32p)
 from .solver import SolverError, check
 
@@ -215,7 +216,26 @@ class GridScene:
         """Inner PCG iterations per scheduling tick (graph replay)."""
         check(lib().vx_scene_set_schedule(self.handle, int(iterations_per_tick)))
 
+    def solve_timed(self, params=None, **kw) -> dict:
+        """vx_scene_solve_timed: one solve with every bond SpMV and update
+        dispatch timestamped; the report plus "spmv" / "update" statistics in
+        microseconds per dispatch."""
+        p = params if params is not None else solve_params(**kw)
+        rep = SolveReport()
+        times = KernelTimes()
+        check(lib().vx_scene_solve_timed(self.handle, ctypes.byref(p), ctypes.byref(rep), ctypes.byref(times)))
+        out = _report(rep)
+        out["spmv"] = _report(times.spmv)
+        out["update"] = _report(times.update)
+        return out
+
     def time_spmv(self, reps=50) -> float:
         us = ctypes.c_double(0.0)
         check(lib().vx_scene_time_spmv(self.handle, int(reps), ctypes.byref(us)))
         return float(us.value)
+
+
+def set_launch_mode(direct: bool) -> None:
+    """vx_set_launch_mode: PCG chunks replayed as captured hipGraphs (the
+    default) or launched kernel by kernel (direct), with identical results."""
+    check(lib().vx_set_launch_mode(VX_LAUNCH_DIRECT if direct else VX_LAUNCH_GRAPHS))
