@@ -1101,6 +1101,204 @@ __global__ __launch_bounds__(kBlock) void pcg_bond_spmv(DevArena A, OneScene one
     tile_store<1>(part, A, c.t, out, lds);
 }
 
+// The bond SpMV of large arenas (EARLY launches without red-black scenes):
+// the same arithmetic, in the same order, as pcg_bond_spmv, with the
+// neighbour loads streamed axis by axis instead of all at once. The own row,
+// the x axis and the y axis's vectors are issued before the partial sums; the
+// y axis's blocks and the z axis's vectors once the x axis has been consumed,
+// the z axis's blocks after the y axis, and a scene's pre-summed r.z and r.r
+// are read directly instead of through a workgroup sum. Fewer than two axes'
+// values are live, so the kernel fits 6 waves per SIMD (three 512-thread
+// workgroups per CU) instead of 4. FP32 only (the FP64 instance would spill).
+// Measured in-solve: 96x96x64 20.5 -> 18.4 us per launch; 192x192x96 111.7
+// -> 110.4 us (that launch is not occupancy-bound).
+template <class S>
+__device__ __forceinline__ void ld_axis_vec(const S *__restrict__ z, const S *__restrict__ p_old, size_t R, uint32_t om,
+                                            uint32_t op, bool inm, bool inp, S (&zm)[3], S (&pm)[3], S (&zq)[3],
+                                            S (&pq)[3]) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        if (!inm) {
+            zm[q] = ld_row(z + q * R, om);
+            pm[q] = ld_row(p_old + q * R, om);
+        }
+        if (!inp) {
+            zq[q] = ld_row(z + q * R, op);
+            pq[q] = ld_row(p_old + q * R, op);
+        }
+    }
+}
+template <class S>
+__device__ __forceinline__ void ld_axis_blk(const S *__restrict__ Ka, size_t R, uint32_t om, uint32_t ov, S (&km)[6],
+                                            S (&kp)[6]) {
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        km[q] = ld_row(Ka + q * R, om);
+        kp[q] = ld_row(Ka + q * R, ov);  // zero when the +axis bond is absent
+    }
+}
+// One axis of the bond sum: the -axis neighbour (block km, direction formed
+// from zm, pm) then the +axis one, accumulated into a as pcg_bond_spmv does.
+template <class S>
+__device__ __forceinline__ void axis_sum(S beta, bool lo, const S (&zm)[3], const S (&pm)[3], const S (&zq)[3],
+                                         const S (&pq)[3], const S (&km_in)[6], const S (&kp)[6], S &a0, S &a1, S &a2) {
+    S km[6], u[3], w[3];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) km[q] = lo ? km_in[q] : S(0);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        u[q] = fmaS<S>(beta, pm[q], zm[q]);
+        w[q] = fmaS<S>(beta, pq[q], zq[q]);
+    }
+    a0 = fmaS<S>(km[2], u[2], fmaS<S>(km[1], u[1], fmaS<S>(km[0], u[0], a0)));
+    a1 = fmaS<S>(km[4], u[2], fmaS<S>(km[3], u[1], fmaS<S>(km[1], u[0], a1)));
+    a2 = fmaS<S>(km[5], u[2], fmaS<S>(km[4], u[1], fmaS<S>(km[2], u[0], a2)));
+    a0 = fmaS<S>(kp[2], w[2], fmaS<S>(kp[1], w[1], fmaS<S>(kp[0], w[0], a0)));
+    a1 = fmaS<S>(kp[4], w[2], fmaS<S>(kp[3], w[1], fmaS<S>(kp[1], w[0], a1)));
+    a2 = fmaS<S>(kp[5], w[2], fmaS<S>(kp[4], w[1], fmaS<S>(kp[2], w[0], a2)));
+}
+template <class S, bool ONE>
+__global__ __launch_bounds__(kBlock, 6) void pcg_bond_spmv_big(DevArena A, OneScene one, int odd) {
+    Tile c;
+    if (!inner_tile<S, ONE>(A, one, c)) return;
+    __shared__ double lds[2][kWaves];
+    SlotState *st = A.st + c.slot;
+    const uint32_t done_in = st->done;
+    if (done_in) return;  // the scene's inner solve has ended: no row loads
+    const uint32_t k = st->k_spmv;
+    const uint32_t seg = st->seg;
+    const double rho_s[2] = {st->rho[0], st->rho[1]};
+    const double rr0_s = st->rr0, bb = st->bb;
+    const SlotParams prm = A.prm[c.slot];
+    // A scene of more than kBigTiles tiles has its r.z and r.r pre-summed in
+    // its state (scene_reduce): every thread reads them (uniform loads) and
+    // no workgroup sum is needed. Smaller scenes of the same arena re-sum
+    // their partials.
+    const bool presummed = ONE || c.g.ntiles > kBigTiles;
+    double sums[2];
+    const int kinds[2] = {kRz, kRr};
+    if (presummed) {
+        sums[0] = st->red[kRz];
+        sums[1] = st->red[kRr];
+    } else {
+        scene_load<2>(sums, A, c.g, kinds);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    const size_t R = A.R;
+    const S *__restrict__ z = Inner<S>::z(A);
+    const S *__restrict__ p_old = Inner<S>::p(A, odd);
+    const S *__restrict__ K = Inner<S>::k(A);
+    const S *__restrict__ Dp = Inner<S>::d(A);
+    const uint32_t v = c.row ? c.v : c.g.voff;  // rows past the scene load row 0 and discard
+    uint32_t cc[3];
+    coords(c.g, c.row ? c.local : 0u, cc);
+    constexpr uint32_t sz = sizeof(S);
+    const uint32_t ov = v * sz;
+    // In-tile neighbours come from the staged own values in LDS (FP32).
+    constexpr bool kStage = std::is_same<S, float>::value;
+    __shared__ S stage[kStage ? 6 : 1][kStage ? kBlock : 1];
+    uint32_t om[3], op[3], str[3];
+    bool lo[3], inm[3] = {false, false, false}, inp[3] = {false, false, false};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        str[a] = stride_of(c.g, a);
+        const bool hi = cc[a] + 1 < extent_of(c.g, a);
+        lo[a] = cc[a] > 0;
+        om[a] = lo[a] ? ov - str[a] * sz : ov;
+        op[a] = hi ? ov + str[a] * sz : ov;
+        if constexpr (kStage) {
+            inm[a] = lo[a] && threadIdx.x >= str[a];
+            inp[a] = hi && threadIdx.x + str[a] < kBlock;
+        }
+    }
+    S z0[3], p0[3], Dv[6];
+    S zm0[3], pm0[3], zq0[3], pq0[3], km0[6], kp0[6];
+    S zm1[3], pm1[3], zq1[3], pq1[3], km1[6], kp1[6];
+    S zm2[3], pm2[3], zq2[3], pq2[3], km2[6], kp2[6];
+    const uint8_t m = A.mask[v];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        z0[q] = ld_row(z + q * R, ov);
+        p0[q] = ld_row(p_old + q * R, ov);
+    }
+#pragma unroll
+    for (int q = 0; q < 6; ++q) Dv[q] = ld_row(Dp + q * R, ov);
+    ld_axis_vec<S>(z, p_old, R, om[0], op[0], inm[0], inp[0], zm0, pm0, zq0, pq0);
+    ld_axis_blk<S>(K, R, om[0], ov, km0, kp0);
+    ld_axis_vec<S>(z, p_old, R, om[1], op[1], inm[1], inp[1], zm1, pm1, zq1, pq1);
+    __builtin_amdgcn_sched_barrier(0);
+    if constexpr (kStage) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            stage[q][threadIdx.x] = z0[q];
+            stage[3 + q][threadIdx.x] = p0[q];
+        }
+    }
+    if (presummed) {
+        if constexpr (kStage) __syncthreads();  // the staging stores before the reads below
+    } else {
+        block_sum_all<2>(sums, lds);  // its barriers also order the staging stores before the reads below
+    }
+    S beta;
+    if (!inner_head<S>(st, prm, sums, k, seg, rr0_s, bb, rho_s, c.t == c.g.tile0 && threadIdx.x == 0, odd, beta)) return;
+    auto from_stage = [&](int a, S (&zm)[3], S (&pm)[3], S (&zq)[3], S (&pq)[3]) {
+        if constexpr (kStage) {
+            if (inm[a])
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    zm[q] = stage[q][threadIdx.x - str[a]];
+                    pm[q] = stage[3 + q][threadIdx.x - str[a]];
+                }
+            if (inp[a])
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    zq[q] = stage[q][threadIdx.x + str[a]];
+                    pq[q] = stage[3 + q][threadIdx.x + str[a]];
+                }
+        }
+    };
+    S p[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) p[q] = fmaS<S>(beta, p0[q], z0[q]);
+    S y0 = fmaS<S>(Dv[2], p[2], fmaS<S>(Dv[1], p[1], Dv[0] * p[0]));
+    S y1 = fmaS<S>(Dv[4], p[2], fmaS<S>(Dv[3], p[1], Dv[1] * p[0]));
+    S y2 = fmaS<S>(Dv[5], p[2], fmaS<S>(Dv[4], p[1], Dv[2] * p[0]));
+    S a0 = S(0), a1 = S(0), a2 = S(0);
+    from_stage(0, zm0, pm0, zq0, pq0);
+    axis_sum<S>(beta, lo[0], zm0, pm0, zq0, pq0, km0, kp0, a0, a1, a2);
+    // The x axis is summed before the next loads are issued (its registers
+    // are free again): an empty asm that takes the sums and orders memory.
+    asm volatile("" : "+v"(a0), "+v"(a1), "+v"(a2) : : "memory");
+    ld_axis_blk<S>(K + 6 * R, R, om[1], ov, km1, kp1);
+    ld_axis_vec<S>(z, p_old, R, om[2], op[2], inm[2], inp[2], zm2, pm2, zq2, pq2);
+    __builtin_amdgcn_sched_barrier(0);
+    from_stage(1, zm1, pm1, zq1, pq1);
+    axis_sum<S>(beta, lo[1], zm1, pm1, zq1, pq1, km1, kp1, a0, a1, a2);
+    asm volatile("" : "+v"(a0), "+v"(a1), "+v"(a2) : : "memory");
+    ld_axis_blk<S>(K + 12 * R, R, om[2], ov, km2, kp2);
+    from_stage(2, zm2, pm2, zq2, pq2);
+    axis_sum<S>(beta, lo[2], zm2, pm2, zq2, pq2, km2, kp2, a0, a1, a2);
+    if (m & 1) y0 -= a0;
+    if (m & 2) y1 -= a1;
+    if (m & 4) y2 -= a2;
+    double part[1] = {0.0};
+    if (c.row) {
+        S *__restrict__ p_new = Inner<S>::p(A, !odd);
+        S *__restrict__ ap = Inner<S>::ap(A);
+        st_row(p_new, ov, p[0]);
+        st_row(p_new + R, ov, p[1]);
+        st_row(p_new + 2 * R, ov, p[2]);
+        st_row(ap, ov, y0);
+        st_row(ap + R, ov, y1);
+        st_row(ap + 2 * R, ov, y2);
+        part[0] = double(p[0]) * double(y0);
+        part[0] = __builtin_fma(double(p[1]), double(y1), part[0]);
+        part[0] = __builtin_fma(double(p[2]), double(y2), part[0]);
+    }
+    const int out[1] = {kPap};
+    tile_store<1>(part, A, c.t, out, lds);
+}
+
 // alpha = rho / p.Ap; d += alpha p (d = alpha p on the first iteration of a
 // segment);
 // r -= alpha Ap; z = Dinv r; partials of r.z and r.r. Row loads are issued
@@ -1742,7 +1940,10 @@ void launch_iteration_t(vx_batch *b, const OneScene &one, const dim3 &grid, cons
         if (big) hipLaunchKernelGGL(scene_reduce<S>, sg, blk, 0, ctx.stream, b->A, k0, nk, 0);
     };
     if constexpr (RB) hipLaunchKernelGGL((pcg_rb_red<S, ONE, EARLY>), grid_u, blk, 0, ctx.stream, b->A, one_u, odd);
-    launch_kernel(0, pcg_bond_spmv<S, ONE, RB, EARLY>, grid, blk, b->A, one, odd);
+    if constexpr (EARLY && !RB && std::is_same<S, float>::value)
+        launch_kernel(0, pcg_bond_spmv_big<S, ONE>, grid, blk, b->A, one, odd);
+    else
+        launch_kernel(0, pcg_bond_spmv<S, ONE, RB, EARLY>, grid, blk, b->A, one, odd);
     reduce(kPap, 1);
     launch_kernel(1, pcg_update<S, ONE, RB, EARLY>, grid_u, blk, b->A, one_u, odd);
     reduce(kRz, 2);
