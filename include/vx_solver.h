@@ -106,6 +106,27 @@ int32_t vx_solve_pcg_f64(const float *rhs, const uint32_t *neighbors, const floa
                          uint32_t max_iterations, float tolerance, float *out_correction,
                          uint32_t *out_iterations, float *out_relative_residual);
 
+/*
+ * vx_solve_pcg_noalloc — vx_solve_pcg without any device allocation. No
+ * counterpart in the reference: an entry point for callers that cannot
+ * allocate device memory (tight serving loops, harnesses that forbid
+ * hipMalloc). Same arguments, semantics, statuses and results (bit for bit)
+ * as vx_solve_pcg. Every device buffer of the solve lives in one statically
+ * allocated 64 MiB scratch area per device (held until the process ends),
+ * which bounds n: about 90,000 rows; a larger n returns VX_ERR_TOO_LARGE
+ * before touching the device. The PCG chunks are launched directly (no graph
+ * is instantiated) and the host waits with stream synchronisations (no
+ * events are created). Call vx_init() once at set-up: the first call of the
+ * library initialises the device. The scratch area is shared by every call
+ * on a device and keeps the last call's contents; calls are serialised by
+ * the library's context lock (calls that overlapped on one device would
+ * race on it).
+ */
+int32_t vx_solve_pcg_noalloc(const float *rhs, const uint32_t *neighbors, const float *diagonal,
+                             const float *blocks, const float *inverse_diagonal, uint32_t n,
+                             uint32_t max_iterations, float tolerance, float *out_correction,
+                             uint32_t *out_iterations, float *out_relative_residual);
+
 /* ---- device-resident structured scenes (build extensions) --------------- */
 /*
  * A box scene of nx*ny*nz REVs kept resident in HBM: SoA state, on-device

@@ -61,6 +61,9 @@ Context &context();
 int32_t ensure_init();
 // Named grow-only device buffer of at least `bytes`.
 int32_t buffer(const char *name, size_t bytes, void **out);
+// Device address of the allocation-free row solve's scratch area, looked up
+// at initialisation (vx_rows.hip).
+void rows_scratch_setup();
 
 // XCD-aware tile index: consecutive logical tiles land on the same XCD, so
 // the z-neighbour rows a tile reads were fetched into that XCD's L2 by the

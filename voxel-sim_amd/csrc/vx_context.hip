@@ -59,6 +59,7 @@ int32_t ensure_init() {
     if ((e = hipHostMalloc(&c.pinned, c.pinned_bytes, hipHostMallocDefault)) != hipSuccess)
         return cache_failure("hipHostMalloc", e);
     c.init_status = VX_OK;
+    rows_scratch_setup();
     return VX_OK;
 }
 

@@ -20,6 +20,7 @@
 // waits for the host.
 #include "vx_common.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -379,10 +380,65 @@ struct RowsGraph {
 };
 RowsGraph g_graph[2];
 
+// Scratch area of vx_solve_pcg_noalloc: every device buffer of one f32 row
+// solve (staging copies of the caller's rows, the SoA planes, the vectors,
+// partials, state and output), carved out of one statically allocated array
+// per device. Host code never passes `scratch` itself to a kernel or a copy
+// (in host code it names a host-side stand-in): it uses the device address
+// obtained once per device with hipGetSymbolAddress.
+constexpr size_t kScratchBytes = size_t(64) << 20;  // 64 MiB, held by each device until the process ends
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+void *g_scratch_dev[kMaxDevices] = {};  // device address of `scratch`, per device
+
+// The device address of `scratch` on the current device.
+int32_t scratch_address(void **out) {
+    int dev = 0;
+    VX_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(VX_ERR_DEVICE, "device %d outside the scratch table", dev);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return fail(VX_ERR_DEVICE, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+        g_scratch_dev[dev] = p;
+    }
+    *out = g_scratch_dev[dev];
+    return VX_OK;
+}
+
+}  // namespace
+
+// Called by the library's initialisation with device 0 current: looks the
+// scratch area's device address up once, so the allocation-free entry point
+// does no set-up of its own (it retries if this failed).
+void rows_scratch_setup() {
+    void *p = nullptr;
+    (void)scratch_address(&p);
+}
+
+namespace {
+
+// Bytes of the solve's buffers for n rows and a grid of `grid` workgroups,
+// each rounded up to 256 (the scratch carve-out's alignment).
+template <class S>
+size_t rows_buffer_bytes(size_t N, uint32_t grid, size_t state_bytes) {
+    const size_t sizes[] = {6 * N * 4, 54 * N * 4, 9 * N * 4, 9 * N * 4, 3 * N * 4,  // staging
+                            6 * N * 4, 54 * N * sizeof(S), 9 * N * sizeof(S), 9 * N * sizeof(S),
+                            3 * N * sizeof(S), 3 * N * sizeof(S), 3 * N * sizeof(S), 3 * N * sizeof(S),
+                            3 * N * sizeof(S), 3 * N * sizeof(S), 4 * size_t(grid) * 8, state_bytes, 3 * N * 4};
+    size_t total = 0;
+    for (size_t b : sizes) total += (std::max<size_t>(b, 256) + 255) & ~size_t(255);
+    return total;
+}
+
+// noalloc: the buffers come from the scratch area, the chunks are launched
+// directly (no graph instantiation) and the host waits for each chunk with a
+// stream synchronisation (no events), so the call allocates nothing.
 template <class S>
 int32_t solve_rows(const float *rhs, const uint32_t *neighbors, const float *diagonal, const float *blocks,
                    const float *inverse_diagonal, uint32_t n, uint32_t max_iterations, float tolerance,
-                   float *out_correction, uint32_t *out_iterations, float *out_relative_residual) {
+                   float *out_correction, uint32_t *out_iterations, float *out_relative_residual,
+                   bool noalloc = false) {
     constexpr bool kF64 = sizeof(S) == 8;
     clear_error();
     if (n > 0 && (!rhs || !neighbors || !diagonal || !blocks || !inverse_diagonal || !out_correction))
@@ -402,21 +458,36 @@ int32_t solve_rows(const float *rhs, const uint32_t *neighbors, const float *dia
         if (out_relative_residual) *out_relative_residual = 0.0f;
         return VX_OK;
     }
+    const size_t N = n;
+    uint32_t per_xcd = 0;
+    const uint32_t ntiles = (n + kBlock - 1) / kBlock;
+    const uint32_t grid = xcd_grid(ntiles, &per_xcd);
+    if (noalloc) {
+        const size_t need = rows_buffer_bytes<S>(N, grid, sizeof(RowsState));
+        if (need > kScratchBytes)
+            return fail(VX_ERR_TOO_LARGE, "%u rows need %zu bytes of scratch; the allocation-free solve has %zu", n, need,
+                        kScratchBytes);
+    }
     int32_t st_code = ensure_init();
     if (st_code != VX_OK) return st_code;
     Context &ctx = context();
     std::lock_guard<std::recursive_mutex> lock(ctx.mutex);
     hipStream_t stream = ctx.stream;
     VX_HIP(hipSetDevice(0));
-    const size_t N = n;
-    uint32_t per_xcd = 0;
-    const uint32_t ntiles = (n + kBlock - 1) / kBlock;
-    const uint32_t grid = xcd_grid(ntiles, &per_xcd);
+    char *scratch_base = nullptr;
+    size_t scratch_used = 0;
+    if (noalloc)
+        if (int32_t c = scratch_address(reinterpret_cast<void **>(&scratch_base))) return c;
 
     void *s_nbr, *s_blk, *s_diag, *s_inv, *s_rhs;
     void *d_nbr, *d_blk, *d_diag, *d_inv, *d_dx, *d_r, *d_z, *d_p0, *d_p1, *d_ap, *d_part, *d_state, *d_out;
     const char *pre = kF64 ? "rows64_" : "rows32_";
-    auto B = [&](const char *name, size_t bytes, void **out) {
+    auto B = [&](const char *name, size_t bytes, void **out) -> int32_t {
+        if (noalloc) {  // carve out of the scratch area (size checked above)
+            *out = scratch_base + scratch_used;
+            scratch_used += (std::max<size_t>(bytes, 256) + 255) & ~size_t(255);
+            return VX_OK;
+        }
         std::string key = std::string(pre) + name;
         return buffer(key.c_str(), bytes, out);
     };
@@ -499,7 +570,7 @@ int32_t solve_rows(const float *rhs, const uint32_t *neighbors, const float *dia
                 if (big) hipLaunchKernelGGL(rows_reduce, dim3(1), dim3(kBlock), 0, stream, part, grid, int(kRz), 2, st);
             }
         };
-        const bool direct = ctx.direct_launch;
+        const bool direct = ctx.direct_launch || noalloc;
         RowsGraph &g = g_graph[kF64 ? 1 : 0];
         const void *key[4] = {d_blk, d_p0, d_part, d_state};
         bool reuse = direct || (g.exec && g.n == n && std::memcmp(key, g.key, sizeof key) == 0);
@@ -518,41 +589,59 @@ int32_t solve_rows(const float *rhs, const uint32_t *neighbors, const float *dia
             g.n = n;
             std::memcpy(g.key, key, sizeof key);
         }
-        hipEvent_t ev[2];
-        VX_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
-        VX_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
         uint32_t launched = 0, chunk = 0;
         int32_t rc = VX_OK;
-        // The check after the last update runs in the following rows_spmv, so
-        // replay until one spmv past the cap has been issued.
-        while (launched <= max_iterations) {
-            hipError_t e = hipSuccess;
-            if (direct) {
+        if (noalloc) {
+            // One chunk at a time, the status read after each (no events):
+            // the chunks a lookahead would issue after convergence change
+            // nothing, so the result is the same.
+            while (launched <= max_iterations) {
                 enqueue_chunk();
-                e = hipGetLastError();
-            } else {
-                e = hipGraphLaunch(g.exec, stream);
-            }
-            if (e == hipSuccess) e = hipMemcpyAsync(&pinned[chunk % 2], d_state, sizeof hs, hipMemcpyDeviceToHost, stream);
-            if (e == hipSuccess) e = hipEventRecord(ev[chunk % 2], stream);
-            if (e != hipSuccess) {
-                rc = fail(VX_ERR_DEVICE, "PCG graph launch failed: %s", hipGetErrorString(e));
-                break;
-            }
-            launched += kChunk;
-            if (chunk >= 1) {
-                e = hipEventSynchronize(ev[(chunk - 1) % 2]);
+                hipError_t e = hipGetLastError();
+                if (e == hipSuccess) e = hipMemcpyAsync(&pinned[0], d_state, sizeof hs, hipMemcpyDeviceToHost, stream);
+                if (e == hipSuccess) e = hipStreamSynchronize(stream);
                 if (e != hipSuccess) {
-                    rc = fail(VX_ERR_DEVICE, "PCG status read failed: %s", hipGetErrorString(e));
+                    rc = fail(VX_ERR_DEVICE, "PCG chunk failed: %s", hipGetErrorString(e));
                     break;
                 }
-                if (pinned[(chunk - 1) % 2].done) break;
+                launched += kChunk;
+                if (pinned[0].done) break;
             }
-            chunk += 1;
+        } else {
+            hipEvent_t ev[2];
+            VX_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+            VX_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+            // The check after the last update runs in the following rows_spmv, so
+            // replay until one spmv past the cap has been issued.
+            while (launched <= max_iterations) {
+                hipError_t e = hipSuccess;
+                if (direct) {
+                    enqueue_chunk();
+                    e = hipGetLastError();
+                } else {
+                    e = hipGraphLaunch(g.exec, stream);
+                }
+                if (e == hipSuccess) e = hipMemcpyAsync(&pinned[chunk % 2], d_state, sizeof hs, hipMemcpyDeviceToHost, stream);
+                if (e == hipSuccess) e = hipEventRecord(ev[chunk % 2], stream);
+                if (e != hipSuccess) {
+                    rc = fail(VX_ERR_DEVICE, "PCG graph launch failed: %s", hipGetErrorString(e));
+                    break;
+                }
+                launched += kChunk;
+                if (chunk >= 1) {
+                    e = hipEventSynchronize(ev[(chunk - 1) % 2]);
+                    if (e != hipSuccess) {
+                        rc = fail(VX_ERR_DEVICE, "PCG status read failed: %s", hipGetErrorString(e));
+                        break;
+                    }
+                    if (pinned[(chunk - 1) % 2].done) break;
+                }
+                chunk += 1;
+            }
+            (void)hipStreamSynchronize(stream);
+            (void)hipEventDestroy(ev[0]);
+            (void)hipEventDestroy(ev[1]);
         }
-        (void)hipStreamSynchronize(stream);
-        (void)hipEventDestroy(ev[0]);
-        (void)hipEventDestroy(ev[1]);
         if (rc != VX_OK) return rc;
     }
     hipLaunchKernelGGL(rows_pack_out<S>, dim3(rgrid), dim3(kBlock), 0, stream, n, dx, static_cast<float *>(d_out));
@@ -584,6 +673,14 @@ extern "C" int32_t vx_solve_pcg(const float *rhs, const uint32_t *neighbors, con
                                 float *out_correction, uint32_t *out_iterations, float *out_relative_residual) {
     return vx::solve_rows<float>(rhs, neighbors, diagonal, blocks, inverse_diagonal, n, max_iterations, tolerance,
                                  out_correction, out_iterations, out_relative_residual);
+}
+
+extern "C" int32_t vx_solve_pcg_noalloc(const float *rhs, const uint32_t *neighbors, const float *diagonal,
+                                        const float *blocks, const float *inverse_diagonal, uint32_t n,
+                                        uint32_t max_iterations, float tolerance, float *out_correction,
+                                        uint32_t *out_iterations, float *out_relative_residual) {
+    return vx::solve_rows<float>(rhs, neighbors, diagonal, blocks, inverse_diagonal, n, max_iterations, tolerance,
+                                 out_correction, out_iterations, out_relative_residual, true);
 }
 
 extern "C" int32_t vx_solve_pcg_f64(const float *rhs, const uint32_t *neighbors, const float *diagonal,

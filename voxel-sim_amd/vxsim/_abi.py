@@ -140,6 +140,8 @@ SIGNATURES = [
      [c_fp, c_u32p, c_fp, c_fp, c_fp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float, c_fp, c_u32p, c_fp]),
     ("vx_solve_pcg_f64", ctypes.c_int32,
      [c_fp, c_u32p, c_fp, c_fp, c_fp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float, c_fp, c_u32p, c_fp]),
+    ("vx_solve_pcg_noalloc", ctypes.c_int32,
+     [c_fp, c_u32p, c_fp, c_fp, c_fp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float, c_fp, c_u32p, c_fp]),
     ("vx_scene_create", ctypes.c_int32, [ctypes.POINTER(SceneDesc), ctypes.POINTER(ctypes.c_void_p)]),
     ("vx_scene_destroy", ctypes.c_int32, [ctypes.c_void_p]),
     ("vx_scene_predict", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_double, c_dp]),

@@ -69,3 +69,10 @@ def solve_pcg_f64(rhs, neighbors, diagonal, blocks, inverse_diagonal, max_iterat
     """The plain FP64 GPU solve over the same inputs."""
     return _call(lib().vx_solve_pcg_f64, rhs, neighbors, diagonal, blocks, inverse_diagonal, max_iterations,
                  tolerance)
+
+
+def solve_pcg_noalloc(rhs, neighbors, diagonal, blocks, inverse_diagonal, max_iterations, tolerance):
+    """solve_pcg without device allocation (vx_solve_pcg_noalloc): the same
+    results, at most about 90,000 rows (the library's static scratch area)."""
+    return _call(lib().vx_solve_pcg_noalloc, rhs, neighbors, diagonal, blocks, inverse_diagonal, max_iterations,
+                 tolerance)
