@@ -345,6 +345,20 @@ int32_t vx_scene_set_targets_and_loads(vx_scene *scene, const float *targets, co
  * history 3*n*6 (VX_H_* order), all per +axis bond [axis][voxel]. */
 int32_t vx_scene_download_bonds(vx_scene *scene, uint32_t *neighbor, float *damage, float *plastic_strain,
                                 float *history);
+/* The bond forces of the scene's last substep, per +axis bond, (3, n, 3)
+ * [axis][voxel][component], f64; either pointer may be NULL. For each
+ * constraint of the substep's last assembly (every bond except those whose
+ * endpoints have no free axis and whose first REV is kinematic), at the
+ * solved positions and before the fracture pass updates the bond:
+ *   last_force  F (K (u - plastic CELL)) (1 - damage), zero for a kinematic
+ *               first REV (the reference's BondWorkspace::last_force,
+ *               src/solver.rs:627-640, read by its loading fixtures,
+ *               src/experiments.rs:32-49);
+ *   lambda      -F^T k (p_j - p_i - rest) h^2 (BondWorkspace::lambda,
+ *               src/solver.rs:1824-1827, 1921-1927).
+ * Zero for every other bond. Valid after vx_scene_step or
+ * vx_scene_finish_substep (a later vx_scene_assemble overwrites them). */
+int32_t vx_scene_download_bond_forces(vx_scene *scene, double *last_force, double *lambda);
 
 /* Inner PCG iterations per scheduling tick (one hipGraph replay; default 16,
  * even, at most 256). 0 keeps the current value. */
@@ -409,6 +423,8 @@ int32_t vx_batch_download(vx_batch *batch, uint32_t slot, double *positions, dou
 /* Bond state of one scene, as vx_scene_download_bonds. */
 int32_t vx_batch_download_bonds(vx_batch *batch, uint32_t slot, uint32_t *neighbor, float *damage,
                                 float *plastic_strain, float *history);
+/* Bond forces of one scene, as vx_scene_download_bond_forces. */
+int32_t vx_batch_download_bond_forces(vx_batch *batch, uint32_t slot, double *last_force, double *lambda);
 /* Free a finished (or failed) scene's tiles and slot. */
 int32_t vx_batch_release(vx_batch *batch, uint32_t slot);
 

@@ -118,13 +118,13 @@ def lib():
                                           ctypes.c_uint32, _dp, _u32p, _dp]
         L.vxo_matvec.argtypes = [ctypes.c_int, ctypes.c_uint32, _u32p, _dp, _dp, _dp, _dp]
         L.vxo_step.argtypes = [ctypes.c_int, ctypes.POINTER(_Scene), ctypes.POINTER(_StepConfig), _dp, _dp,
-                               ctypes.POINTER(_StepReport), _dp]
+                               ctypes.POINTER(_StepReport), _dp, _dp]
         L.vxo_pair_limits.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
         L.vxo_fatigue.argtypes = [ctypes.c_int, ctypes.POINTER(_Bond), ctypes.c_double, ctypes.POINTER(_FractureConfig)]
         L.vxo_fracture_update.argtypes = [ctypes.c_int, ctypes.POINTER(_Bond), _dp, _dp, _dp, ctypes.c_double,
                                           ctypes.POINTER(_FractureConfig), ctypes.POINTER(_Dissipation)]
         L.vxo_step_fracture.argtypes = [ctypes.c_int, ctypes.POINTER(_Scene), ctypes.POINTER(_StepConfig), _dp, _dp,
-                                        ctypes.POINTER(_FractureState), ctypes.POINTER(_StepReport)]
+                                        ctypes.POINTER(_FractureState), ctypes.POINTER(_StepReport), _dp, _dp]
         L.vxo_dot_rows.restype = ctypes.c_double
         L.vxo_dot_rows.argtypes = [ctypes.c_uint32, _dp, _dp]
         L.vxo_guarded_ratio.restype = ctypes.c_double
@@ -363,12 +363,14 @@ def step(oscene: OracleScene, cfg, mode=F64, precond=BJ, exact=False, exact_tole
     tg = None if targets is None else _f64(targets).ravel()
     ld = None if loads is None else _f64(loads).ravel()
     forces = np.zeros((3, oscene.n, 3)) if want_forces else None
+    lam = np.zeros((3, oscene.n, 3)) if want_forces else None
     lib().vxo_step(mode, ctypes.byref(oscene.struct), ctypes.byref(c), _p(tg, _dp), _p(ld, _dp),
-                   ctypes.byref(rep), _p(forces, _dp))
+                   ctypes.byref(rep), _p(forces, _dp), _p(lam, _dp))
     out = dict(pcg_iterations=rep.pcg_iterations, passes=rep.passes, solver_residual=rep.solver_residual,
                last_angle=rep.last_angle)
     if want_forces:
         out["last_force"] = forces
+        out["lambda"] = lam
     return out
 
 
@@ -426,7 +428,7 @@ class FractureHistory:
 
 
 def step_fracture(oscene: OracleScene, cfg, fcfg, history: FractureHistory, strength, mode=F64, precond=BJ,
-                  exact=False, exact_tolerance=1e-12, targets=None, loads=None):
+                  exact=False, exact_tolerance=1e-12, targets=None, loads=None, want_forces=False):
     """One fixed step (solve part) with the fracture pass of every substep
     (src/solver.rs:627-665). Updates oscene.neighbor / damage / plastic_strain
     and `history`; returns the step report with the broken-bond count and
@@ -441,11 +443,17 @@ def step_fracture(oscene: OracleScene, cfg, fcfg, history: FractureHistory, stre
     rep = _StepReport()
     tg = None if targets is None else _f64(targets).ravel()
     ld = None if loads is None else _f64(loads).ravel()
+    forces = np.zeros((3, oscene.n, 3)) if want_forces else None
+    lam = np.zeros((3, oscene.n, 3)) if want_forces else None
     lib().vxo_step_fracture(mode, ctypes.byref(oscene.struct), ctypes.byref(c), _p(tg, _dp), _p(ld, _dp),
-                            ctypes.byref(fs), ctypes.byref(rep))
-    return dict(pcg_iterations=rep.pcg_iterations, passes=rep.passes, solver_residual=rep.solver_residual,
-                last_angle=rep.last_angle, broken=fs.broken,
-                dissipated={k: getattr(fs.dissipated, k) for k, _ in _Dissipation._fields_})
+                            ctypes.byref(fs), ctypes.byref(rep), _p(forces, _dp), _p(lam, _dp))
+    out = dict(pcg_iterations=rep.pcg_iterations, passes=rep.passes, solver_residual=rep.solver_residual,
+               last_angle=rep.last_angle, broken=fs.broken,
+               dissipated={k: getattr(fs.dissipated, k) for k, _ in _Dissipation._fields_})
+    if want_forces:
+        out["last_force"] = forces
+        out["lambda"] = lam
+    return out
 
 
 

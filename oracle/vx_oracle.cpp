@@ -1569,12 +1569,44 @@ static void fracture_pass(const System<T> &sys, vxo_scene *s, vxo_fracture_state
     }
 }
 
+// Per constraint of the last assembly at the solved positions, before the
+// fracture pass: the multiplier lambda = -F^T k (p_j - p_i - rest) h^2
+// (src/solver.rs:1824-1827, 1921-1927) and, for a non-kinematic first REV, the
+// bond force F (K (u - plastic CELL)) (1 - damage) (src/solver.rs:627-640).
+// Outputs are (3, n, 3) [axis][voxel][component], zero without a constraint.
+template <class T>
+static void bond_forces_t(const System<T> &sys, vxo_scene *s, T h, double *last_force, double *lambda) {
+    const uint32_t n = s->n;
+    Scene<T> sc{s};
+    if (last_force) std::fill(last_force, last_force + 9 * size_t(n), 0.0);
+    if (lambda) std::fill(lambda, lambda + 9 * size_t(n), 0.0);
+    for (const Constraint<T> &c : sys.constraints) {
+        const size_t b = size_t(c.axis) * n + c.i;
+        if (lambda) {
+            V3<T> force = c.k * (sc.pos(c.j) - sc.pos(c.i) - c.rest);
+            V3<T> l = transpose(c.frame) * force * (-(h * h));
+            lambda[3 * b] = double(l.x);
+            lambda[3 * b + 1] = double(l.y);
+            lambda[3 * b + 2] = double(l.z);
+        }
+        if (!last_force || s->kinematic[c.i]) continue;
+        V3<T> off = s->formation_offset ? sc.bond3(s->formation_offset, c.axis, c.i) : V3<T>{0, 0, 0};
+        V3<T> pl = s->plastic_strain ? sc.bond3(s->plastic_strain, c.axis, c.i) : V3<T>{0, 0, 0};
+        V3<T> u = transpose(c.frame) * (sc.pos(c.j) - sc.pos(c.i)) - bond_thermal_rest(sc, c.i, c.j, c.frame) - off;
+        T dmg = s->damage ? T(s->damage[b]) : T(0);
+        V3<T> f = c.frame * (c.material_k * (u - pl * T(kCell))) * (T(1) - dmg);
+        last_force[3 * b] = double(f.x);
+        last_force[3 * b + 1] = double(f.y);
+        last_force[3 * b + 2] = double(f.z);
+    }
+}
+
 // Solve part of step_with_axis_targets_and_loads (src/solver.rs:446-570,
 // 668-683): predict, impose targets, fit rotations, corotation passes of
 // assemble + PCG + apply, then velocities.
 template <class T>
 static void step_t(vxo_scene *s, const vxo_step_config *cfg, const double *targets, const double *loads,
-                   vxo_step_report *report, double *last_force, vxo_fracture_state *fracture = nullptr) {
+                   vxo_step_report *report, double *last_force, double *lambda, vxo_fracture_state *fracture = nullptr) {
     const uint32_t n = s->n;
     const T h = T(cfg->fixed_dt) / T(cfg->substeps);
     const V3<T> g{T(cfg->gravity[0]), T(cfg->gravity[1]), T(cfg->gravity[2])};
@@ -1643,6 +1675,7 @@ static void step_t(vxo_scene *s, const vxo_step_config *cfg, const double *targe
             rep.last_angle = double(angle);
             if (double(angle) <= cfg->corotation_tolerance && residual <= cfg->solver_relative_tolerance) break;
         }
+        if (last_force || lambda) bond_forces_t<T>(sys, s, h, last_force, lambda);
         if (fracture) fracture_pass<T>(sys, s, fracture, h);
         // Velocities from the solved positions (no contact): src/solver.rs:668-683.
         for (uint32_t i = 0; i < n; ++i) {
@@ -1658,39 +1691,21 @@ static void step_t(vxo_scene *s, const vxo_step_config *cfg, const double *targe
             }
         }
     }
-    if (last_force) {
-        // src/solver.rs:627-640: force of each intact constraint of the last
-        // assembly at the final positions, frame * k * (u - plastic*CELL) * (1 - damage).
-        Scene<T> sc{s};
-        std::fill(last_force, last_force + 9 * size_t(n), 0.0);
-        for (const Constraint<T> &c : sys.constraints) {
-            if (s->kinematic[c.i]) continue;
-            size_t b = size_t(c.axis) * n + c.i;
-            V3<T> off = s->formation_offset ? sc.bond3(s->formation_offset, c.axis, c.i) : V3<T>{0, 0, 0};
-            V3<T> pl = s->plastic_strain ? sc.bond3(s->plastic_strain, c.axis, c.i) : V3<T>{0, 0, 0};
-            V3<T> u = transpose(c.frame) * (sc.pos(c.j) - sc.pos(c.i)) - bond_thermal_rest(sc, c.i, c.j, c.frame) - off;
-            T dmg = s->damage ? T(s->damage[b]) : T(0);
-            V3<T> f = c.frame * (c.material_k * (u - pl * T(kCell))) * (T(1) - dmg);
-            last_force[3 * b] = double(f.x);
-            last_force[3 * b + 1] = double(f.y);
-            last_force[3 * b + 2] = double(f.z);
-        }
-    }
     if (report) *report = rep;
 }
 
 void vxo_step(int mode, vxo_scene *s, const vxo_step_config *cfg, const double *targets, const double *loads,
-              vxo_step_report *report, double *last_force) {
+              vxo_step_report *report, double *last_force, double *lambda) {
     if (mode == VXO_F32)
-        step_t<float>(s, cfg, targets, loads, report, last_force);
+        step_t<float>(s, cfg, targets, loads, report, last_force, lambda);
     else
-        step_t<double>(s, cfg, targets, loads, report, last_force);
+        step_t<double>(s, cfg, targets, loads, report, last_force, lambda);
 }
 
 void vxo_step_fracture(int mode, vxo_scene *s, const vxo_step_config *cfg, const double *targets, const double *loads,
-                       vxo_fracture_state *fracture, vxo_step_report *report) {
+                       vxo_fracture_state *fracture, vxo_step_report *report, double *last_force, double *lambda) {
     if (mode == VXO_F32)
-        step_t<float>(s, cfg, targets, loads, report, nullptr, fracture);
+        step_t<float>(s, cfg, targets, loads, report, last_force, lambda, fracture);
     else
-        step_t<double>(s, cfg, targets, loads, report, nullptr, fracture);
+        step_t<double>(s, cfg, targets, loads, report, last_force, lambda, fracture);
 }

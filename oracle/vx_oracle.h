@@ -258,15 +258,18 @@ typedef struct vxo_step_report {
  * step_with_axis_targets_and_loads (src/solver.rs:446-570, 668-683).
  * targets: n*3 world coordinates, NaN for a free axis (may be NULL).
  * loads: n*3 applied loads (forces + accel*mass) (may be NULL).
- * last_force: optional 3*n*3 output, per +axis bond, of the bond force of the
- * last assembly evaluated at the final positions (src/solver.rs:627-640). */
+ * last_force, lambda: optional 3*n*3 outputs ([axis][voxel][component]) of
+ * the last substep, per constraint of its last assembly at the solved
+ * positions and before any fracture update: the bond force
+ * (src/solver.rs:627-640) and the multiplier (src/solver.rs:1824-1827). */
 void vxo_step(int mode, vxo_scene *s, const vxo_step_config *cfg, const double *targets,
-              const double *loads, vxo_step_report *report, double *last_force);
+              const double *loads, vxo_step_report *report, double *last_force, double *lambda);
 /* vxo_step plus, per substep after the corotation passes and before the
  * velocities, the fracture pass over the last assembly's constraints
  * (src/solver.rs:627-665). Bonds that break leave the topology. */
 void vxo_step_fracture(int mode, vxo_scene *s, const vxo_step_config *cfg, const double *targets,
-                       const double *loads, vxo_fracture_state *fracture, vxo_step_report *report);
+                       const double *loads, vxo_fracture_state *fracture, vxo_step_report *report,
+                       double *last_force, double *lambda);
 
 /* ---- contact projection (src/collision.rs:33-123), f32 ------------------
  * vxo_contact_project restates collision::project: positions and previous

@@ -58,8 +58,12 @@ def main():
                          "tol": 1e-6},
         "wood_limits": {"tension_along": 100e6, "tension_across": 2.5e6, "compression_along": 50e6,
                         "compression_across": 2e6, "shear_along": 12e6, "shear_across": 3e6},
+        # validation/mechanical-tests.txt:3 (and validation/STATUS.md:13, the latest checkpoint)
         "tension_fixture": {"parallel_N": 1022767.250, "perpendicular_N": 25920.164, "ratio": 39.458,
                             "first_breaks": [900, 900]},
+        # validation/STATUS.md:14, the latest checkpoint (validation/mechanical-tests.txt:4, ratio 32.539 and
+        # first breaks 3/4, is the earlier checkpoint of validation/STATUS.md:38)
+        "compression_fixture": {"ratio": 25.877},
     }
     out = os.path.join(os.path.dirname(os.path.abspath(__file__)), "reference_known_answers.json")
     with open(out, "w") as f:

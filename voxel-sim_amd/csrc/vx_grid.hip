@@ -129,8 +129,10 @@ struct DevArena {
     // rotations of the last assembly (4R), which fix each bond's frame
     float *strength, *hist;
     double *rot_asm;
-    // operator
+    // operator; after a substep's passes `force` holds each bond's last force
+    // and `lambda` its multiplier (bond_forces)
     double *k64, *force, *d64, *dinv64, *rhs;  // 18R, 9R, 6R, 6R, 3R
+    double *lambda;                             // 9R ([a][c])
     float *k32, *d32, *dinv32;                  // 18R, 6R, 6R
     // inner vectors
     float *d, *r, *z, *p0, *p1, *ap;
@@ -366,14 +368,12 @@ __global__ __launch_bounds__(kBlock) void assemble_bonds(DevArena A) {
     const d3 pi = ld3(A.pos, R, v);
     const uint8_t mi = A.mask[v];
     const Rev ri = ld_rev(A, v);
-    if (A.prm[c.slot].fracture) {
-        // The fracture pass evaluates every bond in the frame of the last
-        // assembly (src/solver.rs:627-665).
-        A.rot_asm[v] = qi.x;
-        A.rot_asm[R + v] = qi.y;
-        A.rot_asm[2 * R + v] = qi.z;
-        A.rot_asm[3 * R + v] = qi.w;
-    }
+    // The fracture pass and the bond forces evaluate every bond in the frame
+    // of the last assembly (src/solver.rs:627-665).
+    A.rot_asm[v] = qi.x;
+    A.rot_asm[R + v] = qi.y;
+    A.rot_asm[2 * R + v] = qi.z;
+    A.rot_asm[3 * R + v] = qi.w;
     for (int a = 0; a < 3; ++a) {
         double k[6] = {0, 0, 0, 0, 0, 0};
         d3 force{0, 0, 0};
@@ -1690,6 +1690,59 @@ __global__ __launch_bounds__(kBlock) void reduce_angle(DevArena A) {
 // last assembly; fibre alignment and the thermal rest length use the current
 // rotations. A bond whose damage reaches one is removed from the lattice.
 // Per tile: the dissipated energies and the count of broken bonds.
+// After a substep's corotation passes, per constraint of the last assembly
+// (every bond except those whose endpoints have no free axis and whose first
+// REV is kinematic), at the solved positions:
+//   lambda = -F^T k (p_j - p_i - rest) h^2 with the constraint's rotated
+//            stiffness k and rest (src/solver.rs:1824-1827, 1921-1927);
+//   last_force = F (K (u - plastic CELL)) (1 - damage) for a non-kinematic
+//            first REV, u the elastic displacement in the bond frame
+//            (src/solver.rs:627-640),
+// both from the bond state before the fracture pass updates it, into the
+// force and lambda planes (zero where there is no constraint).
+__global__ __launch_bounds__(kBlock) void bond_forces(DevArena A) {
+    Tile c;
+    if (!tile_for(A, B_FINISH, c) || !c.row) return;
+    const size_t R = A.R;
+    const uint32_t v = c.v;
+    const uint8_t bits = A.bonds[v];
+    const bool kin = A.kinematic[v];
+    const uint8_t mi = A.mask[v];
+    const double h = A.prm[c.slot].h;
+    const Rev ri = ld_rev(A, v);
+    const q4 ai = ldq(A.rot_asm, R, v), qi = ldq(A.rot, R, v);
+    const d3 pi = ld3(A.pos, R, v);
+    for (int a = 0; a < 3; ++a) {
+        d3 lam{0, 0, 0}, lf{0, 0, 0};
+        const uint32_t j = v + stride_of(c.g, a);
+        if ((bits >> a & 1) && !(kin && mi == 0 && A.mask[j] == 0)) {
+            const size_t b = size_t(a) * R + v;
+            const size_t o = size_t(a) * 3 * R;
+            const Rev rj = ld_rev(A, j);
+            const q4 aj = ldq(A.rot_asm, R, j), qj = ldq(A.rot, R, j);
+            const d3 off = mk(A.offset[o + v], A.offset[o + R + v], A.offset[o + 2 * R + v]);
+            const d3 pl = mk(A.plastic[o + v], A.plastic[o + R + v], A.plastic[o + 2 * R + v]);
+            const double damage = A.damage[b];
+            const m3 F = basis(a, slerp_half(ai, aj));
+            const d3 pj = ld3(A.pos, R, j);
+            double k[6];
+            d3 rest;
+            bond_constraint(a, ri, ai, rj, aj, double(A.kscale[b]) * fmax(1.0 - damage, 0.001), off, pl, k, rest);
+            lam = mul_t(F, sym_mul(k, pj - pi - rest)) * (-h * h);
+            if (!kin) {
+                double K[6];
+                local_stiffness(F, ri, ai, rj, aj, double(A.kscale[b]), K);
+                const d3 u = mul_t(F, pj - pi) - thermal_rest_local(F, ri, qi, rj, qj) - off;
+                lf = mul(F, sym_mul(K, u - pl * kCell)) * (1.0 - damage);
+            }
+        }
+        for (int q = 0; q < 3; ++q) {
+            A.force[(size_t(a) * 3 + q) * R + v] = q == 0 ? lf.x : (q == 1 ? lf.y : lf.z);
+            A.lambda[(size_t(a) * 3 + q) * R + v] = q == 0 ? lam.x : (q == 1 ? lam.y : lam.z);
+        }
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void fracture_bonds(DevArena A) {
     Tile c;
     if (!tile_for(A, B_FRACTURE, c)) return;
@@ -1799,7 +1852,7 @@ inline uint32_t tiles_for(uint64_t rows) { return uint32_t((rows + kBlock - 1) /
 
 // HBM bytes per arena row: the planes carved in create_batch.
 constexpr size_t kRowBytes = 24 * 5 + 32 + 24 * 3 + 24 + 4 * 9 + 12 + 12 + 4 + 12 + 12 + 36 + 36 + 144 + 72 + 48 + 48 + 24 +
-                             72 + 24 + 24 + 6 * 12 + 6 * 24 + 4 * kStrength + 12 * kHistory + 32;
+                             72 + 24 + 24 + 6 * 12 + 6 * 24 + 4 * kStrength + 12 * kHistory + 32 + 72;
 
 enum Phase : uint32_t { P_IDLE = 0, P_START, P_SOLVE, P_APPLIED, P_FINISHED, P_DONE, P_FAILED };
 
@@ -2126,6 +2179,7 @@ int32_t launch_tick(vx_batch *b, Flight &f) {
         hipLaunchKernelGGL(fit_rotations, grid, blk, 0, s, b->A, uint32_t(B_FITB), 1);
         hipLaunchKernelGGL(reduce_angle, sg, blk, 0, s, b->A);
     }
+    if (any & B_FINISH) hipLaunchKernelGGL(bond_forces, grid, blk, 0, s, b->A);
     f.frac_timed = (any & B_FRACTURE) != 0;
     if (f.frac_timed) {
         VX_HIP(hipEventRecord(f.frac0, s));
@@ -2316,7 +2370,7 @@ int32_t create_batch(uint64_t capacity_rows, uint32_t max_scenes, uint32_t chunk
         {(void **)&A.aniso, 4 * R}, {(void **)&A.dtemp, 4 * R}, {(void **)&A.a_par, 4 * R}, {(void **)&A.a_perp, 4 * R},
         {(void **)&A.loads, 12 * R}, {(void **)&A.kinematic, R}, {(void **)&A.bonds, R}, {(void **)&A.tmask, R},
         {(void **)&A.mask, R}, {(void **)&A.damage, 12 * R}, {(void **)&A.kscale, 12 * R}, {(void **)&A.offset, 36 * R},
-        {(void **)&A.plastic, 36 * R}, {(void **)&A.k64, 144 * R}, {(void **)&A.force, 72 * R}, {(void **)&A.d64, 48 * R},
+        {(void **)&A.plastic, 36 * R}, {(void **)&A.k64, 144 * R}, {(void **)&A.force, 72 * R}, {(void **)&A.lambda, 72 * R}, {(void **)&A.d64, 48 * R},
         {(void **)&A.dinv64, 48 * R}, {(void **)&A.rhs, 24 * R}, {(void **)&A.k32, 72 * R}, {(void **)&A.d32, 24 * R},
         {(void **)&A.dinv32, 24 * R}, {(void **)&A.d, 12 * R}, {(void **)&A.r, 12 * R}, {(void **)&A.z, 12 * R},
         {(void **)&A.p0, 12 * R}, {(void **)&A.p1, 12 * R}, {(void **)&A.ap, 12 * R}, {(void **)&A.dd, 24 * R},
@@ -2877,6 +2931,29 @@ extern "C" int32_t vx_batch_download_bonds(vx_batch *b, uint32_t slot, uint32_t 
     return download_bonds(b, slot, neighbor, damage, plastic_strain, history);
 }
 
+// Each bond's last force and multiplier of the slot's last substep, (3, n, 3)
+// [axis][voxel][component], f64.
+static int32_t download_bond_forces(vx_batch *b, uint32_t slot, double *last_force, double *lambda) {
+    if (int32_t c = drain(b)) return c;
+    const SlotGeo &g = b->slots[slot].geo;
+    const DevArena &A = b->A;
+    const size_t R = A.R, n = g.n;
+    for (int a = 0; a < 3; ++a) {
+        if (last_force)
+            if (int32_t c = download_planes(A.force + size_t(a) * 3 * R, R, g.voff, n, 3, last_force + size_t(a) * n * 3)) return c;
+        if (lambda)
+            if (int32_t c = download_planes(A.lambda + size_t(a) * 3 * R, R, g.voff, n, 3, lambda + size_t(a) * n * 3)) return c;
+    }
+    return VX_OK;
+}
+
+extern "C" int32_t vx_batch_download_bond_forces(vx_batch *b, uint32_t slot, double *last_force, double *lambda) {
+    clear_error();
+    if (!b || slot >= b->max_slots || !b->slots[slot].used) return fail(VX_ERR_INVALID_ARGUMENT, "invalid slot");
+    Lock lock;
+    return download_bond_forces(b, slot, last_force, lambda);
+}
+
 extern "C" int32_t vx_batch_release(vx_batch *b, uint32_t slot) {
     clear_error();
     if (!b || slot >= b->max_slots) return fail(VX_ERR_INVALID_ARGUMENT, "invalid slot");
@@ -3072,6 +3149,13 @@ extern "C" int32_t vx_scene_download(vx_scene *sc, double *positions, double *ro
         if (int32_t c = download_planes(x, b->A.R, g.voff, g.n, 3, correction)) return c;
     }
     return VX_OK;
+}
+
+extern "C" int32_t vx_scene_download_bond_forces(vx_scene *sc, double *last_force, double *lambda) {
+    clear_error();
+    if (int32_t c = scene_ok(sc)) return c;
+    Lock lock;
+    return download_bond_forces(sc->batch, sc->slot, last_force, lambda);
 }
 
 extern "C" int32_t vx_scene_download_bonds(vx_scene *sc, uint32_t *neighbor, float *damage, float *plastic_strain, float *history) {

@@ -176,6 +176,8 @@ SIGNATURES = [
     ("vx_scene_download_bonds", ctypes.c_int32, [ctypes.c_void_p, c_u32p, c_fp, c_fp, c_fp]),
     ("vx_scene_set_targets_and_loads", ctypes.c_int32, [ctypes.c_void_p, c_fp, c_fp]),
     ("vx_batch_download_bonds", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint32, c_u32p, c_fp, c_fp, c_fp]),
+    ("vx_scene_download_bond_forces", ctypes.c_int32, [ctypes.c_void_p, c_dp, c_dp]),
+    ("vx_batch_download_bond_forces", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint32, c_dp, c_dp]),
     ("vx_contact_project", ctypes.c_int32,
      [ctypes.c_uint32, c_fp, c_fp, c_fp, c_fp, c_u8p, c_u8p, c_u32p, ctypes.POINTER(Contact), ctypes.c_uint64,
       ctypes.POINTER(ContactReport)]),

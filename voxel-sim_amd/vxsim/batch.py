@@ -16,7 +16,7 @@ import numpy as np
 from . import lib
 from ._abi import (BatchSceneReport, VX_ERR_TOO_LARGE, VX_SCENE_DONE, VX_SCENE_FAILED, VX_SCENE_FREE,
                    VX_SCENE_RUNNING, c_dp, c_u32p)
-from .grid import _report, download_bonds, scene_desc, step_params
+from .grid import _report, download_bond_forces, download_bonds, scene_desc, step_params
 from .solver import SolverError, check, last_error
 
 FREE, RUNNING, DONE, FAILED = VX_SCENE_FREE, VX_SCENE_RUNNING, VX_SCENE_DONE, VX_SCENE_FAILED
@@ -132,6 +132,10 @@ class Batch:
 
     def download_bonds(self, slot: int) -> dict:
         return download_bonds(lib().vx_batch_download_bonds, self.handle, int(slot), n=self._rows[int(slot)])
+
+    def download_bond_forces(self, slot: int) -> dict:
+        return download_bond_forces(lib().vx_batch_download_bond_forces, self.handle, int(slot),
+                                    n=self._rows[int(slot)])
 
     def release(self, slot: int):
         check(lib().vx_batch_release(self.handle, int(slot)))

@@ -87,6 +87,15 @@ def step_params(fixed_dt=0.001, gravity=(0.0, 0.0, -9.81), substeps=1, corotatio
                       fracture_params(fracture))
 
 
+def download_bond_forces(fn, *args, n: int) -> dict:
+    """Bond forces of the last substep through vx_scene_download_bond_forces /
+    vx_batch_download_bond_forces: last_force and lambda, (3, n, 3) f64."""
+    lf = np.empty((3, n, 3))
+    lam = np.empty((3, n, 3))
+    check(fn(*args, lf.ctypes.data_as(c_dp), lam.ctypes.data_as(c_dp)))
+    return dict(last_force=lf, **{"lambda": lam})
+
+
 def download_bonds(fn, *args, n: int) -> dict:
     """Bond state through vx_scene_download_bonds / vx_batch_download_bonds:
     neighbor (3, n), damage (3, n), plastic_strain (3, n, 3), history (3, n, 6)."""
@@ -196,6 +205,9 @@ class GridScene:
 
     def download_bonds(self) -> dict:
         return download_bonds(lib().vx_scene_download_bonds, self.handle, n=self.n)
+
+    def download_bond_forces(self) -> dict:
+        return download_bond_forces(lib().vx_scene_download_bond_forces, self.handle, n=self.n)
 
     def export_rows(self) -> dict:
         na = self.active
