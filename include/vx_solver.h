@@ -360,6 +360,41 @@ int32_t vx_scene_download_bonds(vx_scene *scene, uint32_t *neighbor, float *dama
  * vx_scene_finish_substep (a later vx_scene_assemble overwrites them). */
 int32_t vx_scene_download_bond_forces(vx_scene *scene, double *last_force, double *lambda);
 
+/* ---- rigid tool (src/tool.rs:26-431) ------------------------------------
+ * A rigid voxel group (the reference's Tool) driven into a scene: once set,
+ * every vx_scene_step advances it per substep after the prediction, as the
+ * reference's step does (src/solver.rs:530-543): microsteps of at most 0.4
+ * REV of travel, per tool cell the scene REVs of the spatial bins its
+ * oriented box touches, a separating-axis contact test, inelastic normal
+ * impulses and Coulomb friction applied in the reference's order, then the
+ * removal of the deepest overlap and up to six re-projections
+ * (Tool::advance, src/tool.rs:192-431). The contact velocity drifts the REVs
+ * by half a substep and enters their final velocities. One tool per
+ * vx_scene; the neighbourhood it can touch in one substep is limited to
+ * 65,536 REVs and each cell to 32 contacts (VX_ERR_TOO_LARGE beyond). */
+typedef struct vx_tool_desc {
+    uint32_t n_cells;
+    uint32_t has_mount;            /* 1: guide mount (Tool::mount_base, src/tool.rs:127-142) */
+    const float *cells;            /* n_cells*3, body frame about the centre of mass (m) */
+    double mass;                   /* kg */
+    double inverse_inertia[3];     /* body-frame principal inverse inertia */
+    double position[3];            /* centre of mass, world */
+    double rotation[4];            /* x, y, z, w */
+    double velocity[3], angular_velocity[3];
+    double external_force[3], external_torque[3];  /* persistent, world frame */
+    double friction;               /* Coulomb coefficient (SceneConfig fracture.friction) */
+    double mount_local_base[3], mount_world_base[3], mount_direction[3], mount_rotation[4];
+} vx_tool_desc;
+typedef struct vx_tool_state {
+    double position[3], rotation[4], velocity[3], angular_velocity[3];
+    double friction_heat;          /* J dissipated by tool friction since the tool was set */
+    uint32_t microsteps, contacts, candidates, status;  /* of the last advance */
+} vx_tool_state;
+/* Set (or with NULL remove) the scene's tool. */
+int32_t vx_scene_set_tool(vx_scene *scene, const vx_tool_desc *tool);
+/* The tool's pose and velocities after the last step. */
+int32_t vx_scene_tool_state(vx_scene *scene, vx_tool_state *state);
+
 /* Inner PCG iterations per scheduling tick (one hipGraph replay; default 16,
  * even, at most 256). 0 keeps the current value. */
 int32_t vx_scene_set_schedule(vx_scene *scene, uint32_t iterations_per_tick);

@@ -76,6 +76,17 @@ class _FractureState(ctypes.Structure):
                 ("broken", ctypes.c_uint32)]
 
 
+class _Tool(ctypes.Structure):
+    _fields_ = [("n_cells", ctypes.c_uint32), ("has_mount", ctypes.c_uint32), ("cells", _dp), ("mass", ctypes.c_double),
+                ("inverse_inertia", ctypes.c_double * 3), ("position", ctypes.c_double * 3),
+                ("rotation", ctypes.c_double * 4), ("velocity", ctypes.c_double * 3),
+                ("angular_velocity", ctypes.c_double * 3), ("external_force", ctypes.c_double * 3),
+                ("external_torque", ctypes.c_double * 3), ("friction", ctypes.c_double),
+                ("mount_local_base", ctypes.c_double * 3), ("mount_world_base", ctypes.c_double * 3),
+                ("mount_direction", ctypes.c_double * 3), ("mount_rotation", ctypes.c_double * 4),
+                ("friction_heat", ctypes.c_double), ("microsteps", ctypes.c_uint32), ("contacts", ctypes.c_uint32)]
+
+
 HISTORY_FIELDS = ("equivalent_plastic_strain", "plastic_work", "fatigue_damage", "previous_stress_ratio",
                   "reversal_stress_ratio", "loading_increment")
 BOND_FIELDS = ("damage",) + HISTORY_FIELDS
@@ -123,6 +134,10 @@ def lib():
         L.vxo_fatigue.argtypes = [ctypes.c_int, ctypes.POINTER(_Bond), ctypes.c_double, ctypes.POINTER(_FractureConfig)]
         L.vxo_fracture_update.argtypes = [ctypes.c_int, ctypes.POINTER(_Bond), _dp, _dp, _dp, ctypes.c_double,
                                           ctypes.POINTER(_FractureConfig), ctypes.POINTER(_Dissipation)]
+        L.vxo_tool_advance.argtypes = [ctypes.c_int, ctypes.POINTER(_Tool), ctypes.c_uint32, _dp, _dp, _dp, _u8p, _dp,
+                                       ctypes.c_double, _dp, _dp]
+        L.vxo_step_tool.argtypes = [ctypes.c_int, ctypes.POINTER(_Scene), ctypes.POINTER(_StepConfig), _dp, _dp,
+                                    ctypes.POINTER(_Tool), ctypes.c_void_p, ctypes.POINTER(_StepReport)]
         L.vxo_step_fracture.argtypes = [ctypes.c_int, ctypes.POINTER(_Scene), ctypes.POINTER(_StepConfig), _dp, _dp,
                                         ctypes.POINTER(_FractureState), ctypes.POINTER(_StepReport), _dp, _dp]
         L.vxo_dot_rows.restype = ctypes.c_double
@@ -522,3 +537,79 @@ def local_continuum_pair(rest_positions, bond_neighbor, i, j):
     bonds = np.ascontiguousarray(bond_neighbor, dtype=np.uint32).reshape(-1)
     return bool(L.vxo_local_continuum_pair(rest.size // 3, rest.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
                                            _p(bonds, _u32p), i, j))
+
+
+# ---- rigid tool (src/tool.rs:192-431) -------------------------------------------
+
+class OracleTool:
+    """The oracle's copy of a tool's state (vxsim.tool.Tool fields)."""
+
+    def __init__(self, tool):
+        self.cells = np.ascontiguousarray(tool.cells, dtype=np.float64)
+        t = _Tool()
+        t.n_cells = len(self.cells)
+        t.has_mount = 1 if tool.mount is not None else 0
+        t.cells = _p(self.cells, _dp)
+        t.mass = float(tool.mass)
+        for name in ("inverse_inertia", "position", "rotation", "velocity", "angular_velocity", "external_force",
+                     "external_torque"):
+            arr = getattr(t, name)
+            for k, x in enumerate(getattr(tool, name)):
+                arr[k] = float(x)
+        t.friction = float(tool.friction)
+        if tool.mount is not None:
+            lb, wb, d, rot = tool.mount
+            for k in range(3):
+                t.mount_local_base[k], t.mount_world_base[k], t.mount_direction[k] = lb[k], wb[k], d[k]
+            for k in range(4):
+                t.mount_rotation[k] = rot[k]
+        self.struct = t
+
+    def state(self) -> dict:
+        t = self.struct
+        return dict(position=np.array(t.position[:]), rotation=np.array(t.rotation[:]),
+                    velocity=np.array(t.velocity[:]), angular_velocity=np.array(t.angular_velocity[:]),
+                    friction_heat=t.friction_heat, microsteps=t.microsteps, contacts=t.contacts)
+
+
+def tool_advance(tool: OracleTool, positions, velocities, mass, kinematic, targets, dt, gravity, mode=F64):
+    """Tool::advance over the given scene state; velocities (n, 3) float64 are
+    updated in place; returns the contact velocity (n, 3)."""
+    pos = np.ascontiguousarray(positions, dtype=np.float64)
+    n = pos.shape[0]
+    assert velocities.dtype == np.float64 and velocities.flags.c_contiguous
+    m = np.ascontiguousarray(mass, dtype=np.float64)
+    kin = np.ascontiguousarray(kinematic, dtype=np.uint8)
+    tg = None if targets is None else _f64(targets).ravel()
+    g = np.ascontiguousarray(gravity, dtype=np.float64)
+    cv = np.zeros((n, 3))
+    lib().vxo_tool_advance(mode, ctypes.byref(tool.struct), n, _p(pos, _dp), _p(velocities, _dp), _p(m, _dp),
+                           _p(kin, _u8p), _p(tg, _dp), float(dt), _p(g, _dp), _p(cv, _dp))
+    return cv
+
+
+def step_tool(oscene: OracleScene, cfg, tool: OracleTool, mode=F64, precond=BJ, exact=False, exact_tolerance=1e-12,
+              targets=None, loads=None, fracture=None):
+    """One fixed step (solve part) with the tool advancing each substep after
+    the prediction; `fracture` is an optional (fcfg, history, strength)."""
+    c = _StepConfig(cfg.fixed_dt, cfg.substeps, cfg.iterations, cfg.corotation_iterations,
+                    cfg.solver_relative_tolerance, cfg.corotation_tolerance,
+                    (ctypes.c_double * 3)(*cfg.gravity), precond, 1 if exact else 0, exact_tolerance)
+    rep = _StepReport()
+    tg = None if targets is None else _f64(targets).ravel()
+    ld = None if loads is None else _f64(loads).ravel()
+    fs = None
+    keep = None
+    if fracture is not None:
+        fcfg, history, strength = fracture
+        keep = np.ascontiguousarray(strength, dtype=np.float64)
+        fs = _FractureState(_p(oscene.neighbor, _u32p), _p(oscene.damage, _dp), _p(oscene.plastic_strain, _dp),
+                            *[_p(getattr(history, k), _dp) for k in HISTORY_FIELDS], _p(keep, _dp), _fcfg(fcfg),
+                            _Dissipation(), 0)
+    lib().vxo_step_tool(mode, ctypes.byref(oscene.struct), ctypes.byref(c), _p(tg, _dp), _p(ld, _dp),
+                        ctypes.byref(tool.struct), ctypes.byref(fs) if fs is not None else None, ctypes.byref(rep))
+    out = dict(pcg_iterations=rep.pcg_iterations, passes=rep.passes, solver_residual=rep.solver_residual,
+               last_angle=rep.last_angle)
+    if fs is not None:
+        out["broken"] = fs.broken
+    return out

@@ -12,6 +12,7 @@
 #include <cstdint>
 #include <cstring>
 #include <limits>
+#include <type_traits>
 #include <vector>
 
 #ifdef _OPENMP
@@ -1606,7 +1607,8 @@ static void bond_forces_t(const System<T> &sys, vxo_scene *s, T h, double *last_
 // assemble + PCG + apply, then velocities.
 template <class T>
 static void step_t(vxo_scene *s, const vxo_step_config *cfg, const double *targets, const double *loads,
-                   vxo_step_report *report, double *last_force, double *lambda, vxo_fracture_state *fracture = nullptr) {
+                   vxo_step_report *report, double *last_force, double *lambda, vxo_fracture_state *fracture = nullptr,
+                   vxo_tool *tool = nullptr) {
     const uint32_t n = s->n;
     const T h = T(cfg->fixed_dt) / T(cfg->substeps);
     const V3<T> g{T(cfg->gravity[0]), T(cfg->gravity[1]), T(cfg->gravity[2])};
@@ -1644,6 +1646,17 @@ static void step_t(vxo_scene *s, const vxo_step_config *cfg, const double *targe
         for (uint32_t i = 0; i < n; ++i)
             for (int a = 0; a < 3; ++a)
                 if (has_target(i, a)) s->positions[3 * i + a] = double(T(targets[3 * i + a]));
+        // The tool's contact impulses, acting near the middle of the substep
+        // (src/solver.rs:530-543).
+        std::vector<double> contact(3 * size_t(n), 0.0);
+        if (tool) {
+            const double gd[3] = {cfg->gravity[0], cfg->gravity[1], cfg->gravity[2]};
+            std::vector<double> mass(s->mass, s->mass + n);
+            vxo_tool_advance(std::is_same<T, float>::value ? VXO_F32 : VXO_F64, tool, n, s->positions, s->velocities,
+                             mass.data(), s->kinematic, targets, double(h), gd, contact.data());
+            for (size_t k = 0; k < 3 * size_t(n); ++k)
+                s->positions[k] = double(T(s->positions[k]) + T(contact[k]) * (T(0.5) * h));
+        }
         std::vector<double> predicted(s->positions, s->positions + 3 * size_t(n));
         fit_rotations<T>(s);
         for (uint32_t pass = 0; pass < cfg->corotation_iterations; ++pass) {
@@ -1686,7 +1699,7 @@ static void step_t(vxo_scene *s, const vxo_step_config *cfg, const double *targe
                     s->velocities[3 * i + a] = double((pa - T(previous[3 * i + a])) / h);
                 } else {
                     T cv = (pa - T(predicted[3 * i + a])) / h;
-                    s->velocities[3 * i + a] = double(drift[i][a] + cv);
+                    s->velocities[3 * i + a] = double(drift[i][a] + cv + T(contact[3 * i + a]));
                 }
             }
         }
@@ -1700,6 +1713,14 @@ void vxo_step(int mode, vxo_scene *s, const vxo_step_config *cfg, const double *
         step_t<float>(s, cfg, targets, loads, report, last_force, lambda);
     else
         step_t<double>(s, cfg, targets, loads, report, last_force, lambda);
+}
+
+void vxo_step_tool(int mode, vxo_scene *s, const vxo_step_config *cfg, const double *targets, const double *loads,
+                   vxo_tool *tool, vxo_fracture_state *fracture, vxo_step_report *report) {
+    if (mode == VXO_F32)
+        step_t<float>(s, cfg, targets, loads, report, nullptr, nullptr, fracture, tool);
+    else
+        step_t<double>(s, cfg, targets, loads, report, nullptr, nullptr, fracture, tool);
 }
 
 void vxo_step_fracture(int mode, vxo_scene *s, const vxo_step_config *cfg, const double *targets, const double *loads,

@@ -271,6 +271,45 @@ void vxo_step_fracture(int mode, vxo_scene *s, const vxo_step_config *cfg, const
                        const double *loads, vxo_fracture_state *fracture, vxo_step_report *report,
                        double *last_force, double *lambda);
 
+/* ---- rigid tool (src/tool.rs:192-431) -------------------------------------
+ * The tool's state: cells (n_cells*3, body frame about the centre of mass),
+ * mass, principal inverse inertia (body frame), pose (rotation x, y, z, w),
+ * velocities, persistent external force and torque, the Coulomb friction
+ * coefficient and an optional guide mount (src/tool.rs:17-24). advance updates
+ * the pose and velocities in place, adds the friction heat and records the
+ * microsteps and contacts of the call. */
+typedef struct vxo_tool {
+    uint32_t n_cells;
+    uint32_t has_mount;
+    const double *cells;
+    double mass;
+    double inverse_inertia[3];
+    double position[3];
+    double rotation[4];
+    double velocity[3];
+    double angular_velocity[3];
+    double external_force[3];
+    double external_torque[3];
+    double friction;
+    double mount_local_base[3], mount_world_base[3], mount_direction[3], mount_rotation[4];
+    double friction_heat;
+    uint32_t microsteps, contacts;
+} vxo_tool;
+/* Tool::advance (src/tool.rs:192-431) over a scene of n voxels: positions
+ * (n*3) are read; velocities (n*3) receive the contact impulses; targets
+ * (n*3, NaN = free axis, may be NULL) and kinematic flags block axes;
+ * contact_velocity (n*3) is the returned per-voxel velocity change. */
+void vxo_tool_advance(int mode, vxo_tool *tool, uint32_t n, const double *positions, double *velocities,
+                      const double *mass, const uint8_t *kinematic, const double *targets, double dt,
+                      const double *gravity, double *contact_velocity);
+/* vxo_step_fracture with a tool (may be NULL; fracture may be NULL): per
+ * substep after the prediction, the tool advances (src/solver.rs:530-534),
+ * positions drift by half a substep of the contact velocity
+ * (src/solver.rs:535-543), and the final velocities of free axes include it
+ * (src/solver.rs:668-683). */
+void vxo_step_tool(int mode, vxo_scene *s, const vxo_step_config *cfg, const double *targets, const double *loads,
+                   vxo_tool *tool, vxo_fracture_state *fracture, vxo_step_report *report);
+
 /* ---- contact projection (src/collision.rs:33-123), f32 ------------------
  * vxo_contact_project restates collision::project: positions and previous
  * (n*3 f32) are updated in place; rest (n*3), mass (n), kinematic (n),

@@ -24,6 +24,8 @@ STRUCTS = {
     "vx_contact_report": _abi.ContactReport,
     "vx_kernel_stats": _abi.KernelStats,
     "vx_kernel_times": _abi.KernelTimes,
+    "vx_tool_desc": _abi.ToolDesc,
+    "vx_tool_state": _abi.ToolState,
 }
 
 

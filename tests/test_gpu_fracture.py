@@ -128,16 +128,13 @@ def exempt(os_, hist, bonds):
     return len(diff)
 
 
-@pytest.mark.parametrize("precond", list(PRECONDITIONERS))
-def test_blade_cut_matches_oracle(gpu, O, scenes, precond):
-    """The cutting block at reduced size (24x24x16 wood, grain Z, blade edge
-    8x3 REVs driven 5 mm per 1 ms step): per-step axis targets through
-    vx_scene_set_targets_and_loads, six corotation passes, fracture every
-    step. Broken-bond set identical up to the 1e-5 exemption; energies,
-    damage and positions to the solve's accuracy."""
+def blade_cut_parity(O, scenes, dims, blade, steps, precond, min_broken):
+    """The cutting block with per-step axis targets through
+    vx_scene_set_targets_and_loads, six corotation passes and fracture every
+    step, against the FP64 oracle: the broken-bond set identical up to the
+    1e-5 exemption; energies, damage and positions to the solve's accuracy."""
     from vxsim import grid
-    blade, steps = (8, 3, 5), 2
-    s = scenes.wood_block_blade(dims=(24, 24, 16), blade=blade)
+    s = scenes.wood_block_blade(dims=dims, blade=blade)
     cfg = s.config
     sp = grid.solve_params(tolerance=1e-11, inner_tolerance=1e-3, max_inner=40000,
                            preconditioner=PRECONDITIONERS[precond])
@@ -159,7 +156,7 @@ def test_blade_cut_matches_oracle(gpu, O, scenes, precond):
                                       loads=s.loads))
         for arr in [os_.damage, os_.plastic_strain] + [getattr(hist, f) for f in HISTORY_FIELDS]:
             arr[...] = arr.astype(np.float32)
-    assert sum(r["broken"] for r in reps_o) > 1000
+    assert sum(r["broken"] for r in reps_o) > min_broken
     differing = exempt(os_, hist, bonds)
     scale = max(sum(r["dissipated"].values()) for r in reps_o)
     for rd, ro in zip(reps_d, reps_o):
@@ -172,6 +169,21 @@ def test_blade_cut_matches_oracle(gpu, O, scenes, precond):
     start = s.positions.astype(np.float64)
     disp_o = os_.positions - start
     assert np.linalg.norm(out["positions"] - start - disp_o) <= 1e-6 * np.linalg.norm(disp_o)
+    return [r["broken"] for r in reps_o], differing
+
+
+@pytest.mark.parametrize("precond", list(PRECONDITIONERS))
+def test_blade_cut_matches_oracle(gpu, O, scenes, precond):
+    """The cutting block at reduced size: 24x24x16 wood, grain Z, blade edge
+    8x3 REVs driven 5 mm per 1 ms step, two steps."""
+    blade_cut_parity(O, scenes, (24, 24, 16), (8, 3, 5), 2, precond, 1000)
+
+
+def test_blade_cut_64x64x32_matches_oracle(gpu, O, scenes):
+    """The cut at a larger size with the full 40x5 blade edge: 64x64x32 wood
+    (131,072 REVs), three steps (about 36,000 bonds broken)."""
+    broken, differing = blade_cut_parity(O, scenes, (64, 64, 32), (40, 5, 20), 3, "block_jacobi", 30000)
+    print("64x64x32 cut: bonds broken per step %s, exempt differences %d" % (broken, differing))
 
 
 def test_targets_and_loads_update(gpu, scenes):

@@ -68,6 +68,7 @@ enum : uint32_t {
     B_F64 = 1u << 11,     // inner solve in FP64 (plain FP64 mode)
     B_FRACTURE = 1u << 12,  // fracture pass of the substep, before the velocities
     B_RB = 1u << 13,        // red-black reduced system (red-black block SGS) instead of block-Jacobi
+    B_TOOL = 1u << 14,      // the scene's rigid tool advances after the prediction
 };
 
 // Partial kinds; kFrac..kFrac+5 are the fracture pass's dissipated energies
@@ -1802,6 +1803,342 @@ __global__ __launch_bounds__(kBlock) void fracture_bonds(DevArena A) {
     tile_store<kFracN>(acc, A, c.t, kinds, lds);
 }
 
+// ---- rigid tool (one-scene arenas) -------------------------------------------
+#include "vx_tool.hpp"
+
+__device__ __forceinline__ t3 arena_pos(const DevArena &A, uint32_t v) {
+    const size_t R = A.R;
+    return t3{A.pos[v], A.pos[R + v], A.pos[2 * R + v]};
+}
+
+// The tool's candidate REVs, per tile: how many lie in its body box.
+__global__ __launch_bounds__(kBlock) void tool_filter_count(DevArena A, ToolBufs B) {
+    Tile c;
+    if (!tile_for(A, B_TOOL, c)) return;
+    __shared__ uint32_t wcnt[kWaves];
+    const bool in = c.row && tool_in_box(*B.dev, arena_pos(A, c.v));
+    const uint64_t m = __ballot(in);
+    if ((threadIdx.x & 63u) == 0) wcnt[threadIdx.x >> 6] = uint32_t(__popcll(m));
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (uint32_t w = 0; w < kWaves; ++w) t += wcnt[w];
+        B.tile_count[c.t - c.g.tile0] = t;
+    }
+}
+
+// Exclusive scan of the tiles' counts (one workgroup); the total is the
+// candidate count.
+__global__ __launch_bounds__(kToolBlock) void tool_filter_scan(ToolBufs B, uint32_t ntiles) {
+    __shared__ uint32_t buf[kToolBlock];
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < ntiles; base += kToolBlock) {
+        const uint32_t i = base + threadIdx.x;
+        const uint32_t v = i < ntiles ? B.tile_count[i] : 0u;
+        buf[threadIdx.x] = v;
+        __syncthreads();
+        for (uint32_t d = 1; d < kToolBlock; d <<= 1) {
+            const uint32_t add = threadIdx.x >= d ? buf[threadIdx.x - d] : 0u;
+            __syncthreads();
+            buf[threadIdx.x] += add;
+            __syncthreads();
+        }
+        if (i < ntiles) B.tile_off[i] = carry + buf[threadIdx.x] - v;
+        const uint32_t total = buf[kToolBlock - 1];
+        __syncthreads();
+        carry += total;
+    }
+    if (threadIdx.x == 0) {
+        B.dev->candidates = carry;
+        if (carry > kToolMaxCand) B.dev->status |= TS_CAND;
+    }
+}
+
+// The candidates in index order: arena row and bin key.
+__global__ __launch_bounds__(kBlock) void tool_filter_write(DevArena A, ToolBufs B) {
+    Tile c;
+    if (!tile_for(A, B_TOOL, c)) return;
+    if (B.dev->candidates > kToolMaxCand) return;
+    __shared__ uint32_t wcnt[kWaves];
+    t3 p{0, 0, 0};
+    bool in = false;
+    if (c.row) {
+        p = arena_pos(A, c.v);
+        in = tool_in_box(*B.dev, p);
+    }
+    const uint64_t m = __ballot(in);
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    if (lane == 0) wcnt[w] = uint32_t(__popcll(m));
+    __syncthreads();
+    if (!in) return;
+    uint32_t at = B.tile_off[c.t - c.g.tile0] + uint32_t(__popcll(m & ((uint64_t(1) << lane) - 1)));
+    for (uint32_t k = 0; k < w; ++k) at += wcnt[k];
+    int32_t k3[3];
+    uint64_t key = ~uint64_t(0);
+    if (tool_key(p, k3))
+        key = tool_pack(k3[0], k3[1], k3[2]);
+    else
+        atomicOr(&B.dev->status, uint32_t(TS_KEY));
+    B.key[at] = key;
+    B.idx[at] = c.v;
+}
+
+// One cell's bins in the reference's order (z, y, x), each bin's REVs in
+// index order: fn(sorted candidate e, arena row v) for every one.
+template <class F>
+__device__ __forceinline__ void tool_cell_pairs(const ToolBufs &B, uint32_t C, t3 center, const t3 (&axes)[3], F &&fn) {
+    int32_t lo[3], hi[3];
+    if (!tool_bounds(center, axes, lo, hi)) return;
+    for (int32_t z = lo[2]; z <= hi[2]; ++z)
+        for (int32_t y = lo[1]; y <= hi[1]; ++y)
+            for (int32_t x = lo[0]; x <= hi[0]; ++x) {
+                const uint64_t key = tool_pack(x, y, z);
+                uint32_t a = 0, b = C;  // first sorted candidate with this key
+                while (a < b) {
+                    const uint32_t mid = (a + b) >> 1;
+                    if (B.key2[mid] < key)
+                        a = mid + 1;
+                    else
+                        b = mid;
+                }
+                for (uint32_t e = a; e < C && B.key2[e] == key; ++e) fn(e, B.idx2[e]);
+            }
+}
+
+// Tool::advance for one substep (one workgroup; src/tool.rs:192-431); dt is
+// the substep length.
+__global__ __launch_bounds__(kToolBlock) void tool_advance(DevArena A, ToolBufs B, double dt) {
+#pragma clang fp contract(off)
+    __shared__ ToolDev T;
+    __shared__ double red_pen[kToolBlock / 64];
+    __shared__ uint32_t red_cell[kToolBlock / 64];
+    __shared__ uint32_t done, lstatus;
+    if (threadIdx.x == 0) {
+        T = *B.dev;
+        lstatus = 0;
+    }
+    __syncthreads();
+    if (T.status) return;
+    const size_t R = A.R;
+    const uint32_t C = T.candidates, nc = T.n_cells;
+    // Sort the candidates by (bin key, row): bitonic over the next power of two.
+    uint32_t P = 1;
+    while (P < C) P <<= 1;
+    for (uint32_t i = threadIdx.x; i < P; i += kToolBlock) {
+        B.key2[i] = i < C ? B.key[i] : ~uint64_t(0);
+        B.idx2[i] = i < C ? B.idx[i] : ~0u;
+    }
+    for (uint32_t i = threadIdx.x; i < 3 * C; i += kToolBlock) B.cv[i] = 0.0;
+    __syncthreads();
+    for (uint32_t k = 2; k <= P; k <<= 1)
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = threadIdx.x; i < P; i += kToolBlock) {
+                const uint32_t l = i ^ j;
+                if (l <= i) continue;
+                const uint64_t ka = B.key2[i], kb = B.key2[l];
+                const uint32_t ia = B.idx2[i], ib = B.idx2[l];
+                const bool greater = ka > kb || (ka == kb && ia > ib);
+                if (greater == ((i & k) == 0)) {
+                    B.key2[i] = kb;
+                    B.key2[l] = ka;
+                    B.idx2[i] = ib;
+                    B.idx2[l] = ia;
+                }
+            }
+            __syncthreads();
+        }
+    uint32_t contacts = 0;
+    for (uint32_t step = 0; step < T.steps; ++step) {
+        if (threadIdx.x == 0) {
+            const double h = T.h;
+            tst(T.vel, tld(T.vel) + tld(T.lin_acc) * h);
+            tst(T.ang, tld(T.ang) + tool_angular(T, tld(T.torque)) * h);
+            tst(T.pos, tld(T.pos) + tld(T.vel) * h);
+            const double angle = tlen(tld(T.ang)) * h;
+            if (angle > 1e-8) {
+                const t3 ax = tdiv(tld(T.ang), tlen(tld(T.ang)));
+                const double sn = sin(angle * 0.5), cs = cos(angle * 0.5);
+                const tq r{ax.x * sn, ax.y * sn, ax.z * sn, cs};
+                const tq q = tqld(T.rot);
+                // Hamilton product r * q, then normalise
+                tq m{r.w * q.x + r.x * q.w + r.y * q.z - r.z * q.y, r.w * q.y - r.x * q.z + r.y * q.w + r.z * q.x,
+                     r.w * q.z + r.x * q.y - r.y * q.x + r.z * q.w, r.w * q.w - r.x * q.x - r.y * q.y - r.z * q.z};
+                const double len = sqrt(((m.x * m.x + m.y * m.y) + m.z * m.z) + m.w * m.w);
+                const double inv = 1.0 / len;
+                tqst(T.rot, tq{m.x * inv, m.y * inv, m.z * inv, m.w * inv});
+            }
+        }
+        __syncthreads();
+        const tq rot = tqld(T.rot);
+        const t3 axes[3] = {trot(rot, tv(1, 0, 0)), trot(rot, tv(0, 1, 0)), trot(rot, tv(0, 0, 1))};
+        // Contacts of every cell, in order, in parallel.
+        for (uint32_t c = threadIdx.x; c < nc; c += kToolBlock) {
+            const t3 center = tld(T.pos) + trot(rot, tld(B.cells + 3 * c));
+            uint32_t n = 0;
+            tool_cell_pairs(B, C, center, axes, [&](uint32_t e, uint32_t v) {
+                t3 nrm;
+                double pen;
+                if (!tool_sat(center, axes, arena_pos(A, v), nrm, pen)) return;
+                if (n < kToolCellSlots) {
+                    ToolContact &tc = B.con[size_t(c) * kToolCellSlots + n];
+                    tc.slot = e;
+                    tst(tc.n, nrm);
+                    tc.pen = pen;
+                } else {
+                    atomicOr(&lstatus, uint32_t(TS_SLOTS));
+                }
+                ++n;
+            });
+            B.ncon[c] = n < kToolCellSlots ? n : kToolCellSlots;
+        }
+        __syncthreads();
+        // The impulses, sequentially in the reference's order.
+        if (threadIdx.x == 0) {
+            t3 correction{0, 0, 0};
+            double max_pen = 0.0;
+            for (uint32_t c = 0; c < nc; ++c) {
+                const uint32_t nn = B.ncon[c];
+                for (uint32_t q = 0; q < nn; ++q) {
+                    const ToolContact tc = B.con[size_t(c) * kToolCellSlots + q];
+                    const uint32_t e = tc.slot, v = B.idx2[e];
+                    const t3 nrm = tld(tc.n);
+                    contacts += 1;
+                    const t3 p = arena_pos(A, v);
+                    const t3 arm = p - tld(T.pos);
+                    t3 vi{A.vel[v], A.vel[R + v], A.vel[2 * R + v]};
+                    const t3 rel = tld(T.vel) + tcross(tld(T.ang), arm) - vi;
+                    const double closing = tdot(rel, nrm);
+                    const bool kin = A.kinematic[v] != 0;
+                    const uint8_t tm = A.tmask[v];
+                    const t3 fr{(!kin && !(tm & 1)) ? 1.0 : 0.0, (!kin && !(tm & 2)) ? 1.0 : 0.0, (!kin && !(tm & 4)) ? 1.0 : 0.0};
+                    const double mi = double(A.mass[v]);
+                    const double wi = !kin ? tdot(thmul(nrm, fr), nrm) / mi : 0.0;
+                    const double denom = wi + tdot(nrm, tool_linear(T, nrm)) + tdot(nrm, tcross(tool_angular(T, tcross(arm, nrm)), arm));
+                    if (closing > 0.0) {
+                        const double ni = closing / fmax(denom, 1e-9);
+                        const t3 nv = nrm * ni;
+                        const t3 dv = tdiv(thmul(nv, fr), mi);
+                        vi = vi + dv;
+                        tst(B.cv + 3 * size_t(e), tld(B.cv + 3 * size_t(e)) + dv);
+                        tst(T.vel, tld(T.vel) - tool_linear(T, nv));
+                        tst(T.ang, tld(T.ang) - tool_angular(T, tcross(arm, nv)));
+                        const t3 after = tld(T.vel) + tcross(tld(T.ang), arm) - vi;
+                        const t3 tangent = after - nrm * tdot(after, nrm);
+                        const double ts = tlen(tangent);
+                        const double rcp = 1.0 / ts;
+                        const t3 tdir = (isfinite(rcp) && rcp > 0.0) ? tangent * rcp : tv(0, 0, 0);
+                        const double tw = !kin ? tdot(thmul(tdir, fr), tdir) / mi : 0.0;
+                        const double tden = tw + tdot(tdir, tool_linear(T, tdir)) + tdot(tdir, tcross(tool_angular(T, tcross(arm, tdir)), arm));
+                        if (ts > 1e-6 && tden > 1e-9) {
+                            const double fm = fmin(ts / tden, T.friction * ni);
+                            const t3 f = tdir * fm;
+                            const t3 fdv = tdiv(thmul(f, fr), mi);
+                            vi = vi + fdv;
+                            tst(B.cv + 3 * size_t(e), tld(B.cv + 3 * size_t(e)) + fdv);
+                            tst(T.vel, tld(T.vel) - tool_linear(T, f));
+                            tst(T.ang, tld(T.ang) - tool_angular(T, tcross(arm, f)));
+                            T.friction_heat += fmax(fm * ts - 0.5 * fm * fm * tden, 0.0);
+                        }
+                        A.vel[v] = vi.x;
+                        A.vel[R + v] = vi.y;
+                        A.vel[2 * R + v] = vi.z;
+                    }
+                    if (tc.pen > max_pen) {
+                        max_pen = tc.pen;
+                        correction = tool_correction(T, nrm, tc.pen);
+                    }
+                }
+            }
+            tst(T.pos, tld(T.pos) + correction);
+            tool_constrain(T);
+        }
+        __syncthreads();
+        // Re-projection until no overlap deeper than 1e-6 is left (at most
+        // six passes): the first deepest overlap in the reference's order.
+        for (int pass = 0; pass < 6; ++pass) {
+            double my_pen = 1e-6;
+            uint32_t my_cell = ~0u;
+            const tq rp = tqld(T.rot);
+            for (uint32_t c = threadIdx.x; c < nc; c += kToolBlock) {
+                const t3 center = tld(T.pos) + trot(rp, tld(B.cells + 3 * c));
+                double cp = 1e-6;
+                t3 cn{0, 0, 0};
+                tool_cell_pairs(B, C, center, axes, [&](uint32_t, uint32_t v) {
+                    t3 nrm;
+                    double pen;
+                    if (!tool_sat(center, axes, arena_pos(A, v), nrm, pen)) return;
+                    if (pen > cp) {
+                        cp = pen;
+                        cn = nrm;
+                    }
+                });
+                B.best_pen[c] = cp;
+                tst(B.con[size_t(c) * kToolCellSlots].n, cn);
+                if (cp > my_pen) {  // cells ascend per thread: the first deepest
+                    my_pen = cp;
+                    my_cell = c;
+                }
+            }
+            // Workgroup reduction: the deepest, the lowest cell among equals.
+            for (int off = 32; off > 0; off >>= 1) {
+                const double op = __shfl_down(my_pen, off);
+                const uint32_t oc = __shfl_down(my_cell, off);
+                if (op > my_pen || (op == my_pen && oc < my_cell)) {
+                    my_pen = op;
+                    my_cell = oc;
+                }
+            }
+            if ((threadIdx.x & 63u) == 0) {
+                red_pen[threadIdx.x >> 6] = my_pen;
+                red_cell[threadIdx.x >> 6] = my_cell;
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                double bp = red_pen[0];
+                uint32_t bc = red_cell[0];
+                for (uint32_t w = 1; w < kToolBlock / 64; ++w)
+                    if (red_pen[w] > bp || (red_pen[w] == bp && red_cell[w] < bc)) {
+                        bp = red_pen[w];
+                        bc = red_cell[w];
+                    }
+                done = bp <= 1e-6 ? 1u : 0u;
+                if (!done) {
+                    tst(T.pos, tld(T.pos) + tool_correction(T, tld(B.con[size_t(bc) * kToolCellSlots].n), bp));
+                    tool_constrain(T);
+                }
+            }
+            __syncthreads();
+            if (done) break;
+        }
+    }
+    // Contact velocity: half a substep of drift, and the final velocities.
+    for (uint32_t e = threadIdx.x; e < C; e += kToolBlock) {
+        const t3 cv = tld(B.cv + 3 * size_t(e));
+        if (cv.x == 0.0 && cv.y == 0.0 && cv.z == 0.0) continue;
+        const uint32_t v = B.idx2[e];
+        const t3 p = arena_pos(A, v) + cv * (0.5 * dt);
+        A.pos[v] = p.x;
+        A.pos[R + v] = p.y;
+        A.pos[2 * R + v] = p.z;
+        A.pred[v] = p.x;
+        A.pred[R + v] = p.y;
+        A.pred[2 * R + v] = p.z;
+        A.drift[v] += cv.x;
+        A.drift[R + v] += cv.y;
+        A.drift[2 * R + v] += cv.z;
+    }
+    if (threadIdx.x == 0) {
+        // The candidates cover every REV a bin of a cell can hold while the
+        // cells stay within `reach` of their starting place.
+        const tq q0 = tqld(T.rot0), q = tqld(T.rot);
+        const double qd = fmin(fabs(((q0.x * q.x + q0.y * q.y) + q0.z * q.z) + q0.w * q.w), 1.0);
+        const double moved = tlen(tld(T.pos) - tld(T.pos0)) + T.radius * 2.0 * acos(qd);
+        T.contacts = contacts;
+        T.status = T.status | lstatus | (moved > T.reach ? uint32_t(TS_MARGIN) : 0u);
+        *B.dev = T;
+    }
+}
+
 // One workgroup per scene: the fracture pass's totals over the scene's tiles
 // (fixed order) into its state.
 __global__ __launch_bounds__(kBlock) void reduce_fracture(DevArena A) {
@@ -1871,7 +2208,15 @@ struct SlotHost {
     double last_rel = 0.0, last_angle = 0.0;
     bool manual = false;  // driven by the vx_scene_* stage calls
     bool has_strength = false;
+    bool has_tool = false;      // a vx_scene with a rigid tool (vx_scene_set_tool)
     vx_fracture_report frac{};  // fracture totals since admission (a vx_scene: this step)
+};
+
+// The device buffers of a one-scene arena's rigid tool.
+struct ToolHost {
+    ToolBufs bufs{};
+    void *alloc = nullptr;
+    uint32_t n_cells = 0;
 };
 
 // One launched tick: the bits it ran with and the scene states read back
@@ -1915,6 +2260,7 @@ struct vx_batch {
     std::vector<uint32_t> spread_last;
     uint32_t spread_tiles = 0;
     bool spread_dirty = true;
+    vx::ToolHost *tool = nullptr;  // the rigid tool of a one-scene arena
 };
 
 struct vx_scene {
@@ -2144,6 +2490,19 @@ int32_t launch_tick(vx_batch *b, Flight &f) {
         b->spread_dirty = false;
     }
     if (any & B_PREDICT) hipLaunchKernelGGL(substep_predict, grid, blk, 0, s, b->A);
+    if ((any & B_TOOL) && b->tool) {
+        // The tool advances over the substep after the prediction
+        // (src/solver.rs:530-543).
+        const SlotHost &sh = b->slots[0];
+        const double hs = sh.params.fixed_dt / double(sh.params.substeps);
+        const ToolBufs &tb = b->tool->bufs;
+        hipLaunchKernelGGL(tool_prepare, dim3(1), dim3(1), 0, s, tb, hs, sh.params.gravity[0], sh.params.gravity[1],
+                           sh.params.gravity[2]);
+        hipLaunchKernelGGL(tool_filter_count, grid, blk, 0, s, b->A, tb);
+        hipLaunchKernelGGL(tool_filter_scan, dim3(1), dim3(kToolBlock), 0, s, tb, sh.geo.ntiles);
+        hipLaunchKernelGGL(tool_filter_write, grid, blk, 0, s, b->A, tb);
+        hipLaunchKernelGGL(tool_advance, dim3(1), dim3(kToolBlock), 0, s, b->A, tb, hs);
+    }
     if (any & B_FIT) hipLaunchKernelGGL(fit_rotations, grid, blk, 0, s, b->A, uint32_t(B_FIT), 0);
     if (any & B_ASSEMBLE) {
         hipLaunchKernelGGL(assemble_bonds, grid, blk, 0, s, b->A);
@@ -2212,6 +2571,7 @@ uint32_t next_bits(const SlotHost &h, const SlotState &st) {
     case P_START:
     case P_FINISHED: {
         uint32_t b = B_PREDICT | B_FIT | B_ASSEMBLE | B_RESID0 | B_INNER | prec;
+        if (h.has_tool) b |= B_TOOL;
         if (h.params.solve.warm_start && h.warm_ok) b |= B_WARM_USE;
         return b;
     }
@@ -2762,6 +3122,11 @@ extern "C" int32_t vx_batch_destroy(vx_batch *b) {
         if (f.frac0) (void)hipEventDestroy(f.frac0);
         if (f.frac1) (void)hipEventDestroy(f.frac1);
     }
+    if (b->tool) {
+        if (b->tool->alloc) (void)hipFree(b->tool->alloc);
+        delete b->tool;
+        b->tool = nullptr;
+    }
     if (b->alloc) (void)hipFree(b->alloc);
     delete b;
     return VX_OK;
@@ -3005,6 +3370,135 @@ extern "C" int32_t vx_scene_destroy(vx_scene *sc) {
     if (!sc) return VX_OK;
     vx_batch_destroy(sc->batch);
     delete sc;
+    return VX_OK;
+}
+
+// ---- rigid tool ------------------------------------------------------------------
+static bool finite3(const double *v) { return std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]); }
+
+extern "C" int32_t vx_scene_set_tool(vx_scene *sc, const vx_tool_desc *d) {
+    clear_error();
+    if (int32_t c = scene_ok(sc)) return c;
+    Lock lock;
+    vx_batch *b = sc->batch;
+    if (int32_t c = drain(b)) return c;
+    if (b->tool) {
+        VX_HIP(hipStreamSynchronize(context().stream));
+        if (b->tool->alloc) VX_HIP(hipFree(b->tool->alloc));
+        delete b->tool;
+        b->tool = nullptr;
+    }
+    b->slots[sc->slot].has_tool = false;
+    if (!d) return VX_OK;
+    if (d->n_cells == 0 || !d->cells || !(d->mass > 0.0) || !std::isfinite(d->mass) || !finite3(d->inverse_inertia) ||
+        !finite3(d->position) || !finite3(d->velocity) || !finite3(d->angular_velocity) || !finite3(d->external_force) ||
+        !finite3(d->external_torque) || !std::isfinite(d->friction) || !(d->friction >= 0.0))
+        return fail(VX_ERR_INVALID_ARGUMENT, "invalid tool description");
+    for (int k = 0; k < 4; ++k)
+        if (!std::isfinite(d->rotation[k])) return fail(VX_ERR_INVALID_ARGUMENT, "invalid tool rotation");
+    const uint32_t nc = d->n_cells, ntiles = b->slots[sc->slot].geo.ntiles;
+    std::vector<double> cells(3 * size_t(nc));
+    double radius = 0.0;
+    for (uint32_t c = 0; c < nc; ++c) {
+        for (int k = 0; k < 3; ++k) {
+            cells[3 * c + k] = double(d->cells[3 * c + k]);
+            if (!std::isfinite(cells[3 * c + k])) return fail(VX_ERR_INVALID_ARGUMENT, "non-finite tool cell");
+        }
+        radius = std::max(radius, std::sqrt((cells[3 * c] * cells[3 * c] + cells[3 * c + 1] * cells[3 * c + 1]) +
+                                            cells[3 * c + 2] * cells[3 * c + 2]));
+    }
+    ToolDev T{};
+    T.mass = d->mass;
+    T.friction = d->friction;
+    T.radius = radius;
+    for (int k = 0; k < 3; ++k) {
+        T.inv_inertia[k] = d->inverse_inertia[k];
+        T.pos[k] = d->position[k];
+        T.vel[k] = d->velocity[k];
+        T.ang[k] = d->angular_velocity[k];
+        T.force[k] = d->external_force[k];
+        T.torque[k] = d->external_torque[k];
+        T.m_local[k] = d->mount_local_base[k];
+        T.m_world[k] = d->mount_world_base[k];
+        T.m_dir[k] = d->mount_direction[k];
+    }
+    for (int k = 0; k < 4; ++k) {
+        T.rot[k] = d->rotation[k];
+        T.m_rot[k] = d->mount_rotation[k];
+    }
+    T.n_cells = nc;
+    T.has_mount = d->has_mount ? 1u : 0u;
+    // One allocation, carved like the arena.
+    ToolHost *th = new ToolHost();
+    ToolBufs &tb = th->bufs;
+    struct Item {
+        void **ptr;
+        size_t bytes;
+    } items[] = {
+        {(void **)&tb.dev, sizeof(ToolDev)}, {(void **)&tb.cells, 24 * size_t(nc)},
+        {(void **)&tb.tile_count, 4 * size_t(ntiles)}, {(void **)&tb.tile_off, 4 * size_t(ntiles)},
+        {(void **)&tb.key, 8 * size_t(kToolMaxCand)}, {(void **)&tb.key2, 8 * size_t(kToolMaxCand)},
+        {(void **)&tb.idx, 4 * size_t(kToolMaxCand)}, {(void **)&tb.idx2, 4 * size_t(kToolMaxCand)},
+        {(void **)&tb.cv, 24 * size_t(kToolMaxCand)}, {(void **)&tb.con, sizeof(ToolContact) * kToolCellSlots * size_t(nc)},
+        {(void **)&tb.ncon, 4 * size_t(nc)}, {(void **)&tb.best_pen, 8 * size_t(nc)}, {(void **)&tb.best_seq, 4 * size_t(nc)},
+    };
+    size_t total = 0;
+    for (auto &it : items) total += (it.bytes + 255) & ~size_t(255);
+    hipError_t e = hipMalloc(&th->alloc, total);
+    if (e != hipSuccess) {
+        delete th;
+        return fail(VX_ERR_TOO_LARGE, "tool buffers of %zu bytes: %s", total, hipGetErrorString(e));
+    }
+    size_t off = 0;
+    for (auto &it : items) {
+        *it.ptr = static_cast<char *>(th->alloc) + off;
+        off += (it.bytes + 255) & ~size_t(255);
+    }
+    th->n_cells = nc;
+    b->tool = th;
+    VX_HIP(hipMemcpy(tb.dev, &T, sizeof T, hipMemcpyHostToDevice));
+    VX_HIP(hipMemcpy(const_cast<double *>(tb.cells), cells.data(), 24 * size_t(nc), hipMemcpyHostToDevice));
+    b->slots[sc->slot].has_tool = true;
+    return VX_OK;
+}
+
+// The tool's device state after the last advance.
+static int32_t read_tool(vx_batch *b, ToolDev &T) {
+    if (!b->tool) return fail(VX_ERR_INVALID_ARGUMENT, "the scene has no tool");
+    if (int32_t c = drain(b)) return c;
+    VX_HIP(hipMemcpy(&T, b->tool->bufs.dev, sizeof T, hipMemcpyDeviceToHost));
+    return VX_OK;
+}
+
+static int32_t tool_status(const ToolDev &T) {
+    if (T.status & TS_CAND)
+        return fail(VX_ERR_TOO_LARGE, "the tool's neighbourhood holds %u REVs, more than %u", T.candidates, kToolMaxCand);
+    if (T.status & TS_SLOTS) return fail(VX_ERR_TOO_LARGE, "a tool cell touched more than %u REVs", kToolCellSlots);
+    if (T.status & TS_KEY) return fail(VX_ERR_INVALID_ARGUMENT, "a REV near the tool lies outside the bin range");
+    if (T.status & TS_MARGIN) return fail(VX_ERR_DEVICE, "the tool moved beyond its candidate margin in one substep");
+    if (T.status & TS_STEPS)
+        return fail(VX_ERR_INVALID_ARGUMENT, "the tool would need more than %u microsteps in one substep", kToolMaxSteps);
+    return VX_OK;
+}
+
+extern "C" int32_t vx_scene_tool_state(vx_scene *sc, vx_tool_state *out) {
+    clear_error();
+    if (int32_t c = scene_ok(sc)) return c;
+    if (!out) return fail(VX_ERR_INVALID_ARGUMENT, "null state");
+    Lock lock;
+    ToolDev T{};
+    if (int32_t c = read_tool(sc->batch, T)) return c;
+    for (int k = 0; k < 3; ++k) {
+        out->position[k] = T.pos[k];
+        out->velocity[k] = T.vel[k];
+        out->angular_velocity[k] = T.ang[k];
+    }
+    for (int k = 0; k < 4; ++k) out->rotation[k] = T.rot[k];
+    out->friction_heat = T.friction_heat;
+    out->microsteps = T.steps;
+    out->contacts = T.contacts;
+    out->candidates = T.candidates;
+    out->status = T.status;
     return VX_OK;
 }
 
@@ -3395,6 +3889,10 @@ extern "C" int32_t vx_scene_step(vx_scene *sc, const vx_step_params *prm, vx_ste
     }
     (void)drain(b);
     h.manual = true;
+    if (status == VX_OK && b->tool) {
+        ToolDev T{};
+        if ((status = read_tool(b, T)) == VX_OK) status = tool_status(T);
+    }
     if (status == VX_OK && h.phase == P_FAILED)
         status = fail(int32_t(h.status), h.status == VX_ERR_NONFINITE_RESIDUAL ? "HIP PCG produced a non-finite residual norm"
                                                                                  : "HIP PCG did not converge within the refinement sweeps");

@@ -76,6 +76,23 @@ class StepParams(ctypes.Structure):
     pass
 
 
+class ToolDesc(ctypes.Structure):
+    _fields_ = [("n_cells", ctypes.c_uint32), ("has_mount", ctypes.c_uint32), ("cells", ctypes.POINTER(ctypes.c_float)),
+                ("mass", ctypes.c_double), ("inverse_inertia", ctypes.c_double * 3), ("position", ctypes.c_double * 3),
+                ("rotation", ctypes.c_double * 4), ("velocity", ctypes.c_double * 3),
+                ("angular_velocity", ctypes.c_double * 3), ("external_force", ctypes.c_double * 3),
+                ("external_torque", ctypes.c_double * 3), ("friction", ctypes.c_double),
+                ("mount_local_base", ctypes.c_double * 3), ("mount_world_base", ctypes.c_double * 3),
+                ("mount_direction", ctypes.c_double * 3), ("mount_rotation", ctypes.c_double * 4)]
+
+
+class ToolState(ctypes.Structure):
+    _fields_ = [("position", ctypes.c_double * 3), ("rotation", ctypes.c_double * 4), ("velocity", ctypes.c_double * 3),
+                ("angular_velocity", ctypes.c_double * 3), ("friction_heat", ctypes.c_double),
+                ("microsteps", ctypes.c_uint32), ("contacts", ctypes.c_uint32), ("candidates", ctypes.c_uint32),
+                ("status", ctypes.c_uint32)]
+
+
 class KernelStats(ctypes.Structure):
     _fields_ = [("dispatches", ctypes.c_uint32), ("reserved", ctypes.c_uint32), ("mean_us", ctypes.c_double),
                 ("median_us", ctypes.c_double), ("min_us", ctypes.c_double), ("max_us", ctypes.c_double),
@@ -177,6 +194,8 @@ SIGNATURES = [
     ("vx_scene_set_targets_and_loads", ctypes.c_int32, [ctypes.c_void_p, c_fp, c_fp]),
     ("vx_batch_download_bonds", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint32, c_u32p, c_fp, c_fp, c_fp]),
     ("vx_scene_download_bond_forces", ctypes.c_int32, [ctypes.c_void_p, c_dp, c_dp]),
+    ("vx_scene_set_tool", ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(ToolDesc)]),
+    ("vx_scene_tool_state", ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(ToolState)]),
     ("vx_batch_download_bond_forces", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint32, c_dp, c_dp]),
     ("vx_contact_project", ctypes.c_int32,
      [ctypes.c_uint32, c_fp, c_fp, c_fp, c_fp, c_u8p, c_u8p, c_u32p, ctypes.POINTER(Contact), ctypes.c_uint64,

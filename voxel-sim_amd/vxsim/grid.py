@@ -206,6 +206,25 @@ class GridScene:
     def download_bonds(self) -> dict:
         return download_bonds(lib().vx_scene_download_bonds, self.handle, n=self.n)
 
+    def set_tool(self, tool) -> None:
+        """vx_scene_set_tool: a vxsim.tool.Tool (or None to remove it); every
+        later step advances it per substep after the prediction."""
+        if tool is None:
+            check(lib().vx_scene_set_tool(self.handle, None))
+            return
+        d, keep = tool.desc()
+        check(lib().vx_scene_set_tool(self.handle, ctypes.byref(d)))
+        del keep
+
+    def tool_state(self) -> dict:
+        from ._abi import ToolState
+        st = ToolState()
+        check(lib().vx_scene_tool_state(self.handle, ctypes.byref(st)))
+        return dict(position=np.array(st.position[:]), rotation=np.array(st.rotation[:]),
+                    velocity=np.array(st.velocity[:]), angular_velocity=np.array(st.angular_velocity[:]),
+                    friction_heat=st.friction_heat, microsteps=st.microsteps, contacts=st.contacts,
+                    candidates=st.candidates, status=st.status)
+
     def download_bond_forces(self) -> dict:
         return download_bond_forces(lib().vx_scene_download_bond_forces, self.handle, n=self.n)
 

@@ -378,6 +378,38 @@ def wood_block_blade(seed: int = 5, dims=(192, 192, 96), blade=(40, 5, 20), spee
     return s
 
 
+def wood_block(seed: int = 5, dims=(192, 192, 96)) -> Scene:
+    """The cutting scene's wood block (grain Z, per-bond stiffness jitter,
+    bottom layer fixed, gravity, up to six corotation passes) without axis
+    targets: the blade acts on it as a rigid tool (`blade_tool`)."""
+    s = wood_block_blade(seed=seed, dims=dims)
+    s.targets = np.full((s.n, 3), np.nan, np.float32)
+    s.name = "wood_block_%dx%dx%d" % tuple(dims)
+    return s
+
+
+def blade_tool(s: Scene, tool_dimensions=(40, 2, 20), speed: float = 5.0, gap: float = 2e-4,
+               offset=(0.3137, 0.1713), friction: float = 0.35, mounted: bool = False):
+    """The blade of the cutting scene as a rigid tool (src/tool.rs:45-110):
+    Tool::new's tapered blade (the 2-REV thickness realised as a 5-REV spine),
+    iron, driven along -Z at `speed`; its edge cell `gap` metres above the
+    block's top face, centred over the face (shifted by `offset` cells in X
+    and Y so no cell face lies on a voxel face). `mounted` slides it on a
+    guide along its direction of travel from its rear face, as the
+    reference's application mounts its driven tools (src/main.rs:21-29):
+    rotation frozen, motion only along -Z."""
+    from .tool import BLADE, Tool
+    top = float(s.keys[:, 2].max() + 1) * float(CELL)
+    cx = float(s.positions[:, 0].mean()) + offset[0] * float(CELL)
+    cy = float(s.positions[:, 1].mean()) + offset[1] * float(CELL)
+    t = Tool.new(BLADE, tool_dimensions, [cx, cy, 0.0], [0.0, 0.0, -1.0], speed, friction=friction)
+    lowest = t.world_cells()[:, 2].min()
+    t.position = t.position + np.array([0.0, 0.0, top + 0.5 * float(CELL) + gap - lowest])
+    if mounted:
+        t.mount_base([float(t.cells[:, 0].min()), 0.0, 0.0], [0.0, 0.0, -1.0])
+    return t
+
+
 def blade_footprint(s: Scene, blade=(40, 5, 20)) -> np.ndarray:
     """Top-layer REVs under the blade's blade[0] x blade[1] edge, centred on
     the box's top face."""
