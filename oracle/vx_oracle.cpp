@@ -135,8 +135,10 @@ template <class T> M3<T> mat_from_quat(Q4<T> q) {
                   {xy - wz, T(1) - (xx + zz), yz + wx},
                   {xz + wy, yz - wx, T(1) - (xx + yy)}}};
 }
-// Quaternion of a rotation matrix: Shepperd's method, branching on the
-// largest of w^2, x^2, y^2, z^2 (via the diagonal) for a stable square root.
+// Quaternion of a rotation matrix: Mike Day's method ("Converting a Rotation
+// Matrix to a Quaternion", 2015), glam Quat::from_mat3's branch structure:
+// the signs of m22 and m00 -/+ m11 pick the largest of w^2, x^2, y^2, z^2
+// for a stable square root.
 template <class T> Q4<T> quat_from_mat(const M3<T> &m) {
     T m00 = m.c[0].x, m01 = m.c[0].y, m02 = m.c[0].z;
     T m10 = m.c[1].x, m11 = m.c[1].y, m12 = m.c[1].z;

@@ -253,3 +253,35 @@ def test_negative_zero_coordinates_keep_the_reference_bits():
     po, _, _, _ = oracle.contact_project(**s, targets=targets)
     zero = (po == 0)
     assert np.array_equal(np.signbit(p[zero]), np.signbit(po[zero]))
+
+
+def test_negative_zero_on_kinematic_voxels_next_to_movers():
+    """Kinematic endpoints receive separate's zero-weight share too
+    (src/collision.rs:657-663): the higher-index endpoint j gets normal * +0,
+    which turns a -0 coordinate into +0, the lower one keeps -0. With the
+    fixed floor (lowest indices) and the top layer kinematic, 20% of the
+    bonds broken, plant -0 in X or Y of the kinematic voxels nearest the
+    box's centre lines (half a cell from their place) and compare every bit,
+    signs of zero included, with the oracle; some of them flip."""
+    s = scenes.contact_box((10, 8, 6), seed=23, jitter=8e-4, movers=0.05, broken=0.2)
+    targets = s.pop("targets")
+    rest = s["rest_positions"]
+    top = rest[:, 2] == rest[:, 2].max()
+    s["kinematic"] = s["kinematic"] | top
+    targets[top] = False
+    kin = np.nonzero(s["kinematic"])[0]
+    planted = []
+    for axis in (0, 1):
+        near = kin[np.abs(rest[kin, axis]) == np.float32(0.005)]
+        for key in ("positions", "previous"):
+            s[key][near, axis] = np.float32(-0.0)
+        planted.append((near, axis))
+    rep, st = check_parity(s, targets=targets, label="-0 on kinematic voxels")
+    p, q, _, _ = contact.project(**s, targets=targets)
+    po, qo, _, _ = oracle.contact_project(**s, targets=targets)
+    flipped = 0
+    for near, axis in planted:
+        for a, b in ((p, po), (q, qo)):
+            assert np.array_equal(np.signbit(a[near, axis]), np.signbit(b[near, axis]))
+        flipped += np.count_nonzero(~np.signbit(po[near, axis]))
+    assert flipped > 0

@@ -108,6 +108,11 @@ __global__ void k_bins(uint32_t n, const float *__restrict__ start, const float 
     if (!ok) {
         atomicOr(err, 2u);
         c = 0;
+    } else if (c > (uint64_t(1) << 31)) {
+        // more entries than the whole call may hold; zeroing it keeps the
+        // scan of at most 2^32 counts of at most 2^31 each from wrapping
+        atomicOr(err, 4u);
+        c = 0;
     }
     count[v] = c;
 }
@@ -1034,7 +1039,7 @@ extern "C" int32_t vx_contact_project(uint32_t n, float *positions, float *previ
         return fail(VX_ERR_INVALID_ARGUMENT,
                     "vx_contact_project: a swept box has no finite bound on some axis or reaches 2^20 cells from the origin");
     const uint64_t entries = pin64[0];
-    if (entries > (uint64_t(1) << 31))
+    if ((pinned[0] & 4) || entries > (uint64_t(1) << 31))
         return fail(VX_ERR_TOO_LARGE, "vx_contact_project: %llu bin entries (swept boxes too large)",
                     (unsigned long long)entries);
 
@@ -1091,7 +1096,7 @@ extern "C" int32_t vx_contact_project(uint32_t n, float *positions, float *previ
         uint32_t *d_clist, *d_wfirst, *d_wlast, *d_moved, *d_ctl;
         float *d_pos_s, *d_prev_s;
         VX_BUF("contact.sep", size_t(P) + 8, d_sep);
-        VX_BUF("contact.cflag", size_t(P) + 8, d_due);
+        VX_BUF("contact.due", size_t(P) + 8, d_due);
         VX_BUF("contact.code_s", size_t(P) + 8, d_code_s);
         VX_BUF("contact.sep_s", size_t(P) + 8, d_sep_s);
         VX_BUF("contact.clist", 4 * size_t(P) + 8, d_clist);

@@ -64,7 +64,9 @@ __device__ __forceinline__ m3 mat_from_quat(q4 q) {
     double wx = q.w * x2, wy = q.w * y2, wz = q.w * z2;
     return {{1.0 - (yy + zz), xy + wz, xz - wy}, {xy - wz, 1.0 - (xx + zz), yz + wx}, {xz + wy, yz - wx, 1.0 - (xx + yy)}};
 }
-// Shepperd's method: branch on the largest of w^2, x^2, y^2, z^2.
+// Mike Day's method ("Converting a Rotation Matrix to a Quaternion", 2015),
+// the branch structure glam's Quat::from_mat3 uses: the sign of m22 and of
+// m00 -/+ m11 picks the largest of w^2, x^2, y^2, z^2 for the square root.
 __device__ __forceinline__ q4 quat_from_mat(const m3 &m) {
     double m00 = m.c0.x, m01 = m.c0.y, m02 = m.c0.z, m10 = m.c1.x, m11 = m.c1.y, m12 = m.c1.z;
     double m20 = m.c2.x, m21 = m.c2.y, m22 = m.c2.z;
