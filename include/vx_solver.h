@@ -369,9 +369,13 @@ int32_t vx_scene_download_bond_forces(vx_scene *scene, double *last_force, doubl
  * impulses and Coulomb friction applied in the reference's order, then the
  * removal of the deepest overlap and up to six re-projections
  * (Tool::advance, src/tool.rs:192-431). The contact velocity drifts the REVs
- * by half a substep and enters their final velocities. One tool per
- * vx_scene; the neighbourhood it can touch in one substep is limited to
- * 65,536 REVs and each cell to 32 contacts (VX_ERR_TOO_LARGE beyond). */
+ * by half a substep and enters their final velocities. After the substep's
+ * velocities a zero-time advance applies impulses and projection once more
+ * and discards its contact velocity (src/solver.rs:701-706). One tool per
+ * vx_scene, advanced by vx_scene_step only (not by the stage calls); the
+ * neighbourhood it can touch in one substep is limited to 65,536 REVs and
+ * each cell to 32 contacts (VX_ERR_TOO_LARGE beyond), a substep to 65,536
+ * microsteps (VX_ERR_INVALID_ARGUMENT). */
 typedef struct vx_tool_desc {
     uint32_t n_cells;
     uint32_t has_mount;            /* 1: guide mount (Tool::mount_base, src/tool.rs:127-142) */
@@ -388,7 +392,9 @@ typedef struct vx_tool_desc {
 typedef struct vx_tool_state {
     double position[3], rotation[4], velocity[3], angular_velocity[3];
     double friction_heat;          /* J dissipated by tool friction since the tool was set */
-    uint32_t microsteps, contacts, candidates, status;  /* of the last advance */
+    /* microsteps of the last substep's timed advance, contacts of both its
+     * advances, candidate REVs of the last advance, status bits */
+    uint32_t microsteps, contacts, candidates, status;
 } vx_tool_state;
 /* Set (or with NULL remove) the scene's tool. */
 int32_t vx_scene_set_tool(vx_scene *scene, const vx_tool_desc *tool);

@@ -1651,9 +1651,10 @@ static void step_t(vxo_scene *s, const vxo_step_config *cfg, const double *targe
         // The tool's contact impulses, acting near the middle of the substep
         // (src/solver.rs:530-543).
         std::vector<double> contact(3 * size_t(n), 0.0);
+        const double gd[3] = {cfg->gravity[0], cfg->gravity[1], cfg->gravity[2]};
+        std::vector<double> mass;
         if (tool) {
-            const double gd[3] = {cfg->gravity[0], cfg->gravity[1], cfg->gravity[2]};
-            std::vector<double> mass(s->mass, s->mass + n);
+            mass.assign(s->mass, s->mass + n);
             vxo_tool_advance(std::is_same<T, float>::value ? VXO_F32 : VXO_F64, tool, n, s->positions, s->velocities,
                              mass.data(), s->kinematic, targets, double(h), gd, contact.data());
             for (size_t k = 0; k < 3 * size_t(n); ++k)
@@ -1704,6 +1705,17 @@ static void step_t(vxo_scene *s, const vxo_step_config *cfg, const double *targe
                     s->velocities[3 * i + a] = double(drift[i][a] + cv + T(contact[3 * i + a]));
                 }
             }
+        }
+        // The tool's zero-time pass on the final state; its contact velocity
+        // is discarded (src/solver.rs:701-706). The tool reports the timed
+        // advance's microsteps and both advances' contacts.
+        if (tool) {
+            const uint32_t steps = tool->microsteps, contacts = tool->contacts;
+            std::vector<double> discard(3 * size_t(n), 0.0);
+            vxo_tool_advance(std::is_same<T, float>::value ? VXO_F32 : VXO_F64, tool, n, s->positions, s->velocities,
+                             mass.data(), s->kinematic, targets, 0.0, gd, discard.data());
+            tool->microsteps = steps;
+            tool->contacts += contacts;
         }
     }
     if (report) *report = rep;

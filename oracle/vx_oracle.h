@@ -306,7 +306,10 @@ void vxo_tool_advance(int mode, vxo_tool *tool, uint32_t n, const double *positi
  * substep after the prediction, the tool advances (src/solver.rs:530-534),
  * positions drift by half a substep of the contact velocity
  * (src/solver.rs:535-543), and the final velocities of free axes include it
- * (src/solver.rs:668-683). */
+ * (src/solver.rs:668-683); then a zero-time advance on the final state
+ * (src/solver.rs:701-706), whose contact velocity is discarded. The tool
+ * keeps the timed advance's microsteps and the sum of both advances'
+ * contacts. */
 void vxo_step_tool(int mode, vxo_scene *s, const vxo_step_config *cfg, const double *targets, const double *loads,
                    vxo_tool *tool, vxo_fracture_state *fracture, vxo_step_report *report);
 

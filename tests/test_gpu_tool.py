@@ -1,5 +1,6 @@
 """GPU parity of the rigid tool (Tool::advance, src/tool.rs:192-431, inside the
-step as src/solver.rs:530-543 and 668-683 place it) against the FP64 oracle,
+step as src/solver.rs:530-543 and 668-683 place it, with the zero-time pass
+of src/solver.rs:701-706 after the velocities) against the FP64 oracle,
 through the C ABI: the tool's pose and velocities and the scene's positions
 and velocities after each step, for the reference's one-REV contact cases
 (off-centre hit, spinning tool with friction) and for the cutting scene's

@@ -1906,8 +1906,8 @@ __device__ __forceinline__ void tool_cell_pairs(const ToolBufs &B, uint32_t C, t
 }
 
 // Tool::advance for one substep (one workgroup; src/tool.rs:192-431); dt is
-// the substep length.
-__global__ __launch_bounds__(kToolBlock) void tool_advance(DevArena A, ToolBufs B, double dt) {
+// the substep length, 0 (timed = 0) for the zero-time pass.
+__global__ __launch_bounds__(kToolBlock) void tool_advance(DevArena A, ToolBufs B, double dt, uint32_t timed) {
 #pragma clang fp contract(off)
     __shared__ ToolDev T;
     __shared__ double red_pen[kToolBlock / 64];
@@ -1928,7 +1928,8 @@ __global__ __launch_bounds__(kToolBlock) void tool_advance(DevArena A, ToolBufs 
         B.key2[i] = i < C ? B.key[i] : ~uint64_t(0);
         B.idx2[i] = i < C ? B.idx[i] : ~0u;
     }
-    for (uint32_t i = threadIdx.x; i < 3 * C; i += kToolBlock) B.cv[i] = 0.0;
+    if (timed)
+        for (uint32_t i = threadIdx.x; i < 3 * C; i += kToolBlock) B.cv[i] = 0.0;
     __syncthreads();
     for (uint32_t k = 2; k <= P; k <<= 1)
         for (uint32_t j = k >> 1; j > 0; j >>= 1) {
@@ -1948,7 +1949,7 @@ __global__ __launch_bounds__(kToolBlock) void tool_advance(DevArena A, ToolBufs 
             __syncthreads();
         }
     uint32_t contacts = 0;
-    for (uint32_t step = 0; step < T.steps; ++step) {
+    for (uint32_t step = 0; step < T.micro; ++step) {
         if (threadIdx.x == 0) {
             const double h = T.h;
             tst(T.vel, tld(T.vel) + tld(T.lin_acc) * h);
@@ -2019,7 +2020,7 @@ __global__ __launch_bounds__(kToolBlock) void tool_advance(DevArena A, ToolBufs 
                         const t3 nv = nrm * ni;
                         const t3 dv = tdiv(thmul(nv, fr), mi);
                         vi = vi + dv;
-                        tst(B.cv + 3 * size_t(e), tld(B.cv + 3 * size_t(e)) + dv);
+                        if (timed) tst(B.cv + 3 * size_t(e), tld(B.cv + 3 * size_t(e)) + dv);
                         tst(T.vel, tld(T.vel) - tool_linear(T, nv));
                         tst(T.ang, tld(T.ang) - tool_angular(T, tcross(arm, nv)));
                         const t3 after = tld(T.vel) + tcross(tld(T.ang), arm) - vi;
@@ -2034,7 +2035,7 @@ __global__ __launch_bounds__(kToolBlock) void tool_advance(DevArena A, ToolBufs 
                             const t3 f = tdir * fm;
                             const t3 fdv = tdiv(thmul(f, fr), mi);
                             vi = vi + fdv;
-                            tst(B.cv + 3 * size_t(e), tld(B.cv + 3 * size_t(e)) + fdv);
+                            if (timed) tst(B.cv + 3 * size_t(e), tld(B.cv + 3 * size_t(e)) + fdv);
                             tst(T.vel, tld(T.vel) - tool_linear(T, f));
                             tst(T.ang, tld(T.ang) - tool_angular(T, tcross(arm, f)));
                             T.friction_heat += fmax(fm * ts - 0.5 * fm * fm * tden, 0.0);
@@ -2111,8 +2112,9 @@ __global__ __launch_bounds__(kToolBlock) void tool_advance(DevArena A, ToolBufs 
             if (done) break;
         }
     }
-    // Contact velocity: half a substep of drift, and the final velocities.
-    for (uint32_t e = threadIdx.x; e < C; e += kToolBlock) {
+    // Contact velocity: half a substep of drift, and the final velocities
+    // (the zero-time pass discards it).
+    for (uint32_t e = threadIdx.x; timed && e < C; e += kToolBlock) {
         const t3 cv = tld(B.cv + 3 * size_t(e));
         if (cv.x == 0.0 && cv.y == 0.0 && cv.z == 0.0) continue;
         const uint32_t v = B.idx2[e];
@@ -2133,7 +2135,7 @@ __global__ __launch_bounds__(kToolBlock) void tool_advance(DevArena A, ToolBufs 
         const tq q0 = tqld(T.rot0), q = tqld(T.rot);
         const double qd = fmin(fabs(((q0.x * q.x + q0.y * q.y) + q0.z * q.z) + q0.w * q.w), 1.0);
         const double moved = tlen(tld(T.pos) - tld(T.pos0)) + T.radius * 2.0 * acos(qd);
-        T.contacts = contacts;
+        T.contacts = T.contacts + contacts;
         T.status = T.status | lstatus | (moved > T.reach ? uint32_t(TS_MARGIN) : 0u);
         *B.dev = T;
     }
@@ -2490,19 +2492,22 @@ int32_t launch_tick(vx_batch *b, Flight &f) {
         b->spread_dirty = false;
     }
     if (any & B_PREDICT) hipLaunchKernelGGL(substep_predict, grid, blk, 0, s, b->A);
-    if ((any & B_TOOL) && b->tool) {
-        // The tool advances over the substep after the prediction
-        // (src/solver.rs:530-543).
+    // The rigid tool of a one-scene arena: one advance over dt (0: the
+    // zero-time pass).
+    auto tool_pass = [&](bool timed) {
         const SlotHost &sh = b->slots[0];
-        const double hs = sh.params.fixed_dt / double(sh.params.substeps);
+        const double dt = timed ? sh.params.fixed_dt / double(sh.params.substeps) : 0.0;
         const ToolBufs &tb = b->tool->bufs;
-        hipLaunchKernelGGL(tool_prepare, dim3(1), dim3(1), 0, s, tb, hs, sh.params.gravity[0], sh.params.gravity[1],
-                           sh.params.gravity[2]);
+        hipLaunchKernelGGL(tool_prepare, dim3(1), dim3(1), 0, s, tb, dt, sh.params.gravity[0], sh.params.gravity[1],
+                           sh.params.gravity[2], uint32_t(timed));
         hipLaunchKernelGGL(tool_filter_count, grid, blk, 0, s, b->A, tb);
         hipLaunchKernelGGL(tool_filter_scan, dim3(1), dim3(kToolBlock), 0, s, tb, sh.geo.ntiles);
         hipLaunchKernelGGL(tool_filter_write, grid, blk, 0, s, b->A, tb);
-        hipLaunchKernelGGL(tool_advance, dim3(1), dim3(kToolBlock), 0, s, b->A, tb, hs);
-    }
+        hipLaunchKernelGGL(tool_advance, dim3(1), dim3(kToolBlock), 0, s, b->A, tb, dt, uint32_t(timed));
+    };
+    // The tool advances over the substep after the prediction
+    // (src/solver.rs:530-543).
+    if ((any & B_TOOL) && (any & B_PREDICT) && b->tool) tool_pass(true);
     if (any & B_FIT) hipLaunchKernelGGL(fit_rotations, grid, blk, 0, s, b->A, uint32_t(B_FIT), 0);
     if (any & B_ASSEMBLE) {
         hipLaunchKernelGGL(assemble_bonds, grid, blk, 0, s, b->A);
@@ -2547,6 +2552,8 @@ int32_t launch_tick(vx_batch *b, Flight &f) {
         VX_HIP(hipEventRecord(f.frac1, s));
     }
     if (any & B_FINISH) hipLaunchKernelGGL(finish_velocities, grid, blk, 0, s, b->A);
+    // Its zero-time pass on the substep's final state (src/solver.rs:701-706).
+    if ((any & B_TOOL) && (any & B_FINISH) && b->tool) tool_pass(false);
     VX_HIP(hipGetLastError());
     hipLaunchKernelGGL(publish_state, dim3(1), blk, 0, s, b->d_st, f.st_dev, b->max_slots);
     VX_HIP(hipGetLastError());
@@ -2561,7 +2568,11 @@ uint32_t solve_bits(const vx_solve_params &s) {
 }
 uint32_t prec_bits(const vx_step_params &p) { return solve_bits(p.solve); }
 // The end of a substep: the fracture pass (when on) and the velocities.
-uint32_t finish_bits(const vx_step_params &p) { return p.fracture.enabled ? uint32_t(B_FINISH | B_FRACTURE) : uint32_t(B_FINISH); }
+// The last tick of a substep: velocities, fracture if enabled, the tool's
+// zero-time pass if the scene has one.
+uint32_t finish_bits(const SlotHost &h) {
+    return uint32_t(B_FINISH) | (h.params.fracture.enabled ? uint32_t(B_FRACTURE) : 0u) | (h.has_tool ? uint32_t(B_TOOL) : 0u);
+}
 
 // Bits of an automatically driven scene for its next tick, from its latest
 // read-back state.
@@ -2581,12 +2592,12 @@ uint32_t next_bits(const SlotHost &h, const SlotState &st) {
             if (h.pass == 0 && h.params.solve.warm_start) b |= B_WARM_SAVE;
             // The reference refits only when another pass may follow
             // (src/solver.rs:561-564).
-            b |= (h.pass + 1 < h.params.corotation_iterations) ? uint32_t(B_FITB) : finish_bits(h.params);
+            b |= (h.pass + 1 < h.params.corotation_iterations) ? uint32_t(B_FITB) : finish_bits(h);
             return b;
         }
         return B_INNER | prec;
     case P_APPLIED:
-        if (h.finish_next) return finish_bits(h.params);
+        if (h.finish_next) return finish_bits(h);
         return B_ASSEMBLE | B_RESID0 | B_INNER | prec;
     default:
         return 0;

@@ -21,7 +21,10 @@
 //    first deepest overlap in that order by a workgroup reduction;
 //  * the contact velocity of each candidate drifts its position by half a
 //    substep (src/solver.rs:535-543) and joins its drift velocity, so the
-//    substep's final velocity includes it (src/solver.rs:668-683).
+//    substep's final velocity includes it (src/solver.rs:668-683);
+//  * after the substep's velocities, a zero-time advance (no motion, one
+//    microstep) applies the impulses and projection once more on the final
+//    state and discards its contact velocities (src/solver.rs:701-706).
 //
 // Arithmetic is FP64 without contraction, the operations in the reference's
 // order (glam's vector rotation and quaternion product).
@@ -40,7 +43,9 @@ struct ToolDev {
     double pos0[3], rot0[4], box_lo[3], box_hi[3];  // pose at the advance's start, candidate box (body frame)
     double h, lin_acc[3], reach;
     double friction_heat;
-    uint32_t n_cells, has_mount, steps, candidates, contacts, status;
+    // steps: microsteps of the substep's timed advance; micro: of the advance
+    // in progress; contacts: summed over the substep's two advances
+    uint32_t n_cells, has_mount, steps, micro, candidates, contacts, status;
 };
 struct ToolContact {
     uint32_t slot, pad;
@@ -215,8 +220,9 @@ __device__ __forceinline__ bool tool_bounds(t3 center, const t3 (&axes)[3], int3
 }
 
 // Start of an advance (one thread): constrain the pose, the microsteps of
-// the substep (src/tool.rs:198-214), the candidate box.
-__global__ void tool_prepare(ToolBufs B, double dt, double gx, double gy, double gz) {
+// the substep (src/tool.rs:198-214), the candidate box. `timed` is 0 for the
+// zero-time pass (dt = 0).
+__global__ void tool_prepare(ToolBufs B, double dt, double gx, double gy, double gz, uint32_t timed) {
 #pragma clang fp contract(off)
     ToolDev &T = *B.dev;
     tool_constrain(T);
@@ -232,7 +238,11 @@ __global__ void tool_prepare(ToolBufs B, double dt, double gx, double gy, double
         T.status = TS_STEPS;
         steps = 1;
     }
-    T.steps = steps;
+    T.micro = steps;
+    if (timed) {
+        T.steps = steps;
+        T.contacts = 0;
+    }
     T.h = dt / double(steps);
     tst(T.lin_acc, lin);
     for (int k = 0; k < 3; ++k) T.pos0[k] = T.pos[k];
@@ -254,7 +264,6 @@ __global__ void tool_prepare(ToolBufs B, double dt, double gx, double gy, double
         T.box_hi[k] = hi[k] + pad;
     }
     T.candidates = 0;
-    T.contacts = 0;
 }
 
 __device__ __forceinline__ bool tool_in_box(const ToolDev &T, t3 p) {
