@@ -531,49 +531,61 @@ def cut_contact(scene, gs, prev_pos, targets, cpu: bool) -> dict:
     return out
 
 
-def cut_bench(steps: int, dims=(192, 192, 96), preconditioner: int = 0, contact: bool = False, cpu: bool = True):
-    """The cutting scene with fracture: the 192x192x96 wood block, the blade's
-    40x5 edge driven 5 mm per 1 ms step through per-step axis targets, up to
-    six corotation passes and the fracture pass every step (broken bonds leave
-    the lattice before the next assembly). Prints one JSON line with the step
-    time, the solve throughput and the bonds broken per step."""
+def cut_bench(steps: int, dims=(192, 192, 96), preconditioner: int = 0, contact: bool = False, cpu: bool = True,
+              speed: float = 5.0):
+    """The cutting scene with fracture: the 192x192x96 wood block (bottom
+    layer fixed, gravity) and the blade as a rigid tool, Tool::new's tapered
+    40x2x20 blade (realised with a 5-REV spine) of iron, mounted on a guide
+    along -Z and launched at `speed` m/s onto the top face; every step
+    advances the tool per substep (and its zero-time pass), runs up to six
+    corotation passes and the fracture pass (broken bonds leave the lattice
+    before the next assembly). Prints one JSON line with the step time, the
+    solve throughput, the bonds broken and the tool's track per step."""
     from vxsim import grid, scenes
-    blade = (40, 5, 20)
-    scene = scenes.wood_block_blade(dims=dims, blade=blade)
+    blade = (40, 2, 20)
+    scene = scenes.wood_block(dims=dims)
+    tool = scenes.blade_tool(scene, blade, speed=speed, mounted=True)
     cfg = scene.config
     sp = grid.solve_params(tolerance=1e-6, inner_tolerance=INNER_TOL, max_inner=40000, max_outer=12,
                            preconditioner=preconditioner)
-    reps = []
-    walls = []
+    reps, walls, track = [], [], []
     contact_out = None
+    no_targets = np.full((scene.n, 3), np.nan, np.float32)
     with grid.GridScene(scene) as gs:
+        gs.set_tool(tool)
         for k in range(steps):
             if contact and k == steps - 1:
                 prev_pos = gs.download()["positions"]
             t = time.perf_counter()
-            step_targets = scenes.blade_targets(scene, blade, 5.0, cfg.fixed_dt, k)
-            gs.set_targets_and_loads(step_targets, scene.loads)
             r = gs.step(fixed_dt=cfg.fixed_dt, gravity=cfg.gravity, substeps=cfg.substeps,
                         corotation_iterations=cfg.corotation_iterations, corotation_tolerance=cfg.corotation_tolerance,
                         solve=sp, fracture=scenes.FractureConfig())
             walls.append(time.perf_counter() - t)
+            st = gs.tool_state()
             r["active"] = gs.active
             reps.append(r)
-            print("cut step %d: %.1f ms, %d solves, %d inner iterations, %d bonds broken, fracture pass %.3f ms" %
+            track.append({"z": round(float(st["position"][2]), 6), "vz": round(float(st["velocity"][2]), 4),
+                          "contacts": int(st["contacts"]), "microsteps": int(st["microsteps"]),
+                          "candidates": int(st["candidates"])})
+            print("cut step %d: %.1f ms, %d solves, %d inner iterations, %d bonds broken, fracture pass %.3f ms, "
+                  "tool z %.5f vz %.3f, %d contacts" %
                   (k, 1000 * walls[-1], r["solves"], r["inner_iterations"], r["fracture"]["bonds_broken"],
-                   1000 * r["fracture"]["seconds"]), file=sys.stderr, flush=True)
+                   1000 * r["fracture"]["seconds"], st["position"][2], st["velocity"][2], st["contacts"]),
+                  file=sys.stderr, flush=True)
         if contact:
-            contact_out = cut_contact(scene, gs, prev_pos, step_targets, cpu)
+            contact_out = cut_contact(scene, gs, prev_pos, no_targets, cpu)
     dof_iters = sum(3.0 * r["active"] * (r["inner_iterations"] + r["outer_sweeps"]) for r in reps)
     line = {"metric": "DOF-PCG-iters/s", "value": dof_iters / sum(walls), "unit": "DOF-PCG-iters/s",
-            "workload": "%s with fracture: %d REVs, blade edge %dx%d REVs at 5 m/s, %d steps of 1 ms, <=%d passes, "
-                        "rel. residual 1e-6" % (scene.name, scene.n, blade[0], blade[1], steps,
-                                               cfg.corotation_iterations),
+            "workload": "%s with fracture: %d REVs, rigid blade tool %dx%dx%d (%d cells, %.2f kg) on a -Z guide "
+                        "launched at %g m/s, %d steps of 1 ms, <=%d passes, rel. residual 1e-6" %
+                        (scene.name, scene.n, blade[0], blade[1], blade[2], len(tool.cells), tool.mass, speed, steps,
+                         cfg.corotation_iterations),
             "preconditioner": "red_black" if preconditioner else "block_jacobi",
             "mean_ms_per_step": 1000 * sum(walls) / len(walls),
             "ms_per_step": [round(1000 * w, 2) for w in walls], "solves": [r["solves"] for r in reps],
             "inner_iterations": [r["inner_iterations"] for r in reps],
             "bonds_broken": [int(r["fracture"]["bonds_broken"]) for r in reps],
+            "tool": track,
             "fracture_pass": {"kernels": "fracture_bonds + reduce_fracture",
                               "ms_per_step": [round(1000 * r["fracture"]["seconds"], 4) for r in reps],
                               "bytes_per_row": FRACTURE_BYTES_PER_ROW,
@@ -582,7 +594,7 @@ def cut_bench(steps: int, dims=(192, 192, 96), preconditioner: int = 0, contact:
                               "peak_gbs": PEAK_HBM_GBS},
             "dissipated_j": {e: sum(r["fracture"][e] for r in reps) for e in
                              ("plastic_heat", "stored_plastic", "surface_work", "damage_heat", "released")},
-            "data": "synthetic", "dtype": "f32 inner / f64 refinement, reductions, assembly and fracture"}
+            "data": "synthetic", "dtype": "f32 inner / f64 refinement, reductions, assembly, fracture and tool"}
     if contact_out is not None:
         line["contact_projection_after_last_step"] = contact_out
     print(json.dumps(line))
@@ -659,7 +671,8 @@ def main():
     ap.add_argument("--pmc-driver", action="store_true", help="short SpMV + copy run for counter passes")
     ap.add_argument("--serve", action="store_true", help="served request stream instead of the step benchmark")
     ap.add_argument("--large", action="store_true", help="192x192x96 block instead of the 64^3 box")
-    ap.add_argument("--cut", action="store_true", help="192x192x96 block cut by the blade with fracture")
+    ap.add_argument("--cut", action="store_true", help="192x192x96 block cut by the blade tool with fracture")
+    ap.add_argument("--cut-speed", type=float, default=5.0, help="--cut: the blade's launch speed (m/s)")
     ap.add_argument("--red-black", action="store_true", help="--cut / --serve with the red-black preconditioner")
     ap.add_argument("--contact", action="store_true", help="contact projection leg only (64^3 contact box)")
     ap.add_argument("--no-contact", action="store_true")
@@ -693,7 +706,7 @@ def main():
         print(json.dumps({"contact_projection": contact_bench(cpu=not args.no_cpu_baseline)}))
         return
     if args.cut:
-        cut_bench(args.steps, preconditioner=1 if args.red_black else 0, contact=args.cut_contact,
+        cut_bench(args.steps, preconditioner=1 if args.red_black else 0, contact=args.cut_contact, speed=args.cut_speed,
                   cpu=not args.no_cpu_baseline)
         return
     if args.large:
