@@ -514,7 +514,12 @@ def cut_contact(scene, gs, prev_pos, targets, cpu: bool) -> dict:
     held = ~np.isnan(targets)
     args = dict(positions=cur, previous=prev_pos.astype(np.float32), rest_positions=rest,
                 mass=scene.mass.astype(np.float32), kinematic=scene.kinematic != 0, bond_neighbor=bonds)
-    runs = [contact.project(**args, targets=held) for _ in range(2)]
+    runs = []
+    for k in range(2):
+        runs.append(contact.project(**args, targets=held))
+        print("cut contact: device call %d: %.1f ms, %d pairs, %d contacts" %
+              (k, runs[-1][3]["device_us"] / 1e3, runs[-1][3]["pairs"], runs[-1][3]["contacts"]),
+              file=sys.stderr, flush=True)
     p, q, c, rep = runs[-1]
     out = {"voxels": scene.n, "broken_bonds_so_far": int(3 * scene.n - np.count_nonzero(bonds != 0xFFFFFFFF)),
            "candidate_pairs": rep["candidate_pairs"], "pairs": rep["pairs"], "contacts": rep["contacts"],
@@ -522,9 +527,12 @@ def cut_contact(scene, gs, prev_pos, targets, cpu: bool) -> dict:
            "device_ms": rep["device_us"] / 1e3, "projection_ms": rep["projection_us"] / 1e3}
     if cpu:
         from oracle import oracle
+        print("cut contact: the oracle on one host thread", file=sys.stderr, flush=True)
+        os.environ["VXO_PROGRESS"] = "1"
         t0 = time.perf_counter()
         po, qo, co, _ = oracle.contact_project(**args, targets=held)
         out["cpu_oracle_ms"] = 1000 * (time.perf_counter() - t0)
+        print("cut contact: oracle %.1f s" % (out["cpu_oracle_ms"] / 1e3), file=sys.stderr, flush=True)
         out["bit_identical_to_oracle"] = bool(
             np.array_equal(p.view(np.uint32), po.view(np.uint32)) and np.array_equal(q.view(np.uint32), qo.view(np.uint32))
             and all(np.array_equal(c[k], co[k]) for k in ("i", "j", "new_impact")))

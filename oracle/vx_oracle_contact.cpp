@@ -24,6 +24,8 @@
 #include <array>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <unordered_map>
 #include <unordered_set>
@@ -120,6 +122,9 @@ std::vector<std::pair<uint32_t, uint32_t>> candidate_pairs(const Scene &s, const
         const uint32_t i = std::min(a, b), j = std::max(a, b);
         if (unique.insert((uint64_t(i) << 32) | j).second) pairs.emplace_back(i, j);
     };
+    const char *env = std::getenv("VXO_PROGRESS");
+    const bool progress = env && env[0] == '1';
+    size_t done = 0;
     for (const auto &kv : bins) {
         const std::vector<uint32_t> &ids = kv.second;
         for (size_t a = 0; a < ids.size(); ++a)
@@ -128,6 +133,10 @@ std::vector<std::pair<uint32_t, uint32_t>> candidate_pairs(const Scene &s, const
         if (f != static_bins.end())
             for (uint32_t i : ids)
                 for (uint32_t j : f->second) add(i, j);
+        if (progress && ++done % (bins.size() / 16 + 1) == 0) {
+            std::fprintf(stderr, "oracle contact: %zu of %zu bins paired, %zu pairs\n", done, bins.size(), pairs.size());
+            std::fflush(stderr);
+        }
     }
     std::sort(pairs.begin(), pairs.end());
     return pairs;
@@ -316,6 +325,15 @@ extern "C" uint64_t vxo_contact_project(uint32_t n, float *positions, float *pre
                                    return local_continuum_pair(s, p.first, p.second);
                                }),
                 pairs.end());
+    // VXO_PROGRESS=1: a line on stderr after the broad phase and each pass
+    // (long single-thread runs on large states).
+    const char *env = std::getenv("VXO_PROGRESS");
+    const bool progress = env && env[0] == '1';
+    if (progress) {
+        std::fprintf(stderr, "oracle contact: %llu candidate pairs, %zu after the local-continuum filter\n",
+                     (unsigned long long)candidates, pairs.size());
+        std::fflush(stderr);
+    }
     std::vector<Contact> contacts;
     uint64_t passes = 0;
     for (int pass = 0; pass < 6; ++pass) {
@@ -346,6 +364,11 @@ extern "C" uint64_t vxo_contact_project(uint32_t n, float *positions, float *pre
             }
             axis = touching_normal(load(s.pos, i), load(s.pos, j), &sign);
             if (axis >= 0) contacts.push_back({i, j, axis, sign, false});
+        }
+        if (progress) {
+            std::fprintf(stderr, "oracle contact: pass %llu done, %zu contacts so far\n", (unsigned long long)passes,
+                         contacts.size());
+            std::fflush(stderr);
         }
         if (!changed) break;
     }
