@@ -7,15 +7,34 @@ import numpy as np
 
 def test_schedule_lookup_buckets_and_defaults(tmp_path):
     from vxsim import tune
-    assert [tune.bucket_of(x) for x in (1, 1.9e4, 2e4, 1e5, 3e6, 1e9)] == [0, 0, 1, 1, 3, 3]
+    # one-scene buckets: 64^3 (262,144 REVs) and 192x192x96 (3.5 M) get their own
+    assert [tune.bucket_of(x) for x in (1, 1.9e4, 2e4, 1e5, 262144, 589824, 3.5e6, 1e9)] == [0, 0, 1, 1, 2, 3, 4, 4]
+    assert [tune.bucket_of(x, "batch") for x in (1, 1.9e4, 2e4, 1e5, 3e6, 1e9)] == [0, 0, 1, 1, 3, 3]
     assert tune.lookup(5e4, "single", table={}) == tune.DEFAULT["single"]
-    t = {"single": {"1": {"iters": 32}}, "batch": {"0": {"chunk": 8, "iters": 8}}}
-    assert tune.lookup(5e4, "single", table=t) == {"iters": 32}
+    t = {"single": {"1": {"iters": 32, "zcols": 0, "deal": 1}}, "batch": {"0": {"chunk": 8, "iters": 8}}}
+    assert tune.lookup(5e4, "single", table=t) == {"iters": 32, "zcols": 0, "deal": 1}
     assert tune.lookup(5e3, "batch", table=t) == {"chunk": 8, "iters": 8}
     assert tune.lookup(5e5, "batch", table=t) == tune.DEFAULT["batch"]
+    # a table written before the shape knobs keeps the default shape
+    assert tune.lookup(5e4, "single", table={"single": {"1": {"iters": 8}}}) == dict(tune.DEFAULT["single"], iters=8)
     path = tmp_path / "t.json"
     path.write_text(json.dumps(t))
     assert tune.load_table(str(path))["single"]["1"]["iters"] == 32
+
+
+def test_shipped_schedule_table_is_well_formed():
+    """The measured table next to vxsim.tune: every one-scene bucket has a
+    schedule and a launch shape the library accepts."""
+    from vxsim import tune
+    t = tune.load_table(tune.TABLE)
+    for b in range(len(tune.BUCKETS["single"])):
+        e = tune.lookup([1e4, 1e5, 3e5, 6e5, 4e6][b], "single", table=t)
+        assert e["iters"] % 2 == 0 and 2 <= e["iters"] <= 256
+        assert e["zcols"] in (0, 1, 2, 4, 8, 16, 32, 64)
+        assert e["deal"] in (0, 1, 2, 3)
+    for b in range(3):
+        e = tune.lookup([1e4, 1e5, 1e6][b], "batch", table=t)
+        assert e["chunk"] >= 1 and e["iters"] % 2 == 0
 
 
 def test_request_stream_is_seeded_and_in_range():

@@ -492,20 +492,20 @@ __device__ __forceinline__ bool inner_bits(uint32_t bits) {
 }
 
 // A one-scene arena's inner kernels: the scene's geometry and the order in
-// which workgroups visit its tiles. Tiles are visited by groups of kZCols
-// tile columns that run through all z: with q tiles per z-plane (rounded),
-// logical index L = ((jb * nzt) + zr) * kZCols + j visits tile
-// zr * q + jb * kZCols + j, so a tile's +-z neighbour tiles are visited
-// kZCols indices before and after it (the same XCD) instead of a whole plane
-// away, and stay in that XCD's L2. kZCols = 0 keeps the plain order.
-#ifndef VX_ZCOLS
-#define VX_ZCOLS 8
-#endif
-constexpr uint32_t kZCols = VX_ZCOLS;
+// which workgroups visit its tiles (its launch shape). Tiles are visited by
+// groups of zc = 2^zs tile columns that run through all z: with q tiles per
+// z-plane (rounded), logical index L = ((jb * nzt) + zr) * zc + j visits tile
+// zr * q + jb * zc + j, so a tile's +-z neighbour tiles are visited zc
+// indices before and after it (the same XCD) instead of a whole plane away,
+// and stay in that XCD's L2. q = 0 keeps the plain order. The shape changes
+// which workgroup runs which tile, never what a tile computes: results are
+// the same bit for bit under every shape.
+constexpr uint32_t kZColsDefault = 8;
 struct OneScene {
     SlotGeo g;
     uint32_t q, nzt, chunk, grid;
     uint32_t rb;  // the scene's bits carry B_RB
+    uint32_t zs;  // log2 of the z-column group
 };
 
 // EARLY (template argument of the inner kernels): test the scene's done flag
@@ -514,25 +514,28 @@ struct OneScene {
 // than kBigTiles tiles, where the saved traffic outweighs the wait for the
 // flag; at smaller sizes that wait lengthens every latency-bound launch.
 
-// Planes of fewer than 16 tiles keep the plain order (q = 0): their z
-// neighbours are already within a few tiles.
-inline OneScene make_one(const SlotGeo &g, bool round_robin = false) {
+// zcols (a power of two, or 0 for the plain order) groups the tile columns;
+// planes of fewer than 16 tiles keep the plain order (their z neighbours are
+// already within a few tiles).
+inline OneScene make_one(const SlotGeo &g, bool round_robin, uint32_t zcols) {
     OneScene o{};
     o.g = g;
     const uint64_t plane = uint64_t(g.nx) * g.ny;
     const uint32_t q = uint32_t(std::max<uint64_t>(1, (plane + kBlock / 2) / kBlock));
     uint64_t space = g.ntiles;
-    if (kZCols && q >= 16) {
+    if (zcols && q >= 16) {
         o.q = q;
+        o.zs = uint32_t(__builtin_ctz(zcols));
         o.nzt = (g.ntiles + q - 1) / q;
-        space = uint64_t((q + kZCols - 1) / kZCols) * kZCols * o.nzt;
+        space = uint64_t((q + zcols - 1) / zcols) * zcols * o.nzt;
     }
     // Logical tiles go to the XCDs either in contiguous eighths or round
-    // robin (logical tile L on XCD L % 8, its z neighbours L +- kZCols on the
-    // same XCD). Measured at 192x192x96: the SpMV is fastest with contiguous
-    // eighths, the kernels after it round robin (solves 3.6 % / 4.4 % shorter
-    // for block-Jacobi / red-black); at 64^3, where the working set stays in
-    // the Infinity Cache, mixing the two mappings costs up to 7 %.
+    // robin (logical tile L on XCD L % 8, its z neighbours L +- zc on the
+    // same XCD). Which is faster depends on the size (the tuned table
+    // chooses per kernel group; the default deal is measured at
+    // 192x192x96: the SpMV fastest with contiguous eighths, the kernels
+    // after it round robin; at 64^3, where the working set stays in the
+    // Infinity Cache, mixing the two mappings costs up to 7 %).
     if (round_robin) {
         o.chunk = 1;
         o.grid = uint32_t((space + kXcds - 1) / kXcds * kXcds);
@@ -544,13 +547,9 @@ inline OneScene make_one(const SlotGeo &g, bool round_robin = false) {
 }
 
 __device__ __forceinline__ uint32_t one_tile(uint32_t L, const OneScene &o) {
-    if constexpr (kZCols == 0) {
-        return L;
-    } else {
-        if (o.q == 0) return L;
-        const uint32_t j = L % kZCols, r = L / kZCols, zr = r % o.nzt, col = (r / o.nzt) * kZCols + j;
-        return col < o.q ? zr * o.q + col : kFreeTile;
-    }
+    if (o.q == 0) return L;
+    const uint32_t j = L & ((1u << o.zs) - 1u), r = L >> o.zs, zr = r % o.nzt, col = ((r / o.nzt) << o.zs) + j;
+    return col < o.q ? zr * o.q + col : kFreeTile;
 }
 
 // The tile of an inner-iteration kernel. ONE: the arena holds a single scene
@@ -2257,6 +2256,9 @@ struct vx_batch {
     bool single = false;  // one-scene arena of a vx_scene (never re-admits)
     uint32_t hw_tiles = 0;  // one past the highest tile in use
     uint32_t iters_per_tick = vx::kInnerChunk;
+    // Launch shape of a one-scene arena's inner kernels (vx_scene_set_shape).
+    uint32_t zcols = vx::kZColsDefault;
+    uint32_t deal = VX_DEAL_AUTO;
     // Bits and launch grid of the last spread_bits enqueued; a tick with the
     // same bits and grid and an unchanged tile table skips it.
     std::vector<uint32_t> spread_last;
@@ -2377,6 +2379,15 @@ void launch_iteration(vx_batch *b, bool rb, const OneScene &one, const dim3 &gri
 // One inner chunk: the refinement residual of scenes whose inner solve has
 // ended (device-decided), then the arena's iterations per tick. Enqueued on
 // the library stream (captured into a graph, or launched directly).
+// The XCD deal of a one-scene arena's SpMV and of the kernels after it.
+// Default: the SpMV in contiguous eighths; the others round robin for scenes
+// of more than kBigTiles tiles (their inputs come from HBM anyway) and
+// contiguous below, so the data a kernel reads was produced on its own XCD.
+bool deal_spmv_rr(const vx_batch *b) { return b->deal == VX_DEAL_ROUND_ROBIN; }
+bool deal_update_rr(const vx_batch *b, bool big) {
+    return b->deal == VX_DEAL_AUTO ? big : b->deal != VX_DEAL_CONTIGUOUS;
+}
+
 void enqueue_inner(vx_batch *b, int f64, bool rb, uint32_t tiles, bool big) {
     Context &ctx = context();
     const dim3 grid(tiles), blk(kBlock), sg(b->max_slots);
@@ -2392,12 +2403,9 @@ void enqueue_inner(vx_batch *b, int f64, bool rb, uint32_t tiles, bool big) {
         if (big) hipLaunchKernelGGL(scene_reduce<float>, sg, blk, 0, ctx.stream, b->A, int(kRz), 2, 0);
     }
     // A one-scene arena (vx_scene_*) never changes its scene: its geometry
-    // goes into the kernel arguments.
-    // Large scenes: the kernels after the SpMV deal their tiles round robin
-    // (their inputs come from HBM anyway); smaller ones keep one mapping for
-    // every kernel, so the data a kernel reads was produced on its own XCD.
-    OneScene one = b->single ? make_one(b->slots[0].geo) : OneScene{};
-    OneScene one_u = b->single ? make_one(b->slots[0].geo, big) : OneScene{};
+    // goes into the kernel arguments, with its launch shape.
+    OneScene one = b->single ? make_one(b->slots[0].geo, deal_spmv_rr(b), b->zcols) : OneScene{};
+    OneScene one_u = b->single ? make_one(b->slots[0].geo, deal_update_rr(b, big), b->zcols) : OneScene{};
     one.rb = one_u.rb = rb ? 1u : 0u;
     const dim3 gi(b->single ? one.grid : tiles), gu(b->single ? one_u.grid : tiles);
     for (uint32_t t = 0; t < b->iters_per_tick; ++t) {
@@ -3376,6 +3384,21 @@ extern "C" int32_t vx_scene_set_schedule(vx_scene *sc, uint32_t iterations_per_t
     return set_schedule(sc->batch, 0, iterations_per_tick);
 }
 
+extern "C" int32_t vx_scene_set_shape(vx_scene *sc, uint32_t zcols, uint32_t deal) {
+    clear_error();
+    if (int32_t c = scene_ok(sc)) return c;
+    if (zcols > 64 || (zcols & (zcols - 1)))
+        return fail(VX_ERR_INVALID_ARGUMENT, "z-column group %u is not 0 or a power of two up to 64", zcols);
+    if (deal > VX_DEAL_ROUND_ROBIN) return fail(VX_ERR_INVALID_ARGUMENT, "unknown XCD deal %u", deal);
+    Lock lock;
+    vx_batch *b = sc->batch;
+    if (int32_t c = drain(b)) return c;
+    b->zcols = zcols;
+    b->deal = deal;
+    reset_graphs(b);
+    return VX_OK;
+}
+
 extern "C" int32_t vx_scene_destroy(vx_scene *sc) {
     clear_error();
     if (!sc) return VX_OK;
@@ -3814,7 +3837,7 @@ extern "C" int32_t vx_scene_time_spmv(vx_scene *sc, uint32_t reps, double *out_u
     hipLaunchKernelGGL(spread_bits, dim3((b->cap_tiles + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx.stream, b->A, BitsArg{});
     b->spread_dirty = true;
     VX_HIP(hipGetLastError());
-    const OneScene one = make_one(g);
+    const OneScene one = make_one(g, deal_spmv_rr(b), b->zcols);
     const bool early = g.ntiles > kBigTiles;  // as the solve's chunks launch it
     // The launches run back to back inside one graph, as the solve runs them,
     // between two events on the library stream.

@@ -19,6 +19,11 @@ VX_ERR_DEVICE = 7
 VX_EMPTY = 0xFFFFFFFF
 VX_LAUNCH_GRAPHS = 0
 VX_LAUNCH_DIRECT = 1
+# Launch shape XCD deals (vx_scene_set_shape)
+VX_DEAL_AUTO = 0
+VX_DEAL_CONTIGUOUS = 1
+VX_DEAL_SPLIT = 2
+VX_DEAL_ROUND_ROBIN = 3
 
 VX_SOLVE_MIXED = 0
 VX_SOLVE_F64 = 1
@@ -176,6 +181,7 @@ SIGNATURES = [
     ("vx_scene_step", ctypes.c_int32,
      [ctypes.c_void_p, ctypes.POINTER(StepParams), ctypes.POINTER(StepReport)]),
     ("vx_scene_set_schedule", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint32]),
+    ("vx_scene_set_shape", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32]),
     ("vx_batch_set_schedule", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32]),
     ("vx_batch_bytes_per_row", ctypes.c_uint64, []),
     ("vx_batch_create", ctypes.c_int32,

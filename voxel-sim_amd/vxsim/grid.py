@@ -13,6 +13,7 @@ import numpy as np
 
 from . import lib
 from ._abi import (FractureParams, KernelTimes, SceneDesc, SolveParams, SolveReport, StepParams, StepReport,
+                   VX_DEAL_AUTO, VX_DEAL_CONTIGUOUS, VX_DEAL_ROUND_ROBIN, VX_DEAL_SPLIT,
                    VX_LAUNCH_DIRECT, VX_LAUNCH_GRAPHS, VX_PRECOND_BLOCK_JACOBI,
                    VX_PRECOND_RED_BLACK, VX_SOLVE_F64, VX_SOLVE_MIXED, c_dp, c_fp, c_u8p, c_u32p)
 from .solver import SolverError, check
@@ -110,8 +111,9 @@ def download_bonds(fn, *args, n: int) -> dict:
 
 class GridScene:
     def __init__(self, scene, tuned: bool = True):
-        """Upload `scene`; with `tuned`, the iterations per tick come from the
-        schedule table for the scene's size (vxsim.tune)."""
+        """Upload `scene`; with `tuned`, the iterations per tick and the
+        launch shape (z-column group, XCD deal) come from the schedule table
+        for the scene's size (vxsim.tune)."""
         desc, keep = scene_desc(scene)
         nx, ny, nz = scene.dims
         handle = ctypes.c_void_p()
@@ -123,7 +125,9 @@ class GridScene:
         self._kinematic = np.asarray(scene.kinematic, np.uint8).copy()
         if tuned:
             from . import tune
-            self.set_schedule(tune.lookup(scene.n, kind="single")["iters"])
+            t = tune.lookup(scene.n, kind="single")
+            self.set_schedule(t["iters"])
+            self.set_shape(t["zcols"], t["deal"])
 
     def close(self):
         if getattr(self, "handle", None):
@@ -246,6 +250,12 @@ class GridScene:
     def set_schedule(self, iterations_per_tick: int):
         """Inner PCG iterations per scheduling tick (graph replay)."""
         check(lib().vx_scene_set_schedule(self.handle, int(iterations_per_tick)))
+
+    def set_shape(self, zcols: int = 8, deal: int = VX_DEAL_AUTO):
+        """vx_scene_set_shape: the tile order of the inner kernels (z-column
+        group, 0 or a power of two) and their XCD deal (VX_DEAL_*); results
+        are identical under every shape."""
+        check(lib().vx_scene_set_shape(self.handle, int(zcols), int(deal)))
 
     def solve_timed(self, params=None, **kw) -> dict:
         """vx_scene_solve_timed: one solve with every bond SpMV and update
