@@ -3433,15 +3433,21 @@ extern "C" int32_t vx_scene_set_tool(vx_scene *sc, const vx_tool_desc *d) {
     const uint32_t nc = d->n_cells, ntiles = b->slots[sc->slot].geo.ntiles;
     std::vector<double> cells(3 * size_t(nc));
     double radius = 0.0;
+    ToolDev T{};
+    for (int k = 0; k < 3; ++k) {
+        T.cell_lo[k] = INFINITY;
+        T.cell_hi[k] = -INFINITY;
+    }
     for (uint32_t c = 0; c < nc; ++c) {
         for (int k = 0; k < 3; ++k) {
             cells[3 * c + k] = double(d->cells[3 * c + k]);
             if (!std::isfinite(cells[3 * c + k])) return fail(VX_ERR_INVALID_ARGUMENT, "non-finite tool cell");
+            T.cell_lo[k] = std::min(T.cell_lo[k], cells[3 * c + k]);
+            T.cell_hi[k] = std::max(T.cell_hi[k], cells[3 * c + k]);
         }
         radius = std::max(radius, std::sqrt((cells[3 * c] * cells[3 * c] + cells[3 * c + 1] * cells[3 * c + 1]) +
                                             cells[3 * c + 2] * cells[3 * c + 2]));
     }
-    ToolDev T{};
     T.mass = d->mass;
     T.friction = d->friction;
     T.radius = radius;

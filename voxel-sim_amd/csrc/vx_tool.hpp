@@ -41,6 +41,7 @@ struct ToolDev {
     double inv_inertia[3], pos[3], rot[4], vel[3], ang[3], force[3], torque[3];
     double m_local[3], m_world[3], m_dir[3], m_rot[4];
     double pos0[3], rot0[4], box_lo[3], box_hi[3];  // pose at the advance's start, candidate box (body frame)
+    double cell_lo[3], cell_hi[3];                   // bounding box of the cells' centres (body frame)
     double h, lin_acc[3], reach;
     double friction_heat;
     // steps: microsteps of the substep's timed advance; micro: of the advance
@@ -253,15 +254,9 @@ __global__ void tool_prepare(ToolBufs B, double dt, double gx, double gy, double
     // advance by `reach`).
     T.reach = 2.0 * (ld + ad) + 2.0 * kToolCell;
     const double pad = T.reach + 4.2 * kToolCell;
-    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (uint32_t c = 0; c < T.n_cells; ++c)
-        for (int k = 0; k < 3; ++k) {
-            lo[k] = fmin(lo[k], B.cells[3 * c + k]);
-            hi[k] = fmax(hi[k], B.cells[3 * c + k]);
-        }
     for (int k = 0; k < 3; ++k) {
-        T.box_lo[k] = lo[k] - pad;
-        T.box_hi[k] = hi[k] + pad;
+        T.box_lo[k] = T.cell_lo[k] - pad;
+        T.box_hi[k] = T.cell_hi[k] + pad;
     }
     T.candidates = 0;
 }
