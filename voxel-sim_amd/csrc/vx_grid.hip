@@ -229,10 +229,22 @@ __device__ __forceinline__ void scene_load(double (&out)[N], const DevArena &A, 
         out[k] = threadIdx.x < g.ntiles ? part[threadIdx.x] : 0.0;
     }
     if (g.ntiles > kBlock) {
+        // Larger scenes (scene_reduce): up to sixteen loads in flight per
+        // thread and kind, added in the same order as one at a time.
+        constexpr uint32_t kAhead = 16;
+        for (uint32_t b = threadIdx.x + kBlock; b < g.ntiles; b += kAhead * kBlock) {
+            double x[N][kAhead];
 #pragma unroll
-        for (int k = 0; k < N; ++k) {
-            const double *part = A.partials + size_t(kind[k]) * A.cap_tiles + g.tile0;
-            for (uint32_t b = threadIdx.x + kBlock; b < g.ntiles; b += kBlock) out[k] += part[b];
+            for (int k = 0; k < N; ++k) {
+                const double *part = A.partials + size_t(kind[k]) * A.cap_tiles + g.tile0;
+#pragma unroll
+                for (uint32_t u = 0; u < kAhead; ++u) x[k][u] = b + u * kBlock < g.ntiles ? part[b + u * kBlock] : 0.0;
+            }
+#pragma unroll
+            for (int k = 0; k < N; ++k)
+#pragma unroll
+                for (uint32_t u = 0; u < kAhead; ++u)
+                    if (b + u * kBlock < g.ntiles) out[k] += x[k][u];
         }
     }
 }
