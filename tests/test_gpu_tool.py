@@ -136,3 +136,16 @@ def test_tool_arguments_and_removal(gpu, scenes):
         gs.set_tool(None)
         with pytest.raises(SolverError):
             gs.tool_state()
+        # 1e6 m/s needs 250,000 microsteps of 0.4 REV in a 1 ms substep: refused
+        # before any microstep runs, and the scene steps again once the tool is sane
+        fast = scenes.blade_tool(s, (4, 2, 4), speed=1e6)
+        gs.set_tool(fast)
+        cfg = s.config
+        step = dict(fixed_dt=cfg.fixed_dt, gravity=cfg.gravity, substeps=1, corotation_iterations=1,
+                    corotation_tolerance=cfg.corotation_tolerance)
+        with pytest.raises(SolverError) as err:
+            gs.step(**step)
+        assert err.value.status == 6  # VX_ERR_INVALID_ARGUMENT
+        gs.set_tool(t)
+        gs.step(**step)
+        assert gs.tool_state()["microsteps"] >= 1

@@ -3934,6 +3934,10 @@ extern "C" int32_t vx_scene_step(vx_scene *sc, const vx_step_params *prm, vx_ste
     h.status = VX_OK;
     h.warm_ok = warm_ok;
     h.phase = P_START;
+    if (b->tool) {
+        hipLaunchKernelGGL(tool_reset, dim3(1), dim3(1), 0, context().stream, b->tool->bufs);
+        VX_HIP(hipGetLastError());
+    }
     uint32_t running = 1;
     int32_t status = VX_OK;
     while (running) {

@@ -220,6 +220,11 @@ __device__ __forceinline__ bool tool_bounds(t3 center, const t3 (&axes)[3], int3
     return tool_key(center - ext, lo) && tool_key(center + ext, hi);
 }
 
+// Start of a step (one thread): the status bits stay set through the step's
+// advances (an advance with a bit set does nothing), so that a failure of any
+// substep's advance is the step's; vx_scene_set_tool starts from zero too.
+__global__ void tool_reset(ToolBufs B) { B.dev->status = 0; }
+
 // Start of an advance (one thread): constrain the pose, the microsteps of
 // the substep (src/tool.rs:198-214), the candidate box. `timed` is 0 for the
 // zero-time pass (dt = 0).
@@ -234,9 +239,8 @@ __global__ void tool_prepare(ToolBufs B, double dt, double gx, double gy, double
     const double ls = ld / (kToolCell * 0.4), as = ad / (kToolCell * 0.4);
     const double s = ceil(fmax(ls, as));
     uint32_t steps = s >= 1.0 ? (s < 4294967295.0 ? uint32_t(s) : 4294967295u) : 1u;
-    T.status = 0;
-    if (steps > kToolMaxSteps) {
-        T.status = TS_STEPS;
+    if (steps > kToolMaxSteps) {  // reported after the step; the advances do nothing until then
+        T.status |= TS_STEPS;
         steps = 1;
     }
     T.micro = steps;
