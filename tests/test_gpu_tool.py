@@ -118,6 +118,44 @@ def test_blade_cut_with_tool_matches_oracle(gpu, O, scenes):
     assert np.linalg.norm(out["positions"] - start - disp_o) <= 1e-6 * np.linalg.norm(disp_o)
 
 
+def _guided_jump(scenes, dims, tilt):
+    """A blade on a guide along -Z whose body is tilted by `tilt` rad from
+    the guide's normal plane, one REV into the side of a wood block: the
+    overlap correction moves a guided tool by pen / (n . d) along its guide
+    (src/tool.rs:150-162), here many REVs in one step, beyond the candidate
+    box the device gathered for the substep's travel."""
+    s = scenes.uniform_box(dims, "wood_builtin", grain=2, anisotropy=1.0)
+    s.config = scenes.StepConfig(fixed_dt=0.001, substeps=1, iterations=40, corotation_iterations=1,
+                                 solver_relative_tolerance=1e-5, corotation_tolerance=1e-5, gravity=(0.0, 0.0, -9.81))
+    lo = s.positions.astype(np.float64).min(axis=0)
+    hi = s.positions.astype(np.float64).max(axis=0)
+    c = 0.5 * (lo + hi)
+    d = np.array([-1.0, 0.0, -tilt])
+    t = vt.Tool.new(vt.BLADE, (4, 2, 4), [0, 0, 0], d / np.linalg.norm(d), 0.0)
+    xmax = (t.world_cells() - t.position)[:, 0].max()
+    t.position = np.array([lo[0] - xmax, c[1], c[2]])
+    t.mount_base([float(t.cells[:, 0].min()), 0.0, 0.0], [0.0, 0.0, -1.0])
+    return s, t
+
+
+@pytest.mark.parametrize("dims,tilt,cells", [((3, 3, 3), 0.1, 10), ((3, 3, 3), 0.02, 50), ((3, 3, 24), 0.02, 8)])
+def test_guided_jump_beyond_the_candidate_box_matches_oracle(gpu, O, scenes, dims, tilt, cells):
+    """The device advance notices the tool leaving its candidate box, undoes
+    the attempt, gathers the candidates of a box twice as large and runs it
+    again (up to 256 REVs of reach): the step equals the oracle's, which bins
+    every REV at every microstep."""
+    s, t = _guided_jump(scenes, dims, tilt)
+    sd, _, out, _ = _device(s, t, 1)
+    so, _, os_, _ = _oracle(O, s, t, 1)
+    moved = np.linalg.norm(so[0]["position"] - t.position) / 0.01
+    assert moved > 0.8 * cells
+    _same_tool(sd[0], so[0], max(1.0, float(np.abs(so[0]["velocity"]).max())), "jump")
+    start = s.positions.astype(np.float64)
+    disp_o = os_.positions - start
+    assert np.linalg.norm(out["positions"] - start - disp_o) <= 1e-6 * np.linalg.norm(disp_o)
+    assert np.abs(out["velocities"] - os_.velocities).max() <= 1e-6 * np.abs(os_.velocities).max()
+
+
 def test_tool_arguments_and_removal(gpu, scenes):
     from vxsim import grid
     from vxsim.solver import SolverError
@@ -128,6 +166,10 @@ def test_tool_arguments_and_removal(gpu, scenes):
             gs.tool_state()  # no tool yet
         bad = vt.Tool.new(vt.BLADE, (4, 2, 4), [0, 0, 1], [0, 0, -1], 1.0)
         bad.mass = -1.0
+        with pytest.raises(SolverError):
+            gs.set_tool(bad)
+        bad = scenes.blade_tool(s, (4, 2, 4), mounted=True)
+        bad.mount = (bad.mount[0], bad.mount[1], np.array([0.0, 0.0, -2.0]), bad.mount[3])  # not a unit guide
         with pytest.raises(SolverError):
             gs.set_tool(bad)
         gs.set_tool(t)

@@ -1916,216 +1916,322 @@ __device__ __forceinline__ void tool_cell_pairs(const ToolBufs &B, uint32_t C, t
             }
 }
 
-// Tool::advance for one substep (one workgroup; src/tool.rs:192-431); dt is
-// the substep length, 0 (timed = 0) for the zero-time pass.
-__global__ __launch_bounds__(kToolBlock) void tool_advance(DevArena A, ToolBufs B, double dt, uint32_t timed) {
+// How far any cell of the tool is from its place at the advance's start (an
+// upper bound: the centre's displacement plus the radius times the angle).
+__device__ __forceinline__ double tool_moved(const ToolDev &T) {
 #pragma clang fp contract(off)
-    __shared__ ToolDev T;
+    const tq q0 = tqld(T.rot0), q = tqld(T.rot);
+    const double qd = fmin(fabs(((q0.x * q.x + q0.y * q.y) + q0.z * q.z) + q0.w * q.w), 1.0);
+    return tlen(tld(T.pos) - tld(T.pos0)) + T.radius * 2.0 * acos(qd);
+}
+
+// Tool::advance for one substep (one workgroup; src/tool.rs:192-431); dt is
+// the substep length, 0 (timed = 0) for the zero-time pass; g: the scene.
+//
+// The candidates cover every REV a bin of a cell can hold while the cells
+// stay within `reach` of their starting place. Every pose change is checked
+// against that before the next contact search. A tool that leaves it (the
+// guided overlap correction moves a mounted tool by pen / (n . d) along its
+// guide, large when a contact's normal is nearly across the guide) undoes the
+// attempt -- the REV velocities it changed are restored, the tool state is
+// the one at the advance's start -- and the workgroup gathers the candidates
+// again with twice the reach, in index order as the filter kernels do, then
+// runs the advance again: the result is the one a large enough box gives at
+// once. After kToolAttempts the step fails (TS_MARGIN).
+constexpr int kToolAttempts = 8;
+__global__ __launch_bounds__(kToolBlock) void tool_advance(DevArena A, ToolBufs B, double dt, uint32_t timed, SlotGeo g) {
+#pragma clang fp contract(off)
+    __shared__ ToolDev T, T0;
     __shared__ double red_pen[kToolBlock / 64];
     __shared__ uint32_t red_cell[kToolBlock / 64];
-    __shared__ uint32_t done, lstatus;
+    __shared__ uint32_t wcnt[kToolBlock / 64];
+    __shared__ uint32_t done, lstatus, escaped, cand;
     if (threadIdx.x == 0) {
         T = *B.dev;
+        T0 = T;
         lstatus = 0;
+        escaped = 0;
+        cand = T.candidates;
     }
     __syncthreads();
     if (T.status) return;
     const size_t R = A.R;
-    const uint32_t C = T.candidates, nc = T.n_cells;
-    // Sort the candidates by (bin key, row): bitonic over the next power of two.
-    uint32_t P = 1;
-    while (P < C) P <<= 1;
-    for (uint32_t i = threadIdx.x; i < P; i += kToolBlock) {
-        B.key2[i] = i < C ? B.key[i] : ~uint64_t(0);
-        B.idx2[i] = i < C ? B.idx[i] : ~0u;
-    }
-    if (timed)
-        for (uint32_t i = threadIdx.x; i < 3 * C; i += kToolBlock) B.cv[i] = 0.0;
-    __syncthreads();
-    for (uint32_t k = 2; k <= P; k <<= 1)
-        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-            for (uint32_t i = threadIdx.x; i < P; i += kToolBlock) {
-                const uint32_t l = i ^ j;
-                if (l <= i) continue;
-                const uint64_t ka = B.key2[i], kb = B.key2[l];
-                const uint32_t ia = B.idx2[i], ib = B.idx2[l];
-                const bool greater = ka > kb || (ka == kb && ia > ib);
-                if (greater == ((i & k) == 0)) {
-                    B.key2[i] = kb;
-                    B.key2[l] = ka;
-                    B.idx2[i] = ib;
-                    B.idx2[l] = ia;
+    const uint32_t nc = T.n_cells;
+    uint32_t contacts = 0;  // thread 0
+    for (int attempt = 0;; ++attempt) {
+        const uint32_t C = cand;
+        // Sort the candidates by (bin key, row): bitonic over the next power of two.
+        uint32_t P = 1;
+        while (P < C) P <<= 1;
+        for (uint32_t i = threadIdx.x; i < P; i += kToolBlock) {
+            B.key2[i] = i < C ? B.key[i] : ~uint64_t(0);
+            B.idx2[i] = i < C ? B.idx[i] : ~0u;
+        }
+        __syncthreads();
+        for (uint32_t k = 2; k <= P; k <<= 1)
+            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                for (uint32_t i = threadIdx.x; i < P; i += kToolBlock) {
+                    const uint32_t l = i ^ j;
+                    if (l <= i) continue;
+                    const uint64_t ka = B.key2[i], kb = B.key2[l];
+                    const uint32_t ia = B.idx2[i], ib = B.idx2[l];
+                    const bool greater = ka > kb || (ka == kb && ia > ib);
+                    if (greater == ((i & k) == 0)) {
+                        B.key2[i] = kb;
+                        B.key2[l] = ka;
+                        B.idx2[i] = ib;
+                        B.idx2[l] = ia;
+                    }
                 }
+                __syncthreads();
+            }
+        // The candidates' velocities at the start (an escaped attempt restores
+        // them) and their contact velocities.
+        for (uint32_t e = threadIdx.x; e < C; e += kToolBlock) {
+            const uint32_t v = B.idx2[e];
+            for (int q = 0; q < 3; ++q) B.vsave[3 * size_t(e) + q] = A.vel[q * R + v];
+            if (timed)
+                for (int q = 0; q < 3; ++q) B.cv[3 * size_t(e) + q] = 0.0;
+        }
+        if (threadIdx.x == 0) contacts = 0;
+        __syncthreads();
+        for (uint32_t step = 0; step < T.micro; ++step) {
+            // Every thread is past its reads of `escaped` and `done` from the
+            // previous microstep before thread 0 writes them again.
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                const double h = T.h;
+                tst(T.vel, tld(T.vel) + tld(T.lin_acc) * h);
+                tst(T.ang, tld(T.ang) + tool_angular(T, tld(T.torque)) * h);
+                tst(T.pos, tld(T.pos) + tld(T.vel) * h);
+                const double angle = tlen(tld(T.ang)) * h;
+                if (angle > 1e-8) {
+                    const t3 ax = tdiv(tld(T.ang), tlen(tld(T.ang)));
+                    const double sn = sin(angle * 0.5), cs = cos(angle * 0.5);
+                    const tq r{ax.x * sn, ax.y * sn, ax.z * sn, cs};
+                    const tq q = tqld(T.rot);
+                    // Hamilton product r * q, then normalise
+                    tq m{r.w * q.x + r.x * q.w + r.y * q.z - r.z * q.y, r.w * q.y - r.x * q.z + r.y * q.w + r.z * q.x,
+                         r.w * q.z + r.x * q.y - r.y * q.x + r.z * q.w, r.w * q.w - r.x * q.x - r.y * q.y - r.z * q.z};
+                    const double len = sqrt(((m.x * m.x + m.y * m.y) + m.z * m.z) + m.w * m.w);
+                    const double inv = 1.0 / len;
+                    tqst(T.rot, tq{m.x * inv, m.y * inv, m.z * inv, m.w * inv});
+                }
+                if (tool_moved(T) > T.reach) escaped = 1;
             }
             __syncthreads();
-        }
-    uint32_t contacts = 0;
-    for (uint32_t step = 0; step < T.micro; ++step) {
-        if (threadIdx.x == 0) {
-            const double h = T.h;
-            tst(T.vel, tld(T.vel) + tld(T.lin_acc) * h);
-            tst(T.ang, tld(T.ang) + tool_angular(T, tld(T.torque)) * h);
-            tst(T.pos, tld(T.pos) + tld(T.vel) * h);
-            const double angle = tlen(tld(T.ang)) * h;
-            if (angle > 1e-8) {
-                const t3 ax = tdiv(tld(T.ang), tlen(tld(T.ang)));
-                const double sn = sin(angle * 0.5), cs = cos(angle * 0.5);
-                const tq r{ax.x * sn, ax.y * sn, ax.z * sn, cs};
-                const tq q = tqld(T.rot);
-                // Hamilton product r * q, then normalise
-                tq m{r.w * q.x + r.x * q.w + r.y * q.z - r.z * q.y, r.w * q.y - r.x * q.z + r.y * q.w + r.z * q.x,
-                     r.w * q.z + r.x * q.y - r.y * q.x + r.z * q.w, r.w * q.w - r.x * q.x - r.y * q.y - r.z * q.z};
-                const double len = sqrt(((m.x * m.x + m.y * m.y) + m.z * m.z) + m.w * m.w);
-                const double inv = 1.0 / len;
-                tqst(T.rot, tq{m.x * inv, m.y * inv, m.z * inv, m.w * inv});
-            }
-        }
-        __syncthreads();
-        const tq rot = tqld(T.rot);
-        const t3 axes[3] = {trot(rot, tv(1, 0, 0)), trot(rot, tv(0, 1, 0)), trot(rot, tv(0, 0, 1))};
-        // Contacts of every cell, in order, in parallel.
-        for (uint32_t c = threadIdx.x; c < nc; c += kToolBlock) {
-            const t3 center = tld(T.pos) + trot(rot, tld(B.cells + 3 * c));
-            uint32_t n = 0;
-            tool_cell_pairs(B, C, center, axes, [&](uint32_t e, uint32_t v) {
-                t3 nrm;
-                double pen;
-                if (!tool_sat(center, axes, arena_pos(A, v), nrm, pen)) return;
-                if (n < kToolCellSlots) {
-                    ToolContact &tc = B.con[size_t(c) * kToolCellSlots + n];
-                    tc.slot = e;
-                    tst(tc.n, nrm);
-                    tc.pen = pen;
-                } else {
-                    atomicOr(&lstatus, uint32_t(TS_SLOTS));
-                }
-                ++n;
-            });
-            B.ncon[c] = n < kToolCellSlots ? n : kToolCellSlots;
-        }
-        __syncthreads();
-        // The impulses, sequentially in the reference's order.
-        if (threadIdx.x == 0) {
-            t3 correction{0, 0, 0};
-            double max_pen = 0.0;
-            for (uint32_t c = 0; c < nc; ++c) {
-                const uint32_t nn = B.ncon[c];
-                for (uint32_t q = 0; q < nn; ++q) {
-                    const ToolContact tc = B.con[size_t(c) * kToolCellSlots + q];
-                    const uint32_t e = tc.slot, v = B.idx2[e];
-                    const t3 nrm = tld(tc.n);
-                    contacts += 1;
-                    const t3 p = arena_pos(A, v);
-                    const t3 arm = p - tld(T.pos);
-                    t3 vi{A.vel[v], A.vel[R + v], A.vel[2 * R + v]};
-                    const t3 rel = tld(T.vel) + tcross(tld(T.ang), arm) - vi;
-                    const double closing = tdot(rel, nrm);
-                    const bool kin = A.kinematic[v] != 0;
-                    const uint8_t tm = A.tmask[v];
-                    const t3 fr{(!kin && !(tm & 1)) ? 1.0 : 0.0, (!kin && !(tm & 2)) ? 1.0 : 0.0, (!kin && !(tm & 4)) ? 1.0 : 0.0};
-                    const double mi = double(A.mass[v]);
-                    const double wi = !kin ? tdot(thmul(nrm, fr), nrm) / mi : 0.0;
-                    const double denom = wi + tdot(nrm, tool_linear(T, nrm)) + tdot(nrm, tcross(tool_angular(T, tcross(arm, nrm)), arm));
-                    if (closing > 0.0) {
-                        const double ni = closing / fmax(denom, 1e-9);
-                        const t3 nv = nrm * ni;
-                        const t3 dv = tdiv(thmul(nv, fr), mi);
-                        vi = vi + dv;
-                        if (timed) tst(B.cv + 3 * size_t(e), tld(B.cv + 3 * size_t(e)) + dv);
-                        tst(T.vel, tld(T.vel) - tool_linear(T, nv));
-                        tst(T.ang, tld(T.ang) - tool_angular(T, tcross(arm, nv)));
-                        const t3 after = tld(T.vel) + tcross(tld(T.ang), arm) - vi;
-                        const t3 tangent = after - nrm * tdot(after, nrm);
-                        const double ts = tlen(tangent);
-                        const double rcp = 1.0 / ts;
-                        const t3 tdir = (isfinite(rcp) && rcp > 0.0) ? tangent * rcp : tv(0, 0, 0);
-                        const double tw = !kin ? tdot(thmul(tdir, fr), tdir) / mi : 0.0;
-                        const double tden = tw + tdot(tdir, tool_linear(T, tdir)) + tdot(tdir, tcross(tool_angular(T, tcross(arm, tdir)), arm));
-                        if (ts > 1e-6 && tden > 1e-9) {
-                            const double fm = fmin(ts / tden, T.friction * ni);
-                            const t3 f = tdir * fm;
-                            const t3 fdv = tdiv(thmul(f, fr), mi);
-                            vi = vi + fdv;
-                            if (timed) tst(B.cv + 3 * size_t(e), tld(B.cv + 3 * size_t(e)) + fdv);
-                            tst(T.vel, tld(T.vel) - tool_linear(T, f));
-                            tst(T.ang, tld(T.ang) - tool_angular(T, tcross(arm, f)));
-                            T.friction_heat += fmax(fm * ts - 0.5 * fm * fm * tden, 0.0);
-                        }
-                        A.vel[v] = vi.x;
-                        A.vel[R + v] = vi.y;
-                        A.vel[2 * R + v] = vi.z;
-                    }
-                    if (tc.pen > max_pen) {
-                        max_pen = tc.pen;
-                        correction = tool_correction(T, nrm, tc.pen);
-                    }
-                }
-            }
-            tst(T.pos, tld(T.pos) + correction);
-            tool_constrain(T);
-        }
-        __syncthreads();
-        // Re-projection until no overlap deeper than 1e-6 is left (at most
-        // six passes): the first deepest overlap in the reference's order.
-        for (int pass = 0; pass < 6; ++pass) {
-            double my_pen = 1e-6;
-            uint32_t my_cell = ~0u;
-            const tq rp = tqld(T.rot);
+            if (escaped) break;
+            const tq rot = tqld(T.rot);
+            const t3 axes[3] = {trot(rot, tv(1, 0, 0)), trot(rot, tv(0, 1, 0)), trot(rot, tv(0, 0, 1))};
+            // Contacts of every cell, in order, in parallel.
             for (uint32_t c = threadIdx.x; c < nc; c += kToolBlock) {
-                const t3 center = tld(T.pos) + trot(rp, tld(B.cells + 3 * c));
-                double cp = 1e-6;
-                t3 cn{0, 0, 0};
-                tool_cell_pairs(B, C, center, axes, [&](uint32_t, uint32_t v) {
+                const t3 center = tld(T.pos) + trot(rot, tld(B.cells + 3 * c));
+                uint32_t n = 0;
+                tool_cell_pairs(B, C, center, axes, [&](uint32_t e, uint32_t v) {
                     t3 nrm;
                     double pen;
                     if (!tool_sat(center, axes, arena_pos(A, v), nrm, pen)) return;
-                    if (pen > cp) {
-                        cp = pen;
-                        cn = nrm;
+                    if (n < kToolCellSlots) {
+                        ToolContact &tc = B.con[size_t(c) * kToolCellSlots + n];
+                        tc.slot = e;
+                        tst(tc.n, nrm);
+                        tc.pen = pen;
+                    } else {
+                        atomicOr(&lstatus, uint32_t(TS_SLOTS));
                     }
+                    ++n;
                 });
-                B.best_pen[c] = cp;
-                tst(B.con[size_t(c) * kToolCellSlots].n, cn);
-                if (cp > my_pen) {  // cells ascend per thread: the first deepest
-                    my_pen = cp;
-                    my_cell = c;
-                }
+                B.ncon[c] = n < kToolCellSlots ? n : kToolCellSlots;
             }
-            // Workgroup reduction: the deepest, the lowest cell among equals.
-            for (int off = 32; off > 0; off >>= 1) {
-                const double op = __shfl_down(my_pen, off);
-                const uint32_t oc = __shfl_down(my_cell, off);
-                if (op > my_pen || (op == my_pen && oc < my_cell)) {
-                    my_pen = op;
-                    my_cell = oc;
+            __syncthreads();
+            // The impulses, sequentially in the reference's order.
+            if (threadIdx.x == 0) {
+                t3 correction{0, 0, 0};
+                double max_pen = 0.0;
+                for (uint32_t c = 0; c < nc; ++c) {
+                    const uint32_t nn = B.ncon[c];
+                    for (uint32_t q = 0; q < nn; ++q) {
+                        const ToolContact tc = B.con[size_t(c) * kToolCellSlots + q];
+                        const uint32_t e = tc.slot, v = B.idx2[e];
+                        const t3 nrm = tld(tc.n);
+                        contacts += 1;
+                        const t3 p = arena_pos(A, v);
+                        const t3 arm = p - tld(T.pos);
+                        t3 vi{A.vel[v], A.vel[R + v], A.vel[2 * R + v]};
+                        const t3 rel = tld(T.vel) + tcross(tld(T.ang), arm) - vi;
+                        const double closing = tdot(rel, nrm);
+                        const bool kin = A.kinematic[v] != 0;
+                        const uint8_t tm = A.tmask[v];
+                        const t3 fr{(!kin && !(tm & 1)) ? 1.0 : 0.0, (!kin && !(tm & 2)) ? 1.0 : 0.0,
+                                    (!kin && !(tm & 4)) ? 1.0 : 0.0};
+                        const double mi = double(A.mass[v]);
+                        const double wi = !kin ? tdot(thmul(nrm, fr), nrm) / mi : 0.0;
+                        const double denom =
+                            wi + tdot(nrm, tool_linear(T, nrm)) + tdot(nrm, tcross(tool_angular(T, tcross(arm, nrm)), arm));
+                        if (closing > 0.0) {
+                            const double ni = closing / fmax(denom, 1e-9);
+                            const t3 nv = nrm * ni;
+                            const t3 dv = tdiv(thmul(nv, fr), mi);
+                            vi = vi + dv;
+                            if (timed) tst(B.cv + 3 * size_t(e), tld(B.cv + 3 * size_t(e)) + dv);
+                            tst(T.vel, tld(T.vel) - tool_linear(T, nv));
+                            tst(T.ang, tld(T.ang) - tool_angular(T, tcross(arm, nv)));
+                            const t3 after = tld(T.vel) + tcross(tld(T.ang), arm) - vi;
+                            const t3 tangent = after - nrm * tdot(after, nrm);
+                            const double ts = tlen(tangent);
+                            const double rcp = 1.0 / ts;
+                            const t3 tdir = (isfinite(rcp) && rcp > 0.0) ? tangent * rcp : tv(0, 0, 0);
+                            const double tw = !kin ? tdot(thmul(tdir, fr), tdir) / mi : 0.0;
+                            const double tden = tw + tdot(tdir, tool_linear(T, tdir)) +
+                                                tdot(tdir, tcross(tool_angular(T, tcross(arm, tdir)), arm));
+                            if (ts > 1e-6 && tden > 1e-9) {
+                                const double fm = fmin(ts / tden, T.friction * ni);
+                                const t3 f = tdir * fm;
+                                const t3 fdv = tdiv(thmul(f, fr), mi);
+                                vi = vi + fdv;
+                                if (timed) tst(B.cv + 3 * size_t(e), tld(B.cv + 3 * size_t(e)) + fdv);
+                                tst(T.vel, tld(T.vel) - tool_linear(T, f));
+                                tst(T.ang, tld(T.ang) - tool_angular(T, tcross(arm, f)));
+                                T.friction_heat += fmax(fm * ts - 0.5 * fm * fm * tden, 0.0);
+                            }
+                            A.vel[v] = vi.x;
+                            A.vel[R + v] = vi.y;
+                            A.vel[2 * R + v] = vi.z;
+                        }
+                        if (tc.pen > max_pen) {
+                            max_pen = tc.pen;
+                            correction = tool_correction(T, nrm, tc.pen);
+                        }
+                    }
                 }
+                tst(T.pos, tld(T.pos) + correction);
+                tool_constrain(T);
+                if (tool_moved(T) > T.reach) escaped = 1;
             }
-            if ((threadIdx.x & 63u) == 0) {
-                red_pen[threadIdx.x >> 6] = my_pen;
-                red_cell[threadIdx.x >> 6] = my_cell;
+            __syncthreads();
+            if (escaped) break;
+            // Re-projection until no overlap deeper than 1e-6 is left (at most
+            // six passes): the first deepest overlap in the reference's order.
+            for (int pass = 0; pass < 6; ++pass) {
+                double my_pen = 1e-6;
+                uint32_t my_cell = ~0u;
+                const tq rp = tqld(T.rot);
+                for (uint32_t c = threadIdx.x; c < nc; c += kToolBlock) {
+                    const t3 center = tld(T.pos) + trot(rp, tld(B.cells + 3 * c));
+                    double cp = 1e-6;
+                    t3 cn{0, 0, 0};
+                    tool_cell_pairs(B, C, center, axes, [&](uint32_t, uint32_t v) {
+                        t3 nrm;
+                        double pen;
+                        if (!tool_sat(center, axes, arena_pos(A, v), nrm, pen)) return;
+                        if (pen > cp) {
+                            cp = pen;
+                            cn = nrm;
+                        }
+                    });
+                    tst(B.con[size_t(c) * kToolCellSlots].n, cn);
+                    if (cp > my_pen) {  // cells ascend per thread: the first deepest
+                        my_pen = cp;
+                        my_cell = c;
+                    }
+                }
+                // Workgroup reduction: the deepest, the lowest cell among equals.
+                for (int off = 32; off > 0; off >>= 1) {
+                    const double op = __shfl_down(my_pen, off);
+                    const uint32_t oc = __shfl_down(my_cell, off);
+                    if (op > my_pen || (op == my_pen && oc < my_cell)) {
+                        my_pen = op;
+                        my_cell = oc;
+                    }
+                }
+                if ((threadIdx.x & 63u) == 0) {
+                    red_pen[threadIdx.x >> 6] = my_pen;
+                    red_cell[threadIdx.x >> 6] = my_cell;
+                }
+                __syncthreads();
+                if (threadIdx.x == 0) {
+                    double bp = red_pen[0];
+                    uint32_t bc = red_cell[0];
+                    for (uint32_t w = 1; w < kToolBlock / 64; ++w)
+                        if (red_pen[w] > bp || (red_pen[w] == bp && red_cell[w] < bc)) {
+                            bp = red_pen[w];
+                            bc = red_cell[w];
+                        }
+                    done = bp <= 1e-6 ? 1u : 0u;
+                    if (!done) {
+                        tst(T.pos, tld(T.pos) + tool_correction(T, tld(B.con[size_t(bc) * kToolCellSlots].n), bp));
+                        tool_constrain(T);
+                        if (tool_moved(T) > T.reach) escaped = 1;
+                    }
+                }
+                __syncthreads();
+                if (done || escaped) break;
+            }
+            if (escaped) break;
+        }
+        if (!escaped) break;
+        // Undo the attempt: the REV velocities it changed and the tool state.
+        for (uint32_t e = threadIdx.x; e < C; e += kToolBlock) {
+            const uint32_t v = B.idx2[e];
+            for (int q = 0; q < 3; ++q) A.vel[q * R + v] = B.vsave[3 * size_t(e) + q];
+        }
+        __syncthreads();  // every thread has read `escaped` before it is cleared
+        if (threadIdx.x == 0) {
+            const double reach = T.reach * 2.0;
+            T = T0;
+            T.reach = reach;
+            const double pad = reach + 4.2 * kToolCell;
+            for (int k = 0; k < 3; ++k) {
+                T.box_lo[k] = T.cell_lo[k] - pad;
+                T.box_hi[k] = T.cell_hi[k] + pad;
+            }
+            escaped = 0;
+            cand = 0;
+            if (attempt + 1 == kToolAttempts) lstatus |= TS_MARGIN;
+        }
+        __syncthreads();
+        if (lstatus & TS_MARGIN) break;
+        // The candidates of the larger box, in index order.
+        for (uint32_t base = 0; base < g.n; base += kToolBlock) {
+            const uint32_t local = base + threadIdx.x;
+            bool in = false;
+            t3 p{0, 0, 0};
+            if (local < g.n) {
+                p = arena_pos(A, g.voff + local);
+                in = tool_in_box(T, p);
+            }
+            const uint64_t m = __ballot(in);
+            const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+            if (lane == 0) wcnt[w] = uint32_t(__popcll(m));
+            __syncthreads();
+            uint32_t at = cand + uint32_t(__popcll(m & ((uint64_t(1) << lane) - 1)));
+            for (uint32_t k = 0; k < w; ++k) at += wcnt[k];
+            if (in && at < kToolMaxCand) {
+                int32_t k3[3];
+                uint64_t key = ~uint64_t(0);
+                if (tool_key(p, k3))
+                    key = tool_pack(k3[0], k3[1], k3[2]);
+                else
+                    atomicOr(&lstatus, uint32_t(TS_KEY));
+                B.key[at] = key;
+                B.idx[at] = g.voff + local;
             }
             __syncthreads();
             if (threadIdx.x == 0) {
-                double bp = red_pen[0];
-                uint32_t bc = red_cell[0];
-                for (uint32_t w = 1; w < kToolBlock / 64; ++w)
-                    if (red_pen[w] > bp || (red_pen[w] == bp && red_cell[w] < bc)) {
-                        bp = red_pen[w];
-                        bc = red_cell[w];
-                    }
-                done = bp <= 1e-6 ? 1u : 0u;
-                if (!done) {
-                    tst(T.pos, tld(T.pos) + tool_correction(T, tld(B.con[size_t(bc) * kToolCellSlots].n), bp));
-                    tool_constrain(T);
-                }
+                uint32_t add = 0;
+                for (uint32_t k = 0; k < kToolBlock / 64; ++k) add += wcnt[k];
+                cand += add;
             }
             __syncthreads();
-            if (done) break;
         }
+        if (threadIdx.x == 0 && cand > kToolMaxCand) lstatus |= TS_CAND;
+        __syncthreads();
+        if (lstatus & (TS_CAND | TS_KEY)) break;
     }
     // Contact velocity: half a substep of drift, and the final velocities
     // (the zero-time pass discards it).
-    for (uint32_t e = threadIdx.x; timed && e < C; e += kToolBlock) {
+    const bool failed = lstatus & (TS_MARGIN | TS_CAND | TS_KEY);
+    for (uint32_t e = threadIdx.x; timed && !failed && e < cand; e += kToolBlock) {
         const t3 cv = tld(B.cv + 3 * size_t(e));
         if (cv.x == 0.0 && cv.y == 0.0 && cv.z == 0.0) continue;
         const uint32_t v = B.idx2[e];
@@ -2141,13 +2247,9 @@ __global__ __launch_bounds__(kToolBlock) void tool_advance(DevArena A, ToolBufs 
         A.drift[2 * R + v] += cv.z;
     }
     if (threadIdx.x == 0) {
-        // The candidates cover every REV a bin of a cell can hold while the
-        // cells stay within `reach` of their starting place.
-        const tq q0 = tqld(T.rot0), q = tqld(T.rot);
-        const double qd = fmin(fabs(((q0.x * q.x + q0.y * q.y) + q0.z * q.z) + q0.w * q.w), 1.0);
-        const double moved = tlen(tld(T.pos) - tld(T.pos0)) + T.radius * 2.0 * acos(qd);
         T.contacts = T.contacts + contacts;
-        T.status = T.status | lstatus | (moved > T.reach ? uint32_t(TS_MARGIN) : 0u);
+        T.candidates = cand;
+        T.status = T.status | lstatus;
         *B.dev = T;
     }
 }
@@ -2523,7 +2625,7 @@ int32_t launch_tick(vx_batch *b, Flight &f) {
         hipLaunchKernelGGL(tool_filter_count, grid, blk, 0, s, b->A, tb);
         hipLaunchKernelGGL(tool_filter_scan, dim3(1), dim3(kToolBlock), 0, s, tb, sh.geo.ntiles);
         hipLaunchKernelGGL(tool_filter_write, grid, blk, 0, s, b->A, tb);
-        hipLaunchKernelGGL(tool_advance, dim3(1), dim3(kToolBlock), 0, s, b->A, tb, dt, uint32_t(timed));
+        hipLaunchKernelGGL(tool_advance, dim3(1), dim3(kToolBlock), 0, s, b->A, tb, dt, uint32_t(timed), sh.geo);
     };
     // The tool advances over the substep after the prediction
     // (src/solver.rs:530-543).
@@ -3442,6 +3544,15 @@ extern "C" int32_t vx_scene_set_tool(vx_scene *sc, const vx_tool_desc *d) {
         return fail(VX_ERR_INVALID_ARGUMENT, "invalid tool description");
     for (int k = 0; k < 4; ++k)
         if (!std::isfinite(d->rotation[k])) return fail(VX_ERR_INVALID_ARGUMENT, "invalid tool rotation");
+    if (d->has_mount) {
+        // the guide direction is a unit vector or zero (a base fixed in place), as mount_base makes it
+        const double dd = (d->mount_direction[0] * d->mount_direction[0] + d->mount_direction[1] * d->mount_direction[1]) +
+                          d->mount_direction[2] * d->mount_direction[2];
+        bool ok = finite3(d->mount_local_base) && finite3(d->mount_world_base) && finite3(d->mount_direction) &&
+                  (dd == 0.0 || std::fabs(dd - 1.0) <= 1e-6);
+        for (int k = 0; k < 4; ++k) ok = ok && std::isfinite(d->mount_rotation[k]);
+        if (!ok) return fail(VX_ERR_INVALID_ARGUMENT, "invalid tool mount");
+    }
     const uint32_t nc = d->n_cells, ntiles = b->slots[sc->slot].geo.ntiles;
     std::vector<double> cells(3 * size_t(nc));
     double radius = 0.0;
@@ -3492,7 +3603,7 @@ extern "C" int32_t vx_scene_set_tool(vx_scene *sc, const vx_tool_desc *d) {
         {(void **)&tb.key, 8 * size_t(kToolMaxCand)}, {(void **)&tb.key2, 8 * size_t(kToolMaxCand)},
         {(void **)&tb.idx, 4 * size_t(kToolMaxCand)}, {(void **)&tb.idx2, 4 * size_t(kToolMaxCand)},
         {(void **)&tb.cv, 24 * size_t(kToolMaxCand)}, {(void **)&tb.con, sizeof(ToolContact) * kToolCellSlots * size_t(nc)},
-        {(void **)&tb.ncon, 4 * size_t(nc)}, {(void **)&tb.best_pen, 8 * size_t(nc)}, {(void **)&tb.best_seq, 4 * size_t(nc)},
+        {(void **)&tb.ncon, 4 * size_t(nc)}, {(void **)&tb.vsave, 24 * size_t(kToolMaxCand)},
     };
     size_t total = 0;
     for (auto &it : items) total += (it.bytes + 255) & ~size_t(255);

@@ -29,7 +29,7 @@
 // Arithmetic is FP64 without contraction, the operations in the reference's
 // order (glam's vector rotation and quaternion product).
 
-constexpr uint32_t kToolMaxCand = 1u << 16;   // candidate REVs per advance
+constexpr uint32_t kToolMaxCand = 1u << 18;   // candidate REVs per advance
 constexpr uint32_t kToolCellSlots = 32;       // contacts per tool cell per pass
 constexpr uint32_t kToolBlock = 512;
 constexpr double kToolCell = 0.01;            // src/scene.rs:10
@@ -61,8 +61,7 @@ struct ToolBufs {
     double *cv;             // contact velocity, 3 per sorted candidate
     ToolContact *con;       // kToolCellSlots per cell
     uint32_t *ncon;         // per cell
-    double *best_pen;       // per cell (projection passes)
-    uint32_t *best_seq;
+    double *vsave;          // velocities at the advance's start, 3 per sorted candidate
 };
 
 struct t3 {
