@@ -373,9 +373,10 @@ int32_t vx_scene_download_bond_forces(vx_scene *scene, double *last_force, doubl
  * velocities a zero-time advance applies impulses and projection once more
  * and discards its contact velocity (src/solver.rs:701-706). One tool per
  * vx_scene, advanced by vx_scene_step only (not by the stage calls); the
- * neighbourhood it can touch in one substep is limited to 65,536 REVs and
+ * neighbourhood it can touch in one substep is limited to 262,144 REVs and
  * each cell to 32 contacts (VX_ERR_TOO_LARGE beyond), a substep to 65,536
- * microsteps (VX_ERR_INVALID_ARGUMENT). */
+ * microsteps (VX_ERR_INVALID_ARGUMENT); a tool thrown further than 256 times
+ * its substep's margin fails the step with VX_ERR_DEVICE. */
 typedef struct vx_tool_desc {
     uint32_t n_cells;
     uint32_t has_mount;            /* 1: guide mount (Tool::mount_base, src/tool.rs:127-142) */
