@@ -122,7 +122,7 @@ def _guided_jump(scenes, dims, tilt):
     """A blade on a guide along -Z whose body is tilted by `tilt` rad from
     the guide's normal plane, one REV into the side of a wood block: the
     overlap correction moves a guided tool by pen / (n . d) along its guide
-    (src/tool.rs:150-162), here many REVs in one step, beyond the candidate
+    (src/tool.rs:169-177), here many REVs in one step, beyond the candidate
     box the device gathered for the substep's travel."""
     s = scenes.uniform_box(dims, "wood_builtin", grain=2, anisotropy=1.0)
     s.config = scenes.StepConfig(fixed_dt=0.001, substeps=1, iterations=40, corotation_iterations=1,
