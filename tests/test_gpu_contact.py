@@ -285,3 +285,18 @@ def test_negative_zero_on_kinematic_voxels_next_to_movers():
             assert np.array_equal(np.signbit(a[near, axis]), np.signbit(b[near, axis]))
         flipped += np.count_nonzero(~np.signbit(po[near, axis]))
     assert flipped > 0
+
+
+def test_quiet_state_is_settled_in_one_parallel_pass():
+    """A state the projection leaves unmoved (a cracked lattice box projected
+    twice by the oracle: one pass, touching contacts only) is settled by
+    evaluating every pair once, all at once: no window runs, and positions,
+    previous positions and contacts equal the oracle's bit for bit."""
+    s = scenes.contact_box((24, 24, 16), seed=4, jitter=0.0, sag=0.0, movers=0.0, broken=0.1)
+    targets = s.pop("targets")
+    for _ in range(2):
+        po, qo, _, _ = oracle.contact_project(**s, targets=targets)
+        s["positions"], s["previous"] = po, qo
+    rep, st = check_parity(s, targets=targets, label="quiet state")
+    assert st["passes"] == 1 and st["contacts"] > 1000
+    assert rep["rounds"] == 0 and rep["evaluated"] == rep["pairs"]

@@ -548,7 +548,8 @@ __device__ __forceinline__ void slot_counts(const bool (&f)[NF][kR], uint32_t (*
 // Control words of one call, in device memory (the passes are separate
 // launches enqueued without host round trips).
 enum : int {
-    C_DONE = 0, C_OK, C_CHANGED, C_BAD, C_PASSES, C_WINDOWS, C_RESTORE, C_REDONE, C_EVALS, C_CANDS, C_FULL, C_PREV, C_WORDS
+    C_DONE = 0, C_OK, C_CHANGED, C_BAD, C_PASSES, C_WINDOWS, C_RESTORE, C_REDONE, C_EVALS, C_CANDS, C_FULL, C_PREV,
+    C_ANYSEP, C_WORDS
 };
 
 // Workgroup minimum, broadcast; the two LDS rows alternate as in slot_counts.
@@ -891,6 +892,39 @@ __global__ void k_restore_pairs(uint32_t P, const uint32_t *ctl, uint8_t *__rest
 
 __global__ void k_restore_done(uint32_t *ctl) { ctl[C_RESTORE] = 0; }
 
+// A first pass in which no pair separates moves nothing, so every pair's
+// outcome is its outcome on the starting positions, and `changed` stays false:
+// the projection ends after it (src/collision.rs:105-107). k_precheck
+// evaluates every pair on the starting positions at once and keeps its
+// contact code in code_s; if none separates, k_quiet_pairs / k_quiet_done
+// record those contacts and end the call, and the window passes return at
+// once. Otherwise nothing here is used (code_s is the later passes' snapshot,
+// written before they read it).
+__global__ void k_precheck(uint32_t P, const uint64_t *__restrict__ pairs, const float *rec, uint8_t *__restrict__ code_s,
+                           uint32_t *ctl) {
+    const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    bool sep = false;
+    if (m < P) {
+        const uint64_t key = pairs[m];
+        const Outcome o = evaluate(uint32_t(key >> 32), uint32_t(key), rec);
+        code_s[m] = o.code;
+        sep = o.sep;
+    }
+    const int c = __syncthreads_count(sep);
+    if (threadIdx.x == 0 && c) atomicOr(ctl + C_ANYSEP, 1u);
+}
+__global__ void k_quiet_pairs(uint32_t P, const uint32_t *ctl, const uint8_t *__restrict__ code_s,
+                              uint8_t *__restrict__ code) {
+    const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= P || ctl[C_ANYSEP]) return;
+    code[m] = code_s[m];
+}
+__global__ void k_quiet_done(uint32_t P, uint32_t *ctl) {
+    if (ctl[C_ANYSEP]) return;
+    ctl[C_DONE] = 1;
+    ctl[C_EVALS] += P;
+}
+
 __global__ void k_ctl_init(uint32_t *ctl) {
     for (int w = 0; w < C_WORDS; ++w) ctl[w] = 0;
     ctl[C_PASSES] = 1;
@@ -1119,12 +1153,16 @@ extern "C" int32_t vx_contact_project(uint32_t n, float *positions, float *previ
                                size_t(P), s));
         if ((st = tmp(tb, &d_tmp)) != VX_OK) return st;
         VX_HIP(hipEventRecord(ev[2], s));
+        const uint32_t vb = blocks_for(N, 256), pb = blocks_for(P, 256);
+        // a first pass that would move nothing, done at once
+        k_precheck<<<pb, 256, 0, s>>>(P, d_pairs, rec, d_code_s, d_ctl);
+        k_quiet_pairs<<<pb, 256, 0, s>>>(P, d_ctl, d_code_s, d_code);
+        k_quiet_done<<<1, 1, 0, s>>>(P, d_ctl);
         // pass 1 over every pair, then passes 2..6 over candidates (k_pass_begin stops them
         // once a pass moved nothing)
         k_project<<<1, kGs, 0, s>>>(d_pairs, nullptr, nullptr, P, rec, d_prev, d_code, d_sep, d_wfirst, d_wlast,
                                     d_ctl);
         VX_HIP(hipGetLastError());
-        const uint32_t vb = blocks_for(N, 256), pb = blocks_for(P, 256);
         for (int pass = 1; pass < kPasses; ++pass) {
             k_pass_begin<<<1, 1, 0, s>>>(d_ctl);
             k_pass_voxels<<<vb, 256, 0, s>>>(n, d_ctl, d_moved, d_wfirst, d_wlast, rec, d_pos_s, d_prev, d_prev_s);
