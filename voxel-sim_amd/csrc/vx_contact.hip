@@ -549,7 +549,7 @@ __device__ __forceinline__ void slot_counts(const bool (&f)[NF][kR], uint32_t (*
 // launches enqueued without host round trips).
 enum : int {
     C_DONE = 0, C_OK, C_CHANGED, C_BAD, C_PASSES, C_WINDOWS, C_RESTORE, C_REDONE, C_EVALS, C_CANDS, C_FULL, C_PREV,
-    C_ANYSEP, C_WORDS
+    C_FIRST, C_WORDS
 };
 
 // Workgroup minimum, broadcast; the two LDS rows alternate as in slot_counts.
@@ -581,9 +581,11 @@ __device__ __forceinline__ uint32_t block_min(uint32_t v, uint32_t (*mins)[kGsWa
 __global__ __launch_bounds__(kGs) void k_project(const uint64_t *__restrict__ pairs, const uint32_t *__restrict__ list,
                                                  const uint32_t *__restrict__ list_count, uint32_t npairs, float *rec,
                                                  float *prev, uint8_t *code, uint8_t *sep_last, uint32_t *wfirst,
-                                                 uint32_t *wlast, uint32_t *ctl) {
+                                                 uint32_t *wlast, uint32_t *ctl, int from_first = 0) {
     if (ctl[C_DONE] | ctl[C_OK] | (list ? ctl[C_FULL] : 0u)) return;
-    const uint32_t count = list ? *list_count : npairs;
+    // from_first: the first pass, from its first separating pair on (k_precheck)
+    const uint32_t first = (!list && from_first) ? min(ctl[C_FIRST], npairs) : 0u;
+    const uint32_t count = list ? *list_count : npairs - first;
     __shared__ uint32_t hkey[kHash], hw[kHash], hb[kHash];
     __shared__ uint64_t pre[2][kWin];
     __shared__ uint32_t prem[2][kWin];
@@ -610,7 +612,7 @@ __global__ __launch_bounds__(kGs) void k_project(const uint64_t *__restrict__ pa
             sl[r].k = s;
             sl[r].f = s < count ? S_LIVE : 0u;
             if (s < count) {
-                sl[r].m = list ? list[s] : s;
+                sl[r].m = list ? list[s] : first + s;
                 const uint64_t key = pairs[sl[r].m];
                 sl[r].i = uint32_t(key >> 32);
                 sl[r].j = uint32_t(key);
@@ -629,7 +631,7 @@ __global__ __launch_bounds__(kGs) void k_project(const uint64_t *__restrict__ pa
                 ahead_m[r] = 0;
                 ahead[r] = 0;
                 if (x < count - next) {
-                    ahead_m[r] = list ? list[next + x] : next + x;
+                    ahead_m[r] = list ? list[next + x] : first + next + x;
                     ahead[r] = pairs[ahead_m[r]];
                 }
             }
@@ -892,42 +894,45 @@ __global__ void k_restore_pairs(uint32_t P, const uint32_t *ctl, uint8_t *__rest
 
 __global__ void k_restore_done(uint32_t *ctl) { ctl[C_RESTORE] = 0; }
 
-// A first pass in which no pair separates moves nothing, so every pair's
-// outcome is its outcome on the starting positions, and `changed` stays false:
-// the projection ends after it (src/collision.rs:105-107). k_precheck
-// evaluates every pair on the starting positions at once and keeps its
-// contact code in code_s; if none separates, k_quiet_pairs / k_quiet_done
-// record those contacts and end the call, and the window passes return at
-// once. Otherwise nothing here is used (code_s is the later passes' snapshot,
-// written before they read it).
+// In the first pass, the pairs before the first one that separates move
+// nothing, so each of their outcomes is its outcome on the starting
+// positions. k_precheck evaluates every pair on the starting positions at
+// once, keeps its contact code in code_s and finds that first separating pair
+// (C_FIRST); k_first_pairs records the contacts of the pairs before it and
+// k_first_done counts them, and the first window pass starts there. A pass in
+// which no pair separates (C_FIRST = none) moves nothing and leaves `changed`
+// false: the projection ends with it (src/collision.rs:105-107). code_s is
+// the later passes' snapshot, written before they read it.
 __global__ void k_precheck(uint32_t P, const uint64_t *__restrict__ pairs, const float *rec, uint8_t *__restrict__ code_s,
                            uint32_t *ctl) {
     const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
-    bool sep = false;
+    uint32_t first = kNoMark;
     if (m < P) {
         const uint64_t key = pairs[m];
         const Outcome o = evaluate(uint32_t(key >> 32), uint32_t(key), rec);
         code_s[m] = o.code;
-        sep = o.sep;
+        if (o.sep) first = m;
     }
-    const int c = __syncthreads_count(sep);
-    if (threadIdx.x == 0 && c) atomicOr(ctl + C_ANYSEP, 1u);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) first = min(first, uint32_t(__shfl_xor(int(first), off, 64)));
+    if ((threadIdx.x & 63) == 0 && first != kNoMark) atomicMin(ctl + C_FIRST, first);
 }
-__global__ void k_quiet_pairs(uint32_t P, const uint32_t *ctl, const uint8_t *__restrict__ code_s,
+__global__ void k_first_pairs(uint32_t P, const uint32_t *ctl, const uint8_t *__restrict__ code_s,
                               uint8_t *__restrict__ code) {
     const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= P || ctl[C_ANYSEP]) return;
+    if (m >= P || m >= ctl[C_FIRST]) return;
     code[m] = code_s[m];
 }
-__global__ void k_quiet_done(uint32_t P, uint32_t *ctl) {
-    if (ctl[C_ANYSEP]) return;
-    ctl[C_DONE] = 1;
-    ctl[C_EVALS] += P;
+__global__ void k_first_done(uint32_t P, uint32_t *ctl) {
+    const uint32_t first = min(ctl[C_FIRST], P);
+    ctl[C_EVALS] += first;
+    if (first == P) ctl[C_DONE] = 1;
 }
 
 __global__ void k_ctl_init(uint32_t *ctl) {
     for (int w = 0; w < C_WORDS; ++w) ctl[w] = 0;
     ctl[C_PASSES] = 1;
+    ctl[C_FIRST] = kNoMark;
 }
 
 // Voxel records for the projection, and the positions back out of them.
@@ -1154,14 +1159,14 @@ extern "C" int32_t vx_contact_project(uint32_t n, float *positions, float *previ
         if ((st = tmp(tb, &d_tmp)) != VX_OK) return st;
         VX_HIP(hipEventRecord(ev[2], s));
         const uint32_t vb = blocks_for(N, 256), pb = blocks_for(P, 256);
-        // a first pass that would move nothing, done at once
+        // the first pass's pairs before its first separating one, at once
         k_precheck<<<pb, 256, 0, s>>>(P, d_pairs, rec, d_code_s, d_ctl);
-        k_quiet_pairs<<<pb, 256, 0, s>>>(P, d_ctl, d_code_s, d_code);
-        k_quiet_done<<<1, 1, 0, s>>>(P, d_ctl);
-        // pass 1 over every pair, then passes 2..6 over candidates (k_pass_begin stops them
-        // once a pass moved nothing)
+        k_first_pairs<<<pb, 256, 0, s>>>(P, d_ctl, d_code_s, d_code);
+        k_first_done<<<1, 1, 0, s>>>(P, d_ctl);
+        // the rest of pass 1 window by window, then passes 2..6 over candidates
+        // (k_pass_begin stops them once a pass moved nothing)
         k_project<<<1, kGs, 0, s>>>(d_pairs, nullptr, nullptr, P, rec, d_prev, d_code, d_sep, d_wfirst, d_wlast,
-                                    d_ctl);
+                                    d_ctl, 1);
         VX_HIP(hipGetLastError());
         for (int pass = 1; pass < kPasses; ++pass) {
             k_pass_begin<<<1, 1, 0, s>>>(d_ctl);
