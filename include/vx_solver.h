@@ -412,13 +412,15 @@ int32_t vx_scene_set_schedule(vx_scene *scene, uint32_t iterations_per_tick);
  * z neighbours run a few workgroups before and after it on the same XCD
  * (0: plain tile order; else a power of two up to 64; default 8); deal
  * spreads the tiles over the 8 XCDs in contiguous eighths or round robin,
- * for the SpMV and for the kernels after it. Results are identical bit for
- * bit under every shape; only the time changes. */
+ * for the SpMV and for the kernels after it; update_threads is the workgroup
+ * of the block-Jacobi update kernel per 512-row tile (512: one row per
+ * thread, the default; 256: two). Results are identical bit for bit under
+ * every shape; only the time changes. */
 #define VX_DEAL_AUTO 0        /* SpMV contiguous; the rest round robin above 1,024 tiles (default) */
 #define VX_DEAL_CONTIGUOUS 1  /* every inner kernel in contiguous eighths */
 #define VX_DEAL_SPLIT 2       /* SpMV contiguous, the kernels after it round robin */
 #define VX_DEAL_ROUND_ROBIN 3 /* every inner kernel round robin */
-int32_t vx_scene_set_shape(vx_scene *scene, uint32_t zcols, uint32_t deal);
+int32_t vx_scene_set_shape(vx_scene *scene, uint32_t zcols, uint32_t deal, uint32_t update_threads);
 
 /* ---- batched and served scenes (build extensions) ----------------------- */
 /*

@@ -1,6 +1,7 @@
 """Launch shapes of a one-scene arena (vx_scene_set_shape: the z-column
-group of the tile order and the XCD deal of the inner kernels) change which
-workgroup runs which tile, never what a tile computes: a step under every
+group of the tile order, the XCD deal of the inner kernels and the update
+kernel's threads per tile) change which workgroup runs which tile and how
+many rows each thread holds, never what a tile computes: a step under every
 shape equals the default one bit for bit, on a block whose z-planes span
 enough tiles for the grouping to act (18 tiles per plane), below and above
 the 1,024-tile size where the large-scene kernels take over."""
@@ -9,7 +10,7 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-SHAPES = [(0, 1), (4, 2), (16, 3), (1, 0), (64, 1)]
+SHAPES = [(0, 1, 512), (4, 2, 256), (16, 3, 512), (1, 0, 256), (64, 1, 256), (8, 0, 256)]
 
 
 def _step(scenes, dims, shape):
@@ -43,7 +44,7 @@ def test_shape_arguments_are_checked(gpu, scenes):
     from vxsim.solver import SolverError
     s = scenes.wood_block_blade(dims=(8, 8, 8))
     with grid.GridScene(s, tuned=False) as gs:
-        for bad in ((3, 0), (128, 0), (8, 4)):
+        for bad in ((3, 0, 512), (128, 0, 512), (8, 4, 512), (8, 0, 128), (8, 0, 1024)):
             with pytest.raises(SolverError):
                 gs.set_shape(*bad)
-        gs.set_shape(0, 3)
+        gs.set_shape(0, 3, 256)

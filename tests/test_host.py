@@ -11,8 +11,9 @@ def test_schedule_lookup_buckets_and_defaults(tmp_path):
     assert [tune.bucket_of(x) for x in (1, 1.9e4, 2e4, 1e5, 262144, 589824, 3.5e6, 1e9)] == [0, 0, 1, 1, 2, 3, 4, 4]
     assert [tune.bucket_of(x, "batch") for x in (1, 1.9e4, 2e4, 1e5, 3e6, 1e9)] == [0, 0, 1, 1, 3, 3]
     assert tune.lookup(5e4, "single", table={}) == tune.DEFAULT["single"]
-    t = {"single": {"1": {"iters": 32, "zcols": 0, "deal": 1}}, "batch": {"0": {"chunk": 8, "iters": 8}}}
-    assert tune.lookup(5e4, "single", table=t) == {"iters": 32, "zcols": 0, "deal": 1}
+    t = {"single": {"1": {"iters": 32, "zcols": 0, "deal": 1, "update_threads": 256}},
+         "batch": {"0": {"chunk": 8, "iters": 8}}}
+    assert tune.lookup(5e4, "single", table=t) == {"iters": 32, "zcols": 0, "deal": 1, "update_threads": 256}
     assert tune.lookup(5e3, "batch", table=t) == {"chunk": 8, "iters": 8}
     assert tune.lookup(5e5, "batch", table=t) == tune.DEFAULT["batch"]
     # a table written before the shape knobs keeps the default shape
@@ -32,6 +33,7 @@ def test_shipped_schedule_table_is_well_formed():
         assert e["iters"] % 2 == 0 and 2 <= e["iters"] <= 256
         assert e["zcols"] in (0, 1, 2, 4, 8, 16, 32, 64)
         assert e["deal"] in (0, 1, 2, 3)
+        assert e["update_threads"] in (256, 512)
     for b in range(3):
         e = tune.lookup([1e4, 1e5, 1e6][b], "batch", table=t)
         assert e["chunk"] >= 1 and e["iters"] % 2 == 0

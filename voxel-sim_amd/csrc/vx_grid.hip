@@ -219,20 +219,23 @@ __device__ __forceinline__ Rev ld_rev(const DevArena &A, uint32_t v) {
 // thread of the workgroup (fixed order). scene_load is the per-thread part;
 // block_sum_all completes it. inner_load is scene_load for the inner PCG
 // partials, which a large scene takes pre-summed from its state.
+// t: the thread's place in a 512-thread workgroup (a 256-thread workgroup
+// with two rows per thread loads for t and t + 256).
 template <int N>
-__device__ __forceinline__ void scene_load(double (&out)[N], const DevArena &A, const SlotGeo &g, const int (&kind)[N]) {
+__device__ __forceinline__ void scene_load(double (&out)[N], const DevArena &A, const SlotGeo &g, const int (&kind)[N],
+                                           uint32_t t = threadIdx.x) {
     // One load per thread covers scenes of up to kBlock tiles without a loop
     // (a loop would wait for each load before the row loads are issued).
 #pragma unroll
     for (int k = 0; k < N; ++k) {
         const double *part = A.partials + size_t(kind[k]) * A.cap_tiles + g.tile0;
-        out[k] = threadIdx.x < g.ntiles ? part[threadIdx.x] : 0.0;
+        out[k] = t < g.ntiles ? part[t] : 0.0;
     }
     if (g.ntiles > kBlock) {
         // Larger scenes (scene_reduce): up to sixteen loads in flight per
         // thread and kind, added in the same order as one at a time.
         constexpr uint32_t kAhead = 16;
-        for (uint32_t b = threadIdx.x + kBlock; b < g.ntiles; b += kAhead * kBlock) {
+        for (uint32_t b = t + kBlock; b < g.ntiles; b += kAhead * kBlock) {
             double x[N][kAhead];
 #pragma unroll
             for (int k = 0; k < N; ++k) {
@@ -249,13 +252,14 @@ __device__ __forceinline__ void scene_load(double (&out)[N], const DevArena &A, 
     }
 }
 template <int N>
-__device__ __forceinline__ void inner_load(double (&out)[N], const DevArena &A, const SlotGeo &g, const int (&kind)[N]) {
+__device__ __forceinline__ void inner_load(double (&out)[N], const DevArena &A, const SlotGeo &g, const int (&kind)[N],
+                                           uint32_t t = threadIdx.x) {
     if (g.ntiles > kBigTiles) {
 #pragma unroll
-        for (int k = 0; k < N; ++k) out[k] = threadIdx.x == 0 ? A.st[g.slot].red[kind[k]] : 0.0;
+        for (int k = 0; k < N; ++k) out[k] = t == 0 ? A.st[g.slot].red[kind[k]] : 0.0;
         return;
     }
-    scene_load<N>(out, A, g, kind);
+    scene_load<N>(out, A, g, kind, t);
 }
 template <int N>
 __device__ __forceinline__ void scene_sum(double (&out)[N], const DevArena &A, const SlotGeo &g, const int (&kind)[N],
@@ -1406,6 +1410,114 @@ __global__ __launch_bounds__(kBlock) void pcg_update(DevArena A, OneScene one, i
     tile_store<2>(part, A, c.t, out, lds);
 }
 
+// pcg_update with two rows per thread: a 256-thread workgroup per 512-row
+// tile of a block-Jacobi scene (a launch shape the autotuner may choose).
+// Thread t holds rows t and t + 256, so wave w holds the 64-row groups w and
+// w + 4; each group's wave sum goes to the slot the 512-thread form gives
+// that group and the slots are added in the same order, and the partials
+// re-summed at the head are split the same way: every value has the bits of
+// the 512-thread form.
+constexpr uint32_t kHalf = kBlock / 2;
+template <int N>
+__device__ __forceinline__ void block_sum_pair(double (&v0)[N], const double (&v1)[N], double (*lds)[kWaves]) {
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    lds_barrier();  // a previous use of lds is finished
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        const double s0 = wave_sum(v0[k]), s1 = wave_sum(v1[k]);
+        if (lane == 0) {
+            lds[k][wave] = s0;
+            lds[k][wave + kWaves / 2] = s1;
+        }
+    }
+    lds_barrier();
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        double s = lds[k][0];
+#pragma unroll
+        for (int w = 1; w < kWaves; ++w) s += lds[k][w];
+        v0[k] = s;
+    }
+}
+template <class S, bool ONE, bool EARLY>
+__global__ __launch_bounds__(kHalf) void pcg_update_pair(DevArena A, OneScene one, int odd) {
+    Tile c;
+    if (!inner_tile<S, ONE>(A, one, c)) return;
+    const uint32_t l1 = c.local + kHalf;
+    const bool act[2] = {c.row, l1 < c.g.n};
+    const uint32_t v[2] = {c.row ? c.v : c.g.voff, act[1] ? c.g.voff + l1 : c.g.voff};
+    __shared__ double lds[2][kWaves];
+    SlotState *st = A.st + c.slot;
+    const uint32_t done_in = st->done;
+    if constexpr (EARLY) {
+        if (done_in) return;  // the scene's inner solve has ended: no row loads
+    }
+    const uint32_t k = st->k_update;
+    const uint32_t seg = st->seg;
+    const double rho_s[2] = {st->rho[0], st->rho[1]};
+    double pap[1], pap1[1];
+    const int kinds[1] = {kPap};
+    inner_load<1>(pap, A, c.g, kinds, threadIdx.x);
+    inner_load<1>(pap1, A, c.g, kinds, threadIdx.x + kHalf);
+    __builtin_amdgcn_sched_barrier(0);
+    const size_t R = A.R;
+    const S *__restrict__ p = Inner<S>::p(A, !odd);
+    const S *__restrict__ ap = Inner<S>::ap(A);
+    S *__restrict__ d = Inner<S>::dv(A);
+    S *__restrict__ r = Inner<S>::r(A);
+    S *__restrict__ z = Inner<S>::z(A);
+    const S *__restrict__ dinv = Inner<S>::dinv(A);
+    S pv[2][3], av[2][3], rv[2][3], dv[2][3], Di[2][6];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const uint32_t ov = v[h] * uint32_t(sizeof(S));
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            pv[h][q] = ld_row(p + q * R, ov);
+            av[h][q] = ld_row(ap + q * R, ov);
+            rv[h][q] = ld_row(r + q * R, ov);
+            dv[h][q] = ld_row(d + q * R, ov);
+        }
+#pragma unroll
+        for (int q = 0; q < 6; ++q) Di[h][q] = ld_row(dinv + q * R, ov);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    block_sum_pair<1>(pap, pap1, lds);
+    if (done_in) return;
+    const S alpha = pap[0] > 1e-300 ? S(rho_s[k & 1] / pap[0]) : S(0);
+    const bool first = k == seg;  // d restarts after each replacement
+    if (c.t == c.g.tile0 && threadIdx.x == 0) {
+        st->k_spmv = k + 1;
+        st->iters = k + 1;
+    }
+    double part[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        if (!act[h]) continue;
+        const uint32_t ov = v[h] * uint32_t(sizeof(S));
+        S ri[3], zi[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            st_row(d + q * R, ov, fmaS<S>(alpha, pv[h][q], first ? S(0) : dv[h][q]));
+            ri[q] = fmaS<S>(-alpha, av[h][q], rv[h][q]);
+            st_row(r + q * R, ov, ri[q]);
+        }
+        zi[0] = fmaS<S>(Di[h][2], ri[2], fmaS<S>(Di[h][1], ri[1], Di[h][0] * ri[0]));
+        zi[1] = fmaS<S>(Di[h][4], ri[2], fmaS<S>(Di[h][3], ri[1], Di[h][1] * ri[0]));
+        zi[2] = fmaS<S>(Di[h][5], ri[2], fmaS<S>(Di[h][4], ri[1], Di[h][2] * ri[0]));
+#pragma unroll
+        for (int q = 0; q < 3; ++q) st_row(z + q * R, ov, zi[q]);
+        part[h][0] = __builtin_fma(double(ri[2]), double(zi[2]), __builtin_fma(double(ri[1]), double(zi[1]), double(ri[0]) * double(zi[0])));
+        part[h][1] = __builtin_fma(double(ri[2]), double(ri[2]), __builtin_fma(double(ri[1]), double(ri[1]), double(ri[0]) * double(ri[0])));
+    }
+    block_sum_pair<2>(part[0], part[1], lds);
+    if (threadIdx.x == 0) {
+        A.partials[size_t(kRz) * A.cap_tiles + c.t] = part[0][0];
+        A.partials[size_t(kRr) * A.cap_tiles + c.t] = part[0][1];
+    }
+}
+
 // Whether the residual kernels act on a scene: `forced` launches run the
 // first residual of a solve (B_RESID0); the launch at the head of an inner
 // chunk runs the refinement residual once the inner solve has ended and the
@@ -2373,6 +2485,7 @@ struct vx_batch {
     // Launch shape of a one-scene arena's inner kernels (vx_scene_set_shape).
     uint32_t zcols = vx::kZColsDefault;
     uint32_t deal = VX_DEAL_AUTO;
+    uint32_t update_threads = vx::kBlock;  // threads per tile of the block-Jacobi update
     // Bits and launch grid of the last spread_bits enqueued; a tick with the
     // same bits and grid and an unchanged tile table skips it.
     std::vector<uint32_t> spread_last;
@@ -2462,6 +2575,13 @@ void launch_iteration_t(vx_batch *b, const OneScene &one, const dim3 &grid, cons
     else
         launch_kernel(0, pcg_bond_spmv<S, ONE, RB, EARLY>, grid, blk, b->A, one, odd);
     reduce(kPap, 1);
+    if constexpr (ONE && !RB) {
+        if (b->update_threads == kHalf) {
+            launch_kernel(1, pcg_update_pair<S, ONE, EARLY>, grid_u, dim3(kHalf), b->A, one_u, odd);
+            reduce(kRz, 2);
+            return;
+        }
+    }
     launch_kernel(1, pcg_update<S, ONE, RB, EARLY>, grid_u, blk, b->A, one_u, odd);
     reduce(kRz, 2);
 }
@@ -3498,17 +3618,20 @@ extern "C" int32_t vx_scene_set_schedule(vx_scene *sc, uint32_t iterations_per_t
     return set_schedule(sc->batch, 0, iterations_per_tick);
 }
 
-extern "C" int32_t vx_scene_set_shape(vx_scene *sc, uint32_t zcols, uint32_t deal) {
+extern "C" int32_t vx_scene_set_shape(vx_scene *sc, uint32_t zcols, uint32_t deal, uint32_t update_threads) {
     clear_error();
     if (int32_t c = scene_ok(sc)) return c;
     if (zcols > 64 || (zcols & (zcols - 1)))
         return fail(VX_ERR_INVALID_ARGUMENT, "z-column group %u is not 0 or a power of two up to 64", zcols);
     if (deal > VX_DEAL_ROUND_ROBIN) return fail(VX_ERR_INVALID_ARGUMENT, "unknown XCD deal %u", deal);
+    if (update_threads != kBlock && update_threads != kHalf)
+        return fail(VX_ERR_INVALID_ARGUMENT, "update threads per tile %u is not %u or %u", update_threads, kBlock, kHalf);
     Lock lock;
     vx_batch *b = sc->batch;
     if (int32_t c = drain(b)) return c;
     b->zcols = zcols;
     b->deal = deal;
+    b->update_threads = update_threads;
     reset_graphs(b);
     return VX_OK;
 }

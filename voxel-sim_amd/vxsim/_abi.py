@@ -181,7 +181,7 @@ SIGNATURES = [
     ("vx_scene_step", ctypes.c_int32,
      [ctypes.c_void_p, ctypes.POINTER(StepParams), ctypes.POINTER(StepReport)]),
     ("vx_scene_set_schedule", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint32]),
-    ("vx_scene_set_shape", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32]),
+    ("vx_scene_set_shape", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]),
     ("vx_batch_set_schedule", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32]),
     ("vx_batch_bytes_per_row", ctypes.c_uint64, []),
     ("vx_batch_create", ctypes.c_int32,

@@ -127,7 +127,7 @@ class GridScene:
             from . import tune
             t = tune.lookup(scene.n, kind="single")
             self.set_schedule(t["iters"])
-            self.set_shape(t["zcols"], t["deal"])
+            self.set_shape(t["zcols"], t["deal"], t["update_threads"])
 
     def close(self):
         if getattr(self, "handle", None):
@@ -251,11 +251,12 @@ class GridScene:
         """Inner PCG iterations per scheduling tick (graph replay)."""
         check(lib().vx_scene_set_schedule(self.handle, int(iterations_per_tick)))
 
-    def set_shape(self, zcols: int = 8, deal: int = VX_DEAL_AUTO):
+    def set_shape(self, zcols: int = 8, deal: int = VX_DEAL_AUTO, update_threads: int = 512):
         """vx_scene_set_shape: the tile order of the inner kernels (z-column
-        group, 0 or a power of two) and their XCD deal (VX_DEAL_*); results
-        are identical under every shape."""
-        check(lib().vx_scene_set_shape(self.handle, int(zcols), int(deal)))
+        group, 0 or a power of two), their XCD deal (VX_DEAL_*) and the
+        update kernel's threads per 512-row tile (512 or 256); results are
+        identical under every shape."""
+        check(lib().vx_scene_set_shape(self.handle, int(zcols), int(deal), int(update_threads)))
 
     def solve_timed(self, params=None, **kw) -> dict:
         """vx_scene_solve_timed: one solve with every bond SpMV and update

@@ -5,19 +5,20 @@ launch shapes and keeps the fastest:
 
 * one-scene arenas (`GridScene`): the launch shape of the inner kernels --
   the z-column group of the tile order (0, 4, 8, 16) and the XCD deal of the
-  SpMV and of the kernels after it (vx_scene_set_shape) -- and then the
-  inner iterations per tick (the length of the hipGraph-captured chunk
-  between host reads of the scene state), timed over whole solves of the
-  bucket's representative scene;
+  SpMV and of the kernels after it, then the update kernel's workgroup per
+  512-row tile (512 threads with one row each or 256 with two)
+  (vx_scene_set_shape) -- and then the inner iterations per tick (the length
+  of the hipGraph-captured chunk between host reads of the scene state),
+  timed over whole solves of the bucket's representative scene;
 * multi-scene arenas (`Batch`, the serving loop): the workgroup-to-tile XCD
   interleave (tiles per XCD chunk) and iterations per tick, timed over one
   step of an arena of boxes.
 
 A candidate replaces the default only when it is at least 1 % faster. The
-tile itself (512 rows, one workgroup of 512 threads, SoA planes) is the
-arena's storage unit and does not vary; the shape decides which workgroup
-runs which tile, and the results are identical bit for bit under every
-shape.
+tile itself (512 rows, SoA planes, per-tile partial sums) is the arena's
+storage unit and does not vary; the shape decides which workgroup runs which
+tile and how many threads it has, and the results are identical bit for bit
+under every shape.
 
 The result is a JSON table. A table measured on MI355X ships next to this
 module (`schedule_table.json`); `lookup` reads it (or the file named by
@@ -35,7 +36,9 @@ import time
 TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "schedule_table.json")
 # Upper REV bounds of the size buckets (the last one is open).
 BUCKETS = {"single": [2e4, 2e5, 4e5, 1.5e6, math.inf], "batch": [2e4, 2e5, 2e6, math.inf]}
-DEFAULT = {"single": {"iters": 16, "zcols": 8, "deal": 0}, "batch": {"chunk": 16, "iters": 16}}
+DEFAULT = {"single": {"iters": 16, "zcols": 8, "deal": 0, "update_threads": 512},
+           "batch": {"chunk": 16, "iters": 16}}
+UPDATE_THREADS = (512, 256)
 ZCOLS = (0, 4, 8, 16)
 DEALS = (0, 1, 2, 3)  # VX_DEAL_AUTO, _CONTIGUOUS, _SPLIT, _ROUND_ROBIN
 _cache = {}
@@ -109,30 +112,36 @@ def tune_single(scene, iters_candidates=(8, 16, 32), reps: int = 2, log=None) ->
         gs.fit_rotations()
         gs.assemble(h)
 
-        def measure(iters, zcols, deal):
+        def measure(iters, zcols, deal, upd):
             gs.set_schedule(iters)
-            gs.set_shape(zcols, deal)
+            gs.set_shape(zcols, deal, upd)
             gs.solve(sp)  # captures the chunk graphs
             best = math.inf
             for _ in range(reps):
                 t = time.perf_counter()
                 gs.solve(sp)
                 best = min(best, time.perf_counter() - t)
-            key = "%d/%d/%d" % (iters, zcols, deal)
+            key = "%d/%d/%d/%d" % (iters, zcols, deal, upd)
             times[key] = best
             if log:
                 log("  %s: %.3f ms" % (key, 1e3 * best))
             return best
 
-        shape = (d["zcols"], d["deal"])
+        shape = (d["zcols"], d["deal"], d["update_threads"])
         t_best = measure(d["iters"], *shape)
         for zc in ZCOLS:
             for dl in DEALS:
-                if (zc, dl) == (d["zcols"], d["deal"]):
+                if (zc, dl) == shape[:2]:
                     continue
-                t = measure(d["iters"], zc, dl)
+                t = measure(d["iters"], zc, dl, shape[2])
                 if t < 0.99 * t_best:
-                    shape, t_best = (zc, dl), t
+                    shape, t_best = (zc, dl, shape[2]), t
+        for upd in UPDATE_THREADS:
+            if upd == shape[2]:
+                continue
+            t = measure(d["iters"], shape[0], shape[1], upd)
+            if t < 0.99 * t_best:
+                shape, t_best = (shape[0], shape[1], upd), t
         iters = d["iters"]
         for it in iters_candidates:
             if it == d["iters"]:
@@ -140,7 +149,7 @@ def tune_single(scene, iters_candidates=(8, 16, 32), reps: int = 2, log=None) ->
             t = measure(it, *shape)
             if t < 0.99 * t_best:
                 iters, t_best = it, t
-    return {"iters": iters, "zcols": shape[0], "deal": shape[1], "seconds": times}
+    return {"iters": iters, "zcols": shape[0], "deal": shape[1], "update_threads": shape[2], "seconds": times}
 
 
 def tune_batch(revs: float, chunk_candidates=(4, 8, 16, 32, 64), iters_candidates=(8, 16, 32),
@@ -191,9 +200,10 @@ def main():
         s = single_scene(b)
         log("single bucket %d (%s, %d REVs)" % (b, s.name, s.n))
         r = tune_single(s, log=log)
-        table["single"][str(b)] = {"iters": r["iters"], "zcols": r["zcols"], "deal": r["deal"]}
+        table["single"][str(b)] = {k: r[k] for k in ("iters", "zcols", "deal", "update_threads")}
         table["measured"]["single_%d" % b] = r["seconds"]
-        log("single bucket %d: iters %d, zcols %d, deal %d" % (b, r["iters"], r["zcols"], r["deal"]))
+        log("single bucket %d: iters %d, zcols %d, deal %d, update threads %d" %
+            (b, r["iters"], r["zcols"], r["deal"], r["update_threads"]))
     if not args.no_batch:
         reps = {0: 8e3, 1: 6e4, 2: 5e5}
         for b, revs in reps.items():
