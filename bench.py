@@ -45,6 +45,7 @@ TRAFFIC_FILE_LARGE = os.path.join(ROOT, "profiles", "pcg_bond_spmv_192x192x96_tr
 # up to 1e-3 relative L2 error against the FP64 oracle on these boxes (condition
 # numbers near 1e6); 1e-10 keeps it below 1e-6 (tests/test_gpu_configs.py).
 BATCH_TOL = 1e-10
+CUT_STEPS = 10          # steps of the cutting scene in the default line
 COPY_BYTES = 1 << 30      # calibration copy for the HBM byte counters
 
 
@@ -541,6 +542,11 @@ def cut_contact(scene, gs, prev_pos, targets, cpu: bool) -> dict:
 
 def cut_bench(steps: int, dims=(192, 192, 96), preconditioner: int = 0, contact: bool = False, cpu: bool = True,
               speed: float = 5.0):
+    print(json.dumps(cut_run(steps, dims, preconditioner, contact, cpu, speed)))
+
+
+def cut_run(steps: int, dims=(192, 192, 96), preconditioner: int = 0, contact: bool = False, cpu: bool = True,
+            speed: float = 5.0, quiet: bool = False) -> dict:
     """The cutting scene with fracture: the 192x192x96 wood block (bottom
     layer fixed, gravity) and the blade as a rigid tool, Tool::new's tapered
     40x2x20 blade (realised with a 5-REV spine) of iron, mounted on a guide
@@ -548,7 +554,8 @@ def cut_bench(steps: int, dims=(192, 192, 96), preconditioner: int = 0, contact:
     advances the tool per substep (and its zero-time pass), runs up to six
     corotation passes and the fracture pass (broken bonds leave the lattice
     before the next assembly). Prints one JSON line with the step time, the
-    solve throughput, the bonds broken and the tool's track per step."""
+    solve throughput, the bonds broken and the tool's track per step (quiet:
+    no line per step)."""
     from vxsim import grid, scenes
     blade = (40, 2, 20)
     scene = scenes.wood_block(dims=dims)
@@ -575,11 +582,12 @@ def cut_bench(steps: int, dims=(192, 192, 96), preconditioner: int = 0, contact:
             track.append({"z": round(float(st["position"][2]), 6), "vz": round(float(st["velocity"][2]), 4),
                           "contacts": int(st["contacts"]), "microsteps": int(st["microsteps"]),
                           "candidates": int(st["candidates"])})
-            print("cut step %d: %.1f ms, %d solves, %d inner iterations, %d bonds broken, fracture pass %.3f ms, "
-                  "tool z %.5f vz %.3f, %d contacts" %
-                  (k, 1000 * walls[-1], r["solves"], r["inner_iterations"], r["fracture"]["bonds_broken"],
-                   1000 * r["fracture"]["seconds"], st["position"][2], st["velocity"][2], st["contacts"]),
-                  file=sys.stderr, flush=True)
+            if not quiet:
+                print("cut step %d: %.1f ms, %d solves, %d inner iterations, %d bonds broken, fracture pass %.3f ms, "
+                      "tool z %.5f vz %.3f, %d contacts" %
+                      (k, 1000 * walls[-1], r["solves"], r["inner_iterations"], r["fracture"]["bonds_broken"],
+                       1000 * r["fracture"]["seconds"], st["position"][2], st["velocity"][2], st["contacts"]),
+                      file=sys.stderr, flush=True)
         if contact:
             contact_out = cut_contact(scene, gs, prev_pos, no_targets, cpu)
     dof_iters = sum(3.0 * r["active"] * (r["inner_iterations"] + r["outer_sweeps"]) for r in reps)
@@ -605,7 +613,7 @@ def cut_bench(steps: int, dims=(192, 192, 96), preconditioner: int = 0, contact:
             "data": "synthetic", "dtype": "f32 inner / f64 refinement, reductions, assembly, fracture and tool"}
     if contact_out is not None:
         line["contact_projection_after_last_step"] = contact_out
-    print(json.dumps(line))
+    return line
 
 
 def contact_bench(dims=(64, 64, 64), reps: int = 3, cpu: bool = True) -> dict:
@@ -695,6 +703,7 @@ def main():
                     help="requests of the default served leg (the first arrivals of the 200/s stream)")
     ap.add_argument("--no-serve", action="store_true", help="skip the default served leg")
     ap.add_argument("--no-large-roofline", action="store_true", help="skip the 192x192x96 in-solve SpMV roofline")
+    ap.add_argument("--no-cut", action="store_true", help="skip the default line's %d-step cut leg" % CUT_STEPS)
     ap.add_argument("--direct-launch", action="store_true",
                     help="launch the PCG chunks kernel by kernel instead of replaying captured hipGraphs "
                          "(identical results; for dispatch tracing)")
@@ -861,6 +870,10 @@ def main():
         print("bench: 192x192x96 roofline done", file=sys.stderr, flush=True)
     if not args.no_contact:
         extra["contact_projection"] = contact_bench(cpu=cpu_on)
+    if not args.no_cut:
+        # the cutting scene (the blade as the rigid tool at 5 m/s, fracture every step), bounded
+        extra["cut"] = cut_run(CUT_STEPS, cpu=False, quiet=True)
+        print("bench: cut leg done", file=sys.stderr, flush=True)
 
     cpu = None
     if cpu_on:

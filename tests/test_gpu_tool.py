@@ -118,6 +118,29 @@ def test_blade_cut_with_tool_matches_oracle(gpu, O, scenes):
     assert np.linalg.norm(out["positions"] - start - disp_o) <= 1e-6 * np.linalg.norm(disp_o)
 
 
+def test_rounded_tool_two_substeps_matches_oracle(gpu, O, scenes):
+    """The rounded tool (a 14-REV stone sphere, src/tool.rs:78-84) dropped at
+    3 m/s onto a 20x20x12 wood block, two substeps per step (each with its
+    advance and zero-time pass), three steps."""
+    import dataclasses
+    s = scenes.wood_block(dims=(20, 20, 12))
+    s.config = dataclasses.replace(s.config, substeps=2)
+    top = float(s.keys[:, 2].max() + 1) * 0.01
+    c = s.positions.astype(np.float64).mean(axis=0)
+    t = vt.Tool.new(vt.ROUNDED, (14, 14, 14), [0, 0, 0], [0, 0, -1], 3.0, friction=0.35)
+    low = (t.world_cells() - t.position)[:, 2].min()
+    t.position = np.array([c[0] + 0.0013, c[1] - 0.0021, top + 0.005 + 2e-4 - low])
+    sd, _, out, _ = _device(s, t, 3)
+    so, _, os_, _ = _oracle(O, s, t, 3)
+    for k in range(3):
+        _same_tool(sd[k], so[k], 3.0, ("step", k))
+    assert min(x["contacts"] for x in sd) > 50
+    start = s.positions.astype(np.float64)
+    disp_o = os_.positions - start
+    assert np.linalg.norm(out["positions"] - start - disp_o) <= 1e-6 * np.linalg.norm(disp_o)
+    assert np.abs(out["velocities"] - os_.velocities).max() <= 1e-6 * np.abs(os_.velocities).max()
+
+
 def _guided_jump(scenes, dims, tilt):
     """A blade on a guide along -Z whose body is tilted by `tilt` rad from
     the guide's normal plane, one REV into the side of a wood block: the
