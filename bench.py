@@ -864,7 +864,12 @@ def main():
         print("bench: batched legs done", file=sys.stderr, flush=True)
     if not args.no_serve:
         extra["served"] = served(args.serve_count, compare=not args.no_compare, cpu=cpu_on)
-        print("bench: served leg done", file=sys.stderr, flush=True)
+        # the same requests with the red-black preconditioner (half the iterations)
+        rbs = served(args.serve_count, compare=False, cpu=False, preconditioner=1)
+        rbs["p50_vs_block_jacobi"] = extra["served"]["p50_s"] / rbs["p50_s"]
+        rbs["p99_vs_block_jacobi"] = extra["served"]["p99_s"] / rbs["p99_s"]
+        extra["served_red_black"] = rbs
+        print("bench: served legs done", file=sys.stderr, flush=True)
     if not args.no_large_roofline:
         extra["roofline_hbm_out_of_cache"] = large_roofline()
         print("bench: 192x192x96 roofline done", file=sys.stderr, flush=True)
