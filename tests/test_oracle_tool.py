@@ -190,3 +190,30 @@ def test_oblique_guide_preserves_its_line_and_projected_force(O, scenes, mode):
     displacement = st["position"] + vt._qrot64(st["rotation"], base) - world_base
     assert np.linalg.norm(np.cross(displacement, direction)) < 2e-7
     assert np.linalg.norm(displacement) > 1e-3
+
+
+@pytest.mark.parametrize("shape", [vt.ROD, vt.ROUNDED])
+def test_rod_and_rounded_tool_invariants(shape):
+    """Tool::new's rod and rounded shapes (src/tool.rs:60-100), which the
+    reference's tests do not reach, by invariants that a shifted or transposed
+    grid would break: the cells centred on the origin, the mass as the cell
+    count x density x CELL^3, the sphere's inertia equal about every axis, and
+    the rod's inertia about its axis equal to the cells' distances from it
+    (plus each cube's own CELL^2 / 6) with a circular section (equal inertia
+    about the two other axes)."""
+    length = 12
+    t = vt.Tool.new(shape, (length, 5, 20), [0.3, 0.2, 0.1], [1, 0, 0], 0.0)
+    c = t.cells.astype(np.float64)
+    assert np.abs(c.mean(axis=0)).max() < 1e-6
+    density = {vt.ROD: 650.0, vt.ROUNDED: 2700.0}[shape]
+    assert t.mass == pytest.approx(len(c) * density * 1e-6, rel=1e-6)
+    inertia = 1.0 / t.inverse_inertia
+    m = t.mass / len(c)
+    if shape == vt.ROUNDED:
+        assert len(c) == 1472  # offsets (i + 0.5 - 7) with |p|^2 <= 49
+        assert np.ptp(inertia) <= 1e-5 * inertia.max()
+    else:
+        assert len(c) == 12 * length  # 12 of the 16 section offsets lie within radius 2
+        assert np.ptp(c[:, 0]) == pytest.approx((length - 1) * 0.01, rel=1e-6)
+        assert inertia[0] == pytest.approx(m * ((c[:, 1] ** 2 + c[:, 2] ** 2).sum() + len(c) * 1e-4 / 6), rel=1e-5)
+        assert inertia[1] == pytest.approx(inertia[2], rel=1e-5)
