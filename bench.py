@@ -626,16 +626,32 @@ def contact_bench(dims=(64, 64, 64), reps: int = 3, cpu: bool = True) -> dict:
     from vxsim import contact, scenes
     s = scenes.contact_box(dims, seed=2)
     targets = s.pop("targets")
+    contact.set_method(contact.AUTO)
     runs = [contact.project(**s, targets=targets) for _ in range(reps)]
     p, q, c, rep = runs[-1]
     dev_us = sorted(r[3]["device_us"] for r in runs)[reps // 2]
     proj_us = sorted(r[3]["projection_us"] for r in runs)[reps // 2]
     out = {"workload": "contact_box %dx%dx%d seed 2: %d voxels" % (dims + (len(p),)),
            "candidate_pairs": rep["candidate_pairs"], "pairs": rep["pairs"], "contacts": rep["contacts"],
-           "passes": rep["passes"], "windows": rep["rounds"], "device_ms": dev_us / 1e3,
+           "passes": rep["passes"], "relaxed_passes": rep["relaxed_passes"],
+           "relaxation_rounds": rep["relaxation_rounds"], "windows": rep["rounds"], "device_ms": dev_us / 1e3,
            "projection_ms": proj_us / 1e3, "broad_phase_ms": (dev_us - proj_us) / 1e3,
-           "us_per_window": proj_us / max(rep["rounds"], 1), "pair_evaluations": rep["evaluated"],
-           "passes_redone": rep["redone"], "pair_evaluations_per_s": rep["evaluated"] / (proj_us * 1e-6)}
+           "pair_evaluations": rep["evaluated"], "pair_evaluations_per_s": rep["evaluated"] / (proj_us * 1e-6)}
+    # the same call with every pass by the window walker (identical results)
+    contact.set_method(contact.WINDOWS)
+    try:
+        wr = [contact.project(**s, targets=targets) for _ in range(reps)]
+    finally:
+        contact.set_method(contact.AUTO)
+    wp, wq, wc, wrep = wr[-1]
+    w_us = sorted(r[3]["device_us"] for r in wr)[reps // 2]
+    out["windows_method"] = {"device_ms": w_us / 1e3, "windows": wrep["rounds"],
+                             "us_per_window": sorted(r[3]["projection_us"] for r in wr)[reps // 2] / max(wrep["rounds"], 1),
+                             "pair_evaluations": wrep["evaluated"], "passes_redone": wrep["redone"],
+                             "relaxation_speedup": w_us / dev_us,
+                             "same_result": bool(np.array_equal(wp.view(np.uint32), p.view(np.uint32))
+                                                 and np.array_equal(wq.view(np.uint32), q.view(np.uint32))
+                                                 and np.array_equal(wc["i"], c["i"]) and np.array_equal(wc["j"], c["j"]))}
     if cpu:
         from oracle import oracle
         t0 = time.perf_counter()

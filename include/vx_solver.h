@@ -526,10 +526,30 @@ typedef struct vx_contact_report {
     uint32_t rounds;          /* device windows over the pair lists */
     double device_us;         /* device time, inputs resident, no transfers */
     double projection_us;     /* of which the Gauss-Seidel passes */
-    uint64_t evaluated;       /* pairs taken into the passes (all of them in the first) */
-    uint32_t redone;          /* passes after the first run again with more pairs */
+    uint64_t evaluated;       /* pair evaluations, in windows and relaxation rounds */
+    uint32_t redone;          /* window passes after the first run again with more pairs */
+    uint32_t relaxed_passes;  /* passes computed by relaxation (the rest by windows) */
+    uint32_t relaxation_rounds; /* relaxation rounds, all passes (those left to windows included) */
     uint32_t reserved;
 } vx_contact_report;
+
+/* How the passes are computed, for every later call in the process (the
+ * results are the same bit for bit either way):
+ *   VX_CONTACT_AUTO     relaxation: every pair evaluated on the positions the
+ *                       current outcomes of its earlier pairs give, repeated
+ *                       until no outcome changes (a fixed point is the
+ *                       sequential sweep's result); a pass that does not get
+ *                       there within the budget (max_rounds rounds with dirty
+ *                       pairs per call, 1536 when 0; 16 evaluations per pair
+ *                       and pass; 16 separating pairs per voxel) is run by
+ *                       windows from its start, and so are the passes after
+ *                       it. The default.
+ *   VX_CONTACT_WINDOWS  every pass by windows: one workgroup walks the sorted
+ *                       pairs, committing at once the pairs whose predecessors
+ *                       cannot change what they read. */
+#define VX_CONTACT_AUTO 0
+#define VX_CONTACT_WINDOWS 1
+int32_t vx_contact_set_method(int32_t method, uint32_t max_rounds);
 
 int32_t vx_contact_project(uint32_t n, float *positions, float *previous, const float *rest_positions,
                            const float *mass, const uint8_t *kinematic, const uint8_t *target_axes,

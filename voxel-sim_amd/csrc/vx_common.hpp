@@ -54,6 +54,9 @@ struct Context {
     // captured hipGraphs (vx_set_launch_mode; the same kernels, arguments and
     // order).
     bool direct_launch = false;
+    // How vx_contact_project runs its passes (vx_contact_set_method).
+    int32_t contact_method = VX_CONTACT_AUTO;
+    uint32_t contact_rounds = 1536;  // relaxation rounds per call before the passes left go to windows
 };
 
 Context &context();

@@ -85,50 +85,139 @@ __global__ void k_adjacency(uint32_t n, const uint32_t *__restrict__ bond, uint3
 }
 
 // The bin range of each voxel's swept box (src/collision.rs:383-386, 432-439).
+// Also the bounding box of every swept box's bins, per workgroup:
+// box[6 b + 0..2] the minima (as x + 2^20, so unsigned comparisons order
+// them), box[6 b + 3..5] the maxima.
 __global__ void k_bins(uint32_t n, const float *__restrict__ start, const float *__restrict__ pos,
-                       int32_t *__restrict__ lohi, uint64_t *__restrict__ count, uint32_t *__restrict__ err) {
+                       int32_t *__restrict__ lohi, uint64_t *__restrict__ count, uint32_t *__restrict__ err,
+                       uint32_t *__restrict__ box) {
     const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= n) return;
-    const float m = 0.5f + 2e-4f;
-    uint64_t c = 1;
-    bool ok = true;
-    for (int a = 0; a < 3; ++a) {
-        const float s = start[size_t(v) * 3 + a], p = pos[size_t(v) * 3 + a];
-        const float lo = floorf(fminf(s, p) / kCell - m);
-        const float hi = floorf(fmaxf(s, p) / kCell + m);
-        // NaN fails both comparisons
-        if (!(lo >= -kBinLimit && hi < kBinLimit)) {
-            ok = false;
-            break;
+    uint32_t bmin[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, bmax[3] = {0, 0, 0};
+    if (v < n) {
+        const float m = 0.5f + 2e-4f;
+        uint64_t c = 1;
+        bool ok = true;
+        int32_t lo_[3], hi_[3];
+        for (int a = 0; a < 3; ++a) {
+            const float s = start[size_t(v) * 3 + a], p = pos[size_t(v) * 3 + a];
+            const float lo = floorf(fminf(s, p) / kCell - m);
+            const float hi = floorf(fmaxf(s, p) / kCell + m);
+            // NaN fails both comparisons
+            if (!(lo >= -kBinLimit && hi < kBinLimit)) {
+                ok = false;
+                break;
+            }
+            lo_[a] = int32_t(lo);
+            hi_[a] = int32_t(hi);
+            lohi[size_t(v) * 6 + a] = lo_[a];
+            lohi[size_t(v) * 6 + 3 + a] = hi_[a];
+            c *= uint64_t(int64_t(hi) - int64_t(lo) + 1);
         }
-        lohi[size_t(v) * 6 + a] = int32_t(lo);
-        lohi[size_t(v) * 6 + 3 + a] = int32_t(hi);
-        c *= uint64_t(int64_t(hi) - int64_t(lo) + 1);
+        if (!ok) {
+            atomicOr(err, 2u);
+            c = 0;
+        } else if (c > (uint64_t(1) << 31)) {
+            // more entries than the whole call may hold; zeroing it keeps the
+            // scan of at most 2^32 counts of at most 2^31 each from wrapping
+            atomicOr(err, 4u);
+            c = 0;
+        }
+        if (c) {
+            for (int a = 0; a < 3; ++a) {
+                bmin[a] = uint32_t(lo_[a] + kBinBias);
+                bmax[a] = uint32_t(hi_[a] + kBinBias);
+            }
+        }
+        count[v] = c;
     }
-    if (!ok) {
-        atomicOr(err, 2u);
-        c = 0;
-    } else if (c > (uint64_t(1) << 31)) {
-        // more entries than the whole call may hold; zeroing it keeps the
-        // scan of at most 2^32 counts of at most 2^31 each from wrapping
-        atomicOr(err, 4u);
-        c = 0;
+    // workgroup minima and maxima, one partial row per workgroup (k_box_reduce
+    // folds them: many atomics on six words would serialise)
+    __shared__ uint32_t red[6][4];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            bmin[a] = min(bmin[a], uint32_t(__shfl_xor(int(bmin[a]), off, 64)));
+            bmax[a] = max(bmax[a], uint32_t(__shfl_xor(int(bmax[a]), off, 64)));
+        }
     }
-    count[v] = c;
+    const uint32_t wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        for (int a = 0; a < 3; ++a) {
+            red[a][wave] = bmin[a];
+            red[3 + a][wave] = bmax[a];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const uint32_t q = threadIdx.x;
+        uint32_t r = red[q][0];
+        for (uint32_t w = 1; w < blockDim.x / 64; ++w) r = q < 3 ? min(r, red[q][w]) : max(r, red[q][w]);
+        box[size_t(blockIdx.x) * 6 + q] = r;
+    }
+}
+// Fold the per-workgroup rows of k_bins (minima: 0xFFFFFFFF for a workgroup
+// without a box) into out[0..5]; one workgroup.
+__global__ void k_box_reduce(uint32_t rows, const uint32_t *__restrict__ part, uint32_t *out) {
+    uint32_t r[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0, 0, 0};
+    for (uint32_t b = threadIdx.x; b < rows; b += blockDim.x)
+        for (int q = 0; q < 6; ++q) r[q] = q < 3 ? min(r[q], part[size_t(b) * 6 + q]) : max(r[q], part[size_t(b) * 6 + q]);
+    __shared__ uint32_t red[6][16];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const uint32_t o = uint32_t(__shfl_xor(int(r[q]), off, 64));
+            r[q] = q < 3 ? min(r[q], o) : max(r[q], o);
+        }
+        if ((threadIdx.x & 63) == 0) red[q][threadIdx.x >> 6] = r[q];
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const uint32_t q = threadIdx.x;
+        uint32_t x = red[q][0];
+        for (uint32_t w = 1; w < blockDim.x / 64; ++w) x = q < 3 ? min(x, red[q][w]) : max(x, red[q][w]);
+        out[q] = x;
+    }
+}
+// Sum of the per-workgroup candidate counts of k_bin_pairs; one workgroup.
+__global__ void k_sum_reduce(uint32_t rows, const unsigned long long *__restrict__ part, unsigned long long *out) {
+    unsigned long long x = 0;
+    for (uint32_t b = threadIdx.x; b < rows; b += blockDim.x) x += part[b];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    __shared__ unsigned long long red[16];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (uint32_t w = 0; w < blockDim.x / 64; ++w) t += red[w];
+        *out = t;
+    }
 }
 
-__device__ __forceinline__ uint64_t bin_code(int x, int y, int z) {
-    return (uint64_t(uint32_t(z + kBinBias)) << 42) | (uint64_t(uint32_t(y + kBinBias)) << 21) |
-           uint64_t(uint32_t(x + kBinBias));
-}
-__device__ __forceinline__ int bin_axis(uint64_t code, int a) {
-    return int(uint32_t(code >> (21 * a)) & ((1u << 21) - 1)) - kBinBias;
-}
+// Bin codes count the bins of the call's bounding box of swept boxes:
+// ((z - z0) Y + (y - y0)) X + (x - x0), so the radix sort of the entries
+// takes only the bits that box needs (the broad phase has checked that every
+// bound is below 2^20 cells, so X Y Z < 2^63).
+struct BinFrame {
+    int x0, y0, z0;
+    uint64_t X, Y;
+    __device__ __forceinline__ uint64_t code(int x, int y, int z) const {
+        return (uint64_t(z - z0) * Y + uint64_t(y - y0)) * X + uint64_t(x - x0);
+    }
+    __device__ __forceinline__ void decode(uint64_t c, int &x, int &y, int &z) const {
+        x = int(c % X) + x0;
+        c /= X;
+        y = int(c % Y) + y0;
+        z = int(c / Y) + z0;
+    }
+};
 
 // One entry per covered bin; `ends` is the inclusive scan of the counts.
 __global__ void k_emit_bins(uint32_t n, const int32_t *__restrict__ lohi, const uint64_t *__restrict__ count,
                             const uint64_t *__restrict__ ends, uint64_t *__restrict__ keys,
-                            uint32_t *__restrict__ vals) {
+                            uint32_t *__restrict__ vals, BinFrame fr) {
     const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n) return;
     uint64_t o = ends[v] - count[v];
@@ -136,7 +225,7 @@ __global__ void k_emit_bins(uint32_t n, const int32_t *__restrict__ lohi, const 
     for (int z = b[2]; z <= b[5]; ++z)
         for (int y = b[1]; y <= b[4]; ++y)
             for (int x = b[0]; x <= b[3]; ++x) {
-                keys[o] = bin_code(x, y, z);
+                keys[o] = fr.code(x, y, z);
                 vals[o] = v;
                 ++o;
             }
@@ -175,36 +264,59 @@ __device__ bool local_continuum(const float *__restrict__ rest, const uint32_t *
 
 // For entry e of the bin-sorted list: the candidate pairs it forms with the
 // later entries of its bin, counted (EMIT = false) or written (EMIT = true).
+// Pairs are written as (i << nb) | j with nb bits per voxel index (the radix
+// sort then takes 2 nb bits) and unpacked to (i << 32) | j after the sort.
 template <bool EMIT>
 __global__ void k_bin_pairs(uint64_t total, const uint64_t *__restrict__ keys, const uint32_t *__restrict__ vals,
                             const int32_t *__restrict__ lohi, const uint8_t *__restrict__ flags,
                             const float *__restrict__ rest, const uint32_t *__restrict__ adj,
                             uint64_t *__restrict__ count, const uint64_t *__restrict__ ends,
-                            uint64_t *__restrict__ out, unsigned long long *__restrict__ candidates) {
+                            uint64_t *__restrict__ out, unsigned long long *__restrict__ candidates, BinFrame fr,
+                            uint32_t nb) {
     const uint64_t e = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (e >= total) return;
-    const uint64_t key = keys[e];
-    const uint32_t v = vals[e];
-    const bool kin_v = kinematic(flags[v]);
-    const int bx = bin_axis(key, 0), by = bin_axis(key, 1), bz = bin_axis(key, 2);
-    uint64_t kept = 0, cand = 0;
-    uint64_t o = EMIT ? ends[e] - count[e] : 0;
-    for (uint64_t f = e + 1; f < total && keys[f] == key; ++f) {
-        const uint32_t u = vals[f];
-        if (kin_v && kinematic(flags[u])) continue;
-        // emit from the lowest corner of the two ranges' intersection only
-        const int32_t *lu = lohi + size_t(u) * 6, *lv = lohi + size_t(v) * 6;
-        if (max(lu[0], lv[0]) != bx || max(lu[1], lv[1]) != by || max(lu[2], lv[2]) != bz) continue;
-        ++cand;
-        const uint32_t i = min(u, v), j = max(u, v);
-        if (local_continuum(rest, adj, i, j)) continue;
-        if (EMIT) out[o++] = (uint64_t(i) << 32) | j;
-        ++kept;
+    unsigned long long cand = 0;
+    if (e < total) {
+        const uint64_t key = keys[e];
+        const uint32_t v = vals[e];
+        const bool kin_v = kinematic(flags[v]);
+        int bx, by, bz;
+        fr.decode(key, bx, by, bz);
+        uint64_t kept = 0;
+        uint64_t o = EMIT ? ends[e] - count[e] : 0;
+        const int32_t *lv = lohi + size_t(v) * 6;
+        const int lv0 = lv[0], lv1 = lv[1], lv2 = lv[2];
+        for (uint64_t f = e + 1; f < total && keys[f] == key; ++f) {
+            const uint32_t u = vals[f];
+            if (kin_v && kinematic(flags[u])) continue;
+            // emit from the lowest corner of the two ranges' intersection only
+            const int32_t *lu = lohi + size_t(u) * 6;
+            if (max(lu[0], lv0) != bx || max(lu[1], lv1) != by || max(lu[2], lv2) != bz) continue;
+            ++cand;
+            const uint32_t i = min(u, v), j = max(u, v);
+            if (local_continuum(rest, adj, i, j)) continue;
+            if (EMIT) out[o++] = (uint64_t(i) << nb) | j;
+            ++kept;
+        }
+        if (!EMIT) count[e] = kept;
     }
     if (!EMIT) {
-        count[e] = kept;
-        if (cand) atomicAdd(candidates, (unsigned long long)cand);
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) cand += __shfl_xor(cand, off, 64);
+        __shared__ unsigned long long red[4];
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = cand;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned long long t = 0;
+            for (uint32_t w = 0; w < blockDim.x / 64; ++w) t += red[w];
+            candidates[blockIdx.x] = t;  // per workgroup; k_sum_reduce adds them
+        }
     }
+}
+__global__ void k_unpack_pairs(uint32_t P, uint64_t *pairs, uint32_t nb) {
+    const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= P) return;
+    const uint64_t k = pairs[m];
+    pairs[m] = ((k >> nb) << 32) | (k & ((uint64_t(1) << nb) - 1));
 }
 
 // ---- narrow phase: the reference's tests, f32 ---------------------------------
@@ -301,6 +413,10 @@ struct Outcome {
     uint8_t real_i = 0, real_j = 0;  // components with a nonzero delta
     uint8_t plus0_i = 0, plus0_j = 0;  // components with a +0 delta that can meet a -0
     float di[3] = {0, 0, 0}, dj[3] = {0, 0, 0};
+    // when sep: the normal axis, its sign, and the amounts separate() moves i and j by
+    int8_t axis = 0;
+    bool negative = false;
+    float s_i = 0.0f, s_j = 0.0f;
 };
 
 __device__ __forceinline__ bool is_zero(float x) { return (__float_as_uint(x) & 0x7fffffffu) == 0; }
@@ -340,21 +456,21 @@ __device__ __forceinline__ bool swept_surely_none(const V3 &d0, const V3 &rel) {
 // the +0-over--0 store.
 constexpr int kRec = 8;  // floats per voxel record
 constexpr int kNegZeroShift = 8;
-__device__ Outcome evaluate(uint32_t i, uint32_t j, const float *rec) {
+// The pass body on explicit positions pi, pj; the starts, masses and flags
+// come from the voxel records (a0.w, a1 and b0.w, b1).
+__device__ __forceinline__ Outcome evaluate_at(uint32_t i, uint32_t j, const V3 &pi, const float4 &a0,
+                                               const float4 &a1, const V3 &pj, const float4 &b0, const float4 &b1) {
     Outcome o;
     o.i = i;
     o.j = j;
-    const float4 *r4 = reinterpret_cast<const float4 *>(rec);
-    const float4 a0 = r4[2 * size_t(i)], a1 = r4[2 * size_t(i) + 1];
-    const float4 b0 = r4[2 * size_t(j)], b1 = r4[2 * size_t(j) + 1];
     const uint32_t wi_flags = __float_as_uint(a1.w), wj_flags = __float_as_uint(b1.w);
     const uint8_t fi = uint8_t(wi_flags), fj = uint8_t(wj_flags);
     // components that were -0 on entry (position or previous): the only ones a +0 can change
     const uint8_t nz_i = uint8_t((wi_flags >> kNegZeroShift) & 7u), nz_j = uint8_t((wj_flags >> kNegZeroShift) & 7u);
     o.movable_i = !kinematic(fi);
     o.movable_j = !kinematic(fj);
-    const V3 pi{{a0.x, a0.y, a0.z}}, si{{a0.w, a1.x, a1.y}};
-    const V3 pj{{b0.x, b0.y, b0.z}}, sj{{b0.w, b1.x, b1.y}};
+    const V3 si{{a0.w, a1.x, a1.y}};
+    const V3 sj{{b0.w, b1.x, b1.y}};
     const V3 start_delta = sub3(sj, si);
     float sign = 1.0f, amount = 0.0f, depth = 0.0f;
     {
@@ -394,6 +510,10 @@ __device__ Outcome evaluate(uint32_t i, uint32_t j, const float *rec) {
         if (denom > 0.0f) {
             o.sep = true;
             const float s_i = amount * wi / denom, s_j = amount * wj / denom;
+            o.axis = int8_t(axis);
+            o.negative = sign < 0.0f;
+            o.s_i = s_i;
+            o.s_j = s_j;
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 const float nc = c == axis ? sign : 0.0f;
@@ -407,6 +527,12 @@ __device__ Outcome evaluate(uint32_t i, uint32_t j, const float *rec) {
         }
     }
     return o;
+}
+__device__ Outcome evaluate(uint32_t i, uint32_t j, const float *rec) {
+    const float4 *r4 = reinterpret_cast<const float4 *>(rec);
+    const float4 a0 = r4[2 * size_t(i)], a1 = r4[2 * size_t(i) + 1];
+    const float4 b0 = r4[2 * size_t(j)], b1 = r4[2 * size_t(j) + 1];
+    return evaluate_at(i, j, V3{{a0.x, a0.y, a0.z}}, a0, a1, V3{{b0.x, b0.y, b0.z}}, b0, b1);
 }
 
 // ---- projection ----------------------------------------------------------------
@@ -976,12 +1102,412 @@ __global__ void k_contact_scatter(uint32_t npairs, const uint64_t *__restrict__ 
     out[ends[k] - 1] = r;
 }
 
+// ---- passes by relaxation --------------------------------------------------------
+//
+// A pass's result is the unique solution of its own recurrence: pair k reads
+// the positions its two voxels have after the moves of the earlier pairs that
+// touch them (in pair order), and its outcome is the pass body on those
+// positions. Evaluating every pair on the positions that its earlier pairs'
+// current outcomes give, and repeating until no outcome changes, reaches that
+// solution: a fixed point satisfies the recurrence pair by pair, in order. It
+// takes about as many rounds as the longest chain of moves that feed each
+// other (6-7 per pass on the 64^3 contact box), where the window walker needs
+// one sequential window per conflict (9,329 there).
+//
+// State, all in device memory:
+//  * per pair, its current outcome: rflag (separates, normal axis and sign,
+//    contact code) and rsi, rsj (the amounts separate() moves i and j by);
+//  * per voxel, its movers: the pairs with a separating outcome that touch it,
+//    ascending, at most kMov (mcnt, mlist; bit 31 of an entry: the voxel is the
+//    pair's i);
+//  * per voxel, the pairs that touch it, ascending: jlist[joff, jend) holds
+//    the pairs (i, v), and [pstart, pend) is the range of pairs (v, j) in the
+//    sorted list (they follow every (i, v) since i < v).
+// A round evaluates the dirty pairs on their before-states (k_rx_eval),
+// records the outcomes that changed and the voxels they touch (k_rx_apply),
+// and rebuilds those voxels' movers, marking dirty every pair after the
+// smallest changed one (k_rx_rebuild). The dirty list may hold a pair twice
+// (marked from both voxels); both evaluations give the same outcome. The
+// positions stay as they were at the pass's start until the fixed point;
+// then k_rx_commit_voxels applies each voxel's moves in pair order (every
+// component, as separate() adds them, signed zeros included) and
+// k_rx_commit_pairs keeps the first contacts and the outcomes the next pass
+// starts from. A pass that overflows a list, or runs past the call's round
+// budget or its own evaluation budget, is left to the window walker from its
+// start, with the passes after it.
+constexpr uint32_t kMov = 16;             // movers per voxel
+constexpr uint32_t kRxRounds = 1536;      // rounds per call before the walker takes the passes left (default)
+constexpr uint64_t kRxEvalBudget = 16;    // pair evaluations per pass, in units of the pair count
+constexpr uint32_t kRxChunk = 8;          // rounds enqueued between two host checks
+constexpr uint32_t kRxMaxPairs = 1u << 30;
+enum : int { RX_CNT0 = 0, RX_CNT1, RX_VCNT, RX_OVER, RX_ROUNDS, RX_WORDS };  // RX_ROUNDS: rounds with dirty pairs
+constexpr uint32_t RF_SEP = 1u, RF_NEG = 8u;  // rflag: bit 0 separates, bits 1-2 axis, bit 3 negative, 8-15 code
+constexpr int kRxBlocks = 2048;           // grid of the list kernels (grid-stride loops), a pass's first round
+constexpr int kRxBlocksLater = 512;       // at most, later rounds
+
+__device__ __forceinline__ uint32_t rf_axis(uint32_t f) { return (f >> 1) & 3u; }
+
+// The move separate() (src/collision.rs:642-665) gives one voxel of a pair:
+// i moves by -n * s_i and j by +n * s_j, every component added.
+__device__ __forceinline__ void add_move(float p[3], uint32_t f, float s, bool side_i) {
+    const uint32_t axis = rf_axis(f);
+    const float sign = (f & RF_NEG) ? -1.0f : 1.0f;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float nc = uint32_t(c) == axis ? sign : 0.0f;
+        p[c] += side_i ? -nc * s : nc * s;
+    }
+}
+
+// Position of voxel v just before pair k: its position at the start of the
+// pass plus the moves of its movers below k, in pair order.
+__device__ __forceinline__ void before_state(float p[3], uint32_t v, uint32_t k, const uint32_t *__restrict__ mcnt,
+                                             const uint32_t *__restrict__ mlist, const uint32_t *__restrict__ rflag,
+                                             const float *__restrict__ rsi, const float *__restrict__ rsj) {
+    const uint32_t c = mcnt[v];
+    for (uint32_t q = 0; q < c; ++q) {
+        const uint32_t e = mlist[size_t(v) * kMov + q];
+        const uint32_t m = e & 0x7FFFFFFFu;
+        if (m >= k) break;
+        const bool side_i = e >> 31;
+        add_move(p, rflag[m], side_i ? rsi[m] : rsj[m], side_i);
+    }
+}
+
+// Reserve `take` consecutive entries of a list per lane; every lane of the
+// wave calls this together. Returns the lane's first entry, or kNoMark (and
+// sets *over) when the list's capacity would be exceeded.
+__device__ __forceinline__ uint32_t wave_reserve(uint32_t take, uint32_t *count, uint32_t cap, uint32_t *over) {
+    const uint32_t lane = threadIdx.x & 63;
+    uint32_t x = take;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d, 64);
+        if (lane >= uint32_t(d)) x += y;
+    }
+    const uint32_t total = __shfl(x, 63, 64);
+    uint32_t base = 0;
+    if (lane == 63 && total) base = atomicAdd(count, total);
+    base = __shfl(base, 63, 64);
+    const uint32_t first = base + x - take;
+    if (take && uint64_t(first) + take > cap) {
+        *over = 1;
+        return kNoMark;
+    }
+    return first;
+}
+
+// Pairs per voxel. jkey[m] = j of pair m; sorted stably by j with the pair
+// indices as values, the pairs (i, v) of each voxel v come out in ascending
+// order: jlist[joff[v], jend[v]). The pairs (v, j) are the range
+// [pstart[v], pend[v]) of the sorted list.
+__global__ void k_rx_jkeys(uint32_t P, const uint64_t *__restrict__ pairs, uint32_t *jkey) {
+    const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m < P) jkey[m] = uint32_t(pairs[m]);
+}
+__global__ void k_rx_bounds(uint32_t P, const uint64_t *__restrict__ pairs, const uint32_t *__restrict__ jkey,
+                            uint32_t *joff, uint32_t *jend, uint32_t *pstart, uint32_t *pend) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= P) return;
+    const uint32_t j = jkey[t];
+    if (t == 0 || jkey[t - 1] != j) joff[j] = t;
+    if (t == P - 1 || jkey[t + 1] != j) jend[j] = t + 1;
+    const uint32_t i = uint32_t(pairs[t] >> 32);
+    if (t == 0 || uint32_t(pairs[t - 1] >> 32) != i) pstart[i] = t;
+    if (t == P - 1 || uint32_t(pairs[t + 1] >> 32) != i) pend[i] = t + 1;
+}
+
+// The relaxation's state at the start of a call: no movers, no changed
+// voxel, no stamps, no pair ranges, every pair without an outcome.
+__global__ void k_rx_clear(uint32_t n, uint32_t P, uint32_t *mcnt, uint32_t *vchg, uint32_t *vstamp, uint32_t *joff,
+                           uint32_t *jend, uint32_t *pstart, uint32_t *pend, uint32_t *rflag, float *rsi, float *rsj,
+                           uint32_t *rx) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) {
+        mcnt[t] = 0;
+        vchg[t] = kNoMark;
+        vstamp[t] = 0;
+        joff[t] = 0;
+        jend[t] = 0;
+        pstart[t] = 0;
+        pend[t] = 0;
+    }
+    if (t < P) {
+        rflag[t] = 0;
+        rsi[t] = 0.0f;
+        rsj[t] = 0.0f;
+    }
+    if (t < 16) rx[t] = 0;
+}
+// First pass: every pair is dirty, nothing moves yet.
+__global__ void k_rx_init_first(uint32_t P, uint32_t *dlist, uint32_t *rx, uint32_t cur) {
+    const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m < P) dlist[m] = m;
+    if (m == 0) {
+        rx[RX_CNT0 + cur] = P;
+        rx[RX_OVER] = 0;
+    }
+}
+// Later passes. A pair whose voxels no separating pair touched in the previous
+// pass, and which did not separate there itself, would repeat its outcome
+// (the one it holds). The others are the pairs that touch a voxel with
+// movers: they are listed dirty (twice when both voxels have movers) and
+// start from no outcome, and the movers are cleared.
+__global__ __launch_bounds__(256) void k_rx_init_pass(uint32_t n, uint32_t *mcnt, const uint32_t *__restrict__ joff,
+                                                      const uint32_t *__restrict__ jend,
+                                                      const uint32_t *__restrict__ jlist,
+                                                      const uint32_t *__restrict__ pstart,
+                                                      const uint32_t *__restrict__ pend, uint32_t *rflag, float *rsi,
+                                                      float *rsj, uint32_t *dlist, uint32_t *rx, uint32_t cur,
+                                                      uint32_t cap) {
+    // two voxels per wave, 32 lanes each walking the voxel's pairs together
+    const uint32_t lane = threadIdx.x & 63, half = lane >> 5, sub = lane & 31;
+    const uint32_t v = (blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6)) * 2 + half;
+    uint32_t j0 = 0, nj = 0, r0 = 0, nr = 0;
+    const bool act = v < n && mcnt[v] != 0;
+    if (act) {
+        j0 = joff[v];
+        nj = jend[v] - j0;
+        r0 = pstart[v];
+        nr = pend[v] - r0;
+    }
+    const uint32_t total = nj + nr;
+    uint32_t iters = (total + 31) / 32;
+    iters = max(iters, uint32_t(__shfl_xor(int(iters), 32, 64)));
+    for (uint32_t it = 0; it < iters; ++it) {
+        const uint32_t t = it * 32 + sub;
+        const bool live = t < total;
+        uint32_t m = 0;
+        if (live) {
+            m = t < nj ? jlist[j0 + t] : r0 + (t - nj);
+            rflag[m] = 0;
+            rsi[m] = 0.0f;
+            rsj[m] = 0.0f;
+        }
+        const uint32_t pos = wave_reserve(live ? 1u : 0u, rx + RX_CNT0 + cur, cap, rx + RX_OVER);
+        if (live && pos != kNoMark) dlist[pos] = m;
+    }
+    if (act && sub == 0) mcnt[v] = 0;
+}
+__global__ void k_rx_init_words(uint32_t *rx, uint32_t cur) {
+    rx[RX_CNT0 + cur] = 0;
+    rx[RX_OVER] = 0;
+}
+
+__device__ __forceinline__ void pack_rx(const Outcome &o, uint32_t &f, float &si, float &sj) {
+    f = uint32_t(o.code) << 8;
+    si = 0.0f;
+    sj = 0.0f;
+    if (o.sep) {
+        f |= RF_SEP | (uint32_t(o.axis) << 1) | (o.negative ? RF_NEG : 0u);
+        si = o.s_i;
+        sj = o.s_j;
+    }
+}
+
+// Evaluate the dirty pairs of list `cur` on their before-states; outcomes
+// staged by list entry.
+__global__ __launch_bounds__(256) void k_rx_eval(uint32_t cur, const uint32_t *__restrict__ dlist, uint32_t *rx,
+                                                 uint32_t cap, const uint64_t *__restrict__ pairs,
+                                                 const float *__restrict__ rec, const uint32_t *__restrict__ mcnt,
+                                                 const uint32_t *__restrict__ mlist, const uint32_t *__restrict__ rflag,
+                                                 const float *__restrict__ rsi, const float *__restrict__ rsj,
+                                                 uint32_t *__restrict__ sflag, float *__restrict__ ssi,
+                                                 float *__restrict__ ssj, unsigned long long *evals) {
+    if (rx[RX_OVER]) return;
+    const uint32_t count = min(rx[RX_CNT0 + cur], cap);
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g == 0) {
+        rx[RX_CNT0 + (cur ^ 1u)] = 0;
+        rx[RX_VCNT] = 0;
+        if (count) {
+            atomicAdd(evals, (unsigned long long)count);
+            rx[RX_ROUNDS] += 1;
+        }
+    }
+    const float4 *r4 = reinterpret_cast<const float4 *>(rec);
+    for (uint32_t x = g; x < count; x += gridDim.x * blockDim.x) {
+        const uint32_t k = dlist[x];
+        const uint64_t key = pairs[k];
+        const uint32_t i = uint32_t(key >> 32), j = uint32_t(key);
+        const float4 a0 = r4[2 * size_t(i)], a1 = r4[2 * size_t(i) + 1];
+        const float4 b0 = r4[2 * size_t(j)], b1 = r4[2 * size_t(j) + 1];
+        float pi[3] = {a0.x, a0.y, a0.z}, pj[3] = {b0.x, b0.y, b0.z};
+        before_state(pi, i, k, mcnt, mlist, rflag, rsi, rsj);
+        before_state(pj, j, k, mcnt, mlist, rflag, rsi, rsj);
+        const Outcome o = evaluate_at(i, j, V3{{pi[0], pi[1], pi[2]}}, a0, a1, V3{{pj[0], pj[1], pj[2]}}, b0, b1);
+        uint32_t f;
+        float si, sj;
+        pack_rx(o, f, si, sj);
+        sflag[x] = f;
+        ssi[x] = si;
+        ssj[x] = sj;
+    }
+}
+
+// Record the staged outcomes; a changed move marks both voxels (the smallest
+// changed pair per voxel in vchg; each voxel listed once per round by its
+// stamp). A changed contact code alone changes no position.
+__global__ __launch_bounds__(256) void k_rx_apply(uint32_t cur, uint32_t stamp, const uint32_t *__restrict__ dlist,
+                                                  uint32_t *rx, uint32_t cap, const uint64_t *__restrict__ pairs,
+                                                  uint32_t *rflag, float *rsi, float *rsj,
+                                                  const uint32_t *__restrict__ sflag, const float *__restrict__ ssi,
+                                                  const float *__restrict__ ssj, uint32_t *vchg, uint32_t *vstamp,
+                                                  uint32_t *vlist, uint32_t n) {
+    if (rx[RX_OVER]) return;
+    const uint32_t count = min(rx[RX_CNT0 + cur], cap);
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride) {
+        const uint32_t x = base + threadIdx.x;
+        uint32_t add_i = 0, add_j = 0, i = 0, j = 0;
+        if (x < count) {
+            const uint32_t k = dlist[x];
+            const uint32_t f = sflag[x], of = rflag[k];
+            const float a = ssi[x], b = ssj[x];
+            const bool moves = ((f ^ of) & 0xFu) || __float_as_uint(a) != __float_as_uint(rsi[k]) ||
+                               __float_as_uint(b) != __float_as_uint(rsj[k]);
+            if (moves || f != of) {
+                rflag[k] = f;
+                rsi[k] = a;
+                rsj[k] = b;
+            }
+            if (moves) {
+                const uint64_t key = pairs[k];
+                i = uint32_t(key >> 32);
+                j = uint32_t(key);
+                atomicMin(vchg + i, k);
+                atomicMin(vchg + j, k);
+                add_i = atomicExch(vstamp + i, stamp) != stamp ? 1u : 0u;
+                add_j = atomicExch(vstamp + j, stamp) != stamp ? 1u : 0u;
+            }
+        }
+        const uint32_t pos = wave_reserve(add_i + add_j, rx + RX_VCNT, n, rx + RX_OVER);
+        if (pos != kNoMark) {
+            uint32_t q = pos;
+            if (add_i) vlist[q++] = i;
+            if (add_j) vlist[q] = j;
+        }
+    }
+}
+
+// Rebuild the movers of the voxels whose moves changed, and list as dirty the
+// pairs after the smallest changed one that touch them. Two voxels per wave,
+// 32 lanes each walking the voxel's pairs together in pair order (the sorted
+// (i, v) list, then the (v, j) range); the movers are placed by ballot.
+__global__ __launch_bounds__(256) void k_rx_rebuild(uint32_t cur, uint32_t *rx, uint32_t cap,
+                                                    const uint32_t *__restrict__ vlist, const uint32_t *__restrict__ joff,
+                                                    const uint32_t *__restrict__ jend,
+                                                    const uint32_t *__restrict__ jlist,
+                                                    const uint32_t *__restrict__ pstart,
+                                                    const uint32_t *__restrict__ pend,
+                                                    const uint32_t *__restrict__ rflag, uint32_t *mcnt, uint32_t *mlist,
+                                                    uint32_t *vchg, uint32_t *dnext) {
+    if (rx[RX_OVER]) return;
+    const uint32_t count = rx[RX_VCNT];
+    const uint32_t lane = threadIdx.x & 63, half = lane >> 5, sub = lane & 31;
+    const uint32_t waves = gridDim.x * (blockDim.x / 64);
+    const uint32_t wave_id = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    for (uint32_t base = wave_id * 2; base < count; base += waves * 2) {
+        const uint32_t x = base + half;
+        const bool act = x < count;
+        uint32_t v = 0, from = kNoMark, j0 = 0, nj = 0, r0 = 0, nr = 0;
+        if (act) {
+            v = vlist[x];
+            from = vchg[v];
+            j0 = joff[v];
+            nj = jend[v] - j0;
+            r0 = pstart[v];
+            nr = pend[v] - r0;
+        }
+        const uint32_t total = nj + nr;
+        uint32_t iters = (total + 31) / 32;
+        iters = max(iters, uint32_t(__shfl_xor(int(iters), 32, 64)));
+        uint32_t c = 0;  // movers placed so far (the same in every lane of the half)
+        for (uint32_t it = 0; it < iters; ++it) {
+            const uint32_t t = it * 32 + sub;
+            const bool live = act && t < total;
+            uint32_t m = 0;
+            bool sep = false, dirty = false, side_i = false;
+            if (live) {
+                side_i = t >= nj;
+                m = side_i ? r0 + (t - nj) : jlist[j0 + t];
+                sep = rflag[m] & RF_SEP;
+                dirty = from != kNoMark && m > from;
+            }
+            const uint32_t hm = uint32_t(__ballot(sep) >> (32 * half));
+            const uint32_t rank = c + uint32_t(__popc(hm & ((1u << sub) - 1u)));
+            if (sep && rank < kMov) mlist[size_t(v) * kMov + rank] = m | (side_i ? 0x80000000u : 0u);
+            c += uint32_t(__popc(hm));
+            const uint32_t pos = wave_reserve(dirty ? 1u : 0u, rx + RX_CNT0 + (cur ^ 1u), cap, rx + RX_OVER);
+            if (dirty && pos != kNoMark) dnext[pos] = m;
+        }
+        if (act && sub == 0) {
+            if (c > kMov) rx[RX_OVER] = 1;
+            mcnt[v] = min(c, kMov);
+            vchg[v] = kNoMark;
+        }
+    }
+}
+
+// The fixed point's moves, per voxel in pair order, on positions and
+// previous positions; wlast / wfirst as the window walker's next pass reads
+// them (moved voxels), `changed` when any pair separated.
+__global__ void k_rx_commit_voxels(uint32_t n, const uint32_t *__restrict__ mcnt, const uint32_t *__restrict__ mlist,
+                                   const uint32_t *__restrict__ rflag, const float *__restrict__ rsi,
+                                   const float *__restrict__ rsj, float *rec, float *prev, uint32_t *wfirst,
+                                   uint32_t *wlast, uint32_t *ctl) {
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    const uint32_t c = mcnt[v];
+    wfirst[v] = kNoMark;
+    wlast[v] = c ? 1u : 0u;
+    if (!c) return;
+    float p[3], q[3];
+    for (int a = 0; a < 3; ++a) {
+        p[a] = rec[size_t(v) * kRec + a];
+        q[a] = prev[size_t(v) * 3 + a];
+    }
+    for (uint32_t t = 0; t < c; ++t) {
+        const uint32_t e = mlist[size_t(v) * kMov + t];
+        const uint32_t m = e & 0x7FFFFFFFu;
+        const bool side_i = e >> 31;
+        const float s = side_i ? rsi[m] : rsj[m];
+        add_move(p, rflag[m], s, side_i);
+        add_move(q, rflag[m], s, side_i);
+    }
+    for (int a = 0; a < 3; ++a) {
+        rec[size_t(v) * kRec + a] = p[a];
+        prev[size_t(v) * 3 + a] = q[a];
+    }
+    ctl[C_CHANGED] = 1;
+}
+// First contacts (the earliest pass that finds one) and this pass's outcomes.
+__global__ void k_rx_commit_pairs(uint32_t P, const uint32_t *__restrict__ rflag, uint8_t *code, uint8_t *sep_last) {
+    const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= P) return;
+    const uint32_t f = rflag[m];
+    if (!code[m]) code[m] = uint8_t(f >> 8);
+    sep_last[m] = (f & RF_SEP) ? 1 : 0;
+}
+__global__ void k_set_word(uint32_t *w, uint32_t value) { *w = value; }
+
 inline uint32_t blocks_for(uint64_t n, uint32_t per) { return uint32_t((n + per - 1) / per); }
 
 }  // namespace
 }  // namespace vx
 
 using namespace vx;
+
+extern "C" int32_t vx_contact_set_method(int32_t method, uint32_t max_rounds) {
+    clear_error();
+    if (method != VX_CONTACT_AUTO && method != VX_CONTACT_WINDOWS)
+        return fail(VX_ERR_INVALID_ARGUMENT, "contact method must be VX_CONTACT_AUTO or VX_CONTACT_WINDOWS");
+    Context &ctx = context();
+    std::lock_guard<std::recursive_mutex> lock(ctx.mutex);
+    ctx.contact_method = method;
+    ctx.contact_rounds = max_rounds ? max_rounds : kRxRounds;
+    return VX_OK;
+}
 
 extern "C" int32_t vx_contact_project(uint32_t n, float *positions, float *previous, const float *rest_positions,
                                       const float *mass, const uint8_t *kinematic_flags, const uint8_t *target_axes,
@@ -1056,8 +1582,13 @@ extern "C" int32_t vx_contact_project(uint32_t n, float *positions, float *previ
     VX_HIP(hipMemcpyAsync(d_start, d_prev, 12 * N, hipMemcpyDeviceToDevice, s));
     VX_HIP(hipMemsetAsync(d_adj, 0xFF, 24 * N, s));
     VX_HIP(hipMemsetAsync(d_err, 0, 64, s));
+    uint32_t *d_box = d_err + 2;  // bin bounding box: minima (words 2-4), maxima (5-7)
+    uint32_t *d_boxpart;
+    const uint32_t nblocks = blocks_for(N, 256);
+    VX_BUF("contact.boxpart", 24 * size_t(nblocks), d_boxpart);
     k_adjacency<<<blocks_for(3 * N, 256), 256, 0, s>>>(n, d_bond, d_adj, d_err);
-    k_bins<<<blocks_for(N, 256), 256, 0, s>>>(n, d_start, d_pos, d_lohi, d_cnt, d_err);
+    k_bins<<<nblocks, 256, 0, s>>>(n, d_start, d_pos, d_lohi, d_cnt, d_err, d_boxpart);
+    k_box_reduce<<<1, 256, 0, s>>>(nblocks, d_boxpart, d_box);
     VX_HIP(hipGetLastError());
 
     // scratch for rocprim: grown to the largest request
@@ -1070,7 +1601,7 @@ extern "C" int32_t vx_contact_project(uint32_t n, float *positions, float *previ
 
     uint32_t *pinned = static_cast<uint32_t *>(ctx.pinned);
     uint64_t *pin64 = reinterpret_cast<uint64_t *>(pinned + 64);
-    VX_HIP(hipMemcpyAsync(pinned, d_err, 4, hipMemcpyDeviceToHost, s));
+    VX_HIP(hipMemcpyAsync(pinned, d_err, 32, hipMemcpyDeviceToHost, s));
     VX_HIP(hipMemcpyAsync(pin64, d_ends + (N - 1), 8, hipMemcpyDeviceToHost, s));
     VX_HIP(hipStreamSynchronize(s));
     if (pinned[0] & 1) return fail(VX_ERR_INVALID_ARGUMENT, "vx_contact_project: bond neighbor index out of range");
@@ -1081,6 +1612,25 @@ extern "C" int32_t vx_contact_project(uint32_t n, float *positions, float *previ
     if ((pinned[0] & 4) || entries > (uint64_t(1) << 31))
         return fail(VX_ERR_TOO_LARGE, "vx_contact_project: %llu bin entries (swept boxes too large)",
                     (unsigned long long)entries);
+    // the bins' bounding box, and the bits its codes and the voxel indices take
+    BinFrame frame{0, 0, 0, 1, 1};
+    uint64_t cells = 1;
+    if (entries > 0) {
+        const uint32_t *bx = pinned + 2;
+        frame.x0 = int(bx[0]) - kBinBias;
+        frame.y0 = int(bx[1]) - kBinBias;
+        frame.z0 = int(bx[2]) - kBinBias;
+        frame.X = uint64_t(bx[3] - bx[0]) + 1;
+        frame.Y = uint64_t(bx[4] - bx[1]) + 1;
+        cells = frame.X * frame.Y * (uint64_t(bx[5] - bx[2]) + 1);
+    }
+    auto bit_width = [](uint64_t x) {
+        int b = 0;
+        while (b < 64 && (x >> b) != 0) ++b;
+        return b;
+    };
+    const int bin_bits = std::max(1, bit_width(cells - 1));
+    const uint32_t nb = uint32_t(std::max(1, bit_width(uint64_t(n) - 1)));
 
     uint64_t *d_keys, *d_keys2, *d_ecount, *d_eends;
     uint32_t *d_vals, *d_vals2;
@@ -1090,15 +1640,18 @@ extern "C" int32_t vx_contact_project(uint32_t n, float *positions, float *previ
     VX_BUF("contact.vals2", 4 * entries, d_vals2);
     VX_BUF("contact.ecount", 8 * entries, d_ecount);
     VX_BUF("contact.eends", 8 * entries, d_eends);
-    k_emit_bins<<<blocks_for(N, 256), 256, 0, s>>>(n, d_lohi, d_cnt, d_ends, d_keys, d_vals);
+    k_emit_bins<<<blocks_for(N, 256), 256, 0, s>>>(n, d_lohi, d_cnt, d_ends, d_keys, d_vals, frame);
     VX_HIP(hipGetLastError());
     tb = 0;
-    VX_HIP(rocprim::radix_sort_pairs(nullptr, tb, d_keys, d_keys2, d_vals, d_vals2, size_t(entries), 0, 63, s));
+    VX_HIP(rocprim::radix_sort_pairs(nullptr, tb, d_keys, d_keys2, d_vals, d_vals2, size_t(entries), 0, bin_bits, s));
     if ((st = tmp(tb, &d_tmp)) != VX_OK) return st;
-    VX_HIP(rocprim::radix_sort_pairs(d_tmp, tb, d_keys, d_keys2, d_vals, d_vals2, size_t(entries), 0, 63, s));
+    VX_HIP(rocprim::radix_sort_pairs(d_tmp, tb, d_keys, d_keys2, d_vals, d_vals2, size_t(entries), 0, bin_bits, s));
     const uint32_t eblocks = blocks_for(entries, 256);
+    unsigned long long *d_candpart;
+    VX_BUF("contact.candpart", 8 * size_t(eblocks), d_candpart);
     k_bin_pairs<false><<<eblocks, 256, 0, s>>>(entries, d_keys2, d_vals2, d_lohi, d_flags, d_rest, d_adj, d_ecount,
-                                               nullptr, nullptr, d_cand);
+                                               nullptr, nullptr, d_candpart, frame, nb);
+    k_sum_reduce<<<1, 256, 0, s>>>(eblocks, d_candpart, d_cand);
     VX_HIP(hipGetLastError());
     tb = 0;
     VX_HIP(rocprim::inclusive_scan(nullptr, tb, d_ecount, d_eends, size_t(entries), rocprim::plus<uint64_t>(), s));
@@ -1125,12 +1678,13 @@ extern "C" int32_t vx_contact_project(uint32_t n, float *positions, float *previ
     uint64_t ncontacts = 0;
     if (P > 0) {
         k_bin_pairs<true><<<eblocks, 256, 0, s>>>(entries, d_keys2, d_vals2, d_lohi, d_flags, d_rest, d_adj, d_ecount,
-                                                  d_eends, d_pk, nullptr);
+                                                  d_eends, d_pk, nullptr, frame, nb);
         VX_HIP(hipGetLastError());
         tb = 0;
-        VX_HIP(rocprim::radix_sort_keys(nullptr, tb, d_pk, d_pairs, size_t(P), 0, 64, s));
+        VX_HIP(rocprim::radix_sort_keys(nullptr, tb, d_pk, d_pairs, size_t(P), 0, int(2 * nb), s));
         if ((st = tmp(tb, &d_tmp)) != VX_OK) return st;
-        VX_HIP(rocprim::radix_sort_keys(d_tmp, tb, d_pk, d_pairs, size_t(P), 0, 64, s));
+        VX_HIP(rocprim::radix_sort_keys(d_tmp, tb, d_pk, d_pairs, size_t(P), 0, int(2 * nb), s));
+        k_unpack_pairs<<<blocks_for(P, 256), 256, 0, s>>>(P, d_pairs, nb);
         uint8_t *d_sep, *d_due, *d_code_s, *d_sep_s;
         uint32_t *d_clist, *d_wfirst, *d_wlast, *d_moved, *d_ctl;
         float *d_pos_s, *d_prev_s;
@@ -1159,16 +1713,169 @@ extern "C" int32_t vx_contact_project(uint32_t n, float *positions, float *previ
         if ((st = tmp(tb, &d_tmp)) != VX_OK) return st;
         VX_HIP(hipEventRecord(ev[2], s));
         const uint32_t vb = blocks_for(N, 256), pb = blocks_for(P, 256);
-        // the first pass's pairs before its first separating one, at once
-        k_precheck<<<pb, 256, 0, s>>>(P, d_pairs, rec, d_code_s, d_ctl);
-        k_first_pairs<<<pb, 256, 0, s>>>(P, d_ctl, d_code_s, d_code);
-        k_first_done<<<1, 1, 0, s>>>(P, d_ctl);
-        // the rest of pass 1 window by window, then passes 2..6 over candidates
+        // Passes by relaxation while they reach their fixed point within budget;
+        // the window walker takes the first pass that does not, from its start.
+        uint32_t relaxed = 0;  // passes done by relaxation
+        bool finished = false;  // a relaxed pass moved nothing, or all passes are done
+        if (ctx.contact_method == VX_CONTACT_AUTO && P < kRxMaxPairs) {
+            uint32_t *d_rflag, *d_sflag, *d_dl0, *d_dl1, *d_mcnt, *d_mlist, *d_vchg, *d_vstamp, *d_vlist, *d_jend,
+                *d_joff, *d_jkey, *d_jkey2, *d_jlist, *d_pstart, *d_pend, *d_rx;
+            float *d_rsi, *d_rsj, *d_ssi, *d_ssj;
+            VX_BUF("contact.rx.rflag", 4 * size_t(P), d_rflag);
+            VX_BUF("contact.rx.rsi", 4 * size_t(P), d_rsi);
+            VX_BUF("contact.rx.rsj", 4 * size_t(P), d_rsj);
+            VX_BUF("contact.rx.sflag", 4 * size_t(P), d_sflag);
+            VX_BUF("contact.rx.ssi", 4 * size_t(P), d_ssi);
+            VX_BUF("contact.rx.ssj", 4 * size_t(P), d_ssj);
+            VX_BUF("contact.rx.dl0", 4 * size_t(P), d_dl0);
+            VX_BUF("contact.rx.dl1", 4 * size_t(P), d_dl1);
+            VX_BUF("contact.rx.jlist", 4 * size_t(P), d_jlist);
+            VX_BUF("contact.rx.jkey", 4 * size_t(P), d_jkey);
+            VX_BUF("contact.rx.jkey2", 4 * size_t(P), d_jkey2);
+            VX_BUF("contact.rx.mlist", 4 * size_t(kMov) * N, d_mlist);
+            VX_BUF("contact.rx.mcnt", 4 * N, d_mcnt);
+            VX_BUF("contact.rx.vchg", 4 * N, d_vchg);
+            VX_BUF("contact.rx.vstamp", 4 * N, d_vstamp);
+            VX_BUF("contact.rx.vlist", 4 * N, d_vlist);
+            VX_BUF("contact.rx.joff", 4 * N, d_joff);
+            VX_BUF("contact.rx.jend", 4 * N, d_jend);
+            VX_BUF("contact.rx.pstart", 4 * N, d_pstart);
+            VX_BUF("contact.rx.pend", 4 * N, d_pend);
+            VX_BUF("contact.rx.ctl", 64, d_rx);
+            unsigned long long *d_evals = reinterpret_cast<unsigned long long *>(d_rx + 8);
+            uint32_t *dl[2] = {d_dl0, d_dl1};
+            k_rx_clear<<<blocks_for(std::max<uint64_t>(N, P), 256), 256, 0, s>>>(
+                n, P, d_mcnt, d_vchg, d_vstamp, d_joff, d_jend, d_pstart, d_pend, d_rflag, d_rsi, d_rsj, d_rx);
+            // the pairs of every voxel: (i, v) by a stable sort on v, (v, j) as ranges
+            k_rx_jkeys<<<pb, 256, 0, s>>>(P, d_pairs, d_jkey);
+            size_t sb = 0;
+            VX_HIP(rocprim::radix_sort_pairs(nullptr, sb, d_jkey, d_jkey2, rocprim::counting_iterator<uint32_t>(0),
+                                             d_jlist, size_t(P), 0, int(nb), s));
+            void *d_stmp = nullptr;
+            if ((st = buffer("contact.rx.tmp", sb ? sb : 16, &d_stmp)) != VX_OK) return st;
+            VX_HIP(rocprim::radix_sort_pairs(d_stmp, sb, d_jkey, d_jkey2, rocprim::counting_iterator<uint32_t>(0),
+                                             d_jlist, size_t(P), 0, int(nb), s));
+            k_rx_bounds<<<pb, 256, 0, s>>>(P, d_pairs, d_jkey2, d_joff, d_jend, d_pstart, d_pend);
+            VX_HIP(hipGetLastError());
+            uint32_t *pin_rx = pinned + 256;  // rx words (16: the evaluations, u64, at word 8), then ctl at 272
+            uint32_t *pin_ctl = pinned + 272;
+            uint32_t R = 0;                   // rounds enqueued so far; list R & 1 is the next round's
+            uint32_t rx_rounds = 0;           // rounds that had dirty pairs
+            uint64_t evals_before = 0;        // evaluations of the passes before this one
+            // the first round of the first pass evaluates every pair; later rounds
+            // are sized by the last dirty count the host has seen
+            const uint32_t lb_all = std::min<uint32_t>(kRxBlocks, pb);
+            auto grid_for = [&](uint64_t count) {
+                return std::max<uint32_t>(64, std::min<uint32_t>(lb_all, blocks_for(count, 256)));
+            };
+            uint64_t seen = P;       // dirty pairs of the next round, as last seen
+            bool pass_start = true;  // the next round is a pass's first
+            uint64_t evals = 0;      // evaluations so far, as last seen
+            auto enqueue_init = [&](uint32_t pass) {
+                const uint32_t cur = R & 1u;
+                if (pass == 0) {
+                    k_rx_init_first<<<pb, 256, 0, s>>>(P, dl[cur], d_rx, cur);
+                } else {
+                    k_rx_init_words<<<1, 1, 0, s>>>(d_rx, cur);
+                    k_rx_init_pass<<<blocks_for(N, 8), 256, 0, s>>>(n, d_mcnt, d_joff, d_jend, d_jlist, d_pstart,
+                                                                    d_pend, d_rflag,
+                                                      d_rsi, d_rsj, dl[cur], d_rx, cur, P);
+                }
+            };
+            auto enqueue_commit = [&]() {
+                k_set_word<<<1, 1, 0, s>>>(d_ctl + C_CHANGED, 0);
+                k_rx_commit_voxels<<<vb, 256, 0, s>>>(n, d_mcnt, d_mlist, d_rflag, d_rsi, d_rsj, rec, d_prev,
+                                                      d_wfirst, d_wlast, d_ctl);
+                k_rx_commit_pairs<<<pb, 256, 0, s>>>(P, d_rflag, d_code, d_sep);
+            };
+            // One host check per chunk of rounds. A pass that converges is
+            // committed, and the next pass's start and first chunk are enqueued
+            // before the next check, which then also reads whether the committed
+            // pass changed anything: if not, the projection ended with it and the
+            // pass just enqueued found no dirty pair (no separating pair, no
+            // movers) and changed nothing.
+            enqueue_init(0);
+            uint32_t pass = 0;          // the pass being relaxed
+            bool check_changed = false; // this check also answers whether pass - 1 changed anything
+            for (;;) {
+                const uint32_t left = ctx.contact_rounds - std::min(ctx.contact_rounds, rx_rounds);
+                bool converged = false, over = false;
+                if (left > 0) {
+                    const uint32_t chunk = std::min(kRxChunk, left);
+                    for (uint32_t q = 0; q < chunk; ++q, ++R) {
+                        const uint32_t c = R & 1u;
+                        // a pass's first round takes the whole list it starts from;
+                        // the later ones take what the last check saw, or less
+                        const uint32_t g = pass_start ? lb_all : std::min<uint32_t>(kRxBlocksLater, grid_for(seen));
+                        pass_start = false;
+                        k_rx_eval<<<g, 256, 0, s>>>(c, dl[c], d_rx, P, d_pairs, rec, d_mcnt, d_mlist, d_rflag, d_rsi,
+                                                    d_rsj, d_sflag, d_ssi, d_ssj, d_evals);
+                        k_rx_apply<<<g, 256, 0, s>>>(c, R + 1, dl[c], d_rx, P, d_pairs, d_rflag, d_rsi, d_rsj,
+                                                     d_sflag, d_ssi, d_ssj, d_vchg, d_vstamp, d_vlist, n);
+                        k_rx_rebuild<<<g, 256, 0, s>>>(c, d_rx, P, d_vlist, d_joff, d_jend, d_jlist, d_pstart, d_pend,
+                                                       d_rflag, d_mcnt, d_mlist, d_vchg, dl[c ^ 1u]);
+                    }
+                    VX_HIP(hipGetLastError());
+                    VX_HIP(hipMemcpyAsync(pin_rx, d_rx, 16 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+                    if (check_changed)
+                        VX_HIP(hipMemcpyAsync(pin_ctl, d_ctl, 4 * C_WORDS, hipMemcpyDeviceToHost, s));
+                    VX_HIP(hipStreamSynchronize(s));
+                    std::memcpy(&evals, pin_rx + 8, 8);
+                    rx_rounds = pin_rx[RX_ROUNDS];
+                    over = pin_rx[RX_OVER] != 0;
+                    seen = pin_rx[RX_CNT0 + (R & 1u)];
+                    converged = !over && seen == 0;
+                    if (check_changed) {
+                        check_changed = false;
+                        if (!pin_ctl[C_CHANGED]) {  // src/collision.rs:105-107
+                            finished = true;
+                            break;
+                        }
+                    }
+                }
+                if (!converged) {
+                    // more rounds, or the walker does this pass from its start
+                    // (`changed` still holds the previous pass's value, which the
+                    // walker's next pass reads)
+                    if (left > 0 && !over && evals - evals_before <= kRxEvalBudget * P) continue;
+                    rep.evaluated += evals - evals_before;
+                    break;
+                }
+                rep.evaluated += evals - evals_before;
+                evals_before = evals;
+                enqueue_commit();
+                ++relaxed;
+                if (relaxed == uint32_t(kPasses)) {
+                    finished = true;
+                    break;
+                }
+                ++pass;
+                enqueue_init(pass);
+                pass_start = true;
+                check_changed = true;
+            }
+            rep.relaxation_rounds = rx_rounds;
+            VX_HIP(hipGetLastError());
+            rep.relaxed_passes = relaxed;
+            if (!finished && relaxed > 0) k_set_word<<<1, 1, 0, s>>>(d_ctl + C_PASSES, relaxed);
+        }
+        if (finished) {
+            k_set_word<<<1, 1, 0, s>>>(d_ctl + C_PASSES, relaxed);
+            k_set_word<<<1, 1, 0, s>>>(d_ctl + C_DONE, 1);
+        }
+        if (!finished && relaxed == 0) {
+            // the first pass's pairs before its first separating one, at once
+            k_precheck<<<pb, 256, 0, s>>>(P, d_pairs, rec, d_code_s, d_ctl);
+            k_first_pairs<<<pb, 256, 0, s>>>(P, d_ctl, d_code_s, d_code);
+            k_first_done<<<1, 1, 0, s>>>(P, d_ctl);
+            // the rest of pass 1 window by window
+            k_project<<<1, kGs, 0, s>>>(d_pairs, nullptr, nullptr, P, rec, d_prev, d_code, d_sep, d_wfirst, d_wlast,
+                                        d_ctl, 1);
+            VX_HIP(hipGetLastError());
+        }
+        // passes after the walker's or the relaxation's last one, over candidates
         // (k_pass_begin stops them once a pass moved nothing)
-        k_project<<<1, kGs, 0, s>>>(d_pairs, nullptr, nullptr, P, rec, d_prev, d_code, d_sep, d_wfirst, d_wlast,
-                                    d_ctl, 1);
-        VX_HIP(hipGetLastError());
-        for (int pass = 1; pass < kPasses; ++pass) {
+        for (int pass = std::max<int>(1, int(relaxed)); !finished && pass < kPasses; ++pass) {
             k_pass_begin<<<1, 1, 0, s>>>(d_ctl);
             k_pass_voxels<<<vb, 256, 0, s>>>(n, d_ctl, d_moved, d_wfirst, d_wlast, rec, d_pos_s, d_prev, d_prev_s);
             k_pass_pairs<<<pb, 256, 0, s>>>(P, d_ctl, d_pairs, d_moved, d_sep, d_due, d_code, d_code_s, d_sep_s);
@@ -1204,7 +1911,7 @@ extern "C" int32_t vx_contact_project(uint32_t n, float *positions, float *previ
         rep.passes = pinned[C_PASSES];
         rep.rounds = pinned[C_WINDOWS];
         rep.redone = pinned[C_REDONE];
-        rep.evaluated = pinned[C_EVALS];
+        rep.evaluated += pinned[C_EVALS];
     } else {
         VX_HIP(hipEventRecord(ev[1], s));
         VX_HIP(hipStreamSynchronize(s));

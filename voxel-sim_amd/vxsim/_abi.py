@@ -149,6 +149,7 @@ class ContactReport(ctypes.Structure):
     _fields_ = [("candidate_pairs", ctypes.c_uint64), ("pairs", ctypes.c_uint64), ("contacts", ctypes.c_uint64),
                 ("passes", ctypes.c_uint32), ("rounds", ctypes.c_uint32), ("device_us", ctypes.c_double),
                 ("projection_us", ctypes.c_double), ("evaluated", ctypes.c_uint64), ("redone", ctypes.c_uint32),
+                ("relaxed_passes", ctypes.c_uint32), ("relaxation_rounds", ctypes.c_uint32),
                 ("reserved", ctypes.c_uint32)]
 
 
@@ -203,6 +204,7 @@ SIGNATURES = [
     ("vx_scene_set_tool", ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(ToolDesc)]),
     ("vx_scene_tool_state", ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(ToolState)]),
     ("vx_batch_download_bond_forces", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint32, c_dp, c_dp]),
+    ("vx_contact_set_method", ctypes.c_int32, [ctypes.c_int32, ctypes.c_uint32]),
     ("vx_contact_project", ctypes.c_int32,
      [ctypes.c_uint32, c_fp, c_fp, c_fp, c_fp, c_u8p, c_u8p, c_u32p, ctypes.POINTER(Contact), ctypes.c_uint64,
       ctypes.POINTER(ContactReport)]),

@@ -27,6 +27,20 @@ def _f32(a, n, k):
     return a.reshape(-1).copy()
 
 
+AUTO = 0     # VX_CONTACT_AUTO: relaxation to each pass's fixed point, windows for a pass beyond its budget
+WINDOWS = 1  # VX_CONTACT_WINDOWS: every pass by the window walker
+
+
+def set_method(method: int, max_rounds: int = 0) -> None:
+    """How later calls compute the passes (process-wide; identical results).
+    max_rounds: relaxation rounds per call before the passes left go to
+    windows (0: the library's default)."""
+    L = lib()
+    st = L.vx_contact_set_method(int(method), int(max_rounds))
+    if st != _abi.VX_OK:
+        raise SolverError(st, L.vx_last_error().decode(errors="replace"))
+
+
 def project(positions, previous, rest_positions, mass, kinematic, bond_neighbor, targets=None, capacity=None):
     """Project voxel pairs apart; returns (positions, previous, contacts, report).
 
