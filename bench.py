@@ -524,8 +524,23 @@ def cut_contact(scene, gs, prev_pos, targets, cpu: bool) -> dict:
     p, q, c, rep = runs[-1]
     out = {"voxels": scene.n, "broken_bonds_so_far": int(3 * scene.n - np.count_nonzero(bonds != 0xFFFFFFFF)),
            "candidate_pairs": rep["candidate_pairs"], "pairs": rep["pairs"], "contacts": rep["contacts"],
-           "passes": rep["passes"], "windows": rep["rounds"], "pair_evaluations": rep["evaluated"],
-           "device_ms": rep["device_us"] / 1e3, "projection_ms": rep["projection_us"] / 1e3}
+           "passes": rep["passes"], "relaxed_passes": rep["relaxed_passes"],
+           "relaxation_rounds": rep["relaxation_rounds"], "windows": rep["rounds"],
+           "pair_evaluations": rep["evaluated"], "device_ms": rep["device_us"] / 1e3,
+           "projection_ms": rep["projection_us"] / 1e3}
+    # the same call with every pass by the window walker
+    contact.set_method(contact.WINDOWS)
+    try:
+        wp, wq, wc, wrep = contact.project(**args, targets=held)
+    finally:
+        contact.set_method(contact.AUTO)
+    out["windows_method"] = {
+        "device_ms": wrep["device_us"] / 1e3, "windows": wrep["rounds"], "pair_evaluations": wrep["evaluated"],
+        "same_result": bool(np.array_equal(wp.view(np.uint32), p.view(np.uint32))
+                            and np.array_equal(wq.view(np.uint32), q.view(np.uint32))
+                            and all(np.array_equal(wc[k], c[k]) for k in ("i", "j", "new_impact")))}
+    print("cut contact: windows method %.1f ms, same result %s" %
+          (out["windows_method"]["device_ms"], out["windows_method"]["same_result"]), file=sys.stderr, flush=True)
     if cpu:
         from oracle import oracle
         print("cut contact: the oracle on one host thread", file=sys.stderr, flush=True)

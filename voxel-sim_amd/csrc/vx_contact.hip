@@ -266,38 +266,54 @@ __device__ bool local_continuum(const float *__restrict__ rest, const uint32_t *
 // later entries of its bin, counted (EMIT = false) or written (EMIT = true).
 // Pairs are written as (i << nb) | j with nb bits per voxel index (the radix
 // sort then takes 2 nb bits) and unpacked to (i << 32) | j after the sort.
+// The counting pass leaves in kmask[e] which of the entry's next 63 bin
+// mates it keeps (bit 63: a kept mate further on, so the writing pass walks
+// the bin again), so the writing pass repeats no test.
+constexpr uint64_t kMaskWalk = uint64_t(1) << 63;
 template <bool EMIT>
 __global__ void k_bin_pairs(uint64_t total, const uint64_t *__restrict__ keys, const uint32_t *__restrict__ vals,
                             const int32_t *__restrict__ lohi, const uint8_t *__restrict__ flags,
                             const float *__restrict__ rest, const uint32_t *__restrict__ adj,
                             uint64_t *__restrict__ count, const uint64_t *__restrict__ ends,
                             uint64_t *__restrict__ out, unsigned long long *__restrict__ candidates, BinFrame fr,
-                            uint32_t nb) {
+                            uint32_t nb, uint64_t *__restrict__ kmask) {
     const uint64_t e = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     unsigned long long cand = 0;
     if (e < total) {
-        const uint64_t key = keys[e];
         const uint32_t v = vals[e];
-        const bool kin_v = kinematic(flags[v]);
-        int bx, by, bz;
-        fr.decode(key, bx, by, bz);
-        uint64_t kept = 0;
         uint64_t o = EMIT ? ends[e] - count[e] : 0;
-        const int32_t *lv = lohi + size_t(v) * 6;
-        const int lv0 = lv[0], lv1 = lv[1], lv2 = lv[2];
-        for (uint64_t f = e + 1; f < total && keys[f] == key; ++f) {
-            const uint32_t u = vals[f];
-            if (kin_v && kinematic(flags[u])) continue;
-            // emit from the lowest corner of the two ranges' intersection only
-            const int32_t *lu = lohi + size_t(u) * 6;
-            if (max(lu[0], lv0) != bx || max(lu[1], lv1) != by || max(lu[2], lv2) != bz) continue;
-            ++cand;
-            const uint32_t i = min(u, v), j = max(u, v);
-            if (local_continuum(rest, adj, i, j)) continue;
-            if (EMIT) out[o++] = (uint64_t(i) << nb) | j;
-            ++kept;
+        const uint64_t mask = EMIT ? kmask[e] : 0;
+        if (EMIT && !(mask & kMaskWalk)) {
+            for (uint64_t b = mask; b; b &= b - 1) {
+                const uint32_t u = vals[e + 1 + uint32_t(__ffsll((unsigned long long)b) - 1)];
+                out[o++] = (uint64_t(min(u, v)) << nb) | max(u, v);
+            }
+        } else {
+            const uint64_t key = keys[e];
+            const bool kin_v = kinematic(flags[v]);
+            int bx, by, bz;
+            fr.decode(key, bx, by, bz);
+            uint64_t kept = 0, bits = 0;
+            const int32_t *lv = lohi + size_t(v) * 6;
+            const int lv0 = lv[0], lv1 = lv[1], lv2 = lv[2];
+            for (uint64_t f = e + 1; f < total && keys[f] == key; ++f) {
+                const uint32_t u = vals[f];
+                if (kin_v && kinematic(flags[u])) continue;
+                // emit from the lowest corner of the two ranges' intersection only
+                const int32_t *lu = lohi + size_t(u) * 6;
+                if (max(lu[0], lv0) != bx || max(lu[1], lv1) != by || max(lu[2], lv2) != bz) continue;
+                ++cand;
+                const uint32_t i = min(u, v), j = max(u, v);
+                if (local_continuum(rest, adj, i, j)) continue;
+                if (EMIT) out[o++] = (uint64_t(i) << nb) | j;
+                bits |= f - e - 1 < 63 ? uint64_t(1) << (f - e - 1) : kMaskWalk;
+                ++kept;
+            }
+            if (!EMIT) {
+                count[e] = kept;
+                kmask[e] = bits;
+            }
         }
-        if (!EMIT) count[e] = kept;
     }
     if (!EMIT) {
 #pragma unroll
@@ -1118,7 +1134,8 @@ __global__ void k_contact_scatter(uint32_t npairs, const uint64_t *__restrict__ 
 //  * per pair, its current outcome: rflag (separates, normal axis and sign,
 //    contact code) and rsi, rsj (the amounts separate() moves i and j by);
 //  * per voxel, its movers: the pairs with a separating outcome that touch it,
-//    ascending, at most kMov (mcnt, mlist; bit 31 of an entry: the voxel is the
+//    ascending, in mlist[moff[v], moff[v] + mcnt[v]) (room for every pair of
+//    the voxel, so the list never overflows; bit 31 of an entry: the voxel is the
 //    pair's i);
 //  * per voxel, the pairs that touch it, ascending: jlist[joff, jend) holds
 //    the pairs (i, v), and [pstart, pend) is the range of pairs (v, j) in the
@@ -1135,7 +1152,6 @@ __global__ void k_contact_scatter(uint32_t npairs, const uint64_t *__restrict__ 
 // starts from. A pass that overflows a list, or runs past the call's round
 // budget or its own evaluation budget, is left to the window walker from its
 // start, with the passes after it.
-constexpr uint32_t kMov = 16;             // movers per voxel
 constexpr uint32_t kRxRounds = 1536;      // rounds per call before the walker takes the passes left (default)
 constexpr uint64_t kRxEvalBudget = 16;    // pair evaluations per pass, in units of the pair count
 constexpr uint32_t kRxChunk = 8;          // rounds enqueued between two host checks
@@ -1162,11 +1178,13 @@ __device__ __forceinline__ void add_move(float p[3], uint32_t f, float s, bool s
 // Position of voxel v just before pair k: its position at the start of the
 // pass plus the moves of its movers below k, in pair order.
 __device__ __forceinline__ void before_state(float p[3], uint32_t v, uint32_t k, const uint32_t *__restrict__ mcnt,
-                                             const uint32_t *__restrict__ mlist, const uint32_t *__restrict__ rflag,
-                                             const float *__restrict__ rsi, const float *__restrict__ rsj) {
+                                             const uint32_t *__restrict__ moff, const uint32_t *__restrict__ mlist,
+                                             const uint32_t *__restrict__ rflag, const float *__restrict__ rsi,
+                                             const float *__restrict__ rsj) {
     const uint32_t c = mcnt[v];
+    const uint32_t *ml = mlist + moff[v];
     for (uint32_t q = 0; q < c; ++q) {
-        const uint32_t e = mlist[size_t(v) * kMov + q];
+        const uint32_t e = ml[q];
         const uint32_t m = e & 0x7FFFFFFFu;
         if (m >= k) break;
         const bool side_i = e >> 31;
@@ -1217,6 +1235,11 @@ __global__ void k_rx_bounds(uint32_t P, const uint64_t *__restrict__ pairs, cons
     if (t == P - 1 || uint32_t(pairs[t + 1] >> 32) != i) pend[i] = t + 1;
 }
 
+__global__ void k_rx_degrees(uint32_t n, const uint32_t *__restrict__ joff, const uint32_t *__restrict__ jend,
+                             const uint32_t *__restrict__ pstart, const uint32_t *__restrict__ pend, uint32_t *deg) {
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < n) deg[v] = (jend[v] - joff[v]) + (pend[v] - pstart[v]);
+}
 // The relaxation's state at the start of a call: no movers, no changed
 // voxel, no stamps, no pair ranges, every pair without an outcome.
 __global__ void k_rx_clear(uint32_t n, uint32_t P, uint32_t *mcnt, uint32_t *vchg, uint32_t *vstamp, uint32_t *joff,
@@ -1260,34 +1283,47 @@ __global__ __launch_bounds__(256) void k_rx_init_pass(uint32_t n, uint32_t *mcnt
                                                       const uint32_t *__restrict__ pend, uint32_t *rflag, float *rsi,
                                                       float *rsj, uint32_t *dlist, uint32_t *rx, uint32_t cur,
                                                       uint32_t cap) {
-    // two voxels per wave, 32 lanes each walking the voxel's pairs together
+    // each wave takes 64 voxels and walks the pairs of those with movers, two
+    // at a time, 32 lanes per voxel
     const uint32_t lane = threadIdx.x & 63, half = lane >> 5, sub = lane & 31;
-    const uint32_t v = (blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6)) * 2 + half;
-    uint32_t j0 = 0, nj = 0, r0 = 0, nr = 0;
-    const bool act = v < n && mcnt[v] != 0;
-    if (act) {
-        j0 = joff[v];
-        nj = jend[v] - j0;
-        r0 = pstart[v];
-        nr = pend[v] - r0;
-    }
-    const uint32_t total = nj + nr;
-    uint32_t iters = (total + 31) / 32;
-    iters = max(iters, uint32_t(__shfl_xor(int(iters), 32, 64)));
-    for (uint32_t it = 0; it < iters; ++it) {
-        const uint32_t t = it * 32 + sub;
-        const bool live = t < total;
-        uint32_t m = 0;
-        if (live) {
-            m = t < nj ? jlist[j0 + t] : r0 + (t - nj);
-            rflag[m] = 0;
-            rsi[m] = 0.0f;
-            rsj[m] = 0.0f;
+    const uint32_t v0 = (blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6)) * 64;
+    uint64_t todo = __ballot(v0 + lane < n && mcnt[v0 + lane] != 0);
+    while (todo) {
+        const int a = __ffsll((unsigned long long)todo) - 1;
+        todo &= todo - 1;
+        int b = -1;
+        if (todo) {
+            b = __ffsll((unsigned long long)todo) - 1;
+            todo &= todo - 1;
         }
-        const uint32_t pos = wave_reserve(live ? 1u : 0u, rx + RX_CNT0 + cur, cap, rx + RX_OVER);
-        if (live && pos != kNoMark) dlist[pos] = m;
+        const int mine = half ? b : a;
+        const bool act = mine >= 0;
+        const uint32_t v = v0 + uint32_t(act ? mine : 0);
+        uint32_t j0 = 0, nj = 0, r0 = 0, nr = 0;
+        if (act) {
+            j0 = joff[v];
+            nj = jend[v] - j0;
+            r0 = pstart[v];
+            nr = pend[v] - r0;
+        }
+        const uint32_t total = nj + nr;
+        uint32_t iters = (total + 31) / 32;
+        iters = max(iters, uint32_t(__shfl_xor(int(iters), 32, 64)));
+        for (uint32_t it = 0; it < iters; ++it) {
+            const uint32_t t = it * 32 + sub;
+            const bool live = t < total;
+            uint32_t m = 0;
+            if (live) {
+                m = t < nj ? jlist[j0 + t] : r0 + (t - nj);
+                rflag[m] = 0;
+                rsi[m] = 0.0f;
+                rsj[m] = 0.0f;
+            }
+            const uint32_t pos = wave_reserve(live ? 1u : 0u, rx + RX_CNT0 + cur, cap, rx + RX_OVER);
+            if (live && pos != kNoMark) dlist[pos] = m;
+        }
+        if (act && sub == 0) mcnt[v] = 0;
     }
-    if (act && sub == 0) mcnt[v] = 0;
 }
 __global__ void k_rx_init_words(uint32_t *rx, uint32_t cur) {
     rx[RX_CNT0 + cur] = 0;
@@ -1310,7 +1346,8 @@ __device__ __forceinline__ void pack_rx(const Outcome &o, uint32_t &f, float &si
 __global__ __launch_bounds__(256) void k_rx_eval(uint32_t cur, const uint32_t *__restrict__ dlist, uint32_t *rx,
                                                  uint32_t cap, const uint64_t *__restrict__ pairs,
                                                  const float *__restrict__ rec, const uint32_t *__restrict__ mcnt,
-                                                 const uint32_t *__restrict__ mlist, const uint32_t *__restrict__ rflag,
+                                                 const uint32_t *__restrict__ moff, const uint32_t *__restrict__ mlist,
+                                                 const uint32_t *__restrict__ rflag,
                                                  const float *__restrict__ rsi, const float *__restrict__ rsj,
                                                  uint32_t *__restrict__ sflag, float *__restrict__ ssi,
                                                  float *__restrict__ ssj, unsigned long long *evals) {
@@ -1333,8 +1370,8 @@ __global__ __launch_bounds__(256) void k_rx_eval(uint32_t cur, const uint32_t *_
         const float4 a0 = r4[2 * size_t(i)], a1 = r4[2 * size_t(i) + 1];
         const float4 b0 = r4[2 * size_t(j)], b1 = r4[2 * size_t(j) + 1];
         float pi[3] = {a0.x, a0.y, a0.z}, pj[3] = {b0.x, b0.y, b0.z};
-        before_state(pi, i, k, mcnt, mlist, rflag, rsi, rsj);
-        before_state(pj, j, k, mcnt, mlist, rflag, rsi, rsj);
+        before_state(pi, i, k, mcnt, moff, mlist, rflag, rsi, rsj);
+        before_state(pj, j, k, mcnt, moff, mlist, rflag, rsi, rsj);
         const Outcome o = evaluate_at(i, j, V3{{pi[0], pi[1], pi[2]}}, a0, a1, V3{{pj[0], pj[1], pj[2]}}, b0, b1);
         uint32_t f;
         float si, sj;
@@ -1400,8 +1437,9 @@ __global__ __launch_bounds__(256) void k_rx_rebuild(uint32_t cur, uint32_t *rx, 
                                                     const uint32_t *__restrict__ jlist,
                                                     const uint32_t *__restrict__ pstart,
                                                     const uint32_t *__restrict__ pend,
-                                                    const uint32_t *__restrict__ rflag, uint32_t *mcnt, uint32_t *mlist,
-                                                    uint32_t *vchg, uint32_t *dnext) {
+                                                    const uint32_t *__restrict__ rflag, uint32_t *mcnt,
+                                                    const uint32_t *__restrict__ moff, uint32_t *mlist, uint32_t *vchg,
+                                                    uint32_t *dnext) {
     if (rx[RX_OVER]) return;
     const uint32_t count = rx[RX_VCNT];
     const uint32_t lane = threadIdx.x & 63, half = lane >> 5, sub = lane & 31;
@@ -1410,10 +1448,11 @@ __global__ __launch_bounds__(256) void k_rx_rebuild(uint32_t cur, uint32_t *rx, 
     for (uint32_t base = wave_id * 2; base < count; base += waves * 2) {
         const uint32_t x = base + half;
         const bool act = x < count;
-        uint32_t v = 0, from = kNoMark, j0 = 0, nj = 0, r0 = 0, nr = 0;
+        uint32_t v = 0, from = kNoMark, j0 = 0, nj = 0, r0 = 0, nr = 0, mo = 0;
         if (act) {
             v = vlist[x];
             from = vchg[v];
+            mo = moff[v];
             j0 = joff[v];
             nj = jend[v] - j0;
             r0 = pstart[v];
@@ -1436,14 +1475,13 @@ __global__ __launch_bounds__(256) void k_rx_rebuild(uint32_t cur, uint32_t *rx, 
             }
             const uint32_t hm = uint32_t(__ballot(sep) >> (32 * half));
             const uint32_t rank = c + uint32_t(__popc(hm & ((1u << sub) - 1u)));
-            if (sep && rank < kMov) mlist[size_t(v) * kMov + rank] = m | (side_i ? 0x80000000u : 0u);
+            if (sep) mlist[mo + rank] = m | (side_i ? 0x80000000u : 0u);
             c += uint32_t(__popc(hm));
             const uint32_t pos = wave_reserve(dirty ? 1u : 0u, rx + RX_CNT0 + (cur ^ 1u), cap, rx + RX_OVER);
             if (dirty && pos != kNoMark) dnext[pos] = m;
         }
         if (act && sub == 0) {
-            if (c > kMov) rx[RX_OVER] = 1;
-            mcnt[v] = min(c, kMov);
+            mcnt[v] = c;
             vchg[v] = kNoMark;
         }
     }
@@ -1452,7 +1490,8 @@ __global__ __launch_bounds__(256) void k_rx_rebuild(uint32_t cur, uint32_t *rx, 
 // The fixed point's moves, per voxel in pair order, on positions and
 // previous positions; wlast / wfirst as the window walker's next pass reads
 // them (moved voxels), `changed` when any pair separated.
-__global__ void k_rx_commit_voxels(uint32_t n, const uint32_t *__restrict__ mcnt, const uint32_t *__restrict__ mlist,
+__global__ void k_rx_commit_voxels(uint32_t n, const uint32_t *__restrict__ mcnt, const uint32_t *__restrict__ moff,
+                                   const uint32_t *__restrict__ mlist,
                                    const uint32_t *__restrict__ rflag, const float *__restrict__ rsi,
                                    const float *__restrict__ rsj, float *rec, float *prev, uint32_t *wfirst,
                                    uint32_t *wlast, uint32_t *ctl) {
@@ -1468,7 +1507,7 @@ __global__ void k_rx_commit_voxels(uint32_t n, const uint32_t *__restrict__ mcnt
         q[a] = prev[size_t(v) * 3 + a];
     }
     for (uint32_t t = 0; t < c; ++t) {
-        const uint32_t e = mlist[size_t(v) * kMov + t];
+        const uint32_t e = mlist[moff[v] + t];
         const uint32_t m = e & 0x7FFFFFFFu;
         const bool side_i = e >> 31;
         const float s = side_i ? rsi[m] : rsj[m];
@@ -1648,9 +1687,11 @@ extern "C" int32_t vx_contact_project(uint32_t n, float *positions, float *previ
     VX_HIP(rocprim::radix_sort_pairs(d_tmp, tb, d_keys, d_keys2, d_vals, d_vals2, size_t(entries), 0, bin_bits, s));
     const uint32_t eblocks = blocks_for(entries, 256);
     unsigned long long *d_candpart;
+    uint64_t *d_kmask;
     VX_BUF("contact.candpart", 8 * size_t(eblocks), d_candpart);
+    VX_BUF("contact.kmask", 8 * entries, d_kmask);
     k_bin_pairs<false><<<eblocks, 256, 0, s>>>(entries, d_keys2, d_vals2, d_lohi, d_flags, d_rest, d_adj, d_ecount,
-                                               nullptr, nullptr, d_candpart, frame, nb);
+                                               nullptr, nullptr, d_candpart, frame, nb, d_kmask);
     k_sum_reduce<<<1, 256, 0, s>>>(eblocks, d_candpart, d_cand);
     VX_HIP(hipGetLastError());
     tb = 0;
@@ -1678,7 +1719,7 @@ extern "C" int32_t vx_contact_project(uint32_t n, float *positions, float *previ
     uint64_t ncontacts = 0;
     if (P > 0) {
         k_bin_pairs<true><<<eblocks, 256, 0, s>>>(entries, d_keys2, d_vals2, d_lohi, d_flags, d_rest, d_adj, d_ecount,
-                                                  d_eends, d_pk, nullptr, frame, nb);
+                                                  d_eends, d_pk, nullptr, frame, nb, d_kmask);
         VX_HIP(hipGetLastError());
         tb = 0;
         VX_HIP(rocprim::radix_sort_keys(nullptr, tb, d_pk, d_pairs, size_t(P), 0, int(2 * nb), s));
@@ -1718,8 +1759,8 @@ extern "C" int32_t vx_contact_project(uint32_t n, float *positions, float *previ
         uint32_t relaxed = 0;  // passes done by relaxation
         bool finished = false;  // a relaxed pass moved nothing, or all passes are done
         if (ctx.contact_method == VX_CONTACT_AUTO && P < kRxMaxPairs) {
-            uint32_t *d_rflag, *d_sflag, *d_dl0, *d_dl1, *d_mcnt, *d_mlist, *d_vchg, *d_vstamp, *d_vlist, *d_jend,
-                *d_joff, *d_jkey, *d_jkey2, *d_jlist, *d_pstart, *d_pend, *d_rx;
+            uint32_t *d_rflag, *d_sflag, *d_dl0, *d_dl1, *d_mcnt, *d_mlist, *d_moff, *d_mdeg, *d_vchg, *d_vstamp,
+                *d_vlist, *d_jend, *d_joff, *d_jkey, *d_jkey2, *d_jlist, *d_pstart, *d_pend, *d_rx;
             float *d_rsi, *d_rsj, *d_ssi, *d_ssj;
             VX_BUF("contact.rx.rflag", 4 * size_t(P), d_rflag);
             VX_BUF("contact.rx.rsi", 4 * size_t(P), d_rsi);
@@ -1732,7 +1773,9 @@ extern "C" int32_t vx_contact_project(uint32_t n, float *positions, float *previ
             VX_BUF("contact.rx.jlist", 4 * size_t(P), d_jlist);
             VX_BUF("contact.rx.jkey", 4 * size_t(P), d_jkey);
             VX_BUF("contact.rx.jkey2", 4 * size_t(P), d_jkey2);
-            VX_BUF("contact.rx.mlist", 4 * size_t(kMov) * N, d_mlist);
+            VX_BUF("contact.rx.mlist", 8 * size_t(P), d_mlist);
+            VX_BUF("contact.rx.moff", 4 * N, d_moff);
+            VX_BUF("contact.rx.mdeg", 4 * N, d_mdeg);
             VX_BUF("contact.rx.mcnt", 4 * N, d_mcnt);
             VX_BUF("contact.rx.vchg", 4 * N, d_vchg);
             VX_BUF("contact.rx.vstamp", 4 * N, d_vstamp);
@@ -1756,6 +1799,13 @@ extern "C" int32_t vx_contact_project(uint32_t n, float *positions, float *previ
             VX_HIP(rocprim::radix_sort_pairs(d_stmp, sb, d_jkey, d_jkey2, rocprim::counting_iterator<uint32_t>(0),
                                              d_jlist, size_t(P), 0, int(nb), s));
             k_rx_bounds<<<pb, 256, 0, s>>>(P, d_pairs, d_jkey2, d_joff, d_jend, d_pstart, d_pend);
+            // each voxel's mover list has room for all its pairs
+            k_rx_degrees<<<vb, 256, 0, s>>>(n, d_joff, d_jend, d_pstart, d_pend, d_mdeg);
+            size_t db = 0;
+            VX_HIP(rocprim::exclusive_scan(nullptr, db, d_mdeg, d_moff, 0u, N, rocprim::plus<uint32_t>(), s));
+            void *d_dtmp = nullptr;
+            if ((st = buffer("contact.rx.tmp2", db ? db : 16, &d_dtmp)) != VX_OK) return st;
+            VX_HIP(rocprim::exclusive_scan(d_dtmp, db, d_mdeg, d_moff, 0u, N, rocprim::plus<uint32_t>(), s));
             VX_HIP(hipGetLastError());
             uint32_t *pin_rx = pinned + 256;  // rx words (16: the evaluations, u64, at word 8), then ctl at 272
             uint32_t *pin_ctl = pinned + 272;
@@ -1777,14 +1827,14 @@ extern "C" int32_t vx_contact_project(uint32_t n, float *positions, float *previ
                     k_rx_init_first<<<pb, 256, 0, s>>>(P, dl[cur], d_rx, cur);
                 } else {
                     k_rx_init_words<<<1, 1, 0, s>>>(d_rx, cur);
-                    k_rx_init_pass<<<blocks_for(N, 8), 256, 0, s>>>(n, d_mcnt, d_joff, d_jend, d_jlist, d_pstart,
+                    k_rx_init_pass<<<blocks_for(N, 256), 256, 0, s>>>(n, d_mcnt, d_joff, d_jend, d_jlist, d_pstart,
                                                                     d_pend, d_rflag,
                                                       d_rsi, d_rsj, dl[cur], d_rx, cur, P);
                 }
             };
             auto enqueue_commit = [&]() {
                 k_set_word<<<1, 1, 0, s>>>(d_ctl + C_CHANGED, 0);
-                k_rx_commit_voxels<<<vb, 256, 0, s>>>(n, d_mcnt, d_mlist, d_rflag, d_rsi, d_rsj, rec, d_prev,
+                k_rx_commit_voxels<<<vb, 256, 0, s>>>(n, d_mcnt, d_moff, d_mlist, d_rflag, d_rsi, d_rsj, rec, d_prev,
                                                       d_wfirst, d_wlast, d_ctl);
                 k_rx_commit_pairs<<<pb, 256, 0, s>>>(P, d_rflag, d_code, d_sep);
             };
@@ -1808,12 +1858,12 @@ extern "C" int32_t vx_contact_project(uint32_t n, float *positions, float *previ
                         // the later ones take what the last check saw, or less
                         const uint32_t g = pass_start ? lb_all : std::min<uint32_t>(kRxBlocksLater, grid_for(seen));
                         pass_start = false;
-                        k_rx_eval<<<g, 256, 0, s>>>(c, dl[c], d_rx, P, d_pairs, rec, d_mcnt, d_mlist, d_rflag, d_rsi,
+                        k_rx_eval<<<g, 256, 0, s>>>(c, dl[c], d_rx, P, d_pairs, rec, d_mcnt, d_moff, d_mlist, d_rflag, d_rsi,
                                                     d_rsj, d_sflag, d_ssi, d_ssj, d_evals);
                         k_rx_apply<<<g, 256, 0, s>>>(c, R + 1, dl[c], d_rx, P, d_pairs, d_rflag, d_rsi, d_rsj,
                                                      d_sflag, d_ssi, d_ssj, d_vchg, d_vstamp, d_vlist, n);
                         k_rx_rebuild<<<g, 256, 0, s>>>(c, d_rx, P, d_vlist, d_joff, d_jend, d_jlist, d_pstart, d_pend,
-                                                       d_rflag, d_mcnt, d_mlist, d_vchg, dl[c ^ 1u]);
+                                                       d_rflag, d_mcnt, d_moff, d_mlist, d_vchg, dl[c ^ 1u]);
                     }
                     VX_HIP(hipGetLastError());
                     VX_HIP(hipMemcpyAsync(pin_rx, d_rx, 16 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
