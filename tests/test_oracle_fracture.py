@@ -125,6 +125,23 @@ def test_wood_has_tension_and_compression_anisotropy(O, mode):
 
 
 @pytest.mark.parametrize("mode", MODES)
+def test_wood_limits_along_and_across_the_grain(O, mode):
+    """actual_vertical_wood_bonds_are_stronger_than_horizontal_bonds
+    (src/solver.rs:1046-1051): a bond along the grain of solid wood holds 100 MPa
+    in tension, 50 MPa in compression and 12 MPa in shear, one across it 2.5,
+    2 and 3 MPa, each within 1 Pa."""
+    import json
+    import os
+    g = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_known_answers.json")))["wood_limits"]
+    w = strength("wood_builtin")
+    along = O.pair_limits(w, w, [1, 1], [1, 1], mode)
+    across = O.pair_limits(w, w, [0, 0], [1, 1], mode)
+    assert abs(along[0] - g["tension_along"]) < 1.0 and abs(across[0] - g["tension_across"]) < 1.0
+    assert abs(along[1] - g["compression_along"]) < 1.0 and abs(across[1] - g["compression_across"]) < 1.0
+    assert abs(along[2] - g["shear_along"]) < 1.0 and abs(across[2] - g["shear_across"]) < 1.0
+
+
+@pytest.mark.parametrize("mode", MODES)
 def test_wood_fracture_energy_follows_bond_alignment(O, mode):
     w = strength("wood_builtin")
     gc = lambda al, an: O.pair_limits(w, w, [al, al], [an, an], mode)[7]  # noqa: E731

@@ -54,6 +54,22 @@ def test_dot_and_ratio_known_answers(O, known):
     assert O.guarded_ratio(1.0, 1e-31) == 0.0
 
 
+def test_seventeen_partials_known_answer(O, known):
+    """The reference's 17-partial reduction check (validation/cuda_sparse_probe.py:232-245):
+    partials 1..17 reduce to 153. Here the partials are the per-256-row blocks
+    of a row dot product (the first component of each block carries the value,
+    the rest are zero), over 17 blocks as its 4,203-row captures use."""
+    g = known["partial_sum_17"]
+    vals = np.array(g["values"], dtype=np.float64)
+    rows = 256 * len(vals)
+    left = np.zeros(3 * rows)
+    left[np.arange(len(vals)) * 3 * 256] = vals
+    right = np.ones(3 * rows)
+    assert O.dot_rows(left, right) == g["expected"] == float(vals.sum())
+    # a duplicated or dropped partial would change the total
+    assert O.dot_rows(left[: 3 * (rows - 256)], right[: 3 * (rows - 256)]) == g["expected"] - vals[-1]
+
+
 def synthetic_rows(nx, ny, nz, known):
     g = known["synthetic_matvec"]
     n = nx * ny * nz

@@ -16,26 +16,38 @@
 //    Pairs joined by intact bonds inside their lattice cell are dropped
 //    (local_continuum_pair, :520-550) and the rest radix sorted by (i, j).
 //
-//  * Projection. One workgroup of 1024 threads takes a window of the smallest
-//    unprocessed pairs, evaluates every pair on the current positions
-//    (swept, overlap and touching tests, :552-621, :441-460), and commits the
-//    pairs whose sequential predecessors cannot change what they read: a pair
-//    is held back when an earlier pair of the window moves one of its voxels,
-//    or when an earlier held-back pair shares a voxel that can move (not
-//    kinematic). Committed pairs touch disjoint moving voxels, so they
-//    commute with the held-back ones and the result equals the sequential
-//    sweep. The window's smallest pair always commits.
+//  * Passes by relaxation (the default). A pass's result is the solution of
+//    its pair recurrence: pair k evaluates the pass body (swept, overlap and
+//    touching tests, :552-621, :441-460) on the positions its voxels have
+//    after the moves of the earlier pairs that touch them. Every pair is
+//    evaluated on the positions the current outcomes of its earlier pairs
+//    give, the pairs whose inputs changed are evaluated again, and so on until
+//    no outcome changes: that fixed point is the sequential sweep's result,
+//    reached in about as many rounds as the longest chain of moves feeding
+//    each other. Only then are the moves applied, per voxel in pair order.
+//
+//  * Passes by windows (VX_CONTACT_WINDOWS, and any pass the relaxation does
+//    not finish within its budget). One workgroup of 1024 threads takes a
+//    window of the smallest unprocessed pairs, evaluates every pair on the
+//    current positions, and commits the pairs whose sequential predecessors
+//    cannot change what they read: a pair is held back when an earlier pair
+//    of the window moves one of its voxels, or when an earlier held-back pair
+//    shares a voxel that can move (not kinematic). Committed pairs touch
+//    disjoint moving voxels, so they commute with the held-back ones and the
+//    result equals the sequential sweep. The window's smallest pair always
+//    commits.
 //
 //  * Later passes. A pair whose voxels have not moved since its last
-//    evaluation, which did not separate, would repeat that outcome; the passes
-//    after the first evaluate candidates only, check every skipped pair
-//    afterwards, and redo the pass from a snapshot with more pairs when one
-//    was due.
+//    evaluation, which did not separate, would repeat that outcome: both
+//    methods start a later pass from the other pairs only (the windows check
+//    every skipped pair afterwards and redo the pass from a snapshot with
+//    more pairs when one was due).
 //
 //  * Signed zeros. `separate` also adds exact zeros to the components off the
 //    normal (and to immovable endpoints). Adding -0 changes nothing and adding
 //    +0 only turns a -0 into +0, so those additions commute with everything:
-//    they run after the real moves of a window, as a store of +0 over -0.
+//    the windows run them after the real moves of a window, as a store of +0
+//    over -0; the relaxation adds every component as separate() does.
 //
 // All arithmetic is f32 in the reference's order without contraction (Rust
 // never fuses a multiply and an add), with IEEE division.
