@@ -288,22 +288,24 @@ def batched(steps: int = 1, count: int = 256, preconditioner: int = 0, cpu: bool
     return out
 
 
-def served(count: int, compare: bool = True, cpu: bool = True, preconditioner: int = 0) -> dict:
+def served(count: int, compare: bool = True, cpu: bool = True, preconditioner: int = 0, policy: str = "sjf") -> dict:
     """The served stream at its size mix, bounded: the first `count` arrivals
     of Poisson arrivals at 200/s, REVs log-uniform in [1e3, 2e6], aspect
     ratios U[0.5, 2], ten warm-started steps per request, solves to relative
     residual BATCH_TOL, continuous admission into an arena sized from a 200 GB
-    HBM budget. p50/p99 latency (arrival to result on the host) and aggregate
+    HBM budget, scheduled by `policy` (vxsim.serve: "sjf" holds the larger
+    scenes while the smaller ones run, "fifo" runs every admitted scene in
+    every tick). p50/p99 latency (arrival to result on the host) and aggregate
     DOF-PCG-iters/s, beside the plain FP64 structured solve handling the same
-    requests one at a time; with `cpu`, the three smallest requests' ten steps
-    against the oracle's FP64 steps."""
+    requests one at a time in arrival order; with `cpu`, the three smallest
+    requests' ten steps against the oracle's FP64 steps."""
     from vxsim import batch, grid, serve
     sp = grid.solve_params(tolerance=BATCH_TOL, inner_tolerance=INNER_TOL, warm_start=True, max_inner=40000,
                            preconditioner=preconditioner)
     t = time.perf_counter()
     reqs = serve.request_stream(200.0, 60.0, 1e3, 2e6, 10, seed=4, max_requests=count, solve=sp)
     gen_s = time.perf_counter() - t
-    with serve.Server.for_budget(200 << 30, max_scenes=256, keep_results=cpu) as S:
+    with serve.Server.for_budget(200 << 30, max_scenes=256, keep_results=cpu, policy=policy) as S:
         t0 = time.perf_counter()
         S.run(reqs)
         wall = time.perf_counter() - t0
@@ -316,7 +318,7 @@ def served(count: int, compare: bool = True, cpu: bool = True, preconditioner: i
     out = {"workload": "first %d arrivals of the served stream (Poisson 200/s, REVs log-uniform [1e3, 2e6], "
                        "aspect U[0.5, 2], 10 warm-started steps each, rel. residual %g, 200 GB HBM budget)"
                        % (count, BATCH_TOL),
-           "p50_s": lat[50], "p99_s": lat[99], "requests": len(reqs), "done": len(done),
+           "policy": policy, "p50_s": lat[50], "p99_s": lat[99], "requests": len(reqs), "done": len(done),
            "revs": revs, "total_revs": int(sum(revs)), "dof_iters_per_s": dof_iters / wall, "wall_s": wall,
            "ticks": ticks, "scene_generation_s": gen_s,
            "max_rel_residual": max(r.report["last_relative_residual"] for r in reqs if r.report)}
@@ -895,6 +897,11 @@ def main():
         print("bench: batched legs done", file=sys.stderr, flush=True)
     if not args.no_serve:
         extra["served"] = served(args.serve_count, compare=not args.no_compare, cpu=cpu_on)
+        # the same requests with every admitted scene in every tick
+        fifo = served(args.serve_count, compare=False, cpu=False, policy="fifo")
+        fifo["p50_vs_sjf"] = fifo["p50_s"] / extra["served"]["p50_s"]
+        fifo["p99_vs_sjf"] = fifo["p99_s"] / extra["served"]["p99_s"]
+        extra["served_fifo"] = fifo
         # the same requests with the red-black preconditioner (half the iterations)
         rbs = served(args.serve_count, compare=False, cpu=False, preconditioner=1)
         rbs["p50_vs_block_jacobi"] = extra["served"]["p50_s"] / rbs["p50_s"]

@@ -483,6 +483,11 @@ int32_t vx_batch_download_bonds(vx_batch *batch, uint32_t slot, uint32_t *neighb
                                 float *plastic_strain, float *history);
 /* Bond forces of one scene, as vx_scene_download_bond_forces. */
 int32_t vx_batch_download_bond_forces(vx_batch *batch, uint32_t slot, double *last_force, double *lambda);
+/* Hold (hold != 0) or resume a running scene: a held scene sits out the
+ * ticks (its state and place in the arena are kept; a tick already in flight
+ * completes) until it is resumed. The serving loop holds the larger scenes
+ * while the smaller ones finish (shortest job first). */
+int32_t vx_batch_hold(vx_batch *batch, uint32_t slot, uint32_t hold);
 /* Free a finished (or failed) scene's tiles and slot. */
 int32_t vx_batch_release(vx_batch *batch, uint32_t slot);
 

@@ -46,7 +46,7 @@ def test_admission_waits_for_free_tiles(gpu, stream):
     from vxsim import batch, serve
     biggest = max(-(-r.scene.n // batch.TILE) for r in stream)
     # Room for about two of the largest boxes: later requests must wait.
-    with serve.Server(capacity_rows=2 * biggest * 512, max_scenes=4) as S:
+    with serve.Server(capacity_rows=2 * biggest * 512, max_scenes=4, policy="fifo") as S:
         S.run(stream)
     assert all(r.state == batch.DONE for r in stream)
     assert all(r.admitted >= r.arrival and r.finished > r.admitted for r in stream)
@@ -69,3 +69,33 @@ def test_served_red_black_results_equal_results_alone(gpu):
         ref = alone(req)
         for k in ("positions", "rotations", "velocities"):
             assert np.array_equal(req.result[k], ref[k]), (req.rid, k)
+
+
+def test_sjf_holds_the_larger_scenes_and_results_equal_results_alone(gpu, stream):
+    """Shortest job first with room for about two small scenes per tick: the
+    larger scenes are held (no work, state kept in HBM) while smaller ones run,
+    then resumed; every result is still bit-identical to the scene alone, and
+    the smallest request is back before the largest."""
+    from vxsim import batch, serve
+    sizes = sorted(r.scene.n for r in stream)
+    with serve.Server(capacity_rows=64 * 512, max_scenes=8, keep_results=True, policy="sjf",
+                      active_revs=sizes[1] + sizes[2]) as S:
+        S.run(stream)
+    for req in stream:
+        assert req.state == batch.DONE, (req.rid, req.status, req.report)
+        ref = alone(req)
+        for k in ("positions", "rotations", "velocities"):
+            assert np.array_equal(req.result[k], ref[k]), (req.rid, k)
+    small = min(stream, key=lambda r: r.scene.n)
+    large = max(stream, key=lambda r: r.scene.n)
+    assert small.finished < large.finished
+    finishes = sorted(r.finished for r in stream)
+    assert large.finished >= finishes[len(finishes) // 2]
+
+
+def test_policy_arguments_are_checked(gpu):
+    from vxsim import serve
+    with pytest.raises(ValueError):
+        serve.Server(capacity_rows=512, policy="lifo")
+    with pytest.raises(ValueError):
+        serve.Server(capacity_rows=512, active_revs=0)

@@ -138,7 +138,12 @@ struct DevArena {
     // inner vectors
     float *d, *r, *z, *p0, *p1, *ap;
     double *dd, *rd, *zd, *p0d, *p1d, *apd;
+    // With a list: workgroup L of a launch takes tile tile_list[L] (tiles of
+    // the scenes in the tick, padded with kNoTile), so a tick of a few scenes
+    // does not launch the tiles of the others.
+    const uint32_t *tile_list;
 };
+constexpr uint32_t kNoTile = 0xFFFFFFFFu;
 
 // ---- tile context -----------------------------------------------------------
 // Physical workgroup b runs on XCD b % 8; logical tiles are dealt to the XCDs
@@ -178,7 +183,8 @@ __global__ __launch_bounds__(kBlock) void spread_bits(DevArena A, BitsArg arg) {
 // The tile of this workgroup and its scene; false for a free tile. The
 // scene's phase bits are loaded but not tested.
 __device__ __forceinline__ bool tile_base(const DevArena &A, Tile &c) {
-    c.t = logical_tile(blockIdx.x, A.chunk);
+    const uint32_t L = logical_tile(blockIdx.x, A.chunk);
+    c.t = A.tile_list ? A.tile_list[L] : L;
     if (c.t >= A.cap_tiles) return false;
     c.g = A.tiles[c.t];
     c.bits = A.tile_bits[c.t];
@@ -2434,6 +2440,7 @@ struct SlotHost {
     uint64_t inner = 0, sweeps = 0;
     double last_rel = 0.0, last_angle = 0.0;
     bool manual = false;  // driven by the vx_scene_* stage calls
+    bool held = false;    // sits out the ticks until released (vx_batch_hold)
     bool has_strength = false;
     bool has_tool = false;      // a vx_scene with a rigid tool (vx_scene_set_tool)
     vx_fracture_report frac{};  // fracture totals since admission (a vx_scene: this step)
@@ -2454,6 +2461,7 @@ struct Flight {
     SlotState *st_dev = nullptr;  // its device address
     hipEvent_t done = nullptr;
     hipEvent_t frac0 = nullptr, frac1 = nullptr;  // around the fracture kernels
+    uint32_t *list = nullptr;                     // pinned: the tick's tile list (when it has one)
     bool active = false;
     bool frac_timed = false;
 };
@@ -2492,6 +2500,9 @@ struct vx_batch {
     uint32_t spread_tiles = 0;
     bool spread_dirty = true;
     vx::ToolHost *tool = nullptr;  // the rigid tool of a one-scene arena
+    // Tile list of the last tick that had one (its device copy is d_list).
+    uint32_t *d_list = nullptr;
+    std::vector<uint32_t> list_last;
 };
 
 struct vx_scene {
@@ -2683,7 +2694,8 @@ bool any_big(const vx_batch *b) {
 
 int32_t inner_graph(vx_batch *b, int f64, bool rb, uint32_t tiles, hipGraphExec_t *out) {
     const bool big = any_big(b);
-    const uint64_t key = (uint64_t(tiles) << 3) | (uint64_t(rb) << 2) | (uint64_t(big) << 1) | uint64_t(f64);
+    const uint64_t key = (uint64_t(tiles) << 4) | (uint64_t(b->A.tile_list != nullptr) << 3) | (uint64_t(rb) << 2) |
+                         (uint64_t(big) << 1) | uint64_t(f64);
     for (auto &kv : b->g_inner)
         if (kv.first == key) {
             *out = kv.second;
@@ -2717,9 +2729,45 @@ int32_t launch_tick(vx_batch *b, Flight &f) {
             rb0[prec] = rb0[prec] || red_black;
         }
     }
-    const uint32_t tiles = launch_tiles(b);
+    // Tiles in use (rounded up), the physical range the tile bits cover; the
+    // launches cover it too unless the tick's scenes hold at most half of it,
+    // in which case they take the list of those scenes' tiles.
+    const uint32_t span = launch_tiles(b);
+    uint32_t tiles = span;
+    b->A.tile_list = nullptr;
+    if (!b->single) {
+        uint64_t act = 0;
+        std::vector<std::pair<uint32_t, uint32_t>> runs;  // (first tile, tiles) of the tick's scenes
+        for (uint32_t i = 0; i < b->max_slots; ++i)
+            if (f.bits[i] && b->slots[i].used) {
+                runs.emplace_back(b->slots[i].geo.tile0, b->slots[i].geo.ntiles);
+                act += b->slots[i].geo.ntiles;
+            }
+        const uint64_t unit = uint64_t(kXcds) * b->chunk;
+        uint64_t g = unit;
+        while (g < act) g *= 2;
+        if (2 * g <= span) {
+            std::sort(runs.begin(), runs.end());
+            uint32_t k = 0;
+            for (const auto &r : runs)
+                for (uint32_t t = 0; t < r.second; ++t) f.list[k++] = r.first + t;
+            while (k < g) f.list[k++] = kNoTile;
+            if (b->list_last.size() != g || !std::equal(f.list, f.list + g, b->list_last.begin())) {
+                VX_HIP(hipMemcpyAsync(b->d_list, f.list, sizeof(uint32_t) * g, hipMemcpyHostToDevice, s));
+                b->list_last.assign(f.list, f.list + g);
+            }
+            b->A.tile_list = b->d_list;
+            tiles = uint32_t(g);
+        }
+    }
+    // every kernel of this tick takes the arena with or without the list; the
+    // arena is reset to no list once they are enqueued
+    struct ListReset {
+        DevArena &A;
+        ~ListReset() { A.tile_list = nullptr; }
+    } list_reset{b->A};
     const dim3 grid(tiles), blk(kBlock), sg(b->max_slots);
-    if (b->spread_dirty || tiles != b->spread_tiles || !std::equal(f.bits, f.bits + b->max_slots, b->spread_last.begin())) {
+    if (b->spread_dirty || span != b->spread_tiles || !std::equal(f.bits, f.bits + b->max_slots, b->spread_last.begin())) {
         BitsArg arg{};
         if (b->max_slots <= kArgSlots) {
             arg.n = b->max_slots;
@@ -2727,10 +2775,10 @@ int32_t launch_tick(vx_batch *b, Flight &f) {
         } else {
             VX_HIP(hipMemcpyAsync(b->d_bits, f.bits, sizeof(uint32_t) * b->max_slots, hipMemcpyHostToDevice, s));
         }
-        const uint32_t nb = std::max((tiles + kBlock - 1) / kBlock, (arg.n + kBlock - 1) / kBlock);
+        const uint32_t nb = std::max((span + kBlock - 1) / kBlock, (arg.n + kBlock - 1) / kBlock);
         hipLaunchKernelGGL(spread_bits, dim3(nb), blk, 0, s, b->A, arg);
         std::copy(f.bits, f.bits + b->max_slots, b->spread_last.begin());
-        b->spread_tiles = tiles;
+        b->spread_tiles = span;
         b->spread_dirty = false;
     }
     if (any & B_PREDICT) hipLaunchKernelGGL(substep_predict, grid, blk, 0, s, b->A);
@@ -2991,7 +3039,7 @@ int32_t create_batch(uint64_t capacity_rows, uint32_t max_scenes, uint32_t chunk
         {(void **)&A.apd, 24 * R}, {(void **)&A.strength, 4 * kStrength * R}, {(void **)&A.hist, 12 * kHistory * R},
         {(void **)&A.rot_asm, 32 * R}, {(void **)&A.partials, 8 * kSlots * size_t(b->cap_tiles)},
         {(void **)&b->d_tiles, sizeof(SlotGeo) * b->cap_tiles}, {(void **)&A.tile_bits, 4 * size_t(b->cap_tiles)},
-        {(void **)&b->d_bits, 4 * size_t(max_scenes)},
+        {(void **)&b->d_bits, 4 * size_t(max_scenes)}, {(void **)&b->d_list, 4 * size_t(b->cap_tiles)},
         {(void **)&b->d_geo, sizeof(SlotGeo) * max_scenes}, {(void **)&b->d_prm, sizeof(SlotParams) * max_scenes},
         {(void **)&b->d_st, sizeof(SlotState) * max_scenes},
     };
@@ -3028,6 +3076,7 @@ int32_t create_batch(uint64_t capacity_rows, uint32_t max_scenes, uint32_t chunk
               hipStreamSynchronize(ctx.stream) == hipSuccess;
     for (auto &f : b->fl) {
         ok = ok && hipHostMalloc((void **)&f.bits, sizeof(uint32_t) * max_scenes, hipHostMallocDefault) == hipSuccess &&
+             hipHostMalloc((void **)&f.list, sizeof(uint32_t) * b->cap_tiles, hipHostMallocDefault) == hipSuccess &&
              hipHostMalloc((void **)&f.st, sizeof(SlotState) * max_scenes, hipHostMallocCoherent) == hipSuccess &&
              hipHostGetDevicePointer((void **)&f.st_dev, f.st, 0) == hipSuccess &&
              hipEventCreateWithFlags(&f.done, hipEventDisableTiming) == hipSuccess &&
@@ -3038,6 +3087,7 @@ int32_t create_batch(uint64_t capacity_rows, uint32_t max_scenes, uint32_t chunk
     if (!ok) {
         for (auto &f : b->fl) {
             if (f.bits) (void)hipHostFree(f.bits);
+            if (f.list) (void)hipHostFree(f.list);
             if (f.st) (void)hipHostFree(f.st);
             if (f.done) (void)hipEventDestroy(f.done);
             if (f.frac0) (void)hipEventDestroy(f.frac0);
@@ -3370,6 +3420,7 @@ extern "C" int32_t vx_batch_destroy(vx_batch *b) {
     reset_graphs(b);
     for (auto &f : b->fl) {
         if (f.bits) (void)hipHostFree(f.bits);
+        if (f.list) (void)hipHostFree(f.list);
         if (f.st) (void)hipHostFree(f.st);
         if (f.done) (void)hipEventDestroy(f.done);
         if (f.frac0) (void)hipEventDestroy(f.frac0);
@@ -3435,7 +3486,7 @@ extern "C" int32_t vx_batch_tick(vx_batch *b, uint32_t *out_running) {
     for (uint32_t i = 0; i < b->max_slots; ++i) {
         const SlotHost &h = b->slots[i];
         uint32_t bits = 0;
-        if (h.used && !h.manual && !(b->inflight[i] & ~kChunkBits)) bits = next_bits(h, b->known[i]);
+        if (h.used && !h.manual && !h.held && !(b->inflight[i] & ~kChunkBits)) bits = next_bits(h, b->known[i]);
         f.bits[i] = bits;
         launched += bits ? 1 : 0;
     }
@@ -3570,6 +3621,16 @@ extern "C" int32_t vx_batch_download_bond_forces(vx_batch *b, uint32_t slot, dou
     if (!b || slot >= b->max_slots || !b->slots[slot].used) return fail(VX_ERR_INVALID_ARGUMENT, "invalid slot");
     Lock lock;
     return download_bond_forces(b, slot, last_force, lambda);
+}
+
+extern "C" int32_t vx_batch_hold(vx_batch *b, uint32_t slot, uint32_t hold) {
+    clear_error();
+    if (!b || slot >= b->max_slots) return fail(VX_ERR_INVALID_ARGUMENT, "invalid slot");
+    Lock lock;
+    SlotHost &h = b->slots[slot];
+    if (!h.used) return fail(VX_ERR_INVALID_ARGUMENT, "slot %u holds no scene", slot);
+    h.held = hold != 0;
+    return VX_OK;
 }
 
 extern "C" int32_t vx_batch_release(vx_batch *b, uint32_t slot) {

@@ -197,6 +197,7 @@ SIGNATURES = [
     ("vx_batch_states", ctypes.c_int32, [ctypes.c_void_p, c_u32p]),
     ("vx_batch_download", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint32, c_dp, c_dp, c_dp]),
     ("vx_batch_release", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint32]),
+    ("vx_batch_hold", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32]),
     ("vx_scene_download_bonds", ctypes.c_int32, [ctypes.c_void_p, c_u32p, c_fp, c_fp, c_fp]),
     ("vx_scene_set_targets_and_loads", ctypes.c_int32, [ctypes.c_void_p, c_fp, c_fp]),
     ("vx_batch_download_bonds", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint32, c_u32p, c_fp, c_fp, c_fp]),

@@ -137,6 +137,10 @@ class Batch:
         return download_bond_forces(lib().vx_batch_download_bond_forces, self.handle, int(slot),
                                     n=self._rows[int(slot)])
 
+    def hold(self, slot: int, hold: bool = True):
+        """Hold a running scene out of the ticks (or resume it)."""
+        check(lib().vx_batch_hold(self.handle, int(slot), 1 if hold else 0))
+
     def release(self, slot: int):
         check(lib().vx_batch_release(self.handle, int(slot)))
         self._rows.pop(int(slot), None)

@@ -2,12 +2,25 @@
 
 Requests (a box scene, a number of fixed steps, step parameters) arrive over
 time. Each scheduler iteration admits every queued request that has arrived
-and fits in the arena (oldest first; a request that does not fit yet waits
-while smaller ones behind it may go), advances all admitted scenes by one
-tick, and retires finished scenes: their state is read back, their tiles
-freed. Matrices, rotations and warm-start vectors stay resident in HBM for
-the whole life of a scene. The arena is sized from an HBM budget
-(`vx_batch_bytes_per_row`).
+and fits in the arena (a request that does not fit yet waits while others
+behind it may go), advances the admitted scenes by one tick, and retires
+finished scenes: their state is read back, their tiles freed. Matrices,
+rotations and warm-start vectors stay resident in HBM for the whole life of a
+scene. The arena is sized from an HBM budget (`vx_batch_bytes_per_row`).
+
+Policies:
+  * "fifo": admission oldest first, every admitted scene in every tick. A
+    tick's launches cover every running scene's tiles, so a small request
+    takes about as long as the largest one it shares the arena with.
+  * "sjf" (shortest job first, the default): admission smallest first (REVs x
+    steps, divided by 1 + waiting time / `aging_s`), and the small running
+    scenes (fewer than active_revs / 2 REVs) run first: each tick takes the
+    smallest of them up to `active_revs` REVs together (at least one), and
+    holds the rest (vx_batch_hold: state kept in HBM, no work; a tick
+    launches only its scenes' tiles). The large scenes all run together, as
+    many per tick as possible, when no small scene runs, or in every tick once
+    one of them has waited `aging_s` since its arrival (a stream of small
+    requests does not hold them forever).
 
 Latency of a request = time its result is back on the host - its arrival.
 """
@@ -43,13 +56,51 @@ class Request:
         return None if self.finished is None else self.finished - self.arrival
 
 
+POLICIES = ("sjf", "fifo")
+
+
 class Server:
     def __init__(self, capacity_rows: int, max_scenes: int = 256, xcd_chunk_tiles: int = 0,
-                 keep_results: bool = False):
+                 keep_results: bool = False, policy: str = "sjf", active_revs: int = 1 << 20,
+                 aging_s: float = 10.0):
+        if policy not in POLICIES:
+            raise ValueError("policy must be one of %s" % (POLICIES,))
+        if active_revs <= 0 or aging_s <= 0:
+            raise ValueError("active_revs and aging_s must be positive")
         self.batch = _batch.Batch(capacity_rows, max_scenes, xcd_chunk_tiles)
         self.max_scenes = max_scenes
         self.keep_results = keep_results
+        self.policy = policy
+        self.active_revs = int(active_revs)
+        self.aging_s = float(aging_s)
         self.ticks = 0
+
+    def _age(self, req: "Request", now: float) -> float:
+        return 1.0 + max(now - req.arrival, 0.0) / self.aging_s
+
+    def _key(self, req: "Request", now: float) -> float:
+        """Size of a request for shortest-job-first, reduced by its waiting time."""
+        return req.scene.n * req.steps / self._age(req, now)
+
+    def _choose(self, running: dict, held: dict, now: float) -> None:
+        """Hold or resume the running scenes for the next tick (policy "sjf")."""
+        small = [slot for slot in running if running[slot].scene.n < self.active_revs / 2]
+        large = [slot for slot in running if running[slot].scene.n >= self.active_revs / 2]
+        run = set()
+        total = 0
+        for slot in sorted(small, key=lambda q: self._key(running[q], now)):
+            n = running[slot].scene.n
+            if run and total + n > self.active_revs:
+                break
+            run.add(slot)
+            total += n
+        if not small or any(now - running[q].arrival >= self.aging_s for q in large):
+            run.update(large)
+        for slot in running:
+            hold = slot not in run
+            if held.get(slot, False) != hold:
+                self.batch.hold(slot, hold)
+                held[slot] = hold
 
     @classmethod
     def for_budget(cls, hbm_bytes: int, **kw) -> "Server":
@@ -69,9 +120,11 @@ class Server:
         """Serve `requests` (sorted by arrival) to completion; arrival times
         are honoured against `clock` from the start of the call."""
         B = self.batch
+        sjf = self.policy == "sjf"
         pending = sorted(requests, key=lambda r: r.arrival)
         queue: List[Request] = []
         running = {}  # slot -> Request
+        held = {}     # slot -> whether the scene is held
         t0 = clock()
         nxt = 0
         while nxt < len(pending) or queue or running:
@@ -79,8 +132,11 @@ class Server:
             while nxt < len(pending) and pending[nxt].arrival <= now:
                 queue.append(pending[nxt])
                 nxt += 1
-            # admission: oldest first, skipping requests that do not fit yet
+            # admission: oldest (fifo) or smallest (sjf) first, skipping
+            # requests that do not fit yet
             if queue:
+                if sjf:
+                    queue.sort(key=lambda r: self._key(r, now))
                 free = B.free_rows()[0]
                 kept = []
                 for req in queue:
@@ -102,6 +158,8 @@ class Server:
                     if wait > 0:
                         time.sleep(min(wait, 0.01))
                 continue
+            if sjf:
+                self._choose(running, held, clock() - t0)
             B.tick()
             self.ticks += 1
             if on_tick:
@@ -121,6 +179,7 @@ class Server:
                     req.status = rep["status"]
                     req.report = rep
                     B.release(slot)
+                    held.pop(slot, None)
         return requests
 
 
