@@ -59,3 +59,61 @@ def test_large_block_scene():
     assert targeted.sum() == 6 * 2 and np.isnan(s.targets[targeted, :2]).all()
     assert (s.keys[targeted, 2] == 9).all() and (s.kinematic[s.keys[:, 2] == 0] == 1).all()
     assert s.bond_count() == 23 * 20 * 10 + 24 * 19 * 10 + 24 * 20 * 9
+
+
+class _FakeBatch:
+    """Records vx_batch_hold calls instead of making them."""
+    def __init__(self):
+        self.calls = []
+
+    def hold(self, slot, hold=True):
+        self.calls.append((slot, bool(hold)))
+
+
+def _server(active_revs, aging_s=10.0):
+    from vxsim import serve
+    S = serve.Server.__new__(serve.Server)
+    S.batch, S.active_revs, S.aging_s, S.policy = _FakeBatch(), active_revs, aging_s, "sjf"
+    return S
+
+
+def _req(rid, n, arrival=0.0, steps=10):
+    from types import SimpleNamespace
+    from vxsim import serve
+    return serve.Request(rid=rid, arrival=arrival, scene=SimpleNamespace(n=n), steps=steps, params=None)
+
+
+def test_sjf_runs_the_small_scenes_first_and_the_large_together():
+    """Small scenes (below half the active budget) run smallest first up to the
+    budget; large ones are held while any small one runs, then all run
+    together; a large one that has waited the aging time runs with them."""
+    S = _server(active_revs=1000)
+    running = {0: _req(0, 900), 1: _req(1, 100), 2: _req(2, 300), 3: _req(3, 450), 4: _req(4, 2000)}
+    held = {}
+    S._choose(running, held, now=1.0)
+    # small: 100, 300, 450 (< 500); 100 + 300 fits, + 450 would pass 1000
+    runs = lambda: {s for s in running if not held.get(s, False)}  # noqa: E731
+    assert runs() == {1, 2}
+    # holds are sent only when a scene's state changes
+    n = len(S.batch.calls)
+    S._choose(running, held, now=1.0)
+    assert len(S.batch.calls) == n
+    # the small ones finish: the large ones run together
+    for s in (1, 2, 3):
+        running.pop(s)
+    S._choose(running, held, now=2.0)
+    assert runs() == {0, 4}
+    # a new small request holds them again, until one of them has waited the aging time
+    running[5] = _req(5, 50, arrival=2.0)
+    S._choose(running, held, now=3.0)
+    assert runs() == {5}
+    S._choose(running, held, now=10.5)
+    assert runs() == {0, 4, 5}
+
+
+def test_sjf_orders_by_size_and_steps_with_aging():
+    S = _server(active_revs=1 << 20, aging_s=10.0)
+    small_late, big_early = _req(0, 1000, arrival=5.0), _req(1, 3000, arrival=0.0)
+    assert S._key(small_late, 5.0) < S._key(big_early, 5.0)
+    # after 40 s of waiting the large request's key has dropped below the new one's
+    assert S._key(big_early, 40.0) < S._key(_req(2, 1000, arrival=40.0), 40.0)
