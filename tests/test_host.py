@@ -88,10 +88,10 @@ def test_sjf_runs_the_small_scenes_first_and_the_large_together():
     budget; large ones are held while any small one runs, then all run
     together; a large one that has waited the aging time runs with them."""
     S = _server(active_revs=1000)
-    running = {0: _req(0, 900), 1: _req(1, 100), 2: _req(2, 300), 3: _req(3, 450), 4: _req(4, 2000)}
+    running = {0: _req(0, 900), 1: _req(1, 100), 2: _req(2, 480), 3: _req(3, 490), 4: _req(4, 2000)}
     held = {}
     S._choose(running, held, now=1.0)
-    # small: 100, 300, 450 (< 500); 100 + 300 fits, + 450 would pass 1000
+    # small: 100, 480, 490 (< 500); 100 + 480 fits, + 490 would pass 1000
     runs = lambda: {s for s in running if not held.get(s, False)}  # noqa: E731
     assert runs() == {1, 2}
     # holds are sent only when a scene's state changes
