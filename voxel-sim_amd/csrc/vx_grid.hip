@@ -2425,6 +2425,7 @@ constexpr size_t kRowBytes = 24 * 5 + 32 + 24 * 3 + 24 + 4 * 9 + 12 + 12 + 4 + 1
                              72 + 24 + 24 + 6 * 12 + 6 * 24 + 4 * kStrength + 12 * kHistory + 32 + 72;
 
 enum Phase : uint32_t { P_IDLE = 0, P_START, P_SOLVE, P_APPLIED, P_FINISHED, P_DONE, P_FAILED };
+constexpr uint32_t kNoSeg = 0xFFFFFFFFu;
 
 struct SlotHost {
     bool used = false;
@@ -2441,6 +2442,10 @@ struct SlotHost {
     double last_rel = 0.0, last_angle = 0.0;
     bool manual = false;  // driven by the vx_scene_* stage calls
     bool held = false;    // sits out the ticks until released (vx_batch_hold)
+    // The last inner residual seen in the current segment (for the chunk
+    // length of a one-scene arena): iteration, r.r, segment start.
+    uint32_t seen_k = 0, seen_seg = kNoSeg;
+    double seen_rr = 0.0;
     bool has_strength = false;
     bool has_tool = false;      // a vx_scene with a rigid tool (vx_scene_set_tool)
     vx_fracture_report frac{};  // fracture totals since admission (a vx_scene: this step)
@@ -2462,6 +2467,7 @@ struct Flight {
     hipEvent_t done = nullptr;
     hipEvent_t frac0 = nullptr, frac1 = nullptr;  // around the fracture kernels
     uint32_t *list = nullptr;                     // pinned: the tick's tile list (when it has one)
+    uint32_t iters = 0;                           // inner iterations of its chunk
     bool active = false;
     bool frac_timed = false;
 };
@@ -2633,7 +2639,7 @@ bool deal_update_rr(const vx_batch *b, bool big) {
     return b->deal == VX_DEAL_AUTO ? big : b->deal != VX_DEAL_CONTIGUOUS;
 }
 
-void enqueue_inner(vx_batch *b, int f64, bool rb, uint32_t tiles, bool big) {
+void enqueue_inner(vx_batch *b, int f64, bool rb, uint32_t tiles, bool big, uint32_t iters) {
     Context &ctx = context();
     const dim3 grid(tiles), blk(kBlock), sg(b->max_slots);
     if (f64) {
@@ -2653,7 +2659,7 @@ void enqueue_inner(vx_batch *b, int f64, bool rb, uint32_t tiles, bool big) {
     OneScene one_u = b->single ? make_one(b->slots[0].geo, deal_update_rr(b, big), b->zcols) : OneScene{};
     one.rb = one_u.rb = rb ? 1u : 0u;
     const dim3 gi(b->single ? one.grid : tiles), gu(b->single ? one_u.grid : tiles);
-    for (uint32_t t = 0; t < b->iters_per_tick; ++t) {
+    for (uint32_t t = 0; t < iters; ++t) {
         if (f64)
             launch_iteration<double>(b, rb, one, gi, one_u, gu, int(t & 1), big);
         else
@@ -2661,11 +2667,11 @@ void enqueue_inner(vx_batch *b, int f64, bool rb, uint32_t tiles, bool big) {
     }
 }
 
-int32_t capture_inner(vx_batch *b, int f64, bool rb, uint32_t tiles, bool big, hipGraphExec_t *out) {
+int32_t capture_inner(vx_batch *b, int f64, bool rb, uint32_t tiles, bool big, uint32_t iters, hipGraphExec_t *out) {
     Context &ctx = context();
     hipGraph_t graph;
     VX_HIP(hipStreamBeginCapture(ctx.stream, hipStreamCaptureModeThreadLocal));
-    enqueue_inner(b, f64, rb, tiles, big);
+    enqueue_inner(b, f64, rb, tiles, big, iters);
     VX_HIP(hipStreamEndCapture(ctx.stream, &graph));
     hipError_t e = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
@@ -2692,18 +2698,66 @@ bool any_big(const vx_batch *b) {
     return false;
 }
 
-int32_t inner_graph(vx_batch *b, int f64, bool rb, uint32_t tiles, hipGraphExec_t *out) {
+int32_t inner_graph(vx_batch *b, int f64, bool rb, uint32_t tiles, uint32_t iters, hipGraphExec_t *out) {
     const bool big = any_big(b);
-    const uint64_t key = (uint64_t(tiles) << 4) | (uint64_t(b->A.tile_list != nullptr) << 3) | (uint64_t(rb) << 2) |
-                         (uint64_t(big) << 1) | uint64_t(f64);
+    const uint64_t key = (uint64_t(iters) << 48) | (uint64_t(tiles) << 4) | (uint64_t(b->A.tile_list != nullptr) << 3) |
+                         (uint64_t(rb) << 2) | (uint64_t(big) << 1) | uint64_t(f64);
     for (auto &kv : b->g_inner)
         if (kv.first == key) {
             *out = kv.second;
             return VX_OK;
         }
-    if (int32_t c = capture_inner(b, f64, rb, tiles, big, out)) return c;
+    if (int32_t c = capture_inner(b, f64, rb, tiles, big, iters, out)) return c;
     b->g_inner.emplace_back(key, *out);
     return VX_OK;
+}
+
+// Inner iterations of the next chunk of a one-scene arena. A chunk's head
+// replaces the residual when the segment before it has ended, so the
+// iterations of a chunk after its segment's end do nothing (their launches
+// still load their rows). When the last state read back shows the inner
+// recurrence at the solve's tolerance, the solve most likely ends at the next
+// head: the chunk gets a short body. Otherwise, from the last two residuals
+// read back in the segment, the segment's end is extrapolated at the
+// per-iteration rate of its last stretch; when it falls inside the next chunk
+// (after the chunk in flight), the chunk is cut to end a few iterations after
+// it. A wrong guess costs a chunk boundary (too short) or the same waste as
+// before (too long), never a result bit: the iterations are those of the same
+// device-decided recurrence either way.
+uint32_t chunk_iters(SlotHost &h, const SlotState &st, uint32_t full, uint32_t in_flight) {
+    if (st.done != kRun) {
+        h.seen_seg = kNoSeg;
+        // The inner recurrence has reached the solve's tolerance: the next head
+        // replaces the residual and most likely ends the solve, so the chunk
+        // after it (or this one's body, with no chunk in flight) would be spent
+        // on a finished solve.
+        const double tol = h.params.solve.tolerance;
+        if (st.done == kConv && st.rr_last <= tol * tol * st.bb) return std::min(4u, full);
+        return full;
+    }
+    const uint32_t k = st.k_update;
+    const double rr = st.rr_last;
+    if (h.seen_seg != st.seg || k <= h.seen_k || !(rr > 0.0) || !(h.seen_rr > 0.0)) {
+        h.seen_seg = st.seg;
+        h.seen_k = k;
+        h.seen_rr = rr;
+        return full;
+    }
+    // per iteration, < 0 when converging: the slower of the last stretch and the
+    // segment so far (an early guess cuts the chunk before the end, whose
+    // remainder the next full chunk then wastes)
+    const double rate_last = std::log(rr / h.seen_rr) / double(k - h.seen_k);
+    const double rate_seg = k > st.seg && st.rr0 > 0.0 ? std::log(rr / st.rr0) / double(k - st.seg) : rate_last;
+    const double rate = std::max(rate_last, rate_seg);
+    h.seen_k = k;
+    h.seen_rr = rr;
+    const double it = h.params.solve.inner_tolerance, tol = h.params.solve.tolerance;
+    const double target = std::max(it * it * st.rr0, tol * tol * st.bb);
+    if (!(rate < 0.0) || !(target > 0.0) || !(rr > target)) return full;
+    const double left = std::log(target / rr) / rate - double(in_flight);  // after the chunk in flight
+    if (!(left > 0.0) || left >= double(full)) return full;
+    const uint32_t c = (uint32_t(left * 1.1 + 2.0) + 3u) & ~3u;  // a few past the guess, a multiple of 4
+    return std::max(4u, std::min(c, full));
 }
 
 // Enqueue one tick for the bits in f.bits: the pre-inner kernels, one inner
@@ -2817,15 +2871,20 @@ int32_t launch_tick(vx_batch *b, Flight &f) {
         if (big) hipLaunchKernelGGL(scene_reduce<double>, sg, blk, 0, s, b->A, int(kRz), 2, 1);
     }
     VX_HIP(hipGetLastError());
+    f.iters = (inner[0] || inner[1]) ? b->iters_per_tick : 0u;
+    if (b->single && (inner[0] || inner[1])) {
+        const Flight &in_flight = b->fl[b->cur ^ 1];
+        f.iters = chunk_iters(b->slots[0], b->known[0], b->iters_per_tick, in_flight.active ? in_flight.iters : 0u);
+    }
     for (int prec = 0; prec < 2; ++prec) {
         if (!inner[prec]) continue;
         if (ctx.direct_launch || g_timer) {
-            enqueue_inner(b, prec, rb[prec], tiles, big);
+            enqueue_inner(b, prec, rb[prec], tiles, big, f.iters);
             VX_HIP(hipGetLastError());
             continue;
         }
         hipGraphExec_t g = nullptr;
-        if (int32_t c = inner_graph(b, prec, rb[prec], tiles, &g)) return c;
+        if (int32_t c = inner_graph(b, prec, rb[prec], tiles, f.iters, &g)) return c;
         VX_HIP(hipGraphLaunch(g, s));
     }
     if (any & (B_APPLY | B_WARM_SAVE)) hipLaunchKernelGGL(apply_correction, grid, blk, 0, s, b->A);
