@@ -1152,11 +1152,11 @@ __global__ void k_contact_scatter(uint32_t npairs, const uint64_t *__restrict__ 
 //  * per voxel, the pairs that touch it, ascending: jlist[joff, jend) holds
 //    the pairs (i, v), and [pstart, pend) is the range of pairs (v, j) in the
 //    sorted list (they follow every (i, v) since i < v).
-// A round evaluates the dirty pairs on their before-states (k_rx_eval),
-// records the outcomes that changed and the voxels they touch (k_rx_apply),
-// and rebuilds those voxels' movers, marking dirty every pair after the
-// smallest changed one (k_rx_rebuild). The dirty list may hold a pair twice
-// (marked from both voxels); both evaluations give the same outcome. The
+// A round evaluates the dirty pairs on their before-states and records the
+// outcomes that changed and the voxels they touch (k_rx_eval), then rebuilds
+// those voxels' movers, marking dirty every pair after the smallest changed
+// one (k_rx_rebuild). The dirty list may hold a pair twice (marked from both
+// voxels); both evaluations give the same outcome. The
 // positions stay as they were at the pass's start until the fixed point;
 // then k_rx_commit_voxels applies each voxel's moves in pair order (every
 // component, as separate() adds them, signed zeros included) and
@@ -1168,7 +1168,9 @@ constexpr uint32_t kRxRounds = 1536;      // rounds per call before the walker t
 constexpr uint64_t kRxEvalBudget = 16;    // pair evaluations per pass, in units of the pair count
 constexpr uint32_t kRxChunk = 8;          // rounds enqueued between two host checks
 constexpr uint32_t kRxMaxPairs = 1u << 30;
-enum : int { RX_CNT0 = 0, RX_CNT1, RX_VCNT, RX_OVER, RX_ROUNDS, RX_WORDS };  // RX_ROUNDS: rounds with dirty pairs
+// control words: dirty-list and changed-voxel counts by round parity, overflow,
+// rounds with dirty pairs (the evaluations, u64, follow at word 8)
+enum : int { RX_CNT0 = 0, RX_CNT1, RX_VCNT0, RX_VCNT1, RX_OVER, RX_ROUNDS, RX_WORDS };
 constexpr uint32_t RF_SEP = 1u, RF_NEG = 8u;  // rflag: bit 0 separates, bits 1-2 axis, bit 3 negative, 8-15 code
 constexpr int kRxBlocks = 2048;           // grid of the list kernels (grid-stride loops), a pass's first round
 constexpr int kRxBlocksLater = 512;       // at most, later rounds
@@ -1353,84 +1355,67 @@ __device__ __forceinline__ void pack_rx(const Outcome &o, uint32_t &f, float &si
     }
 }
 
-// Evaluate the dirty pairs of list `cur` on their before-states; outcomes
-// staged by list entry.
-__global__ __launch_bounds__(256) void k_rx_eval(uint32_t cur, const uint32_t *__restrict__ dlist, uint32_t *rx,
-                                                 uint32_t cap, const uint64_t *__restrict__ pairs,
+// Evaluate the dirty pairs of list `cur` on their before-states and record
+// the outcomes that changed: a changed move marks both voxels (the smallest
+// changed pair per voxel in vchg; each voxel listed once per round by its
+// stamp); a changed contact code alone changes no position. A pair may read a
+// mover that another thread of this round is rewriting (a torn or newer
+// value): that mover's move changed, so the rebuild lists the pair again for
+// the next round, and the round that ends the relaxation changes no move, so
+// every pair's last evaluation read settled values.
+__global__ __launch_bounds__(256) void k_rx_eval(uint32_t cur, uint32_t stamp, const uint32_t *__restrict__ dlist,
+                                                 uint32_t *rx, uint32_t cap, const uint64_t *__restrict__ pairs,
                                                  const float *__restrict__ rec, const uint32_t *__restrict__ mcnt,
                                                  const uint32_t *__restrict__ moff, const uint32_t *__restrict__ mlist,
-                                                 const uint32_t *__restrict__ rflag,
-                                                 const float *__restrict__ rsi, const float *__restrict__ rsj,
-                                                 uint32_t *__restrict__ sflag, float *__restrict__ ssi,
-                                                 float *__restrict__ ssj, unsigned long long *evals) {
+                                                 uint32_t *rflag, float *rsi, float *rsj, uint32_t *vchg,
+                                                 uint32_t *vstamp, uint32_t *vlist, uint32_t n,
+                                                 unsigned long long *evals) {
     if (rx[RX_OVER]) return;
     const uint32_t count = min(rx[RX_CNT0 + cur], cap);
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g == 0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        // the lists the previous round's rebuild consumed take this round's rebuild's output
         rx[RX_CNT0 + (cur ^ 1u)] = 0;
-        rx[RX_VCNT] = 0;
+        rx[RX_VCNT0 + (cur ^ 1u)] = 0;
         if (count) {
             atomicAdd(evals, (unsigned long long)count);
             rx[RX_ROUNDS] += 1;
         }
     }
     const float4 *r4 = reinterpret_cast<const float4 *>(rec);
-    for (uint32_t x = g; x < count; x += gridDim.x * blockDim.x) {
-        const uint32_t k = dlist[x];
-        const uint64_t key = pairs[k];
-        const uint32_t i = uint32_t(key >> 32), j = uint32_t(key);
-        const float4 a0 = r4[2 * size_t(i)], a1 = r4[2 * size_t(i) + 1];
-        const float4 b0 = r4[2 * size_t(j)], b1 = r4[2 * size_t(j) + 1];
-        float pi[3] = {a0.x, a0.y, a0.z}, pj[3] = {b0.x, b0.y, b0.z};
-        before_state(pi, i, k, mcnt, moff, mlist, rflag, rsi, rsj);
-        before_state(pj, j, k, mcnt, moff, mlist, rflag, rsi, rsj);
-        const Outcome o = evaluate_at(i, j, V3{{pi[0], pi[1], pi[2]}}, a0, a1, V3{{pj[0], pj[1], pj[2]}}, b0, b1);
-        uint32_t f;
-        float si, sj;
-        pack_rx(o, f, si, sj);
-        sflag[x] = f;
-        ssi[x] = si;
-        ssj[x] = sj;
-    }
-}
-
-// Record the staged outcomes; a changed move marks both voxels (the smallest
-// changed pair per voxel in vchg; each voxel listed once per round by its
-// stamp). A changed contact code alone changes no position.
-__global__ __launch_bounds__(256) void k_rx_apply(uint32_t cur, uint32_t stamp, const uint32_t *__restrict__ dlist,
-                                                  uint32_t *rx, uint32_t cap, const uint64_t *__restrict__ pairs,
-                                                  uint32_t *rflag, float *rsi, float *rsj,
-                                                  const uint32_t *__restrict__ sflag, const float *__restrict__ ssi,
-                                                  const float *__restrict__ ssj, uint32_t *vchg, uint32_t *vstamp,
-                                                  uint32_t *vlist, uint32_t n) {
-    if (rx[RX_OVER]) return;
-    const uint32_t count = min(rx[RX_CNT0 + cur], cap);
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride) {
         const uint32_t x = base + threadIdx.x;
         uint32_t add_i = 0, add_j = 0, i = 0, j = 0;
         if (x < count) {
             const uint32_t k = dlist[x];
-            const uint32_t f = sflag[x], of = rflag[k];
-            const float a = ssi[x], b = ssj[x];
-            const bool moves = ((f ^ of) & 0xFu) || __float_as_uint(a) != __float_as_uint(rsi[k]) ||
-                               __float_as_uint(b) != __float_as_uint(rsj[k]);
+            const uint64_t key = pairs[k];
+            i = uint32_t(key >> 32);
+            j = uint32_t(key);
+            const float4 a0 = r4[2 * size_t(i)], a1 = r4[2 * size_t(i) + 1];
+            const float4 b0 = r4[2 * size_t(j)], b1 = r4[2 * size_t(j) + 1];
+            float pi[3] = {a0.x, a0.y, a0.z}, pj[3] = {b0.x, b0.y, b0.z};
+            before_state(pi, i, k, mcnt, moff, mlist, rflag, rsi, rsj);
+            before_state(pj, j, k, mcnt, moff, mlist, rflag, rsi, rsj);
+            const Outcome o = evaluate_at(i, j, V3{{pi[0], pi[1], pi[2]}}, a0, a1, V3{{pj[0], pj[1], pj[2]}}, b0, b1);
+            uint32_t f;
+            float si, sj;
+            pack_rx(o, f, si, sj);
+            const uint32_t of = rflag[k];
+            const bool moves = ((f ^ of) & 0xFu) || __float_as_uint(si) != __float_as_uint(rsi[k]) ||
+                               __float_as_uint(sj) != __float_as_uint(rsj[k]);
             if (moves || f != of) {
                 rflag[k] = f;
-                rsi[k] = a;
-                rsj[k] = b;
+                rsi[k] = si;
+                rsj[k] = sj;
             }
             if (moves) {
-                const uint64_t key = pairs[k];
-                i = uint32_t(key >> 32);
-                j = uint32_t(key);
                 atomicMin(vchg + i, k);
                 atomicMin(vchg + j, k);
                 add_i = atomicExch(vstamp + i, stamp) != stamp ? 1u : 0u;
                 add_j = atomicExch(vstamp + j, stamp) != stamp ? 1u : 0u;
             }
         }
-        const uint32_t pos = wave_reserve(add_i + add_j, rx + RX_VCNT, n, rx + RX_OVER);
+        const uint32_t pos = wave_reserve(add_i + add_j, rx + RX_VCNT0 + cur, n, rx + RX_OVER);
         if (pos != kNoMark) {
             uint32_t q = pos;
             if (add_i) vlist[q++] = i;
@@ -1453,7 +1438,7 @@ __global__ __launch_bounds__(256) void k_rx_rebuild(uint32_t cur, uint32_t *rx, 
                                                     const uint32_t *__restrict__ moff, uint32_t *mlist, uint32_t *vchg,
                                                     uint32_t *dnext) {
     if (rx[RX_OVER]) return;
-    const uint32_t count = rx[RX_VCNT];
+    const uint32_t count = rx[RX_VCNT0 + cur];
     const uint32_t lane = threadIdx.x & 63, half = lane >> 5, sub = lane & 31;
     const uint32_t waves = gridDim.x * (blockDim.x / 64);
     const uint32_t wave_id = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
@@ -1771,15 +1756,12 @@ extern "C" int32_t vx_contact_project(uint32_t n, float *positions, float *previ
         uint32_t relaxed = 0;  // passes done by relaxation
         bool finished = false;  // a relaxed pass moved nothing, or all passes are done
         if (ctx.contact_method == VX_CONTACT_AUTO && P < kRxMaxPairs) {
-            uint32_t *d_rflag, *d_sflag, *d_dl0, *d_dl1, *d_mcnt, *d_mlist, *d_moff, *d_mdeg, *d_vchg, *d_vstamp,
+            uint32_t *d_rflag, *d_dl0, *d_dl1, *d_mcnt, *d_mlist, *d_moff, *d_mdeg, *d_vchg, *d_vstamp,
                 *d_vlist, *d_jend, *d_joff, *d_jkey, *d_jkey2, *d_jlist, *d_pstart, *d_pend, *d_rx;
-            float *d_rsi, *d_rsj, *d_ssi, *d_ssj;
+            float *d_rsi, *d_rsj;
             VX_BUF("contact.rx.rflag", 4 * size_t(P), d_rflag);
             VX_BUF("contact.rx.rsi", 4 * size_t(P), d_rsi);
             VX_BUF("contact.rx.rsj", 4 * size_t(P), d_rsj);
-            VX_BUF("contact.rx.sflag", 4 * size_t(P), d_sflag);
-            VX_BUF("contact.rx.ssi", 4 * size_t(P), d_ssi);
-            VX_BUF("contact.rx.ssj", 4 * size_t(P), d_ssj);
             VX_BUF("contact.rx.dl0", 4 * size_t(P), d_dl0);
             VX_BUF("contact.rx.dl1", 4 * size_t(P), d_dl1);
             VX_BUF("contact.rx.jlist", 4 * size_t(P), d_jlist);
@@ -1870,10 +1852,8 @@ extern "C" int32_t vx_contact_project(uint32_t n, float *positions, float *previ
                         // the later ones take what the last check saw, or less
                         const uint32_t g = pass_start ? lb_all : std::min<uint32_t>(kRxBlocksLater, grid_for(seen));
                         pass_start = false;
-                        k_rx_eval<<<g, 256, 0, s>>>(c, dl[c], d_rx, P, d_pairs, rec, d_mcnt, d_moff, d_mlist, d_rflag, d_rsi,
-                                                    d_rsj, d_sflag, d_ssi, d_ssj, d_evals);
-                        k_rx_apply<<<g, 256, 0, s>>>(c, R + 1, dl[c], d_rx, P, d_pairs, d_rflag, d_rsi, d_rsj,
-                                                     d_sflag, d_ssi, d_ssj, d_vchg, d_vstamp, d_vlist, n);
+                        k_rx_eval<<<g, 256, 0, s>>>(c, R + 1, dl[c], d_rx, P, d_pairs, rec, d_mcnt, d_moff, d_mlist,
+                                                    d_rflag, d_rsi, d_rsj, d_vchg, d_vstamp, d_vlist, n, d_evals);
                         k_rx_rebuild<<<g, 256, 0, s>>>(c, d_rx, P, d_vlist, d_joff, d_jend, d_jlist, d_pstart, d_pend,
                                                        d_rflag, d_mcnt, d_moff, d_mlist, d_vchg, dl[c ^ 1u]);
                     }
