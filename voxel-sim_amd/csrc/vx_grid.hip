@@ -2719,18 +2719,20 @@ int32_t inner_graph(vx_batch *b, int f64, bool rb, uint32_t tiles, uint32_t iter
 // recurrence at the solve's tolerance, the solve most likely ends at the next
 // head: the chunk gets a short body. Otherwise, from the last two residuals
 // read back in the segment, the segment's end is extrapolated at the
-// per-iteration rate of its last stretch; when it falls inside the next chunk
-// (after the chunk in flight), the chunk is cut to end a few iterations after
-// it. A wrong guess costs a chunk boundary (too short) or the same waste as
-// before (too long), never a result bit: the iterations are those of the same
-// device-decided recurrence either way.
+// per-iteration rate of its last stretch (CG speeds up within a segment, so
+// the latest rate is the closest guess). When the end falls inside the next
+// chunk (after the chunk in flight), the chunk is cut to end two iterations
+// after it. When it falls inside the chunk in flight, that chunk either was
+// cut for it already (the next chunk then starts the new segment: full
+// length) or is a full chunk, and the next chunk is kept short to catch an end
+// that comes later than guessed. A wrong guess costs a chunk boundary (too
+// short) or part of a chunk spent after the end (too long), never a result
+// bit: the iterations are those of the same device-decided recurrence either
+// way.
 uint32_t chunk_iters(SlotHost &h, const SlotState &st, uint32_t full, uint32_t in_flight) {
+    constexpr uint32_t kCatch = 8;  // the short chunk after a full one that should hold the end
     if (st.done != kRun) {
         h.seen_seg = kNoSeg;
-        // The inner recurrence has reached the solve's tolerance: the next head
-        // replaces the residual and most likely ends the solve, so the chunk
-        // after it (or this one's body, with no chunk in flight) would be spent
-        // on a finished solve.
         const double tol = h.params.solve.tolerance;
         if (st.done == kConv && st.rr_last <= tol * tol * st.bb) return std::min(4u, full);
         return full;
@@ -2743,20 +2745,20 @@ uint32_t chunk_iters(SlotHost &h, const SlotState &st, uint32_t full, uint32_t i
         h.seen_rr = rr;
         return full;
     }
-    // per iteration, < 0 when converging: the slower of the last stretch and the
-    // segment so far (an early guess cuts the chunk before the end, whose
-    // remainder the next full chunk then wastes)
+    // per iteration, < 0 when converging: the last stretch's, or the
+    // segment's so far when the last stretch did not converge
     const double rate_last = std::log(rr / h.seen_rr) / double(k - h.seen_k);
     const double rate_seg = k > st.seg && st.rr0 > 0.0 ? std::log(rr / st.rr0) / double(k - st.seg) : rate_last;
-    const double rate = std::max(rate_last, rate_seg);
+    const double rate = rate_last < 0.0 ? rate_last : rate_seg;
     h.seen_k = k;
     h.seen_rr = rr;
     const double it = h.params.solve.inner_tolerance, tol = h.params.solve.tolerance;
     const double target = std::max(it * it * st.rr0, tol * tol * st.bb);
     if (!(rate < 0.0) || !(target > 0.0) || !(rr > target)) return full;
     const double left = std::log(target / rr) / rate - double(in_flight);  // after the chunk in flight
-    if (!(left > 0.0) || left >= double(full)) return full;
-    const uint32_t c = (uint32_t(left * 1.1 + 2.0) + 3u) & ~3u;  // a few past the guess, a multiple of 4
+    if (!(left < double(full))) return full;
+    if (!(left > 0.0)) return in_flight > 0 && in_flight < full ? full : std::min(kCatch, full);
+    const uint32_t c = (uint32_t(left + 2.0) + 3u) & ~3u;  // two past the guess, a multiple of 4
     return std::max(4u, std::min(c, full));
 }
 
