@@ -242,6 +242,41 @@ def large_roofline() -> dict:
     return out
 
 
+def shard_slabs(depths=(24, 12)) -> dict:
+    """One GPU's share of the 192x192x96 block under Z-slab decomposition (4
+    and 8 GPUs): the 192x192xd slab solved on its own at its real shape, its
+    in-solve SpMV roofline and its time per inner iteration (one solve to
+    relative residual 1e-9, after a warm-up solve), beside the whole block's.
+    The exchange a decomposition adds per iteration is not part of it."""
+    from vxsim import grid, scenes
+    out = {}
+    for d in (96,) + tuple(depths):
+        scene = scenes.wood_block_blade(dims=(192, 192, d))
+        cfg = scene.config
+        h = cfg.fixed_dt / cfg.substeps
+        sp = grid.solve_params(tolerance=1e-9, inner_tolerance=INNER_TOL, max_inner=40000, max_outer=12)
+        with grid.GridScene(scene) as gs:
+            gs.predict(h, cfg.gravity)
+            gs.fit_rotations()
+            gs.assemble(h)
+            gs.solve(sp)  # warm-up (first touch of the inner planes)
+            t0 = time.perf_counter()
+            rep = gs.solve(sp)
+            wall = time.perf_counter() - t0
+            r = spmv_in_solve(gs, scene, sp)
+        its = rep["inner_iterations"] + rep["outer_sweeps"]
+        out["192x192x%d" % d] = {
+            "rows": scene.n, "gpus_sharing_the_block": 96 // d, "solve_ms": 1e3 * wall,
+            "inner_iterations": rep["inner_iterations"], "outer_sweeps": rep["outer_sweeps"],
+            "us_per_iteration": 1e6 * wall / its, "spmv_avg_us": r["avg_us"], "spmv_frac": r["frac"],
+            "update_avg_us": r["update_kernel"]["avg_us"]}
+    whole = out["192x192x96"]["us_per_iteration"]
+    for d in depths:
+        e = out["192x192x%d" % d]
+        e["iteration_vs_whole_block_share"] = e["us_per_iteration"] / (whole / e["gpus_sharing_the_block"])
+    return out
+
+
 def batched(steps: int = 1, count: int = 256, preconditioner: int = 0, cpu: bool = False):
     """256 independent boxes (edges U[8, 48], one face fixed, random initial
     velocities) on one tile arena, `steps` steps each, all scenes advanced
@@ -910,6 +945,7 @@ def main():
         print("bench: served legs done", file=sys.stderr, flush=True)
     if not args.no_large_roofline:
         extra["roofline_hbm_out_of_cache"] = large_roofline()
+        extra["shard_slabs"] = shard_slabs()
         print("bench: 192x192x96 roofline done", file=sys.stderr, flush=True)
     if not args.no_contact:
         extra["contact_projection"] = contact_bench(cpu=cpu_on)
