@@ -69,6 +69,7 @@ enum : uint32_t {
     B_FRACTURE = 1u << 12,  // fracture pass of the substep, before the velocities
     B_RB = 1u << 13,        // red-black reduced system (red-black block SGS) instead of block-Jacobi
     B_TOOL = 1u << 14,      // the scene's rigid tool advances after the prediction
+    B_MG = 1u << 15,        // aggregation multigrid cycle as the preconditioner (one-scene arenas, FP32 inner)
 };
 
 // Partial kinds; kFrac..kFrac+5 are the fracture pass's dissipated energies
@@ -2472,6 +2473,452 @@ struct Flight {
     bool frac_timed = false;
 };
 
+// ---- aggregation multigrid preconditioner ------------------------------------
+// One V(1,1) cycle of aggregation multigrid as the preconditioner of the
+// FP32 inner PCG of a one-scene arena (VX_PRECOND_MULTIGRID). Level l + 1
+// aggregates the nodes of level l in 2x2x2 boxes (the last box along an axis
+// of odd extent holds one layer): the prolongation P copies a coarse node's
+// value to its children, component by component, on the components the child
+// leaves free, and the coarse operator is the Galerkin product P^T A P. With
+// the bond stencil that product keeps the stencil's shape: per coarse node a
+// symmetric diagonal block D (the children's free diagonal blocks, less the
+// couplings between children in both directions) and one 3x3 coupling C per
+// axis to the next node along it, A[I, I + e] = -C and A[I + e, I] = -C^T.
+// A component no child leaves free is fixed on the coarse level too (identity
+// diagonal, no coupling). The fine level is the scene's own operator: D and
+// the bond blocks k with the free-component masks, A[v, v + e] = -F_v k F_j.
+// Smoothing is damped block-Jacobi, x += w D^-1 (r - A x), once before
+// (from x = 0: x = w D^-1 r) and once after the coarse correction; the
+// coarsest level (at most kMgCoarsest nodes) is solved exactly by its
+// Cholesky factor, e = W^T W r with W = L^-1 computed once per assembly in
+// FP64. The hierarchy is built in FP64 from the FP64 operator after every
+// assembly and applied in FP32. The cycle is a fixed symmetric positive
+// definite linear operator, so the outer iteration stays conjugate gradients.
+constexpr float kMgOmega = 0.6f;
+constexpr uint32_t kMgCoarsest = 128;          // nodes of the level solved directly, at most
+constexpr uint32_t kMgMaxDense = 3 * kMgCoarsest;
+constexpr uint32_t kMgMaxLevels = 16;
+constexpr int kMgBlock = 256;
+constexpr int kMgDenseBlock = 512;             // one workgroup for the dense factor and its use
+
+struct MgLevel {
+    uint32_t nx, ny, nz, n;  // node grid of the level
+    float *D, *C, *Dinv;     // 6n symmetric, 27n ([axis][3x3 row-major]), 6n symmetric
+    float *r, *x, *res;      // 3n each: right-hand side, smoothed iterate, residual (after the cycle: the result)
+    uint8_t *mask;           // free components
+    double *D64, *C64;       // FP64 copies of D and C (the next level is built from them)
+};
+
+__device__ __forceinline__ int sym_at(int r, int c) {
+    return r <= c ? (r == 0 ? c : (r == 1 ? 2 + c : 5)) : (c == 0 ? r : (c == 1 ? 2 + r : 5));
+}
+__device__ __forceinline__ void node_xyz(uint32_t i, uint32_t nx, uint32_t ny, uint32_t (&c)[3]) {
+    c[0] = i % nx;
+    const uint32_t t = i / nx;
+    c[1] = t % ny;
+    c[2] = t / ny;
+}
+__device__ __forceinline__ uint32_t node_stride(uint32_t nx, uint32_t ny, int a) { return a == 0 ? 1u : (a == 1 ? nx : nx * ny); }
+
+// Store a level's node from its FP64 diagonal block and couplings.
+__device__ __forceinline__ void mg_store_node(const MgLevel &L, uint32_t I, double (&D)[3][3], const double (&C)[3][3][3],
+                                              uint32_t fm) {
+    for (int r = 0; r < 3; ++r)
+        if (!(fm >> r & 1u)) D[r][r] = 1.0;  // its row and column are zero: identity
+    double s[6] = {D[0][0], D[0][1], D[0][2], D[1][1], D[1][2], D[2][2]}, si[6];
+    sym_inverse(s, si);
+    for (int q = 0; q < 6; ++q) {
+        L.D64[size_t(q) * L.n + I] = s[q];
+        L.D[size_t(q) * L.n + I] = float(s[q]);
+        L.Dinv[size_t(q) * L.n + I] = float(si[q]);
+    }
+    for (int a = 0; a < 3; ++a)
+        for (int e = 0; e < 9; ++e) {
+            L.C64[size_t(a * 9 + e) * L.n + I] = C[a][e / 3][e % 3];
+            L.C[size_t(a * 9 + e) * L.n + I] = float(C[a][e / 3][e % 3]);
+        }
+    L.mask[I] = uint8_t(fm);
+}
+
+// Level 1 from the scene's FP64 operator: one thread per coarse node.
+__global__ __launch_bounds__(kMgBlock) void mg_build_fine(DevArena A, SlotGeo g, MgLevel L) {
+    const uint32_t I = blockIdx.x * kMgBlock + threadIdx.x;
+    if (I >= L.n) return;
+    uint32_t ci[3];
+    node_xyz(I, L.nx, L.ny, ci);
+    const size_t R = A.R;
+    const uint32_t ext[3] = {g.nx, g.ny, g.nz};
+    double D[3][3] = {}, C[3][3][3] = {};
+    uint32_t fm = 0;
+    for (int d = 0; d < 8; ++d) {
+        const uint32_t off[3] = {uint32_t(d & 1), uint32_t(d >> 1 & 1), uint32_t(d >> 2)};
+        const uint32_t c[3] = {2 * ci[0] + off[0], 2 * ci[1] + off[1], 2 * ci[2] + off[2]};
+        if (c[0] >= ext[0] || c[1] >= ext[1] || c[2] >= ext[2]) continue;
+        const uint32_t v = g.voff + c[0] + g.nx * (c[1] + g.ny * c[2]);
+        const uint32_t m = A.mask[v];
+        fm |= m;
+        const double f[3] = {double(m & 1u), double(m >> 1 & 1u), double(m >> 2 & 1u)};
+        for (int r = 0; r < 3; ++r)
+            for (int q = 0; q < 3; ++q) D[r][q] += f[r] * A.d64[size_t(sym_at(r, q)) * R + v] * f[q];
+        for (int a = 0; a < 3; ++a) {
+            if (c[a] + 1 >= ext[a]) continue;  // no +a neighbour
+            const uint32_t mj = A.mask[v + stride_of(g, a)];
+            const double fj[3] = {double(mj & 1u), double(mj >> 1 & 1u), double(mj >> 2 & 1u)};
+            double k[3][3];
+            for (int r = 0; r < 3; ++r)
+                for (int q = 0; q < 3; ++q) k[r][q] = f[r] * A.k64[(size_t(a) * 6 + sym_at(r, q)) * R + v] * fj[q];
+            if (off[a] == 0) {  // the neighbour is a child of the same node
+                for (int r = 0; r < 3; ++r)
+                    for (int q = 0; q < 3; ++q) D[r][q] -= k[r][q] + k[q][r];
+            } else {  // it is a child of the next node along a
+                for (int r = 0; r < 3; ++r)
+                    for (int q = 0; q < 3; ++q) C[a][r][q] += k[r][q];
+            }
+        }
+    }
+    mg_store_node(L, I, D, C, fm);
+}
+
+// Level l + 1 from level l (S).
+__global__ __launch_bounds__(kMgBlock) void mg_build_coarse(MgLevel S, MgLevel L) {
+    const uint32_t I = blockIdx.x * kMgBlock + threadIdx.x;
+    if (I >= L.n) return;
+    uint32_t ci[3];
+    node_xyz(I, L.nx, L.ny, ci);
+    const uint32_t ext[3] = {S.nx, S.ny, S.nz};
+    double D[3][3] = {}, C[3][3][3] = {};
+    uint32_t fm = 0;
+    for (int d = 0; d < 8; ++d) {
+        const uint32_t off[3] = {uint32_t(d & 1), uint32_t(d >> 1 & 1), uint32_t(d >> 2)};
+        const uint32_t c[3] = {2 * ci[0] + off[0], 2 * ci[1] + off[1], 2 * ci[2] + off[2]};
+        if (c[0] >= ext[0] || c[1] >= ext[1] || c[2] >= ext[2]) continue;
+        const uint32_t i = c[0] + S.nx * (c[1] + S.ny * c[2]);
+        const uint32_t m = S.mask[i];
+        fm |= m;
+        const double f[3] = {double(m & 1u), double(m >> 1 & 1u), double(m >> 2 & 1u)};
+        for (int r = 0; r < 3; ++r)
+            for (int q = 0; q < 3; ++q) D[r][q] += f[r] * S.D64[size_t(sym_at(r, q)) * S.n + i] * f[q];
+        for (int a = 0; a < 3; ++a) {
+            if (c[a] + 1 >= ext[a]) continue;
+            double k[3][3];  // S's couplings carry their masks already
+            for (int e = 0; e < 9; ++e) k[e / 3][e % 3] = S.C64[size_t(a * 9 + e) * S.n + i];
+            if (off[a] == 0) {
+                for (int r = 0; r < 3; ++r)
+                    for (int q = 0; q < 3; ++q) D[r][q] -= k[r][q] + k[q][r];
+            } else {
+                for (int r = 0; r < 3; ++r)
+                    for (int q = 0; q < 3; ++q) C[a][r][q] += k[r][q];
+            }
+        }
+    }
+    mg_store_node(L, I, D, C, fm);
+}
+
+// The coarsest level as a dense n3 x n3 matrix (n3 = 3n, row 3I + r): each
+// thread writes the three rows of its node.
+__global__ __launch_bounds__(kMgBlock) void mg_dense(MgLevel L, double *M, uint32_t n3) {
+    const uint32_t I = blockIdx.x * kMgBlock + threadIdx.x;
+    if (I >= L.n) return;
+    uint32_t ci[3];
+    node_xyz(I, L.nx, L.ny, ci);
+    const uint32_t ext[3] = {L.nx, L.ny, L.nz};
+    for (int r = 0; r < 3; ++r) {
+        double *row = M + size_t(3 * I + r) * n3;
+        for (uint32_t j = 0; j < n3; ++j) row[j] = 0.0;
+        for (int q = 0; q < 3; ++q) row[3 * I + q] = L.D64[size_t(sym_at(r, q)) * L.n + I];
+        for (int a = 0; a < 3; ++a) {
+            const uint32_t s = node_stride(L.nx, L.ny, a);
+            if (ci[a] + 1 < ext[a])
+                for (int q = 0; q < 3; ++q) row[3 * (I + s) + q] = -L.C64[size_t(a * 9 + r * 3 + q) * L.n + I];
+            if (ci[a] > 0)
+                for (int q = 0; q < 3; ++q) row[3 * (I - s) + q] = -L.C64[size_t(a * 9 + q * 3 + r) * L.n + I - s];
+        }
+    }
+}
+
+// Cholesky factor of M in place (lower triangle), then W = L^-1 (lower,
+// row-major in W). One workgroup; a non-positive pivot sets *err and is
+// replaced by 1 (the cycle then stays defined; the solve reports the status).
+__global__ __launch_bounds__(kMgDenseBlock) void mg_cholesky(double *M, double *W, uint32_t n3, uint32_t *err) {
+    __shared__ double col[kMgMaxDense];
+    const uint32_t t = threadIdx.x;
+    for (uint32_t k = 0; k < n3; ++k) {
+        __syncthreads();  // the previous step's updates are done
+        const double piv = M[size_t(k) * n3 + k];
+        const double d = piv > 0.0 ? sqrt(piv) : 1.0;
+        if (!(piv > 0.0) && t == 0) *err = 1u;
+        for (uint32_t i = k + 1 + t; i < n3; i += kMgDenseBlock) {
+            const double l = M[size_t(i) * n3 + k] / d;
+            M[size_t(i) * n3 + k] = l;
+            col[i] = l;
+        }
+        __syncthreads();  // column k is scaled; every thread has read the pivot
+        if (t == 0) M[size_t(k) * n3 + k] = d;
+        for (uint32_t i = k + 1 + t; i < n3; i += kMgDenseBlock) {
+            const double li = col[i];
+            double *row = M + size_t(i) * n3;
+            for (uint32_t j = k + 1; j <= i; ++j) row[j] -= li * col[j];
+        }
+    }
+    __syncthreads();
+    // column j of L^-1 by forward substitution (one thread per column)
+    for (uint32_t j = t; j < n3; j += kMgDenseBlock) {
+        for (uint32_t i = 0; i < j; ++i) W[size_t(i) * n3 + j] = 0.0;
+        for (uint32_t i = j; i < n3; ++i) {
+            double s = i == j ? 1.0 : 0.0;
+            for (uint32_t m = j; m < i; ++m) s -= M[size_t(i) * n3 + m] * W[size_t(m) * n3 + j];
+            W[size_t(i) * n3 + j] = s / M[size_t(i) * n3 + i];
+        }
+    }
+}
+
+// x0 = w D^-1 r of fine row v (zero on fixed components: r is zero there and
+// D^-1 is the identity).
+__device__ __forceinline__ void mg_x0(const DevArena &A, uint32_t v, float (&x)[3]) {
+    const size_t R = A.R;
+    float r[3], di[6];
+    for (int q = 0; q < 3; ++q) r[q] = A.r[q * R + v];
+    for (int q = 0; q < 6; ++q) di[q] = A.dinv32[q * R + v];
+    x[0] = kMgOmega * __builtin_fmaf(di[2], r[2], __builtin_fmaf(di[1], r[1], di[0] * r[0]));
+    x[1] = kMgOmega * __builtin_fmaf(di[4], r[2], __builtin_fmaf(di[3], r[1], di[1] * r[0]));
+    x[2] = kMgOmega * __builtin_fmaf(di[5], r[2], __builtin_fmaf(di[4], r[1], di[2] * r[0]));
+}
+__device__ __forceinline__ void sym_mul_acc(const float (&s)[6], const float (&x)[3], float (&y)[3]) {
+    y[0] = __builtin_fmaf(s[2], x[2], __builtin_fmaf(s[1], x[1], __builtin_fmaf(s[0], x[0], y[0])));
+    y[1] = __builtin_fmaf(s[4], x[2], __builtin_fmaf(s[3], x[1], __builtin_fmaf(s[1], x[0], y[1])));
+    y[2] = __builtin_fmaf(s[5], x[2], __builtin_fmaf(s[4], x[1], __builtin_fmaf(s[2], x[0], y[2])));
+}
+
+// Fine level, before the coarse correction: res = r - A x0 with x0 = w D^-1 r.
+__global__ __launch_bounds__(kMgBlock) void mg_fine_down(DevArena A, SlotGeo g, const SlotState *st, float *res) {
+    const uint32_t loc = blockIdx.x * kMgBlock + threadIdx.x;
+    if (st->done || loc >= g.n) return;
+    const size_t R = A.R;
+    const uint32_t v = g.voff + loc;
+    uint32_t cc[3];
+    coords(g, loc, cc);
+    float x[3], d[6], y[3] = {0.0f, 0.0f, 0.0f}, acc[3] = {0.0f, 0.0f, 0.0f};
+    mg_x0(A, v, x);
+    for (int q = 0; q < 6; ++q) d[q] = A.d32[q * R + v];
+    sym_mul_acc(d, x, y);
+    for (int a = 0; a < 3; ++a) {
+        const uint32_t s = stride_of(g, a);
+        float xn[3], k[6];
+        if (cc[a] + 1 < extent_of(g, a)) {
+            mg_x0(A, v + s, xn);
+            for (int q = 0; q < 6; ++q) k[q] = A.k32[(size_t(a) * 6 + q) * R + v];
+            sym_mul_acc(k, xn, acc);
+        }
+        if (cc[a] > 0) {
+            mg_x0(A, v - s, xn);
+            for (int q = 0; q < 6; ++q) k[q] = A.k32[(size_t(a) * 6 + q) * R + v - s];
+            sym_mul_acc(k, xn, acc);
+        }
+    }
+    const uint32_t m = A.mask[v];
+    for (int q = 0; q < 3; ++q) res[size_t(q) * g.n + loc] = A.r[q * R + v] - (y[q] - ((m >> q & 1u) ? acc[q] : 0.0f));
+}
+
+// Restriction to level L (children from the fine level or from level S) and
+// the pre-smoothing there: r_I = sum of the children's free residuals,
+// x_I = w D_I^-1 r_I.
+__device__ __forceinline__ void mg_restrict_store(const MgLevel &L, uint32_t I, const float (&r)[3]) {
+    float di[6], x[3] = {0.0f, 0.0f, 0.0f};
+    for (int q = 0; q < 6; ++q) di[q] = L.Dinv[size_t(q) * L.n + I];
+    sym_mul_acc(di, r, x);
+    for (int q = 0; q < 3; ++q) {
+        L.r[size_t(q) * L.n + I] = r[q];
+        L.x[size_t(q) * L.n + I] = kMgOmega * x[q];
+    }
+}
+__global__ __launch_bounds__(kMgBlock) void mg_restrict_fine(DevArena A, SlotGeo g, const SlotState *st,
+                                                             const float *res, MgLevel L) {
+    const uint32_t I = blockIdx.x * kMgBlock + threadIdx.x;
+    if (st->done || I >= L.n) return;
+    uint32_t ci[3];
+    node_xyz(I, L.nx, L.ny, ci);
+    float r[3] = {0.0f, 0.0f, 0.0f};
+    for (int d = 0; d < 8; ++d) {
+        const uint32_t c[3] = {2 * ci[0] + (d & 1), 2 * ci[1] + (d >> 1 & 1), 2 * ci[2] + (d >> 2)};
+        if (c[0] >= g.nx || c[1] >= g.ny || c[2] >= g.nz) continue;
+        const uint32_t loc = c[0] + g.nx * (c[1] + g.ny * c[2]);
+        const uint32_t m = A.mask[g.voff + loc];
+        for (int q = 0; q < 3; ++q)
+            if (m >> q & 1u) r[q] += res[size_t(q) * g.n + loc];
+    }
+    mg_restrict_store(L, I, r);
+}
+__global__ __launch_bounds__(kMgBlock) void mg_restrict_coarse(MgLevel S, MgLevel L, const SlotState *st) {
+    const uint32_t I = blockIdx.x * kMgBlock + threadIdx.x;
+    if (st->done || I >= L.n) return;
+    uint32_t ci[3];
+    node_xyz(I, L.nx, L.ny, ci);
+    float r[3] = {0.0f, 0.0f, 0.0f};
+    for (int d = 0; d < 8; ++d) {
+        const uint32_t c[3] = {2 * ci[0] + (d & 1), 2 * ci[1] + (d >> 1 & 1), 2 * ci[2] + (d >> 2)};
+        if (c[0] >= S.nx || c[1] >= S.ny || c[2] >= S.nz) continue;
+        const uint32_t i = c[0] + S.nx * (c[1] + S.ny * c[2]);
+        const uint32_t m = S.mask[i];
+        for (int q = 0; q < 3; ++q)
+            if (m >> q & 1u) r[q] += S.res[size_t(q) * S.n + i];
+    }
+    mg_restrict_store(L, I, r);
+}
+
+// y = A_L x at node I of a coarse level, with x given per node by `xat`.
+template <class X>
+__device__ __forceinline__ void mg_coarse_apply(const MgLevel &L, uint32_t I, const X &xat, float (&y)[3]) {
+    uint32_t ci[3];
+    node_xyz(I, L.nx, L.ny, ci);
+    const uint32_t ext[3] = {L.nx, L.ny, L.nz};
+    float x[3], d[6];
+    xat(I, x);
+    for (int q = 0; q < 6; ++q) d[q] = L.D[size_t(q) * L.n + I];
+    y[0] = y[1] = y[2] = 0.0f;
+    sym_mul_acc(d, x, y);
+    for (int a = 0; a < 3; ++a) {
+        const uint32_t s = node_stride(L.nx, L.ny, a);
+        float xn[3];
+        if (ci[a] + 1 < ext[a]) {  // -C x_{I+e}
+            xat(I + s, xn);
+            for (int r = 0; r < 3; ++r)
+                for (int q = 0; q < 3; ++q) y[r] -= L.C[size_t(a * 9 + r * 3 + q) * L.n + I] * xn[q];
+        }
+        if (ci[a] > 0) {  // -C_{I-e}^T x_{I-e}
+            xat(I - s, xn);
+            for (int r = 0; r < 3; ++r)
+                for (int q = 0; q < 3; ++q) y[r] -= L.C[size_t(a * 9 + q * 3 + r) * L.n + I - s] * xn[q];
+        }
+    }
+}
+
+// Coarse level, before the correction from below: res = r - A x.
+__global__ __launch_bounds__(kMgBlock) void mg_coarse_down(MgLevel L, const SlotState *st) {
+    const uint32_t I = blockIdx.x * kMgBlock + threadIdx.x;
+    if (st->done || I >= L.n) return;
+    auto xat = [&](uint32_t u, float (&x)[3]) {
+        for (int q = 0; q < 3; ++q) x[q] = L.x[size_t(q) * L.n + u];
+    };
+    float y[3];
+    mg_coarse_apply(L, I, xat, y);
+    for (int q = 0; q < 3; ++q) L.res[size_t(q) * L.n + I] = L.r[size_t(q) * L.n + I] - y[q];
+}
+
+// The coarsest level: res = W^T (W r), one workgroup.
+__global__ __launch_bounds__(kMgDenseBlock) void mg_coarsest(MgLevel L, const double *W, uint32_t n3, const SlotState *st) {
+    __shared__ double rv[kMgMaxDense], uv[kMgMaxDense];
+    if (st->done) return;
+    const uint32_t t = threadIdx.x;
+    for (uint32_t m = t; m < n3; m += kMgDenseBlock) rv[m] = double(L.r[size_t(m % 3) * L.n + m / 3]);
+    __syncthreads();
+    for (uint32_t i = t; i < n3; i += kMgDenseBlock) {
+        double s = 0.0;
+        for (uint32_t m = 0; m <= i; ++m) s = __builtin_fma(W[size_t(i) * n3 + m], rv[m], s);
+        uv[i] = s;
+    }
+    __syncthreads();
+    for (uint32_t m = t; m < n3; m += kMgDenseBlock) {
+        double s = 0.0;
+        for (uint32_t i = m; i < n3; ++i) s = __builtin_fma(W[size_t(i) * n3 + m], uv[i], s);
+        L.res[size_t(m % 3) * L.n + m / 3] = float(s);
+    }
+}
+
+// Coarse level, after the correction: x' = x + P e (e: the coarser level's
+// result, in P.res), then x'' = x' + w D^-1 (r - A x'), stored in L.res (the
+// level's result).
+__global__ __launch_bounds__(kMgBlock) void mg_coarse_up(MgLevel L, MgLevel P, const SlotState *st) {
+    const uint32_t I = blockIdx.x * kMgBlock + threadIdx.x;
+    if (st->done || I >= L.n) return;
+    auto xat = [&](uint32_t u, float (&x)[3]) {
+        uint32_t c[3];
+        node_xyz(u, L.nx, L.ny, c);
+        const uint32_t pu = c[0] / 2 + P.nx * (c[1] / 2 + P.ny * (c[2] / 2));
+        const uint32_t m = L.mask[u];
+        for (int q = 0; q < 3; ++q)
+            x[q] = L.x[size_t(q) * L.n + u] + ((m >> q & 1u) ? P.res[size_t(q) * P.n + pu] : 0.0f);
+    };
+    float y[3], x[3], di[6], dx[3] = {0.0f, 0.0f, 0.0f}, rr[3];
+    mg_coarse_apply(L, I, xat, y);
+    xat(I, x);
+    for (int q = 0; q < 6; ++q) di[q] = L.Dinv[size_t(q) * L.n + I];
+    for (int q = 0; q < 3; ++q) rr[q] = L.r[size_t(q) * L.n + I] - y[q];
+    sym_mul_acc(di, rr, dx);
+    for (int q = 0; q < 3; ++q) L.res[size_t(q) * L.n + I] = __builtin_fmaf(kMgOmega, dx[q], x[q]);
+}
+
+// Fine level, after the correction: z = x' + w D^-1 (r - A x') with
+// x' = x0 + P e, and the tile's partial of r.z (the preconditioned residual
+// the next SpMV's head takes). Tiles as the inner kernels of the scene.
+__global__ __launch_bounds__(kBlock) void mg_fine_up(DevArena A, OneScene one, MgLevel P) {
+    Tile c;
+    if (!inner_tile<float, true>(A, one, c)) return;
+    __shared__ double lds[1][kWaves];
+    if (A.st[c.slot].done) return;
+    const size_t R = A.R;
+    const SlotGeo &g = c.g;
+    double part[1] = {0.0};
+    if (c.row) {
+        const uint32_t v = c.v;
+        auto xat = [&](uint32_t u, float (&x)[3]) {
+            uint32_t cu[3];
+            coords(g, u - g.voff, cu);
+            const uint32_t pu = cu[0] / 2 + P.nx * (cu[1] / 2 + P.ny * (cu[2] / 2));
+            const uint32_t m = A.mask[u];
+            mg_x0(A, u, x);
+            for (int q = 0; q < 3; ++q)
+                if (m >> q & 1u) x[q] += P.res[size_t(q) * P.n + pu];
+        };
+        uint32_t cc[3];
+        coords(g, c.local, cc);
+        float x[3], d[6], y[3] = {0.0f, 0.0f, 0.0f}, acc[3] = {0.0f, 0.0f, 0.0f};
+        xat(v, x);
+        for (int q = 0; q < 6; ++q) d[q] = A.d32[q * R + v];
+        sym_mul_acc(d, x, y);
+        for (int a = 0; a < 3; ++a) {
+            const uint32_t s = stride_of(g, a);
+            float xn[3], k[6];
+            if (cc[a] + 1 < extent_of(g, a)) {
+                xat(v + s, xn);
+                for (int q = 0; q < 6; ++q) k[q] = A.k32[(size_t(a) * 6 + q) * R + v];
+                sym_mul_acc(k, xn, acc);
+            }
+            if (cc[a] > 0) {
+                xat(v - s, xn);
+                for (int q = 0; q < 6; ++q) k[q] = A.k32[(size_t(a) * 6 + q) * R + v - s];
+                sym_mul_acc(k, xn, acc);
+            }
+        }
+        const uint32_t m = A.mask[v];
+        float rr[3], di[6], dz[3] = {0.0f, 0.0f, 0.0f}, z[3];
+        for (int q = 0; q < 3; ++q) rr[q] = A.r[q * R + v] - (y[q] - ((m >> q & 1u) ? acc[q] : 0.0f));
+        for (int q = 0; q < 6; ++q) di[q] = A.dinv32[q * R + v];
+        sym_mul_acc(di, rr, dz);
+        for (int q = 0; q < 3; ++q) {
+            z[q] = __builtin_fmaf(kMgOmega, dz[q], x[q]);
+            A.z[q * R + v] = z[q];
+        }
+        const float r0 = A.r[v], r1 = A.r[R + v], r2 = A.r[2 * R + v];
+        part[0] = __builtin_fma(double(r2), double(z[2]), __builtin_fma(double(r1), double(z[1]), double(r0) * double(z[0])));
+    }
+    const int out[1] = {kRz};
+    tile_store<1>(part, A, c.t, out, lds);
+}
+
+__global__ void k_zero_word(uint32_t *w) { *w = 0u; }
+
+// Host side: the hierarchy of a one-scene arena (device buffers in one
+// allocation, freed with the arena).
+struct MgHost {
+    std::vector<MgLevel> lv;  // levels 1..L
+    float *fres = nullptr;    // the fine level's residual (3n)
+    double *M = nullptr, *W = nullptr;
+    uint32_t *err = nullptr;
+    uint32_t n3 = 0;
+    void *alloc = nullptr;
+    uint64_t built = ~0ull;   // assembly generation the hierarchy was built from
+};
+
 }  // namespace
 }  // namespace vx
 
@@ -2509,6 +2956,10 @@ struct vx_batch {
     // Tile list of the last tick that had one (its device copy is d_list).
     uint32_t *d_list = nullptr;
     std::vector<uint32_t> list_last;
+    // Multigrid hierarchy of a one-scene arena (built on first use) and the
+    // count of assemblies enqueued, which tells whether it is current.
+    vx::MgHost *mg = nullptr;
+    uint64_t asm_gen = 0;
 };
 
 struct vx_scene {
@@ -2578,14 +3029,113 @@ void launch_kernel(int k, K kernel, const dim3 &grid, const dim3 &blk, Args... a
         hipLaunchKernelGGL(kernel, grid, blk, 0, ctx.stream, args...);
 }
 
+// The multigrid hierarchy of a one-scene arena: level grids and buffers,
+// allocated once (the scene's geometry never changes).
+int32_t mg_prepare(vx_batch *b) {
+    if (b->mg) return VX_OK;
+    if (!b->single) return fail(VX_ERR_INVALID_ARGUMENT, "the multigrid preconditioner needs a one-scene arena");
+    const SlotGeo &g = b->slots[0].geo;
+    std::vector<MgLevel> lv;
+    uint32_t nx = g.nx, ny = g.ny, nz = g.nz;
+    do {
+        nx = (nx + 1) / 2;
+        ny = (ny + 1) / 2;
+        nz = (nz + 1) / 2;
+        MgLevel L{};
+        L.nx = nx;
+        L.ny = ny;
+        L.nz = nz;
+        L.n = nx * ny * nz;
+        lv.push_back(L);
+    } while (uint64_t(nx) * ny * nz > kMgCoarsest && lv.size() < kMgMaxLevels);
+    if (lv.back().n > kMgCoarsest) return fail(VX_ERR_TOO_LARGE, "multigrid: the scene is too elongated to coarsen");
+    const uint32_t n3 = 3 * lv.back().n;
+    auto up = [](size_t x) { return (x + 255) & ~size_t(255); };
+    size_t bytes = up(sizeof(float) * 3 * size_t(g.n)) + 2 * up(sizeof(double) * size_t(n3) * n3) + up(sizeof(uint32_t));
+    for (const MgLevel &L : lv) bytes += up(sizeof(float) * 48 * size_t(L.n)) + up(L.n) + up(sizeof(double) * 33 * size_t(L.n));
+    void *mem = nullptr;
+    VX_HIP(hipMalloc(&mem, bytes));
+    MgHost *h = new MgHost;
+    h->alloc = mem;
+    char *q = static_cast<char *>(mem);
+    auto take = [&](size_t nbytes) {
+        char *r = q;
+        q += up(nbytes);
+        return r;
+    };
+    h->fres = reinterpret_cast<float *>(take(sizeof(float) * 3 * size_t(g.n)));
+    h->M = reinterpret_cast<double *>(take(sizeof(double) * size_t(n3) * n3));
+    h->W = reinterpret_cast<double *>(take(sizeof(double) * size_t(n3) * n3));
+    h->err = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t)));
+    for (MgLevel &L : lv) {
+        float *f = reinterpret_cast<float *>(take(sizeof(float) * 48 * size_t(L.n)));
+        L.D = f;
+        L.C = f + 6 * size_t(L.n);
+        L.Dinv = f + 33 * size_t(L.n);
+        L.r = f + 39 * size_t(L.n);
+        L.x = f + 42 * size_t(L.n);
+        L.res = f + 45 * size_t(L.n);
+        L.mask = reinterpret_cast<uint8_t *>(take(L.n));
+        double *d = reinterpret_cast<double *>(take(sizeof(double) * 33 * size_t(L.n)));
+        L.D64 = d;
+        L.C64 = d + 6 * size_t(L.n);
+    }
+    h->lv = lv;
+    h->n3 = n3;
+    b->mg = h;
+    return VX_OK;
+}
+inline dim3 mg_grid(uint32_t n) { return dim3((n + kMgBlock - 1) / kMgBlock); }
+
+// Build the hierarchy from the scene's current FP64 operator (after an
+// assembly), on the library stream.
+void mg_setup(vx_batch *b) {
+    hipStream_t s = context().stream;
+    MgHost &h = *b->mg;
+    const SlotGeo &g = b->slots[0].geo;
+    hipLaunchKernelGGL(mg_build_fine, mg_grid(h.lv[0].n), dim3(kMgBlock), 0, s, b->A, g, h.lv[0]);
+    for (size_t l = 1; l < h.lv.size(); ++l)
+        hipLaunchKernelGGL(mg_build_coarse, mg_grid(h.lv[l].n), dim3(kMgBlock), 0, s, h.lv[l - 1], h.lv[l]);
+    hipLaunchKernelGGL(mg_dense, mg_grid(h.lv.back().n), dim3(kMgBlock), 0, s, h.lv.back(), h.M, h.n3);
+    hipLaunchKernelGGL(k_zero_word, dim3(1), dim3(1), 0, s, h.err);
+    hipLaunchKernelGGL(mg_cholesky, dim3(1), dim3(kMgDenseBlock), 0, s, h.M, h.W, h.n3, h.err);
+}
+
+// One V-cycle z = M^-1 r on the fine level and the partials of r.z; every
+// kernel returns at once when the scene's inner solve has ended.
+void mg_cycle(vx_batch *b, const OneScene &one_u, bool big) {
+    hipStream_t s = context().stream;
+    const MgHost &h = *b->mg;
+    const SlotGeo &g = b->slots[0].geo;
+    const SlotState *st = b->A.st + g.slot;
+    const size_t nl = h.lv.size();
+    hipLaunchKernelGGL(mg_fine_down, mg_grid(g.n), dim3(kMgBlock), 0, s, b->A, g, st, h.fres);
+    hipLaunchKernelGGL(mg_restrict_fine, mg_grid(h.lv[0].n), dim3(kMgBlock), 0, s, b->A, g, st, h.fres, h.lv[0]);
+    for (size_t l = 0; l + 1 < nl; ++l) {
+        hipLaunchKernelGGL(mg_coarse_down, mg_grid(h.lv[l].n), dim3(kMgBlock), 0, s, h.lv[l], st);
+        hipLaunchKernelGGL(mg_restrict_coarse, mg_grid(h.lv[l + 1].n), dim3(kMgBlock), 0, s, h.lv[l], h.lv[l + 1], st);
+    }
+    hipLaunchKernelGGL(mg_coarsest, dim3(1), dim3(kMgDenseBlock), 0, s, h.lv[nl - 1], h.W, h.n3, st);
+    for (size_t l = nl - 1; l-- > 0;)
+        hipLaunchKernelGGL(mg_coarse_up, mg_grid(h.lv[l].n), dim3(kMgBlock), 0, s, h.lv[l], h.lv[l + 1], st);
+    hipLaunchKernelGGL(mg_fine_up, dim3(one_u.grid), dim3(kBlock), 0, s, b->A, one_u, h.lv[0]);
+    if (big) hipLaunchKernelGGL(scene_reduce<float>, dim3(b->max_slots), dim3(kBlock), 0, s, b->A, int(kRz), 1, 0);
+}
+
 template <class S, bool ONE, bool RB, bool EARLY>
 void launch_iteration_t(vx_batch *b, const OneScene &one, const dim3 &grid, const OneScene &one_u, const dim3 &grid_u,
-                        int odd, bool big) {
+                        int odd, bool big, bool mg) {
     Context &ctx = context();
     const dim3 blk(kBlock), sg(b->max_slots);
     auto reduce = [&](int k0, int nk) {
         if (big) hipLaunchKernelGGL(scene_reduce<S>, sg, blk, 0, ctx.stream, b->A, k0, nk, 0);
     };
+    // With multigrid the iteration starts with the cycle: z = M^-1 r and the
+    // partials of r.z replace those of the update before (whose z = D^-1 r
+    // the cycle overwrites).
+    if constexpr (ONE && !RB && std::is_same<S, float>::value) {
+        if (mg) mg_cycle(b, one_u, big);
+    }
     if constexpr (RB) hipLaunchKernelGGL((pcg_rb_red<S, ONE, EARLY>), grid_u, blk, 0, ctx.stream, b->A, one_u, odd);
     if constexpr (EARLY && !RB && std::is_same<S, float>::value)
         launch_kernel(0, pcg_bond_spmv_big<S, ONE>, grid, blk, b->A, one, odd);
@@ -2608,23 +3158,23 @@ void launch_iteration_t(vx_batch *b, const OneScene &one, const dim3 &grid, cons
 // round-robin mapping `one_u` / `grid_u`.
 template <class S, bool EARLY>
 void launch_iteration_e(vx_batch *b, bool rb, const OneScene &one, const dim3 &grid, const OneScene &one_u,
-                        const dim3 &grid_u, int odd, bool big) {
+                        const dim3 &grid_u, int odd, bool big, bool mg) {
     if (rb && b->single)
-        launch_iteration_t<S, true, true, EARLY>(b, one, grid, one_u, grid_u, odd, big);
+        launch_iteration_t<S, true, true, EARLY>(b, one, grid, one_u, grid_u, odd, big, false);
     else if (rb)
-        launch_iteration_t<S, false, true, EARLY>(b, one, grid, one_u, grid_u, odd, big);
+        launch_iteration_t<S, false, true, EARLY>(b, one, grid, one_u, grid_u, odd, big, false);
     else if (b->single)
-        launch_iteration_t<S, true, false, EARLY>(b, one, grid, one_u, grid_u, odd, big);
+        launch_iteration_t<S, true, false, EARLY>(b, one, grid, one_u, grid_u, odd, big, mg);
     else
-        launch_iteration_t<S, false, false, EARLY>(b, one, grid, one_u, grid_u, odd, big);
+        launch_iteration_t<S, false, false, EARLY>(b, one, grid, one_u, grid_u, odd, big, false);
 }
 template <class S>
 void launch_iteration(vx_batch *b, bool rb, const OneScene &one, const dim3 &grid, const OneScene &one_u,
-                      const dim3 &grid_u, int odd, bool big) {
+                      const dim3 &grid_u, int odd, bool big, bool mg) {
     if (big)
-        launch_iteration_e<S, true>(b, rb, one, grid, one_u, grid_u, odd, big);
+        launch_iteration_e<S, true>(b, rb, one, grid, one_u, grid_u, odd, big, mg);
     else
-        launch_iteration_e<S, false>(b, rb, one, grid, one_u, grid_u, odd, big);
+        launch_iteration_e<S, false>(b, rb, one, grid, one_u, grid_u, odd, big, mg);
 }
 
 // One inner chunk: the refinement residual of scenes whose inner solve has
@@ -2639,7 +3189,7 @@ bool deal_update_rr(const vx_batch *b, bool big) {
     return b->deal == VX_DEAL_AUTO ? big : b->deal != VX_DEAL_CONTIGUOUS;
 }
 
-void enqueue_inner(vx_batch *b, int f64, bool rb, uint32_t tiles, bool big, uint32_t iters) {
+void enqueue_inner(vx_batch *b, int f64, bool rb, uint32_t tiles, bool big, uint32_t iters, bool mg) {
     Context &ctx = context();
     const dim3 grid(tiles), blk(kBlock), sg(b->max_slots);
     if (f64) {
@@ -2661,17 +3211,18 @@ void enqueue_inner(vx_batch *b, int f64, bool rb, uint32_t tiles, bool big, uint
     const dim3 gi(b->single ? one.grid : tiles), gu(b->single ? one_u.grid : tiles);
     for (uint32_t t = 0; t < iters; ++t) {
         if (f64)
-            launch_iteration<double>(b, rb, one, gi, one_u, gu, int(t & 1), big);
+            launch_iteration<double>(b, rb, one, gi, one_u, gu, int(t & 1), big, false);
         else
-            launch_iteration<float>(b, rb, one, gi, one_u, gu, int(t & 1), big);
+            launch_iteration<float>(b, rb, one, gi, one_u, gu, int(t & 1), big, mg);
     }
 }
 
-int32_t capture_inner(vx_batch *b, int f64, bool rb, uint32_t tiles, bool big, uint32_t iters, hipGraphExec_t *out) {
+int32_t capture_inner(vx_batch *b, int f64, bool rb, uint32_t tiles, bool big, uint32_t iters, bool mg,
+                      hipGraphExec_t *out) {
     Context &ctx = context();
     hipGraph_t graph;
     VX_HIP(hipStreamBeginCapture(ctx.stream, hipStreamCaptureModeThreadLocal));
-    enqueue_inner(b, f64, rb, tiles, big, iters);
+    enqueue_inner(b, f64, rb, tiles, big, iters, mg);
     VX_HIP(hipStreamEndCapture(ctx.stream, &graph));
     hipError_t e = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
@@ -2698,16 +3249,17 @@ bool any_big(const vx_batch *b) {
     return false;
 }
 
-int32_t inner_graph(vx_batch *b, int f64, bool rb, uint32_t tiles, uint32_t iters, hipGraphExec_t *out) {
+int32_t inner_graph(vx_batch *b, int f64, bool rb, uint32_t tiles, uint32_t iters, bool mg, hipGraphExec_t *out) {
     const bool big = any_big(b);
-    const uint64_t key = (uint64_t(iters) << 48) | (uint64_t(tiles) << 4) | (uint64_t(b->A.tile_list != nullptr) << 3) |
-                         (uint64_t(rb) << 2) | (uint64_t(big) << 1) | uint64_t(f64);
+    const uint64_t key = (uint64_t(iters) << 48) | (uint64_t(mg) << 47) | (uint64_t(tiles) << 4) |
+                         (uint64_t(b->A.tile_list != nullptr) << 3) | (uint64_t(rb) << 2) | (uint64_t(big) << 1) |
+                         uint64_t(f64);
     for (auto &kv : b->g_inner)
         if (kv.first == key) {
             *out = kv.second;
             return VX_OK;
         }
-    if (int32_t c = capture_inner(b, f64, rb, tiles, big, iters, out)) return c;
+    if (int32_t c = capture_inner(b, f64, rb, tiles, big, iters, mg, out)) return c;
     b->g_inner.emplace_back(key, *out);
     return VX_OK;
 }
@@ -2770,10 +3322,12 @@ int32_t launch_tick(vx_batch *b, Flight &f) {
     hipStream_t s = ctx.stream;
     uint32_t any = 0;
     bool inner[2] = {false, false}, resid0[2] = {false, false}, rb[2] = {false, false}, rb0[2] = {false, false};
+    bool mg = false;  // the one scene solves with the multigrid cycle (FP32 inner only)
     for (uint32_t i = 0; i < b->max_slots; ++i) {
         if (!b->slots[i].used) f.bits[i] = 0;
         const uint32_t bi = f.bits[i];
         any |= bi;
+        if ((bi & B_MG) && (bi & (B_INNER | B_RESID0)) && !(bi & B_F64)) mg = true;
         const int prec = (bi & B_F64) ? 1 : 0;
         const bool red_black = bi & B_RB;
         if (bi & B_INNER) {
@@ -2858,6 +3412,15 @@ int32_t launch_tick(vx_batch *b, Flight &f) {
     if (any & B_ASSEMBLE) {
         hipLaunchKernelGGL(assemble_bonds, grid, blk, 0, s, b->A);
         hipLaunchKernelGGL(assemble_rows, grid, blk, 0, s, b->A);
+        b->asm_gen += 1;
+    }
+    if (mg) {
+        // the hierarchy follows the latest assembly
+        if (int32_t c = mg_prepare(b)) return c;
+        if (b->mg->built != b->asm_gen) {
+            mg_setup(b);
+            b->mg->built = b->asm_gen;
+        }
     }
     const bool big = any_big(b);
     if (resid0[0]) {
@@ -2880,13 +3443,14 @@ int32_t launch_tick(vx_batch *b, Flight &f) {
     }
     for (int prec = 0; prec < 2; ++prec) {
         if (!inner[prec]) continue;
+        const bool mg_prec = mg && prec == 0;
         if (ctx.direct_launch || g_timer) {
-            enqueue_inner(b, prec, rb[prec], tiles, big, f.iters);
+            enqueue_inner(b, prec, rb[prec], tiles, big, f.iters, mg_prec);
             VX_HIP(hipGetLastError());
             continue;
         }
         hipGraphExec_t g = nullptr;
-        if (int32_t c = inner_graph(b, prec, rb[prec], tiles, f.iters, &g)) return c;
+        if (int32_t c = inner_graph(b, prec, rb[prec], tiles, f.iters, mg_prec, &g)) return c;
         VX_HIP(hipGraphLaunch(g, s));
     }
     if (any & (B_APPLY | B_WARM_SAVE)) hipLaunchKernelGGL(apply_correction, grid, blk, 0, s, b->A);
@@ -2915,7 +3479,8 @@ int32_t launch_tick(vx_batch *b, Flight &f) {
 
 // Bits of every solve tick: the inner precision and the preconditioner.
 uint32_t solve_bits(const vx_solve_params &s) {
-    return (s.mode == VX_SOLVE_F64 ? uint32_t(B_F64) : 0u) | (s.preconditioner == VX_PRECOND_RED_BLACK ? uint32_t(B_RB) : 0u);
+    return (s.mode == VX_SOLVE_F64 ? uint32_t(B_F64) : 0u) | (s.preconditioner == VX_PRECOND_RED_BLACK ? uint32_t(B_RB) : 0u) |
+           (s.preconditioner == VX_PRECOND_MULTIGRID ? uint32_t(B_MG) : 0u);
 }
 uint32_t prec_bits(const vx_step_params &p) { return solve_bits(p.solve); }
 // The end of a substep: the fracture pass (when on) and the velocities.
@@ -3360,8 +3925,8 @@ SlotParams device_params(double h, const double *g, const vx_solve_params *s, co
 }
 
 bool solve_ok(const vx_solve_params &s) {
-    return s.max_inner > 0 && s.tolerance > 0.0 && s.mode <= VX_SOLVE_F64 && s.preconditioner <= VX_PRECOND_RED_BLACK &&
-           s.reserved == 0;
+    return s.max_inner > 0 && s.tolerance > 0.0 && s.mode <= VX_SOLVE_F64 && s.preconditioner <= VX_PRECOND_MULTIGRID &&
+           !(s.preconditioner == VX_PRECOND_MULTIGRID && s.mode != VX_SOLVE_MIXED) && s.reserved == 0;
 }
 
 int32_t admit(vx_batch *b, const vx_scene_desc *d, const vx_step_params *p, uint32_t steps, bool manual, uint32_t *out_slot) {
@@ -3370,6 +3935,8 @@ int32_t admit(vx_batch *b, const vx_scene_desc *d, const vx_step_params *p, uint
     if (int32_t c = check_desc(d, bonds, tmask, mask, &active)) return c;
     if (!manual && (p->substeps == 0 || p->corotation_iterations == 0 || !(p->fixed_dt > 0.0) || !solve_ok(p->solve)))
         return fail(VX_ERR_INVALID_ARGUMENT, "invalid step parameters");
+    if (!b->single && p && p->solve.preconditioner == VX_PRECOND_MULTIGRID)
+        return fail(VX_ERR_INVALID_ARGUMENT, "the multigrid preconditioner needs a one-scene arena (vx_scene_*)");
     uint32_t slot = b->max_slots;
     for (uint32_t i = 0; i < b->max_slots; ++i)
         if (!b->slots[i].used) {
@@ -3492,6 +4059,11 @@ extern "C" int32_t vx_batch_destroy(vx_batch *b) {
         delete b->tool;
         b->tool = nullptr;
     }
+    if (b->mg) {
+        if (b->mg->alloc) (void)hipFree(b->mg->alloc);
+        delete b->mg;
+        b->mg = nullptr;
+    }
     if (b->alloc) (void)hipFree(b->alloc);
     delete b;
     return VX_OK;
@@ -3536,7 +4108,7 @@ extern "C" int32_t vx_batch_admit(vx_batch *b, const vx_scene_desc *d, const vx_
 // inner chunk continues with another chunk (which does nothing if its solve
 // has ended in the meantime); a scene whose tick in flight changes its phase
 // sits this tick out.
-constexpr uint32_t kChunkBits = B_INNER | B_F64 | B_RB;  // bits of a pure inner chunk
+constexpr uint32_t kChunkBits = B_INNER | B_F64 | B_RB | B_MG;  // bits of a pure inner chunk
 extern "C" int32_t vx_batch_tick(vx_batch *b, uint32_t *out_running) {
     clear_error();
     if (!b) return fail(VX_ERR_INVALID_ARGUMENT, "null batch");

@@ -29,6 +29,7 @@ VX_SOLVE_MIXED = 0
 VX_SOLVE_F64 = 1
 VX_PRECOND_BLOCK_JACOBI = 0
 VX_PRECOND_RED_BLACK = 1
+VX_PRECOND_MULTIGRID = 2
 
 VX_SCENE_FREE = 0
 VX_SCENE_RUNNING = 1

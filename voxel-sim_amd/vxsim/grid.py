@@ -14,7 +14,7 @@ import numpy as np
 from . import lib
 from ._abi import (FractureParams, KernelTimes, SceneDesc, SolveParams, SolveReport, StepParams, StepReport,
                    VX_DEAL_AUTO, VX_DEAL_CONTIGUOUS, VX_DEAL_ROUND_ROBIN, VX_DEAL_SPLIT,
-                   VX_LAUNCH_DIRECT, VX_LAUNCH_GRAPHS, VX_PRECOND_BLOCK_JACOBI,
+                   VX_LAUNCH_DIRECT, VX_LAUNCH_GRAPHS, VX_PRECOND_BLOCK_JACOBI, VX_PRECOND_MULTIGRID,
                    VX_PRECOND_RED_BLACK, VX_SOLVE_F64, VX_SOLVE_MIXED, c_dp, c_fp, c_u8p, c_u32p)
 from .solver import SolverError, check
 
@@ -22,13 +22,15 @@ MIXED = VX_SOLVE_MIXED
 F64 = VX_SOLVE_F64
 BLOCK_JACOBI = VX_PRECOND_BLOCK_JACOBI
 RED_BLACK = VX_PRECOND_RED_BLACK
+MULTIGRID = VX_PRECOND_MULTIGRID
 
 
 def solve_params(tolerance=1e-9, inner_tolerance=3e-3, max_inner=4000, max_outer=12, warm_start=False, mode=MIXED,
                  preconditioner=BLOCK_JACOBI):
-    """vx_solve_params. `preconditioner`: BLOCK_JACOBI (the reference's) or
+    """vx_solve_params. `preconditioner`: BLOCK_JACOBI (the reference's),
     RED_BLACK (CG on the red-black reduced system: red-black block symmetric
-    Gauss-Seidel, about half the iterations)."""
+    Gauss-Seidel, about half the iterations) or MULTIGRID (one aggregation
+    multigrid V-cycle per iteration; one-scene arenas, mode MIXED only)."""
     return SolveParams(float(tolerance), float(inner_tolerance), int(max_inner), int(max_outer), 1 if warm_start else 0,
                        int(mode), int(preconditioner), 0)
 
