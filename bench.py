@@ -235,6 +235,19 @@ def large_roofline() -> dict:
         gs.assemble(h)
         gs.solve(sp)  # warm-up (first touch of the inner planes)
         out = spmv_in_solve(gs, scene, sp)
+        # the same system with the multigrid preconditioner, beside block-Jacobi
+        t = time.perf_counter()
+        rbj = gs.solve(sp)
+        tbj = time.perf_counter() - t
+        mg_sp = grid.solve_params(tolerance=1e-9, inner_tolerance=INNER_TOL, max_inner=40000, max_outer=12,
+                                  preconditioner=grid.MULTIGRID)
+        gs.solve(mg_sp)  # builds the hierarchy, captures its graphs
+        t = time.perf_counter()
+        rmg = gs.solve(mg_sp)
+        tmg = time.perf_counter() - t
+    out["solve_block_jacobi"] = {"ms": 1000 * tbj, "inner": rbj["inner_iterations"], "sweeps": rbj["outer_sweeps"]}
+    out["solve_multigrid"] = {"ms": 1000 * tmg, "inner": rmg["inner_iterations"], "sweeps": rmg["outer_sweeps"],
+                              "rel_residual": rmg["relative_residual"], "time_vs_block_jacobi": tbj / tmg}
     out["workload"] = "%s: %d rows per launch, one solve to relative residual 1e-9" % (scene.name, scene.n)
     out["traffic"] = measured_traffic(TRAFFIC_FILE_LARGE)
     out["stream_copy_same_bytes"] = stream_copy(out["bytes_per_launch"] // 2)
@@ -435,14 +448,15 @@ def serve_bench(rate: float, seconds: float, rev_lo: float, rev_hi: float, steps
     print(json.dumps(line))
 
 
-def red_black_steps(scene, step_kw: dict, steps: int, warmup: int, ms_block_jacobi: float) -> dict:
-    """The same steps from the same initial scene with the red-black
-    preconditioner (its wall time per step beside block-Jacobi's)."""
+def red_black_steps(scene, step_kw: dict, steps: int, warmup: int, ms_block_jacobi: float,
+                    preconditioner: int = 1) -> dict:
+    """The same steps from the same initial scene with another preconditioner
+    (red-black by default; its wall time per step beside block-Jacobi's)."""
     from vxsim import grid
     sp = step_kw["solve"]
     kw = dict(step_kw, solve=grid.solve_params(tolerance=sp.tolerance, inner_tolerance=sp.inner_tolerance,
                                                max_inner=sp.max_inner, max_outer=sp.max_outer,
-                                               preconditioner=grid.RED_BLACK))
+                                               preconditioner=preconditioner))
     with grid.GridScene(scene) as gs:
         for _ in range(warmup):
             gs.step(**kw)
@@ -874,6 +888,18 @@ def main():
         t = time.perf_counter()
         rrb5 = gs.solve(rb_sp5)
         trb5 = time.perf_counter() - t
+        # Aggregation multigrid (one V-cycle per iteration) on the same system.
+        mg_sp = grid.solve_params(tolerance=SOLVE_TOL, inner_tolerance=INNER_TOL, max_inner=8000, max_outer=12,
+                                  preconditioner=grid.MULTIGRID)
+        gs.solve(mg_sp)  # builds the hierarchy, captures its graphs
+        t = time.perf_counter()
+        rmg = gs.solve(mg_sp)
+        tmg = time.perf_counter() - t
+        mg_sp5 = grid.solve_params(tolerance=1e-5, inner_tolerance=INNER_TOL, max_inner=8000, max_outer=12,
+                                   preconditioner=grid.MULTIGRID)
+        t = time.perf_counter()
+        rmg5 = gs.solve(mg_sp5)
+        tmg5 = time.perf_counter() - t
         mixed_rate = 3.0 * active * (rm["inner_iterations"] + rm["outer_sweeps"]) / tm
         f64_rate = 3.0 * active * (r64["inner_iterations"] + r64["outer_sweeps"]) / t64
         rows = gs.export_rows()
@@ -916,6 +942,13 @@ def main():
             "red_black_solve_to_1e-5": {"ms": 1000 * trb5, "inner": rrb5["inner_iterations"],
                                         "sweeps": rrb5["outer_sweeps"], "rel_residual": rrb5["relative_residual"],
                                         "time_vs_block_jacobi": t5 / trb5},
+            "multigrid_solve": {"ms": 1000 * tmg, "inner": rmg["inner_iterations"], "sweeps": rmg["outer_sweeps"],
+                                "rel_residual": rmg["relative_residual"], "tolerance": SOLVE_TOL,
+                                "time_vs_block_jacobi": tm / tmg, "time_vs_red_black": trb / tmg,
+                                "note": "solve of an assembled system with the hierarchy already built"},
+            "multigrid_solve_to_1e-5": {"ms": 1000 * tmg5, "inner": rmg5["inner_iterations"],
+                                        "sweeps": rmg5["outer_sweeps"], "rel_residual": rmg5["relative_residual"],
+                                        "time_vs_block_jacobi": t5 / tmg5},
             "mixed_vs_plain_fp64_structured": mixed_rate / f64_rate,
             "mixed_vs_plain_fp64_dropin": mixed_rate / drop_rate,
             "time_to_solution_vs_fp64_structured": t64 / tm,
@@ -924,6 +957,8 @@ def main():
     cpu_on = not args.no_cpu_baseline
     if not args.no_compare:
         extra["red_black_step"] = red_black_steps(scene, step_kw, args.steps, args.warmup, ms_per_step)
+        extra["multigrid_step"] = red_black_steps(scene, step_kw, args.steps, args.warmup, ms_per_step,
+                                                  preconditioner=2)
     if not args.no_batch:
         extra["batched"] = batched(cpu=cpu_on)
         rb = batched(preconditioner=1)

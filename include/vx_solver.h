@@ -219,7 +219,7 @@ typedef struct vx_solve_params {
 /* Aggregation multigrid: one V(1,1) cycle per iteration as the preconditioner
  * of the FP32 inner PCG (2x2x2 aggregates, Galerkin coarse operators with the
  * bond stencil's shape, damped block-Jacobi smoothing, the coarsest level of
- * at most 128 nodes solved by its Cholesky factor), rebuilt after every
+ * at most 64 nodes solved by its Cholesky factor), rebuilt after every
  * assembly. One-scene arenas (vx_scene_*) and VX_SOLVE_MIXED only. No
  * counterpart in the reference: about 25x fewer iterations than
  * block-Jacobi, each a dozen small launches and about 2.2 times the bytes. */

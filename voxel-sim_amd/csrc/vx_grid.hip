@@ -2489,17 +2489,18 @@ struct Flight {
 // the bond blocks k with the free-component masks, A[v, v + e] = -F_v k F_j.
 // Smoothing is damped block-Jacobi, x += w D^-1 (r - A x), once before
 // (from x = 0: x = w D^-1 r) and once after the coarse correction; the
-// coarsest level (at most kMgCoarsest nodes) is solved exactly by its
-// Cholesky factor, e = W^T W r with W = L^-1 computed once per assembly in
-// FP64. The hierarchy is built in FP64 from the FP64 operator after every
+// coarsest level (at most kMgCoarsest nodes) is solved exactly, e = A^-1 r,
+// with the inverse found once per assembly in FP64 from its Cholesky factor
+// (A^-1 = W^T W, W = L^-1). The hierarchy is built in FP64 from the FP64 operator after every
 // assembly and applied in FP32. The cycle is a fixed symmetric positive
 // definite linear operator, so the outer iteration stays conjugate gradients.
 constexpr float kMgOmega = 0.6f;
-constexpr uint32_t kMgCoarsest = 128;          // nodes of the level solved directly, at most
+constexpr uint32_t kMgCoarsest = 64;           // nodes of the level solved directly, at most
 constexpr uint32_t kMgMaxDense = 3 * kMgCoarsest;
 constexpr uint32_t kMgMaxLevels = 16;
 constexpr int kMgBlock = 256;
 constexpr int kMgDenseBlock = 512;             // one workgroup for the dense factor and its use
+constexpr uint32_t kMgChunk = 8;               // inner iterations per chunk at most
 
 struct MgLevel {
     uint32_t nx, ny, nz, n;  // node grid of the level
@@ -2636,39 +2637,79 @@ __global__ __launch_bounds__(kMgBlock) void mg_dense(MgLevel L, double *M, uint3
     }
 }
 
-// Cholesky factor of M in place (lower triangle), then W = L^-1 (lower,
-// row-major in W). One workgroup; a non-positive pivot sets *err and is
-// replaced by 1 (the cycle then stays defined; the solve reports the status).
-__global__ __launch_bounds__(kMgDenseBlock) void mg_cholesky(double *M, double *W, uint32_t n3, uint32_t *err) {
-    __shared__ double col[kMgMaxDense];
-    const uint32_t t = threadIdx.x;
+// Cholesky factor of M (in LDS, the lower triangle packed by rows: at most
+// 148 KB for 192 unknowns), then W = L^-1 (lower), stored in FP32 (W^T W
+// stays symmetric positive definite whatever W's rounding). M is banded
+// (|row - column| <= bw: the stencil's reach in the lexicographic node
+// order), and so is L. One workgroup, right-looking: at step k the column
+// below the pivot is scaled and the trailing band updated, one column per
+// thread. L then goes to global memory (over M, packed) and the same LDS
+// holds W while it is found row by row by forward substitution on the rows
+// of the identity (FP64 sums), one column per thread; finally M receives the
+// inverse W^T W in FP64. A non-positive pivot sets *err and is replaced by 1
+// (the cycle stays defined).
+__device__ __forceinline__ uint32_t tri(uint32_t i, uint32_t j) { return i * (i + 1) / 2 + j; }
+constexpr uint32_t kMgLds = kMgMaxDense * (kMgMaxDense + 1) / 2 * 8;  // bytes: packed FP64 L, or padded FP32 W
+static_assert(kMgMaxDense * (kMgMaxDense + 1) * 4 <= kMgLds, "W fits where L was");
+constexpr uint32_t kMgLoads = 8;  // loads in flight per thread when moving the dense blocks
+__global__ __launch_bounds__(kMgDenseBlock) void mg_cholesky(double *M, uint32_t n3, uint32_t bw, uint32_t *err) {
+    __shared__ double lds[kMgLds / 8];
+    __shared__ double lrow[kMgMaxDense];
+    double *Lp = lds;
+    const uint32_t t = threadIdx.x, nn = n3 * n3;
+    for (uint32_t e0 = t; e0 < nn; e0 += kMgDenseBlock * kMgLoads) {  // the lower triangle of M
+        double v[kMgLoads];
+#pragma unroll
+        for (uint32_t u = 0; u < kMgLoads; ++u) {
+            const uint32_t e = e0 + u * kMgDenseBlock;
+            v[u] = e < nn ? M[e] : 0.0;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kMgLoads; ++u) {
+            const uint32_t e = e0 + u * kMgDenseBlock, i = e / n3, j = e % n3;
+            if (e < nn && j <= i) Lp[tri(i, j)] = v[u];
+        }
+    }
     for (uint32_t k = 0; k < n3; ++k) {
         __syncthreads();  // the previous step's updates are done
-        const double piv = M[size_t(k) * n3 + k];
+        const double piv = Lp[tri(k, k)];
         const double d = piv > 0.0 ? sqrt(piv) : 1.0;
         if (!(piv > 0.0) && t == 0) *err = 1u;
-        for (uint32_t i = k + 1 + t; i < n3; i += kMgDenseBlock) {
-            const double l = M[size_t(i) * n3 + k] / d;
-            M[size_t(i) * n3 + k] = l;
-            col[i] = l;
-        }
+        const uint32_t end = min(n3, k + bw + 1);  // rows below k with entries in column k
+        for (uint32_t i = k + 1 + t; i < end; i += kMgDenseBlock) Lp[tri(i, k)] /= d;
         __syncthreads();  // column k is scaled; every thread has read the pivot
-        if (t == 0) M[size_t(k) * n3 + k] = d;
-        for (uint32_t i = k + 1 + t; i < n3; i += kMgDenseBlock) {
-            const double li = col[i];
-            double *row = M + size_t(i) * n3;
-            for (uint32_t j = k + 1; j <= i; ++j) row[j] -= li * col[j];
+        if (t == 0) Lp[tri(k, k)] = d;
+        for (uint32_t j = k + 1 + t; j < end; j += kMgDenseBlock) {
+            const double lj = Lp[tri(j, k)];
+            for (uint32_t i = j; i < end; ++i) Lp[tri(i, j)] -= Lp[tri(i, k)] * lj;
         }
     }
     __syncthreads();
-    // column j of L^-1 by forward substitution (one thread per column)
-    for (uint32_t j = t; j < n3; j += kMgDenseBlock) {
-        for (uint32_t i = 0; i < j; ++i) W[size_t(i) * n3 + j] = 0.0;
-        for (uint32_t i = j; i < n3; ++i) {
-            double s = i == j ? 1.0 : 0.0;
-            for (uint32_t m = j; m < i; ++m) s -= M[size_t(i) * n3 + m] * W[size_t(m) * n3 + j];
-            W[size_t(i) * n3 + j] = s / M[size_t(i) * n3 + i];
+    const uint32_t nt = n3 * (n3 + 1) / 2;
+    for (uint32_t e = t; e < nt; e += kMgDenseBlock) M[e] = Lp[e];  // L, packed
+    __syncthreads();  // L is in global memory and every thread is done with Lp
+    float *Wl = reinterpret_cast<float *>(lds);  // W[k][c] at k * (n3 + 1) + c
+    const uint32_t ld = n3 + 1;
+    for (uint32_t e = t; e < n3 * ld; e += kMgDenseBlock) Wl[e] = 0.0f;
+    __syncthreads();
+    for (uint32_t k = 0; k < n3; ++k) {
+        const uint32_t m0 = k > bw ? k - bw : 0u;
+        for (uint32_t m = m0 + t; m <= k; m += kMgDenseBlock) lrow[m] = M[tri(k, m)];  // row k of L
+        __syncthreads();
+        const double lkk = lrow[k];
+        for (uint32_t c = t; c <= k; c += kMgDenseBlock) {
+            double s = c == k ? 1.0 : 0.0;
+            for (uint32_t m = max(c, m0); m < k; ++m) s -= lrow[m] * double(Wl[m * ld + c]);
+            Wl[k * ld + c] = float(s / lkk);
         }
+        __syncthreads();  // row k is complete (and lrow free) before the next row
+    }
+    // the inverse of the coarsest matrix, A^-1 = W^T W (FP64, over M)
+    for (uint32_t e = t; e < nn; e += kMgDenseBlock) {
+        const uint32_t a = e / n3, b = e % n3;
+        double acc = 0.0;
+        for (uint32_t k = max(a, b); k < n3; ++k) acc = __builtin_fma(double(Wl[k * ld + a]), double(Wl[k * ld + b]), acc);
+        M[e] = acc;
     }
 }
 
@@ -2804,24 +2845,17 @@ __global__ __launch_bounds__(kMgBlock) void mg_coarse_down(MgLevel L, const Slot
     for (int q = 0; q < 3; ++q) L.res[size_t(q) * L.n + I] = L.r[size_t(q) * L.n + I] - y[q];
 }
 
-// The coarsest level: res = W^T (W r), one workgroup.
-__global__ __launch_bounds__(kMgDenseBlock) void mg_coarsest(MgLevel L, const double *W, uint32_t n3, const SlotState *st) {
-    __shared__ double rv[kMgMaxDense], uv[kMgMaxDense];
+// The coarsest level: res = A^-1 r with the dense inverse found at set-up;
+// one wave per row, its lanes splitting the row (the matrix is symmetric).
+__global__ __launch_bounds__(kMgDenseBlock) void mg_coarsest(MgLevel L, const double *Ainv, uint32_t n3, const SlotState *st) {
     if (st->done) return;
-    const uint32_t t = threadIdx.x;
-    for (uint32_t m = t; m < n3; m += kMgDenseBlock) rv[m] = double(L.r[size_t(m % 3) * L.n + m / 3]);
-    __syncthreads();
-    for (uint32_t i = t; i < n3; i += kMgDenseBlock) {
-        double s = 0.0;
-        for (uint32_t m = 0; m <= i; ++m) s = __builtin_fma(W[size_t(i) * n3 + m], rv[m], s);
-        uv[i] = s;
-    }
-    __syncthreads();
-    for (uint32_t m = t; m < n3; m += kMgDenseBlock) {
-        double s = 0.0;
-        for (uint32_t i = m; i < n3; ++i) s = __builtin_fma(W[size_t(i) * n3 + m], uv[i], s);
-        L.res[size_t(m % 3) * L.n + m / 3] = float(s);
-    }
+    const uint32_t lane = threadIdx.x & 63u, i = blockIdx.x * (kMgDenseBlock / 64) + (threadIdx.x >> 6);
+    if (i >= n3) return;  // the whole wave
+    double s = 0.0;
+    for (uint32_t m = lane; m < n3; m += 64)
+        s = __builtin_fma(Ainv[size_t(i) * n3 + m], double(L.r[size_t(m % 3) * L.n + m / 3]), s);
+    s = wave_sum(s);
+    if (lane == 0) L.res[size_t(i % 3) * L.n + i / 3] = float(s);
 }
 
 // Coarse level, after the correction: x' = x + P e (e: the coarser level's
@@ -2912,7 +2946,7 @@ __global__ void k_zero_word(uint32_t *w) { *w = 0u; }
 struct MgHost {
     std::vector<MgLevel> lv;  // levels 1..L
     float *fres = nullptr;    // the fine level's residual (3n)
-    double *M = nullptr, *W = nullptr;
+    double *M = nullptr;  // the coarsest level, dense; after set-up its inverse
     uint32_t *err = nullptr;
     uint32_t n3 = 0;
     void *alloc = nullptr;
@@ -3051,7 +3085,7 @@ int32_t mg_prepare(vx_batch *b) {
     if (lv.back().n > kMgCoarsest) return fail(VX_ERR_TOO_LARGE, "multigrid: the scene is too elongated to coarsen");
     const uint32_t n3 = 3 * lv.back().n;
     auto up = [](size_t x) { return (x + 255) & ~size_t(255); };
-    size_t bytes = up(sizeof(float) * 3 * size_t(g.n)) + 2 * up(sizeof(double) * size_t(n3) * n3) + up(sizeof(uint32_t));
+    size_t bytes = up(sizeof(float) * 3 * size_t(g.n)) + up(sizeof(double) * size_t(n3) * n3) + up(sizeof(uint32_t));
     for (const MgLevel &L : lv) bytes += up(sizeof(float) * 48 * size_t(L.n)) + up(L.n) + up(sizeof(double) * 33 * size_t(L.n));
     void *mem = nullptr;
     VX_HIP(hipMalloc(&mem, bytes));
@@ -3065,7 +3099,6 @@ int32_t mg_prepare(vx_batch *b) {
     };
     h->fres = reinterpret_cast<float *>(take(sizeof(float) * 3 * size_t(g.n)));
     h->M = reinterpret_cast<double *>(take(sizeof(double) * size_t(n3) * n3));
-    h->W = reinterpret_cast<double *>(take(sizeof(double) * size_t(n3) * n3));
     h->err = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t)));
     for (MgLevel &L : lv) {
         float *f = reinterpret_cast<float *>(take(sizeof(float) * 48 * size_t(L.n)));
@@ -3098,7 +3131,9 @@ void mg_setup(vx_batch *b) {
         hipLaunchKernelGGL(mg_build_coarse, mg_grid(h.lv[l].n), dim3(kMgBlock), 0, s, h.lv[l - 1], h.lv[l]);
     hipLaunchKernelGGL(mg_dense, mg_grid(h.lv.back().n), dim3(kMgBlock), 0, s, h.lv.back(), h.M, h.n3);
     hipLaunchKernelGGL(k_zero_word, dim3(1), dim3(1), 0, s, h.err);
-    hipLaunchKernelGGL(mg_cholesky, dim3(1), dim3(kMgDenseBlock), 0, s, h.M, h.W, h.n3, h.err);
+    const MgLevel &Lc = h.lv.back();
+    const uint32_t bw = std::min<uint32_t>(h.n3 - 1, 3 * Lc.nx * Lc.ny + 2);  // the stencil's reach in DOF order
+    hipLaunchKernelGGL(mg_cholesky, dim3(1), dim3(kMgDenseBlock), 0, s, h.M, h.n3, bw, h.err);
 }
 
 // One V-cycle z = M^-1 r on the fine level and the partials of r.z; every
@@ -3115,7 +3150,8 @@ void mg_cycle(vx_batch *b, const OneScene &one_u, bool big) {
         hipLaunchKernelGGL(mg_coarse_down, mg_grid(h.lv[l].n), dim3(kMgBlock), 0, s, h.lv[l], st);
         hipLaunchKernelGGL(mg_restrict_coarse, mg_grid(h.lv[l + 1].n), dim3(kMgBlock), 0, s, h.lv[l], h.lv[l + 1], st);
     }
-    hipLaunchKernelGGL(mg_coarsest, dim3(1), dim3(kMgDenseBlock), 0, s, h.lv[nl - 1], h.W, h.n3, st);
+    hipLaunchKernelGGL(mg_coarsest, dim3((h.n3 + kMgDenseBlock / 64 - 1) / (kMgDenseBlock / 64)), dim3(kMgDenseBlock), 0, s,
+                       h.lv[nl - 1], h.M, h.n3, st);
     for (size_t l = nl - 1; l-- > 0;)
         hipLaunchKernelGGL(mg_coarse_up, mg_grid(h.lv[l].n), dim3(kMgBlock), 0, s, h.lv[l], h.lv[l + 1], st);
     hipLaunchKernelGGL(mg_fine_up, dim3(one_u.grid), dim3(kBlock), 0, s, b->A, one_u, h.lv[0]);
@@ -3439,7 +3475,9 @@ int32_t launch_tick(vx_batch *b, Flight &f) {
     f.iters = (inner[0] || inner[1]) ? b->iters_per_tick : 0u;
     if (b->single && (inner[0] || inner[1])) {
         const Flight &in_flight = b->fl[b->cur ^ 1];
-        f.iters = chunk_iters(b->slots[0], b->known[0], b->iters_per_tick, in_flight.active ? in_flight.iters : 0u);
+        // with multigrid a segment takes about a dozen iterations: shorter chunks
+        const uint32_t full = mg ? std::min(b->iters_per_tick, kMgChunk) : b->iters_per_tick;
+        f.iters = chunk_iters(b->slots[0], b->known[0], full, in_flight.active ? in_flight.iters : 0u);
     }
     for (int prec = 0; prec < 2; ++prec) {
         if (!inner[prec]) continue;
