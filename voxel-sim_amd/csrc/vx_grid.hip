@@ -2641,20 +2641,21 @@ __global__ __launch_bounds__(kMgBlock) void mg_dense(MgLevel L, double *M, uint3
 // 148 KB for 192 unknowns), then W = L^-1 (lower), stored in FP32 (W^T W
 // stays symmetric positive definite whatever W's rounding). M is banded
 // (|row - column| <= bw: the stencil's reach in the lexicographic node
-// order), and so is L. One workgroup, right-looking: at step k the column
-// below the pivot is scaled and the trailing band updated, one column per
-// thread. L then goes to global memory (over M, packed) and the same LDS
-// holds W while it is found row by row by forward substitution on the rows
-// of the identity (FP64 sums), one column per thread; finally M receives the
-// inverse W^T W in FP64. A non-positive pivot sets *err and is replaced by 1
-// (the cycle stays defined).
+// order), and so is L. One workgroup factors it (right-looking: at step k
+// the column below the pivot is scaled and the trailing band updated by
+// every thread); L then goes to global memory (over M, packed) and the same LDS
+// holds W while it is found row by row by forward substitution (one column
+// per thread, FP64 sums, the band rows of L staged in LDS); finally M
+// receives the inverse W^T W in FP64. A non-positive
+// pivot sets *err and is replaced by 1 (the cycle stays defined).
 __device__ __forceinline__ uint32_t tri(uint32_t i, uint32_t j) { return i * (i + 1) / 2 + j; }
 constexpr uint32_t kMgLds = kMgMaxDense * (kMgMaxDense + 1) / 2 * 8;  // bytes: packed FP64 L, or padded FP32 W
 static_assert(kMgMaxDense * (kMgMaxDense + 1) * 4 <= kMgLds, "W fits where L was");
 constexpr uint32_t kMgLoads = 8;  // loads in flight per thread when moving the dense blocks
+constexpr uint32_t kMgRowBlock = 8;  // rows of L staged together while W is found
 __global__ __launch_bounds__(kMgDenseBlock) void mg_cholesky(double *M, uint32_t n3, uint32_t bw, uint32_t *err) {
     __shared__ double lds[kMgLds / 8];
-    __shared__ double lrow[kMgMaxDense];
+    __shared__ double lblk[kMgRowBlock * kMgMaxDense];
     double *Lp = lds;
     const uint32_t t = threadIdx.x, nn = n3 * n3;
     for (uint32_t e0 = t; e0 < nn; e0 += kMgDenseBlock * kMgLoads) {  // the lower triangle of M
@@ -2671,7 +2672,7 @@ __global__ __launch_bounds__(kMgDenseBlock) void mg_cholesky(double *M, uint32_t
         }
     }
     for (uint32_t k = 0; k < n3; ++k) {
-        __syncthreads();  // the previous step's updates are done
+        __syncthreads();  // the triangle is in LDS; the previous step's updates are done
         const double piv = Lp[tri(k, k)];
         const double d = piv > 0.0 ? sqrt(piv) : 1.0;
         if (!(piv > 0.0) && t == 0) *err = 1u;
@@ -2679,9 +2680,14 @@ __global__ __launch_bounds__(kMgDenseBlock) void mg_cholesky(double *M, uint32_t
         for (uint32_t i = k + 1 + t; i < end; i += kMgDenseBlock) Lp[tri(i, k)] /= d;
         __syncthreads();  // column k is scaled; every thread has read the pivot
         if (t == 0) Lp[tri(k, k)] = d;
-        for (uint32_t j = k + 1 + t; j < end; j += kMgDenseBlock) {
-            const double lj = Lp[tri(j, k)];
-            for (uint32_t i = j; i < end; ++i) Lp[tri(i, j)] -= Lp[tri(i, k)] * lj;
+        // the trailing band, (i, j) with k < j <= i < end, spread over every thread
+        const uint32_t nb = end - k - 1, ne = nb * (nb + 1) / 2;
+        for (uint32_t e = t; e < ne; e += kMgDenseBlock) {
+            uint32_t ii = uint32_t((sqrtf(8.0f * float(e) + 1.0f) - 1.0f) * 0.5f);
+            while (ii * (ii + 1) / 2 > e) --ii;  // exact for any rounding of the root
+            while ((ii + 1) * (ii + 2) / 2 <= e) ++ii;
+            const uint32_t i = k + 1 + ii, j = k + 1 + (e - ii * (ii + 1) / 2);
+            Lp[tri(i, j)] -= Lp[tri(i, k)] * Lp[tri(j, k)];
         }
     }
     __syncthreads();
@@ -2692,17 +2698,26 @@ __global__ __launch_bounds__(kMgDenseBlock) void mg_cholesky(double *M, uint32_t
     const uint32_t ld = n3 + 1;
     for (uint32_t e = t; e < n3 * ld; e += kMgDenseBlock) Wl[e] = 0.0f;
     __syncthreads();
-    for (uint32_t k = 0; k < n3; ++k) {
-        const uint32_t m0 = k > bw ? k - bw : 0u;
-        for (uint32_t m = m0 + t; m <= k; m += kMgDenseBlock) lrow[m] = M[tri(k, m)];  // row k of L
-        __syncthreads();
-        const double lkk = lrow[k];
-        for (uint32_t c = t; c <= k; c += kMgDenseBlock) {
-            double s = c == k ? 1.0 : 0.0;
-            for (uint32_t m = max(c, m0); m < k; ++m) s -= lrow[m] * double(Wl[m * ld + c]);
-            Wl[k * ld + c] = float(s / lkk);
+    // W row by row: W[k][c] = (e_k - sum_{m<k} L[k][m] W[m][c])_c / L[k][k] for
+    // c <= k, one column per thread; L[k][m] is zero for m < k - bw. The band
+    // rows of L are staged in LDS eight at a time.
+    for (uint32_t k0 = 0; k0 < n3; k0 += kMgRowBlock) {
+        for (uint32_t r = 0; r < kMgRowBlock && k0 + r < n3; ++r) {
+            const uint32_t k = k0 + r, m0 = k > bw ? k - bw : 0u;
+            for (uint32_t m = m0 + t; m <= k; m += kMgDenseBlock) lblk[r * kMgMaxDense + (m - m0)] = M[tri(k, m)];
         }
-        __syncthreads();  // row k is complete (and lrow free) before the next row
+        __syncthreads();
+        for (uint32_t r = 0; r < kMgRowBlock && k0 + r < n3; ++r) {
+            const uint32_t k = k0 + r, m0 = k > bw ? k - bw : 0u;
+            const double *lr = lblk + r * kMgMaxDense;  // lr[m - m0] = L[k][m]
+            const double lkk = lr[k - m0];
+            for (uint32_t c = t; c <= k; c += kMgDenseBlock) {
+                double s = c == k ? 1.0 : 0.0;
+                for (uint32_t m = max(c, m0); m < k; ++m) s -= lr[m - m0] * double(Wl[m * ld + c]);
+                Wl[k * ld + c] = float(s / lkk);
+            }
+            __syncthreads();  // row k is complete before the next rows read it (and lblk before its reload)
+        }
     }
     // the inverse of the coarsest matrix, A^-1 = W^T W (FP64, over M)
     for (uint32_t e = t; e < nn; e += kMgDenseBlock) {
@@ -2713,16 +2728,12 @@ __global__ __launch_bounds__(kMgDenseBlock) void mg_cholesky(double *M, uint32_t
     }
 }
 
-// x0 = w D^-1 r of fine row v (zero on fixed components: r is zero there and
-// D^-1 is the identity).
+// x0 = w D^-1 r of fine row v: the update kernel (or the residual
+// replacement) before the cycle leaves z = D^-1 r (zero on fixed components:
+// r is zero there and D^-1 is the identity).
 __device__ __forceinline__ void mg_x0(const DevArena &A, uint32_t v, float (&x)[3]) {
     const size_t R = A.R;
-    float r[3], di[6];
-    for (int q = 0; q < 3; ++q) r[q] = A.r[q * R + v];
-    for (int q = 0; q < 6; ++q) di[q] = A.dinv32[q * R + v];
-    x[0] = kMgOmega * __builtin_fmaf(di[2], r[2], __builtin_fmaf(di[1], r[1], di[0] * r[0]));
-    x[1] = kMgOmega * __builtin_fmaf(di[4], r[2], __builtin_fmaf(di[3], r[1], di[1] * r[0]));
-    x[2] = kMgOmega * __builtin_fmaf(di[5], r[2], __builtin_fmaf(di[4], r[1], di[2] * r[0]));
+    for (int q = 0; q < 3; ++q) x[q] = kMgOmega * A.z[q * R + v];
 }
 __device__ __forceinline__ void sym_mul_acc(const float (&s)[6], const float (&x)[3], float (&y)[3]) {
     y[0] = __builtin_fmaf(s[2], x[2], __builtin_fmaf(s[1], x[1], __builtin_fmaf(s[0], x[0], y[0])));
@@ -2731,7 +2742,8 @@ __device__ __forceinline__ void sym_mul_acc(const float (&s)[6], const float (&x
 }
 
 // Fine level, before the coarse correction: res = r - A x0 with x0 = w D^-1 r.
-__global__ __launch_bounds__(kMgBlock) void mg_fine_down(DevArena A, SlotGeo g, const SlotState *st, float *res) {
+__global__ __launch_bounds__(kMgBlock) void mg_fine_down(DevArena A, SlotGeo g, const SlotState *st, float *res,
+                                                         float *x0) {
     const uint32_t loc = blockIdx.x * kMgBlock + threadIdx.x;
     if (st->done || loc >= g.n) return;
     const size_t R = A.R;
@@ -2757,7 +2769,10 @@ __global__ __launch_bounds__(kMgBlock) void mg_fine_down(DevArena A, SlotGeo g, 
         }
     }
     const uint32_t m = A.mask[v];
-    for (int q = 0; q < 3; ++q) res[size_t(q) * g.n + loc] = A.r[q * R + v] - (y[q] - ((m >> q & 1u) ? acc[q] : 0.0f));
+    for (int q = 0; q < 3; ++q) {
+        res[size_t(q) * g.n + loc] = A.r[q * R + v] - (y[q] - ((m >> q & 1u) ? acc[q] : 0.0f));
+        x0[size_t(q) * g.n + loc] = x[q];  // for the fine level's post-smoothing (z is overwritten there)
+    }
 }
 
 // Restriction to level L (children from the fine level or from level S) and
@@ -2884,7 +2899,7 @@ __global__ __launch_bounds__(kMgBlock) void mg_coarse_up(MgLevel L, MgLevel P, c
 // Fine level, after the correction: z = x' + w D^-1 (r - A x') with
 // x' = x0 + P e, and the tile's partial of r.z (the preconditioned residual
 // the next SpMV's head takes). Tiles as the inner kernels of the scene.
-__global__ __launch_bounds__(kBlock) void mg_fine_up(DevArena A, OneScene one, MgLevel P) {
+__global__ __launch_bounds__(kBlock) void mg_fine_up(DevArena A, OneScene one, MgLevel P, const float *x0) {
     Tile c;
     if (!inner_tile<float, true>(A, one, c)) return;
     __shared__ double lds[1][kWaves];
@@ -2898,10 +2913,9 @@ __global__ __launch_bounds__(kBlock) void mg_fine_up(DevArena A, OneScene one, M
             uint32_t cu[3];
             coords(g, u - g.voff, cu);
             const uint32_t pu = cu[0] / 2 + P.nx * (cu[1] / 2 + P.ny * (cu[2] / 2));
-            const uint32_t m = A.mask[u];
-            mg_x0(A, u, x);
+            const uint32_t m = A.mask[u], lu = u - g.voff;
             for (int q = 0; q < 3; ++q)
-                if (m >> q & 1u) x[q] += P.res[size_t(q) * P.n + pu];
+                x[q] = x0[size_t(q) * g.n + lu] + ((m >> q & 1u) ? P.res[size_t(q) * P.n + pu] : 0.0f);
         };
         uint32_t cc[3];
         coords(g, c.local, cc);
@@ -2946,6 +2960,7 @@ __global__ void k_zero_word(uint32_t *w) { *w = 0u; }
 struct MgHost {
     std::vector<MgLevel> lv;  // levels 1..L
     float *fres = nullptr;    // the fine level's residual (3n)
+    float *fx0 = nullptr;     // the fine level's pre-smoothed iterate (3n)
     double *M = nullptr;  // the coarsest level, dense; after set-up its inverse
     uint32_t *err = nullptr;
     uint32_t n3 = 0;
@@ -3085,7 +3100,7 @@ int32_t mg_prepare(vx_batch *b) {
     if (lv.back().n > kMgCoarsest) return fail(VX_ERR_TOO_LARGE, "multigrid: the scene is too elongated to coarsen");
     const uint32_t n3 = 3 * lv.back().n;
     auto up = [](size_t x) { return (x + 255) & ~size_t(255); };
-    size_t bytes = up(sizeof(float) * 3 * size_t(g.n)) + up(sizeof(double) * size_t(n3) * n3) + up(sizeof(uint32_t));
+    size_t bytes = 2 * up(sizeof(float) * 3 * size_t(g.n)) + up(sizeof(double) * size_t(n3) * n3) + up(sizeof(uint32_t));
     for (const MgLevel &L : lv) bytes += up(sizeof(float) * 48 * size_t(L.n)) + up(L.n) + up(sizeof(double) * 33 * size_t(L.n));
     void *mem = nullptr;
     VX_HIP(hipMalloc(&mem, bytes));
@@ -3098,6 +3113,7 @@ int32_t mg_prepare(vx_batch *b) {
         return r;
     };
     h->fres = reinterpret_cast<float *>(take(sizeof(float) * 3 * size_t(g.n)));
+    h->fx0 = reinterpret_cast<float *>(take(sizeof(float) * 3 * size_t(g.n)));
     h->M = reinterpret_cast<double *>(take(sizeof(double) * size_t(n3) * n3));
     h->err = reinterpret_cast<uint32_t *>(take(sizeof(uint32_t)));
     for (MgLevel &L : lv) {
@@ -3144,7 +3160,7 @@ void mg_cycle(vx_batch *b, const OneScene &one_u, bool big) {
     const SlotGeo &g = b->slots[0].geo;
     const SlotState *st = b->A.st + g.slot;
     const size_t nl = h.lv.size();
-    hipLaunchKernelGGL(mg_fine_down, mg_grid(g.n), dim3(kMgBlock), 0, s, b->A, g, st, h.fres);
+    hipLaunchKernelGGL(mg_fine_down, mg_grid(g.n), dim3(kMgBlock), 0, s, b->A, g, st, h.fres, h.fx0);
     hipLaunchKernelGGL(mg_restrict_fine, mg_grid(h.lv[0].n), dim3(kMgBlock), 0, s, b->A, g, st, h.fres, h.lv[0]);
     for (size_t l = 0; l + 1 < nl; ++l) {
         hipLaunchKernelGGL(mg_coarse_down, mg_grid(h.lv[l].n), dim3(kMgBlock), 0, s, h.lv[l], st);
@@ -3154,7 +3170,7 @@ void mg_cycle(vx_batch *b, const OneScene &one_u, bool big) {
                        h.lv[nl - 1], h.M, h.n3, st);
     for (size_t l = nl - 1; l-- > 0;)
         hipLaunchKernelGGL(mg_coarse_up, mg_grid(h.lv[l].n), dim3(kMgBlock), 0, s, h.lv[l], h.lv[l + 1], st);
-    hipLaunchKernelGGL(mg_fine_up, dim3(one_u.grid), dim3(kBlock), 0, s, b->A, one_u, h.lv[0]);
+    hipLaunchKernelGGL(mg_fine_up, dim3(one_u.grid), dim3(kBlock), 0, s, b->A, one_u, h.lv[0], h.fx0);
     if (big) hipLaunchKernelGGL(scene_reduce<float>, dim3(b->max_slots), dim3(kBlock), 0, s, b->A, int(kRz), 1, 0);
 }
 
