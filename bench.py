@@ -606,6 +606,32 @@ def cut_contact(scene, gs, prev_pos, targets, cpu: bool) -> dict:
     return out
 
 
+def multigrid_bench(reps: int = 3) -> None:
+    """Solves of the 64^3 mixed box's first assembled system with the
+    multigrid preconditioner (the hierarchy built by the first), beside one
+    block-Jacobi solve; one JSON line (the run the multigrid kernel trace
+    profiles)."""
+    from vxsim import grid, scenes
+    scene = scenes.large_64cube()
+    cfg = scene.config
+    h = cfg.fixed_dt / cfg.substeps
+    with grid.GridScene(scene) as gs:
+        gs.predict(h, cfg.gravity)
+        gs.fit_rotations()
+        gs.assemble(h)
+        runs = []
+        for prec in [grid.MULTIGRID] * reps + [grid.BLOCK_JACOBI]:
+            sp = grid.solve_params(tolerance=SOLVE_TOL, inner_tolerance=INNER_TOL, max_inner=8000, max_outer=12,
+                                   preconditioner=prec)
+            t = time.perf_counter()
+            r = gs.solve(sp)
+            runs.append({"preconditioner": "multigrid" if prec == grid.MULTIGRID else "block_jacobi",
+                         "ms": 1000 * (time.perf_counter() - t), "inner": r["inner_iterations"],
+                         "sweeps": r["outer_sweeps"], "rel_residual": r["relative_residual"]})
+    print(json.dumps({"workload": "large_64x64x64_mixed: first assembled system, rel. residual %g" % SOLVE_TOL,
+                      "solves": runs}))
+
+
 def cut_bench(steps: int, dims=(192, 192, 96), preconditioner: int = 0, contact: bool = False, cpu: bool = True,
               speed: float = 5.0):
     print(json.dumps(cut_run(steps, dims, preconditioner, contact, cpu, speed)))
@@ -662,7 +688,7 @@ def cut_run(steps: int, dims=(192, 192, 96), preconditioner: int = 0, contact: b
                         "launched at %g m/s, %d steps of 1 ms, <=%d passes, rel. residual 1e-6" %
                         (scene.name, scene.n, blade[0], blade[1], blade[2], len(tool.cells), tool.mass, speed, steps,
                          cfg.corotation_iterations),
-            "preconditioner": "red_black" if preconditioner else "block_jacobi",
+            "preconditioner": {0: "block_jacobi", 1: "red_black", 2: "multigrid"}[preconditioner],
             "mean_ms_per_step": 1000 * sum(walls) / len(walls),
             "ms_per_step": [round(1000 * w, 2) for w in walls], "solves": [r["solves"] for r in reps],
             "inner_iterations": [r["inner_iterations"] for r in reps],
@@ -772,6 +798,8 @@ def main():
     ap.add_argument("--cut", action="store_true", help="192x192x96 block cut by the blade tool with fracture")
     ap.add_argument("--cut-speed", type=float, default=5.0, help="--cut: the blade's launch speed (m/s)")
     ap.add_argument("--red-black", action="store_true", help="--cut / --serve with the red-black preconditioner")
+    ap.add_argument("--multigrid", action="store_true",
+                    help="--cut with the aggregation multigrid preconditioner; alone: multigrid solves of the 64^3 system")
     ap.add_argument("--contact", action="store_true", help="contact projection leg only (64^3 contact box)")
     ap.add_argument("--no-contact", action="store_true")
     ap.add_argument("--cut-contact", action="store_true",
@@ -804,9 +832,12 @@ def main():
     if args.contact:
         print(json.dumps({"contact_projection": contact_bench(cpu=not args.no_cpu_baseline)}))
         return
+    if args.multigrid and not args.cut:
+        multigrid_bench()
+        return
     if args.cut:
-        cut_bench(args.steps, preconditioner=1 if args.red_black else 0, contact=args.cut_contact, speed=args.cut_speed,
-                  cpu=not args.no_cpu_baseline)
+        cut_bench(args.steps, preconditioner=2 if args.multigrid else (1 if args.red_black else 0),
+                  contact=args.cut_contact, speed=args.cut_speed, cpu=not args.no_cpu_baseline)
         return
     if args.large:
         large_bench(args.steps, args.warmup, compare=not args.no_compare)

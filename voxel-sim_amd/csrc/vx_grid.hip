@@ -2505,7 +2505,7 @@ constexpr uint32_t kMgChunk = 8;               // inner iterations per chunk at 
 struct MgLevel {
     uint32_t nx, ny, nz, n;  // node grid of the level
     float *D, *C, *Dinv;     // 6n symmetric, 27n ([axis][3x3 row-major]), 6n symmetric
-    float *r, *x, *res;      // 3n each: right-hand side, smoothed iterate, residual (after the cycle: the result)
+    float *r, *x, *res;      // 3n each: right-hand side, pre-smoothed iterate, residual (after the cycle: the level's result)
     uint8_t *mask;           // free components
     double *D64, *C64;       // FP64 copies of D and C (the next level is built from them)
 };
