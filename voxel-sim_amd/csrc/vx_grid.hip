@@ -2500,7 +2500,7 @@ constexpr uint32_t kMgMaxDense = 3 * kMgCoarsest;
 constexpr uint32_t kMgMaxLevels = 16;
 constexpr int kMgBlock = 256;
 constexpr int kMgDenseBlock = 512;             // one workgroup for the dense factor and its use
-constexpr uint32_t kMgChunk = 8;               // inner iterations per chunk at most
+constexpr uint32_t kMgChunk = 4;               // inner iterations per chunk at most
 
 struct MgLevel {
     uint32_t nx, ny, nz, n;  // node grid of the level
