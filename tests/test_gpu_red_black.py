@@ -131,8 +131,8 @@ def test_invalid_preconditioner_is_rejected(gpu, scenes):
     from vxsim.solver import SolverError
     s = scenes.large_64cube(dims=(6, 5, 4))
     gs, _ = device_assembled(s, s.config.fixed_dt, s.config.gravity)
-    with pytest.raises(SolverError):
-        gs.solve(grid.solve_params(tolerance=1e-9, preconditioner=2))
+    with pytest.raises(SolverError):  # 0, 1 and 2 (multigrid) are the preconditioners
+        gs.solve(grid.solve_params(tolerance=1e-9, preconditioner=3))
     gs.close()
 
 
