@@ -259,6 +259,17 @@ int32_t vx_scene_download(vx_scene *scene, double *positions, double *rotations,
  * voxel index of each active row. Any pointer may be NULL. */
 int32_t vx_scene_export_rows(vx_scene *scene, uint32_t *active, uint32_t *neighbors, double *blocks,
                              double *diagonal, double *rhs, uint32_t *out_active_count);
+/* The multigrid hierarchy the last VX_PRECOND_MULTIGRID solve or step of the
+ * scene built (VX_ERR_INVALID_ARGUMENT before the first): level `level` (1 =
+ * the first coarse level) as its node grid (dims[3]; always written when
+ * non-NULL, so a call with the other pointers NULL sizes the buffers; 0 0 0
+ * past the coarsest level), its diagonal blocks (6 per node, xx xy xz yy yz
+ * zz), its couplings to the next node along each axis (27 per node, [axis][3x3
+ * row-major], A[I, I+e] = -C) and its free-component masks (bit c: component
+ * c free), node-major, in FP32 as the cycle reads them. No counterpart in the
+ * reference: for checking the Galerkin products. Any pointer may be NULL. */
+int32_t vx_scene_multigrid_level(vx_scene *scene, uint32_t level, uint32_t *dims, float *diagonal, float *couplings,
+                                 uint8_t *masks);
 /* Mean duration (microseconds) of the FP32 structured bond SpMV kernel in a
  * graph of `reps` back-to-back launches on the library stream (launched one by
  * one under VX_LAUNCH_DIRECT), timed with HIP events; the median of five

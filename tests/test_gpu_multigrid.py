@@ -119,3 +119,90 @@ def test_multigrid_arguments(gpu, scenes):
             B.admit(s, 1, grid.step_params(fixed_dt=c.fixed_dt, gravity=c.gravity, substeps=1,
                                            corotation_iterations=1, corotation_tolerance=c.corotation_tolerance,
                                            solve=grid.solve_params(preconditioner=grid.MULTIGRID)))
+
+
+def galerkin(A, keys, n3):
+    """P^T A P for 2x2x2 aggregation of the nodes at integer `keys`
+    (component by component), with the coarse keys; A is scipy sparse over
+    3 * len(keys) unknowns."""
+    import scipy.sparse as sp
+    ck = keys // 2
+    u, agg = np.unique(ck, axis=0, return_inverse=True)
+    agg = agg.reshape(-1)
+    n = len(keys)
+    P = sp.csr_matrix((np.ones(3 * n), (np.arange(3 * n), np.repeat(3 * agg, 3) + np.tile(np.arange(3), n))),
+                      shape=(3 * n, 3 * len(u)))
+    return (P.T @ A @ P).tocsr(), u
+
+
+def rows_matrix(rows):
+    import scipy.sparse as sp
+    na = len(rows["active"])
+    nbr = rows["neighbors"].astype(np.int64)
+    I, J, V = [], [], []
+    bi = np.arange(na)
+    for q in range(6):
+        ok = nbr[:, q] != 0xFFFFFFFF
+        blk = rows["blocks"][:, 9 * q:9 * q + 9].reshape(na, 3, 3)
+        for a in range(3):
+            for b in range(3):
+                I.append(3 * bi[ok] + a)
+                J.append(3 * nbr[ok, q] + b)
+                V.append(blk[ok, a, b])
+    D = rows["diagonal"].reshape(na, 3, 3)
+    for a in range(3):
+        for b in range(3):
+            I.append(3 * bi + a)
+            J.append(3 * bi + b)
+            V.append(D[:, a, b])
+    return sp.csr_matrix((np.concatenate(V), (np.concatenate(I), np.concatenate(J))), shape=(3 * na, 3 * na))
+
+
+def check_level(lev, Ac, ck):
+    """The device level's diagonal blocks and couplings against the Galerkin
+    product on every node that has a child in the product."""
+    nx, ny, nz = lev["dims"]
+    sym = [[0, 1, 2], [1, 3, 4], [2, 4, 5]]
+    A = Ac.toarray() if Ac.shape[0] <= 6000 else None
+    scale = float(np.abs(Ac.data).max())
+    pos = {tuple(k): i for i, k in enumerate(ck)}
+    worst = 0.0
+    for i, k in enumerate(ck):
+        I = int(k[0] + nx * (k[1] + ny * k[2]))
+        blk = Ac[3 * i:3 * i + 3, 3 * i:3 * i + 3].toarray() if A is None else A[3 * i:3 * i + 3, 3 * i:3 * i + 3]
+        dev = np.array([[lev["diagonal"][I, sym[r][c]] for c in range(3)] for r in range(3)], np.float64)
+        worst = max(worst, np.abs(dev - blk).max() / scale)
+        for a in range(3):
+            kn = list(k)
+            kn[a] += 1
+            j = pos.get(tuple(kn))
+            ref = np.zeros((3, 3)) if j is None else -(
+                Ac[3 * i:3 * i + 3, 3 * j:3 * j + 3].toarray() if A is None else A[3 * i:3 * i + 3, 3 * j:3 * j + 3])
+            worst = max(worst, np.abs(lev["couplings"][I, a].astype(np.float64) - ref).max() / scale)
+        assert lev["masks"][I] == 7
+    return worst
+
+
+def test_multigrid_levels_are_galerkin_products(gpu, scenes):
+    """Levels 1 and 2 of the device hierarchy against P^T A P computed here
+    from the device's own exported FP64 system (2x2x2 aggregates of the
+    active rows; the scene has no per-axis targets, so every active row is
+    free in every component): diagonal blocks and couplings to 1e-6 of the
+    largest entry (FP32 storage)."""
+    from vxsim import grid
+    s = scenes.large_64cube(dims=(20, 18, 15))
+    gs, _ = device_assembled(s, s.config.fixed_dt, s.config.gravity)
+    with pytest.raises(Exception):
+        gs.multigrid_level(1)  # no hierarchy before a multigrid solve
+    gs.solve(tolerance=1e-6, preconditioner=grid.MULTIGRID)
+    rows = gs.export_rows()
+    lev1, lev2 = gs.multigrid_level(1), gs.multigrid_level(2)
+    past = gs.multigrid_level(20)
+    gs.close()
+    assert lev1["dims"] == (10, 9, 8) and lev2["dims"] == (5, 5, 4) and past is None
+    keys = (np.asarray(s.keys) - np.asarray(s.keys).min(axis=0))[rows["active"]]  # lattice coordinates from 0
+    A1, ck1 = galerkin(rows_matrix(rows), keys, 3)
+    A2, ck2 = galerkin(A1, ck1, 3)
+    w1, w2 = check_level(lev1, A1, ck1), check_level(lev2, A2, ck2)
+    print("Galerkin levels: max deviation %.2e, %.2e of the largest entry" % (w1, w2))
+    assert w1 <= 1e-6 and w2 <= 1e-6

@@ -4733,6 +4733,42 @@ extern "C" int32_t vx_scene_set_targets_and_loads(vx_scene *sc, const float *tar
 // structured operator: slots in incidence order (-x, +x, -y, +y, -z, +z with
 // absent bonds compacted out), off-diagonal blocks -P_i k P_j, zero blocks and
 // EMPTY for inactive neighbours.
+extern "C" int32_t vx_scene_multigrid_level(vx_scene *sc, uint32_t level, uint32_t *dims, float *diagonal,
+                                            float *couplings, uint8_t *masks) {
+    clear_error();
+    if (int32_t c = scene_ok(sc)) return c;
+    Lock lock;
+    vx_batch *b = sc->batch;
+    if (!b->mg || b->mg->built == ~0ull || level == 0)
+        return fail(VX_ERR_INVALID_ARGUMENT, "no multigrid hierarchy (solve with VX_PRECOND_MULTIGRID first) or level 0");
+    if (int32_t c = drain(b)) return c;
+    VX_HIP(hipStreamSynchronize(context().stream));
+    if (level > b->mg->lv.size()) {
+        if (dims) dims[0] = dims[1] = dims[2] = 0;
+        return VX_OK;
+    }
+    const MgLevel &L = b->mg->lv[level - 1];
+    if (dims) {
+        dims[0] = L.nx;
+        dims[1] = L.ny;
+        dims[2] = L.nz;
+    }
+    const size_t n = L.n;
+    auto node_major = [&](const float *dev, int planes, float *out) -> int32_t {
+        std::vector<float> h(size_t(planes) * n);
+        VX_HIP(hipMemcpy(h.data(), dev, sizeof(float) * h.size(), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; ++i)
+            for (int q = 0; q < planes; ++q) out[i * planes + q] = h[size_t(q) * n + i];
+        return VX_OK;
+    };
+    if (diagonal)
+        if (int32_t c = node_major(L.D, 6, diagonal)) return c;
+    if (couplings)
+        if (int32_t c = node_major(L.C, 27, couplings)) return c;
+    if (masks) VX_HIP(hipMemcpy(masks, L.mask, n, hipMemcpyDeviceToHost));
+    return VX_OK;
+}
+
 extern "C" int32_t vx_scene_export_rows(vx_scene *sc, uint32_t *active_out, uint32_t *neighbors, double *blocks, double *diagonal,
                                         double *rhs_out, uint32_t *out_active_count) {
     clear_error();

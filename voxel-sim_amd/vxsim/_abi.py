@@ -177,6 +177,7 @@ SIGNATURES = [
     ("vx_scene_finish_substep", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_double]),
     ("vx_scene_download", ctypes.c_int32, [ctypes.c_void_p, c_dp, c_dp, c_dp, c_dp]),
     ("vx_scene_export_rows", ctypes.c_int32, [ctypes.c_void_p, c_u32p, c_u32p, c_dp, c_dp, c_dp, c_u32p]),
+    ("vx_scene_multigrid_level", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint32, c_u32p, c_fp, c_fp, c_u8p]),
     ("vx_scene_time_spmv", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint32, c_dp]),
     ("vx_scene_solve_timed", ctypes.c_int32,
      [ctypes.c_void_p, ctypes.POINTER(SolveParams), ctypes.POINTER(SolveReport), ctypes.POINTER(KernelTimes)]),

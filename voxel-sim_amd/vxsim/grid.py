@@ -249,6 +249,23 @@ class GridScene:
             raise SolverError(6, "active row count mismatch %d != %d" % (cnt.value, na))
         return dict(active=active, neighbors=nbr, blocks=blocks, diagonal=diag, rhs=rhs)
 
+    def multigrid_level(self, level: int):
+        """vx_scene_multigrid_level: level `level` (1 = the first coarse level)
+        of the hierarchy the last multigrid solve built, as dict(dims,
+        diagonal (n, 6), couplings (n, 3, 3, 3) [axis][row][column], masks
+        (n,)); None past the coarsest level."""
+        dims = np.zeros(3, np.uint32)
+        check(lib().vx_scene_multigrid_level(self.handle, int(level), dims.ctypes.data_as(c_u32p), None, None, None))
+        n = int(dims[0]) * int(dims[1]) * int(dims[2])
+        if n == 0:
+            return None
+        d = np.empty((n, 6), np.float32)
+        c = np.empty((n, 27), np.float32)
+        m = np.empty(n, np.uint8)
+        check(lib().vx_scene_multigrid_level(self.handle, int(level), dims.ctypes.data_as(c_u32p),
+                                             d.ctypes.data_as(c_fp), c.ctypes.data_as(c_fp), m.ctypes.data_as(c_u8p)))
+        return dict(dims=tuple(int(x) for x in dims), diagonal=d, couplings=c.reshape(n, 3, 3, 3), masks=m)
+
     def set_schedule(self, iterations_per_tick: int):
         """Inner PCG iterations per scheduling tick (graph replay)."""
         check(lib().vx_scene_set_schedule(self.handle, int(iterations_per_tick)))
