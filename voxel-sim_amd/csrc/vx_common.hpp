@@ -103,6 +103,20 @@ __device__ __forceinline__ void st_row(T *plane, uint32_t byte_off, T value) {
     VX_GLOBAL char *base = reinterpret_cast<VX_GLOBAL char *>(uniform_addr(plane));
     *reinterpret_cast<VX_GLOBAL T *>(base + byte_off) = value;
 }
+// Streaming (nontemporal) forms of the row access, for planes a launch reads
+// once or writes for a later kernel, when the arena's planes are too large
+// to stay in the caches between kernels anyway: their lines then do not
+// displace the neighbour planes that the same launch re-reads.
+template <class T>
+__device__ __forceinline__ T ld_row_nt(const T *plane, uint32_t byte_off) {
+    const VX_GLOBAL char *base = reinterpret_cast<const VX_GLOBAL char *>(uniform_addr(plane));
+    return __builtin_nontemporal_load(reinterpret_cast<const VX_GLOBAL T *>(base + byte_off));
+}
+template <class T>
+__device__ __forceinline__ void st_row_nt(T *plane, uint32_t byte_off, T value) {
+    VX_GLOBAL char *base = reinterpret_cast<VX_GLOBAL char *>(uniform_addr(plane));
+    __builtin_nontemporal_store(value, reinterpret_cast<VX_GLOBAL T *>(base + byte_off));
+}
 
 // ---- reductions -------------------------------------------------------------
 // Cross-workgroup sums never use atomics or fences: each kernel writes one
