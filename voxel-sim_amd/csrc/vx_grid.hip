@@ -2756,6 +2756,8 @@ __device__ __forceinline__ void sym_mul_acc(const float (&s)[6], const float (&x
 }
 
 // Fine level, before the coarse correction: res = r - A x0 with x0 = w D^-1 r.
+// NT (arenas beyond the Infinity Cache): D and the stores stream.
+template <bool NT>
 __global__ __launch_bounds__(kMgBlock) void mg_fine_down(DevArena A, SlotGeo g, const SlotState *st, float *res,
                                                          float *x0) {
     const uint32_t loc = blockIdx.x * kMgBlock + threadIdx.x;
@@ -2766,7 +2768,7 @@ __global__ __launch_bounds__(kMgBlock) void mg_fine_down(DevArena A, SlotGeo g, 
     coords(g, loc, cc);
     float x[3], d[6], y[3] = {0.0f, 0.0f, 0.0f}, acc[3] = {0.0f, 0.0f, 0.0f};
     mg_x0(A, v, x);
-    for (int q = 0; q < 6; ++q) d[q] = A.d32[q * R + v];
+    for (int q = 0; q < 6; ++q) d[q] = NT ? __builtin_nontemporal_load(A.d32 + q * R + v) : A.d32[q * R + v];
     sym_mul_acc(d, x, y);
     for (int a = 0; a < 3; ++a) {
         const uint32_t s = stride_of(g, a);
@@ -2784,8 +2786,14 @@ __global__ __launch_bounds__(kMgBlock) void mg_fine_down(DevArena A, SlotGeo g, 
     }
     const uint32_t m = A.mask[v];
     for (int q = 0; q < 3; ++q) {
-        res[size_t(q) * g.n + loc] = A.r[q * R + v] - (y[q] - ((m >> q & 1u) ? acc[q] : 0.0f));
-        x0[size_t(q) * g.n + loc] = x[q];  // for the fine level's post-smoothing (z is overwritten there)
+        const float rq = A.r[q * R + v] - (y[q] - ((m >> q & 1u) ? acc[q] : 0.0f));
+        if constexpr (NT) {
+            __builtin_nontemporal_store(rq, res + size_t(q) * g.n + loc);
+            __builtin_nontemporal_store(x[q], x0 + size_t(q) * g.n + loc);
+        } else {
+            res[size_t(q) * g.n + loc] = rq;
+            x0[size_t(q) * g.n + loc] = x[q];  // for the fine level's post-smoothing (z is overwritten there)
+        }
     }
 }
 
@@ -2913,6 +2921,8 @@ __global__ __launch_bounds__(kMgBlock) void mg_coarse_up(MgLevel L, MgLevel P, c
 // Fine level, after the correction: z = x' + w D^-1 (r - A x') with
 // x' = x0 + P e, and the tile's partial of r.z (the preconditioned residual
 // the next SpMV's head takes). Tiles as the inner kernels of the scene.
+// NT: D, D^-1 and the store of z stream.
+template <bool NT>
 __global__ __launch_bounds__(kBlock) void mg_fine_up(DevArena A, OneScene one, MgLevel P, const float *x0) {
     Tile c;
     if (!inner_tile<float, true>(A, one, c)) return;
@@ -2935,7 +2945,7 @@ __global__ __launch_bounds__(kBlock) void mg_fine_up(DevArena A, OneScene one, M
         coords(g, c.local, cc);
         float x[3], d[6], y[3] = {0.0f, 0.0f, 0.0f}, acc[3] = {0.0f, 0.0f, 0.0f};
         xat(v, x);
-        for (int q = 0; q < 6; ++q) d[q] = A.d32[q * R + v];
+        for (int q = 0; q < 6; ++q) d[q] = NT ? __builtin_nontemporal_load(A.d32 + q * R + v) : A.d32[q * R + v];
         sym_mul_acc(d, x, y);
         for (int a = 0; a < 3; ++a) {
             const uint32_t s = stride_of(g, a);
@@ -2954,11 +2964,12 @@ __global__ __launch_bounds__(kBlock) void mg_fine_up(DevArena A, OneScene one, M
         const uint32_t m = A.mask[v];
         float rr[3], di[6], dz[3] = {0.0f, 0.0f, 0.0f}, z[3];
         for (int q = 0; q < 3; ++q) rr[q] = A.r[q * R + v] - (y[q] - ((m >> q & 1u) ? acc[q] : 0.0f));
-        for (int q = 0; q < 6; ++q) di[q] = A.dinv32[q * R + v];
+        for (int q = 0; q < 6; ++q) di[q] = NT ? __builtin_nontemporal_load(A.dinv32 + q * R + v) : A.dinv32[q * R + v];
         sym_mul_acc(di, rr, dz);
         for (int q = 0; q < 3; ++q) {
             z[q] = __builtin_fmaf(kMgOmega, dz[q], x[q]);
-            A.z[q * R + v] = z[q];
+            if constexpr (NT) __builtin_nontemporal_store(z[q], A.z + q * R + v);
+            else A.z[q * R + v] = z[q];
         }
         const float r0 = A.r[v], r1 = A.r[R + v], r2 = A.r[2 * R + v];
         part[0] = __builtin_fma(double(r2), double(z[2]), __builtin_fma(double(r1), double(z[1]), double(r0) * double(z[0])));
@@ -3168,13 +3179,14 @@ void mg_setup(vx_batch *b) {
 
 // One V-cycle z = M^-1 r on the fine level and the partials of r.z; every
 // kernel returns at once when the scene's inner solve has ended.
+template <bool NT>
 void mg_cycle(vx_batch *b, const OneScene &one_u, bool big) {
     hipStream_t s = context().stream;
     const MgHost &h = *b->mg;
     const SlotGeo &g = b->slots[0].geo;
     const SlotState *st = b->A.st + g.slot;
     const size_t nl = h.lv.size();
-    hipLaunchKernelGGL(mg_fine_down, mg_grid(g.n), dim3(kMgBlock), 0, s, b->A, g, st, h.fres, h.fx0);
+    hipLaunchKernelGGL(mg_fine_down<NT>, mg_grid(g.n), dim3(kMgBlock), 0, s, b->A, g, st, h.fres, h.fx0);
     hipLaunchKernelGGL(mg_restrict_fine, mg_grid(h.lv[0].n), dim3(kMgBlock), 0, s, b->A, g, st, h.fres, h.lv[0]);
     for (size_t l = 0; l + 1 < nl; ++l) {
         hipLaunchKernelGGL(mg_coarse_down, mg_grid(h.lv[l].n), dim3(kMgBlock), 0, s, h.lv[l], st);
@@ -3184,7 +3196,7 @@ void mg_cycle(vx_batch *b, const OneScene &one_u, bool big) {
                        h.lv[nl - 1], h.M, h.n3, st);
     for (size_t l = nl - 1; l-- > 0;)
         hipLaunchKernelGGL(mg_coarse_up, mg_grid(h.lv[l].n), dim3(kMgBlock), 0, s, h.lv[l], h.lv[l + 1], st);
-    hipLaunchKernelGGL(mg_fine_up, dim3(one_u.grid), dim3(kBlock), 0, s, b->A, one_u, h.lv[0], h.fx0);
+    hipLaunchKernelGGL(mg_fine_up<NT>, dim3(one_u.grid), dim3(kBlock), 0, s, b->A, one_u, h.lv[0], h.fx0);
     if (big) hipLaunchKernelGGL(scene_reduce<float>, dim3(b->max_slots), dim3(kBlock), 0, s, b->A, int(kRz), 1, 0);
 }
 
@@ -3200,7 +3212,7 @@ void launch_iteration_t(vx_batch *b, const OneScene &one, const dim3 &grid, cons
     // partials of r.z replace those of the update before (whose z = D^-1 r
     // the cycle overwrites).
     if constexpr (ONE && !RB && std::is_same<S, float>::value) {
-        if (mg) mg_cycle(b, one_u, big);
+        if (mg) mg_cycle<NT>(b, one_u, big);
     }
     if constexpr (RB) hipLaunchKernelGGL((pcg_rb_red<S, ONE, EARLY>), grid_u, blk, 0, ctx.stream, b->A, one_u, odd);
     if constexpr (EARLY && !RB && std::is_same<S, float>::value)
